@@ -1,0 +1,185 @@
+/*
+ * threepoint_amd.h -- C ABI of the MI355X-native <Map^3> / Gamma_i modelling path.
+ *
+ * Plain pointers and sizes only.  Every entry point returns 0 on success and a
+ * nonzero TP_E* code on failure (the reason is in tp_last_error()); no entry
+ * point calls exit().  Host pointers are read/written synchronously: the call
+ * returns only after the device work it started has finished, and no pointer
+ * is retained after return (the contract of the reference's integrand_v
+ * callback, classed_version/cubature.h:55-57 and hcubature.cpp:247-265).
+ *
+ * Each entry point names the reference interface it replaces.
+ */
+#ifndef THREEPOINT_AMD_H
+#define THREEPOINT_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors */
+enum {
+    TP_OK = 0,
+    TP_EINVAL = 1,    /* bad argument (sizes, dims, NULL pointers)            */
+    TP_EHIP = 2,      /* HIP runtime failure                                  */
+    TP_ENONFINITE = 3,/* non-finite ell or B_kappa value (the reference exits) */
+    TP_ECAPACITY = 4, /* a fixed-capacity buffer would overflow               */
+    TP_ECALLBACK = 5, /* the integrand callback returned nonzero              */
+    TP_ENODEV = 6     /* no usable HIP device                                 */
+};
+const char *tp_last_error(void);
+
+/* ------------------------------------------------------------ integrand modes */
+enum {
+    TP_MODE_FP64 = 0,      /* plain FP64 integrand                                  */
+    TP_MODE_FP32 = 1,      /* FP32 bispectrum, FP64 z-sum / cubature accumulation    */
+    TP_MODE_SURROGATE = 2, /* MLP surrogate for log|B_kappa| + exact-check fallback  */
+    TP_MODE_ANALYTIC = 3   /* the reference's test_analytical bispectrum
+                              (classed_version/bispectrum.hpp:18, bispectrum.cpp:242-255) */
+};
+
+/* ---------------------------------------------------------------- context */
+typedef struct tp_ctx tp_ctx;
+/* One context per device and host thread; owns a HIP stream and staging buffers. */
+int tp_ctx_create(int device, tp_ctx **out);
+int tp_ctx_destroy(tp_ctx *ctx);
+int tp_device_count(int *n);
+
+/* ------------------------------------------------------------- cosmologies */
+/* Replaces cosmology (classed_version/cosmology.hpp:11-29): same fields. */
+typedef struct {
+    double h, sigma8, omb, ns, w, om, zmax; /* ow = 1-om and omc = om-omb are derived */
+} tp_cosmology;
+
+typedef struct tp_cosmo tp_cosmo; /* device-resident tables of ncosmo cosmologies */
+
+/* Replaces BispectrumCalculator(cosmology*, n_of_z, n_z, z_max) + set_cosmology
+ * (classed_version/bispectrum.cpp:73-79, 111-164).  Builds, in FP64 on the GPU,
+ * f_K, g, D1, r_sigma, n_eff, ncur on z_i = i*z_max/n_z for each cosmology, and
+ * the 96-node Limber tables used by the integrand kernels.
+ * nz: n_z values per cosmology; nz_stride = 0 shares one n(z) among all. */
+int tp_cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *params, const double *nz,
+                   int n_z, size_t nz_stride, double z_max, tp_cosmo **out);
+int tp_cosmo_destroy(tp_cosmo *c);
+int tp_cosmo_count(const tp_cosmo *c, int *ncosmo, int *n_z);
+/* Copies one cosmology's tables to the host (each array n_z doubles). */
+int tp_cosmo_tables(const tp_cosmo *c, int icosmo, double *f_K, double *g, double *D1,
+                    double *r_sigma, double *n_eff, double *ncur, double *norm);
+
+/* -------------------------------------------------------------- point batches */
+/* Replaces BispectrumCalculator::bkappa (classed_version/bispectrum.cpp:238-286):
+ * out[i] = B_kappa(ell[3i], ell[3i+1], ell[3i+2]). */
+int tp_bkappa_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *ell123,
+                   double *out, int mode);
+
+/* Replaces the integrand_v adapter ApertureStatistics::integrand
+ * (classed_version/apertureStatistics.cpp:254-279): x is npts*3 (l1, l2, phi),
+ * fval[i] = l1 l2 B_kappa(l1,l2,l3) uhat(l1 th1) uhat(l2 th2) uhat(l3 th3). */
+int tp_map3_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const double theta[3], size_t npts,
+                 const double *x, double *fval, int mode);
+
+/* Replaces ApertureStatistics::MapMapMap (classed_version/apertureStatistics.cpp:321-353)
+ * exactly as the reference drives it: tp_hcubature_v over [1, 10/theta_min]^2 x [0, 2pi]
+ * (ERROR_L1, relative tolerance reltol) whose integrand callback sends each round's
+ * points through tp_map3_eval (one host<->device round trip per round, as
+ * cuda_version/apertureStatistics.cu:155-229 does).  Result / (2 pi)^3. */
+int tp_map3_single(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const double theta[3], double reltol,
+                   int mode, double *val, double *err, long long *neval);
+
+/* Replaces ApertureStatistics::MapMapMap (classed_version/apertureStatistics.cpp:321-353)
+ * for many triples at once: each integral is the 3-D h-adaptive cubature over
+ * [1, 10/theta_min]^2 x [0, 2pi] with relative tolerance reltol (ERROR_L1), result
+ * divided by (2 pi)^3.  Integrals advance together; each keeps the decision sequence
+ * of the one-at-a-time algorithm.  icosmo[t] selects the cosmology of triple t.
+ * neval (optional) receives the integrand points used per triple. */
+int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, const int *icosmo,
+                  const double *theta_rad, double reltol, int mode, double *val, double *err,
+                  long long *neval);
+
+/* Statistics of the last tp_map3_batch call on this context. */
+typedef struct {
+    long long rounds, regions, points, launches;
+    double kernel_ms;            /* summed duration of the integrand/rule kernels */
+    double wall_ms;              /* whole call                                     */
+    long long surrogate_points;  /* points evaluated by the surrogate              */
+    long long exact_checks;      /* points re-evaluated exactly                    */
+    long long fallback_batches;  /* rounds that fell back to exact B_kappa         */
+    double max_check_relerr;     /* worst relative error seen by the exact checks  */
+} tp_batch_stats;
+int tp_last_batch_stats(const tp_ctx *ctx, tp_batch_stats *out);
+
+/* ---------------------------------------------------------------- Gamma_i */
+/* Replaces GammaCalculator(bk, prec_h, prec_k, "centroid") + initialize_bessel
+ * (classed_version/gamma.cpp:6-23, 133-176): Ogata node tables for J6 and J2. */
+int tp_gamma_setup(tp_ctx *ctx, double prec_h, double prec_k_arg);
+/* Replaces integrand_gamma{0,1}_cubature_and_ogata (classed_version/gamma.cpp:338-359,
+ * 522-543): which = 0 (Gamma0 integrand) or 1 (Gamma1 integrand); x is npts*3
+ * (z, phi, psi); fval is npts*2 (re, im). */
+int tp_gamma_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, int which, double x1, double x2,
+                  double x3, size_t npts, const double *x, double *fval);
+/* Replaces GammaCalculator::gamma0..gamma3 (classed_version/gamma.cpp:27-81) for many
+ * triangles: out_re/out_im[4*t + i] = Gamma_i(x1,x2,x3) of triangle t (rad), centroid
+ * convention, hcubature over (z, phi, psi) with relative tolerance reltol. */
+int tp_gamma_batch(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t ntri, const double *x123,
+                   double reltol, double *out_re, double *out_im, long long *neval);
+
+/* ------------------------------------------------------------- cubature */
+/* Replaces hcubature_v (classed_version/cubature.h:90-94, hcubature.cpp:1059-1068):
+ * same arguments, same callback layout x[i*ndim+j], fval[i*fdim+k], same return
+ * convention (0 success, 1 failure or callback abort).  norm takes the
+ * error_norm values of cubature.h:63-70. */
+typedef int (*tp_integrand_v)(unsigned ndim, size_t npt, const double *x, void *fdata,
+                              unsigned fdim, double *fval);
+int tp_hcubature_v(unsigned fdim, tp_integrand_v f, void *fdata, unsigned dim,
+                   const double *xmin, const double *xmax, size_t maxEval, double reqAbsError,
+                   double reqRelError, int norm, double *val, double *err);
+/* integrand points evaluated by the last tp_hcubature_v call on this thread */
+long long tp_hcubature_last_neval(void);
+
+/* -------------------------------------------------------------- surrogate */
+/* The learned part: an MLP 8 -> 4x256 (SiLU) -> 1 for log|B_kappa(l1,l2,l3; cosmology)|
+ * over inputs (ln l1, ln l2, cos phi, Omega_m, sigma8, h, w, n_s).  No counterpart in
+ * the reference; it serves TP_MODE_SURROGATE. */
+typedef struct {
+    long long nsamples;   /* training samples generated by the exact GPU path     */
+    int epochs;           /* passes over the samples                              */
+    int batch;            /* minibatch size                                       */
+    double lr, lr_final;  /* Adam learning rate, cosine-decayed to lr_final       */
+    double lnl_min, lnl_max;          /* ln l box                                 */
+    double om[2], s8[2], h[2], w[2], ns[2]; /* cosmology box                      */
+    double omb;           /* fixed Omega_b                                        */
+    uint64_t seed;
+    int n_z;              /* redshift bins of the exact tables                    */
+    double z_max;
+    const double *nz;     /* n(z) used to generate samples (n_z values)           */
+} tp_train_config;
+typedef struct {
+    double final_loss, val_rmse, val_maxerr; /* in log|B| units on held-out samples */
+    double gen_ms, train_ms;
+    long long steps;
+} tp_train_report;
+int tp_surrogate_train(tp_ctx *ctx, const tp_train_config *cfg, tp_train_report *rep);
+int tp_surrogate_save(tp_ctx *ctx, const char *path);
+int tp_surrogate_load(tp_ctx *ctx, const char *path);
+/* Surrogate settings used by TP_MODE_SURROGATE: check_frac of each batch's points
+ * are re-evaluated exactly; if the surrogate's error on them exceeds tol the batch
+ * falls back to exact B_kappa. */
+int tp_surrogate_config(tp_ctx *ctx, double check_frac, double tol, uint64_t seed);
+/* y[i] = surrogate log|B_kappa| at inputs in[8*i..] (raw, unnormalised). */
+int tp_surrogate_predict(tp_ctx *ctx, size_t n, const double *in8, double *y);
+/* Raw MLP access for gradient checks: parameters as one FP32 vector (layout
+ * W1,b1,...,W5,b5; Wk row-major [out][in]). */
+int tp_mlp_num_params(size_t *n);
+int tp_mlp_get_params(tp_ctx *ctx, float *p);
+int tp_mlp_set_params(tp_ctx *ctx, const float *p);
+/* One forward+backward pass on a batch of normalised inputs x (n*8) and targets t (n):
+ * loss = mean (y - t)^2 / 2; grad (n_params) receives d loss / d params. */
+int tp_mlp_loss_grad(tp_ctx *ctx, size_t n, const float *x, const float *t, double *loss,
+                     float *grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

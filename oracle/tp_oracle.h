@@ -1,0 +1,82 @@
+/*
+ * tp_oracle.h -- CPU FP64 oracle for the <Map^3> / Gamma_i modelling path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in threepoint_amd/ links, loads or calls
+ * this library.  It is used by tests/ (as the checker), by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg.
+ *
+ * It is a plain restatement of the reference's classed_version CPU path
+ * (bispectrum.cpp, apertureStatistics.cpp, gamma.cpp, hcubature.cpp,
+ * helper.cpp, cosmology.cpp), with the GSL pieces that path calls
+ * (gsl_integration_qag, gsl_sf_bessel_Jn, gsl_sf_bessel_zero_Jnu) replaced
+ * by project-written routines that follow the published algorithms.
+ * Each function in tp_oracle.cpp cites the reference file:line it follows.
+ */
+#ifndef TP_ORACLE_H
+#define TP_ORACLE_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same callback convention as the reference's cubature.h:55-57. */
+typedef int (*orc_integrand_v)(unsigned ndim, size_t npt, const double *x,
+                               void *fdata, unsigned fdim, double *fval);
+
+/* error_norm values follow cubature.h:63-70 */
+enum { ORC_ERROR_INDIVIDUAL = 0, ORC_ERROR_PAIRED, ORC_ERROR_L2, ORC_ERROR_L1, ORC_ERROR_LINF };
+
+int orc_hcubature_v(unsigned fdim, orc_integrand_v f, void *fdata, unsigned dim,
+                    const double *xmin, const double *xmax, size_t maxEval,
+                    double reqAbsError, double reqRelError, int norm,
+                    double *val, double *err);
+/* number of integrand points the last orc_hcubature_v call on this thread evaluated */
+long long orc_last_neval(void);
+long long orc_last_nregions(void);
+
+/* GSL-qag-equivalent (GK15, key=1) adaptive quadrature of a 1-D function. */
+typedef double (*orc_f1)(double x, void *p);
+double orc_qag15(orc_f1 f, void *p, double a, double b, double epsabs, double epsrel, int limit);
+
+/* Gauss-Legendre n-point nodes/weights on [-1,1] (full set, ascending). */
+void orc_gauss_legendre(int n, double *x, double *w);
+
+/* io (helper.cpp / cosmology.cpp) */
+int orc_read_cosmology(const char *fn, double *out7 /* h s8 omb ns w om zmax */);
+int orc_read_n_of_z(const char *fn, int n_bins, double zmax, double *nz_out);
+
+/* Bispectrum calculator handle.  params7 = {h, sigma8, Omega_b, n_s, w, Omega_m, z_max}. */
+void *orc_create(const double *params7, const double *nz, int n_z, double z_max);
+void orc_destroy(void *h);
+void orc_set_analytic(void *h, int on); /* reference's test_analytical switch (bispectrum.hpp:18) */
+void orc_get_tables(void *h, double *fK, double *g, double *D1, double *rs, double *neff,
+                    double *ncur, double *norm);
+double orc_linear_pk(void *h, double k);
+double orc_sigmam(void *h, double r, int j);
+double orc_lgr(void *h, double z);
+double orc_f_K_at_z(void *h, double z);
+double orc_bispec(void *h, double k1, double k2, double k3, double z);
+double orc_integrand_bkappa(void *h, double z, double l1, double l2, double l3);
+double orc_bkappa(void *h, double l1, double l2, double l3);
+void orc_bkappa_batch(void *h, long long n, const double *l123, double *out, int nthreads);
+double orc_map3_integrand(void *h, double l1, double l2, double phi, const double *thetas);
+void orc_map3_integrand_batch(void *h, long long n, const double *x, const double *thetas,
+                              double *out, int nthreads);
+int orc_map3(void *h, const double *thetas_rad, double reltol, int nthreads,
+             double *val, double *err, long long *neval);
+
+/* Gamma_i (gamma.cpp) */
+double orc_bessel_jn(int n, double x);
+double orc_bessel_zero_jn(int n, int s);
+int orc_gamma_init(void *h, double prec_h, double prec_k_arg);
+int orc_gamma_integrand(void *h, int which, double z, double phi, double psi,
+                        double x1, double x2, double x3, double *re, double *im);
+int orc_gamma(void *h, int which, double x1, double x2, double x3, double reltol,
+              int nthreads, double *re, double *im, long long *neval);
+void orc_ggg_analytic(double r1, double r2, double r3, double *re, double *im);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,359 @@
+"""threepoint_amd -- MI355X-native <Map^3> / Gamma_i modelling path.
+
+Python mirror of the reference's class API (classed_version/bispectrum.hpp,
+apertureStatistics.hpp, gamma.hpp) over the C ABI in include/threepoint_amd.h.
+All numerics run in the HIP library threepoint_amd/_lib/libthreepoint_amd.so;
+there is no CPU fallback: importing works without a GPU, but every compute call
+needs the library and a HIP device and raises otherwise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libthreepoint_amd.so")
+
+MODE_FP64, MODE_FP32, MODE_SURROGATE, MODE_ANALYTIC = 0, 1, 2, 3
+ERROR_INDIVIDUAL, ERROR_PAIRED, ERROR_L2, ERROR_L1, ERROR_LINF = 0, 1, 2, 3, 4
+ARCMIN = 2.9088820866e-4  # classed_version/helper.cpp:139
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
+_ip = ctypes.POINTER(ctypes.c_int)
+_llp = ctypes.POINTER(ctypes.c_longlong)
+
+INTEGRAND_V = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_uint, ctypes.c_size_t, _dp, ctypes.c_void_p,
+                               ctypes.c_uint, _dp)
+
+
+class TpError(RuntimeError):
+    pass
+
+
+class tp_cosmology(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("h", "sigma8", "omb", "ns", "w", "om", "zmax")]
+
+
+class tp_batch_stats(ctypes.Structure):
+    _fields_ = [("rounds", ctypes.c_longlong), ("regions", ctypes.c_longlong), ("points", ctypes.c_longlong),
+                ("launches", ctypes.c_longlong), ("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
+                ("surrogate_points", ctypes.c_longlong), ("exact_checks", ctypes.c_longlong),
+                ("fallback_batches", ctypes.c_longlong), ("max_check_relerr", ctypes.c_double)]
+
+
+class tp_train_config(ctypes.Structure):
+    _fields_ = [("nsamples", ctypes.c_longlong), ("epochs", ctypes.c_int), ("batch", ctypes.c_int),
+                ("lr", ctypes.c_double), ("lr_final", ctypes.c_double),
+                ("lnl_min", ctypes.c_double), ("lnl_max", ctypes.c_double),
+                ("om", ctypes.c_double * 2), ("s8", ctypes.c_double * 2), ("h", ctypes.c_double * 2),
+                ("w", ctypes.c_double * 2), ("ns", ctypes.c_double * 2), ("omb", ctypes.c_double),
+                ("seed", ctypes.c_uint64), ("n_z", ctypes.c_int), ("z_max", ctypes.c_double),
+                ("nz", _dp)]
+
+
+class tp_train_report(ctypes.Structure):
+    _fields_ = [("final_loss", ctypes.c_double), ("val_rmse", ctypes.c_double), ("val_maxerr", ctypes.c_double),
+                ("gen_ms", ctypes.c_double), ("train_ms", ctypes.c_double), ("steps", ctypes.c_longlong)]
+
+
+_SIGS = {
+    "tp_last_error": (ctypes.c_char_p, []),
+    "tp_device_count": (ctypes.c_int, [_ip]),
+    "tp_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "tp_ctx_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "tp_cosmo_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(tp_cosmology), _dp, ctypes.c_int,
+                                      ctypes.c_size_t, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)]),
+    "tp_cosmo_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "tp_cosmo_count": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip]),
+    "tp_cosmo_tables": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "tp_bkappa_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp, _dp,
+                                      ctypes.c_int]),
+    "tp_map3_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_size_t, _dp, _dp,
+                                    ctypes.c_int]),
+    "tp_map3_single": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_double,
+                                      ctypes.c_int, _dp, _dp, _llp]),
+    "tp_map3_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp, ctypes.c_double,
+                                     ctypes.c_int, _dp, _dp, _llp]),
+    "tp_last_batch_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(tp_batch_stats)]),
+    "tp_gamma_setup": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    "tp_gamma_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_size_t, _dp, _dp]),
+    "tp_gamma_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
+                                      ctypes.c_double, _dp, _dp, _llp]),
+    "tp_hcubature_v": (ctypes.c_int, [ctypes.c_uint, INTEGRAND_V, ctypes.c_void_p, ctypes.c_uint, _dp, _dp,
+                                      ctypes.c_size_t, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp]),
+    "tp_hcubature_last_neval": (ctypes.c_longlong, []),
+    "tp_surrogate_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(tp_train_config),
+                                          ctypes.POINTER(tp_train_report)]),
+    "tp_surrogate_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "tp_surrogate_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "tp_surrogate_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]),
+    "tp_surrogate_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _dp, _dp]),
+    "tp_mlp_num_params": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t)]),
+    "tp_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, _fp]),
+    "tp_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, _fp]),
+    "tp_mlp_loss_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _fp, _fp, _dp, _fp]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded HIP library (raises TpError when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TpError("libthreepoint_amd.so is not built; run __graft_entry__.build() "
+                          "or python -m threepoint_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        msg = lib().tp_last_error().decode(errors="replace")
+        raise TpError("%s failed (code %d): %s" % (what, rc, msg))
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    rc = lib().tp_device_count(ctypes.byref(n))
+    return n.value if rc == 0 else 0
+
+
+class Context:
+    """One device + stream (tp_ctx)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().tp_ctx_create(device, ctypes.byref(h)), "tp_ctx_create")
+        self.h = h
+        self.device = device
+
+    def close(self):
+        if self.h:
+            lib().tp_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self):
+        s = tp_batch_stats()
+        _check(lib().tp_last_batch_stats(self.h, ctypes.byref(s)), "tp_last_batch_stats")
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class Cosmology:
+    """classed_version/cosmology.hpp:11-29 (ow and omc derived as in cosmology.cpp:71-72)."""
+
+    KEYS = {"h": "h", "sigma_8": "sigma8", "Omega_b": "omb", "n_s": "ns", "w": "w", "Omega_m": "om", "z_max": "zmax"}
+
+    def __init__(self, h=0.7, sigma8=0.82, omb=0.047, ns=0.97, w=-1.0, om=0.3, zmax=3.0):
+        self.h, self.sigma8, self.omb, self.ns, self.w, self.om, self.zmax = h, sigma8, omb, ns, w, om, zmax
+
+    @property
+    def ow(self):
+        return 1 - self.om
+
+    @property
+    def omc(self):
+        return self.om - self.omb
+
+    @classmethod
+    def from_file(cls, fn):
+        from .io import read_cosmology
+        return read_cosmology(fn)
+
+    def as_struct(self):
+        return tp_cosmology(self.h, self.sigma8, self.omb, self.ns, self.w, self.om, self.zmax)
+
+
+class CosmologyTables:
+    """Device-resident tables of one or more cosmologies (tp_cosmo)."""
+
+    def __init__(self, cosmos, nz, n_z=None, z_max=3.0, ctx=None):
+        self.ctx = ctx or default_context()
+        if isinstance(cosmos, Cosmology):
+            cosmos = [cosmos]
+        self.cosmos = list(cosmos)
+        nz = np.ascontiguousarray(nz, dtype=np.float64)
+        if nz.ndim == 1:
+            n_z = nz.size if n_z is None else n_z
+            stride = 0
+        else:
+            n_z = nz.shape[1]
+            stride = n_z
+        arr = (tp_cosmology * len(self.cosmos))(*[c.as_struct() for c in self.cosmos])
+        h = ctypes.c_void_p()
+        _check(lib().tp_cosmo_build(self.ctx.h, len(self.cosmos), arr, _d(nz), n_z, stride, z_max, ctypes.byref(h)),
+               "tp_cosmo_build")
+        self.h = h
+        self.n_z = n_z
+        self.z_max = z_max
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().tp_cosmo_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __len__(self):
+        return len(self.cosmos)
+
+    def tables(self, i=0):
+        out = [np.zeros(self.n_z) for _ in range(6)]
+        norm = ctypes.c_double()
+        _check(lib().tp_cosmo_tables(self.h, i, *[_d(a) for a in out], ctypes.byref(norm)), "tp_cosmo_tables")
+        return dict(zip(["f_K", "g", "D1", "r_sigma", "n_eff", "ncur"], out), norm=norm.value)
+
+    def bkappa(self, ell123, icosmo=0, mode=MODE_FP64):
+        ell = np.ascontiguousarray(ell123, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(ell.shape[0])
+        _check(lib().tp_bkappa_eval(self.ctx.h, self.h, icosmo, ell.shape[0], _d(ell), _d(out), mode),
+               "tp_bkappa_eval")
+        return out
+
+    def map3_integrand(self, x, thetas, icosmo=0, mode=MODE_FP64):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        th = np.ascontiguousarray(thetas, dtype=np.float64)
+        out = np.zeros(x.shape[0])
+        _check(lib().tp_map3_eval(self.ctx.h, self.h, icosmo, _d(th), x.shape[0], _d(x), _d(out), mode),
+               "tp_map3_eval")
+        return out
+
+    def map3_single(self, thetas, reltol=1e-4, icosmo=0, mode=MODE_FP64):
+        th = np.ascontiguousarray(thetas, dtype=np.float64)
+        v, e, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        _check(lib().tp_map3_single(self.ctx.h, self.h, icosmo, _d(th), reltol, mode, ctypes.byref(v),
+                                    ctypes.byref(e), ctypes.byref(n)), "tp_map3_single")
+        return v.value, e.value, n.value
+
+    def map3_batch(self, thetas, icosmo=None, reltol=1e-4, mode=MODE_FP64):
+        th = np.ascontiguousarray(thetas, dtype=np.float64).reshape(-1, 3)
+        n = th.shape[0]
+        ic = np.zeros(n, dtype=np.int32) if icosmo is None else np.ascontiguousarray(icosmo, dtype=np.int32)
+        val, err = np.zeros(n), np.zeros(n)
+        ne = np.zeros(n, dtype=np.int64)
+        _check(lib().tp_map3_batch(self.ctx.h, self.h, n, ic.ctypes.data_as(_ip), _d(th), reltol, mode, _d(val),
+                                   _d(err), ne.ctypes.data_as(_llp)), "tp_map3_batch")
+        return val, err, ne
+
+    def gamma_eval(self, which, x123, pts, icosmo=0):
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros((pts.shape[0], 2))
+        _check(lib().tp_gamma_eval(self.ctx.h, self.h, icosmo, which, x123[0], x123[1], x123[2], pts.shape[0],
+                                   _d(pts), _d(out)), "tp_gamma_eval")
+        return out[:, 0] + 1j * out[:, 1]
+
+    def gamma_batch(self, x123, reltol=1e-3, icosmo=0):
+        x = np.ascontiguousarray(x123, dtype=np.float64).reshape(-1, 3)
+        n = x.shape[0]
+        re, im = np.zeros(4 * n), np.zeros(4 * n)
+        ne = np.zeros(4 * n, dtype=np.int64)
+        _check(lib().tp_gamma_batch(self.ctx.h, self.h, icosmo, n, _d(x), reltol, _d(re), _d(im),
+                                    ne.ctypes.data_as(_llp)), "tp_gamma_batch")
+        return (re + 1j * im).reshape(n, 4), ne.reshape(n, 4)
+
+
+def gamma_setup(prec_h=1e-3, prec_k_arg=5.0, ctx=None):
+    ctx = ctx or default_context()
+    _check(lib().tp_gamma_setup(ctx.h, prec_h, prec_k_arg), "tp_gamma_setup")
+
+
+def hcubature_v(fdim, f, dim, xmin, xmax, maxEval=0, reqAbsError=0.0, reqRelError=1e-4, norm=ERROR_L1):
+    """classed_version/cubature.h:90-94 with a Python callable f(x[npt, dim]) -> fval[npt, fdim]."""
+    def cb(ndim, npt, x, fdata, fd, fval):
+        try:
+            xs = np.ctypeslib.as_array(x, shape=(npt * ndim,)).reshape(npt, ndim)
+            v = np.asarray(f(xs), dtype=np.float64).reshape(npt * fd)
+            np.ctypeslib.as_array(fval, shape=(npt * fd,))[:] = v
+            return 0
+        except Exception:
+            return 1
+    cfun = INTEGRAND_V(cb)
+    xmin = np.ascontiguousarray(xmin, dtype=np.float64)
+    xmax = np.ascontiguousarray(xmax, dtype=np.float64)
+    val, err = np.zeros(fdim), np.zeros(fdim)
+    rc = lib().tp_hcubature_v(fdim, cfun, None, dim, _d(xmin), _d(xmax), maxEval, reqAbsError, reqRelError, norm,
+                              _d(val), _d(err))
+    return rc, val, err, lib().tp_hcubature_last_neval()
+
+
+class BispectrumCalculator:
+    """classed_version/bispectrum.hpp:113-509 (n(z) constructor, :207)."""
+
+    def __init__(self, cosmo, n_of_z, n_z, z_max, ctx=None):
+        self.cosmo = cosmo
+        self.z_max = z_max
+        self.tables_ = CosmologyTables([cosmo], np.asarray(n_of_z, dtype=np.float64)[:n_z], n_z, z_max, ctx)
+
+    def bkappa(self, ell1, ell2, ell3, mode=MODE_FP64):
+        return float(self.tables_.bkappa(np.array([[ell1, ell2, ell3]]), 0, mode)[0])
+
+
+class ApertureStatistics:
+    """classed_version/apertureStatistics.hpp:18-322: MapMapMap over (l1, l2, phi) with hcubature."""
+
+    def __init__(self, bispectrum, reltol=1e-4, mode=MODE_FP64):
+        self.bk = bispectrum
+        self.reltol = reltol
+        self.mode = mode
+
+    def MapMapMap(self, thetas):
+        return self.bk.tables_.map3_single(thetas, self.reltol, 0, self.mode)[0]
+
+
+class GammaCalculator:
+    """classed_version/gamma.hpp: Gamma_i via cubature + Ogata (centroid convention)."""
+
+    def __init__(self, bispectrum, prec_h, prec_k, triangle_center="centroid", reltol=1e-3):
+        if triangle_center != "centroid":
+            raise ValueError("only the centroid convention is provided")
+        self.bk = bispectrum
+        self.reltol = reltol
+        gamma_setup(prec_h, prec_k, self.bk.tables_.ctx)
+
+    def gammas(self, x1, x2, x3):
+        g, _ = self.bk.tables_.gamma_batch(np.array([[x1, x2, x3]]), self.reltol)
+        return g[0]
+
+    def gamma0(self, x1, x2, x3):
+        return self.gammas(x1, x2, x3)[0]
+
+    def gamma1(self, x1, x2, x3):
+        return self.gammas(x1, x2, x3)[1]
+
+    def gamma2(self, x1, x2, x3):
+        return self.gammas(x1, x2, x3)[2]
+
+    def gamma3(self, x1, x2, x3):
+        return self.gammas(x1, x2, x3)[3]

@@ -1,0 +1,222 @@
+// tp_device.h -- device functions shared by the integrand, region, Gamma and
+// surrogate kernels.  See tp_map3.hip for the overview.
+#pragma once
+#include "tp_internal.h"
+
+namespace tpd {
+constexpr double kPi = 3.14159265358979323846;
+}
+
+template <class T> struct TpMath;
+template <> struct TpMath<double> {
+    static __device__ __forceinline__ double ex(double x) { return exp(x); }
+    static __device__ __forceinline__ double lg(double x) { return log(x); }
+};
+template <> struct TpMath<float> {
+    // native transcendental unit: v_exp_f32 / v_log_f32 with a scale
+    static __device__ __forceinline__ float ex(float x) { return __expf(x); }
+    static __device__ __forceinline__ float lg(float x) { return __logf(x); }
+};
+
+template <class T>
+__device__ __forceinline__ T tp_f2_tree(T k1, T k2, T k3)
+{
+    // classed_version/bispectrum.cpp:629-633 (scale-free, so q stands for k)
+    T c12 = T(0.5) * (k3 * k3 - k1 * k1 - k2 * k2) / (k1 * k2);
+    return T(5. / 7.) + T(0.5) * c12 * (k1 / k2 + k2 / k1) + T(2. / 7.) * c12 * c12;
+}
+
+// Eisenstein & Hu no-wiggle P_lin at k = q / r_sigma (classed_version/bispectrum.cpp:811-841)
+template <class T>
+__device__ __forceinline__ T tp_plin_q(const TpNodeT<T> &N, const TpEHT<T> &E, T q, T lq)
+{
+    T kk = q * N.inv_rs * E.h;  // 1/Mpc
+    T x = E.sh043 * kk;
+    T x2 = x * x;
+    T geff = E.omh2 * (E.sqrt_alnu + E.one_m_sqrt_alnu / (T(1) + x2 * x2));
+    T qeff = kk / geff * E.theta2;
+    T L = TpMath<T>::lg(E.e_const + E.cL * qeff);
+    T Cq = T(14.4) + T(325.0) / (T(1) + T(60.5) * TpMath<T>::ex(T(1.11) * TpMath<T>::lg(qeff)));
+    T r = L / (L + Cq * qeff * qeff);
+    T delk = TpMath<T>::ex(E.three_ns * (lq + N.lnkoff)) * r * r;
+    return N.pk_pref * delk / (q * q * q);
+}
+
+// BiHalofit matter bispectrum at one z-node, classed_version/bispectrum.cpp:166-236
+template <class T>
+__device__ __forceinline__ T tp_bispec_q(const TpNodeT<T> &N, const TpEHT<T> &E, T q1, T q2, T q3)
+{
+    T a = q1, b = q2, c = q3, t;
+    if (a < b) { t = a; a = b; b = t; }
+    if (b < c) { t = b; b = c; c = t; }
+    if (a < b) { t = a; a = b; b = t; }
+    T r1 = c / a, r2 = (b + c - a) / a;                                       // eq. B8
+    T an = TpMath<T>::ex(N.lnA0 + N.m031ln10 * TpMath<T>::ex(N.c1 * TpMath<T>::lg(r1)));
+    T alphan = TpMath<T>::ex(N.a0ln + N.a1ln * r2 * r2);
+    if (alphan > N.acap) alphan = N.acap;
+    T betan = TpMath<T>::ex(N.b0ln + N.b1ln * r2);
+    T qq[3] = {q1, q2, q3};
+    T pse[3];
+    T bs1h = T(1), damp = T(1);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        T q = qq[i];
+        T lq = TpMath<T>::lg(q);
+        T oneh = an * TpMath<T>::ex(alphan * lq) + N.bn * TpMath<T>::ex(betan * lq);   // eq. B4
+        bs1h *= T(1) / (oneh * (T(1) + T(1) / (N.cn * q)));
+        T pl = tp_plin_q(N, E, q, lq);
+        T pq = N.pn * q;
+        pse[i] = (T(1) + N.fn * q * q) / (T(1) + N.gn * q + N.hn * q * q) * N.D1sq * pl +
+                 T(1) / ((N.mn * TpMath<T>::ex(N.mun * lq) + N.nn * TpMath<T>::ex(N.nun * lq)) *
+                         (T(1) + T(1) / (pq * pq * pq)));                                  // eq. B6
+        damp *= T(1) + N.en * q;
+    }
+    T bs3h = T(2) * ((tp_f2_tree(q1, q2, q3) + N.dn * q3) * pse[0] * pse[1] +
+                     (tp_f2_tree(q2, q3, q1) + N.dn * q1) * pse[1] * pse[2] +
+                     (tp_f2_tree(q3, q1, q2) + N.dn * q2) * pse[2] * pse[0]);             // eq. B5
+    return bs1h + bs3h / damp;
+}
+
+// Limber sum over the 96 z-nodes (FP64 accumulation), without the zero-ell guard
+template <class T>
+__device__ __forceinline__ double tp_bkappa_sum(const TpNodeT<T> *__restrict__ nodes, const TpEHT<T> &E,
+                                                double l1, double l2, double l3)
+{
+    double acc = 0.0;
+    const T L1 = (T)l1, L2 = (T)l2, L3 = (T)l3;
+#pragma unroll 2
+    for (int n = 0; n < TP_NNODES; ++n) {
+        const TpNodeT<T> &N = nodes[n];
+        T ks = N.ks;
+        T b = tp_bispec_q<T>(N, E, L1 * ks, L2 * ks, L3 * ks);
+        acc += (double)(N.wz * b);
+    }
+    return acc;
+}
+
+// the reference's test_analytical B_kappa (classed_version/bispectrum.cpp:242-255, 989-996)
+__device__ __forceinline__ double tp_bkappa_analytic(double l1, double l2, double l3)
+{
+    const double wts[9] = {0, 1.0e+8, 1.0e+6, 1.0e+4, 1.0e+2, 1.0e+0, 5.0e-3, 1.0e-5, 1.0e-6};
+    const double avs[9] = {3.0e+3, 1.0e+4, 3.0e+4, 1.0e+5, 3.0e+5, 1.0e+6, 3.0e+6, 1.0e+7, 3.0e+7};
+    double phi = acos((l1 * l1 + l2 * l2 - l3 * l3) / (2 * l1 * l2));
+    double cphi = cos(phi);
+    double s = 0;
+    for (int i = 1; i < 9; ++i) {
+        double a = avs[i];
+        double a2 = a * a, a4 = a2 * a2;
+        double bis_pref = -(tpd::kPi * tpd::kPi) / (8 * 72 * (a4 * a4));
+        double pref = (l1 * l1) * (l2 * l2) * (l1 * l1 + l2 * l2 + 2 * l1 * l2 * cphi);
+        double ex = exp(-(l1 * l1 + l2 * l2 + l1 * l2 * cphi) / (6 * a));
+        s += wts[i] * (bis_pref * pref * ex);
+    }
+    return s * (3. * (2 * tpd::kPi) * (2 * tpd::kPi));
+}
+
+__device__ __forceinline__ double tp_uhat(double eta)
+{
+    double t = 0.5 * eta * eta;  // classed_version/apertureStatistics.cpp:10-14
+    return t * exp(-t);
+}
+
+// l3 exactly as the reference forms it (no contraction: l1 = l2, phi = pi gives 0)
+__device__ __forceinline__ double tp_ell3(double l1, double l2, double phi)
+{
+#pragma clang fp contract(off)
+    double s = l1 * l1 + l2 * l2 + 2 * l1 * l2 * cos(phi);
+    return sqrt(s);
+}
+
+template <class T>
+__device__ __forceinline__ double tp_map3_integrand(const TpNodeT<T> *__restrict__ nodes, const TpEHT<T> &E,
+                                                    double l1, double l2, double phi, double th1, double th2,
+                                                    double th3, int mode, int *flag)
+{
+    // classed_version/apertureStatistics.cpp:16-28
+    double l3 = tp_ell3(l1, l2, phi);
+    if (!isfinite(l1) || !isfinite(l2) || !isfinite(l3)) {
+        atomicOr(flag, 1);
+        return __builtin_nan("");
+    }
+    double bk;
+    if (l1 == 0 || l2 == 0 || l3 == 0)
+        bk = 0.0;  // classed_version/bispectrum.cpp:240-241
+    else if (mode == TP_MODE_ANALYTIC)
+        bk = tp_bkappa_analytic(l1, l2, l3);
+    else
+        bk = tp_bkappa_sum<T>(nodes, E, l1, l2, l3);
+    if (!isfinite(bk)) atomicOr(flag, 2);
+    return l1 * l2 * bk * tp_uhat(l1 * th1) * tp_uhat(l2 * th2) * tp_uhat(l3 * th3);
+}
+
+// --------------------------------------------------- Genz-Malik 3-D rule
+// Point p of a region, in the reference's order (classed_version/hcubature.cpp:
+// 343-414, 476-503): center; -+lambda2, -+lambda4 on each axis; (lambda4,lambda4)
+// pairs; 2^3 lambda5 corners in Gray-code order.  code: 0 centre, +-1 lambda2,
+// +-2 lambda4, +-3 lambda5.
+__constant__ signed char c_gm3_code[TP_NPTS_GM3][3] = {
+    {0, 0, 0},
+    {-1, 0, 0}, {1, 0, 0}, {-2, 0, 0}, {2, 0, 0},
+    {0, -1, 0}, {0, 1, 0}, {0, -2, 0}, {0, 2, 0},
+    {0, 0, -1}, {0, 0, 1}, {0, 0, -2}, {0, 0, 2},
+    {-2, -2, 0}, {2, -2, 0}, {2, 2, 0}, {-2, 2, 0},
+    {-2, 0, -2}, {2, 0, -2}, {2, 0, 2}, {-2, 0, 2},
+    {0, -2, -2}, {0, 2, -2}, {0, 2, 2}, {0, -2, 2},
+    {3, 3, 3}, {-3, 3, 3}, {-3, -3, 3}, {3, -3, 3},
+    {3, -3, -3}, {-3, -3, -3}, {-3, 3, -3}, {3, 3, -3}};
+
+__device__ __forceinline__ double tp_gm_coord(double c, double hw, int code)
+{
+#pragma clang fp contract(off)
+    const double l2 = 0.3585685828003180919906451539079374954541;  // sqrt(9/70)
+    const double l4 = 0.9486832980505137995996680633298155601160;  // sqrt(9/10)
+    const double l5 = 0.6882472016116852977216287342936235251269;  // sqrt(9/19)
+    int a = code < 0 ? -code : code;
+    if (a == 0) return c;
+    double lam = (a == 1) ? l2 : (a == 2 ? l4 : l5);
+    double w = hw * lam;
+    return code < 0 ? c - w : c + w;
+}
+
+__device__ __forceinline__ void gm_point3(const TpRegion &R, int p, double *x0, double *x1, double *x2)
+{
+    *x0 = tp_gm_coord(R.c[0], R.hw[0], c_gm3_code[p][0]);
+    *x1 = tp_gm_coord(R.c[1], R.hw[1], c_gm3_code[p][1]);
+    *x2 = tp_gm_coord(R.c[2], R.hw[2], c_gm3_code[p][2]);
+}
+
+// value, error and split dimension of one 3-D region from its 33 values
+// (classed_version/hcubature.cpp:515-575, fdim = 1)
+__device__ __forceinline__ void gm_rule3(double vol, const double *v, TpRegionOut *out)
+{
+#pragma clang fp contract(off)
+    const double l2 = 0.3585685828003180919906451539079374954541;
+    const double l4 = 0.9486832980505137995996680633298155601160;
+    const double ratio = (l2 * l2) / (l4 * l4);
+    const double w1 = -10936.0 / 19683.0, w3 = 620.0 / 19683.0, w5 = 6859.0 / 19683.0 / 8.0;
+    const double wE1 = -1671.0 / 729.0, wE3 = -35.0 / 1458.0;
+    const double wt2 = 980. / 6561., wt4 = 200. / 19683., wtE2 = 245. / 486., wtE4 = 25. / 729.;
+    double val0 = v[0], sum2 = 0, sum3 = 0, sum4 = 0, sum5 = 0;
+    double diff[3];
+    for (int k = 0; k < 3; ++k) {
+        double a0 = v[1 + 4 * k], a1 = v[2 + 4 * k], a2 = v[3 + 4 * k], a3 = v[4 + 4 * k];
+        sum2 += a0 + a1;
+        sum3 += a2 + a3;
+        diff[k] = fabs(a0 + a1 - 2 * val0 - ratio * (a2 + a3 - 2 * val0));
+    }
+    for (int k = 0; k < 12; ++k) sum4 += v[13 + k];
+    for (int k = 0; k < 8; ++k) sum5 += v[25 + k];
+    double result = vol * (w1 * val0 + wt2 * sum2 + w3 * sum3 + wt4 * sum4 + w5 * sum5);
+    double res5th = vol * (wE1 * val0 + wtE2 * sum2 + wE3 * sum3 + wtE4 * sum4);
+    out->val = result;
+    out->err = fabs(res5th - result);
+    int dd = 0;
+    double md = 0;
+    for (int i = 0; i < 3; ++i)
+        if (diff[i] > md) {
+            md = diff[i];
+            dd = i;
+        }
+    out->split = dd;
+    out->flag = (isfinite(result) && isfinite(res5th)) ? 0 : 1;
+}

@@ -1,0 +1,156 @@
+// tp_internal.h -- data layouts shared by the host runtime and the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "threepoint_amd.h"
+
+#define TP_NNODES 96      // Gauss-Legendre z-nodes of the Limber integral
+#define TP_NZ_MAX 1024    // max redshift bins of the tables
+#define TP_NPTS_GM3 33    // Genz-Malik points per 3-D region
+
+// ---------------------------------------------------------------------------
+// Per-cosmology constants, FP64, built on the device by tp_cosmo_build.
+// Everything that depends on the cosmology only (Eisenstein & Hu constants,
+// sigma8 normalisation, Limber prefactor) is folded here once.
+// ---------------------------------------------------------------------------
+struct TpCosmoConst {
+    double h, sigma8, omb, omc, ns, w, om, ow, zmax;
+    double norm;          // sigma8 / sigma_TopHat(8 Mpc/h) with norm=1
+    // Eisenstein & Hu (1998) no-wiggle fit, classed_version/bispectrum.cpp:811-841
+    double eh_fb, eh_sqrt_alnu, eh_sh, eh_omh2, eh_theta2, eh_cL;
+    double limber_pref;   // 27/8 om^3 (H0/c)^5
+    int n_z;
+    int pad;
+};
+
+// ---------------------------------------------------------------------------
+// One Limber z-node of one cosmology.  B_kappa(l) = sum_n wz_n * B(l * ks_n, z_n).
+// All z-only factors of the BiHalofit fit (Takahashi et al. 2020, app. B) are
+// pre-evaluated; the per-point work is what depends on the wavenumbers.
+// ---------------------------------------------------------------------------
+template <class T>
+struct TpNodeT {
+    T wz;        // W_i dx * 27/8 om^3 (H0/c)^5 * (g (1+z))^3 / (f_K E(z))
+    T ks;        // r_sigma / f_K : q = ell * ks (dimensionless wavenumber)
+    T inv_rs;    // 1 / r_sigma   : k = q / r_sigma [h/Mpc]
+    T lnA0;      // ln 10 * (-2.167 - 2.944 L - 1.106 L^2 - 2.865 L^3), L = log10(D1 sigma8)
+    T c1;        // 10^(0.182 + 0.57 n_eff)
+    T m031ln10;  // -0.310 ln 10
+    T bn, cn;    // B7 parameters
+    T a0ln, a1ln;// alpha_n = 10^(a0 + a1 r2^2) -> exp(a0ln + a1ln r2^2)
+    T acap;      // 1 - 2/3 n_s
+    T b0ln, b1ln;// beta_n  = exp(b0ln + b1ln r2)
+    T fn, gn, hn, mn, nn, mun, nun, pn, en; // B6/B9 parameters
+    T D1sq;      // D1^2
+    T dn;        // F2 correction 10^(-0.483 + 0.892 L - 0.086 Omega_m(z))
+    T lnkoff;    // ln(2997.9 / r_sigma)
+    T pk_pref;   // 2 pi^2 h^3 norm^2 / (h / r_sigma)^3 : P = pk_pref * delk' / q^3
+    T pad0;
+};
+typedef TpNodeT<double> TpNode64;
+typedef TpNodeT<float> TpNode32;
+
+// Per-cosmology EH constants needed inside the per-point kernel.
+template <class T>
+struct TpEHT {
+    T h, sqrt_alnu, one_m_sqrt_alnu, sh043, omh2, theta2, cL, three_ns, e_const;
+    T pad[7];
+};
+typedef TpEHT<double> TpEH64;
+typedef TpEHT<float> TpEH32;
+
+// ---------------------------------------------------------------------------
+// Device tables of ncosmo cosmologies (all FP64 except the FP32 node copies).
+// ---------------------------------------------------------------------------
+struct tp_cosmo {
+    int device = 0;
+    int ncosmo = 0, n_z = 0;
+    double z_max = 0;
+    TpCosmoConst *d_const = nullptr;  // [ncosmo]
+    double *d_tab = nullptr;          // [ncosmo][6][n_z]: fK, g, D1, rs, neff, ncur
+    double *d_nz = nullptr;           // [ncosmo][n_z]
+    TpNode64 *d_node64 = nullptr;     // [ncosmo][96]
+    TpNode32 *d_node32 = nullptr;     // [ncosmo][96]
+    TpEH64 *d_eh64 = nullptr;         // [ncosmo]
+    TpEH32 *d_eh32 = nullptr;         // [ncosmo]
+    int *d_err = nullptr;             // build error flag
+    std::vector<TpCosmoConst> h_const;
+};
+
+// ---------------------------------------------------------------------------
+// Batched-engine region record and result (device <-> host, FP64).
+// ---------------------------------------------------------------------------
+struct TpRegion {
+    double c[3], hw[3];
+    double vol;
+    int integral;  // index into the integral table of the launch
+    int pad;
+};
+struct TpRegionOut {
+    double val, err;
+    int split;
+    int flag;      // nonzero: non-finite integrand value inside the region
+};
+struct TpIntegralDesc {
+    double th[3];  // aperture radii [rad]
+    int icosmo;
+    int pad;
+};
+
+// Growable device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes);
+    void release();
+};
+struct HostPinned {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes);
+    void release();
+};
+
+struct TpSurrogate;  // learned B_kappa model (tp_mlp.hip)
+
+struct tp_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    DevBuf d_in, d_out, d_regions, d_rout, d_fvals, d_desc, d_flag, d_misc;
+    HostPinned h_in, h_out, h_regions, h_rout;
+    tp_batch_stats stats{};
+    // Gamma (Ogata) tables
+    double gamma_prec_h = 0;
+    unsigned gamma_prec_k = 0;
+    double *d_ogata = nullptr;  // [4][prec_k]: psi6, prod6, psi2, prod2
+    TpSurrogate *sur = nullptr;
+    double sur_check_frac = 0.01, sur_tol = 1e-3;
+    uint64_t sur_seed = 0x5eed;
+    int autotune_block = 0;     // 0 = not tuned yet
+};
+
+// Error helpers (thread-local last error string).
+int tp_set_error(int code, const std::string &msg);
+#define TP_HIP(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return tp_set_error(TP_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Launchers implemented in the .hip files (all asynchronous on ctx->stream).
+int tp_launch_map3_points(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const double *th, size_t npts,
+                          const double *d_x, double *d_f, int mode, int *d_flag);
+int tp_launch_bkappa_points(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *d_l,
+                            double *d_out, int mode, int *d_flag);
+int tp_launch_regions(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegion *d_reg,
+                      const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int mode,
+                      int *d_flag);
+int tp_launch_gamma_points(tp_ctx *ctx, const tp_cosmo *c, int icosmo, int which, double x1, double x2,
+                           double x3, size_t npts, const double *d_x, double *d_f, int *d_flag);

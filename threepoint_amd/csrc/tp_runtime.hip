@@ -1,0 +1,417 @@
+// tp_runtime.hip -- C ABI runtime: contexts, staging buffers, the plain per-batch
+// integrand path, the drop-in hcubature_v, and the batched many-integral engine.
+#include <omp.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "tp_cubature.h"
+#include "tp_internal.h"
+
+static thread_local std::string g_err;
+
+int tp_set_error(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+extern "C" const char *tp_last_error(void) { return g_err.c_str(); }
+
+int DevBuf::ensure(size_t bytes)
+{
+    if (bytes <= cap) return 0;
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, cap * 2);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) return tp_set_error(TP_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    cap = want;
+    return 0;
+}
+void DevBuf::release()
+{
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+int HostPinned::ensure(size_t bytes)
+{
+    if (bytes <= cap) return 0;
+    if (p) hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t want = std::max(bytes, cap * 2);
+    hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+    if (e != hipSuccess) return tp_set_error(TP_EHIP, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    cap = want;
+    return 0;
+}
+void HostPinned::release()
+{
+    if (p) hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+extern "C" int tp_device_count(int *n)
+{
+    int k = 0;
+    hipError_t e = hipGetDeviceCount(&k);
+    if (e != hipSuccess) {
+        *n = 0;
+        return tp_set_error(TP_ENODEV, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+    }
+    *n = k;
+    return 0;
+}
+
+extern "C" int tp_ctx_create(int device, tp_ctx **out)
+{
+    if (!out) return tp_set_error(TP_EINVAL, "tp_ctx_create: NULL out");
+    int n = 0;
+    if (tp_device_count(&n) || n == 0) return tp_set_error(TP_ENODEV, "tp_ctx_create: no HIP device");
+    if (device < 0 || device >= n) return tp_set_error(TP_EINVAL, "tp_ctx_create: bad device index");
+    TP_HIP(hipSetDevice(device));
+    tp_ctx *c = new tp_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return tp_set_error(TP_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+    }
+    hipEventCreate(&c->ev0);
+    hipEventCreate(&c->ev1);
+    if (c->d_flag.ensure(256)) {
+        delete c;
+        return TP_EHIP;
+    }
+    *out = c;
+    return 0;
+}
+
+void tp_surrogate_free(tp_ctx *ctx);  // tp_mlp.hip
+
+extern "C" int tp_ctx_destroy(tp_ctx *ctx)
+{
+    if (!ctx) return 0;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    tp_surrogate_free(ctx);
+    for (DevBuf *b : {&ctx->d_in, &ctx->d_out, &ctx->d_regions, &ctx->d_rout, &ctx->d_fvals, &ctx->d_desc,
+                      &ctx->d_flag, &ctx->d_misc})
+        b->release();
+    for (HostPinned *b : {&ctx->h_in, &ctx->h_out, &ctx->h_regions, &ctx->h_rout}) b->release();
+    if (ctx->d_ogata) hipFree(ctx->d_ogata);
+    hipEventDestroy(ctx->ev0);
+    hipEventDestroy(ctx->ev1);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return 0;
+}
+
+extern "C" int tp_last_batch_stats(const tp_ctx *ctx, tp_batch_stats *out)
+{
+    if (!ctx || !out) return tp_set_error(TP_EINVAL, "tp_last_batch_stats: NULL");
+    *out = ctx->stats;
+    return 0;
+}
+
+static int check_cosmo(tp_ctx *ctx, const tp_cosmo *c, int icosmo)
+{
+    if (!ctx || !c) return tp_set_error(TP_EINVAL, "NULL context or cosmology");
+    if (icosmo < 0 || icosmo >= c->ncosmo) return tp_set_error(TP_EINVAL, "cosmology index out of range");
+    if (c->device != ctx->device) return tp_set_error(TP_EINVAL, "cosmology lives on another device");
+    return 0;
+}
+
+static int read_flag(tp_ctx *ctx, int *flag)
+{
+    TP_HIP(hipMemcpyAsync(ctx->h_out.p, ctx->d_flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(flag, ctx->h_out.p, sizeof(int));
+    return 0;
+}
+
+// Runs one batch of points through a point kernel: host in -> device -> host out.
+template <class Launch>
+static int run_points(tp_ctx *ctx, size_t npts, size_t in_per, size_t out_per, const double *in, double *out,
+                      Launch launch, const char *what)
+{
+    if (npts == 0) return 0;
+    size_t bin = npts * in_per * sizeof(double), bout = npts * out_per * sizeof(double);
+    int rc;
+    if ((rc = ctx->d_in.ensure(bin)) || (rc = ctx->d_out.ensure(bout)) || (rc = ctx->h_in.ensure(bin)) ||
+        (rc = ctx->h_out.ensure(std::max(bout, (size_t)64))))
+        return rc;
+    std::memcpy(ctx->h_in.p, in, bin);
+    TP_HIP(hipMemsetAsync(ctx->d_flag.p, 0, sizeof(int), ctx->stream));
+    TP_HIP(hipMemcpyAsync(ctx->d_in.p, ctx->h_in.p, bin, hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = launch((const double *)ctx->d_in.p, (double *)ctx->d_out.p, (int *)ctx->d_flag.p))) return rc;
+    TP_HIP(hipMemcpyAsync(ctx->h_out.p, ctx->d_out.p, bout, hipMemcpyDeviceToHost, ctx->stream));
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    std::memcpy(out, ctx->h_out.p, bout);
+    int flag = 0;
+    if ((rc = read_flag(ctx, &flag))) return rc;
+    if (flag) return tp_set_error(TP_ENONFINITE, std::string(what) + ": non-finite ell or B_kappa value");
+    return 0;
+}
+
+extern "C" int tp_bkappa_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *ell123,
+                              double *out, int mode)
+{
+    int rc = check_cosmo(ctx, c, icosmo);
+    if (rc) return rc;
+    if (npts && (!ell123 || !out)) return tp_set_error(TP_EINVAL, "tp_bkappa_eval: NULL buffer");
+    if (mode != TP_MODE_FP64 && mode != TP_MODE_FP32 && mode != TP_MODE_ANALYTIC)
+        return tp_set_error(TP_EINVAL, "tp_bkappa_eval: mode must be FP64, FP32 or ANALYTIC");
+    TP_HIP(hipSetDevice(ctx->device));
+    return run_points(
+        ctx, npts, 3, 1, ell123, out,
+        [&](const double *dx, double *dy, int *fl) { return tp_launch_bkappa_points(ctx, c, icosmo, npts, dx, dy, mode, fl); },
+        "tp_bkappa_eval");
+}
+
+extern "C" int tp_map3_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const double theta[3], size_t npts,
+                            const double *x, double *fval, int mode)
+{
+    int rc = check_cosmo(ctx, c, icosmo);
+    if (rc) return rc;
+    if (!theta || (npts && (!x || !fval))) return tp_set_error(TP_EINVAL, "tp_map3_eval: NULL buffer");
+    if (mode != TP_MODE_FP64 && mode != TP_MODE_FP32 && mode != TP_MODE_ANALYTIC)
+        return tp_set_error(TP_EINVAL, "tp_map3_eval: mode must be FP64, FP32 or ANALYTIC");
+    TP_HIP(hipSetDevice(ctx->device));
+    return run_points(
+        ctx, npts, 3, 1, x, fval,
+        [&](const double *dx, double *dy, int *fl) {
+            return tp_launch_map3_points(ctx, c, icosmo, theta, npts, dx, dy, mode, fl);
+        },
+        "tp_map3_eval");
+}
+
+// ------------------------------------------------------------- hcubature_v
+static thread_local long long g_last_neval = 0;
+extern "C" long long tp_hcubature_last_neval(void) { return g_last_neval; }
+
+extern "C" int tp_hcubature_v(unsigned fdim, tp_integrand_v f, void *fdata, unsigned dim, const double *xmin,
+                              const double *xmax, size_t maxEval, double reqAbsError, double reqRelError, int norm,
+                              double *val, double *err)
+{
+    // classed_version/hcubature.cpp:1023-1068
+    g_last_neval = 0;
+    if (fdim == 0) return 0;
+    if (dim == 0) {
+        if (f(0, 1, xmin, fdata, fdim, val)) return 1;
+        for (unsigned i = 0; i < fdim; ++i) err[i] = 0;
+        return 0;
+    }
+    tpc::AdaptiveIntegral I;
+    if (!I.init(dim, fdim, xmin, xmax, maxEval, reqAbsError, reqRelError, norm)) {
+        for (unsigned i = 0; i < fdim; ++i) {
+            val[i] = 0;
+            err[i] = HUGE_VAL;
+        }
+        tp_set_error(TP_EINVAL, "tp_hcubature_v: unsupported dim/fdim/norm");
+        return 1;
+    }
+    const unsigned np = tpc::rule_points(dim);
+    std::vector<double> pts, vals;
+    for (;;) {
+        const auto &pend = I.pending();
+        size_t nR = pend.size();
+        pts.resize(nR * np * dim);
+        vals.resize(nR * np * fdim);
+        for (size_t r = 0; r < nR; ++r) tpc::rule_points_of(dim, I.regions()[pend[r]], pts.data() + r * np * dim);
+        if (f(dim, nR * np, pts.data(), fdata, fdim, vals.data())) {
+            tp_set_error(TP_ECALLBACK, "tp_hcubature_v: integrand callback returned nonzero");
+            return 1;
+        }
+        for (size_t r = 0; r < nR; ++r)
+            tpc::rule_apply(dim, fdim, I.regions()[pend[r]], vals.data() + r * np * fdim);
+        I.absorb();
+        if (I.advance()) break;
+    }
+    I.result(val, err);
+    g_last_neval = (long long)I.num_eval();
+    return 0;
+}
+
+// --------------------------------------------------- one triple, plain path
+struct Map3CbData {
+    tp_ctx *ctx;
+    const tp_cosmo *c;
+    int icosmo, mode;
+    double th[3];
+    int rc;
+};
+
+static int map3_cb(unsigned ndim, size_t npt, const double *x, void *fdata, unsigned fdim, double *fval)
+{
+    // the integrand_v adapter of classed_version/apertureStatistics.cpp:254-279, on the GPU
+    Map3CbData *d = (Map3CbData *)fdata;
+    if (ndim != 3 || fdim != 1) {
+        d->rc = tp_set_error(TP_EINVAL, "map3 integrand: wrong dimensions");
+        return 1;
+    }
+    d->rc = tp_map3_eval(d->ctx, d->c, d->icosmo, d->th, npt, x, fval, d->mode);
+    return d->rc ? 1 : 0;
+}
+
+extern "C" int tp_map3_single(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const double theta[3], double reltol,
+                              int mode, double *val, double *err, long long *neval)
+{
+    // classed_version/apertureStatistics.cpp:321-353, 3-D branch
+    int rc = check_cosmo(ctx, c, icosmo);
+    if (rc) return rc;
+    if (!theta || !val) return tp_set_error(TP_EINVAL, "tp_map3_single: NULL");
+    double tmin = std::min(theta[0], std::min(theta[1], theta[2]));
+    if (!(tmin > 0)) return tp_set_error(TP_EINVAL, "tp_map3_single: theta must be positive");
+    double lMax = 10. / tmin;
+    double vmin[3] = {1., 1., 0.}, vmax[3] = {lMax, lMax, 2 * M_PI};
+    Map3CbData d{ctx, c, icosmo, mode, {theta[0], theta[1], theta[2]}, 0};
+    double r = 0, e = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    int st = tp_hcubature_v(1, map3_cb, &d, 3, vmin, vmax, 0, 0, reltol, tpc::L1, &r, &e);
+    if (st) return d.rc ? d.rc : tp_set_error(TP_EINVAL, "tp_map3_single: cubature failed");
+    *val = r / 8 / M_PI / M_PI / M_PI;
+    if (err) *err = e / 8 / M_PI / M_PI / M_PI;
+    if (neval) *neval = g_last_neval;
+    ctx->stats = tp_batch_stats{};
+    ctx->stats.points = g_last_neval;
+    ctx->stats.wall_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+// ----------------------------------------------------------- batched engine
+int tp_surrogate_round(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegion *d_reg,
+                       const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int *d_flag,
+                       int *fell_back);  // tp_mlp.hip
+
+extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, const int *icosmo,
+                             const double *theta_rad, double reltol, int mode, double *val, double *err,
+                             long long *neval)
+{
+    if (!ctx || !c || !icosmo || !theta_rad || !val) return tp_set_error(TP_EINVAL, "tp_map3_batch: NULL");
+    if (mode < TP_MODE_FP64 || mode > TP_MODE_ANALYTIC) return tp_set_error(TP_EINVAL, "tp_map3_batch: bad mode");
+    if (mode == TP_MODE_SURROGATE && !ctx->sur)
+        return tp_set_error(TP_EINVAL, "tp_map3_batch: surrogate mode needs a trained or loaded surrogate");
+    if (c->device != ctx->device) return tp_set_error(TP_EINVAL, "tp_map3_batch: cosmology on another device");
+    if (ntriples == 0) return 0;
+    TP_HIP(hipSetDevice(ctx->device));
+    auto t0 = std::chrono::steady_clock::now();
+    ctx->stats = tp_batch_stats{};
+    std::vector<TpIntegralDesc> desc(ntriples);
+    std::vector<tpc::AdaptiveIntegral> I(ntriples);
+    for (size_t t = 0; t < ntriples; ++t) {
+        const double *th = theta_rad + 3 * t;
+        if (icosmo[t] < 0 || icosmo[t] >= c->ncosmo) return tp_set_error(TP_EINVAL, "tp_map3_batch: bad icosmo");
+        double tmin = std::min(th[0], std::min(th[1], th[2]));
+        if (!(tmin > 0)) return tp_set_error(TP_EINVAL, "tp_map3_batch: theta must be positive");
+        desc[t] = TpIntegralDesc{{th[0], th[1], th[2]}, icosmo[t], 0};
+        double lMax = 10. / tmin;  // classed_version/apertureStatistics.cpp:325-326
+        double vmin[3] = {1., 1., 0.}, vmax[3] = {lMax, lMax, 2 * M_PI};
+        I[t].init(3, 1, vmin, vmax, 0, 0, reltol, tpc::L1);
+    }
+    int rc;
+    if ((rc = ctx->d_desc.ensure(sizeof(TpIntegralDesc) * ntriples))) return rc;
+    TP_HIP(hipMemcpyAsync(ctx->d_desc.p, desc.data(), sizeof(TpIntegralDesc) * ntriples, hipMemcpyHostToDevice,
+                          ctx->stream));
+    std::vector<size_t> active(ntriples), offs;
+    for (size_t t = 0; t < ntriples; ++t) active[t] = t;
+    const int nthreads = std::max(1, std::min(omp_get_max_threads(), 16));
+    while (!active.empty()) {
+        size_t na = active.size();
+        offs.assign(na + 1, 0);
+        for (size_t a = 0; a < na; ++a) offs[a + 1] = offs[a] + I[active[a]].pending().size();
+        size_t nreg = offs[na];
+        if ((rc = ctx->h_regions.ensure(sizeof(TpRegion) * nreg)) || (rc = ctx->d_regions.ensure(sizeof(TpRegion) * nreg)) ||
+            (rc = ctx->h_rout.ensure(sizeof(TpRegionOut) * nreg)) || (rc = ctx->d_rout.ensure(sizeof(TpRegionOut) * nreg)) ||
+            (rc = ctx->d_fvals.ensure(sizeof(double) * TP_NPTS_GM3 * nreg)))
+            return rc;
+        TpRegion *hr = (TpRegion *)ctx->h_regions.p;
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nthreads) if (na > 64)
+        for (size_t a = 0; a < na; ++a) {
+            tpc::AdaptiveIntegral &A = I[active[a]];
+            const auto &pend = A.pending();
+            for (size_t k = 0; k < pend.size(); ++k) {
+                const tpc::Region &R = A.regions()[pend[k]];
+                TpRegion &o = hr[offs[a] + k];
+                for (int d = 0; d < 3; ++d) {
+                    o.c[d] = R.c[d];
+                    o.hw[d] = R.hw[d];
+                }
+                o.vol = R.vol;
+                o.integral = (int)active[a];
+                o.pad = 0;
+            }
+        }
+        TP_HIP(hipMemsetAsync(ctx->d_flag.p, 0, sizeof(int), ctx->stream));
+        TP_HIP(hipMemcpyAsync(ctx->d_regions.p, hr, sizeof(TpRegion) * nreg, hipMemcpyHostToDevice, ctx->stream));
+        TP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        if (mode == TP_MODE_SURROGATE) {
+            int fb = 0;
+            rc = tp_surrogate_round(ctx, c, nreg, (const TpRegion *)ctx->d_regions.p,
+                                    (const TpIntegralDesc *)ctx->d_desc.p, (double *)ctx->d_fvals.p,
+                                    (TpRegionOut *)ctx->d_rout.p, (int *)ctx->d_flag.p, &fb);
+            if (rc) return rc;
+            ctx->stats.fallback_batches += fb;
+        } else {
+            rc = tp_launch_regions(ctx, c, nreg, (const TpRegion *)ctx->d_regions.p,
+                                   (const TpIntegralDesc *)ctx->d_desc.p, (double *)ctx->d_fvals.p,
+                                   (TpRegionOut *)ctx->d_rout.p, mode, (int *)ctx->d_flag.p);
+            if (rc) return rc;
+        }
+        TP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        TP_HIP(hipMemcpyAsync(ctx->h_rout.p, ctx->d_rout.p, sizeof(TpRegionOut) * nreg, hipMemcpyDeviceToHost,
+                              ctx->stream));
+        int flag = 0;
+        if ((rc = read_flag(ctx, &flag))) return rc;
+        float ms = 0;
+        hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+        ctx->stats.kernel_ms += ms;
+        ctx->stats.launches += 1;
+        ctx->stats.rounds += 1;
+        ctx->stats.regions += (long long)nreg;
+        ctx->stats.points += (long long)nreg * TP_NPTS_GM3;
+        if (flag) return tp_set_error(TP_ENONFINITE, "tp_map3_batch: non-finite ell or B_kappa value");
+        const TpRegionOut *ro = (const TpRegionOut *)ctx->h_rout.p;
+        std::vector<char> finished(na, 0);
+        int bad = 0;
+#pragma omp parallel for schedule(dynamic, 8) num_threads(nthreads) if (na > 64) reduction(| : bad)
+        for (size_t a = 0; a < na; ++a) {
+            tpc::AdaptiveIntegral &A = I[active[a]];
+            const auto &pend = A.pending();
+            for (size_t k = 0; k < pend.size(); ++k) {
+                const TpRegionOut &o = ro[offs[a] + k];
+                tpc::Region &R = A.regions()[pend[k]];
+                R.val[0] = o.val;
+                R.err[0] = o.err;
+                R.split = (unsigned)o.split;
+                R.errmax = o.err;
+                bad |= o.flag;
+            }
+            A.absorb();
+            finished[a] = A.advance() ? 1 : 0;
+        }
+        if (bad) return tp_set_error(TP_ENONFINITE, "tp_map3_batch: non-finite region estimate");
+        std::vector<size_t> next;
+        next.reserve(na);
+        for (size_t a = 0; a < na; ++a)
+            if (!finished[a]) next.push_back(active[a]);
+        active.swap(next);
+    }
+    for (size_t t = 0; t < ntriples; ++t) {
+        double r, e;
+        I[t].result(&r, &e);
+        val[t] = r / 8 / M_PI / M_PI / M_PI;
+        if (err) err[t] = e / 8 / M_PI / M_PI / M_PI;
+        if (neval) neval[t] = (long long)I[t].num_eval();
+    }
+    ctx->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}

@@ -1,0 +1,142 @@
+"""Seeded synthetic inputs of the measured workloads.
+
+  * fiducial cosmology: flat wCDM h=0.7, Omega_m=0.3, Omega_b=0.047, sigma8=0.82,
+    n_s=0.97, w=-1; source n(z) ~ z^2 exp(-(z/0.5)^1.5) tabulated on [0, 3] with
+    dz = 0.01, resampled to 100 bins with z_max = 3 (classed_version/helper.cpp:31-96).
+  * map3_c0: radii {2, 4, 8, 16} arcmin -> 20 triples theta1 <= theta2 <= theta3.
+  * map3_c1: 10 log-spaced radii 0.5 ... 32 arcmin -> 220 triples.
+  * gamma_c2: TreeCorr-style (r, u, v) grid, r in 10 log bins 0.1..100 arcmin,
+    u in 10 linear bins over [0, 1], v in 10 linear bins over [-1, 1] -> 1000 triangles.
+  * lhs_c3: 512-point Latin hypercube over (Omega_m, sigma8, h, w, n_s) with
+    Omega_b = 0.047; radii {2, 4, 8, 16, 32} arcmin -> 35 triples per cosmology.
+"""
+import numpy as np
+
+from . import ARCMIN, Cosmology
+
+FIDUCIAL = dict(h=0.7, sigma8=0.82, omb=0.047, ns=0.97, w=-1.0, om=0.3, zmax=3.0)
+N_Z = 100
+Z_MAX = 3.0
+LHS_BOX = dict(om=(0.2, 0.4), sigma8=(0.7, 0.95), h=(0.6, 0.8), w=(-1.2, -0.8), ns=(0.92, 1.0))
+
+
+def fiducial():
+    return Cosmology(**FIDUCIAL)
+
+
+def nz_file_rows():
+    """The (z, n) rows of the synthetic n(z) file: 301 samples on [0, 3], dz = 0.01."""
+    z = np.arange(301) * 0.01
+    n = z ** 2 * np.exp(-(z / 0.5) ** 1.5)
+    return z, n
+
+
+def resample_n_of_z(zs, ns, n_bins=N_Z, z_max=Z_MAX):
+    """classed_version/helper.cpp:60-95: linear resampling to z_i = i z_max / n_bins, sum n dz = 1."""
+    zs = np.asarray(zs, dtype=np.float64)
+    ns = np.asarray(ns, dtype=np.float64)
+    nf = zs.size
+    zmin_f, zmax_f = zs[0], zs[-1]
+    dz_f = (zmax_f - zmin_f) / (nf - 1)
+    dz = z_max / n_bins
+    out = np.zeros(n_bins)
+    for i in range(n_bins):
+        z = i * dz
+        ix = int((z - zmin_f) / dz_f)
+        dix = (z - zmin_f) / dz_f - ix
+        v = 0.0
+        if 0 <= ix < nf - 1:
+            v = ns[ix + 1] * dix + ns[ix] * (1 - dix)
+        if ix == nf - 1:
+            v = ns[nf - 1]
+        out[i] = v
+    norm = 0.0
+    for v in out:  # std::accumulate order
+        norm += v
+    if norm == 0:
+        raise ValueError("sum of n(z) is zero")
+    norm *= dz
+    return out / norm
+
+
+def fiducial_nz():
+    z, n = nz_file_rows()
+    return resample_n_of_z(z, n)
+
+
+def triples(radii_arcmin):
+    """Unordered triples theta_i <= theta_j <= theta_k (cuda_version/calculateApertureStatistics.cu:92-115)."""
+    r = list(radii_arcmin)
+    out = []
+    for i in range(len(r)):
+        for j in range(i, len(r)):
+            for k in range(j, len(r)):
+                out.append((r[i], r[j], r[k]))
+    return np.array(out, dtype=np.float64)
+
+
+def map3_c0_radii():
+    return [2.0, 4.0, 8.0, 16.0]
+
+
+def map3_c1_radii():
+    return [0.5 * 64.0 ** (k / 9.0) for k in range(10)]
+
+
+def map3_c3_radii():
+    return [2.0, 4.0, 8.0, 16.0, 32.0]
+
+
+def to_rad(th_arcmin):
+    return np.asarray(th_arcmin, dtype=np.float64) * ARCMIN
+
+
+def gamma_c2_triangles(rsteps=10, usteps=10, vsteps=10, rmin=0.1, rmax=100.0, umin=0.0, umax=1.0, vmin=-1.0,
+                       vmax=1.0):
+    """cuda_version/calculateGamma.cu:102-121: bin centres and (r1, r2, r3) in rad."""
+    lrmin, lrmax = np.log(rmin), np.log(rmax)
+    rows = []
+    for i in range(rsteps):
+        r = np.exp(lrmin + (lrmax - lrmin) / rsteps * (i + 0.5))
+        for j in range(usteps):
+            u = umin + (umax - umin) / usteps * (j + 0.5)
+            for k in range(vsteps):
+                v = vmin + (vmax - vmin) / vsteps * (k + 0.5)
+                r2 = r * np.pi / 180. / 60.
+                r3 = r2 * u
+                r1 = v * r3 + r2
+                rows.append((r1, r2, r3, r, u, v))
+    return np.array(rows)
+
+
+def splitmix64(state):
+    state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+    return state, z ^ (z >> 31)
+
+
+def lhs_c3(n=512, seed=0xC3):
+    """Latin hypercube: per dimension, a seeded permutation of n strata, uniform jitter inside each."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    names = ["om", "sigma8", "h", "w", "ns"]
+    cols = {}
+    for nm in names:
+        lo, hi = LHS_BOX[nm]
+        perm = rng.permutation(n)
+        u = (perm + rng.random(n)) / n
+        cols[nm] = lo + (hi - lo) * u
+    return [Cosmology(h=cols["h"][i], sigma8=cols["sigma8"][i], omb=0.047, ns=cols["ns"][i], w=cols["w"][i],
+                      om=cols["om"][i], zmax=Z_MAX) for i in range(n)]
+
+
+def bkappa_c1_points(n=1000000, seed=0xC1B0D5):
+    """(l1, l2, phi) with ln l1, ln l2 ~ U[1, 12] and phi ~ U[0, pi]; l3 from the law of cosines."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    u = rng.random((n, 3))
+    l1 = np.exp(1.0 + 11.0 * u[:, 0])
+    l2 = np.exp(1.0 + 11.0 * u[:, 1])
+    phi = np.pi * u[:, 2]
+    l3 = np.sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * np.cos(phi))
+    return np.stack([l1, l2, phi], 1), np.stack([l1, l2, l3], 1)
