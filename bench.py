@@ -1,0 +1,193 @@
+#!/usr/bin/env python
+"""<Map^3> throughput on one MI355X: plain FP64 path vs batched / reduced-precision forms.
+
+Workload (one "step"): the 220 aperture-radius triples of 10 log-spaced radii
+0.5 ... 32 arcmin, flat wCDM h=0.7, Omega_m=0.3, Omega_b=0.047, sigma8=0.82,
+n_s=0.97, w=-1, n(z) ~ z^2 exp(-(z/0.5)^1.5) on [0, 3] (100 bins), each
+<Map^3> an h-adaptive cubature over (l1, l2, phi) with relative tolerance 1e-4.
+Synthetic inputs (no data files); the cosmology tables are built on the GPU once,
+before the timed region, and stay resident in HBM.
+
+Paths timed in the same run:
+  plain_fp64    one triple after another through tp_map3_single: the reference's
+                hcubature_v driving an integrand callback that crosses to the GPU
+                every cubature round (cuda_version/apertureStatistics.cu:155-229).
+  batched_fp64  all triples advance together, one integrand + rule launch per round.
+  batched_fp32  same, FP32 bispectrum with FP64 z-sum and cubature accumulation.
+  surrogate     batched, learned B_kappa with exact checks (when a model is present).
+
+value = triples/s of the fastest path whose max relative error against the
+batched FP64 result (itself within 1e-6 of the CPU FP64 oracle) is <= 1e-3.
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
+FLOPS_PER_ELEM = 20.0
+
+
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def run_path(tabs, ctx, thetas, path, steps, warmup, reltol):
+    import threepoint_amd as tp
+    mode = {"plain_fp64": tp.MODE_FP64, "batched_fp64": tp.MODE_FP64, "batched_fp32": tp.MODE_FP32,
+            "surrogate": tp.MODE_SURROGATE}[path]
+    n = thetas.shape[0]
+
+    def one_step():
+        if path == "plain_fp64":
+            vals = np.zeros(n)
+            pts = 0
+            for i in range(n):
+                v, e, ne = tabs.map3_single(thetas[i], reltol, 0, mode)
+                vals[i] = v
+                pts += ne
+            return vals, {"points": pts}
+        v, e, ne = tabs.map3_batch(thetas, None, reltol, mode)
+        return v, ctx.stats()
+
+    for _ in range(warmup):
+        one_step()
+    agg = {"points": 0, "integrand_ms": 0.0, "kernel_ms": 0.0, "launches": 0, "rounds": 0}
+    t0 = time.perf_counter()
+    vals = None
+    for _ in range(steps):
+        vals, st = one_step()
+        for k in agg:
+            if k in st:
+                agg[k] += st[k]
+    dt = time.perf_counter() - t0
+    agg["ms_per_step"] = 1e3 * dt / steps
+    agg["triples_per_s"] = n * steps / dt
+    return vals, agg
+
+
+def cpu_baseline(thetas, seconds_target, reltol):
+    """The CPU FP64 oracle (OpenMP over each integrand batch) on a bounded sample of the triples."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _oracle import Oracle
+    from threepoint_amd import workloads as W
+    cores = os.cpu_count() or 1
+    cores = min(cores, 64)
+    o = Oracle(W.fiducial(), W.fiducial_nz())
+    # sample: every 11th triple of the 220 (20 triples spread over all radius combinations),
+    # stopped once the time budget is spent
+    order = list(range(0, thetas.shape[0], 11))
+    done, t0 = 0, time.perf_counter()
+    for i in order:
+        o.map3(thetas[i], reltol, cores)
+        done += 1
+        if time.perf_counter() - t0 > seconds_target:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "triples/s", "cores": cores, "kind": "port",
+            "sample": "%d of the 220 triples (every 11th, in order) through the FP64 CPU oracle, OpenMP %d threads, "
+                      "%s, %.1f s" % (done, cores, cpu_model(), dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--reltol", type=float, default=1e-4)
+    ap.add_argument("--paths", default="plain_fp64,batched_fp64,batched_fp32,surrogate")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--surrogate", default=os.path.join(ROOT, "threepoint_amd", "models", "bkappa_mlp.bin"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the path shards by cosmology/triple with no exchange "
+              "(see DESIGN.md), and multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+
+    import threepoint_amd as tp
+    from threepoint_amd import workloads as W
+
+    ctx = tp.default_context()
+    t0 = time.perf_counter()
+    tabs = tp.CosmologyTables([W.fiducial()], W.fiducial_nz(), 100, 3.0, ctx)
+    tables_ms = 1e3 * (time.perf_counter() - t0)
+    thetas = W.to_rad(W.triples(W.map3_c1_radii()))
+
+    paths = [p for p in args.paths.split(",") if p]
+    have_sur = False
+    if "surrogate" in paths:
+        have_sur = os.path.exists(args.surrogate) and tp.lib().tp_surrogate_load(ctx.h, args.surrogate.encode()) == 0
+        if not have_sur:
+            paths.remove("surrogate")
+
+    census = tp.census_map3_point()
+    fe_point = tp.flop_equivalents_per_point(FLOPS_PER_ELEM)
+    results, values = {}, {}
+    for p in paths:
+        print("bench: path %s ..." % p, file=sys.stderr, flush=True)
+        steps = 1 if p == "plain_fp64" else args.steps
+        v, agg = run_path(tabs, ctx, thetas, p, steps, 0 if p == "plain_fp64" else args.warmup, args.reltol)
+        values[p] = v
+        results[p] = agg
+    ref = values.get("batched_fp64")
+    for p in results:
+        if ref is not None:
+            # triples whose first cubature region integrates to exactly 0 end at 0 in every path
+            # (the reference algorithm's behaviour); they must stay exactly 0
+            nz = ref != 0
+            err = np.abs(values[p][nz] - ref[nz]) / np.abs(ref[nz])
+            zero_ok = bool(np.all(values[p][~nz] == 0))
+            results[p]["max_rel_err_vs_fp64"] = float(err.max()) if zero_ok else float("inf")
+            results[p]["zero_triples"] = int((~nz).sum())
+        r = results[p]
+        if r.get("integrand_ms", 0) > 0:
+            ach = fe_point * r["points"] / (r["integrand_ms"] * 1e-3)
+            peak = PEAK["fp32"] if p == "batched_fp32" else PEAK["fp64"]
+            r["integrand_kernel"] = {"achieved_flops": ach, "peak": peak, "frac": ach / peak,
+                                     "avg_launch_ms": r["integrand_ms"] / max(1, r["launches"])}
+    ok = [p for p in results if results[p].get("max_rel_err_vs_fp64", 1.0) <= 1e-3]
+    head = max(ok, key=lambda p: results[p]["triples_per_s"]) if ok else "batched_fp64"
+    hr = results[head]
+    dom = hr.get("integrand_kernel") or results["batched_fp64"]["integrand_kernel"]
+    roof = {"bound": "valu", "kernel": "k_region_points<%s>" % ("float" if head == "batched_fp32" else "double"),
+            "achieved": dom["achieved_flops"] / 1e12, "peak": dom["peak"] / 1e12, "unit": "TFLOP/s",
+            "frac": dom["frac"], "traffic": None,
+            "work_per_point_flop_eq": fe_point, "flops_per_elementary_call": FLOPS_PER_ELEM,
+            "census": census}
+    line = {
+        "metric": "Map3 triples/s on 1 GPU: plain FP64 vs surrogate form at <=1e-3 rel err",
+        "value": hr["triples_per_s"], "unit": "triples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": hr["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if head == "batched_fp32" else ("bf16+f64" if head == "surrogate" else "f64"),
+        "data": "synthetic",
+        "config": {"workload": "Map3 of 220 triples (10 log radii 0.5-32 arcmin), fiducial flat LCDM, "
+                               "100 z-bins, hcubature rel tol %g" % args.reltol,
+                   "path": head, "triples_per_step": int(thetas.shape[0]), "tables_ms": tables_ms},
+        "paths": results,
+        "speedup_vs_plain_fp64": (hr["triples_per_s"] / results["plain_fp64"]["triples_per_s"]
+                                  if "plain_fp64" in results else None),
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        print("bench: cpu baseline ...", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(thetas, args.cpu_seconds, args.reltol)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -102,6 +102,7 @@ int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, const int *ic
 typedef struct {
     long long rounds, regions, points, launches;
     double kernel_ms;            /* summed duration of the integrand/rule kernels */
+    double integrand_ms;         /* summed duration of the integrand (point) kernel only */
     double wall_ms;              /* whole call                                     */
     long long surrogate_points;  /* points evaluated by the surrogate              */
     long long exact_checks;      /* points re-evaluated exactly                    */
@@ -109,6 +110,12 @@ typedef struct {
     double max_check_relerr;     /* worst relative error seen by the exact checks  */
 } tp_batch_stats;
 int tp_last_batch_stats(const tp_ctx *ctx, tp_batch_stats *out);
+
+/* Algorithmic operation census of one <Map^3> integrand point of the FP64/FP32 kernels:
+ * out[0] flops per z-node, out[1] exp/log calls per z-node, out[2] flops and out[3]
+ * elementary calls per point outside the node loop, out[4] z-nodes per point.  Counted by
+ * instantiating the kernel's own device functions with a counting scalar on the host. */
+int tp_census_map3_point(double *out);
 
 /* ---------------------------------------------------------------- Gamma_i */
 /* Replaces GammaCalculator(bk, prec_h, prec_k, "centroid") + initialize_bessel

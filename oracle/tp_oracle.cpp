@@ -937,7 +937,11 @@ struct Calc {
     { /* CV/bispectrum.cpp:238-286 with GQ96_of_bdelta CV:1010-1020 */
         if (ell1 == 0 || ell2 == 0 || ell3 == 0) return 0;
         if (analytic) {
-            double phi = std::acos((ell1 * ell1 + ell2 * ell2 - ell3 * ell3) / (2 * ell1 * ell2));
+            /* the argument is clamped to [-1, 1]; the reference's bkappa branch has no guard and
+               returns NaN where rounding pushes it past 1 (phi = 0, pi) */
+            double carg = (ell1 * ell1 + ell2 * ell2 - ell3 * ell3) / (2 * ell1 * ell2);
+            carg = std::min(1.0, std::max(-1.0, carg));
+            double phi = std::acos(carg);
             static const double wts[9] = {0, 1.0e+8, 1.0e+6, 1.0e+4, 1.0e+2, 1.0e+0, 5.0e-3, 1.0e-5, 1.0e-6};
             static const double avs[9] = {3.0e+3, 1.0e+4, 3.0e+4, 1.0e+5, 3.0e+5, 1.0e+6, 3.0e+6, 1.0e+7, 3.0e+7};
             double temp = 0;

@@ -37,7 +37,8 @@ class tp_cosmology(ctypes.Structure):
 
 class tp_batch_stats(ctypes.Structure):
     _fields_ = [("rounds", ctypes.c_longlong), ("regions", ctypes.c_longlong), ("points", ctypes.c_longlong),
-                ("launches", ctypes.c_longlong), ("kernel_ms", ctypes.c_double), ("wall_ms", ctypes.c_double),
+                ("launches", ctypes.c_longlong), ("kernel_ms", ctypes.c_double), ("integrand_ms", ctypes.c_double),
+                ("wall_ms", ctypes.c_double),
                 ("surrogate_points", ctypes.c_longlong), ("exact_checks", ctypes.c_longlong),
                 ("fallback_batches", ctypes.c_longlong), ("max_check_relerr", ctypes.c_double)]
 
@@ -84,6 +85,7 @@ _SIGS = {
     "tp_hcubature_v": (ctypes.c_int, [ctypes.c_uint, INTEGRAND_V, ctypes.c_void_p, ctypes.c_uint, _dp, _dp,
                                       ctypes.c_size_t, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp]),
     "tp_hcubature_last_neval": (ctypes.c_longlong, []),
+    "tp_census_map3_point": (ctypes.c_int, [_dp]),
     "tp_surrogate_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(tp_train_config),
                                           ctypes.POINTER(tp_train_report)]),
     "tp_surrogate_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
@@ -282,6 +284,19 @@ class CosmologyTables:
         _check(lib().tp_gamma_batch(self.ctx.h, self.h, icosmo, n, _d(x), reltol, _d(re), _d(im),
                                     ne.ctypes.data_as(_llp)), "tp_gamma_batch")
         return (re + 1j * im).reshape(n, 4), ne.reshape(n, 4)
+
+
+def census_map3_point():
+    """Algorithmic op census of one <Map^3> integrand point (see tp_census_map3_point)."""
+    out = np.zeros(5)
+    _check(lib().tp_census_map3_point(_d(out)), "tp_census_map3_point")
+    return dict(node_flops=out[0], node_elem=out[1], point_flops=out[2], point_elem=out[3], nodes=out[4])
+
+
+def flop_equivalents_per_point(flops_per_elem=20.0):
+    c = census_map3_point()
+    return c["nodes"] * (c["node_flops"] + flops_per_elem * c["node_elem"]) + c["point_flops"] + \
+        flops_per_elem * c["point_elem"]
 
 
 def gamma_setup(prec_h=1e-3, prec_k_arg=5.0, ctx=None):

@@ -18,7 +18,7 @@ LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libthreepoint_amd.so")
 
 HIP_SOURCES = ["tp_tables.hip", "tp_map3.hip", "tp_runtime.hip", "tp_gamma.hip", "tp_mlp.hip"]
-CPP_SOURCES = ["tp_cubature.cpp", "tp_classes.cpp", "tp_io.cpp"]
+CPP_SOURCES = ["tp_cubature.cpp", "tp_census.cpp", "tp_classes.cpp", "tp_io.cpp"]
 APPS = ["calculateApertureStatistics", "calculateGamma"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

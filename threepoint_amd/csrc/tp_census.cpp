@@ -1,0 +1,80 @@
+// tp_census.cpp -- algorithmic operation census of one <Map^3> integrand point.
+//
+// The kernel's own per-point and per-node functions (tp_device.h) are instantiated
+// on the host with a counting scalar: every +, -, *, / is one flop (a fused
+// multiply-add is counted as its multiply and its add, i.e. two), every exp / log
+// is one elementary-function call.  Comparisons, selects and loads are not counted.
+#include <cmath>
+
+#include "tp_device.h"
+
+namespace {
+struct OpCounts {
+    long long flops = 0, trans = 0;
+};
+thread_local OpCounts g_cnt;
+
+struct Cnt {
+    double v;
+    Cnt(double x = 0) : v(x) {}
+};
+inline Cnt operator+(Cnt a, Cnt b) { ++g_cnt.flops; return Cnt(a.v + b.v); }
+inline Cnt operator-(Cnt a, Cnt b) { ++g_cnt.flops; return Cnt(a.v - b.v); }
+inline Cnt operator*(Cnt a, Cnt b) { ++g_cnt.flops; return Cnt(a.v * b.v); }
+inline Cnt operator/(Cnt a, Cnt b) { ++g_cnt.flops; return Cnt(a.v / b.v); }
+inline Cnt operator-(Cnt a) { return Cnt(-a.v); }
+inline Cnt &operator+=(Cnt &a, Cnt b) { a = a + b; return a; }
+inline Cnt &operator*=(Cnt &a, Cnt b) { a = a * b; return a; }
+inline bool operator<(Cnt a, Cnt b) { return a.v < b.v; }
+inline bool operator>(Cnt a, Cnt b) { return a.v > b.v; }
+}  // namespace
+
+template <> struct TpMath<Cnt> {
+    static Cnt ex(Cnt x) { ++g_cnt.trans; return Cnt(std::exp(x.v)); }
+    static Cnt lg(Cnt x) { ++g_cnt.trans; return Cnt(std::log(x.v)); }
+};
+
+// out: [0] flops per node, [1] elementary calls per node, [2] flops per point outside
+// the node loop, [3] elementary calls per point outside the node loop, [4] nodes per point
+extern "C" int tp_census_map3_point(double *out)
+{
+    TpNodeT<Cnt> N;
+    double *dst = nullptr;
+    (void)dst;
+    // representative node values (the census does not depend on them except through branches)
+    Cnt vals[sizeof(TpNodeT<Cnt>) / sizeof(Cnt)];
+    for (auto &v : vals) v = Cnt(0.5);
+    N = *reinterpret_cast<TpNodeT<Cnt> *>(vals);
+    N.acap = Cnt(10.0);
+    TpEHT<Cnt> E;
+    Cnt ev[sizeof(TpEHT<Cnt>) / sizeof(Cnt)];
+    for (auto &v : ev) v = Cnt(0.7);
+    E = *reinterpret_cast<TpEHT<Cnt> *>(ev);
+    // per point: l3, the hoisted scale-free quantities, the three filters and the final product
+    g_cnt = OpCounts();
+    Cnt l1(300.), l2(500.), phi(1.0);
+    Cnt s = l1 * l1 + l2 * l2 + Cnt(2) * l1 * l2 * TpMath<Cnt>::ex(phi);  // cos ~ one elementary call
+    Cnt l3 = TpMath<Cnt>::ex(s);                                           // sqrt ~ one elementary call
+    (void)l3;
+    TpPointPreT<Cnt> P;
+    tp_point_pre<Cnt>(Cnt(300.), Cnt(500.), Cnt(600.), &P);
+    Cnt f = l1 * l2 * Cnt(1.0) * tp_uhat<Cnt>(l1 * Cnt(1e-3)) * tp_uhat<Cnt>(l2 * Cnt(1e-3)) *
+            tp_uhat<Cnt>(Cnt(600.) * Cnt(1e-3));
+    (void)f;
+    OpCounts point = g_cnt;
+    // per node: wavenumbers, logs, bispectrum, weight and accumulation (tp_bkappa_sum body)
+    g_cnt = OpCounts();
+    Cnt L[3] = {Cnt(300.), Cnt(500.), Cnt(600.)};
+    Cnt q[3] = {L[0] * N.ks, L[1] * N.ks, L[2] * N.ks};
+    Cnt lq[3] = {P.ll[0] + N.lnks, P.ll[1] + N.lnks, P.ll[2] + N.lnks};
+    Cnt b = tp_bispec_node<Cnt>(N, E, q, lq, P.f2t, P.lr1, P.r2);
+    Cnt acc(0.0);
+    acc += N.wz * b;
+    OpCounts node = g_cnt;
+    out[0] = (double)node.flops;
+    out[1] = (double)node.trans;
+    out[2] = (double)point.flops;
+    out[3] = (double)point.trans;
+    out[4] = (double)TP_NNODES;
+    return 0;
+}

@@ -9,8 +9,8 @@ constexpr double kPi = 3.14159265358979323846;
 
 template <class T> struct TpMath;
 template <> struct TpMath<double> {
-    static __device__ __forceinline__ double ex(double x) { return exp(x); }
-    static __device__ __forceinline__ double lg(double x) { return log(x); }
+    static __host__ __device__ __forceinline__ double ex(double x) { return exp(x); }
+    static __host__ __device__ __forceinline__ double lg(double x) { return log(x); }
 };
 template <> struct TpMath<float> {
     // native transcendental unit: v_exp_f32 / v_log_f32 with a scale
@@ -19,7 +19,7 @@ template <> struct TpMath<float> {
 };
 
 template <class T>
-__device__ __forceinline__ T tp_f2_tree(T k1, T k2, T k3)
+__host__ __device__ __forceinline__ T tp_f2_tree(T k1, T k2, T k3)
 {
     // classed_version/bispectrum.cpp:629-633 (scale-free, so q stands for k)
     T c12 = T(0.5) * (k3 * k3 - k1 * k1 - k2 * k2) / (k1 * k2);
@@ -28,7 +28,7 @@ __device__ __forceinline__ T tp_f2_tree(T k1, T k2, T k3)
 
 // Eisenstein & Hu no-wiggle P_lin at k = q / r_sigma (classed_version/bispectrum.cpp:811-841)
 template <class T>
-__device__ __forceinline__ T tp_plin_q(const TpNodeT<T> &N, const TpEHT<T> &E, T q, T lq)
+__host__ __device__ __forceinline__ T tp_plin_q(const TpNodeT<T> &N, const TpEHT<T> &E, T q, T lq)
 {
     T kk = q * N.inv_rs * E.h;  // 1/Mpc
     T x = E.sh043 * kk;
@@ -42,7 +42,68 @@ __device__ __forceinline__ T tp_plin_q(const TpNodeT<T> &N, const TpEHT<T> &E, T
     return N.pk_pref * delk / (q * q * q);
 }
 
+// Scale-free per-point quantities, evaluated once per point in FP64: q_i = l_i ks_n for
+// every node n, so ratios of wavenumbers, the F2 tree kernels (classed_version/
+// bispectrum.cpp:629-633) and r1, r2 of eq. B8 (:179-192) do not depend on the node.
+template <class T>
+struct TpPointPreT {
+    T l[3];    // ell1, ell2, ell3
+    T ll[3];   // ln ell_i
+    T f2t[3];  // F2_tree(k1,k2,k3), F2_tree(k2,k3,k1), F2_tree(k3,k1,k2)
+    T lr1;     // ln r1, r1 = q_min / q_max
+    T r2;      // (q_mid + q_min - q_max) / q_max
+};
+typedef TpPointPreT<double> TpPointPre;
+
+template <class T>
+__host__ __device__ __forceinline__ void tp_point_pre(T l1, T l2, T l3, TpPointPreT<T> *P)
+{
+    P->l[0] = l1;
+    P->l[1] = l2;
+    P->l[2] = l3;
+    P->ll[0] = TpMath<T>::lg(l1);
+    P->ll[1] = TpMath<T>::lg(l2);
+    P->ll[2] = TpMath<T>::lg(l3);
+    P->f2t[0] = tp_f2_tree<T>(l1, l2, l3);
+    P->f2t[1] = tp_f2_tree<T>(l2, l3, l1);
+    P->f2t[2] = tp_f2_tree<T>(l3, l1, l2);
+    T a = l1, b = l2, c = l3, t;
+    if (a < b) { t = a; a = b; b = t; }
+    if (b < c) { t = b; b = c; c = t; }
+    if (a < b) { t = a; a = b; b = t; }
+    P->lr1 = TpMath<T>::lg(c / a);
+    P->r2 = (b + c - a) / a;
+}
+
 // BiHalofit matter bispectrum at one z-node, classed_version/bispectrum.cpp:166-236
+template <class T>
+__host__ __device__ __forceinline__ T tp_bispec_node(const TpNodeT<T> &N, const TpEHT<T> &E, const T q[3], const T lq[3],
+                                            const T f2t[3], T lr1, T r2)
+{
+    T an = TpMath<T>::ex(N.lnA0 + N.m031ln10 * TpMath<T>::ex(N.c1 * lr1));   // eq. B7
+    T alphan = TpMath<T>::ex(N.a0ln + N.a1ln * r2 * r2);
+    if (alphan > N.acap) alphan = N.acap;
+    T betan = TpMath<T>::ex(N.b0ln + N.b1ln * r2);
+    T pse[3];
+    T bs1h = T(1), damp = T(1);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        const T qi = q[i], lqi = lq[i];
+        T oneh = an * TpMath<T>::ex(alphan * lqi) + N.bn * TpMath<T>::ex(betan * lqi);   // eq. B4
+        bs1h *= T(1) / (oneh * (T(1) + T(1) / (N.cn * qi)));
+        T pl = tp_plin_q(N, E, qi, lqi);
+        T pq = N.pn * qi;
+        pse[i] = (T(1) + N.fn * qi * qi) / (T(1) + N.gn * qi + N.hn * qi * qi) * N.D1sq * pl +
+                 T(1) / ((N.mn * TpMath<T>::ex(N.mun * lqi) + N.nn * TpMath<T>::ex(N.nun * lqi)) *
+                         (T(1) + T(1) / (pq * pq * pq)));                                  // eq. B6
+        damp *= T(1) + N.en * qi;
+    }
+    T bs3h = T(2) * ((f2t[0] + N.dn * q[2]) * pse[0] * pse[1] + (f2t[1] + N.dn * q[0]) * pse[1] * pse[2] +
+                     (f2t[2] + N.dn * q[1]) * pse[2] * pse[0]);                            // eq. B5 with F2 (:614-627)
+    return bs1h + bs3h / damp;
+}
+
+// Same bispectrum from raw wavenumbers (used where the per-point hoisting does not apply)
 template <class T>
 __device__ __forceinline__ T tp_bispec_q(const TpNodeT<T> &N, const TpEHT<T> &E, T q1, T q2, T q3)
 {
@@ -50,31 +111,10 @@ __device__ __forceinline__ T tp_bispec_q(const TpNodeT<T> &N, const TpEHT<T> &E,
     if (a < b) { t = a; a = b; b = t; }
     if (b < c) { t = b; b = c; c = t; }
     if (a < b) { t = a; a = b; b = t; }
-    T r1 = c / a, r2 = (b + c - a) / a;                                       // eq. B8
-    T an = TpMath<T>::ex(N.lnA0 + N.m031ln10 * TpMath<T>::ex(N.c1 * TpMath<T>::lg(r1)));
-    T alphan = TpMath<T>::ex(N.a0ln + N.a1ln * r2 * r2);
-    if (alphan > N.acap) alphan = N.acap;
-    T betan = TpMath<T>::ex(N.b0ln + N.b1ln * r2);
-    T qq[3] = {q1, q2, q3};
-    T pse[3];
-    T bs1h = T(1), damp = T(1);
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        T q = qq[i];
-        T lq = TpMath<T>::lg(q);
-        T oneh = an * TpMath<T>::ex(alphan * lq) + N.bn * TpMath<T>::ex(betan * lq);   // eq. B4
-        bs1h *= T(1) / (oneh * (T(1) + T(1) / (N.cn * q)));
-        T pl = tp_plin_q(N, E, q, lq);
-        T pq = N.pn * q;
-        pse[i] = (T(1) + N.fn * q * q) / (T(1) + N.gn * q + N.hn * q * q) * N.D1sq * pl +
-                 T(1) / ((N.mn * TpMath<T>::ex(N.mun * lq) + N.nn * TpMath<T>::ex(N.nun * lq)) *
-                         (T(1) + T(1) / (pq * pq * pq)));                                  // eq. B6
-        damp *= T(1) + N.en * q;
-    }
-    T bs3h = T(2) * ((tp_f2_tree(q1, q2, q3) + N.dn * q3) * pse[0] * pse[1] +
-                     (tp_f2_tree(q2, q3, q1) + N.dn * q1) * pse[1] * pse[2] +
-                     (tp_f2_tree(q3, q1, q2) + N.dn * q2) * pse[2] * pse[0]);             // eq. B5
-    return bs1h + bs3h / damp;
+    T q[3] = {q1, q2, q3};
+    T lq[3] = {TpMath<T>::lg(q1), TpMath<T>::lg(q2), TpMath<T>::lg(q3)};
+    T f2t[3] = {tp_f2_tree(q1, q2, q3), tp_f2_tree(q2, q3, q1), tp_f2_tree(q3, q1, q2)};
+    return tp_bispec_node<T>(N, E, q, lq, f2t, TpMath<T>::lg(c / a), (b + c - a) / a);
 }
 
 // Limber sum over the 96 z-nodes (FP64 accumulation), without the zero-ell guard
@@ -82,13 +122,20 @@ template <class T>
 __device__ __forceinline__ double tp_bkappa_sum(const TpNodeT<T> *__restrict__ nodes, const TpEHT<T> &E,
                                                 double l1, double l2, double l3)
 {
+    TpPointPre P;
+    tp_point_pre<double>(l1, l2, l3, &P);
+    const T L[3] = {(T)l1, (T)l2, (T)l3};
+    const T LL[3] = {(T)P.ll[0], (T)P.ll[1], (T)P.ll[2]};
+    const T F2T[3] = {(T)P.f2t[0], (T)P.f2t[1], (T)P.f2t[2]};
+    const T LR1 = (T)P.lr1, R2 = (T)P.r2;
     double acc = 0.0;
-    const T L1 = (T)l1, L2 = (T)l2, L3 = (T)l3;
 #pragma unroll 2
     for (int n = 0; n < TP_NNODES; ++n) {
         const TpNodeT<T> &N = nodes[n];
-        T ks = N.ks;
-        T b = tp_bispec_q<T>(N, E, L1 * ks, L2 * ks, L3 * ks);
+        const T ks = N.ks, lks = N.lnks;
+        T q[3] = {L[0] * ks, L[1] * ks, L[2] * ks};
+        T lq[3] = {LL[0] + lks, LL[1] + lks, LL[2] + lks};
+        T b = tp_bispec_node<T>(N, E, q, lq, F2T, LR1, R2);
         acc += (double)(N.wz * b);
     }
     return acc;
@@ -99,8 +146,9 @@ __device__ __forceinline__ double tp_bkappa_analytic(double l1, double l2, doubl
 {
     const double wts[9] = {0, 1.0e+8, 1.0e+6, 1.0e+4, 1.0e+2, 1.0e+0, 5.0e-3, 1.0e-5, 1.0e-6};
     const double avs[9] = {3.0e+3, 1.0e+4, 3.0e+4, 1.0e+5, 3.0e+5, 1.0e+6, 3.0e+6, 1.0e+7, 3.0e+7};
-    double phi = acos((l1 * l1 + l2 * l2 - l3 * l3) / (2 * l1 * l2));
-    double cphi = cos(phi);
+    // cos(acos(x)) with x clamped to [-1, 1]: rounding can push |x| past 1 at phi = 0, pi
+    double cphi = (l1 * l1 + l2 * l2 - l3 * l3) / (2 * l1 * l2);
+    cphi = fmin(1.0, fmax(-1.0, cphi));
     double s = 0;
     for (int i = 1; i < 9; ++i) {
         double a = avs[i];
@@ -113,10 +161,11 @@ __device__ __forceinline__ double tp_bkappa_analytic(double l1, double l2, doubl
     return s * (3. * (2 * tpd::kPi) * (2 * tpd::kPi));
 }
 
-__device__ __forceinline__ double tp_uhat(double eta)
+template <class T>
+__host__ __device__ __forceinline__ T tp_uhat(T eta)
 {
-    double t = 0.5 * eta * eta;  // classed_version/apertureStatistics.cpp:10-14
-    return t * exp(-t);
+    T t = T(0.5) * eta * eta;  // classed_version/apertureStatistics.cpp:10-14
+    return t * TpMath<T>::ex(-t);
 }
 
 // l3 exactly as the reference forms it (no contraction: l1 = l2, phi = pi gives 0)
@@ -146,7 +195,7 @@ __device__ __forceinline__ double tp_map3_integrand(const TpNodeT<T> *__restrict
     else
         bk = tp_bkappa_sum<T>(nodes, E, l1, l2, l3);
     if (!isfinite(bk)) atomicOr(flag, 2);
-    return l1 * l2 * bk * tp_uhat(l1 * th1) * tp_uhat(l2 * th2) * tp_uhat(l3 * th3);
+    return l1 * l2 * bk * tp_uhat<double>(l1 * th1) * tp_uhat<double>(l2 * th2) * tp_uhat<double>(l3 * th3);
 }
 
 // --------------------------------------------------- Genz-Malik 3-D rule

@@ -85,7 +85,7 @@ __device__ bool z_record(const TpCosmoConst &C, const double *T, int n_z, double
     N->lnkoff = log(2997.9 / rs);
     double hr = C.h / rs;
     N->pk_pref = C.h * C.h * C.h * 2.0 * tpd::kPi * tpd::kPi * C.norm * C.norm / (hr * hr * hr);
-    N->pad0 = 0;
+    N->lnks = log(N->ks);
     return true;
 }
 

@@ -50,7 +50,7 @@ struct TpNodeT {
     T dn;        // F2 correction 10^(-0.483 + 0.892 L - 0.086 Omega_m(z))
     T lnkoff;    // ln(2997.9 / r_sigma)
     T pk_pref;   // 2 pi^2 h^3 norm^2 / (h / r_sigma)^3 : P = pk_pref * delk' / q^3
-    T pad0;
+    T lnks;      // ln(ks)
 };
 typedef TpNodeT<double> TpNode64;
 typedef TpNodeT<float> TpNode32;
@@ -121,7 +121,7 @@ struct TpSurrogate;  // learned B_kappa model (tp_mlp.hip)
 struct tp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;  // evm: after the integrand kernel
     DevBuf d_in, d_out, d_regions, d_rout, d_fvals, d_desc, d_flag, d_misc;
     HostPinned h_in, h_out, h_regions, h_rout;
     tp_batch_stats stats{};

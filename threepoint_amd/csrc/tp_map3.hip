@@ -129,6 +129,7 @@ int tp_launch_regions(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegio
     else
         hipLaunchKernelGGL(k_region_points<double>, dim3(grid_for(np, B)), dim3(B), 0, ctx->stream, c->d_node64,
                            c->d_eh64, nreg, d_reg, d_desc, d_fvals, mode, d_flag);
+    TP_HIP(hipEventRecord(ctx->evm, ctx->stream));
     hipLaunchKernelGGL(k_region_rule, dim3(grid_for(nreg, B)), dim3(B), 0, ctx->stream, nreg, d_reg, d_fvals, d_out);
     TP_HIP(hipGetLastError());
     return 0;

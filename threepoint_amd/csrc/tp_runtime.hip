@@ -24,10 +24,10 @@ extern "C" const char *tp_last_error(void) { return g_err.c_str(); }
 int DevBuf::ensure(size_t bytes)
 {
     if (bytes <= cap) return 0;
+    size_t want = std::max(bytes, cap * 2);
     if (p) hipFree(p);
     p = nullptr;
     cap = 0;
-    size_t want = std::max(bytes, cap * 2);
     hipError_t e = hipMalloc(&p, want);
     if (e != hipSuccess) return tp_set_error(TP_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
     cap = want;
@@ -42,10 +42,10 @@ void DevBuf::release()
 int HostPinned::ensure(size_t bytes)
 {
     if (bytes <= cap) return 0;
+    size_t want = std::max(bytes, cap * 2);
     if (p) hipHostFree(p);
     p = nullptr;
     cap = 0;
-    size_t want = std::max(bytes, cap * 2);
     hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
     if (e != hipSuccess) return tp_set_error(TP_EHIP, std::string("hipHostMalloc: ") + hipGetErrorString(e));
     cap = want;
@@ -86,6 +86,7 @@ extern "C" int tp_ctx_create(int device, tp_ctx **out)
     }
     hipEventCreate(&c->ev0);
     hipEventCreate(&c->ev1);
+    hipEventCreate(&c->evm);
     if (c->d_flag.ensure(256)) {
         delete c;
         return TP_EHIP;
@@ -109,6 +110,7 @@ extern "C" int tp_ctx_destroy(tp_ctx *ctx)
     if (ctx->d_ogata) hipFree(ctx->d_ogata);
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
+    hipEventDestroy(ctx->evm);
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -131,6 +133,8 @@ static int check_cosmo(tp_ctx *ctx, const tp_cosmo *c, int icosmo)
 
 static int read_flag(tp_ctx *ctx, int *flag)
 {
+    int rc = ctx->h_out.ensure(64);
+    if (rc) return rc;
     TP_HIP(hipMemcpyAsync(ctx->h_out.p, ctx->d_flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     TP_HIP(hipStreamSynchronize(ctx->stream));
     std::memcpy(flag, ctx->h_out.p, sizeof(int));
@@ -371,9 +375,11 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
                               ctx->stream));
         int flag = 0;
         if ((rc = read_flag(ctx, &flag))) return rc;
-        float ms = 0;
+        float ms = 0, msp = 0;
         hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+        if (mode != TP_MODE_SURROGATE) hipEventElapsedTime(&msp, ctx->ev0, ctx->evm);
         ctx->stats.kernel_ms += ms;
+        ctx->stats.integrand_ms += msp;
         ctx->stats.launches += 1;
         ctx->stats.rounds += 1;
         ctx->stats.regions += (long long)nreg;

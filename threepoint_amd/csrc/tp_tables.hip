@@ -70,13 +70,13 @@ __device__ double sigmam_wave(const TpCosmoConst &C, double r, int j, double nor
     const double eps = 1.e-4;
     double k1 = 2. * kPi / r, k2 = 2. * kPi / r;
     double xx = 0, xxpp = -1.0;
-    for (int outer = 0; outer < 64; ++outer) {
+    for (int outer = 0; outer < 30; ++outer) {  // bounded; converges after a few range doublings
         k1 = k1 / 10.0;
         k2 = k2 * 2.0;
         double a = log(k1), b = log(k2);
         double xxp = -1.0;
         int n = 2;
-        for (int inner = 0; inner < 40; ++inner) {
+        for (int inner = 0; inner < 22; ++inner) {  // n <= 2^23
             n = n * 2;
             double hh = (b - a) / (double)n;
             double part = 0.;
@@ -109,7 +109,7 @@ __device__ double lgr(const TpCosmoConst &C, double z)
     double a = 1. / (1. + z), a0 = 1. / 1100.;
     double yp = -1., y0 = 0;
     int n = 10;
-    for (int it = 0; it < 30; ++it) {
+    for (int it = 0; it < 16; ++it) {  // n <= 655360 steps
         n *= 2;
         double h = (log(a) - log(a0)) / n;
         double x = log(a0);
@@ -465,7 +465,7 @@ __global__ void k_nodes(int ncosmo, int n_z, const TpCosmoConst *cc, const doubl
     // P(k) = h^3 2 pi^2 / kk^3 * norm^2 (kk 2997.9/h)^(3+ns) T^2 with kk = q h / r_sigma
     double hr = C.h / rs;
     N.pk_pref = C.h * C.h * C.h * 2.0 * kPi * kPi * C.norm * C.norm / (hr * hr * hr);
-    N.pad0 = 0;
+    N.lnks = log(N.ks);
     n64[t] = N;
     TpNode32 M;
     const double *src = &N.wz;
