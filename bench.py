@@ -45,7 +45,7 @@ def cpu_model():
     return platform.processor()
 
 
-def run_path(tabs, ctx, thetas, path, steps, warmup, reltol):
+def run_path(tabs, ctx, thetas, path, steps, warmup, reltol, icosmo=None):
     import threepoint_amd as tp
     mode = {"plain_fp64": tp.MODE_FP64, "batched_fp64": tp.MODE_FP64, "batched_fp32": tp.MODE_FP32,
             "surrogate": tp.MODE_SURROGATE}[path]
@@ -56,11 +56,11 @@ def run_path(tabs, ctx, thetas, path, steps, warmup, reltol):
             vals = np.zeros(n)
             pts = 0
             for i in range(n):
-                v, e, ne = tabs.map3_single(thetas[i], reltol, 0, mode)
+                v, e, ne = tabs.map3_single(thetas[i], reltol, 0 if icosmo is None else int(icosmo[i]), mode)
                 vals[i] = v
                 pts += ne
             return vals, {"points": pts}
-        v, e, ne = tabs.map3_batch(thetas, None, reltol, mode)
+        v, e, ne = tabs.map3_batch(thetas, icosmo, reltol, mode)
         return v, ctx.stats()
 
     for _ in range(warmup):
@@ -111,6 +111,7 @@ def main():
     ap.add_argument("--paths", default="plain_fp64,batched_fp64,batched_fp32,surrogate")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="c1", choices=["c1", "c3"])
     ap.add_argument("--surrogate", default=os.path.join(ROOT, "threepoint_amd", "models", "bkappa_mlp.bin"))
     args = ap.parse_args()
     if args.gpus != 1:
@@ -123,9 +124,21 @@ def main():
 
     ctx = tp.default_context()
     t0 = time.perf_counter()
-    tabs = tp.CosmologyTables([W.fiducial()], W.fiducial_nz(), 100, 3.0, ctx)
+    icosmo = None
+    if args.workload == "c1":
+        tabs = tp.CosmologyTables([W.fiducial()], W.fiducial_nz(), 100, 3.0, ctx)
+        thetas = W.to_rad(W.triples(W.map3_c1_radii()))
+        wl = ("Map3 of 220 triples (10 log radii 0.5-32 arcmin), fiducial flat LCDM, 100 z-bins, "
+              "hcubature rel tol %g" % args.reltol)
+    else:
+        cos = W.lhs_c3(512)
+        tabs = tp.CosmologyTables(cos, W.fiducial_nz(), 100, 3.0, ctx)
+        tri = W.to_rad(W.triples(W.map3_c3_radii()))
+        thetas = np.tile(tri, (512, 1))
+        icosmo = np.repeat(np.arange(512, dtype=np.int32), tri.shape[0])
+        wl = ("Map3 of 512 Latin-hypercube wCDM cosmologies x 35 triples (radii 2-32 arcmin) = 17920 integrals, "
+              "per-cosmology FP64 GPU tables, 100 z-bins, hcubature rel tol %g" % args.reltol)
     tables_ms = 1e3 * (time.perf_counter() - t0)
-    thetas = W.to_rad(W.triples(W.map3_c1_radii()))
 
     paths = [p for p in args.paths.split(",") if p]
     have_sur = False
@@ -140,12 +153,15 @@ def main():
     for p in paths:
         print("bench: path %s ..." % p, file=sys.stderr, flush=True)
         steps = 1 if p == "plain_fp64" else args.steps
-        v, agg = run_path(tabs, ctx, thetas, p, steps, 0 if p == "plain_fp64" else args.warmup, args.reltol)
+        th_p, ic_p = thetas, icosmo
+        if p == "plain_fp64" and args.workload == "c3":  # one cosmology's 35 triples
+            th_p, ic_p = thetas[:35], icosmo[:35]
+        v, agg = run_path(tabs, ctx, th_p, p, steps, 0 if p == "plain_fp64" else args.warmup, args.reltol, ic_p)
         values[p] = v
         results[p] = agg
     ref = values.get("batched_fp64")
     for p in results:
-        if ref is not None:
+        if ref is not None and values[p].shape == ref.shape:
             # triples whose first cubature region integrates to exactly 0 end at 0 in every path
             # (the reference algorithm's behaviour); they must stay exactly 0
             nz = ref != 0
@@ -153,6 +169,14 @@ def main():
             zero_ok = bool(np.all(values[p][~nz] == 0))
             results[p]["max_rel_err_vs_fp64"] = float(err.max()) if zero_ok else float("inf")
             results[p]["zero_triples"] = int((~nz).sum())
+            if p != "batched_fp64":
+                idx = np.nonzero(nz)[0]
+                results[p]["rel_err_p50_p99_p999"] = [float(np.percentile(err, q)) for q in (50, 99, 99.9)]
+                results[p]["triples_over_1e-3"] = int((err > 1e-3).sum())
+                w = int(idx[np.argmax(err)])
+                results[p]["worst_triple"] = {"index": w, "theta_rad": [float(t) for t in thetas[w]],
+                                              "icosmo": int(icosmo[w]) if icosmo is not None else 0,
+                                              "fp64": float(ref[w]), "this_path": float(values[p][w])}
         r = results[p]
         if r.get("integrand_ms", 0) > 0:
             ach = fe_point * r["points"] / (r["integrand_ms"] * 1e-3)
@@ -175,15 +199,14 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if head == "batched_fp32" else ("bf16+f64" if head == "surrogate" else "f64"),
         "data": "synthetic",
-        "config": {"workload": "Map3 of 220 triples (10 log radii 0.5-32 arcmin), fiducial flat LCDM, "
-                               "100 z-bins, hcubature rel tol %g" % args.reltol,
+        "config": {"workload": wl,
                    "path": head, "triples_per_step": int(thetas.shape[0]), "tables_ms": tables_ms},
         "paths": results,
         "speedup_vs_plain_fp64": (hr["triples_per_s"] / results["plain_fp64"]["triples_per_s"]
                                   if "plain_fp64" in results else None),
         "roofline": roof,
     }
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload == "c1":
         print("bench: cpu baseline ...", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(thetas, args.cpu_seconds, args.reltol)
     print(json.dumps(line))

@@ -74,6 +74,16 @@ int tp_cosmo_tables(const tp_cosmo *c, int icosmo, double *f_K, double *g, doubl
 int tp_bkappa_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *ell123,
                    double *out, int mode);
 
+/* Replaces BispectrumCalculator::bispec (classed_version/bispectrum.cpp:166-236) with the
+ * table interpolation of :506-513: in is npts*4 (k1, k2, k3 [h/Mpc], z); out is B [(Mpc/h)^6]. */
+int tp_bispec_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *k123z, double *out);
+
+/* Replaces BispectrumCalculator::integrand_bkappa (classed_version/bispectrum.cpp:471-541):
+ * in is npts*4 (z, ell1, ell2, ell3); out = g^3 (1+z)^3 B(ell/f_K, z) / (f_K E(z)), 0 at z = 0
+ * or ell3 = 0. */
+int tp_integrand_bkappa_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *zell,
+                             double *out);
+
 /* Replaces the integrand_v adapter ApertureStatistics::integrand
  * (classed_version/apertureStatistics.cpp:254-279): x is npts*3 (l1, l2, phi),
  * fval[i] = l1 l2 B_kappa(l1,l2,l3) uhat(l1 th1) uhat(l2 th2) uhat(l3 th3). */

@@ -1,0 +1,142 @@
+/*
+ * threepoint_amd.hpp -- C++ drop-in class API over the C ABI (threepoint_amd.h).
+ *
+ * Same class names, constructors and member signatures as the reference's
+ * classed_version headers, so code written against them recompiles unchanged:
+ *   cosmology               classed_version/cosmology.hpp:11-29
+ *   read_n_of_z, read_thetas, convert_angle_to_rad, convert_rad_to_angle
+ *                           classed_version/helper.hpp (helper.cpp:31-200)
+ *   configGamma, read_gamma_config
+ *                           cuda_version/helpers.cuh (helpers.cu:242-330)
+ *   BispectrumCalculator    classed_version/bispectrum.hpp:113-509
+ *   ApertureStatistics      classed_version/apertureStatistics.hpp:18-322
+ *   GammaCalculator         classed_version/gamma.hpp:17-495
+ * All numerics run on the GPU through the C ABI; objects hold non-owning
+ * pointers exactly like the reference (the caller keeps cosmology and
+ * BispectrumCalculator alive).  Instances are not thread-safe.
+ */
+#ifndef THREEPOINT_AMD_HPP
+#define THREEPOINT_AMD_HPP
+
+#include <array>
+#include <complex>
+#include <iosfwd>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "threepoint_amd.h"
+
+class cosmology {
+  public:
+    double h = 0.7, sigma8 = 0.82, omb = 0.047, omc = 0.253, ns = 0.97, w = -1, om = 0.3, ow = 0.7, zmax = 3.0;
+    cosmology() {}
+    explicit cosmology(const std::string &fn_parameters);  // throws std::runtime_error on bad files
+};
+std::ostream &operator<<(std::ostream &out, const cosmology &cosmo);
+
+void read_n_of_z(const std::string &fn, const int &n_bins, const double &zMax, std::vector<double> &nz);
+void read_thetas(const std::string &fn, std::vector<double> &thetas);
+double convert_angle_to_rad(const double &value, const std::string &unit = "arcmin");
+double convert_rad_to_angle(const double &value, const std::string &unit = "arcmin");
+
+struct configGamma {
+    int rsteps = 10, usteps = 10, vsteps = 10;
+    double umin = 0, umax = 1, vmin = 0, vmax = 1, rmin = 0.1, rmax = 100;
+};
+void read_gamma_config(const std::string &fn, configGamma &config);
+std::ostream &operator<<(std::ostream &out, const configGamma &config);
+
+struct ell_params {
+    double ell1, ell2, ell3;
+};
+
+/* Shared GPU context of the façade objects (one per process and device). */
+tp_ctx *threepoint_default_context(int device = 0);
+
+class BispectrumCalculator {
+  public:
+    cosmology *cosmo = nullptr;
+    double norm = 0;
+    double z_max = 0;
+
+    BispectrumCalculator() {}
+    BispectrumCalculator(cosmology *cosmo, const std::vector<double> &n_of_z, int n_z, double z_max);
+    ~BispectrumCalculator();
+    BispectrumCalculator(const BispectrumCalculator &) = delete;
+    BispectrumCalculator &operator=(const BispectrumCalculator &) = delete;
+
+    void set_cosmology(cosmology *cosmo);  // rebuilds the GPU tables
+    double bispec(double k1, double k2, double k3, double z);
+    double bkappa(double ell1, double ell2, double ell3);
+    double integrand_bkappa(double z, ell_params p);
+
+    /* batched forms (one GPU launch) */
+    void bkappa(const std::vector<double> &ell123, std::vector<double> &out, int mode = TP_MODE_FP64);
+
+    tp_ctx *ctx() const { return ctx_; }
+    const tp_cosmo *tables() const { return tables_; }
+    int n_redshift_bins() const { return n_z_; }
+
+  private:
+    tp_ctx *ctx_ = nullptr;
+    tp_cosmo *tables_ = nullptr;
+    std::vector<double> n_of_z_;
+    int n_z_ = 0;
+};
+
+class ApertureStatistics;
+struct ApertureStatisticsContainer {
+    ApertureStatistics *aperturestatistics;
+    std::vector<double> thetas;
+};
+
+class ApertureStatistics {
+  public:
+    BispectrumCalculator *Bispectrum_;
+    double l1_ = 0, l2_ = 0;
+    double phiMin = 0;
+    double phiMax = 6.283185307179586;
+    double lMin = 1;
+    double lMax = 1e4;
+    double reltol = 1e-4;     /* hcubature relative tolerance (the GPU CLI's 1e-4) */
+    int mode = TP_MODE_FP64;  /* integrand precision / method */
+
+    explicit ApertureStatistics(BispectrumCalculator *Bispectrum);
+    double uHat(const double &eta);
+    double integrand(const double &l1, const double &l2, const double &phi, std::vector<double> thetas);
+    static int integrand(unsigned ndim, size_t npts, const double *vars, void *thisPtr, unsigned fdim,
+                         double *value);
+    /* one triple, the reference's way: hcubature_v over (l1, l2, phi), GPU integrand per round */
+    double MapMapMap(const std::vector<double> &thetas);
+    /* many triples in one batched run (same decision sequence per triple) */
+    std::vector<double> MapMapMap(const std::vector<std::array<double, 3>> &thetas);
+};
+
+class GammaCalculator {
+  public:
+    GammaCalculator(BispectrumCalculator *Bispectrum, double prec_h, double prec_k, std::string triangle_center);
+    std::complex<double> gamma0(double x1, double x2, double x3);
+    std::complex<double> gamma1(double x1, double x2, double x3);
+    std::complex<double> gamma2(double x1, double x2, double x3);
+    std::complex<double> gamma3(double x1, double x2, double x3);
+    /* all four natural components of many triangles: out[t] = {Gamma0..Gamma3} */
+    std::vector<std::array<std::complex<double>, 4>> gammas(const std::vector<std::array<double, 3>> &x123);
+    double reltol = 1e-3;  /* hcubature relative tolerance (classed_version/gamma.cpp:375) */
+
+  private:
+    BispectrumCalculator *Bispectrum_;
+    std::array<std::complex<double>, 4> all(double x1, double x2, double x3);
+};
+
+/* same callback type and entry point as classed_version/cubature.h:55-57, 90-94 */
+typedef tp_integrand_v integrand_v;
+enum error_norm { ERROR_INDIVIDUAL = 0, ERROR_PAIRED, ERROR_L2, ERROR_L1, ERROR_LINF };
+inline int hcubature_v(unsigned fdim, integrand_v f, void *fdata, unsigned dim, const double *xmin,
+                       const double *xmax, size_t maxEval, double reqAbsError, double reqRelError, error_norm norm,
+                       double *val, double *err)
+{
+    return tp_hcubature_v(fdim, f, fdata, dim, xmin, xmax, maxEval, reqAbsError, reqRelError, (int)norm, val, err);
+}
+
+#endif

@@ -1,0 +1,88 @@
+// calculateGamma.x -- drop-in for cuda_version/calculateGamma.cu:21-152.
+//
+//   calculateGamma.x <cosmology file> <gamma config file> <output file> <n(z) file> [device]
+//                    [--nz-bins N] [--prec-h H] [--prec-k K] [--reltol X]
+//
+// Gamma_0..Gamma_3 (centroid convention) on the (r, u, v) grid of the config file, r2 = r pi/180/60,
+// r3 = r2 u, r1 = v r3 + r2 (the TreeCorr binning of cuda_version/calculateGamma.cu:107-121), one
+// output line per triangle: Re/Im Gamma0 ... Re/Im Gamma3, r, u, v.  All triangles of one r bin are
+// evaluated in one batched GPU pass and written before the next r bin starts.
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <fstream>
+#include <iostream>
+#include <vector>
+
+#include "threepoint_amd.hpp"
+
+int main(int argc, char **argv)
+{
+    const char *usage = "calculateGamma.x <cosmology> <gamma config> <output> <n(z)> [device] [--nz-bins N]\n"
+                        "                 [--prec-h H] [--prec-k K] [--reltol X]\n";
+    if (argc < 5) {
+        std::cerr << usage;
+        return 1;
+    }
+    int nbins = 100, device = 0;
+    double prec_h = 1e-3, prec_k = 5.0, reltol = 1e-3;
+    for (int i = 5; i < argc; ++i) {
+        std::string a = argv[i];
+        if (a == "--nz-bins" && i + 1 < argc) nbins = std::atoi(argv[++i]);
+        else if (a == "--prec-h" && i + 1 < argc) prec_h = std::atof(argv[++i]);
+        else if (a == "--prec-k" && i + 1 < argc) prec_k = std::atof(argv[++i]);
+        else if (a == "--reltol" && i + 1 < argc) reltol = std::atof(argv[++i]);
+        else if (i == 5 && a.size() && a[0] != '-') device = std::atoi(argv[i]);
+        else {
+            std::cerr << usage;
+            return 1;
+        }
+    }
+    try {
+        if (device != 0) threepoint_default_context(device);
+        cosmology cosmo(argv[1]);
+        std::vector<double> nz;
+        read_n_of_z(argv[4], nbins, cosmo.zmax, nz);
+        std::ofstream out(argv[3]);
+        if (!out.is_open()) {
+            std::cerr << "Couldn't open " << argv[3] << std::endl;
+            return 1;
+        }
+        configGamma config;
+        read_gamma_config(argv[2], config);
+        std::cerr << "Using cosmology:" << std::endl << cosmo << "Writing to:" << argv[3] << std::endl << config;
+        BispectrumCalculator bk(&cosmo, nz, nbins, cosmo.zmax);
+        GammaCalculator gc(&bk, prec_h, prec_k, "centroid");
+        gc.reltol = reltol;
+        double lrmin = std::log(config.rmin), lrmax = std::log(config.rmax);
+        auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < config.rsteps; i++) {
+            double r = std::exp(lrmin + (lrmax - lrmin) / config.rsteps * (i + 0.5));
+            std::vector<std::array<double, 3>> x;
+            std::vector<std::array<double, 2>> uv;
+            for (int j = 0; j < config.usteps; j++) {
+                double u = config.umin + (config.umax - config.umin) / config.usteps * (j + 0.5);
+                for (int k = 0; k < config.vsteps; k++) {
+                    double v = config.vmin + (config.vmax - config.vmin) / config.vsteps * (k + 0.5);
+                    double r2 = r * M_PI / 180. / 60.;
+                    double r3 = r2 * u;
+                    double r1 = v * r3 + r2;
+                    x.push_back({r1, r2, r3});
+                    uv.push_back({u, v});
+                }
+            }
+            auto g = gc.gammas(x);
+            for (size_t t = 0; t < x.size(); ++t) {
+                out << g[t][0].real() << " " << g[t][0].imag() << " " << g[t][1].real() << " " << g[t][1].imag() << " "
+                    << g[t][2].real() << " " << g[t][2].imag() << " " << g[t][3].real() << " " << g[t][3].imag() << " "
+                    << r << " " << uv[t][0] << " " << uv[t][1] << std::endl;
+            }
+            double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            std::cerr << "r bin " << i + 1 << "/" << config.rsteps << " done, " << el << " s" << std::endl;
+        }
+    } catch (const std::exception &e) {
+        std::cerr << e.what() << std::endl;
+        return 1;
+    }
+    return 0;
+}

@@ -1,0 +1,154 @@
+// tp_classes.cpp -- the drop-in class API (threepoint_amd.hpp) over the C ABI.
+#include <cmath>
+#include <iostream>
+#include <mutex>
+#include <stdexcept>
+
+#include "threepoint_amd.hpp"
+
+static void check(int rc, const char *what)
+{
+    if (rc) throw std::runtime_error(std::string(what) + ": " + tp_last_error());
+}
+
+tp_ctx *threepoint_default_context(int device)
+{
+    static std::mutex mu;
+    static tp_ctx *ctx[64] = {nullptr};
+    std::lock_guard<std::mutex> lk(mu);
+    if (device < 0 || device >= 64) throw std::runtime_error("threepoint_default_context: bad device");
+    if (!ctx[device]) check(tp_ctx_create(device, &ctx[device]), "tp_ctx_create");
+    return ctx[device];
+}
+
+BispectrumCalculator::BispectrumCalculator(cosmology *c, const std::vector<double> &n_of_z, int n_z, double zmax)
+    : cosmo(c), z_max(zmax), n_of_z_(n_of_z), n_z_(n_z)
+{
+    // classed_version/bispectrum.cpp:73-79 -> initialize :91-109 -> set_cosmology :111-164
+    if ((int)n_of_z.size() < n_z) throw std::runtime_error("BispectrumCalculator: n_of_z shorter than n_z");
+    ctx_ = threepoint_default_context(0);
+    set_cosmology(c);
+}
+
+BispectrumCalculator::~BispectrumCalculator()
+{
+    if (tables_) tp_cosmo_destroy(tables_);
+}
+
+void BispectrumCalculator::set_cosmology(cosmology *c)
+{
+    cosmo = c;
+    if (tables_) tp_cosmo_destroy(tables_);
+    tables_ = nullptr;
+    tp_cosmology p{c->h, c->sigma8, c->omb, c->ns, c->w, c->om, z_max};
+    check(tp_cosmo_build(ctx_, 1, &p, n_of_z_.data(), n_z_, 0, z_max, &tables_), "tp_cosmo_build");
+    check(tp_cosmo_tables(tables_, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &norm), "tp_cosmo_tables");
+}
+
+double BispectrumCalculator::bispec(double k1, double k2, double k3, double z)
+{
+    double in[4] = {k1, k2, k3, z}, out = 0;
+    check(tp_bispec_eval(ctx_, tables_, 0, 1, in, &out), "tp_bispec_eval");
+    return out;
+}
+
+double BispectrumCalculator::bkappa(double ell1, double ell2, double ell3)
+{
+    double in[3] = {ell1, ell2, ell3}, out = 0;
+    check(tp_bkappa_eval(ctx_, tables_, 0, 1, in, &out, TP_MODE_FP64), "tp_bkappa_eval");
+    return out;
+}
+
+void BispectrumCalculator::bkappa(const std::vector<double> &ell123, std::vector<double> &out, int mode)
+{
+    out.assign(ell123.size() / 3, 0.0);
+    check(tp_bkappa_eval(ctx_, tables_, 0, out.size(), ell123.data(), out.data(), mode), "tp_bkappa_eval");
+}
+
+double BispectrumCalculator::integrand_bkappa(double z, ell_params p)
+{
+    double in[4] = {z, p.ell1, p.ell2, p.ell3}, out = 0;
+    check(tp_integrand_bkappa_eval(ctx_, tables_, 0, 1, in, &out), "tp_integrand_bkappa_eval");
+    return out;
+}
+
+ApertureStatistics::ApertureStatistics(BispectrumCalculator *b) : Bispectrum_(b) {}
+
+double ApertureStatistics::uHat(const double &eta)
+{
+    double t = 0.5 * eta * eta;  // classed_version/apertureStatistics.cpp:10-14
+    return t * std::exp(-t);
+}
+
+double ApertureStatistics::integrand(const double &l1, const double &l2, const double &phi, std::vector<double> th)
+{
+    double x[3] = {l1, l2, phi}, f = 0;
+    check(tp_map3_eval(Bispectrum_->ctx(), Bispectrum_->tables(), 0, th.data(), 1, x, &f, mode), "tp_map3_eval");
+    return f;
+}
+
+int ApertureStatistics::integrand(unsigned ndim, size_t npts, const double *vars, void *thisPtr, unsigned fdim,
+                                  double *value)
+{
+    // the integrand_v adapter, classed_version/apertureStatistics.cpp:254-279: one GPU batch
+    if (fdim != 1 || ndim != 3) return 1;
+    ApertureStatisticsContainer *c = (ApertureStatisticsContainer *)thisPtr;
+    ApertureStatistics *a = c->aperturestatistics;
+    return tp_map3_eval(a->Bispectrum_->ctx(), a->Bispectrum_->tables(), 0, c->thetas.data(), npts, vars, value,
+                        a->mode) ? 1 : 0;
+}
+
+double ApertureStatistics::MapMapMap(const std::vector<double> &thetas)
+{
+    // classed_version/apertureStatistics.cpp:321-353 (3-D branch)
+    if (thetas.size() != 3) throw std::runtime_error("MapMapMap: need three aperture radii");
+    double val = 0, err = 0;
+    long long n = 0;
+    check(tp_map3_single(Bispectrum_->ctx(), Bispectrum_->tables(), 0, thetas.data(), reltol, mode, &val, &err, &n),
+          "tp_map3_single");
+    lMax = 10. / std::min(thetas[0], std::min(thetas[1], thetas[2]));
+    return val;
+}
+
+std::vector<double> ApertureStatistics::MapMapMap(const std::vector<std::array<double, 3>> &thetas)
+{
+    std::vector<double> th, val(thetas.size()), err(thetas.size());
+    std::vector<int> ic(thetas.size(), 0);
+    for (auto &t : thetas) th.insert(th.end(), t.begin(), t.end());
+    check(tp_map3_batch(Bispectrum_->ctx(), Bispectrum_->tables(), thetas.size(), ic.data(), th.data(), reltol, mode,
+                        val.data(), err.data(), nullptr),
+          "tp_map3_batch");
+    return val;
+}
+
+GammaCalculator::GammaCalculator(BispectrumCalculator *b, double prec_h, double prec_k, std::string center)
+    : Bispectrum_(b)
+{
+    // classed_version/gamma.cpp:6-23
+    if (center != "centroid")
+        throw std::runtime_error("GammaCalculator: only the centroid triangle center is provided");
+    check(tp_gamma_setup(b->ctx(), prec_h, prec_k), "tp_gamma_setup");
+}
+
+std::array<std::complex<double>, 4> GammaCalculator::all(double x1, double x2, double x3)
+{
+    return gammas({{{x1, x2, x3}}})[0];
+}
+
+std::vector<std::array<std::complex<double>, 4>> GammaCalculator::gammas(const std::vector<std::array<double, 3>> &x)
+{
+    std::vector<double> in, re(4 * x.size()), im(4 * x.size());
+    for (auto &t : x) in.insert(in.end(), t.begin(), t.end());
+    check(tp_gamma_batch(Bispectrum_->ctx(), Bispectrum_->tables(), 0, x.size(), in.data(), reltol, re.data(),
+                         im.data(), nullptr),
+          "tp_gamma_batch");
+    std::vector<std::array<std::complex<double>, 4>> out(x.size());
+    for (size_t t = 0; t < x.size(); ++t)
+        for (int i = 0; i < 4; ++i) out[t][i] = std::complex<double>(re[4 * t + i], im[4 * t + i]);
+    return out;
+}
+
+std::complex<double> GammaCalculator::gamma0(double x1, double x2, double x3) { return all(x1, x2, x3)[0]; }
+std::complex<double> GammaCalculator::gamma1(double x1, double x2, double x3) { return all(x1, x2, x3)[1]; }
+std::complex<double> GammaCalculator::gamma2(double x1, double x2, double x3) { return all(x1, x2, x3)[2]; }
+std::complex<double> GammaCalculator::gamma3(double x1, double x2, double x3) { return all(x1, x2, x3)[3]; }

@@ -211,6 +211,37 @@ __global__ void __launch_bounds__(kGammaBlock) k_gamma_points(const TpCosmoConst
     f[2 * p + 1] = im;
 }
 
+// single-z bispectrum / Limber integrand (one thread per point)
+__global__ void __launch_bounds__(128) k_zpoint(const TpCosmoConst *__restrict__ cc, const double *__restrict__ tab,
+                                                const TpEH64 *__restrict__ eh, int n_z, int icosmo, int what,
+                                                size_t npts, const double *__restrict__ in, double *__restrict__ out,
+                                                int *flag)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const double *p = in + 4 * i;
+    const TpCosmoConst &C = cc[icosmo];
+    double v = 0.0;
+    TpNode64 N;
+    if (what == 0) {  // bispec(k1, k2, k3, z): B at wavenumbers k [h/Mpc]
+        double z = p[3];
+        if (z_record(C, tab + (size_t)icosmo * 6 * n_z, n_z, z, &N)) {
+            double rs = 1.0 / N.inv_rs;
+            v = tp_bispec_q<double>(N, eh[icosmo], p[0] * rs, p[1] * rs, p[2] * rs);
+        } else {  // z == 0: tables at z = 0 (the reference has no z guard in bispec)
+            z_record(C, tab + (size_t)icosmo * 6 * n_z, n_z, 1e-300, &N);
+            double rs = 1.0 / N.inv_rs;
+            v = tp_bispec_q<double>(N, eh[icosmo], p[0] * rs, p[1] * rs, p[2] * rs);
+        }
+    } else {          // integrand_bkappa(z, ell1, ell2, ell3)
+        double z = p[0];
+        if (p[3] != 0.0 && z_record(C, tab + (size_t)icosmo * 6 * n_z, n_z, z, &N))
+            v = N.wz * tp_bispec_q<double>(N, eh[icosmo], p[1] * N.ks, p[2] * N.ks, p[3] * N.ks);
+    }
+    if (!isfinite(v)) atomicOr(flag, 8);
+    out[i] = v;
+}
+
 // ---------------------------------------------------------- host Bessel
 // J_n(x): power series (x < 2), Miller backward recurrence normalised by the
 // Neumann sum J0 + 2 sum J_2k = 1 (x < 30), Hankel's asymptotic expansion (x >= 30);
@@ -285,6 +316,41 @@ struct GammaIntegral {
 };
 
 }  // namespace
+
+static int zpoint_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, int what, size_t npts, const double *in,
+                       double *out)
+{
+    if (!ctx || !c || (npts && (!in || !out))) return tp_set_error(TP_EINVAL, "NULL argument");
+    if (icosmo < 0 || icosmo >= c->ncosmo) return tp_set_error(TP_EINVAL, "cosmology index out of range");
+    if (npts == 0) return 0;
+    TP_HIP(hipSetDevice(ctx->device));
+    int rc;
+    size_t bi = npts * 4 * sizeof(double), bo = npts * sizeof(double);
+    if ((rc = ctx->d_in.ensure(bi)) || (rc = ctx->d_out.ensure(bo))) return rc;
+    TP_HIP(hipMemsetAsync(ctx->d_flag.p, 0, sizeof(int), ctx->stream));
+    TP_HIP(hipMemcpyAsync(ctx->d_in.p, in, bi, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_zpoint, dim3((unsigned)((npts + 127) / 128)), dim3(128), 0, ctx->stream, c->d_const,
+                       c->d_tab, c->d_eh64, c->n_z, icosmo, what, npts, (const double *)ctx->d_in.p,
+                       (double *)ctx->d_out.p, (int *)ctx->d_flag.p);
+    TP_HIP(hipGetLastError());
+    TP_HIP(hipMemcpyAsync(out, ctx->d_out.p, bo, hipMemcpyDeviceToHost, ctx->stream));
+    int flag = 0;
+    TP_HIP(hipMemcpyAsync(&flag, ctx->d_flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    if (flag) return tp_set_error(TP_ENONFINITE, "non-finite bispectrum value");
+    return 0;
+}
+
+extern "C" int tp_bispec_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *k123z, double *out)
+{
+    return zpoint_eval(ctx, c, icosmo, 0, npts, k123z, out);
+}
+
+extern "C" int tp_integrand_bkappa_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *zell,
+                                        double *out)
+{
+    return zpoint_eval(ctx, c, icosmo, 1, npts, zell, out);
+}
 
 extern "C" int tp_gamma_setup(tp_ctx *ctx, double prec_h, double prec_k_arg)
 {

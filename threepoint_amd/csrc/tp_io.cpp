@@ -1,0 +1,142 @@
+// tp_io.cpp -- input readers of the drop-in API (file formats of the reference).
+#include <cmath>
+#include <fstream>
+#include <iostream>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+
+#include "threepoint_amd.hpp"
+
+cosmology::cosmology(const std::string &fn_parameters)
+{
+    // classed_version/cosmology.cpp:10-73: "name value" lines, '#' comments
+    std::ifstream input(fn_parameters.c_str());
+    if (input.fail()) throw std::runtime_error("cosmology: Could not open " + fn_parameters);
+    std::string line;
+    while (std::getline(input, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream iss(line);
+        std::string name;
+        double value = 0;
+        iss >> name >> value;
+        if (name == "h") h = value;
+        else if (name == "sigma_8") sigma8 = value;
+        else if (name == "Omega_b") omb = value;
+        else if (name == "n_s") ns = value;
+        else if (name == "w") w = value;
+        else if (name == "Omega_m") om = value;
+        else if (name == "z_max") zmax = value;
+        else throw std::runtime_error("cosmology: Parameter file in wrong format");
+    }
+    ow = 1 - om;
+    omc = om - omb;
+}
+
+std::ostream &operator<<(std::ostream &out, const cosmology &c)
+{
+    out << "h:" << c.h << std::endl << "sigma_8:" << c.sigma8 << std::endl << "Omega_b:" << c.omb << std::endl
+        << "n_s:" << c.ns << std::endl << "w:" << c.w << std::endl << "Omega_m:" << c.om << std::endl
+        << "z_max:" << c.zmax << std::endl;
+    return out;
+}
+
+void read_n_of_z(const std::string &fn, const int &n_bins, const double &zMax, std::vector<double> &nz)
+{
+    // classed_version/helper.cpp:31-96: linear resampling to z_i = i zMax / n_bins, sum n dz = 1
+    std::ifstream input(fn.c_str());
+    if (input.fail()) throw std::runtime_error("read_n_of_z: Could not open " + fn);
+    std::vector<double> zs, ns;
+    std::string line;
+    while (std::getline(input, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream iss(line);
+        double z, v;
+        iss >> z >> v;
+        zs.push_back(z);
+        ns.push_back(v);
+    }
+    if (zs.size() < 2) throw std::runtime_error("read_n_of_z: need at least two rows in " + fn);
+    int nf = (int)zs.size();
+    double zmin_f = zs.front(), zmax_f = zs.back();
+    double dz_f = (zmax_f - zmin_f) / (nf - 1);
+    double dz = zMax / n_bins;
+    for (int i = 0; i < n_bins; i++) {
+        double z = i * dz;
+        int ix = int((z - zmin_f) / dz_f);
+        double dix = (z - zmin_f) / dz_f - ix;
+        double v = 0;
+        if (ix >= 0 && ix < nf - 1) v = ns.at(ix + 1) * dix + ns.at(ix) * (1 - dix);
+        if (ix == nf - 1) v = ns.at(nf - 1);
+        nz.push_back(v);
+    }
+    double norm = std::accumulate(nz.end() - n_bins, nz.end(), 0.0);
+    if (norm == 0) throw std::runtime_error("sum of n(z) is zero! Check " + fn);
+    norm *= dz;
+    for (int i = 0; i < n_bins; i++) nz.at(nz.size() - n_bins + i) /= norm;
+}
+
+void read_thetas(const std::string &fn, std::vector<double> &thetas)
+{
+    // classed_version/helper.cpp:178-200: one theta per line, '#' comments
+    std::ifstream input(fn.c_str());
+    if (input.fail()) throw std::runtime_error("read_thetas: Couldn't open " + fn);
+    std::string line;
+    while (std::getline(input, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream iss(line);
+        double t;
+        if (iss >> t) thetas.push_back(t);
+    }
+}
+
+double convert_angle_to_rad(const double &value, const std::string &unit)
+{
+    // classed_version/helper.cpp:132-155 (note: 2.9088820866e-4, not pi/10800)
+    if (unit == "arcmin") return 2.9088820866e-4 * value;
+    if (unit == "deg") return 0.017453 * value;
+    if (unit == "rad") return value;
+    throw std::runtime_error("Unit not correctly specified. Needs to be arcmin, deg, or rad.");
+}
+
+double convert_rad_to_angle(const double &value, const std::string &unit)
+{
+    // classed_version/helper.cpp:157-176
+    if (unit == "arcmin") return 3437.74677 * value;
+    if (unit == "deg") return 57.3 * value;
+    throw std::runtime_error("Unit not correctly specified. Needs to be arcmin or deg.");
+}
+
+void read_gamma_config(const std::string &fn, configGamma &config)
+{
+    // cuda_version/helpers.cu:242-310: "name value" lines
+    std::ifstream input(fn.c_str());
+    if (input.fail()) throw std::runtime_error("read_gamma_config: Could not open " + fn);
+    std::string line;
+    while (std::getline(input, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream iss(line);
+        std::string name;
+        double v = 0;
+        iss >> name >> v;
+        if (name == "rmin") config.rmin = v;
+        else if (name == "rmax") config.rmax = v;
+        else if (name == "umin") config.umin = v;
+        else if (name == "umax") config.umax = v;
+        else if (name == "vmin") config.vmin = v;
+        else if (name == "vmax") config.vmax = v;
+        else if (name == "rsteps") config.rsteps = (int)std::round(v);
+        else if (name == "usteps") config.usteps = (int)std::round(v);
+        else if (name == "vsteps") config.vsteps = (int)std::round(v);
+        else throw std::runtime_error("read_gamma_config: unknown key " + name);
+    }
+}
+
+std::ostream &operator<<(std::ostream &out, const configGamma &c)
+{
+    out << "Binning r in " << c.rsteps << " bins from " << c.rmin << " to " << c.rmax << std::endl;
+    out << "Binning u in " << c.usteps << " bins from " << c.umin << " to " << c.umax << std::endl;
+    out << "Binning v in " << c.vsteps << " bins from " << c.vmin << " to " << c.vmax << std::endl;
+    out << "Setting the triangle center as Centroid" << std::endl;
+    return out;
+}
