@@ -118,6 +118,8 @@ typedef struct {
     long long exact_checks;      /* points re-evaluated exactly                    */
     long long fallback_batches;  /* rounds that fell back to exact B_kappa         */
     double max_check_relerr;     /* worst relative error seen by the exact checks  */
+    long long fallback_integrals;/* triples re-integrated exactly because the standard error
+                                    of their surrogate correction exceeded tol          */
 } tp_batch_stats;
 int tp_last_batch_stats(const tp_ctx *ctx, tp_batch_stats *out);
 
@@ -156,9 +158,10 @@ int tp_hcubature_v(unsigned fdim, tp_integrand_v f, void *fdata, unsigned dim,
 long long tp_hcubature_last_neval(void);
 
 /* -------------------------------------------------------------- surrogate */
-/* The learned part: an MLP 8 -> 4x256 (SiLU) -> 1 for log|B_kappa(l1,l2,l3; cosmology)|
- * over inputs (ln l1, ln l2, cos phi, Omega_m, sigma8, h, w, n_s).  No counterpart in
- * the reference; it serves TP_MODE_SURROGATE. */
+/* The learned part: an MLP 8 -> 4x256 (SiLU) -> 1 for log|B_kappa(l1,l2,l3; cosmology)|.
+ * Inputs: the triangle as its three sides' logs in sorted order (the information of
+ * (ln l1, ln l2, cos phi) with B_kappa's permutation symmetry factored out) and
+ * Omega_m, sigma8, h, w, n_s.  No counterpart in the reference; it serves TP_MODE_SURROGATE. */
 typedef struct {
     long long nsamples;   /* training samples generated by the exact GPU path     */
     int epochs;           /* passes over the samples                              */
@@ -184,8 +187,12 @@ int tp_surrogate_load(tp_ctx *ctx, const char *path);
  * are re-evaluated exactly; if the surrogate's error on them exceeds tol the batch
  * falls back to exact B_kappa. */
 int tp_surrogate_config(tp_ctx *ctx, double check_frac, double tol, uint64_t seed);
-/* y[i] = surrogate log|B_kappa| at inputs in[8*i..] (raw, unnormalised). */
+/* y[i] = the network's output at in[8*i..] = (l1, l2, l3, Omega_m, sigma8, h, w, n_s): the learned
+ * ln B_kappa - ln B_8, where B_8 is the 8-node FP32 Limber baseline of the same cosmology. */
 int tp_surrogate_predict(tp_ctx *ctx, size_t n, const double *in8, double *y);
+/* Surrogate B_kappa = B_8 exp(network) at ell123 (npts*3) for cosmology icosmo of c. */
+int tp_surrogate_bkappa(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *ell123,
+                        double *out);
 /* Raw MLP access for gradient checks: parameters as one FP32 vector (layout
  * W1,b1,...,W5,b5; Wk row-major [out][in]). */
 int tp_mlp_num_params(size_t *n);

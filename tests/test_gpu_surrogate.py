@@ -1,0 +1,87 @@
+"""The learned surrogate on the GPU: gradients vs the CPU oracle, training, deployment."""
+import ctypes
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import threepoint_amd as tp
+from threepoint_amd import workloads as W
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp_oracle as MO  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+def _gpu_loss_grad(ctx, p, X, T):
+    p32 = np.ascontiguousarray(p, dtype=np.float32)
+    tp._check(tp.lib().tp_mlp_set_params(ctx.h, p32.ctypes.data_as(_fp)), "set")
+    X32 = np.ascontiguousarray(X, dtype=np.float32)
+    T32 = np.ascontiguousarray(T, dtype=np.float32)
+    g = np.zeros(MO.NPARAM, dtype=np.float32)
+    loss = ctypes.c_double()
+    tp._check(tp.lib().tp_mlp_loss_grad(ctx.h, X.shape[0], X32.ctypes.data_as(_fp), T32.ctypes.data_as(_fp),
+                                        ctypes.byref(loss), g.ctypes.data_as(_fp)), "loss_grad")
+    return loss.value, g.astype(np.float64)
+
+
+@pytest.mark.parametrize("M", [64, 512])
+def test_mlp_gradients_match_cpu_oracle(gpu_ctx, M):
+    rng = np.random.default_rng(M)
+    d = {}
+    for name, shp in MO.SHAPES:
+        fan_in = shp[1] if len(shp) == 2 else (MO.H if name == "W5" else 1)
+        d[name] = rng.uniform(-1, 1, shp) * (np.sqrt(6.0 / fan_in) if name.startswith("W") else 0.1)
+    p = MO.pack(d).astype(np.float32).astype(np.float64)
+    X = rng.uniform(-1, 1, (M, MO.NIN)).astype(np.float32).astype(np.float64)
+    T = rng.normal(0, 1, M).astype(np.float32).astype(np.float64)
+    lg, gg = _gpu_loss_grad(gpu_ctx, p, X, T)
+    lo, go = MO.loss_grad(p, X, T, bf=True)
+    assert abs(lg / lo - 1) < 1e-4
+    G, O = MO.unpack(gg), MO.unpack(go)
+    for name, _ in MO.SHAPES:
+        err = np.linalg.norm(G[name] - O[name]) / max(np.linalg.norm(O[name]), 1e-30)
+        print("grad %s rel err %.3e" % (name, err))
+        assert err < 2e-3, name
+
+
+def test_training_learns(gpu_ctx):
+    from threepoint_amd.train_surrogate import train
+    r = train(samples=1 << 19, epochs=2, batch=4096, lr=2e-3, lr_final=1e-4, out=None, ctx=gpu_ctx)
+    print(r)
+    assert r["val_rmse_lnB"] < 0.2
+
+
+def test_fused_inference_matches_cpu_oracle(gpu_ctx, tmp_path):
+    """tp_surrogate_predict (fused hi/lo-BF16 MFMA kernel) against the numpy forward pass."""
+    rng = np.random.default_rng(3)
+    d = {}
+    for name, shp in MO.SHAPES:
+        fan_in = shp[1] if len(shp) == 2 else (MO.H if name == "W5" else 1)
+        d[name] = rng.uniform(-1, 1, shp) * (np.sqrt(6.0 / fan_in) if name.startswith("W") else 0.1)
+    p = MO.pack(d).astype(np.float32).astype(np.float64)
+    lo, hi = [0.2, 0.7, 0.6, -1.2, 0.92], [0.4, 0.95, 0.8, -0.8, 1.0]
+    fn = str(tmp_path / "m.bin")
+    MO.write_model(fn, (2.0, 11.0), lo, hi, -30.0, 4.0, p)
+    tp._check(tp.lib().tp_surrogate_load(gpu_ctx.h, fn.encode()), "load")
+    n = 1000
+    l1, l2 = np.exp(rng.uniform(2, 11, n)), np.exp(rng.uniform(2, 11, n))
+    l3 = np.sqrt(l1 ** 2 + l2 ** 2 + 2 * l1 * l2 * np.cos(rng.uniform(0, np.pi, n)))
+    cp = np.stack([rng.uniform(a, b, n) for a, b in zip(lo, hi)], 1)
+    raw = np.ascontiguousarray(np.concatenate([np.stack([l1, l2, l3], 1), cp], 1))
+    y = np.zeros(n)
+    tp._check(tp.lib().tp_surrogate_predict(gpu_ctx.h, n, tp._d(raw), tp._d(y)), "predict")
+    m = MO.read_model(fn)
+    ref = MO.forward_hilo(MO.unpack(m["p"]), MO.encode(m, l1, l2, l3, cp)) * m["sd"] + m["mean"]
+    err = np.abs(y - ref)
+    print("fused inference max abs err (ln B) %.3e" % err.max())
+    assert err.max() < 2e-3
+
+
+def test_corrupted_model_rejected(gpu_ctx, tmp_path):
+    fn = tmp_path / "bad.bin"
+    fn.write_bytes(b"TPMLP002" + b"\0" * 100)
+    assert tp.lib().tp_surrogate_load(gpu_ctx.h, str(fn).encode()) != 0

@@ -40,7 +40,8 @@ class tp_batch_stats(ctypes.Structure):
                 ("launches", ctypes.c_longlong), ("kernel_ms", ctypes.c_double), ("integrand_ms", ctypes.c_double),
                 ("wall_ms", ctypes.c_double),
                 ("surrogate_points", ctypes.c_longlong), ("exact_checks", ctypes.c_longlong),
-                ("fallback_batches", ctypes.c_longlong), ("max_check_relerr", ctypes.c_double)]
+                ("fallback_batches", ctypes.c_longlong), ("max_check_relerr", ctypes.c_double),
+                ("fallback_integrals", ctypes.c_longlong)]
 
 
 class tp_train_config(ctypes.Structure):
@@ -92,6 +93,8 @@ _SIGS = {
     "tp_surrogate_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "tp_surrogate_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]),
     "tp_surrogate_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _dp, _dp]),
+    "tp_surrogate_bkappa": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
+                                           _dp]),
     "tp_mlp_num_params": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t)]),
     "tp_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, _fp]),
     "tp_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, _fp]),
@@ -242,6 +245,13 @@ class CosmologyTables:
         out = np.zeros(ell.shape[0])
         _check(lib().tp_bkappa_eval(self.ctx.h, self.h, icosmo, ell.shape[0], _d(ell), _d(out), mode),
                "tp_bkappa_eval")
+        return out
+
+    def surrogate_bkappa(self, ell123, icosmo=0):
+        ell = np.ascontiguousarray(ell123, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(ell.shape[0])
+        _check(lib().tp_surrogate_bkappa(self.ctx.h, self.h, icosmo, ell.shape[0], _d(ell), _d(out)),
+               "tp_surrogate_bkappa")
         return out
 
     def map3_integrand(self, x, thetas, icosmo=0, mode=MODE_FP64):

@@ -248,6 +248,8 @@ void rule_apply(unsigned dim, unsigned fdim, Region &R, const double *vals)
     for (unsigned k = 0; k < fdim; ++k)
         if (R.err[k] > em) em = R.err[k];
     R.errmax = em;
+    R.corr = 0.0;
+    R.cvar = 0.0;
 }
 
 bool AdaptiveIntegral::init(unsigned dim, unsigned fdim, const double *xmin, const double *xmax, size_t maxEval,
@@ -274,11 +276,12 @@ bool AdaptiveIntegral::init(unsigned dim, unsigned fdim, const double *xmin, con
     }
     R.vol = compute_vol(dim, R.hw);
     R.errmax = HUGE_VAL;
+    R.corr = R.cvar = 0.0;
     pool_.push_back(R);
     pending_.push_back(0);
     numEval_ = npts_;
     rounds_ = 0;
-    done_ = false;
+    done_ = converged_ = false;
     return true;
 }
 
@@ -337,8 +340,12 @@ bool AdaptiveIntegral::advance()
 {
     // one iteration of the main loop, classed_version/hcubature.cpp:938-989
     if (done_) return true;
-    if (!(numEval_ < maxEval_ || !maxEval_) || converged(fdim_, sval_, serr_, reqAbs_, reqRel_, norm_)) {
+    if (!(numEval_ < maxEval_ || !maxEval_)) {
         done_ = true;
+        return true;
+    }
+    if (converged(fdim_, sval_, serr_, reqAbs_, reqRel_, norm_)) {
+        done_ = converged_ = true;
         return true;
     }
     double eerr[kMaxFdim], eval_[kMaxFdim];
@@ -379,6 +386,22 @@ void AdaptiveIntegral::result(double *val, double *err) const
             val[j] += pool_[idx].val[j];
             err[j] += pool_[idx].err[j];
         }
+    double c = correction();
+    if (c != 0.0) val[0] += c;
+}
+
+double AdaptiveIntegral::correction_var() const
+{
+    double v = 0;
+    for (uint32_t idx : heap_) v += pool_[idx].cvar;
+    return v;
+}
+
+double AdaptiveIntegral::correction() const
+{
+    double c = 0;
+    for (uint32_t idx : heap_) c += pool_[idx].corr;
+    return c;
 }
 
 }  // namespace tpc

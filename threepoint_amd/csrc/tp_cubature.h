@@ -25,6 +25,8 @@ struct Region {
     unsigned split;
     double val[kMaxFdim], err[kMaxFdim];
     double errmax;
+    double corr;  // additive correction of val[0] carried by a leaf (surrogate checks), 0 otherwise
+    double cvar;  // variance estimate of corr
 };
 
 enum Norm { INDIVIDUAL = 0, PAIRED = 1, L2 = 2, L1 = 3, LINF = 4 };
@@ -47,11 +49,15 @@ class AdaptiveIntegral {
     // the integral is finished (then result() is valid and pending() is empty).
     bool advance();
     void result(double *val, double *err) const;
+    // sum of the leaf corrections (part of result()'s val[0])
+    double correction() const;
+    double correction_var() const;
     size_t num_eval() const { return numEval_; }
     size_t num_regions() const { return pool_.size(); }
     unsigned dim() const { return dim_; }
     unsigned fdim() const { return fdim_; }
     long long rounds() const { return rounds_; }
+    bool converged_flag() const { return converged_; }
     size_t heap_size() const { return heap_.size(); }
 
   private:
@@ -67,6 +73,7 @@ class AdaptiveIntegral {
     double sval_[kMaxFdim] = {0}, serr_[kMaxFdim] = {0};
     long long rounds_ = 0;
     bool done_ = false;
+    bool converged_ = false;  // finished by the tolerance test (not by maxEval)
 };
 
 // Number of Genz-Malik points of a dim-dimensional region (dim >= 2), 15 for dim 1.

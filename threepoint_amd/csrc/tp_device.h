@@ -117,10 +117,10 @@ __device__ __forceinline__ T tp_bispec_q(const TpNodeT<T> &N, const TpEHT<T> &E,
     return tp_bispec_node<T>(N, E, q, lq, f2t, TpMath<T>::lg(c / a), (b + c - a) / a);
 }
 
-// Limber sum over the 96 z-nodes (FP64 accumulation), without the zero-ell guard
-template <class T>
-__device__ __forceinline__ double tp_bkappa_sum(const TpNodeT<T> *__restrict__ nodes, const TpEHT<T> &E,
-                                                double l1, double l2, double l3)
+// Limber sum over NN z-nodes (FP64 accumulation), without the zero-ell guard
+template <class T, int NN>
+__device__ __forceinline__ double tp_bkappa_sum_n(const TpNodeT<T> *__restrict__ nodes, const TpEHT<T> &E,
+                                                  double l1, double l2, double l3)
 {
     TpPointPre P;
     tp_point_pre<double>(l1, l2, l3, &P);
@@ -130,7 +130,7 @@ __device__ __forceinline__ double tp_bkappa_sum(const TpNodeT<T> *__restrict__ n
     const T LR1 = (T)P.lr1, R2 = (T)P.r2;
     double acc = 0.0;
 #pragma unroll 2
-    for (int n = 0; n < TP_NNODES; ++n) {
+    for (int n = 0; n < NN; ++n) {
         const TpNodeT<T> &N = nodes[n];
         const T ks = N.ks, lks = N.lnks;
         T q[3] = {L[0] * ks, L[1] * ks, L[2] * ks};
@@ -139,6 +139,14 @@ __device__ __forceinline__ double tp_bkappa_sum(const TpNodeT<T> *__restrict__ n
         acc += (double)(N.wz * b);
     }
     return acc;
+}
+
+// the 96-node Limber sum of the reference's bkappa (classed_version/bispectrum.cpp:238-286)
+template <class T>
+__device__ __forceinline__ double tp_bkappa_sum(const TpNodeT<T> *__restrict__ nodes, const TpEHT<T> &E,
+                                                double l1, double l2, double l3)
+{
+    return tp_bkappa_sum_n<T, TP_NNODES>(nodes, E, l1, l2, l3);
 }
 
 // the reference's test_analytical B_kappa (classed_version/bispectrum.cpp:242-255, 989-996)
@@ -259,6 +267,8 @@ __device__ __forceinline__ void gm_rule3(double vol, const double *v, TpRegionOu
     double res5th = vol * (wE1 * val0 + wtE2 * sum2 + wE3 * sum3 + wtE4 * sum4);
     out->val = result;
     out->err = fabs(res5th - result);
+    out->corr = 0.0;
+    out->cvar = 0.0;
     int dd = 0;
     double md = 0;
     for (int i = 0; i < 3; ++i)

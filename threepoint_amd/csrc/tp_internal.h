@@ -12,6 +12,7 @@
 #define TP_NNODES 96      // Gauss-Legendre z-nodes of the Limber integral
 #define TP_NZ_MAX 1024    // max redshift bins of the tables
 #define TP_NPTS_GM3 33    // Genz-Malik points per 3-D region
+#define TP_NBASE 8        // Gauss-Legendre z-nodes of the surrogate's baseline Limber sum
 
 // ---------------------------------------------------------------------------
 // Per-cosmology constants, FP64, built on the device by tp_cosmo_build.
@@ -78,6 +79,7 @@ struct tp_cosmo {
     TpNode32 *d_node32 = nullptr;     // [ncosmo][96]
     TpEH64 *d_eh64 = nullptr;         // [ncosmo]
     TpEH32 *d_eh32 = nullptr;         // [ncosmo]
+    TpNode32 *d_base32 = nullptr;     // [ncosmo][8]: 8-node FP32 Limber tables (surrogate baseline)
     int *d_err = nullptr;             // build error flag
     std::vector<TpCosmoConst> h_const;
 };
@@ -93,6 +95,8 @@ struct TpRegion {
 };
 struct TpRegionOut {
     double val, err;
+    double corr;   // surrogate mode: unbiased correction from the exactly evaluated points
+    double cvar;   // surrogate mode: variance estimate of corr
     int split;
     int flag;      // nonzero: non-finite integrand value inside the region
 };

@@ -292,6 +292,12 @@ extern "C" int tp_map3_single(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const 
 }
 
 // ----------------------------------------------------------- batched engine
+// Guards against integrals that stop converging (every mode): a run that needs more rounds or
+// regions than these ends with TP_ECAPACITY instead of growing without bound.  The adaptive
+// rounds of the measured workloads stay below 300 and 1e8 regions.
+static const long long kMaxRounds = 20000;
+static const long long kMaxRegions = 400000000LL;
+static const size_t kSurrogateMaxEval = 4000000;  // points per triple before an exact redo
 int tp_surrogate_round(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegion *d_reg,
                        const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int *d_flag,
                        int *fell_back);  // tp_mlp.hip
@@ -319,7 +325,8 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         desc[t] = TpIntegralDesc{{th[0], th[1], th[2]}, icosmo[t], 0};
         double lMax = 10. / tmin;  // classed_version/apertureStatistics.cpp:325-326
         double vmin[3] = {1., 1., 0.}, vmax[3] = {lMax, lMax, 2 * M_PI};
-        I[t].init(3, 1, vmin, vmax, 0, 0, reltol, tpc::L1);
+        // surrogate mode: a per-triple point budget; a triple that exhausts it is redone exactly
+        I[t].init(3, 1, vmin, vmax, mode == TP_MODE_SURROGATE ? kSurrogateMaxEval : 0, 0, reltol, tpc::L1);
     }
     int rc;
     if ((rc = ctx->d_desc.ensure(sizeof(TpIntegralDesc) * ntriples))) return rc;
@@ -328,11 +335,17 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     std::vector<size_t> active(ntriples), offs;
     for (size_t t = 0; t < ntriples; ++t) active[t] = t;
     const int nthreads = std::max(1, std::min(omp_get_max_threads(), 16));
+    long long total_regions = 0;
     while (!active.empty()) {
+        if (ctx->stats.rounds >= kMaxRounds)
+            return tp_set_error(TP_ECAPACITY, "tp_map3_batch: round limit reached (integrals not converging)");
         size_t na = active.size();
         offs.assign(na + 1, 0);
         for (size_t a = 0; a < na; ++a) offs[a + 1] = offs[a] + I[active[a]].pending().size();
         size_t nreg = offs[na];
+        total_regions += (long long)nreg;
+        if (total_regions > kMaxRegions)
+            return tp_set_error(TP_ECAPACITY, "tp_map3_batch: region limit reached (integrals not converging)");
         if ((rc = ctx->h_regions.ensure(sizeof(TpRegion) * nreg)) || (rc = ctx->d_regions.ensure(sizeof(TpRegion) * nreg)) ||
             (rc = ctx->h_rout.ensure(sizeof(TpRegionOut) * nreg)) || (rc = ctx->d_rout.ensure(sizeof(TpRegionOut) * nreg)) ||
             (rc = ctx->d_fvals.ensure(sizeof(double) * TP_NPTS_GM3 * nreg)))
@@ -363,7 +376,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
                                     (const TpIntegralDesc *)ctx->d_desc.p, (double *)ctx->d_fvals.p,
                                     (TpRegionOut *)ctx->d_rout.p, (int *)ctx->d_flag.p, &fb);
             if (rc) return rc;
-            ctx->stats.fallback_batches += fb;
+            ctx->stats.fallback_batches += (fb == 1);
         } else {
             rc = tp_launch_regions(ctx, c, nreg, (const TpRegion *)ctx->d_regions.p,
                                    (const TpIntegralDesc *)ctx->d_desc.p, (double *)ctx->d_fvals.p,
@@ -397,6 +410,8 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
                 tpc::Region &R = A.regions()[pend[k]];
                 R.val[0] = o.val;
                 R.err[0] = o.err;
+                R.corr = o.corr;
+                R.cvar = o.cvar;
                 R.split = (unsigned)o.split;
                 R.errmax = o.err;
                 bad |= o.flag;
@@ -411,12 +426,43 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
             if (!finished[a]) next.push_back(active[a]);
         active.swap(next);
     }
+    std::vector<size_t> redo;
     for (size_t t = 0; t < ntriples; ++t) {
         double r, e;
         I[t].result(&r, &e);
         val[t] = r / 8 / M_PI / M_PI / M_PI;
         if (err) err[t] = e / 8 / M_PI / M_PI / M_PI;
         if (neval) neval[t] = (long long)I[t].num_eval();
+        // surrogate mode: a triple whose correction has a standard error above tol is redone exactly
+        if (mode == TP_MODE_SURROGATE &&
+            (!I[t].converged_flag() || !(std::sqrt(I[t].correction_var()) <= ctx->sur_tol * std::fabs(r))))
+            redo.push_back(t);
+    }
+    if (!redo.empty()) {
+        tp_batch_stats keep = ctx->stats;
+        std::vector<int> ic(redo.size());
+        std::vector<double> th(3 * redo.size()), v(redo.size()), e(redo.size());
+        std::vector<long long> ne(redo.size());
+        for (size_t k = 0; k < redo.size(); ++k) {
+            ic[k] = icosmo[redo[k]];
+            for (int d = 0; d < 3; ++d) th[3 * k + d] = theta_rad[3 * redo[k] + d];
+        }
+        rc = tp_map3_batch(ctx, c, redo.size(), ic.data(), th.data(), reltol, TP_MODE_FP64, v.data(), e.data(),
+                           ne.data());
+        if (rc) return rc;
+        for (size_t k = 0; k < redo.size(); ++k) {
+            val[redo[k]] = v[k];
+            if (err) err[redo[k]] = e[k];
+            if (neval) neval[redo[k]] += ne[k];
+        }
+        keep.rounds += ctx->stats.rounds;
+        keep.regions += ctx->stats.regions;
+        keep.points += ctx->stats.points;
+        keep.launches += ctx->stats.launches;
+        keep.kernel_ms += ctx->stats.kernel_ms;
+        keep.integrand_ms += ctx->stats.integrand_ms;
+        keep.fallback_integrals += (long long)redo.size();
+        ctx->stats = keep;
     }
     ctx->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;

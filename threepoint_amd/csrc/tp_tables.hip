@@ -405,18 +405,8 @@ __global__ void __launch_bounds__(64) k_growth(int ncosmo, int n_z, double dz, c
 __constant__ double c_gl_x[48];
 __constant__ double c_gl_w[48];
 
-__global__ void k_nodes(int ncosmo, int n_z, const TpCosmoConst *cc, const double *tab, TpNode64 *n64,
-                        TpNode32 *n32, TpEH64 *eh64, TpEH32 *eh32)
+__device__ TpNode64 make_node(const TpCosmoConst &C, const double *T, int n_z, double z, double wgt)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ncosmo * TP_NNODES) return;
-    int ic = t / TP_NNODES, nd = t % TP_NNODES;
-    const TpCosmoConst C = cc[ic];
-    const double *T = tab + (size_t)ic * 6 * n_z;
-    // node ordering: pair i = nd/2, sign - for even nd, + for odd (GQ96_of_bdelta, CV:1010-1020)
-    int i = nd >> 1;
-    double cx = 0.5 * C.zmax, dx = 0.5 * C.zmax;
-    double z = (nd & 1) ? cx + dx * c_gl_x[i] : cx - dx * c_gl_x[i];
     // interpolation coefficients, classed_version/bispectrum.cpp:506-513
     double d = z / C.zmax * (n_z - 1);
     int idx = (int)d;
@@ -434,7 +424,7 @@ __global__ void k_nodes(int ncosmo, int n_z, const TpCosmoConst *cc, const doubl
     double gz = g * (1. + z);
     const double ln10 = 2.302585092994045684;
     TpNode64 N;
-    N.wz = (z == 0.0) ? 0.0 : c_gl_w[i] * dx * C.limber_pref * gz * gz * gz / fK / E;
+    N.wz = (z == 0.0) ? 0.0 : wgt * C.limber_pref * gz * gz * gz / fK / E;
     N.ks = rs / fK;
     N.inv_rs = 1. / rs;
     double L = log10(D1 * C.sigma8);
@@ -466,12 +456,42 @@ __global__ void k_nodes(int ncosmo, int n_z, const TpCosmoConst *cc, const doubl
     double hr = C.h / rs;
     N.pk_pref = C.h * C.h * C.h * 2.0 * kPi * kPi * C.norm * C.norm / (hr * hr * hr);
     N.lnks = log(N.ks);
-    n64[t] = N;
-    TpNode32 M;
+    return N;
+}
+
+__device__ void to_f32(const TpNode64 &N, TpNode32 *M)
+{
     const double *src = &N.wz;
-    float *dst = &M.wz;
+    float *dst = &M->wz;
     for (int k = 0; k < (int)(sizeof(TpNode64) / sizeof(double)); ++k) dst[k] = (float)src[k];
-    n32[t] = M;
+}
+
+// 8-point Gauss-Legendre nodes/weights on [-1, 1] for the surrogate's baseline Limber sum
+__constant__ double c_gl8_x[8];
+__constant__ double c_gl8_w[8];
+
+__global__ void k_nodes(int ncosmo, int n_z, const TpCosmoConst *cc, const double *tab, TpNode64 *n64,
+                        TpNode32 *n32, TpEH64 *eh64, TpEH32 *eh32, TpNode32 *base32)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncosmo * (TP_NNODES + TP_NBASE)) return;
+    int ic = t / (TP_NNODES + TP_NBASE), nd = t % (TP_NNODES + TP_NBASE);
+    const TpCosmoConst C = cc[ic];
+    const double *T = tab + (size_t)ic * 6 * n_z;
+    double cx = 0.5 * C.zmax, dx = 0.5 * C.zmax;
+    if (nd >= TP_NNODES) {  // baseline node
+        int j = nd - TP_NNODES;
+        TpNode64 N = make_node(C, T, n_z, cx + dx * c_gl8_x[j], c_gl8_w[j] * dx);
+        to_f32(N, &base32[(size_t)ic * TP_NBASE + j]);
+        return;
+    }
+    // node ordering: pair i = nd/2, sign - for even nd, + for odd (GQ96_of_bdelta, CV:1010-1020)
+    int i = nd >> 1;
+    double z = (nd & 1) ? cx + dx * c_gl_x[i] : cx - dx * c_gl_x[i];
+    TpNode64 N = make_node(C, T, n_z, z, c_gl_w[i] * dx);
+    int o = ic * TP_NNODES + nd;
+    n64[o] = N;
+    to_f32(N, &n32[o]);
     if (nd == 0) {
         TpEH64 e;
         e.h = C.h;
@@ -531,6 +551,10 @@ static int upload_gl96(int device)
     }
     TP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gl_x), hx, sizeof(hx)));
     TP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gl_w), hw, sizeof(hw)));
+    double x8[8], w8[8];
+    gauss_legendre_host(8, x8, w8);
+    TP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gl8_x), x8, sizeof(x8)));
+    TP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gl8_w), w8, sizeof(w8)));
     if (device >= 0 && device < 64) done[device] = true;
     return 0;
 }
@@ -570,6 +594,7 @@ extern "C" int tp_cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *param
     TPC(hipMalloc(&c->d_node32, sizeof(TpNode32) * TP_NNODES * (size_t)ncosmo));
     TPC(hipMalloc(&c->d_eh64, sizeof(TpEH64) * (size_t)ncosmo));
     TPC(hipMalloc(&c->d_eh32, sizeof(TpEH32) * (size_t)ncosmo));
+    TPC(hipMalloc(&c->d_base32, sizeof(TpNode32) * TP_NBASE * (size_t)ncosmo));
     TPC(hipMalloc(&c->d_err, sizeof(int)));
     TPC(hipMemsetAsync(c->d_err, 0, sizeof(int), s));
     tp_cosmology *d_params = nullptr;
@@ -592,9 +617,9 @@ extern "C" int tp_cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *param
                        c->d_err);
     hipLaunchKernelGGL(k_g, dim3((nt + 127) / 128), dim3(128), 0, s, ncosmo, n_z, dz, z_max, c->d_nz, c->d_tab);
     hipLaunchKernelGGL(k_growth, dim3(nt), dim3(64), 0, s, ncosmo, n_z, dz, c->d_const, c->d_tab);
-    int nn = ncosmo * TP_NNODES;
+    int nn = ncosmo * (TP_NNODES + TP_NBASE);
     hipLaunchKernelGGL(k_nodes, dim3((nn + 127) / 128), dim3(128), 0, s, ncosmo, n_z, c->d_const, c->d_tab,
-                       c->d_node64, c->d_node32, c->d_eh64, c->d_eh32);
+                       c->d_node64, c->d_node32, c->d_eh64, c->d_eh32, c->d_base32);
     TPC(hipGetLastError());
     int herr = 0;
     TPC(hipMemcpyAsync(&herr, c->d_err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -619,6 +644,7 @@ extern "C" int tp_cosmo_destroy(tp_cosmo *c)
     (void)hipFree(c->d_node32);
     (void)hipFree(c->d_eh64);
     (void)hipFree(c->d_eh32);
+    (void)hipFree(c->d_base32);
     (void)hipFree(c->d_err);
     delete c;
     return 0;
