@@ -128,6 +128,10 @@ int tp_last_batch_stats(const tp_ctx *ctx, tp_batch_stats *out);
  * elementary calls per point outside the node loop, out[4] z-nodes per point.  Counted by
  * instantiating the kernel's own device functions with a counting scalar on the host. */
 int tp_census_map3_point(double *out);
+/* Census of the Gamma_i integrand (tp_gamma_eval, tp_gamma_batch): out[0] flops and out[1]
+ * elementary calls per Ogata term (a cubature point has 3 (prec_k - 1) terms), out[2] flops
+ * and out[3] elementary calls of the per-point triangle shape. */
+int tp_census_gamma_term(double *out);
 
 /* ---------------------------------------------------------------- Gamma_i */
 /* Replaces GammaCalculator(bk, prec_h, prec_k, "centroid") + initialize_bessel

@@ -87,6 +87,7 @@ _SIGS = {
                                       ctypes.c_size_t, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp]),
     "tp_hcubature_last_neval": (ctypes.c_longlong, []),
     "tp_census_map3_point": (ctypes.c_int, [_dp]),
+    "tp_census_gamma_term": (ctypes.c_int, [_dp]),
     "tp_surrogate_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(tp_train_config),
                                           ctypes.POINTER(tp_train_report)]),
     "tp_surrogate_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
@@ -307,6 +308,20 @@ def flop_equivalents_per_point(flops_per_elem=20.0):
     c = census_map3_point()
     return c["nodes"] * (c["node_flops"] + flops_per_elem * c["node_elem"]) + c["point_flops"] + \
         flops_per_elem * c["point_elem"]
+
+
+def census_gamma_term():
+    """Algorithmic op census of one Ogata term of the Gamma_i integrand (see tp_census_gamma_term)."""
+    out = np.zeros(4)
+    _check(lib().tp_census_gamma_term(_d(out)), "tp_census_gamma_term")
+    return dict(term_flops=out[0], term_elem=out[1], shape_flops=out[2], shape_elem=out[3])
+
+
+def flop_equivalents_per_gamma_point(prec_k, flops_per_elem=20.0):
+    """Work of one Gamma cubature point: 3 (prec_k - 1) Ogata terms plus the per-point shape."""
+    c = census_gamma_term()
+    return 3 * (prec_k - 1) * (c["term_flops"] + flops_per_elem * c["term_elem"]) + c["shape_flops"] + \
+        flops_per_elem * c["shape_elem"]
 
 
 def gamma_setup(prec_h=1e-3, prec_k_arg=5.0, ctx=None):

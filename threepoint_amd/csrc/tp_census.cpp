@@ -32,6 +32,7 @@ inline bool operator>(Cnt a, Cnt b) { return a.v > b.v; }
 template <> struct TpMath<Cnt> {
     static Cnt ex(Cnt x) { ++g_cnt.trans; return Cnt(std::exp(x.v)); }
     static Cnt lg(Cnt x) { ++g_cnt.trans; return Cnt(std::log(x.v)); }
+    static Cnt rcp(Cnt x) { ++g_cnt.flops; return Cnt(1.0 / x.v); }
 };
 
 // out: [0] flops per node, [1] elementary calls per node, [2] flops per point outside
@@ -76,5 +77,35 @@ extern "C" int tp_census_map3_point(double *out)
     out[2] = (double)point.flops;
     out[3] = (double)point.trans;
     out[4] = (double)TP_NNODES;
+    return 0;
+}
+
+// out: [0] flops per Ogata term, [1] elementary calls per term (a Gamma cubature point has
+// 3 (prec_k - 1) terms), [2] flops and [3] elementary calls of the per-point shape
+extern "C" int tp_census_gamma_term(double *out)
+{
+    Cnt vals[sizeof(TpNodeT<Cnt>) / sizeof(Cnt)];
+    for (auto &v : vals) v = Cnt(0.5);
+    TpNodeT<Cnt> N = *reinterpret_cast<TpNodeT<Cnt> *>(vals);
+    N.acap = Cnt(10.0);
+    Cnt ev[sizeof(TpEHT<Cnt>) / sizeof(Cnt)];
+    for (auto &v : ev) v = Cnt(0.7);
+    TpEHT<Cnt> E = *reinterpret_cast<TpEHT<Cnt> *>(ev);
+    g_cnt = OpCounts();
+    // shape: cos psi, sin psi, cos phi (3 calls), u3^2, sqrt (1 call), ln A_j (3 calls)
+    Cnt cps = TpMath<Cnt>::ex(Cnt(0.3)), sps = TpMath<Cnt>::ex(Cnt(0.3)), cph = TpMath<Cnt>::ex(Cnt(0.3));
+    Cnt u3 = TpMath<Cnt>::ex(cps * cps + sps * sps + Cnt(2) * cps * sps * cph);
+    TpPointPreT<Cnt> P;
+    tp_point_pre<Cnt>(cps, sps, u3, &P);
+    for (int j = 0; j < 3; ++j) (void)TpMath<Cnt>::lg(Cnt(2.0));
+    OpCounts shape = g_cnt;
+    g_cnt = OpCounts();
+    Cnt acc(0.0);
+    acc += tp_ogata_term<Cnt>(N, E, P, Cnt(30.0), Cnt(3.4), Cnt(0.1));
+    OpCounts term = g_cnt;
+    out[0] = (double)term.flops;
+    out[1] = (double)term.trans;
+    out[2] = (double)shape.flops;
+    out[3] = (double)shape.trans;
     return 0;
 }

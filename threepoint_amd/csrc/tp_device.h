@@ -11,11 +11,13 @@ template <class T> struct TpMath;
 template <> struct TpMath<double> {
     static __host__ __device__ __forceinline__ double ex(double x) { return exp(x); }
     static __host__ __device__ __forceinline__ double lg(double x) { return log(x); }
+    static __host__ __device__ __forceinline__ double rcp(double x) { return 1.0 / x; }  // IEEE division
 };
 template <> struct TpMath<float> {
-    // native transcendental unit: v_exp_f32 / v_log_f32 with a scale
+    // native transcendental unit: v_exp_f32 / v_log_f32 with a scale, v_rcp_f32 (1 ulp)
     static __device__ __forceinline__ float ex(float x) { return __expf(x); }
     static __device__ __forceinline__ float lg(float x) { return __logf(x); }
+    static __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 };
 
 template <class T>
@@ -26,20 +28,27 @@ __host__ __device__ __forceinline__ T tp_f2_tree(T k1, T k2, T k3)
     return T(5. / 7.) + T(0.5) * c12 * (k1 / k2 + k2 / k1) + T(2. / 7.) * c12 * c12;
 }
 
-// Eisenstein & Hu no-wiggle P_lin at k = q / r_sigma (classed_version/bispectrum.cpp:811-841)
+// Eisenstein & Hu no-wiggle P_lin at k = q / r_sigma (classed_version/bispectrum.cpp:811-841).
+// The quotients of the reference are merged into two reciprocals:
+//   q_eff = k theta^2 / Gamma_eff, Gamma_eff = om h^2 (sqrt(alpha_nu) + (1 - sqrt(alpha_nu)) / (1 + x^4))
+//         = k theta^2 (1 + x^4) / (om h^2 (sqrt(alpha_nu) (1 + x^4) + 1 - sqrt(alpha_nu)))
+//   T = L / (L + C q_eff^2), C = 14.4 + 325 / den, den = 1 + 60.5 q_eff^1.11
+//     = L den / (L den + (14.4 den + 325) q_eff^2)
+//   P_lin ~ Delta^2 / q^3 = exp(3 n_s (ln q + ln(2997.9 / r_sigma))) T^2 / q^3
+//                        = exp(3 n_s (ln q + ln(2997.9 / r_sigma)) - 3 ln q) T^2
 template <class T>
 __host__ __device__ __forceinline__ T tp_plin_q(const TpNodeT<T> &N, const TpEHT<T> &E, T q, T lq)
 {
     T kk = q * N.inv_rs * E.h;  // 1/Mpc
     T x = E.sh043 * kk;
     T x2 = x * x;
-    T geff = E.omh2 * (E.sqrt_alnu + E.one_m_sqrt_alnu / (T(1) + x2 * x2));
-    T qeff = kk / geff * E.theta2;
+    T d = T(1) + x2 * x2;
+    T qeff = kk * E.theta2 * d * TpMath<T>::rcp(E.omh2 * (E.sqrt_alnu * d + E.one_m_sqrt_alnu));
     T L = TpMath<T>::lg(E.e_const + E.cL * qeff);
-    T Cq = T(14.4) + T(325.0) / (T(1) + T(60.5) * TpMath<T>::ex(T(1.11) * TpMath<T>::lg(qeff)));
-    T r = L / (L + Cq * qeff * qeff);
-    T delk = TpMath<T>::ex(E.three_ns * (lq + N.lnkoff)) * r * r;
-    return N.pk_pref * delk / (q * q * q);
+    T den = T(1) + T(60.5) * TpMath<T>::ex(T(1.11) * TpMath<T>::lg(qeff));
+    T Ld = L * den;
+    T r = Ld * TpMath<T>::rcp(Ld + (T(14.4) * den + T(325.0)) * qeff * qeff);
+    return N.pk_pref * TpMath<T>::ex(E.three_ns * (lq + N.lnkoff) - T(3) * lq) * r * r;
 }
 
 // Scale-free per-point quantities, evaluated once per point in FP64: q_i = l_i ks_n for
@@ -75,7 +84,11 @@ __host__ __device__ __forceinline__ void tp_point_pre(T l1, T l2, T l3, TpPointP
     P->r2 = (b + c - a) / a;
 }
 
-// BiHalofit matter bispectrum at one z-node, classed_version/bispectrum.cpp:166-236
+// BiHalofit matter bispectrum at one z-node, classed_version/bispectrum.cpp:166-236.
+// Quotients are rewritten with one reciprocal each:
+//   1 / (I_1h (1 + 1/(c_n q)))  = c_n q / (I_1h (c_n q + 1))                    (eq. B4, B7)
+//   (1 + f_n q^2) / (1 + g_n q + h_n q^2) D^2 P_lin + 1 / (M (1 + 1/(p_n q)^3))
+//     = (A D^2 P_lin C + B (p_n q)^3) / (B C),  C = M ((p_n q)^3 + 1)            (eq. B6)
 template <class T>
 __host__ __device__ __forceinline__ T tp_bispec_node(const TpNodeT<T> &N, const TpEHT<T> &E, const T q[3], const T lq[3],
                                             const T f2t[3], T lr1, T r2)
@@ -90,17 +103,20 @@ __host__ __device__ __forceinline__ T tp_bispec_node(const TpNodeT<T> &N, const 
     for (int i = 0; i < 3; ++i) {
         const T qi = q[i], lqi = lq[i];
         T oneh = an * TpMath<T>::ex(alphan * lqi) + N.bn * TpMath<T>::ex(betan * lqi);   // eq. B4
-        bs1h *= T(1) / (oneh * (T(1) + T(1) / (N.cn * qi)));
+        T cq = N.cn * qi;
+        bs1h *= cq * TpMath<T>::rcp(oneh * (cq + T(1)));
         T pl = tp_plin_q(N, E, qi, lqi);
         T pq = N.pn * qi;
-        pse[i] = (T(1) + N.fn * qi * qi) / (T(1) + N.gn * qi + N.hn * qi * qi) * N.D1sq * pl +
-                 T(1) / ((N.mn * TpMath<T>::ex(N.mun * lqi) + N.nn * TpMath<T>::ex(N.nun * lqi)) *
-                         (T(1) + T(1) / (pq * pq * pq)));                                  // eq. B6
+        T pq3 = pq * pq * pq;
+        T A = T(1) + N.fn * qi * qi;
+        T B = T(1) + N.gn * qi + N.hn * qi * qi;
+        T C = (N.mn * TpMath<T>::ex(N.mun * lqi) + N.nn * TpMath<T>::ex(N.nun * lqi)) * (pq3 + T(1));
+        pse[i] = (A * N.D1sq * pl * C + B * pq3) * TpMath<T>::rcp(B * C);  // eq. B6
         damp *= T(1) + N.en * qi;
     }
     T bs3h = T(2) * ((f2t[0] + N.dn * q[2]) * pse[0] * pse[1] + (f2t[1] + N.dn * q[0]) * pse[1] * pse[2] +
                      (f2t[2] + N.dn * q[1]) * pse[2] * pse[0]);                            // eq. B5 with F2 (:614-627)
-    return bs1h + bs3h / damp;
+    return bs1h + bs3h * TpMath<T>::rcp(damp);
 }
 
 // Same bispectrum from raw wavenumbers (used where the per-point hoisting does not apply)
@@ -139,6 +155,20 @@ __device__ __forceinline__ double tp_bkappa_sum_n(const TpNodeT<T> *__restrict__
         acc += (double)(N.wz * b);
     }
     return acc;
+}
+
+// One node of the Ogata sums of classed_version/gamma.cpp:298-314 with the per-point shape
+// hoisted: every term of a cubature point (z, phi, psi) has the sides
+//   l = (psi_k / A) (cos psi, sin psi, sqrt(1 + sin(2 psi) cos phi)),
+// i.e. one triangle shape P (unit scale) at the scale s = psi_k / A.  ln_s = ln psi_k - ln A.
+template <class T>
+__host__ __device__ __forceinline__ T tp_ogata_term(const TpNodeT<T> &N, const TpEHT<T> &E, const TpPointPreT<T> &P,
+                                                     T s, T ln_s, T prodk)
+{
+    const T sk = s * N.ks, lsk = ln_s + N.lnks;
+    T q[3] = {P.l[0] * sk, P.l[1] * sk, P.l[2] * sk};
+    T lq[3] = {P.ll[0] + lsk, P.ll[1] + lsk, P.ll[2] + lsk};
+    return N.wz * tp_bispec_node<T>(N, E, q, lq, P.f2t, P.lr1, P.r2) * prodk;
 }
 
 // the 96-node Limber sum of the reference's bkappa (classed_version/bispectrum.cpp:238-286)

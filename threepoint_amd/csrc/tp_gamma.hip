@@ -153,15 +153,30 @@ __global__ void __launch_bounds__(kGammaBlock) k_gamma_points(const TpCosmoConst
     }
     const double *psik = ogata + (D.which == 0 ? 0 : 2) * (size_t)prec_k;
     const double *prodk = psik + prec_k;
+    const double *lnpsik = ogata + (D.which == 0 ? 4 : 5) * (size_t)prec_k;
     double s[3] = {0, 0, 0};
     if (sok) {
         const TpEH64 E = eh[D.icosmo];
         const double cpsi = cos(psi), spsi = sin(psi), cphi = cos(phi);
         double invA[3] = {1.0 / A[0], 1.0 / A[1], 1.0 / A[2]};
-        for (int k = 1 + threadIdx.x; k < prec_k; k += kGammaBlock) {
-            double pk = psik[k], wk = prodk[k];
+        // every term's triangle has the shape (cos psi, sin psi, u3); away from the degenerate
+        // shape u3 -> 0 (where ell3 = 0 decides the value) the shape is computed once per point
+        const double u3s = cpsi * cpsi + spsi * spsi + 2 * cpsi * spsi * cphi;
+        if (u3s > 1e-6) {
+            TpPointPre P;
+            tp_point_pre<double>(cpsi, spsi, sqrt(u3s), &P);
+            const double lninvA[3] = {-log(A[0]), -log(A[1]), -log(A[2])};
+            for (int k = 1 + threadIdx.x; k < prec_k; k += kGammaBlock) {
+                const double pk = psik[k], wk = prodk[k], lpk = lnpsik[k];
 #pragma unroll
-            for (int j = 0; j < 3; ++j) s[j] += ogata_term(sN, E, pk, wk, invA[j], cpsi, spsi, cphi);
+                for (int j = 0; j < 3; ++j) s[j] += tp_ogata_term<double>(sN, E, P, pk * invA[j], lpk + lninvA[j], wk);
+            }
+        } else {
+            for (int k = 1 + threadIdx.x; k < prec_k; k += kGammaBlock) {
+                const double pk = psik[k], wk = prodk[k];
+#pragma unroll
+                for (int j = 0; j < 3; ++j) s[j] += ogata_term(sN, E, pk, wk, invA[j], cpsi, spsi, cphi);
+            }
         }
     }
     int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -358,7 +373,7 @@ extern "C" int tp_gamma_setup(tp_ctx *ctx, double prec_h, double prec_k_arg)
     if (!ctx || !(prec_h > 0) || !(prec_k_arg > 0)) return tp_set_error(TP_EINVAL, "tp_gamma_setup: bad arguments");
     unsigned pk = (unsigned)(prec_k_arg / prec_h);
     if (pk < 2 || pk > (1u << 24)) return tp_set_error(TP_EINVAL, "tp_gamma_setup: prec_k out of range");
-    std::vector<double> h(4 * (size_t)pk, 0.0);
+    std::vector<double> h(6 * (size_t)pk, 0.0);  // psi6, prod6, psi2, prod2, ln psi6, ln psi2
     auto psi = [](double t) { return t * std::tanh(M_PI * std::sinh(t) / 2); };
     auto psip = [](double t) {
         return (std::sinh(M_PI * std::sinh(t)) + M_PI * t * std::cosh(t)) / (std::cosh(M_PI * std::sinh(t)) + 1);
@@ -375,6 +390,7 @@ extern "C" int tp_gamma_setup(tp_ctx *ctx, double prec_h, double prec_k_arg)
             double prod = w * ps * ps * ps * jn_host(nu, ps) * psip(prec_h * pbz);
             h[(size_t)(2 * which) * pk + i] = ps;
             h[(size_t)(2 * which + 1) * pk + i] = prod;
+            h[(size_t)(4 + which) * pk + i] = std::log(ps);
         }
     }
     TP_HIP(hipSetDevice(ctx->device));
@@ -415,6 +431,7 @@ static int gamma_launch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpGamm
     float ms = 0;
     hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     ctx->stats.kernel_ms += ms;
+    ctx->stats.integrand_ms += ms;
     ctx->stats.launches += 1;
     int flag = 0;
     TP_HIP(hipMemcpy(&flag, ctx->d_flag.p, sizeof(int), hipMemcpyDeviceToHost));

@@ -132,7 +132,7 @@ struct tp_ctx {
     // Gamma (Ogata) tables
     double gamma_prec_h = 0;
     unsigned gamma_prec_k = 0;
-    double *d_ogata = nullptr;  // [4][prec_k]: psi6, prod6, psi2, prod2
+    double *d_ogata = nullptr;  // [6][prec_k]: psi6, prod6, psi2, prod2, ln psi6, ln psi2
     TpSurrogate *sur = nullptr;
     double sur_check_frac = 0.01, sur_tol = 1e-3;
     uint64_t sur_seed = 0x5eed;
