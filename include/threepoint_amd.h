@@ -120,6 +120,8 @@ typedef struct {
     double max_check_relerr;     /* worst relative error seen by the exact checks  */
     long long fallback_integrals;/* triples re-integrated exactly because the standard error
                                     of their surrogate correction exceeded tol          */
+    double host_ms;              /* host time in region gathering and heap decisions   */
+    double wait_ms;              /* host time blocked waiting for the device           */
 } tp_batch_stats;
 int tp_last_batch_stats(const tp_ctx *ctx, tp_batch_stats *out);
 

@@ -41,7 +41,8 @@ class tp_batch_stats(ctypes.Structure):
                 ("wall_ms", ctypes.c_double),
                 ("surrogate_points", ctypes.c_longlong), ("exact_checks", ctypes.c_longlong),
                 ("fallback_batches", ctypes.c_longlong), ("max_check_relerr", ctypes.c_double),
-                ("fallback_integrals", ctypes.c_longlong)]
+                ("fallback_integrals", ctypes.c_longlong), ("host_ms", ctypes.c_double),
+                ("wait_ms", ctypes.c_double)]
 
 
 class tp_train_config(ctypes.Structure):

@@ -96,7 +96,8 @@ static const double kWgk[8] = {0.022935322010529224963732008058970, 0.0630920926
                                0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
                                0.204432940075298892414161999234649, 0.209482141084727828012999174891714};
 
-void rule_points_of(unsigned dim, const Region &R, double *pts)
+template <unsigned MD, unsigned MF>
+void rule_points_of(unsigned dim, const RegionT<MD, MF> &R, double *pts)
 {
     const double *c = R.c, *h = R.hw;
     if (dim == 1) {  // classed_version/hcubature.cpp:663-681
@@ -156,7 +157,8 @@ void rule_points_of(unsigned dim, const Region &R, double *pts)
     }
 }
 
-void rule_apply(unsigned dim, unsigned fdim, Region &R, const double *vals)
+template <unsigned MD, unsigned MF>
+void rule_apply(unsigned dim, unsigned fdim, RegionT<MD, MF> &R, const double *vals)
 {
     if (dim == 1) {  // GK15 with QUADPACK's error rescaling, classed_version/hcubature.cpp:688-754
         for (unsigned k = 0; k < fdim; ++k) {
@@ -252,10 +254,11 @@ void rule_apply(unsigned dim, unsigned fdim, Region &R, const double *vals)
     R.cvar = 0.0;
 }
 
-bool AdaptiveIntegral::init(unsigned dim, unsigned fdim, const double *xmin, const double *xmax, size_t maxEval,
+template <unsigned MD, unsigned MF>
+bool AdaptiveIntegralT<MD, MF>::init(unsigned dim, unsigned fdim, const double *xmin, const double *xmax, size_t maxEval,
                             double reqAbs, double reqRel, int norm)
 {
-    if (dim == 0 || dim > kMaxDim || fdim == 0 || fdim > kMaxFdim) return false;
+    if (dim == 0 || dim > MD || fdim == 0 || fdim > MF) return false;
     if (fdim <= 1) norm = INDIVIDUAL;  // classed_version/hcubature.cpp:919-920
     if (norm < 0 || norm > LINF) return false;
     dim_ = dim;
@@ -268,8 +271,8 @@ bool AdaptiveIntegral::init(unsigned dim, unsigned fdim, const double *xmin, con
     pool_.clear();
     heap_.clear();
     pending_.clear();
-    for (unsigned k = 0; k < kMaxFdim; ++k) sval_[k] = serr_[k] = 0;
-    Region R{};
+    for (unsigned k = 0; k < MF; ++k) sval_[k] = serr_[k] = 0;
+    RegionType R{};
     for (unsigned i = 0; i < dim; ++i) {  // classed_version/hcubature.cpp:160-172
         R.c[i] = 0.5 * (xmin[i] + xmax[i]);
         R.hw[i] = 0.5 * (xmax[i] - xmin[i]);
@@ -285,44 +288,59 @@ bool AdaptiveIntegral::init(unsigned dim, unsigned fdim, const double *xmin, con
     return true;
 }
 
-void AdaptiveIntegral::heap_push(uint32_t idx)
+template <unsigned MD, unsigned MF>
+void AdaptiveIntegralT<MD, MF>::heap_push(uint32_t idx)
 {
-    // classed_version/hcubature.cpp:819-843
-    const Region &r = pool_[idx];
+    // classed_version/hcubature.cpp:808-835
+    const RegionType &r = pool_[idx];
     for (unsigned i = 0; i < fdim_; ++i) {
         sval_[i] += r.val[i];
         serr_[i] += r.err[i];
     }
     size_t ins = heap_.size();
-    heap_.push_back(idx);
+    heap_.push_back(HeapEnt{r.errmax, idx});
     const double key = r.errmax;
+    HeapEnt *H = heap_.data();
     while (ins) {
         size_t parent = (ins - 1) / 2;
-        if (key <= pool_[heap_[parent]].errmax) break;
-        heap_[ins] = heap_[parent];
+        if (key <= H[parent].key) break;
+        H[ins] = H[parent];
         ins = parent;
     }
-    heap_[ins] = idx;
+    H[ins] = HeapEnt{key, idx};
 }
 
-uint32_t AdaptiveIntegral::heap_pop()
+template <unsigned MD, unsigned MF>
+uint32_t AdaptiveIntegralT<MD, MF>::heap_pop()
 {
-    // classed_version/hcubature.cpp:853-890
-    uint32_t ret = heap_[0];
-    size_t n = heap_.size() - 1;
-    heap_[0] = heap_[n];
+    // classed_version/hcubature.cpp:853-890: the last item moves to the root and sinks while a
+    // child's key is larger (the left child wins unless the right one is strictly larger).
+    // Written with a hole and conditional moves; the resulting arrangement is the same.
+    uint32_t ret = heap_[0].idx;
+    const size_t n = heap_.size() - 1;
+    const HeapEnt moved = heap_[n];
     heap_.pop_back();
-    size_t i = 0, child;
-    while ((child = i * 2 + 1) < n) {
-        size_t largest = (pool_[heap_[child]].errmax <= pool_[heap_[i]].errmax) ? i : child;
-        if (++child < n && pool_[heap_[largest]].errmax < pool_[heap_[child]].errmax) largest = child;
-        if (largest == i) break;
-        uint32_t t = heap_[i];
-        heap_[i] = heap_[largest];
-        heap_[largest] = t;
-        i = largest;
+    if (n) {
+        HeapEnt *H = heap_.data();
+        const double km = moved.key;
+        size_t i = 0, child;
+        while ((child = 2 * i + 1) < n) {
+            __builtin_prefetch(H + 4 * i + 3);
+            const double kc = H[child].key;
+            size_t largest = (kc <= km) ? i : child;
+            double kl = (kc <= km) ? km : kc;
+            const size_t c2 = child + 1;
+            if (c2 < n) {
+                const double k2 = H[c2].key;
+                largest = (kl < k2) ? c2 : largest;
+            }
+            if (largest == i) break;
+            H[i] = H[largest];
+            i = largest;
+        }
+        H[i] = moved;
     }
-    const Region &r = pool_[ret];
+    const RegionType &r = pool_[ret];
     for (unsigned k = 0; k < fdim_; ++k) {
         sval_[k] -= r.val[k];
         serr_[k] -= r.err[k];
@@ -330,13 +348,15 @@ uint32_t AdaptiveIntegral::heap_pop()
     return ret;
 }
 
-void AdaptiveIntegral::absorb()
+template <unsigned MD, unsigned MF>
+void AdaptiveIntegralT<MD, MF>::absorb()
 {
     for (uint32_t idx : pending_) heap_push(idx);
     pending_.clear();
 }
 
-bool AdaptiveIntegral::advance()
+template <unsigned MD, unsigned MF>
+bool AdaptiveIntegralT<MD, MF>::advance()
 {
     // one iteration of the main loop, classed_version/hcubature.cpp:938-989
     if (done_) return true;
@@ -348,7 +368,7 @@ bool AdaptiveIntegral::advance()
         done_ = converged_ = true;
         return true;
     }
-    double eerr[kMaxFdim], eval_[kMaxFdim];
+    double eerr[MF], eval_[MF];
     for (unsigned j = 0; j < fdim_; ++j) {
         eerr[j] = serr_[j];
         eval_[j] = sval_[j];
@@ -358,11 +378,11 @@ bool AdaptiveIntegral::advance()
         for (unsigned j = 0; j < fdim_; ++j) eerr[j] -= pool_[a].err[j];
         // cut_region (classed_version/hcubature.cpp:206-218): the kept half halves its
         // volume, the new half recomputes it from the widths.
-        Region &A = pool_[a];
+        RegionType &A = pool_[a];
         unsigned d = A.split;
         A.hw[d] *= 0.5;
         A.vol *= 0.5;
-        Region B = A;
+        RegionType B = A;
         B.vol = compute_vol(dim_, B.hw);
         A.c[d] -= A.hw[d];
         B.c[d] += A.hw[d];
@@ -377,31 +397,44 @@ bool AdaptiveIntegral::advance()
     return false;
 }
 
-void AdaptiveIntegral::result(double *val, double *err) const
+template <unsigned MD, unsigned MF>
+void AdaptiveIntegralT<MD, MF>::result(double *val, double *err) const
 {
     // final re-sum in heap order, classed_version/hcubature.cpp:1000-1008
     for (unsigned j = 0; j < fdim_; ++j) val[j] = err[j] = 0;
-    for (uint32_t idx : heap_)
+    for (const HeapEnt &h : heap_)
         for (unsigned j = 0; j < fdim_; ++j) {
-            val[j] += pool_[idx].val[j];
-            err[j] += pool_[idx].err[j];
+            val[j] += pool_[h.idx].val[j];
+            err[j] += pool_[h.idx].err[j];
         }
     double c = correction();
     if (c != 0.0) val[0] += c;
 }
 
-double AdaptiveIntegral::correction_var() const
+template <unsigned MD, unsigned MF>
+double AdaptiveIntegralT<MD, MF>::correction_var() const
 {
     double v = 0;
-    for (uint32_t idx : heap_) v += pool_[idx].cvar;
+    for (const HeapEnt &h : heap_) v += pool_[h.idx].cvar;
     return v;
 }
 
-double AdaptiveIntegral::correction() const
+template <unsigned MD, unsigned MF>
+double AdaptiveIntegralT<MD, MF>::correction() const
 {
     double c = 0;
-    for (uint32_t idx : heap_) c += pool_[idx].corr;
+    for (const HeapEnt &h : heap_) c += pool_[h.idx].corr;
     return c;
 }
+
+template class AdaptiveIntegralT<kMaxDim, kMaxFdim>;
+template class AdaptiveIntegralT<3, 1>;
+template class AdaptiveIntegralT<3, 2>;
+template void rule_points_of<kMaxDim, kMaxFdim>(unsigned, const RegionT<kMaxDim, kMaxFdim> &, double *);
+template void rule_points_of<3, 1>(unsigned, const RegionT<3, 1> &, double *);
+template void rule_points_of<3, 2>(unsigned, const RegionT<3, 2> &, double *);
+template void rule_apply<kMaxDim, kMaxFdim>(unsigned, unsigned, RegionT<kMaxDim, kMaxFdim> &, const double *);
+template void rule_apply<3, 1>(unsigned, unsigned, RegionT<3, 1> &, const double *);
+template void rule_apply<3, 2>(unsigned, unsigned, RegionT<3, 2> &, const double *);
 
 }  // namespace tpc

@@ -325,7 +325,7 @@ double jn_zero_host(int n, int s)
 }
 
 struct GammaIntegral {
-    tpc::AdaptiveIntegral I;
+    tpc::AdaptiveIntegral32 I;
     size_t tri;
     int comp;
 };
@@ -505,7 +505,7 @@ extern "C" int tp_gamma_batch(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t
         pts.resize(npts * 3);
         vals.resize(npts * 2);
         pdesc.resize(npts);
-#pragma omp parallel for schedule(dynamic, 8) if (na > 64)
+#pragma omp parallel for schedule(dynamic, 1) if (na > 1 && npts > 16384)
         for (size_t a = 0; a < na; ++a) {
             auto &A = G[active[a]].I;
             const auto &pend = A.pending();
@@ -520,7 +520,7 @@ extern "C" int tp_gamma_batch(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t
         ctx->stats.rounds += 1;
         ctx->stats.points += (long long)npts;
         std::vector<char> fin(na, 0);
-#pragma omp parallel for schedule(dynamic, 8) if (na > 64)
+#pragma omp parallel for schedule(dynamic, 1) if (na > 1 && npts > 16384)
         for (size_t a = 0; a < na; ++a) {
             auto &A = G[active[a]].I;
             const auto &pend = A.pending();

@@ -122,6 +122,18 @@ struct HostPinned {
 
 struct TpSurrogate;  // learned B_kappa model (tp_mlp.hip)
 
+// Buffers and events of one group of integrals in the batched engine.  Two groups
+// alternate: the device evaluates one group's regions while the host runs the
+// other group's heap decisions.
+struct TpLane {
+    DevBuf d_regions, d_rout, d_fvals, d_flag;
+    HostPinned h_regions, h_rout, h_flag;
+    hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr, done = nullptr;
+    int create();
+    void release();
+};
+constexpr int TP_NLANES = 2;
+
 struct tp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -137,6 +149,7 @@ struct tp_ctx {
     double sur_check_frac = 0.01, sur_tol = 1e-3;
     uint64_t sur_seed = 0x5eed;
     int autotune_block = 0;     // 0 = not tuned yet
+    TpLane lane[TP_NLANES];
 };
 
 // Error helpers (thread-local last error string).
@@ -155,6 +168,6 @@ int tp_launch_bkappa_points(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t n
                             double *d_out, int mode, int *d_flag);
 int tp_launch_regions(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegion *d_reg,
                       const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int mode,
-                      int *d_flag);
+                      int *d_flag, hipEvent_t evm);
 int tp_launch_gamma_points(tp_ctx *ctx, const tp_cosmo *c, int icosmo, int which, double x1, double x2,
                            double x3, size_t npts, const double *d_x, double *d_f, int *d_flag);

@@ -70,13 +70,17 @@ __global__ void __launch_bounds__(256) k_region_points(const TpNodeT<T> *__restr
     fvals[t] = tp_map3_integrand<T>(nd, eh[D.icosmo], x0, x1, x2, D.th[0], D.th[1], D.th[2], mode, flag);
 }
 
+// Region r's flag: bit 0 = non-finite rule estimate.  Region 0 also carries the point
+// kernel's flag word (non-finite ell: bit 1, non-finite B_kappa: bit 2), so one copy of
+// the region outputs returns every error of the round.
 __global__ void __launch_bounds__(256) k_region_rule(size_t nreg, const TpRegion *__restrict__ reg,
                                                      const double *__restrict__ fvals,
-                                                     TpRegionOut *__restrict__ out)
+                                                     TpRegionOut *__restrict__ out, const int *__restrict__ flag)
 {
     size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nreg) return;
     gm_rule3(reg[r].vol, fvals + r * TP_NPTS_GM3, &out[r]);
+    if (r == 0) out[0].flag |= *flag << 1;
 }
 
 int grid_for(size_t n, int block) { return (int)((n + block - 1) / block); }
@@ -118,7 +122,8 @@ int tp_launch_bkappa_points(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t n
 }
 
 int tp_launch_regions(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegion *d_reg,
-                      const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int mode, int *d_flag)
+                      const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int mode, int *d_flag,
+                      hipEvent_t evm)
 {
     if (nreg == 0) return 0;
     const int B = 256;
@@ -129,8 +134,9 @@ int tp_launch_regions(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegio
     else
         hipLaunchKernelGGL(k_region_points<double>, dim3(grid_for(np, B)), dim3(B), 0, ctx->stream, c->d_node64,
                            c->d_eh64, nreg, d_reg, d_desc, d_fvals, mode, d_flag);
-    TP_HIP(hipEventRecord(ctx->evm, ctx->stream));
-    hipLaunchKernelGGL(k_region_rule, dim3(grid_for(nreg, B)), dim3(B), 0, ctx->stream, nreg, d_reg, d_fvals, d_out);
+    TP_HIP(hipEventRecord(evm, ctx->stream));
+    hipLaunchKernelGGL(k_region_rule, dim3(grid_for(nreg, B)), dim3(B), 0, ctx->stream, nreg, d_reg, d_fvals, d_out,
+                       d_flag);
     TP_HIP(hipGetLastError());
     return 0;
 }

@@ -95,6 +95,24 @@ extern "C" int tp_ctx_create(int device, tp_ctx **out)
     return 0;
 }
 
+int TpLane::create()
+{
+    if (done) return 0;
+    for (hipEvent_t *e : {&ev0, &evm, &ev1, &done}) TP_HIP(hipEventCreate(e));
+    if (h_flag.ensure(64) || d_flag.ensure(256)) return TP_EHIP;
+    return 0;
+}
+void TpLane::release()
+{
+    for (DevBuf *b : {&d_regions, &d_rout, &d_fvals, &d_flag}) b->release();
+    for (HostPinned *b : {&h_regions, &h_rout, &h_flag}) b->release();
+    for (hipEvent_t *e : {&ev0, &evm, &ev1, &done})
+        if (*e) {
+            (void)hipEventDestroy(*e);
+            *e = nullptr;
+        }
+}
+
 void tp_surrogate_free(tp_ctx *ctx);  // tp_mlp.hip
 
 extern "C" int tp_ctx_destroy(tp_ctx *ctx)
@@ -107,6 +125,7 @@ extern "C" int tp_ctx_destroy(tp_ctx *ctx)
                       &ctx->d_flag, &ctx->d_misc})
         b->release();
     for (HostPinned *b : {&ctx->h_in, &ctx->h_out, &ctx->h_regions, &ctx->h_rout}) b->release();
+    for (TpLane &L : ctx->lane) L.release();
     if (ctx->d_ogata) hipFree(ctx->d_ogata);
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
@@ -316,7 +335,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     auto t0 = std::chrono::steady_clock::now();
     ctx->stats = tp_batch_stats{};
     std::vector<TpIntegralDesc> desc(ntriples);
-    std::vector<tpc::AdaptiveIntegral> I(ntriples);
+    std::vector<tpc::AdaptiveIntegral31> I(ntriples);
     for (size_t t = 0; t < ntriples; ++t) {
         const double *th = theta_rad + 3 * t;
         if (icosmo[t] < 0 || icosmo[t] >= c->ncosmo) return tp_set_error(TP_EINVAL, "tp_map3_batch: bad icosmo");
@@ -332,89 +351,135 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     if ((rc = ctx->d_desc.ensure(sizeof(TpIntegralDesc) * ntriples))) return rc;
     TP_HIP(hipMemcpyAsync(ctx->d_desc.p, desc.data(), sizeof(TpIntegralDesc) * ntriples, hipMemcpyHostToDevice,
                           ctx->stream));
-    std::vector<size_t> active(ntriples), offs;
-    for (size_t t = 0; t < ntriples; ++t) active[t] = t;
+    // Integrals are split into groups that alternate on the stream: while the device
+    // evaluates one group's regions, the host absorbs the other group's results and runs
+    // its heap decisions.  Each integral's decision sequence is unchanged.  A surrogate
+    // round synchronises inside (its fallback test reads the checks back), so it uses one group.
+    const int nl = (mode == TP_MODE_SURROGATE) ? 1 : TP_NLANES;
+    struct Group {
+        std::vector<size_t> active, offs;
+        size_t nreg = 0;
+        long long rounds = 0;
+        bool inflight = false;
+    };
+    Group G[TP_NLANES];
+    for (size_t t = 0; t < ntriples; ++t) G[nl == 1 ? 0 : (int)(t % nl)].active.push_back(t);
+    for (int g = 0; g < nl; ++g) {
+        if ((rc = ctx->lane[g].create())) return rc;
+        // the flag word is only ever raised, and a raised flag ends the batch: clear it once per call
+        TP_HIP(hipMemsetAsync(ctx->lane[g].d_flag.p, 0, sizeof(int), ctx->stream));
+    }
     const int nthreads = std::max(1, std::min(omp_get_max_threads(), 16));
     long long total_regions = 0;
-    while (!active.empty()) {
-        if (ctx->stats.rounds >= kMaxRounds)
+    using clk = std::chrono::steady_clock;
+    auto ms_since = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
+
+    // gathers the group's pending regions and queues their evaluation
+    auto launch = [&](int g) -> int {
+        Group &Gr = G[g];
+        TpLane &L = ctx->lane[g];
+        auto th0 = clk::now();
+        if (Gr.rounds >= kMaxRounds)
             return tp_set_error(TP_ECAPACITY, "tp_map3_batch: round limit reached (integrals not converging)");
-        size_t na = active.size();
-        offs.assign(na + 1, 0);
-        for (size_t a = 0; a < na; ++a) offs[a + 1] = offs[a] + I[active[a]].pending().size();
-        size_t nreg = offs[na];
+        const size_t na = Gr.active.size();
+        Gr.offs.assign(na + 1, 0);
+        for (size_t a = 0; a < na; ++a) Gr.offs[a + 1] = Gr.offs[a] + I[Gr.active[a]].pending().size();
+        const size_t nreg = Gr.offs[na];
+        Gr.nreg = nreg;
         total_regions += (long long)nreg;
         if (total_regions > kMaxRegions)
             return tp_set_error(TP_ECAPACITY, "tp_map3_batch: region limit reached (integrals not converging)");
-        if ((rc = ctx->h_regions.ensure(sizeof(TpRegion) * nreg)) || (rc = ctx->d_regions.ensure(sizeof(TpRegion) * nreg)) ||
-            (rc = ctx->h_rout.ensure(sizeof(TpRegionOut) * nreg)) || (rc = ctx->d_rout.ensure(sizeof(TpRegionOut) * nreg)) ||
-            (rc = ctx->d_fvals.ensure(sizeof(double) * TP_NPTS_GM3 * nreg)))
-            return rc;
-        TpRegion *hr = (TpRegion *)ctx->h_regions.p;
-#pragma omp parallel for schedule(dynamic, 8) num_threads(nthreads) if (na > 64)
+        int e;
+        if ((e = L.h_regions.ensure(sizeof(TpRegion) * nreg)) || (e = L.d_regions.ensure(sizeof(TpRegion) * nreg)) ||
+            (e = L.h_rout.ensure(sizeof(TpRegionOut) * nreg)) || (e = L.d_rout.ensure(sizeof(TpRegionOut) * nreg)) ||
+            (e = L.d_fvals.ensure(sizeof(double) * TP_NPTS_GM3 * nreg)))
+            return e;
+        TpRegion *hr = (TpRegion *)L.h_regions.p;
+        const std::vector<size_t> &act = Gr.active;
+        const std::vector<size_t> &offs = Gr.offs;
+        // a few integrals with many regions (the tail of a batch) are worth threads as well
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) if (na > 1 && nreg > 4096)
         for (size_t a = 0; a < na; ++a) {
-            tpc::AdaptiveIntegral &A = I[active[a]];
+            tpc::AdaptiveIntegral31 &A = I[act[a]];
             const auto &pend = A.pending();
             for (size_t k = 0; k < pend.size(); ++k) {
-                const tpc::Region &R = A.regions()[pend[k]];
+                const auto &R = A.regions()[pend[k]];
                 TpRegion &o = hr[offs[a] + k];
                 for (int d = 0; d < 3; ++d) {
                     o.c[d] = R.c[d];
                     o.hw[d] = R.hw[d];
                 }
                 o.vol = R.vol;
-                o.integral = (int)active[a];
+                o.integral = (int)act[a];
                 o.pad = 0;
             }
         }
-        TP_HIP(hipMemsetAsync(ctx->d_flag.p, 0, sizeof(int), ctx->stream));
-        TP_HIP(hipMemcpyAsync(ctx->d_regions.p, hr, sizeof(TpRegion) * nreg, hipMemcpyHostToDevice, ctx->stream));
-        TP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        ctx->stats.host_ms += ms_since(th0);
+        if (mode == TP_MODE_SURROGATE) TP_HIP(hipMemsetAsync(L.d_flag.p, 0, sizeof(int), ctx->stream));
+        TP_HIP(hipMemcpyAsync(L.d_regions.p, hr, sizeof(TpRegion) * nreg, hipMemcpyHostToDevice, ctx->stream));
+        TP_HIP(hipEventRecord(L.ev0, ctx->stream));
         if (mode == TP_MODE_SURROGATE) {
             int fb = 0;
-            rc = tp_surrogate_round(ctx, c, nreg, (const TpRegion *)ctx->d_regions.p,
-                                    (const TpIntegralDesc *)ctx->d_desc.p, (double *)ctx->d_fvals.p,
-                                    (TpRegionOut *)ctx->d_rout.p, (int *)ctx->d_flag.p, &fb);
-            if (rc) return rc;
+            e = tp_surrogate_round(ctx, c, nreg, (const TpRegion *)L.d_regions.p, (const TpIntegralDesc *)ctx->d_desc.p,
+                                   (double *)L.d_fvals.p, (TpRegionOut *)L.d_rout.p, (int *)L.d_flag.p, &fb);
+            if (e) return e;
             ctx->stats.fallback_batches += (fb == 1);
         } else {
-            rc = tp_launch_regions(ctx, c, nreg, (const TpRegion *)ctx->d_regions.p,
-                                   (const TpIntegralDesc *)ctx->d_desc.p, (double *)ctx->d_fvals.p,
-                                   (TpRegionOut *)ctx->d_rout.p, mode, (int *)ctx->d_flag.p);
-            if (rc) return rc;
+            e = tp_launch_regions(ctx, c, nreg, (const TpRegion *)L.d_regions.p, (const TpIntegralDesc *)ctx->d_desc.p,
+                                  (double *)L.d_fvals.p, (TpRegionOut *)L.d_rout.p, mode, (int *)L.d_flag.p, L.evm);
+            if (e) return e;
         }
-        TP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-        TP_HIP(hipMemcpyAsync(ctx->h_rout.p, ctx->d_rout.p, sizeof(TpRegionOut) * nreg, hipMemcpyDeviceToHost,
-                              ctx->stream));
-        int flag = 0;
-        if ((rc = read_flag(ctx, &flag))) return rc;
-        float ms = 0, msp = 0;
-        hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-        if (mode != TP_MODE_SURROGATE) hipEventElapsedTime(&msp, ctx->ev0, ctx->evm);
-        ctx->stats.kernel_ms += ms;
-        ctx->stats.integrand_ms += msp;
+        TP_HIP(hipEventRecord(L.ev1, ctx->stream));
+        TP_HIP(hipMemcpyAsync(L.h_rout.p, L.d_rout.p, sizeof(TpRegionOut) * nreg, hipMemcpyDeviceToHost, ctx->stream));
+        if (mode == TP_MODE_SURROGATE)
+            TP_HIP(hipMemcpyAsync(L.h_flag.p, L.d_flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        TP_HIP(hipEventRecord(L.done, ctx->stream));
+        Gr.inflight = true;
+        Gr.rounds += 1;
         ctx->stats.launches += 1;
-        ctx->stats.rounds += 1;
         ctx->stats.regions += (long long)nreg;
         ctx->stats.points += (long long)nreg * TP_NPTS_GM3;
-        if (flag) return tp_set_error(TP_ENONFINITE, "tp_map3_batch: non-finite ell or B_kappa value");
-        const TpRegionOut *ro = (const TpRegionOut *)ctx->h_rout.p;
+        if (nl == 1) ctx->stats.rounds += 1;  // the surrogate's check sampling is keyed on it
+        return 0;
+    };
+
+    // waits for the group's evaluation, absorbs it and selects the next regions
+    auto complete = [&](int g) -> int {
+        Group &Gr = G[g];
+        TpLane &L = ctx->lane[g];
+        auto tw = clk::now();
+        TP_HIP(hipEventSynchronize(L.done));
+        ctx->stats.wait_ms += ms_since(tw);
+        Gr.inflight = false;
+        float ms = 0, msp = 0;
+        TP_HIP(hipEventElapsedTime(&ms, L.ev0, L.ev1));
+        if (mode != TP_MODE_SURROGATE) TP_HIP(hipEventElapsedTime(&msp, L.ev0, L.evm));
+        ctx->stats.kernel_ms += ms;
+        ctx->stats.integrand_ms += msp;
+        const TpRegionOut *ro = (const TpRegionOut *)L.h_rout.p;
+        // the point kernels' flag word: in region 0's flag (exact rounds) or copied back (surrogate rounds)
+        const int pflag = mode == TP_MODE_SURROGATE ? *(const int *)L.h_flag.p : (Gr.nreg ? ro[0].flag >> 1 : 0);
+        if (pflag) return tp_set_error(TP_ENONFINITE, "tp_map3_batch: non-finite ell or B_kappa value");
+        auto th0 = clk::now();
+        const std::vector<size_t> &act = Gr.active;
+        const std::vector<size_t> &offs = Gr.offs;
+        const size_t na = act.size();
         std::vector<char> finished(na, 0);
         int bad = 0;
-#pragma omp parallel for schedule(dynamic, 8) num_threads(nthreads) if (na > 64) reduction(| : bad)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) if (na > 1 && Gr.nreg > 1024) reduction(| : bad)
         for (size_t a = 0; a < na; ++a) {
-            tpc::AdaptiveIntegral &A = I[active[a]];
+            tpc::AdaptiveIntegral31 &A = I[act[a]];
             const auto &pend = A.pending();
             for (size_t k = 0; k < pend.size(); ++k) {
                 const TpRegionOut &o = ro[offs[a] + k];
-                tpc::Region &R = A.regions()[pend[k]];
+                auto &R = A.regions()[pend[k]];
                 R.val[0] = o.val;
                 R.err[0] = o.err;
                 R.corr = o.corr;
                 R.cvar = o.cvar;
                 R.split = (unsigned)o.split;
                 R.errmax = o.err;
-                bad |= o.flag;
+                bad |= o.flag & 1;
             }
             A.absorb();
             finished[a] = A.advance() ? 1 : 0;
@@ -423,8 +488,33 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         std::vector<size_t> next;
         next.reserve(na);
         for (size_t a = 0; a < na; ++a)
-            if (!finished[a]) next.push_back(active[a]);
-        active.swap(next);
+            if (!finished[a]) next.push_back(act[a]);
+        Gr.active.swap(next);
+        ctx->stats.host_ms += ms_since(th0);
+        return 0;
+    };
+
+    // on an error, let queued work drain before the buffers can be reused
+    auto fail = [&](int e) {
+        (void)hipStreamSynchronize(ctx->stream);
+        return e;
+    };
+    for (int g = 0; g < nl; ++g)
+        if (!G[g].active.empty() && (rc = launch(g))) return fail(rc);
+    for (;;) {
+        bool any = false;
+        for (int g = 0; g < nl; ++g) {
+            if (!G[g].inflight) continue;
+            any = true;
+            if ((rc = complete(g))) return fail(rc);
+            if (!G[g].active.empty() && (rc = launch(g))) return fail(rc);
+        }
+        if (!any) break;
+    }
+    if (nl > 1) {
+        long long r = 0;
+        for (int g = 0; g < nl; ++g) r = std::max(r, G[g].rounds);
+        ctx->stats.rounds = r;
     }
     std::vector<size_t> redo;
     for (size_t t = 0; t < ntriples; ++t) {
@@ -461,6 +551,8 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         keep.launches += ctx->stats.launches;
         keep.kernel_ms += ctx->stats.kernel_ms;
         keep.integrand_ms += ctx->stats.integrand_ms;
+        keep.host_ms += ctx->stats.host_ms;
+        keep.wait_ms += ctx->stats.wait_ms;
         keep.fallback_integrals += (long long)redo.size();
         ctx->stats = keep;
     }
