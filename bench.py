@@ -32,6 +32,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
+# HBM bytes per integrand point of the region kernels: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the
+# C1 batched paths, each counter scaled by its reading for k_copy16 of 256 MiB (FETCH_SIZE x2.000,
+# WRITE_SIZE x1.000).  Algorithmic: 64/33 B of region in + 8 B of integrand value out.
+HBM_BYTES_PER_POINT = {"k_region_points<double>": 12.93, "k_region_points<float>": 11.63}
 FLOPS_PER_ELEM = 20.0
 
 
@@ -133,6 +137,107 @@ def accuracy_study(tabs, ctx, thetas, icosmo, paths, reltol):
                                   "n_over_1e-3": int((err > 1e-3).sum())}), flush=True)
 
 
+def c2_sample(W, n):
+    """n of the 1000 triangles of the 10x10x10 TreeCorr grid: all when n >= 1000, else spread over the
+    r bins, each with its own u and v bin."""
+    tri = W.gamma_c2_triangles()
+    if n >= tri.shape[0]:
+        return np.arange(tri.shape[0]), tri
+    sel = [100 * (i * 10 // n) + 10 * ((7 * i + 4) % 10) + (3 * i + 5) % 10 for i in range(n)]
+    return np.array(sel), tri
+
+
+def gamma_bench(tp, W, ctx, args):
+    """Gamma_0..3 of TreeCorr-grid triangles, Ogata h = 1e-3 with 4999 Bessel-zero nodes, rel tol 1e-3."""
+    t0 = time.perf_counter()
+    tabs = tp.CosmologyTables([W.fiducial()], W.fiducial_nz(), 100, 3.0, ctx)
+    tp.gamma_setup(1e-3, 5.0, ctx)
+    setup_ms = 1e3 * (time.perf_counter() - t0)
+    prec_k = 5000
+    sel, tri = c2_sample(W, args.c2_triangles)
+    x = tri[sel, :3]
+    fe_point = tp.flop_equivalents_per_gamma_point(prec_k, FLOPS_PER_ELEM)
+    results, values = {}, {}
+    for p in [q for q in args.paths.split(",") if q in ("plain_fp64", "batched_fp64")]:
+        print("bench: gamma path %s ..." % p, file=sys.stderr, flush=True)
+        steps = 1
+        agg = {"points": 0, "integrand_ms": 0.0, "launches": 0, "rounds": 0, "host_ms": 0.0}
+        tt = time.perf_counter()
+        for _ in range(steps):
+            if p == "batched_fp64":
+                g, ne = tabs.gamma_batch(x, 1e-3)
+                st = ctx.stats()
+                for k in agg:
+                    agg[k] += st.get(k, 0)
+            else:  # one triangle (its four integrals) per call
+                out = []
+                for i in range(x.shape[0]):
+                    gi, ne = tabs.gamma_batch(x[i:i + 1], 1e-3)
+                    out.append(gi[0])
+                    st = ctx.stats()
+                    for k in agg:
+                        agg[k] += st.get(k, 0)
+                g = np.array(out)
+        dt = time.perf_counter() - tt
+        agg["ms_per_step"] = 1e3 * dt / steps
+        agg["triangles_per_s"] = x.shape[0] * steps / dt
+        agg["points_per_triangle"] = agg["points"] / (steps * x.shape[0])
+        if agg["integrand_ms"] > 0:
+            ach = fe_point * agg["points"] / (agg["integrand_ms"] * 1e-3)
+            agg["integrand_kernel"] = {"achieved_flops": ach, "peak": PEAK["fp64"], "frac": ach / PEAK["fp64"],
+                                       "avg_launch_ms": agg["integrand_ms"] / max(1, agg["launches"])}
+        results[p], values[p] = agg, g
+    if "plain_fp64" in values and "batched_fp64" in values:
+        a, b = values["batched_fp64"], values["plain_fp64"]
+        results["plain_fp64"]["max_abs_diff_vs_batched"] = float(np.max(np.abs(a - b)) / np.max(np.abs(a)))
+    head = "batched_fp64" if "batched_fp64" in results else next(iter(results))
+    hr = results[head]
+    dom = hr["integrand_kernel"]
+    line = {
+        "metric": "Gamma_i triangles/s on 1 GPU (4 complex natural components per triangle)",
+        "value": hr["triangles_per_s"], "unit": "triangles/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": hr["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "Gamma_0..3 of %d of the 1000 TreeCorr-grid triangles (r 0.1-100 arcmin log x 10, "
+                               "u 0-1 x 10, v -1-1 x 10, centroid), fiducial flat LCDM, Ogata h 1e-3 with 4999 "
+                               "zeros of J6 and J2, hcubature (z, phi, psi) rel tol 1e-3" % x.shape[0],
+                   "triangles": [int(i) for i in sel], "path": head, "setup_ms": setup_ms},
+        "paths": results,
+        "roofline": {"bound": "valu", "kernel": "k_gamma_points", "achieved": dom["achieved_flops"] / 1e12,
+                     "peak": dom["peak"] / 1e12, "unit": "TFLOP/s", "frac": dom["frac"], "traffic": None,
+                     "work_per_point_flop_eq": fe_point, "flops_per_elementary_call": FLOPS_PER_ELEM,
+                     "census": tp.census_gamma_term()},
+    }
+    if not args.no_cpu:
+        print("bench: cpu baseline ...", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = gamma_cpu_baseline(W, x, hr["points_per_triangle"], args.cpu_seconds)
+    print(json.dumps(line))
+
+
+def gamma_cpu_baseline(W, x, points_per_triangle, seconds_target):
+    """The FP64 CPU oracle's Gamma integrands (OpenMP over points) on a bounded sample of cubature
+    points; triangles/s follows from the GPU run's integrand points per triangle."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _oracle import Oracle
+    cores = min(os.cpu_count() or 1, 64)
+    o = Oracle(W.fiducial(), W.fiducial_nz())
+    o.gamma_init(1e-3, 5.0)
+    rng = np.random.default_rng(0xC2)
+    done, t0, n = 0, time.perf_counter(), 2 * cores
+    while time.perf_counter() - t0 < seconds_target:
+        pts = np.column_stack([rng.uniform(0, 3.0, n), rng.uniform(0, 2 * np.pi, n), rng.uniform(0, np.pi / 2, n)])
+        for which in (0, 1):  # the Gamma_0 and the Gamma_1..3 integrands, in the 1:3 mix of a triangle
+            o.gamma_integrand_batch(which, x[0], pts[: n // 3] if which == 0 else pts, cores)
+        done += n // 3 + n
+    dt = time.perf_counter() - t0
+    pps = done / dt
+    return {"value": pps / points_per_triangle, "unit": "triangles/s", "cores": cores, "kind": "port",
+            "points_per_s": pps,
+            "sample": "%d Gamma integrand points (uniform in (z, phi, psi), Gamma_0:Gamma_1 integrands 1:3) through "
+                      "the FP64 CPU oracle, OpenMP %d threads, %s, %.1f s; triangles/s = points/s / %.0f points "
+                      "per triangle of the GPU run" % (done, cores, cpu_model(), dt, points_per_triangle)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,8 +247,13 @@ def main():
     ap.add_argument("--paths", default="plain_fp64,batched_fp64,batched_fp32,surrogate")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="c1", choices=["c1", "c3"])
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3"])
+    ap.add_argument("--c2-triangles", type=int, default=10,
+                    help="Gamma workload: number of the 1000 TreeCorr-grid triangles (one per r bin at 10)")
     ap.add_argument("--surrogate", default=os.path.join(ROOT, "threepoint_amd", "models", "bkappa_mlp.bin"))
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first copy this many MiB device-to-device (a kernel of known traffic for "
+                         "calibrating HBM byte counters)")
     ap.add_argument("--accuracy-study", action="store_true",
                     help="compare every path at the tolerance against FP64 at a 10x tighter tolerance, then exit")
     args = ap.parse_args()
@@ -156,6 +266,16 @@ def main():
     from threepoint_amd import workloads as W
 
     ctx = tp.default_context()
+    if args.copy_calibration_mib > 0:
+        import ctypes
+        gb = ctypes.c_double(0)
+        tp._check(tp.lib().tp_hbm_copy_probe(ctx.h, args.copy_calibration_mib << 20, ctypes.byref(gb)),
+                  "tp_hbm_copy_probe")
+        print("bench: copied %d MiB device-to-device (k_copy16) at %.0f GB/s" % (args.copy_calibration_mib, gb.value),
+              file=sys.stderr, flush=True)
+    if args.workload == "c2":
+        gamma_bench(tp, W, ctx, args)
+        return
     t0 = time.perf_counter()
     icosmo = None
     if args.workload == "c1":
@@ -226,9 +346,13 @@ def main():
     head = max(ok, key=lambda p: results[p]["triples_per_s"]) if ok else "batched_fp64"
     hr = results[head]
     dom = hr.get("integrand_kernel") or next(r["integrand_kernel"] for r in results.values() if "integrand_kernel" in r)
-    roof = {"bound": "valu", "kernel": "k_region_points<%s>" % ("float" if head == "batched_fp32" else "double"),
+    kname = "k_region_points<%s>" % ("float" if head == "batched_fp32" else "double")
+    pts_per_launch = hr["points"] / max(1, hr["launches"])
+    roof = {"bound": "valu", "kernel": kname,
             "achieved": dom["achieved_flops"] / 1e12, "peak": dom["peak"] / 1e12, "unit": "TFLOP/s",
-            "frac": dom["frac"], "traffic": None,
+            "frac": dom["frac"], "traffic": HBM_BYTES_PER_POINT[kname] * pts_per_launch,
+            "traffic_unit": "HBM bytes per launch (per-point bytes from the PMC passes x points per launch)",
+            "algorithmic_bytes_per_point": 64 / 33 + 8,
             "work_per_point_flop_eq": fe_point, "flops_per_elementary_call": FLOPS_PER_ELEM,
             "census": census}
     line = {

@@ -125,6 +125,12 @@ typedef struct {
 } tp_batch_stats;
 int tp_last_batch_stats(const tp_ctx *ctx, tp_batch_stats *out);
 
+/* A plain device-to-device copy of nbytes (a multiple of 16) by a vector copy kernel: it reads
+ * nbytes and writes nbytes of HBM, so hardware byte counters can be calibrated against it.
+ * gbps receives the copy's (read + write) rate.  No counterpart in the reference: a
+ * measurement aid for the kernels' HBM traffic. */
+int tp_hbm_copy_probe(tp_ctx *ctx, size_t nbytes, double *gbps);
+
 /* Algorithmic operation census of one <Map^3> integrand point of the FP64/FP32 kernels:
  * out[0] flops per z-node, out[1] exp/log calls per z-node, out[2] flops and out[3]
  * elementary calls per point outside the node loop, out[4] z-nodes per point.  Counted by

@@ -1340,6 +1340,13 @@ int orc_gamma_integrand(void *h, int which, double z, double phi, double psi, do
     return 0;
 }
 
+int orc_gamma_integrand_batch(void *h, int which, double x1, double x2, double x3, size_t npts, const double *x,
+                              double *fval, int nthreads)
+{ /* the integrand_v adapters of CV/gamma.cpp:338-359, 522-543 on npts points (z, phi, psi) */
+    GammaCtx g{(Calc *)h, which == 0 ? 0 : 1, x1, x2, x3, nthreads_or_default(nthreads)};
+    return gamma_cb(3, npts, x, &g, 2, fval);
+}
+
 int orc_gamma(void *h, int which, double x1, double x2, double x3, double reltol, int nthreads, double *re,
               double *im, long long *neval)
 { /* CV/gamma.cpp:27-81 (gamma0..3), 361-377 / 545-563 (cubature + Ogata), centroid rotation 624-643 */

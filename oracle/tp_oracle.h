@@ -72,6 +72,8 @@ double orc_bessel_zero_jn(int n, int s);
 int orc_gamma_init(void *h, double prec_h, double prec_k_arg);
 int orc_gamma_integrand(void *h, int which, double z, double phi, double psi,
                         double x1, double x2, double x3, double *re, double *im);
+int orc_gamma_integrand_batch(void *h, int which, double x1, double x2, double x3, size_t npts,
+                              const double *x, double *fval, int nthreads);
 int orc_gamma(void *h, int which, double x1, double x2, double x3, double reltol,
               int nthreads, double *re, double *im, long long *neval);
 void orc_ggg_analytic(double r1, double r2, double r3, double *re, double *im);

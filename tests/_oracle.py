@@ -43,6 +43,8 @@ _SIGS = {
     "orc_bessel_zero_jn": (ctypes.c_double, [ctypes.c_int, ctypes.c_int]),
     "orc_gamma_init": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
     "orc_gamma_integrand": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_double] * 6 + [_dp, _dp]),
+    "orc_gamma_integrand_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_double] * 3 +
+                                  [ctypes.c_size_t, _dp, _dp, ctypes.c_int]),
     "orc_gamma": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_double] * 4 + [ctypes.c_int, _dp, _dp,
                                                                                              _llp]),
     "orc_ggg_analytic": (None, [ctypes.c_double] * 3 + [_dp, _dp]),
@@ -121,6 +123,13 @@ class Oracle:
         re, im = ctypes.c_double(), ctypes.c_double()
         lib().orc_gamma_integrand(self.h, which, z, phi, psi, x1, x2, x3, ctypes.byref(re), ctypes.byref(im))
         return re.value + 1j * im.value
+
+    def gamma_integrand_batch(self, which, x123, pts, nthreads=0):
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros((pts.shape[0], 2))
+        lib().orc_gamma_integrand_batch(self.h, which, x123[0], x123[1], x123[2], pts.shape[0], d(pts), d(out),
+                                        nthreads)
+        return out[:, 0] + 1j * out[:, 1]
 
     def gamma(self, which, x1, x2, x3, reltol=1e-3, nthreads=0):
         re, im, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()

@@ -85,3 +85,69 @@ def test_corrupted_model_rejected(gpu_ctx, tmp_path):
     fn = tmp_path / "bad.bin"
     fn.write_bytes(b"TPMLP002" + b"\0" * 100)
     assert tp.lib().tp_surrogate_load(gpu_ctx.h, str(fn).encode()) != 0
+
+
+MODEL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "threepoint_amd", "models",
+                     "bkappa_mlp.bin")
+
+
+def _load_model(ctx):
+    tp._check(tp.lib().tp_surrogate_load(ctx.h, MODEL.encode()), "tp_surrogate_load")
+
+
+def _random_triangles(n, seed):
+    rng = np.random.default_rng(seed)
+    l1 = np.exp(rng.uniform(3.0, 10.0, 4 * n))
+    l2 = np.exp(rng.uniform(3.0, 10.0, 4 * n))
+    phi = rng.uniform(0.0, np.pi, 4 * n)
+    l3 = np.sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * np.cos(phi))
+    keep = (l3 > np.exp(2.2)) & (l3 < np.exp(10.8))
+    return np.column_stack([l1, l2, l3])[keep][:n]
+
+
+def test_trained_network_gain_over_baseline(gpu_ctx, fid_tables):
+    """What the network adds: the surrogate is B_8 exp(net), B_8 the 8-node FP32 Limber baseline.
+    Against the exact FP64 B_kappa the trained network must cut the baseline's log error tenfold."""
+    _load_model(gpu_ctx)
+    ell = _random_triangles(4096, 0xB0)
+    exact = fid_tables.bkappa(ell, 0, tp.MODE_FP64)
+    sur = fid_tables.surrogate_bkappa(ell)
+    c = W.fiducial()
+    in8 = np.column_stack([ell, np.tile([c.om, c.sigma8, c.h, c.w, c.ns], (ell.shape[0], 1))])
+    y = np.zeros(ell.shape[0])
+    tp._check(tp.lib().tp_surrogate_predict(gpu_ctx.h, ell.shape[0], tp._d(np.ascontiguousarray(in8)), tp._d(y)),
+              "tp_surrogate_predict")
+    base = sur / np.exp(y)
+    e_sur = np.log(sur / exact)
+    e_base = np.log(base / exact)
+    rms_sur, rms_base = np.sqrt(np.mean(e_sur ** 2)), np.sqrt(np.mean(e_base ** 2))
+    print("log-error rms: trained network %.2e, baseline alone %.2e; max %.2e vs %.2e" %
+          (rms_sur, rms_base, np.abs(e_sur).max(), np.abs(e_base).max()))
+    assert rms_sur < 5e-3
+    assert rms_sur * 10 < rms_base
+
+
+def test_surrogate_mode_gain_in_map3(gpu_ctx, fid_tables):
+    """<Map^3> of the 20 C0 triples: with the trained network most points are served by it and no
+    round falls back; with an untrained network of the same shape the checks reject it and the
+    rounds fall back to exact B_kappa.  Both stay within 1e-3 of the FP64 path."""
+    th = W.to_rad(W.triples(W.map3_c0_radii()))
+    ref, _, _ = fid_tables.map3_batch(th, None, 1e-4, tp.MODE_FP64)
+    nz = ref != 0
+    _load_model(gpu_ctx)
+    v, _, _ = fid_tables.map3_batch(th, None, 1e-4, tp.MODE_SURROGATE)
+    st = gpu_ctx.stats()
+    frac = st["surrogate_points"] / st["points"]
+    err = np.abs(v[nz] / ref[nz] - 1).max()
+    print("trained: surrogate share %.3f, fallback rounds %d, max rel err %.2e" % (frac, st["fallback_batches"], err))
+    assert frac > 0.8 and st["fallback_batches"] == 0 and err < 1e-3
+    rng = np.random.default_rng(7)
+    p = (rng.standard_normal(MO.NPARAM) * 0.05).astype(np.float32)
+    tp._check(tp.lib().tp_mlp_set_params(gpu_ctx.h, p.ctypes.data_as(_fp)), "set")
+    v2, _, _ = fid_tables.map3_batch(th, None, 1e-4, tp.MODE_SURROGATE)
+    st2 = gpu_ctx.stats()
+    frac2 = st2["surrogate_points"] / st2["points"]
+    err2 = np.abs(v2[nz] / ref[nz] - 1).max()
+    print("untrained: surrogate share %.3f, fallback rounds %d, max rel err %.2e" % (frac2, st2["fallback_batches"], err2))
+    _load_model(gpu_ctx)
+    assert frac2 < 0.2 and st2["fallback_batches"] > 0 and err2 < 1e-3

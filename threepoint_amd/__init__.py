@@ -89,6 +89,7 @@ _SIGS = {
     "tp_hcubature_last_neval": (ctypes.c_longlong, []),
     "tp_census_map3_point": (ctypes.c_int, [_dp]),
     "tp_census_gamma_term": (ctypes.c_int, [_dp]),
+    "tp_hbm_copy_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _dp]),
     "tp_surrogate_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(tp_train_config),
                                           ctypes.POINTER(tp_train_report)]),
     "tp_surrogate_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),

@@ -135,6 +135,42 @@ extern "C" int tp_ctx_destroy(tp_ctx *ctx)
     return 0;
 }
 
+namespace {
+__global__ void __launch_bounds__(256) k_copy16(const uint4 *__restrict__ a, uint4 *__restrict__ b, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+}  // namespace
+
+extern "C" int tp_hbm_copy_probe(tp_ctx *ctx, size_t nbytes, double *gbps)
+{
+    if (!ctx || nbytes < 16 || nbytes % 16) return tp_set_error(TP_EINVAL, "tp_hbm_copy_probe: nbytes must be a positive multiple of 16");
+    TP_HIP(hipSetDevice(ctx->device));
+    void *a = nullptr, *b = nullptr;
+    TP_HIP(hipMalloc(&a, nbytes));
+    hipError_t e = hipMalloc(&b, nbytes);
+    if (e != hipSuccess) {
+        (void)hipFree(a);
+        return tp_set_error(TP_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    const size_t n = nbytes / 16;
+    const int blocks = 256 * 16;  // 16 workgroups per CU, grid-stride
+    (void)hipMemsetAsync(a, 1, nbytes, ctx->stream);
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, ctx->stream, (const uint4 *)a, (uint4 *)b, n);  // warm
+    (void)hipEventRecord(ctx->ev0, ctx->stream);
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, ctx->stream, (const uint4 *)a, (uint4 *)b, n);
+    (void)hipEventRecord(ctx->ev1, ctx->stream);
+    e = hipStreamSynchronize(ctx->stream);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    if (e != hipSuccess) return tp_set_error(TP_EHIP, std::string("tp_hbm_copy_probe: ") + hipGetErrorString(e));
+    if (gbps) *gbps = ms > 0 ? 2.0 * (double)nbytes / (ms * 1e-3) / 1e9 : 0.0;
+    return 0;
+}
+
 extern "C" int tp_last_batch_stats(const tp_ctx *ctx, tp_batch_stats *out)
 {
     if (!ctx || !out) return tp_set_error(TP_EINVAL, "tp_last_batch_stats: NULL");
