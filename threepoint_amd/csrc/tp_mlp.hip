@@ -42,6 +42,7 @@ namespace {
 constexpr int H = 256;   // hidden width
 constexpr int NIN = 8;   // inputs
 constexpr int NLAYER_MFMA = 3;
+constexpr int NFRAG = 3 * 16 * 8 * 64;  // 16-byte fragments of the hidden weights in k_mlp_reg's order
 
 typedef __attribute__((__vector_size__(4 * sizeof(short)))) short bf16x4;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
@@ -57,6 +58,8 @@ __device__ __forceinline__ unsigned short f2bf(float f)
 __device__ __forceinline__ float bf2f(unsigned short b) { return __uint_as_float(((unsigned int)b) << 16); }
 
 __device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// inference form: hardware reciprocal (1 ulp) instead of the IEEE division sequence
+__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ float silu_grad(float z)
 {
     float s = 1.0f / (1.0f + __expf(-z));
@@ -391,6 +394,7 @@ int grid(size_t n, int b) { return (int)((n + b - 1) / b); }
 struct TpSurrogate {
     float *P = nullptr, *G = nullptr, *Am = nullptr, *Av = nullptr;
     unsigned short *Wb = nullptr;  // [3][H][H]
+    uint4 *Wf = nullptr;           // [NFRAG] hidden weights, BF16, in k_mlp_reg's fragment order
     // activations / gradients for a batch of capacity cap rows
     int cap = 0;
     float *X = nullptr, *T = nullptr, *Y = nullptr, *dy = nullptr;
@@ -413,6 +417,7 @@ static int sur_alloc(TpSurrogate *s)
     TP_HIP(hipMalloc(&s->Am, NPARAM * sizeof(float)));
     TP_HIP(hipMalloc(&s->Av, NPARAM * sizeof(float)));
     TP_HIP(hipMalloc(&s->Wb, 3 * H * H * sizeof(unsigned short)));
+    TP_HIP(hipMalloc(&s->Wf, NFRAG * sizeof(uint4)));
     TP_HIP(hipMalloc(&s->d_loss, sizeof(double)));
     TP_HIP(hipMemset(s->G, 0, NPARAM * sizeof(float)));
     TP_HIP(hipMemset(s->Am, 0, NPARAM * sizeof(float)));
@@ -446,7 +451,7 @@ static void sur_free(TpSurrogate *s)
 {
     if (!s) return;
     auto F = [](void *p) { if (p) (void)hipFree(p); };
-    F(s->P); F(s->G); F(s->Am); F(s->Av); F(s->Wb); F(s->d_loss);
+    F(s->P); F(s->G); F(s->Am); F(s->Av); F(s->Wb); F(s->Wf); F(s->d_loss);
     F(s->X); F(s->T); F(s->Y); F(s->dy); F(s->parts);
     for (int i = 0; i < 4; ++i) { F(s->Z[i]); F(s->A[i]); F(s->dZ[i]); }
     delete s;
@@ -545,9 +550,17 @@ static int sur_backward(hipStream_t st, TpSurrogate *s, int M)
     return 0;
 }
 
-static int sur_refresh_bf16(hipStream_t st, TpSurrogate *s)
+namespace {
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ void k_weights_frag(const float *__restrict__ P, u32x4 *__restrict__ Wf);
+}
+// BF16 copies of the hidden weights: Wb for the training GEMMs and, unless only training
+// continues, Wf for the inference kernel
+static int sur_refresh_bf16(hipStream_t st, TpSurrogate *s, bool for_inference = true)
 {
     hipLaunchKernelGGL(k_weights_bf16, dim3(grid((size_t)3 * H * H, 256)), dim3(256), 0, st, s->P, s->Wb);
+    if (for_inference)
+        hipLaunchKernelGGL(k_weights_frag, dim3(grid((size_t)NFRAG, 256)), dim3(256), 0, st, s->P, (u32x4 *)s->Wf);
     TP_HIP(hipGetLastError());
     return 0;
 }
@@ -634,132 +647,224 @@ extern "C" int tp_mlp_loss_grad(tp_ctx *ctx, size_t n, const float *x, const flo
     return 0;
 }
 
-// ================================================================ inference-only forward
-namespace {
-
-// forward GEMM epilogue without pre-activation storage: A_out = bf16(silu(acc + bias))
-__global__ void __launch_bounds__(256) k_layer1_inf(const float *X, int M, const float *P, unsigned short *A1)
-{
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)M * H) return;
-    int m = (int)(t / H), n = (int)(t % H);
-    const float *w = P + OFF_W1 + n * NIN;
-    const float *x = X + (size_t)m * NIN;
-    float z = P[OFF_B1 + n];
-#pragma unroll
-    for (int k = 0; k < NIN; ++k) z += w[k] * x[k];
-    A1[t] = f2bf(silu(z));
-}
-
-__global__ void __launch_bounds__(256) k_out_inf(const unsigned short *A4, int M, const float *P, float mean, float sd,
-                                                 double *logb)
-{
-    int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
-    if (wave >= M) return;
-    const unsigned short *a = A4 + (size_t)wave * H;
-    float s = 0.f;
-    for (int k = lane; k < H; k += 64) s += bf2f(a[k]) * P[OFF_W5 + k];
-    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane == 0) logb[wave] = (double)(s + P[OFF_B5]) * (double)sd + (double)mean;
-}
-
-}  // namespace
-
-// ---------------------------------------------------------------- fused inference
-// One workgroup (4 waves) carries 64 points through all layers.  Hidden activations stay in LDS
-// as hi + lo BF16 pairs (a = hi + lo to ~16 significant bits), so the surrogate is a smooth
-// function of its inputs at the 1e-6 level -- a single BF16 rounding of the activations
-// would leave a ~1e-3 pointwise staircase that an adaptive cubature cannot converge through.
-// Layers 2-4: v_mfma_f32_32x32x16_bf16, A = activations (hi and lo, two MFMAs into the same
-// FP32 accumulator), B = BF16 weights read from L2 as 16-byte vectors; wave w owns output
-// columns [64w, 64w+64) for all 64 rows (2 x 2 tiles of 32 x 32).
 namespace {
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
-typedef __attribute__((__vector_size__(8 * sizeof(unsigned short)))) unsigned short u16x8;
-constexpr int FR = 64, FPAD = 8, FLD = H + FPAD;
 
-__global__ void __launch_bounds__(256) k_mlp_fused(const float *__restrict__ X, int M, const float *__restrict__ P,
-                                                   const unsigned short *__restrict__ Wb, float mean, float sd,
-                                                   double *__restrict__ logb)
+// ---------------------------------------------------------------- register-resident inference
+// The network (8 -> 4 x 256 SiLU -> 1) evaluated for the surrogate, one persistent workgroup per
+// CU, 128 points per chunk (32 per wave).  The layers are computed transposed, Z^T = W A^T: the
+// MFMA's A operand is a 32-unit tile of W and its B operand the activations of 32 points, so the
+// 32 x 32 FP32 result of one layer lands in the lane layout the next layer's B operand needs
+// (lane = (point r, half h), accumulator element 4 j + i = unit 8 j + 4 h + i of the tile).
+// Feeding 8 consecutive accumulator elements as one 16-wide K slice pairs K index k with unit
+// u(k) = k with bits 2 and 3 swapped, so the weights are stored with their input columns in that
+// order.  Activations never leave the registers: hidden activations enter the BF16 MFMAs as
+// hi + lo pairs (a = hi + lo to ~16 significant bits, two MFMAs per weight fragment), which keeps
+// the surrogate a smooth function of its inputs -- one BF16 rounding of the activations leaves a
+// ~1e-3 staircase that an adaptive cubature cannot converge through.  Layer 1 (8 inputs) uses the
+// FP32 MFMA on unrounded inputs.  The weights stream through two 32 KB LDS slots, four K-blocks
+// (16 inputs x 256 units each) per slot, loaded straight into LDS one stage ahead.  Two
+// accumulator sets alternate between layers so that the previous layer's bias/SiLU/split work
+// can issue between the current layer's MFMAs.
+constexpr int RB = 128;     // points per chunk: 4 waves x 32
+constexpr int KPS = 4;      // K-blocks per LDS stage (32 KB)
+constexpr int NSTAGE = 12;  // stages per chunk: 3 layers x 16 K-blocks / KPS
+constexpr int SU4 = KPS * 512;  // 16-byte fragments per stage
+constexpr int SP_B1 = H * NIN, SP_W5 = SP_B1 + 4 * H, SP_N = SP_W5 + H;  // LDS copy: W1, b1..b4, W5
+
+__global__ void k_weights_frag(const float *__restrict__ P, u32x4 *__restrict__ Wf)
 {
-    __shared__ __attribute__((aligned(16))) unsigned short sHi[FR][FLD];
-    __shared__ __attribute__((aligned(16))) unsigned short sLo[FR][FLD];
-    __shared__ float sX[FR][NIN];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int row0 = blockIdx.x * FR;
-    for (int e = tid; e < FR * NIN; e += 256) sX[e / NIN][e % NIN] = X[(size_t)row0 * NIN + e];
-    __syncthreads();
-    {   // layer 1 (8 -> 256), FP32: thread = output unit
-        const int n = tid;
-        float w[NIN];
-        for (int k = 0; k < NIN; ++k) w[k] = P[OFF_W1 + n * NIN + k];
-        const float b = P[OFF_B1 + n];
-        for (int r = 0; r < FR; ++r) {
-            float z = b;
-#pragma unroll
-            for (int k = 0; k < NIN; ++k) z += w[k] * sX[r][k];
-            float a = silu(z);
-            unsigned short hi = f2bf(a);
-            sHi[r][n] = hi;
-            sLo[r][n] = f2bf(a - bf2f(hi));
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= NFRAG) return;
+    const int lane = e & 63, t = (e >> 6) & 7, kb = (e >> 9) & 15, l = e >> 13;
+    const int r = lane & 31, h = lane >> 5;
+    const float *W = P + OFF_W[l];
+    unsigned int w[4];
+    for (int q2 = 0; q2 < 4; ++q2) {
+        unsigned int v[2];
+        for (int m = 0; m < 2; ++m) {
+            const int k = 16 * kb + 8 * h + 2 * q2 + m;
+            const int u = (k & ~12) | ((k & 4) << 1) | ((k & 8) >> 1);
+            v[m] = f2bf(W[(size_t)(32 * t + r) * H + u]);
         }
+        w[q2] = v[0] | (v[1] << 16);
     }
-    __syncthreads();
-    const int r32 = lane & 31, kh = 8 * (lane >> 5);
-    for (int l = 0; l < 3; ++l) {
-        const unsigned short *W = Wb + (size_t)l * H * H;
-        f32x16 acc[2][2];
-        for (int a = 0; a < 2; ++a)
-            for (int b = 0; b < 2; ++b)
-                for (int i = 0; i < 16; ++i) acc[a][b][i] = 0.f;
-#pragma unroll 2
-        for (int k0 = 0; k0 < H; k0 += 16) {
-            u16x8 bw[2];
-            for (int ct = 0; ct < 2; ++ct) {
-                const int n = wave * 64 + ct * 32 + r32;
-                bw[ct] = *(const u16x8 *)(W + (size_t)n * H + k0 + kh);
-            }
-            for (int rt = 0; rt < 2; ++rt) {
-                const int r = rt * 32 + r32;
-                u16x8 ah = *(const u16x8 *)&sHi[r][k0 + kh];
-                u16x8 al = *(const u16x8 *)&sLo[r][k0 + kh];
-                for (int ct = 0; ct < 2; ++ct) {
-                    acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, ah),
-                                                                        __builtin_bit_cast(bf16x8, bw[ct]),
-                                                                        acc[rt][ct], 0, 0, 0);
-                    acc[rt][ct] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, al),
-                                                                        __builtin_bit_cast(bf16x8, bw[ct]),
-                                                                        acc[rt][ct], 0, 0, 0);
-                }
-            }
-        }
-        __syncthreads();  // every wave has read the layer input
-        const float *bias = P + OFF_B[l];
-        for (int rt = 0; rt < 2; ++rt)
-            for (int ct = 0; ct < 2; ++ct) {
-                const int col = wave * 64 + ct * 32 + r32;
-                const float bc = bias[col];
+    Wf[e] = u32x4{w[0], w[1], w[2], w[3]};
+}
+
+// Bias, SiLU and the hi/lo BF16 split of half JJ of tile T of a layer's FP32 result: the B
+// operands of the next layer's K slice 2T+JJ.
+template <int T, int JJ>
+__device__ __forceinline__ void mlp_epi_half(const f32x16 (&acc)[8], const float *__restrict__ bias, int h,
+                                             bf16x8 (&Bh)[16], bf16x8 (&Bl)[16])
+{
+    const float4 b0 = *(const float4 *)(bias + 32 * T + 16 * JJ + 4 * h);
+    const float4 b1 = *(const float4 *)(bias + 32 * T + 16 * JJ + 8 + 4 * h);
+    const float b[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+    for (int q = 0; q < 8; ++q) {
+        const float v = silu_fast(acc[T][8 * JJ + q] + b[q]);
+        const __bf16 hi = (__bf16)v;
+        Bh[2 * T + JJ][q] = hi;
+        Bl[2 * T + JJ][q] = (__bf16)(v - (float)hi);
+    }
+}
+
+// One 16-wide K slice: 8 weight fragments times the hi and the lo activations (KB is a template
+// argument so that every register array index is a compile-time constant).
+template <int KB>
+__device__ __forceinline__ void mlp_kslice(const u32x4 *src, f32x16 (&acc)[8], const bf16x8 (&Bh)[16],
+                                           const bf16x8 (&Bl)[16], int lane)
+{
+    bf16x8 a[8];
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int row = rt * 32 + 8 * j + 4 * (lane >> 5) + i;
-                        float a = silu(acc[rt][ct][4 * j + i] + bc);
-                        unsigned short hi = f2bf(a);
-                        sHi[row][col] = hi;
-                        sLo[row][col] = f2bf(a - bf2f(hi));
-                    }
-            }
+    for (int t = 0; t < 8; ++t) a[t] = __builtin_bit_cast(bf16x8, src[t * 64 + lane]);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[t], Bh[KB], acc[t], 0, 0, 0);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[t], Bl[KB], acc[t], 0, 0, 0);
+}
+
+// Stage ST of a hidden layer: the next stage of weights starts loading straight into the other
+// LDS slot (global_load_lds, no registers; the end-of-stage barrier waits for it); then the
+// stage's KPS K-slices from this slot, accumulating into out.  Stage ST needs the previous
+// layer's tiles 2ST and 2ST+1 (K-slices 4ST..4ST+3); the epilogue of tiles 2ST+2 and 2ST+3 is
+// placed in the same stage so that its VALU work can issue while the stage's MFMAs execute
+// (in and out are separate accumulator sets).  gs = the layer's first stage within NSTAGE.
+__device__ __forceinline__ void mlp_stage_load(u32x4 *slot, const u32x4 *__restrict__ Wf, int stage, int wave,
+                                               int lane)
+{
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+        __builtin_amdgcn_global_load_lds((const void *)(Wf + stage * SU4 + m * 256 + wave * 64 + lane),
+                                         (__attribute__((address_space(3))) void *)(slot + m * 256 + wave * 64), 16,
+                                         0, 0);
+}
+
+// Scheduling pattern of one K slice (masks of sched_group_barrier: 0x100 DS read, 0x8 MFMA,
+// 0x2 VALU)
+__device__ __forceinline__ void mlp_interleave()
+{
+    __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
+    }
+}
+
+template <int ST>
+__device__ __forceinline__ void mlp_stages(const f32x16 (&in)[8], f32x16 (&out)[8], const float *__restrict__ bias_in,
+                                           bf16x8 (&Bh)[16], bf16x8 (&Bl)[16], u32x4 (*ring)[SU4],
+                                           const u32x4 *__restrict__ Wf, int gs, int wave, int lane, int h)
+{
+    if constexpr (ST < 16 / KPS) {
+        mlp_stage_load(ring[(ST + 1) & 1], Wf, (gs + ST + 1) % NSTAGE, wave, lane);
+        const u32x4 *src = ring[ST & 1];
+        // K slice k with a quarter of the next tiles' epilogue, scheduled as 8 fragment reads and
+        // then 16 x (1 MFMA + up to 5 VALU) so that the VALU work issues in the MFMAs' shadow;
+        // sched_barrier keeps each slice's reads from being hoisted into the previous slice
+        mlp_kslice<KPS * ST + 0>(src, out, Bh, Bl, lane);
+        if constexpr (ST < 3) mlp_epi_half<2 * ST + 2, 0>(in, bias_in, h, Bh, Bl);
+        mlp_interleave();
+        __builtin_amdgcn_sched_barrier(0);
+        mlp_kslice<KPS * ST + 1>(src + 512, out, Bh, Bl, lane);
+        if constexpr (ST < 3) mlp_epi_half<2 * ST + 2, 1>(in, bias_in, h, Bh, Bl);
+        mlp_interleave();
+        __builtin_amdgcn_sched_barrier(0);
+        mlp_kslice<KPS * ST + 2>(src + 1024, out, Bh, Bl, lane);
+        if constexpr (ST < 3) mlp_epi_half<2 * ST + 3, 0>(in, bias_in, h, Bh, Bl);
+        mlp_interleave();
+        __builtin_amdgcn_sched_barrier(0);
+        mlp_kslice<KPS * ST + 3>(src + 1536, out, Bh, Bl, lane);
+        if constexpr (ST < 3) mlp_epi_half<2 * ST + 3, 1>(in, bias_in, h, Bh, Bl);
+        mlp_interleave();
         __syncthreads();
+        mlp_stages<ST + 1>(in, out, bias_in, Bh, Bl, ring, Wf, gs, wave, lane, h);
     }
-    // output layer: wave w handles rows [16w, 16w+16)
-    const float b5 = P[OFF_B5];
-    for (int rr = 0; rr < 16; ++rr) {
-        const int r = wave * 16 + rr;
-        float s = 0.f;
-        for (int k = lane; k < H; k += 64) s += (bf2f(sHi[r][k]) + bf2f(sLo[r][k])) * P[OFF_W5 + k];
-        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
-        if (lane == 0 && row0 + r < M) logb[row0 + r] = (double)(s + b5) * (double)sd + (double)mean;
+}
+
+// One hidden layer: out = W_L (SiLU(in + bias_in) as hi/lo BF16); tiles 0 and 1 of the
+// input epilogue run before the first stage, the rest inside the stages.
+template <int L>
+__device__ __forceinline__ void mlp_layer(const f32x16 (&in)[8], f32x16 (&out)[8], const float *__restrict__ bias_in,
+                                          bf16x8 (&Bh)[16], bf16x8 (&Bl)[16], u32x4 (*ring)[SU4],
+                                          const u32x4 *__restrict__ Wf, int wave, int lane, int h)
+{
+    mlp_epi_half<0, 0>(in, bias_in, h, Bh, Bl);
+    mlp_epi_half<0, 1>(in, bias_in, h, Bh, Bl);
+    mlp_epi_half<1, 0>(in, bias_in, h, Bh, Bl);
+    mlp_epi_half<1, 1>(in, bias_in, h, Bh, Bl);
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) out[t][i] = 0.f;
+    mlp_stages<0>(in, out, bias_in, Bh, Bl, ring, Wf, L * (16 / KPS), wave, lane, h);
+}
+
+__global__ void __launch_bounds__(256, 1) k_mlp_reg(const float *__restrict__ X, int M, const float *__restrict__ P,
+                                                    const u32x4 *__restrict__ Wf, float mean, float sd,
+                                                    double *__restrict__ logb)
+{
+    __shared__ u32x4 ring[2][SU4];
+    // W1, b1..b4 and W5 in LDS: reads of them after a barrier stay inside the chunk loop
+    // (loads straight from P are loop-invariant and would be hoisted into 800 live registers)
+    __shared__ __attribute__((aligned(16))) float sP[SP_N];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r = lane & 31, h = lane >> 5;
+    const int nchunk = (M + RB - 1) / RB;
+    for (int e = tid; e < SP_N; e += 256) {
+        size_t src;
+        if (e < SP_B1) {
+            src = OFF_W1 + e;
+        } else if (e < SP_W5) {
+            const int which = (e - SP_B1) / H, k = (e - SP_B1) % H;  // b1, b2, b3, b4
+            src = (which == 0 ? OFF_B1 : OFF_B[which - 1]) + k;
+        } else {
+            src = OFF_W5 + (e - SP_W5);
+        }
+        sP[e] = P[src];
+    }
+    mlp_stage_load(ring[0], Wf, 0, wave, lane);
+    __syncthreads();
+#pragma unroll 1
+    for (int c = blockIdx.x; c < nchunk; c += gridDim.x) {
+        const int row = c * RB + wave * 32 + r;
+        const bool valid = row < M;
+        f32x16 acc[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[t][i] = 0.f;
+#pragma unroll
+        for (int cc = 0; cc < NIN / 2; ++cc) {
+            const float xb = valid ? X[(size_t)row * NIN + 2 * cc + h] : 0.f;
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(sP[(32 * t + r) * NIN + 2 * cc + h], xb, acc[t], 0, 0, 0);
+        }
+        bf16x8 Bh[16], Bl[16];
+        f32x16 acc2[8];
+        // an opaque copy of the weight pointer per chunk: the 96 per-lane stage addresses derived
+        // from it are recomputed in the loop instead of being hoisted out of it and spilled
+        const u32x4 *wf = Wf;
+        asm volatile("" : "+s"(wf));
+        mlp_layer<0>(acc, acc2, sP + SP_B1, Bh, Bl, ring, wf, wave, lane, h);
+        mlp_layer<1>(acc2, acc, sP + SP_B1 + H, Bh, Bl, ring, wf, wave, lane, h);
+        mlp_layer<2>(acc, acc2, sP + SP_B1 + 2 * H, Bh, Bl, ring, wf, wave, lane, h);
+        // output layer on the FP32 activations of layer 4
+        float o = 0.f;
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float4 b = *(const float4 *)(sP + SP_B1 + 3 * H + 32 * t + 8 * j + 4 * h);
+                const float4 w = *(const float4 *)(sP + SP_W5 + 32 * t + 8 * j + 4 * h);
+                o += w.x * silu_fast(acc2[t][4 * j] + b.x) + w.y * silu_fast(acc2[t][4 * j + 1] + b.y) +
+                     w.z * silu_fast(acc2[t][4 * j + 2] + b.z) + w.w * silu_fast(acc2[t][4 * j + 3] + b.w);
+            }
+        o += __shfl_xor(o, 32, 64);
+        if (h == 0 && valid) logb[row] = (double)(o + P[OFF_B5]) * (double)sd + (double)mean;
     }
 }
 }  // namespace
@@ -770,8 +875,17 @@ static int sur_infer(hipStream_t st, TpSurrogate *s, const float *X, int M, unsi
 {
     (void)buf0;
     (void)buf1;
-    if (M % FR) return tp_set_error(TP_EINVAL, "sur_infer: rows must be a multiple of 64");
-    hipLaunchKernelGGL(k_mlp_fused, dim3(M / FR), dim3(256), 0, st, X, M, s->P, s->Wb, s->t_mean, s->t_std, logb);
+    if (M <= 0) return 0;
+    static int ncu = 0;  // one persistent workgroup per CU
+    if (!ncu) {
+        int dev = 0;
+        TP_HIP(hipGetDevice(&dev));
+        TP_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        if (ncu < 1) ncu = 1;
+    }
+    const int nchunk = (M + RB - 1) / RB;
+    hipLaunchKernelGGL(k_mlp_reg, dim3(std::min(nchunk, ncu)), dim3(256), 0, st, X, M, s->P, (const u32x4 *)s->Wf,
+                       s->t_mean, s->t_std, logb);
     TP_HIP(hipGetLastError());
     return 0;
 }
@@ -919,7 +1033,7 @@ extern "C" int tp_surrogate_train(tp_ctx *ctx, const tp_train_config *cfg, tp_tr
         float c1 = 1.0f - (float)std::pow(0.9, (double)s->step), c2 = 1.0f - (float)std::pow(0.999, (double)s->step);
         hipLaunchKernelGGL(k_adam, dim3(grid(NPARAM, 256)), dim3(256), 0, st, s->P, s->G, s->Am, s->Av, NPARAM,
                            (float)lr, 0.9f, 0.999f, 1e-8f, c1, c2);
-        if ((rc = sur_refresh_bf16(st, s))) {
+        if ((rc = sur_refresh_bf16(st, s, it == steps - 1))) {
             cleanup();
             return rc;
         }
