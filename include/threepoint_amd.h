@@ -108,6 +108,16 @@ int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, const int *ic
                   const double *theta_rad, double reltol, int mode, double *val, double *err,
                   long long *neval);
 
+/* Launch shape of the batched region kernel: a cubature point's 96 z-nodes are summed by 1, 2, 4
+ * or 8 waves.  By default the shape is tuned per launch-size bucket on the first batched call of
+ * a context and precision (all shapes give bit-identical results).  lanes = 0 restores tuning;
+ * 1, 2, 4 or 8 forces that shape.  No counterpart in the reference (its GPU build has one
+ * fixed launch shape). */
+int tp_set_region_lanes(tp_ctx *ctx, int lanes);
+/* The tuned shapes of mode (TP_MODE_FP64 or TP_MODE_FP32) for launches of < 32k, < 160k,
+ * < 640k and more points (0 = not tuned yet). */
+int tp_region_lanes_table(tp_ctx *ctx, int mode, int *lanes4);
+
 /* Statistics of the last tp_map3_batch call on this context. */
 typedef struct {
     long long rounds, regions, points, launches;

@@ -157,3 +157,28 @@ def test_map3_analytic_matches_oracle(fid_tables, fid_oracle):
 def test_nonfinite_theta_rejected(fid_tables):
     with pytest.raises(tp.TpError):
         fid_tables.map3_batch(np.array([[0.0, 1e-3, 1e-3]]))
+
+
+def test_launch_shapes_bit_identical(gpu_ctx, fid_tables):
+    """Every launch shape of the region kernel (1, 2, 4, 8 waves per point) gives the same bits,
+    so the autotuner's choice never changes a result."""
+    import ctypes
+    th = W.to_rad(W.triples(W.map3_c0_radii()))
+    try:
+        for mode in (tp.MODE_FP64, tp.MODE_FP32):
+            ref = None
+            for lanes in (1, 2, 4, 8):
+                tp._check(tp.lib().tp_set_region_lanes(gpu_ctx.h, lanes), "tp_set_region_lanes")
+                v, e, n = fid_tables.map3_batch(th, None, 1e-4, mode)
+                if ref is None:
+                    ref = (v, e, n)
+                else:
+                    assert np.array_equal(v, ref[0]) and np.array_equal(e, ref[1]) and np.array_equal(n, ref[2]), \
+                        (mode, lanes)
+    finally:
+        tp._check(tp.lib().tp_set_region_lanes(gpu_ctx.h, 0), "tp_set_region_lanes")
+    fid_tables.map3_batch(th[:2], None, 1e-4, tp.MODE_FP32)  # tuned shapes in use
+    tab = (ctypes.c_int * 4)()
+    tp._check(tp.lib().tp_region_lanes_table(gpu_ctx.h, tp.MODE_FP32, tab), "tp_region_lanes_table")
+    print("tuned waves per point (FP32, by launch size):", list(tab))
+    assert all(x in (1, 2, 4, 8) for x in tab)

@@ -90,6 +90,8 @@ _SIGS = {
     "tp_census_map3_point": (ctypes.c_int, [_dp]),
     "tp_census_gamma_term": (ctypes.c_int, [_dp]),
     "tp_hbm_copy_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _dp]),
+    "tp_set_region_lanes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "tp_region_lanes_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "tp_surrogate_train": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(tp_train_config),
                                           ctypes.POINTER(tp_train_report)]),
     "tp_surrogate_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),

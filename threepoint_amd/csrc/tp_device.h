@@ -171,12 +171,65 @@ __host__ __device__ __forceinline__ T tp_ogata_term(const TpNodeT<T> &N, const T
     return N.wz * tp_bispec_node<T>(N, E, q, lq, P.f2t, P.lr1, P.r2) * prodk;
 }
 
-// the 96-node Limber sum of the reference's bkappa (classed_version/bispectrum.cpp:238-286)
+// The 96-node Limber sum of the reference's bkappa (classed_version/bispectrum.cpp:238-286),
+// accumulated in a fixed order: 8 partial sums of 12 consecutive nodes, combined pairwise,
+// ((p0 + p1) + (p2 + p3)) + ((p4 + p5) + (p6 + p7)).  A point whose nodes are split over
+// 1, 2, 4 or 8 waves forms the same tree, so every launch shape gives the same bits.
+constexpr int TP_NPART = 8, TP_PART_NODES = TP_NNODES / TP_NPART;
+
+// partials [part0, part0 + NP) of the tree, combined pairwise (NP a power of two)
+template <class T, int NP>
+__device__ __forceinline__ double tp_bkappa_parts(const TpNodeT<T> *__restrict__ nodes, const TpEHT<T> &E,
+                                                  const TpPointPre &P, int part0)
+{
+    const T L[3] = {(T)P.l[0], (T)P.l[1], (T)P.l[2]};
+    const T LL[3] = {(T)P.ll[0], (T)P.ll[1], (T)P.ll[2]};
+    const T F2T[3] = {(T)P.f2t[0], (T)P.f2t[1], (T)P.f2t[2]};
+    const T LR1 = (T)P.lr1, R2 = (T)P.r2;
+    // one pass over the nodes; at the end of each partial its sum joins a pairwise tree built as
+    // the partials arrive (s1, s2, s4 hold the pending left operands of the levels (p0 + p1),
+    // ((p0 + p1) + (p2 + p3)), ...)
+    double s1 = 0.0, s2 = 0.0, s4 = 0.0, v = 0.0, acc = 0.0;
+    const int n0 = part0 * TP_PART_NODES, n1 = n0 + NP * TP_PART_NODES;
+#pragma unroll 2
+    for (int n = n0; n < n1; ++n) {
+        const TpNodeT<T> &N = nodes[n];
+        const T ks = N.ks, lks = N.lnks;
+        T q[3] = {L[0] * ks, L[1] * ks, L[2] * ks};
+        T lq[3] = {LL[0] + lks, LL[1] + lks, LL[2] + lks};
+        T b = tp_bispec_node<T>(N, E, q, lq, F2T, LR1, R2);
+        acc += (double)(N.wz * b);
+        const int m = n - n0 + 1;
+        if (m % TP_PART_NODES == 0) {  // partial k = m / 12 - 1 complete
+            const int k = m / TP_PART_NODES - 1;
+            v = acc;
+            acc = 0.0;
+            if (k & 1) {
+                v = s1 + v;
+                if (k & 2) {
+                    v = s2 + v;
+                    if (k & 4)
+                        v = s4 + v;
+                    else
+                        s4 = v;
+                } else {
+                    s2 = v;
+                }
+            } else {
+                s1 = v;
+            }
+        }
+    }
+    return v;
+}
+
 template <class T>
 __device__ __forceinline__ double tp_bkappa_sum(const TpNodeT<T> *__restrict__ nodes, const TpEHT<T> &E,
                                                 double l1, double l2, double l3)
 {
-    return tp_bkappa_sum_n<T, TP_NNODES>(nodes, E, l1, l2, l3);
+    TpPointPre P;
+    tp_point_pre<double>(l1, l2, l3, &P);
+    return tp_bkappa_parts<T, TP_NPART>(nodes, E, P, 0);
 }
 
 // the reference's test_analytical B_kappa (classed_version/bispectrum.cpp:242-255, 989-996)
@@ -214,26 +267,50 @@ __device__ __forceinline__ double tp_ell3(double l1, double l2, double phi)
     return sqrt(s);
 }
 
+// The <Map^3> integrand (classed_version/apertureStatistics.cpp:16-28) in two halves around the
+// z-sum, for kernels that split a point's nodes over several waves.  tp_map3_begin returns
+// TP_PT_DONE for a non-finite ell (flagged; the value is NaN), TP_PT_NOSUM when B_kappa needs no
+// z-sum (an ell is 0, or the analytic test bispectrum) and TP_PT_SUM otherwise (P filled).
+enum { TP_PT_DONE = 0, TP_PT_NOSUM = 1, TP_PT_SUM = 2 };
+__device__ __forceinline__ int tp_map3_begin(double l1, double l2, double phi, int mode, int *flag, TpPointPre *P)
+{
+    double l3 = tp_ell3(l1, l2, phi);
+    P->l[0] = l1;
+    P->l[1] = l2;
+    P->l[2] = l3;
+    if (!isfinite(l1) || !isfinite(l2) || !isfinite(l3)) {
+        atomicOr(flag, 1);
+        return TP_PT_DONE;
+    }
+    if (l1 == 0 || l2 == 0 || l3 == 0 || mode == TP_MODE_ANALYTIC) return TP_PT_NOSUM;  // bispectrum.cpp:240-241
+    tp_point_pre<double>(l1, l2, l3, P);
+    return TP_PT_SUM;
+}
+__device__ __forceinline__ double tp_map3_end(const TpPointPre &P, int kind, double bk_sum, double th1, double th2,
+                                              double th3, int *flag)
+{
+    if (kind == TP_PT_DONE) return __builtin_nan("");
+    const double l1 = P.l[0], l2 = P.l[1], l3 = P.l[2];
+    double bk;
+    if (kind == TP_PT_SUM)
+        bk = bk_sum;
+    else if (l1 == 0 || l2 == 0 || l3 == 0)
+        bk = 0.0;
+    else
+        bk = tp_bkappa_analytic(l1, l2, l3);  // TP_MODE_ANALYTIC
+    if (!isfinite(bk)) atomicOr(flag, 2);
+    return l1 * l2 * bk * tp_uhat<double>(l1 * th1) * tp_uhat<double>(l2 * th2) * tp_uhat<double>(l3 * th3);
+}
+
 template <class T>
 __device__ __forceinline__ double tp_map3_integrand(const TpNodeT<T> *__restrict__ nodes, const TpEHT<T> &E,
                                                     double l1, double l2, double phi, double th1, double th2,
                                                     double th3, int mode, int *flag)
 {
-    // classed_version/apertureStatistics.cpp:16-28
-    double l3 = tp_ell3(l1, l2, phi);
-    if (!isfinite(l1) || !isfinite(l2) || !isfinite(l3)) {
-        atomicOr(flag, 1);
-        return __builtin_nan("");
-    }
-    double bk;
-    if (l1 == 0 || l2 == 0 || l3 == 0)
-        bk = 0.0;  // classed_version/bispectrum.cpp:240-241
-    else if (mode == TP_MODE_ANALYTIC)
-        bk = tp_bkappa_analytic(l1, l2, l3);
-    else
-        bk = tp_bkappa_sum<T>(nodes, E, l1, l2, l3);
-    if (!isfinite(bk)) atomicOr(flag, 2);
-    return l1 * l2 * bk * tp_uhat<double>(l1 * th1) * tp_uhat<double>(l2 * th2) * tp_uhat<double>(l3 * th3);
+    TpPointPre P;
+    const int kind = tp_map3_begin(l1, l2, phi, mode, flag, &P);
+    const double s = kind == TP_PT_SUM ? tp_bkappa_parts<T, TP_NPART>(nodes, E, P, 0) : 0.0;
+    return tp_map3_end(P, kind, s, th1, th2, th3, flag);
 }
 
 // --------------------------------------------------- Genz-Malik 3-D rule

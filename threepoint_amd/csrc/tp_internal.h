@@ -148,7 +148,10 @@ struct tp_ctx {
     TpSurrogate *sur = nullptr;
     double sur_check_frac = 0.01, sur_tol = 1e-3;
     uint64_t sur_seed = 0x5eed;
-    int autotune_block = 0;     // 0 = not tuned yet
+    // launch shape of the region-point kernel: waves per point for [FP64, FP32][size bucket]
+    int region_lpp[2][4] = {{1, 1, 1, 1}, {1, 1, 1, 1}};
+    bool region_tuned[2] = {false, false};
+    int region_lpp_force = 0;   // 0 = tuned; 1, 2, 4, 8 = fixed
     TpLane lane[TP_NLANES];
 };
 

@@ -14,6 +14,9 @@
 //    456-577) for many 3-D regions of many integrals per launch -- one thread
 //    per (region, point), then one thread per region for the rule sums, in the
 //    reference's summation order.
+#include <cmath>
+#include <vector>
+
 #include "tp_device.h"
 
 namespace {
@@ -50,24 +53,53 @@ __global__ void __launch_bounds__(256) k_bkappa_points(const TpNodeT<T> *__restr
     out[i] = v;
 }
 
-// One thread per (region, point).  nodes/eh are indexed by the integral's cosmology.
-template <class T>
-__global__ void __launch_bounds__(256) k_region_points(const TpNodeT<T> *__restrict__ nodes,
-                                                       const TpEHT<T> *__restrict__ eh, size_t nreg,
-                                                       const TpRegion *__restrict__ reg,
-                                                       const TpIntegralDesc *__restrict__ desc,
-                                                       double *__restrict__ fvals, int mode, int *flag)
+// Integrand values of the 33 Genz-Malik points of many regions of many integrals.  A point's 96
+// z-nodes are split over LPP waves (1, 2, 4 or 8): wave w of a point group sums the partials
+// [w 8/LPP, (w+1) 8/LPP) of the fixed summation tree with wave-uniform (scalar) node reads, and
+// the group's first wave combines the waves' sums in tree order -- the same bits for every LPP.
+// A workgroup holds max(1, 4/LPP) groups of 64 points.  nodes/eh are indexed by the integral's
+// cosmology.
+template <class T, int LPP>
+__global__ void __launch_bounds__(64 * (LPP < 4 ? 4 : LPP)) k_region_points(
+    const TpNodeT<T> *__restrict__ nodes, const TpEHT<T> *__restrict__ eh, size_t nreg,
+    const TpRegion *__restrict__ reg, const TpIntegralDesc *__restrict__ desc, double *__restrict__ fvals, int mode,
+    int *flag)
 {
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nreg * TP_NPTS_GM3) return;
-    size_t r = t / TP_NPTS_GM3;
-    int p = (int)(t - r * TP_NPTS_GM3);
-    const TpRegion R = reg[r];
-    const TpIntegralDesc D = desc[R.integral];
-    double x0, x1, x2;
-    gm_point3(R, p, &x0, &x1, &x2);
-    const TpNodeT<T> *nd = nodes + (size_t)D.icosmo * TP_NNODES;
-    fvals[t] = tp_map3_integrand<T>(nd, eh[D.icosmo], x0, x1, x2, D.th[0], D.th[1], D.th[2], mode, flag);
+    constexpr int GPB = LPP < 4 ? 4 / LPP : 1;  // point groups per workgroup
+    __shared__ double part[GPB][LPP][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, g = wv / LPP, w = wv % LPP;
+    const size_t t = ((size_t)blockIdx.x * GPB + g) * 64 + lane;
+    const bool live = t < nreg * TP_NPTS_GM3;
+    TpPointPre P;
+    int kind = TP_PT_DONE;
+    double v = 0.0;
+    TpIntegralDesc D{};
+    if (live) {
+        const size_t r = t / TP_NPTS_GM3;
+        const int p = (int)(t - r * TP_NPTS_GM3);
+        const TpRegion R = reg[r];
+        D = desc[R.integral];
+        double x0, x1, x2;
+        gm_point3(R, p, &x0, &x1, &x2);
+        kind = tp_map3_begin(x0, x1, x2, mode, flag, &P);
+        if (kind == TP_PT_SUM)
+            v = tp_bkappa_parts<T, TP_NPART / LPP>(nodes + (size_t)D.icosmo * TP_NNODES, eh[D.icosmo], P,
+                                                   w * (TP_NPART / LPP));
+    }
+    if constexpr (LPP > 1) {
+        part[g][w][lane] = v;
+        __syncthreads();
+        if (w != 0) return;
+        double a[LPP];
+#pragma unroll
+        for (int k = 0; k < LPP; ++k) a[k] = part[g][k][lane];
+#pragma unroll
+        for (int s = 1; s < LPP; s <<= 1)
+#pragma unroll
+            for (int k = 0; k < LPP; k += 2 * s) a[k] = a[k] + a[k + s];
+        v = a[0];
+    }
+    if (live) fvals[t] = tp_map3_end(P, kind, v, D.th[0], D.th[1], D.th[2], flag);
 }
 
 // Region r's flag: bit 0 = non-finite rule estimate.  Region 0 also carries the point
@@ -121,6 +153,38 @@ int tp_launch_bkappa_points(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t n
     return 0;
 }
 
+// one launch of the region-point kernel with LPP waves per point group
+template <class T>
+static void launch_region_points(hipStream_t st, int lpp, const TpNodeT<T> *nodes, const TpEHT<T> *eh, size_t nreg,
+                                 const TpRegion *d_reg, const TpIntegralDesc *d_desc, double *d_fvals, int mode,
+                                 int *d_flag)
+{
+    const size_t np = nreg * TP_NPTS_GM3;
+    const size_t groups = (np + 63) / 64;
+    switch (lpp) {
+    case 2:
+        hipLaunchKernelGGL((k_region_points<T, 2>), dim3((unsigned)((groups + 1) / 2)), dim3(256), 0, st, nodes, eh,
+                           nreg, d_reg, d_desc, d_fvals, mode, d_flag);
+        break;
+    case 4:
+        hipLaunchKernelGGL((k_region_points<T, 4>), dim3((unsigned)groups), dim3(256), 0, st, nodes, eh, nreg, d_reg,
+                           d_desc, d_fvals, mode, d_flag);
+        break;
+    case 8:
+        hipLaunchKernelGGL((k_region_points<T, 8>), dim3((unsigned)groups), dim3(512), 0, st, nodes, eh, nreg, d_reg,
+                           d_desc, d_fvals, mode, d_flag);
+        break;
+    default:
+        hipLaunchKernelGGL((k_region_points<T, 1>), dim3((unsigned)((groups + 3) / 4)), dim3(256), 0, st, nodes, eh,
+                           nreg, d_reg, d_desc, d_fvals, mode, d_flag);
+    }
+}
+
+// size bucket of a launch for the launch-shape table (points: < 32k, < 160k, < 640k, more)
+static int region_bucket(size_t np) { return np < 32768 ? 0 : np < 163840 ? 1 : np < 655360 ? 2 : 3; }
+
+int tp_region_lanes(tp_ctx *ctx, const tp_cosmo *c, int mode, size_t np);  // tp_tune below
+
 int tp_launch_regions(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegion *d_reg,
                       const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int mode, int *d_flag,
                       hipEvent_t evm)
@@ -128,15 +192,119 @@ int tp_launch_regions(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegio
     if (nreg == 0) return 0;
     const int B = 256;
     size_t np = nreg * TP_NPTS_GM3;
+    const int lpp = tp_region_lanes(ctx, c, mode, np);
+    if (lpp < 0) return TP_EHIP;
     if (mode == TP_MODE_FP32)
-        hipLaunchKernelGGL(k_region_points<float>, dim3(grid_for(np, B)), dim3(B), 0, ctx->stream, c->d_node32,
-                           c->d_eh32, nreg, d_reg, d_desc, d_fvals, mode, d_flag);
+        launch_region_points<float>(ctx->stream, lpp, c->d_node32, c->d_eh32, nreg, d_reg, d_desc, d_fvals, mode,
+                                    d_flag);
     else
-        hipLaunchKernelGGL(k_region_points<double>, dim3(grid_for(np, B)), dim3(B), 0, ctx->stream, c->d_node64,
-                           c->d_eh64, nreg, d_reg, d_desc, d_fvals, mode, d_flag);
+        launch_region_points<double>(ctx->stream, lpp, c->d_node64, c->d_eh64, nreg, d_reg, d_desc, d_fvals, mode,
+                                     d_flag);
+    TP_HIP(hipGetLastError());
     TP_HIP(hipEventRecord(evm, ctx->stream));
     hipLaunchKernelGGL(k_region_rule, dim3(grid_for(nreg, B)), dim3(B), 0, ctx->stream, nreg, d_reg, d_fvals, d_out,
                        d_flag);
     TP_HIP(hipGetLastError());
+    return 0;
+}
+
+// ------------------------------------------------------------------ launch-shape autotuner
+// The waves-per-point split of k_region_points is chosen per launch-size bucket by timing every
+// shape on a synthetic batch of that size, once per context and precision, on the first batched
+// call (a few tens of ms).  Small launches (the tail rounds of a batch) favour more waves per
+// point, large ones one wave.  Every shape computes the same bits, so the choice never changes a
+// result.  tp_set_region_lanes forces one shape (0 restores tuning).
+int tp_region_lanes(tp_ctx *ctx, const tp_cosmo *c, int mode, size_t np)
+{
+    if (ctx->region_lpp_force) return ctx->region_lpp_force;
+    const int im = mode == TP_MODE_FP32 ? 1 : 0;
+    if (mode == TP_MODE_ANALYTIC) return 1;
+    if (!ctx->region_tuned[im]) {
+        ctx->region_tuned[im] = true;  // also when tuning fails: then the default shape stays
+        const size_t sizes[4] = {256, 2048, 8192, 32768};  // regions per bucket probe
+        const size_t nmax = sizes[3];
+        TpRegion *d_reg = nullptr;
+        TpIntegralDesc *d_desc = nullptr;
+        double *d_f = nullptr;
+        int *d_fl = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        bool ok = hipMalloc(&d_reg, nmax * sizeof(TpRegion)) == hipSuccess &&
+                  hipMalloc(&d_desc, sizeof(TpIntegralDesc)) == hipSuccess &&
+                  hipMalloc(&d_f, nmax * TP_NPTS_GM3 * sizeof(double)) == hipSuccess &&
+                  hipMalloc(&d_fl, sizeof(int)) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+                  hipEventCreate(&e1) == hipSuccess;
+        if (ok) {
+            // regions spread in log ell over [1, 7000]^2 x [0, 2 pi] for radii (2', 4', 8'), cosmology 0
+            std::vector<TpRegion> hr(nmax);
+            for (size_t k = 0; k < nmax; ++k) {
+                const double u = (double)((k * 2654435761u) % 1000003) / 1000003.0;
+                const double v = (double)((k * 40503u + 17) % 999983) / 999983.0;
+                const double ph = (double)((k * 97u + 5) % 1009) / 1009.0;
+                TpRegion &R = hr[k];
+                R.c[0] = std::exp(8.85 * u);
+                R.c[1] = std::exp(8.85 * v);
+                R.c[2] = 2 * M_PI * ph;
+                R.hw[0] = 0.05 * R.c[0];
+                R.hw[1] = 0.05 * R.c[1];
+                R.hw[2] = 0.05;
+                R.vol = 8 * R.hw[0] * R.hw[1] * R.hw[2];
+                R.integral = 0;
+                R.pad = 0;
+            }
+            const double arcmin = 2.9088820866e-4;
+            TpIntegralDesc hd{{2 * arcmin, 4 * arcmin, 8 * arcmin}, 0, 0};
+            ok = hipMemcpy(d_reg, hr.data(), nmax * sizeof(TpRegion), hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemcpy(d_desc, &hd, sizeof(hd), hipMemcpyHostToDevice) == hipSuccess &&
+                 hipMemset(d_fl, 0, sizeof(int)) == hipSuccess;
+        }
+        const int shapes[4] = {1, 2, 4, 8};
+        for (int b = 0; ok && b < 4; ++b) {
+            float best = 0;
+            for (int s = 0; ok && s < 4; ++s) {
+                float ms = 0;
+                for (int rep = 0; rep < 3; ++rep) {  // first launch warms up
+                    ok = hipEventRecord(e0, ctx->stream) == hipSuccess;
+                    if (im)
+                        launch_region_points<float>(ctx->stream, shapes[s], c->d_node32, c->d_eh32, sizes[b], d_reg,
+                                                    d_desc, d_f, TP_MODE_FP32, d_fl);
+                    else
+                        launch_region_points<double>(ctx->stream, shapes[s], c->d_node64, c->d_eh64, sizes[b], d_reg,
+                                                     d_desc, d_f, TP_MODE_FP64, d_fl);
+                    ok = ok && hipGetLastError() == hipSuccess && hipEventRecord(e1, ctx->stream) == hipSuccess &&
+                         hipEventSynchronize(e1) == hipSuccess;
+                    float t = 0;
+                    if (ok && rep > 0 && hipEventElapsedTime(&t, e0, e1) == hipSuccess) ms += t;
+                }
+                if (ok && (s == 0 || ms < best)) {
+                    best = ms;
+                    ctx->region_lpp[im][b] = shapes[s];
+                }
+            }
+        }
+        if (d_reg) (void)hipFree(d_reg);
+        if (d_desc) (void)hipFree(d_desc);
+        if (d_f) (void)hipFree(d_f);
+        if (d_fl) (void)hipFree(d_fl);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (!ok) return tp_set_error(TP_EHIP, "tp_launch_regions: launch-shape tuning failed"), -1;
+    }
+    return ctx->region_lpp[im][region_bucket(np)];
+}
+
+extern "C" int tp_set_region_lanes(tp_ctx *ctx, int lanes)
+{
+    if (!ctx || !(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8))
+        return tp_set_error(TP_EINVAL, "tp_set_region_lanes: lanes must be 0 (tuned), 1, 2, 4 or 8");
+    ctx->region_lpp_force = lanes;
+    return 0;
+}
+
+extern "C" int tp_region_lanes_table(tp_ctx *ctx, int mode, int *lanes4)
+{
+    if (!ctx || !lanes4 || (mode != TP_MODE_FP64 && mode != TP_MODE_FP32))
+        return tp_set_error(TP_EINVAL, "tp_region_lanes_table: bad arguments");
+    const int im = mode == TP_MODE_FP32 ? 1 : 0;
+    for (int b = 0; b < 4; ++b) lanes4[b] = ctx->region_tuned[im] ? ctx->region_lpp[im][b] : 0;
     return 0;
 }
