@@ -33,9 +33,10 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
 # HBM bytes per integrand point of the region kernels: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the
-# C1 batched paths, each counter scaled by its reading for k_copy16 of 256 MiB (FETCH_SIZE x2.000,
-# WRITE_SIZE x1.000).  Algorithmic: 64/33 B of region in + 8 B of integrand value out.
-HBM_BYTES_PER_POINT = {"k_region_points<double>": 12.93, "k_region_points<float>": 11.63}
+# C1 batched paths (the launch-shape tuner's probe launches excluded), each counter scaled by its reading
+# for k_copy16 of 256 MiB (FETCH_SIZE x2.000, WRITE_SIZE x1.000).  Algorithmic: 64/33 B of region in +
+# 8 B of integrand value out.
+HBM_BYTES_PER_POINT = {"double": 14.00, "float": 13.23}
 FLOPS_PER_ELEM = 20.0
 
 
@@ -346,11 +347,12 @@ def main():
     head = max(ok, key=lambda p: results[p]["triples_per_s"]) if ok else "batched_fp64"
     hr = results[head]
     dom = hr.get("integrand_kernel") or next(r["integrand_kernel"] for r in results.values() if "integrand_kernel" in r)
-    kname = "k_region_points<%s>" % ("float" if head == "batched_fp32" else "double")
+    prec = "float" if head == "batched_fp32" else "double"
+    kname = "k_region_points<%s, waves per point tuned>" % prec
     pts_per_launch = hr["points"] / max(1, hr["launches"])
     roof = {"bound": "valu", "kernel": kname,
             "achieved": dom["achieved_flops"] / 1e12, "peak": dom["peak"] / 1e12, "unit": "TFLOP/s",
-            "frac": dom["frac"], "traffic": HBM_BYTES_PER_POINT[kname] * pts_per_launch,
+            "frac": dom["frac"], "traffic": HBM_BYTES_PER_POINT[prec] * pts_per_launch,
             "traffic_unit": "HBM bytes per launch (per-point bytes from the PMC passes x points per launch)",
             "algorithmic_bytes_per_point": 64 / 33 + 8,
             "work_per_point_flop_eq": fe_point, "flops_per_elementary_call": FLOPS_PER_ELEM,

@@ -353,6 +353,7 @@ extern "C" int tp_map3_single(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const 
 static const long long kMaxRounds = 20000;
 static const long long kMaxRegions = 400000000LL;
 static const size_t kSurrogateMaxEval = 4000000;  // points per triple before an exact redo
+int tp_region_lanes(tp_ctx *ctx, const tp_cosmo *c, int mode, size_t np);  // tp_map3.hip
 int tp_surrogate_round(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegion *d_reg,
                        const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int *d_flag,
                        int *fell_back);  // tp_mlp.hip
@@ -400,6 +401,9 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     };
     Group G[TP_NLANES];
     for (size_t t = 0; t < ntriples; ++t) G[nl == 1 ? 0 : (int)(t % nl)].active.push_back(t);
+    // launch-shape tuning (first batched call of this precision) before any round is timed
+    if (mode == TP_MODE_FP64 || mode == TP_MODE_FP32 || mode == TP_MODE_SURROGATE)
+        if (tp_region_lanes(ctx, c, mode == TP_MODE_FP32 ? TP_MODE_FP32 : TP_MODE_FP64, 0) < 0) return TP_EHIP;
     for (int g = 0; g < nl; ++g) {
         if ((rc = ctx->lane[g].create())) return rc;
         // the flag word is only ever raised, and a raised flag ends the batch: clear it once per call
