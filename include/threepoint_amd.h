@@ -62,6 +62,16 @@ typedef struct tp_cosmo tp_cosmo; /* device-resident tables of ncosmo cosmologie
  * nz: n_z values per cosmology; nz_stride = 0 shares one n(z) among all. */
 int tp_cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *params, const double *nz,
                    int n_z, size_t nz_stride, double z_max, tp_cosmo **out);
+/* Replaces the BispectrumCalculator constructor with a given linear P(k)
+ * (classed_version/bispectrum.cpp:81-89, linear_pk's Pk_given branch :785-806), the input
+ * the cosmosis module passes (classed_version/interface.cpp:176-194): as tp_cosmo_build, with
+ * the linear P(k) of cosmology i the nk pairs (k_h[i*pk_stride + j], p_k[...]) instead of the
+ * Eisenstein & Hu fit.  k [h/Mpc] strictly increasing, P [(Mpc/h)^3]; between entries P is
+ * linear in k, outside the table it is the end value, and sigma8 does not rescale it (as in
+ * the reference).  pk_stride = 0 shares one table.  Surrogate mode rejects such tables. */
+int tp_cosmo_build_pk(tp_ctx *ctx, int ncosmo, const tp_cosmology *params, const double *nz, int n_z,
+                      size_t nz_stride, double z_max, const double *k_h, const double *p_k, int nk,
+                      size_t pk_stride, tp_cosmo **out);
 int tp_cosmo_destroy(tp_cosmo *c);
 int tp_cosmo_count(const tp_cosmo *c, int *ncosmo, int *n_z);
 /* Copies one cosmology's tables to the host (each array n_z doubles). */

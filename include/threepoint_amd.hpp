@@ -21,6 +21,7 @@
 #include <array>
 #include <complex>
 #include <iosfwd>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -62,6 +63,9 @@ class BispectrumCalculator {
 
     BispectrumCalculator() {}
     BispectrumCalculator(cosmology *cosmo, const std::vector<double> &n_of_z, int n_z, double z_max);
+    /* with a given linear P(k), classed_version/bispectrum.hpp:218-219 (bispectrum.cpp:81-89) */
+    BispectrumCalculator(cosmology *cosmo, const std::vector<double> &n_of_z, int n_z, double z_max,
+                         const std::map<double, double> &P_k);
     ~BispectrumCalculator();
     BispectrumCalculator(const BispectrumCalculator &) = delete;
     BispectrumCalculator &operator=(const BispectrumCalculator &) = delete;
@@ -83,6 +87,7 @@ class BispectrumCalculator {
     tp_cosmo *tables_ = nullptr;
     std::vector<double> n_of_z_;
     int n_z_ = 0;
+    std::vector<double> pk_k_, pk_p_;  // given linear P(k), empty: Eisenstein & Hu
 };
 
 class ApertureStatistics;

@@ -30,6 +30,7 @@
 #include <fstream>
 #include <sstream>
 #include <string>
+#include <map>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -623,8 +624,25 @@ struct Calc {
                 (std::pow(z, 5.1843) + 0.7259)) / 2.97653;
     }
 
+    bool Pk_given = false;                 /* CV/bispectrum.cpp:81-89 */
+    std::map<double, double> linearPk_given;
+
     double linear_pk(double k) const
-    { /* CV/bispectrum.cpp:785-841 (Eisenstein & Hu no-wiggle; Pk_given branch absent) */
+    { /* CV/bispectrum.cpp:785-809: the given table (linear in k, end values outside) ... */
+        if (Pk_given) {
+            auto ix = linearPk_given.upper_bound(k);
+            if (ix == linearPk_given.end()) return (--ix)->second;
+            if (ix == linearPk_given.begin()) return ix->second;
+            auto lo = ix;
+            --lo;
+            double diff = (k - lo->first) / (ix->first - lo->first);
+            return diff * ix->second + (1 - diff) * lo->second;
+        }
+        return linear_pk_eh(k);
+    }
+
+    double linear_pk_eh(double k) const
+    { /* ... or Eisenstein & Hu no-wiggle, CV/bispectrum.cpp:811-841 */
         double pk, delk, alnu, geff, qeff, L, C;
         k *= cosmo.h;
         double fc = cosmo.omc / cosmo.om;
@@ -1246,6 +1264,29 @@ void *orc_create(const double *p, const double *nz, int n_z, double z_max)
     c->z_max = z_max;
     c->dz = z_max / ((double)n_z);
     c->nz_file.assign(nz, nz + n_z);
+    c->set_cosmology();
+    return c;
+}
+
+void *orc_create_pk(const double *p, const double *nz, int n_z, double z_max, const double *k, const double *pk,
+                    int nk)
+{ /* CV/bispectrum.cpp:81-89 (n(z) and given linear P(k)) */
+    Calc *c = new Calc();
+    c->cosmo.h = p[0];
+    c->cosmo.sigma8 = p[1];
+    c->cosmo.omb = p[2];
+    c->cosmo.ns = p[3];
+    c->cosmo.w = p[4];
+    c->cosmo.om = p[5];
+    c->cosmo.zmax = p[6];
+    c->cosmo.ow = 1 - c->cosmo.om;
+    c->cosmo.omc = c->cosmo.om - c->cosmo.omb;
+    c->n = n_z;
+    c->z_max = z_max;
+    c->dz = z_max / ((double)n_z);
+    c->nz_file.assign(nz, nz + n_z);
+    c->Pk_given = true;
+    for (int i = 0; i < nk; ++i) c->linearPk_given[k[i]] = pk[i];
     c->set_cosmology();
     return c;
 }

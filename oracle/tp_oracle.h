@@ -48,6 +48,8 @@ int orc_read_n_of_z(const char *fn, int n_bins, double zmax, double *nz_out);
 
 /* Bispectrum calculator handle.  params7 = {h, sigma8, Omega_b, n_s, w, Omega_m, z_max}. */
 void *orc_create(const double *params7, const double *nz, int n_z, double z_max);
+void *orc_create_pk(const double *p, const double *nz, int n_z, double z_max, const double *k, const double *pk,
+                    int nk);
 void orc_destroy(void *h);
 void orc_set_analytic(void *h, int on); /* reference's test_analytical switch (bispectrum.hpp:18) */
 void orc_get_tables(void *h, double *fK, double *g, double *D1, double *rs, double *neff,

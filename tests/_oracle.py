@@ -25,6 +25,7 @@ _SIGS = {
                                     ctypes.c_double, ctypes.c_int]),
     "orc_gauss_legendre": (None, [ctypes.c_int, _dp, _dp]),
     "orc_create": (ctypes.c_void_p, [_dp, _dp, ctypes.c_int, ctypes.c_double]),
+    "orc_create_pk": (ctypes.c_void_p, [_dp, _dp, ctypes.c_int, ctypes.c_double, _dp, _dp, ctypes.c_int]),
     "orc_destroy": (None, [ctypes.c_void_p]),
     "orc_set_analytic": (None, [ctypes.c_void_p, ctypes.c_int]),
     "orc_get_tables": (None, [ctypes.c_void_p] + [_dp] * 7),
@@ -75,10 +76,15 @@ def d(a):
 
 
 class Oracle:
-    def __init__(self, cosmo, nz, n_z=100, z_max=3.0):
+    def __init__(self, cosmo, nz, n_z=100, z_max=3.0, pk=None):
         p = np.array([cosmo.h, cosmo.sigma8, cosmo.omb, cosmo.ns, cosmo.w, cosmo.om, cosmo.zmax], dtype=np.float64)
         nz = np.ascontiguousarray(nz, dtype=np.float64)
-        self.h = lib().orc_create(d(p), d(nz), n_z, z_max)
+        if pk is None:
+            self.h = lib().orc_create(d(p), d(nz), n_z, z_max)
+        else:
+            kk = np.ascontiguousarray(pk[0], dtype=np.float64)
+            pp = np.ascontiguousarray(pk[1], dtype=np.float64)
+            self.h = lib().orc_create_pk(d(p), d(nz), n_z, z_max, d(kk), d(pp), kk.size)
         self.n_z = n_z
 
     def __del__(self):

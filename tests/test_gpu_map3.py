@@ -182,3 +182,32 @@ def test_launch_shapes_bit_identical(gpu_ctx, fid_tables):
     tp._check(tp.lib().tp_region_lanes_table(gpu_ctx.h, tp.MODE_FP32, tab), "tp_region_lanes_table")
     print("tuned waves per point (FP32, by launch size):", list(tab))
     assert all(x in (1, 2, 4, 8) for x in tab)
+
+
+def test_given_pk_matches_oracle(gpu_ctx, fid_nz):
+    """Tables, B_kappa and <Map^3> from a given linear P(k) (the cosmosis input,
+    classed_version/bispectrum.cpp:81-89, 785-806) against the oracle's Pk_given branch."""
+    from _oracle import Oracle
+    eh = Oracle(W.fiducial(), fid_nz)
+    k = np.logspace(-5, 4, 150)
+    p = np.array([eh.scalar("linear_pk", kk) for kk in k]) * (1 + 0.05 * np.sin(3 * np.log(k)))  # not the fit
+    o = Oracle(W.fiducial(), fid_nz, pk=(k, p))
+    tabs = tp.CosmologyTables([W.fiducial()], fid_nz, 100, 3.0, gpu_ctx, pk=(k, p))
+    ot, gt = o.tables(), tabs.tables(0)
+    for name in ("r_sigma", "n_eff", "D1", "f_K", "g"):
+        assert np.max(np.abs(gt[name] - ot[name])) <= 1e-9 * np.max(np.abs(ot[name])), name
+    rng = np.random.default_rng(5)
+    l1 = np.exp(rng.uniform(1, 9, 500)); l2 = np.exp(rng.uniform(1, 9, 500)); phi = rng.uniform(0, np.pi, 500)
+    ell = np.column_stack([l1, l2, np.sqrt(l1 ** 2 + l2 ** 2 + 2 * l1 * l2 * np.cos(phi))])
+    gb, ob = tabs.bkappa(ell, 0, tp.MODE_FP64), o.bkappa(ell)
+    ratio = np.max(ell, axis=1) / np.min(ell, axis=1)
+    assert np.all(np.abs(gb / ob - 1) < 1e-12 + 2e-14 * ratio)
+    th = W.to_rad(W.triples(W.map3_c0_radii()))[[0, 5, 12, 19]]
+    v, _, n = tabs.map3_batch(th, None, 1e-4, tp.MODE_FP64)
+    v32, _, _ = tabs.map3_batch(th, None, 1e-4, tp.MODE_FP32)
+    for j in range(th.shape[0]):
+        vo, _, no = o.map3(th[j], 1e-4)
+        assert abs(v[j] / vo - 1) < 1e-6, (j, v[j], vo)
+        assert abs(v32[j] / vo - 1) < 1e-3
+    with pytest.raises(RuntimeError):
+        tabs.map3_batch(th, None, 1e-4, tp.MODE_SURROGATE)

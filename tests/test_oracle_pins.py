@@ -218,3 +218,36 @@ def test_cubature_degree7_exact():
     exact = (2 ** 8 / 8) * 2 * 1 + (2 ** 4 / 4) * (2 / 3) * ((1.5 ** 3 - 0.5 ** 3) / 3) + (2 / 5) * 2 * (
         (1.5 ** 4 - 0.5 ** 4) / 4) + 2 * 2 * 1
     assert abs(v[0] / exact - 1) < 1e-13
+
+
+def _eh_table(orc, nk=300, kmin=1e-5, kmax=1e4):
+    k = np.logspace(np.log10(kmin), np.log10(kmax), nk)
+    return k, np.array([orc.scalar("linear_pk", kk) for kk in k])
+
+
+def test_given_pk_interpolation_semantics(orc):
+    """linear_pk with a given table (classed_version/bispectrum.cpp:787-806): the table value at a
+    key, the average at an interval's midpoint, linear in k between keys, the end values outside."""
+    k, p = _eh_table(orc, 120)
+    t = O.Oracle(W.fiducial(), W.fiducial_nz(), pk=(k, p))
+    assert t.scalar("linear_pk", k[0] / 3) == p[0]
+    assert t.scalar("linear_pk", k[-1] * 3) == p[-1]
+    assert t.scalar("linear_pk", k[-1]) == p[-1]
+    for j in (0, 17, 60, 118):
+        assert t.scalar("linear_pk", k[j]) == p[j]
+        mid = 0.5 * (k[j] + k[j + 1])
+        assert abs(t.scalar("linear_pk", mid) / (0.5 * (p[j] + p[j + 1])) - 1) < 1e-14
+        q = k[j] + 0.25 * (k[j + 1] - k[j])
+        assert abs(t.scalar("linear_pk", q) / (0.75 * p[j] + 0.25 * p[j + 1]) - 1) < 1e-14
+
+
+def test_given_pk_of_the_fit_reproduces_the_fit(orc):
+    """A fine table of the normalised Eisenstein & Hu P(k) gives the fit's tables and B_kappa to the
+    interpolation error: the given table is used as is (sigma8 does not rescale it again)."""
+    k, p = _eh_table(orc, 3000)
+    t = O.Oracle(W.fiducial(), W.fiducial_nz(), pk=(k, p))
+    a, b = orc.tables(), t.tables()
+    for name in ("r_sigma", "n_eff", "D1"):
+        assert np.max(np.abs(b[name][1:] / a[name][1:] - 1)) < 2e-4, name
+    ell = np.array([[100., 200., 250.], [1000., 1500., 900.], [3000., 300., 2800.]])
+    assert np.max(np.abs(t.bkappa(ell) / orc.bkappa(ell) - 1)) < 1e-3

@@ -67,6 +67,9 @@ _SIGS = {
     "tp_ctx_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "tp_cosmo_build": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(tp_cosmology), _dp, ctypes.c_int,
                                       ctypes.c_size_t, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)]),
+    "tp_cosmo_build_pk": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(tp_cosmology), _dp,
+                                         ctypes.c_int, ctypes.c_size_t, ctypes.c_double, _dp, _dp, ctypes.c_int,
+                                         ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
     "tp_cosmo_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "tp_cosmo_count": (ctypes.c_int, [ctypes.c_void_p, _ip, _ip]),
     "tp_cosmo_tables": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
@@ -205,7 +208,9 @@ class Cosmology:
 class CosmologyTables:
     """Device-resident tables of one or more cosmologies (tp_cosmo)."""
 
-    def __init__(self, cosmos, nz, n_z=None, z_max=3.0, ctx=None):
+    def __init__(self, cosmos, nz, n_z=None, z_max=3.0, ctx=None, pk=None):
+        """pk = (k_h, p_k): a tabulated linear P(k) (shared, or one row per cosmology) instead of the
+        Eisenstein & Hu fit, the reference's Pk_given input (classed_version/bispectrum.cpp:81-89)."""
         self.ctx = ctx or default_context()
         if isinstance(cosmos, Cosmology):
             cosmos = [cosmos]
@@ -219,8 +224,18 @@ class CosmologyTables:
             stride = n_z
         arr = (tp_cosmology * len(self.cosmos))(*[c.as_struct() for c in self.cosmos])
         h = ctypes.c_void_p()
-        _check(lib().tp_cosmo_build(self.ctx.h, len(self.cosmos), arr, _d(nz), n_z, stride, z_max, ctypes.byref(h)),
-               "tp_cosmo_build")
+        if pk is None:
+            _check(lib().tp_cosmo_build(self.ctx.h, len(self.cosmos), arr, _d(nz), n_z, stride, z_max,
+                                        ctypes.byref(h)), "tp_cosmo_build")
+        else:
+            kk = np.ascontiguousarray(pk[0], dtype=np.float64)
+            pp = np.ascontiguousarray(pk[1], dtype=np.float64)
+            if kk.shape != pp.shape:
+                raise ValueError("k and P(k) tables differ in shape")
+            nk = kk.shape[-1]
+            pstride = nk if kk.ndim == 2 else 0
+            _check(lib().tp_cosmo_build_pk(self.ctx.h, len(self.cosmos), arr, _d(nz), n_z, stride, z_max, _d(kk),
+                                           _d(pp), nk, pstride, ctypes.byref(h)), "tp_cosmo_build_pk")
         self.h = h
         self.n_z = n_z
         self.z_max = z_max

@@ -51,6 +51,7 @@ extern "C" int tp_census_map3_point(double *out)
     Cnt ev[sizeof(TpEHT<Cnt>) / sizeof(Cnt)];
     for (auto &v : ev) v = Cnt(0.7);
     E = *reinterpret_cast<TpEHT<Cnt> *>(ev);
+    E.tab = TpPkTable{};  // the census counts the Eisenstein & Hu form
     // per point: l3, the hoisted scale-free quantities, the three filters and the final product
     g_cnt = OpCounts();
     Cnt l1(300.), l2(500.), phi(1.0);
@@ -91,6 +92,7 @@ extern "C" int tp_census_gamma_term(double *out)
     Cnt ev[sizeof(TpEHT<Cnt>) / sizeof(Cnt)];
     for (auto &v : ev) v = Cnt(0.7);
     TpEHT<Cnt> E = *reinterpret_cast<TpEHT<Cnt> *>(ev);
+    E.tab = TpPkTable{};
     g_cnt = OpCounts();
     // shape: cos psi, sin psi, cos phi (3 calls), u3^2, sqrt (1 call), ln A_j (3 calls)
     Cnt cps = TpMath<Cnt>::ex(Cnt(0.3)), sps = TpMath<Cnt>::ex(Cnt(0.3)), cph = TpMath<Cnt>::ex(Cnt(0.3));

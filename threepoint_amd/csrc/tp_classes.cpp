@@ -30,6 +30,21 @@ BispectrumCalculator::BispectrumCalculator(cosmology *c, const std::vector<doubl
     set_cosmology(c);
 }
 
+BispectrumCalculator::BispectrumCalculator(cosmology *c, const std::vector<double> &n_of_z, int n_z, double zmax,
+                                           const std::map<double, double> &P_k)
+    : cosmo(c), z_max(zmax), n_of_z_(n_of_z), n_z_(n_z)
+{
+    // classed_version/bispectrum.cpp:81-89: the map's ascending keys are the table
+    if ((int)n_of_z.size() < n_z) throw std::runtime_error("BispectrumCalculator: n_of_z shorter than n_z");
+    if (P_k.size() < 2) throw std::runtime_error("BispectrumCalculator: P_k needs at least 2 entries");
+    for (const auto &kv : P_k) {
+        pk_k_.push_back(kv.first);
+        pk_p_.push_back(kv.second);
+    }
+    ctx_ = threepoint_default_context(0);
+    set_cosmology(c);
+}
+
 BispectrumCalculator::~BispectrumCalculator()
 {
     if (tables_) tp_cosmo_destroy(tables_);
@@ -41,7 +56,12 @@ void BispectrumCalculator::set_cosmology(cosmology *c)
     if (tables_) tp_cosmo_destroy(tables_);
     tables_ = nullptr;
     tp_cosmology p{c->h, c->sigma8, c->omb, c->ns, c->w, c->om, z_max};
-    check(tp_cosmo_build(ctx_, 1, &p, n_of_z_.data(), n_z_, 0, z_max, &tables_), "tp_cosmo_build");
+    if (pk_k_.empty())
+        check(tp_cosmo_build(ctx_, 1, &p, n_of_z_.data(), n_z_, 0, z_max, &tables_), "tp_cosmo_build");
+    else
+        check(tp_cosmo_build_pk(ctx_, 1, &p, n_of_z_.data(), n_z_, 0, z_max, pk_k_.data(), pk_p_.data(),
+                                (int)pk_k_.size(), 0, &tables_),
+              "tp_cosmo_build_pk");
     check(tp_cosmo_tables(tables_, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, &norm), "tp_cosmo_tables");
 }
 

@@ -1,6 +1,8 @@
 // tp_device.h -- device functions shared by the integrand, region, Gamma and
 // surrogate kernels.  See tp_map3.hip for the overview.
 #pragma once
+#include <type_traits>
+
 #include "tp_internal.h"
 
 namespace tpd {
@@ -28,6 +30,21 @@ __host__ __device__ __forceinline__ T tp_f2_tree(T k1, T k2, T k3)
     return T(5. / 7.) + T(0.5) * c12 * (k1 / k2 + k2 / k1) + T(2. / 7.) * c12 * c12;
 }
 
+// Tabulated linear P(k), classed_version/bispectrum.cpp:787-806: linear interpolation in k between
+// the bracketing entries (k_j <= k < k_j+1), the end values outside the table.
+__host__ __device__ __forceinline__ double tp_pk_table(const TpPkTable &t, double k)
+{
+    const int n = t.nk;
+    if (!(k >= t.k[0])) return t.p[0];
+    if (k >= t.k[n - 1]) return t.p[n - 1];
+    int c = (int)((log(k) - t.lnk0) * t.inv_dlnk);
+    c = c < 0 ? 0 : (c >= t.ng ? t.ng - 1 : c);
+    int j = t.idx[c];
+    while (j < n - 2 && t.k[j + 1] <= k) ++j;
+    const double diff = (k - t.k[j]) / (t.k[j + 1] - t.k[j]);
+    return diff * t.p[j + 1] + (1 - diff) * t.p[j];
+}
+
 // Eisenstein & Hu no-wiggle P_lin at k = q / r_sigma (classed_version/bispectrum.cpp:811-841).
 // The quotients of the reference are merged into two reciprocals:
 //   q_eff = k theta^2 / Gamma_eff, Gamma_eff = om h^2 (sqrt(alpha_nu) + (1 - sqrt(alpha_nu)) / (1 + x^4))
@@ -39,6 +56,8 @@ __host__ __device__ __forceinline__ T tp_f2_tree(T k1, T k2, T k3)
 template <class T>
 __host__ __device__ __forceinline__ T tp_plin_q(const TpNodeT<T> &N, const TpEHT<T> &E, T q, T lq)
 {
+    if constexpr (std::is_same<T, double>::value || std::is_same<T, float>::value)
+        if (E.tab.nk) return (T)tp_pk_table(E.tab, (double)q * (double)N.inv_rs);
     T kk = q * N.inv_rs * E.h;  // 1/Mpc
     T x = E.sh043 * kk;
     T x2 = x * x;

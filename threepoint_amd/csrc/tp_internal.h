@@ -19,6 +19,18 @@
 // Everything that depends on the cosmology only (Eisenstein & Hu constants,
 // sigma8 normalisation, Limber prefactor) is folded here once.
 // ---------------------------------------------------------------------------
+// A tabulated linear P(k) of one cosmology (the reference's Pk_given input,
+// classed_version/bispectrum.cpp:81-89, 785-806); nk == 0 selects Eisenstein & Hu.
+// idx[c] is the table interval at the start of cell c of a uniform ln k grid (minus half a
+// cell), so a lookup starts within a step or two of its interval.
+struct TpPkTable {
+    const double *k = nullptr;  // [nk] ascending, h/Mpc
+    const double *p = nullptr;  // [nk] (Mpc/h)^3
+    const int *idx = nullptr;   // [ng]
+    int nk = 0, ng = 0;
+    double lnk0 = 0, inv_dlnk = 0;
+};
+
 struct TpCosmoConst {
     double h, sigma8, omb, omc, ns, w, om, ow, zmax;
     double norm;          // sigma8 / sigma_TopHat(8 Mpc/h) with norm=1
@@ -27,6 +39,7 @@ struct TpCosmoConst {
     double limber_pref;   // 27/8 om^3 (H0/c)^5
     int n_z;
     int pad;
+    TpPkTable tab;        // tabulated P(k) or none
 };
 
 // ---------------------------------------------------------------------------
@@ -60,7 +73,7 @@ typedef TpNodeT<float> TpNode32;
 template <class T>
 struct TpEHT {
     T h, sqrt_alnu, one_m_sqrt_alnu, sh043, omh2, theta2, cL, three_ns, e_const;
-    T pad[7];
+    TpPkTable tab;  // when tab.nk > 0 the linear P(k) is this table instead of the fit
 };
 typedef TpEHT<double> TpEH64;
 typedef TpEHT<float> TpEH32;
@@ -81,6 +94,8 @@ struct tp_cosmo {
     TpEH32 *d_eh32 = nullptr;         // [ncosmo]
     TpNode32 *d_base32 = nullptr;     // [ncosmo][8]: 8-node FP32 Limber tables (surrogate baseline)
     int *d_err = nullptr;             // build error flag
+    void *d_pk = nullptr;             // tabulated P(k): k, p [ncosmo][nk] then idx [ncosmo][ng]
+    int pk_nk = 0;                    // 0: Eisenstein & Hu
     std::vector<TpCosmoConst> h_const;
 };
 

@@ -1456,6 +1456,7 @@ __global__ void k_surb_combine(size_t n, const double *Bb, double *y)
 extern "C" int tp_surrogate_bkappa(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *ell123,
                                    double *out)
 {
+    if (c && c->pk_nk) return tp_set_error(TP_EINVAL, "tp_surrogate_bkappa: tables from a tabulated P(k)");
     if (!ctx || !ctx->sur || !ctx->sur->trained) return tp_set_error(TP_EINVAL, "tp_surrogate_bkappa: no model");
     if (!c || icosmo < 0 || icosmo >= c->ncosmo) return tp_set_error(TP_EINVAL, "tp_surrogate_bkappa: bad cosmology");
     if (npts == 0) return 0;

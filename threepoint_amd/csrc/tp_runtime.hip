@@ -366,6 +366,9 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     if (mode < TP_MODE_FP64 || mode > TP_MODE_ANALYTIC) return tp_set_error(TP_EINVAL, "tp_map3_batch: bad mode");
     if (mode == TP_MODE_SURROGATE && !ctx->sur)
         return tp_set_error(TP_EINVAL, "tp_map3_batch: surrogate mode needs a trained or loaded surrogate");
+    if (mode == TP_MODE_SURROGATE && c->pk_nk)
+        return tp_set_error(TP_EINVAL, "tp_map3_batch: the surrogate is trained on the Eisenstein & Hu P(k); "
+                                       "tables built from a tabulated P(k) need an exact mode");
     if (c->device != ctx->device) return tp_set_error(TP_EINVAL, "tp_map3_batch: cosmology on another device");
     if (ntriples == 0) return 0;
     TP_HIP(hipSetDevice(ctx->device));

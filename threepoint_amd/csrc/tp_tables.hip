@@ -17,8 +17,11 @@
 //   k_growth        1 wave / (cosmo,bin)   : D1, r_sigma, n_eff, ncur
 //   k_nodes         1 thread / (cosmo,node): 96-node Limber tables (FP64 + FP32 copy)
 #include <cfloat>
+#include <cmath>
+#include <string>
+#include <vector>
 
-#include "tp_internal.h"
+#include "tp_device.h"
 
 namespace {
 
@@ -50,10 +53,16 @@ __device__ __forceinline__ double window(double x, int i)
     return x * x * (1 - x * x) * exp(-x * x);
 }
 
+// linear_pk (classed_version/bispectrum.cpp:785-809): the tabulated P(k) when given, else the fit
+__device__ __forceinline__ double pk_lin(const TpCosmoConst &C, double k, double norm)
+{
+    return C.tab.nk ? tp_pk_table(C.tab, k) : pk_eh(C, k, norm);
+}
+
 __device__ __forceinline__ double sigma_term(const TpCosmoConst &C, double k, double r, int j, double norm)
 {
     double w = window(k * r, j);
-    double kk = k * k * k * pk_eh(C, k, norm);
+    double kk = k * k * k * pk_lin(C, k, norm);
     return (j < 3) ? kk * w * w : kk * w;
 }
 
@@ -307,6 +316,23 @@ __global__ void k_cosmo_const(int ncosmo, const tp_cosmology *p, double z_max, i
     out[ic] = C;
 }
 
+// Points every cosmology's constants at its tabulated P(k) (one table of nk entries per cosmology).
+__global__ void k_set_pk(int ncosmo, TpCosmoConst *cc, const double *k, const double *p, const int *idx, int nk,
+                         int ng, const double *lnk0, const double *inv_dlnk)
+{
+    int ic = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ic >= ncosmo) return;
+    TpPkTable t;
+    t.k = k + (size_t)ic * nk;
+    t.p = p + (size_t)ic * nk;
+    t.idx = idx + (size_t)ic * ng;
+    t.nk = nk;
+    t.ng = ng;
+    t.lnk0 = lnk0[ic];
+    t.inv_dlnk = inv_dlnk[ic];
+    cc[ic].tab = t;
+}
+
 __global__ void __launch_bounds__(64) k_norm(int ncosmo, TpCosmoConst *cc)
 {
     int ic = blockIdx.x;
@@ -503,12 +529,19 @@ __global__ void k_nodes(int ncosmo, int n_z, const TpCosmoConst *cc, const doubl
         e.cL = C.eh_cL;
         e.three_ns = 3. + C.ns;
         e.e_const = 2.718281828;
-        for (int k = 0; k < 7; ++k) e.pad[k] = 0;
+        e.tab = C.tab;
         eh64[ic] = e;
         TpEH32 f;
-        const double *s2 = &e.h;
-        float *d2 = &f.h;
-        for (int k = 0; k < (int)(sizeof(TpEH64) / sizeof(double)); ++k) d2[k] = (float)s2[k];
+        f.h = (float)e.h;
+        f.sqrt_alnu = (float)e.sqrt_alnu;
+        f.one_m_sqrt_alnu = (float)e.one_m_sqrt_alnu;
+        f.sh043 = (float)e.sh043;
+        f.omh2 = (float)e.omh2;
+        f.theta2 = (float)e.theta2;
+        f.cL = (float)e.cL;
+        f.three_ns = (float)e.three_ns;
+        f.e_const = (float)e.e_const;
+        f.tab = C.tab;
         eh32[ic] = f;
     }
 }
@@ -559,11 +592,44 @@ static int upload_gl96(int device)
     return 0;
 }
 
-extern "C" int tp_cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *params, const double *nz, int n_z,
-                              size_t nz_stride, double z_max, tp_cosmo **out)
+static int cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *params, const double *nz, int n_z,
+                       size_t nz_stride, double z_max, const double *pk_k, const double *pk_p, int nk,
+                       size_t pk_stride, tp_cosmo **out)
 {
     if (!ctx || !params || !nz || !out || ncosmo <= 0 || n_z < 2 || n_z > TP_NZ_MAX || !(z_max > 0))
         return tp_set_error(TP_EINVAL, "tp_cosmo_build: bad arguments");
+    // tabulated P(k): strictly increasing positive k, finite P (the reference keeps a std::map)
+    std::vector<double> hk, hp, hl0, hinv;
+    std::vector<int> hidx;
+    const int ng = nk > 0 ? std::max(64, 4 * nk) : 0;
+    if (nk) {
+        if (nk < 2 || !pk_k || !pk_p) return tp_set_error(TP_EINVAL, "tp_cosmo_build_pk: need at least 2 P(k) entries");
+        hk.resize((size_t)ncosmo * nk);
+        hp.resize((size_t)ncosmo * nk);
+        hidx.resize((size_t)ncosmo * ng);
+        hl0.resize(ncosmo);
+        hinv.resize(ncosmo);
+        for (int i = 0; i < ncosmo; ++i) {
+            const double *ks = pk_k + pk_stride * i, *ps = pk_p + pk_stride * i;
+            for (int j = 0; j < nk; ++j) {
+                if (!(ks[j] > 0) || !std::isfinite(ks[j]) || !std::isfinite(ps[j]) || (j && !(ks[j] > ks[j - 1])))
+                    return tp_set_error(TP_EINVAL, "tp_cosmo_build_pk: k must be positive and strictly increasing, "
+                                                   "P finite (cosmology " + std::to_string(i) + ")");
+                hk[(size_t)i * nk + j] = ks[j];
+                hp[(size_t)i * nk + j] = ps[j];
+            }
+            const double l0 = std::log(ks[0]), l1 = std::log(ks[nk - 1]);
+            const double inv = ng / (l1 - l0);
+            hl0[i] = l0;
+            hinv[i] = inv;
+            int j = 0;
+            for (int c = 0; c < ng; ++c) {  // interval at half a cell before the cell start
+                const double kc = std::exp(l0 + (c - 0.5) / inv);
+                while (j < nk - 2 && ks[j + 1] <= kc) ++j;
+                hidx[(size_t)i * ng + c] = j;
+            }
+        }
+    }
     for (int i = 0; i < ncosmo; ++i) {
         const tp_cosmology &p = params[i];
         if (!(p.om > 0) || !(p.h > 0) || !(p.sigma8 > 0) || !(p.omb >= 0) || p.omb >= p.om)
@@ -597,6 +663,23 @@ extern "C" int tp_cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *param
     TPC(hipMalloc(&c->d_base32, sizeof(TpNode32) * TP_NBASE * (size_t)ncosmo));
     TPC(hipMalloc(&c->d_err, sizeof(int)));
     TPC(hipMemsetAsync(c->d_err, 0, sizeof(int), s));
+    double *d_pkk = nullptr, *d_pkp = nullptr, *d_l0 = nullptr, *d_inv = nullptr;
+    int *d_idx = nullptr;
+    if (nk) {
+        const size_t nkt = (size_t)ncosmo * nk, ngt = (size_t)ncosmo * ng;
+        TPC(hipMalloc(&c->d_pk, (2 * nkt + 2 * (size_t)ncosmo) * sizeof(double) + ngt * sizeof(int)));
+        d_pkk = (double *)c->d_pk;
+        d_pkp = d_pkk + nkt;
+        d_l0 = d_pkp + nkt;
+        d_inv = d_l0 + ncosmo;
+        d_idx = (int *)(d_inv + ncosmo);
+        c->pk_nk = nk;
+        TPC(hipMemcpy(d_pkk, hk.data(), nkt * sizeof(double), hipMemcpyHostToDevice));
+        TPC(hipMemcpy(d_pkp, hp.data(), nkt * sizeof(double), hipMemcpyHostToDevice));
+        TPC(hipMemcpy(d_l0, hl0.data(), ncosmo * sizeof(double), hipMemcpyHostToDevice));
+        TPC(hipMemcpy(d_inv, hinv.data(), ncosmo * sizeof(double), hipMemcpyHostToDevice));
+        TPC(hipMemcpy(d_idx, hidx.data(), ngt * sizeof(int), hipMemcpyHostToDevice));
+    }
     tp_cosmology *d_params = nullptr;
     TPC(hipMalloc(&d_params, sizeof(tp_cosmology) * ncosmo));
     TPC(hipMemcpyAsync(d_params, params, sizeof(tp_cosmology) * ncosmo, hipMemcpyHostToDevice, s));
@@ -612,6 +695,9 @@ extern "C" int tp_cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *param
     int nt = ncosmo * n_z;
     hipLaunchKernelGGL(k_cosmo_const, dim3((ncosmo + 63) / 64), dim3(64), 0, s, ncosmo, d_params, z_max, n_z,
                        c->d_const);
+    if (nk)
+        hipLaunchKernelGGL(k_set_pk, dim3((ncosmo + 63) / 64), dim3(64), 0, s, ncosmo, c->d_const, d_pkk, d_pkp,
+                           d_idx, nk, ng, d_l0, d_inv);
     hipLaunchKernelGGL(k_norm, dim3(ncosmo), dim3(64), 0, s, ncosmo, c->d_const);
     hipLaunchKernelGGL(k_fK, dim3((nt + 127) / 128), dim3(128), 0, s, ncosmo, n_z, dz, c->d_const, c->d_tab,
                        c->d_err);
@@ -633,6 +719,20 @@ extern "C" int tp_cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *param
 #undef TPC
 }
 
+extern "C" int tp_cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *params, const double *nz, int n_z,
+                              size_t nz_stride, double z_max, tp_cosmo **out)
+{
+    return cosmo_build(ctx, ncosmo, params, nz, n_z, nz_stride, z_max, nullptr, nullptr, 0, 0, out);
+}
+
+extern "C" int tp_cosmo_build_pk(tp_ctx *ctx, int ncosmo, const tp_cosmology *params, const double *nz, int n_z,
+                                 size_t nz_stride, double z_max, const double *k_h, const double *p_k, int nk,
+                                 size_t pk_stride, tp_cosmo **out)
+{
+    if (nk < 2) return tp_set_error(TP_EINVAL, "tp_cosmo_build_pk: need at least 2 P(k) entries");
+    return cosmo_build(ctx, ncosmo, params, nz, n_z, nz_stride, z_max, k_h, p_k, nk, pk_stride, out);
+}
+
 extern "C" int tp_cosmo_destroy(tp_cosmo *c)
 {
     if (!c) return 0;
@@ -646,6 +746,7 @@ extern "C" int tp_cosmo_destroy(tp_cosmo *c)
     (void)hipFree(c->d_eh32);
     (void)hipFree(c->d_base32);
     (void)hipFree(c->d_err);
+    if (c->d_pk) (void)hipFree(c->d_pk);
     delete c;
     return 0;
 }
