@@ -68,10 +68,22 @@ def build_apps(verbose=False):
     return built
 
 
+COSMOSIS_MODULE = os.path.join(LIBDIR, "threepoint_cosmosis.so")
+
+
+def build_cosmosis_module(verbose=False):
+    """The cosmosis module (setup/execute/cleanup); cosmosis provides the datablock symbols at load."""
+    src = os.path.join(CSRC, "cosmosis", "threepoint_module.cpp")
+    cmd = [HIPCC, "-shared"] + COMMON + [src, "-L" + LIBDIR, "-lthreepoint_amd", "-Wl,-rpath,$ORIGIN",
+                                          "-o", COSMOSIS_MODULE]
+    _run(cmd)
+    return COSMOSIS_MODULE
+
+
 def build(verbose=False):
     lib = build_library(verbose)
     apps = build_apps(verbose)
-    return [lib] + apps
+    return [lib] + apps + [build_cosmosis_module(verbose)]
 
 
 if __name__ == "__main__":
