@@ -176,6 +176,44 @@ int tp_gamma_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, int which, double 
 int tp_gamma_batch(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t ntri, const double *x123,
                    double reltol, double *out_re, double *out_im, long long *neval);
 
+/* ------------------------------------- convergence power spectrum and <Map^2> */
+/* flags of the power-spectrum entry points */
+enum {
+    TP_PK_LINEAR = 1, /* D1^2 P_lin instead of revised Halofit (limber_integrand_power_spectrum's
+                         nonlinear = false, classed_version/bispectrum.cpp:569-572)               */
+    TP_PK_T17 = 2     /* the cuda_version's T17 correction of the Limber integrand and of <Map^2>
+                         (cuda_version/bispectrum.cu:971-979, apertureStatistics.cu:39-46)        */
+};
+/* Replaces BispectrumCalculator::P_k_nonlinear(k, z) (classed_version/bispectrum.cpp:288-367) and,
+ * with TP_PK_LINEAR, linear_pk_at_z (:369-380): kz is npts*2 (k [h/Mpc], z), out npts values
+ * [(Mpc/h)^3].  z outside [0, z_max] reads the nearest table interval (the reference reads past
+ * its tables there). */
+int tp_pk_at_z_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *kz, int flags,
+                    double *out);
+/* Replaces BispectrumCalculator::limber_integrand_power_spectrum(ell, z, nonlinear)
+ * (classed_version/bispectrum.cpp:543-577): ellz is npts*2 (ell, z). */
+int tp_limber_pkappa_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *ellz,
+                          int flags, double *out);
+/* Replaces BispectrumCalculator::Pell (classed_version/bispectrum.cpp:1048-1059, GQ96_of_Pk :1034-1046):
+ * out[i] = 96-point Gauss-Legendre Limber integral of P_NL over [0, z_max] at ell[i] plus the shape-noise
+ * power 0.5 sigma^2 / n_gal (n_gal in rad^-2; the reference's members sigma and n); ell = 0 gives the
+ * shape noise alone. */
+int tp_pell(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t n, const double *ell, int flags, double sigma,
+            double n_gal, double *out);
+/* Replaces BispectrumCalculator::convergence_power_spectrum (classed_version/bispectrum.cpp:382-388)
+ * for n multipoles at once: hcubature over z in [0, z_max] (Gauss-Kronrod, the cubature's 1-D rule),
+ * relative tolerance reltol (the reference's 1e-6), cosmology icosmo[i] (NULL: all 0). */
+int tp_pkappa_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo, const double *ell,
+                    double reltol, int flags, double *out, double *err, long long *neval);
+/* Replaces Map2(theta, ...) of the cuda_version (apertureStatistics.cu:110-136) for one source bin
+ * and n aperture radii at once: 2-D hcubature over ell in [1, 10/theta], z in [0, z_max] of
+ * ell uHat(ell theta)^2 P-Limber-integrand, plus, when sigma != 0, the shape-noise term of
+ * apertureStatistics.cu:43-46 (ngal_arcmin2 in arcmin^-2); result / 2 pi.  reltol is the
+ * reference's 1e-5.  flags: TP_PK_T17 applies the T17 factors (the cuda_version's default). */
+int tp_map2_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo, const double *theta,
+                  double reltol, int flags, double sigma, double ngal_arcmin2, double *out, double *err,
+                  long long *neval);
+
 /* ------------------------------------------------------------- cubature */
 /* Replaces hcubature_v (classed_version/cubature.h:90-94, hcubature.cpp:1059-1068):
  * same arguments, same callback layout x[i*ndim+j], fval[i*fdim+k], same return

@@ -8,8 +8,10 @@
  *                           classed_version/helper.hpp (helper.cpp:31-200)
  *   configGamma, read_gamma_config
  *                           cuda_version/helpers.cuh (helpers.cu:242-330)
+ *   covarianceParameters    cuda_version/cosmology.cuh:44-57
  *   BispectrumCalculator    classed_version/bispectrum.hpp:113-509
- *   ApertureStatistics      classed_version/apertureStatistics.hpp:18-322
+ *   ApertureStatistics      classed_version/apertureStatistics.hpp:18-322 (+ the cuda_version's Map2,
+ *                           apertureStatistics.cu:110-136)
  *   GammaCalculator         classed_version/gamma.hpp:17-495
  * All numerics run on the GPU through the C ABI; objects hold non-owning
  * pointers exactly like the reference (the caller keeps cosmology and
@@ -48,6 +50,17 @@ struct configGamma {
 void read_gamma_config(const std::string &fn, configGamma &config);
 std::ostream &operator<<(std::ostream &out, const configGamma &config);
 
+/* Survey parameters of the covariance executables, cuda_version/cosmology.cuh:44-57 (cosmology.cu:92-179):
+ * thetaMax [rad], galaxy_density [rad^-2], area [rad^2] after reading. */
+class covarianceParameters {
+  public:
+    double thetaMax = 0, shapenoise_sigma = 0, galaxy_density = 0, thetaMax_smaller = 0, area = 0;
+    bool shapenoiseOnly = false;
+    covarianceParameters() {}
+    explicit covarianceParameters(const std::string &fn);  // throws std::runtime_error on bad files
+};
+std::ostream &operator<<(std::ostream &out, const covarianceParameters &covPar);
+
 struct ell_params {
     double ell1, ell2, ell3;
 };
@@ -77,6 +90,19 @@ class BispectrumCalculator {
 
     /* batched forms (one GPU launch) */
     void bkappa(const std::vector<double> &ell123, std::vector<double> &out, int mode = TP_MODE_FP64);
+
+    /* second-order statistics, classed_version/bispectrum.hpp:272-299, 505-508 */
+    double sigma = 0.3;                                        // shape noise (bispectrum.hpp:120)
+    double n = 4096 * 4096 / 10 / 10 * 3.14159265358979323846 * 3.14159265358979323846 / 180 / 180;
+                                                               // galaxy density [rad^-2] (:121)
+    double P_k_nonlinear(double k, double z);
+    double linear_pk_at_z(double k, double z);
+    double convergence_power_spectrum(double ell);
+    double limber_integrand_power_spectrum(double ell, double z, bool nonlinear);
+    double Pell(double ell);
+    /* batched forms: many multipoles in one call */
+    std::vector<double> convergence_power_spectrum(const std::vector<double> &ell);
+    std::vector<double> Pell(const std::vector<double> &ell);
 
     tp_ctx *ctx() const { return ctx_; }
     const tp_cosmo *tables() const { return tables_; }
@@ -116,6 +142,14 @@ class ApertureStatistics {
     double MapMapMap(const std::vector<double> &thetas);
     /* many triples in one batched run (same decision sequence per triple) */
     std::vector<double> MapMapMap(const std::vector<std::array<double, 3>> &thetas);
+
+    /* <Map^2>(theta) of one source bin, the cuda_version's Map2 (apertureStatistics.cu:110-136):
+     * shape noise sigma_epsilon with ngal [arcmin^-2] when sigma_epsilon != 0; t17 applies the T17
+     * factors of that version (its default) */
+    double Map2(double theta, double sigma_epsilon = 0, double ngal = 1, bool t17 = false);
+    std::vector<double> Map2(const std::vector<double> &thetas, double sigma_epsilon = 0, double ngal = 1,
+                             bool t17 = false);
+    double reltol_map2 = 1e-5; /* apertureStatistics.cu:133 */
 };
 
 class GammaCalculator {

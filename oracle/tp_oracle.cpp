@@ -10,6 +10,9 @@
  *   - <Map^3> integrand:     CV/apertureStatistics.cpp:10-28, 254-279, 321-353
  *   - adaptive cubature:     CV/hcubature.cpp:115-1068, CV/converged.h
  *   - Gamma_i (Ogata):       CV/gamma.cpp:6-81, 121-219, 291-377, 462-563, 624-643
+ *   - P_kappa, Halofit:      CV/bispectrum.cpp:288-417, 543-577, 1022-1059
+ *   - <Map^2>, T17 factor:   GV/apertureStatistics.cu:29-136, GV/bispectrum.cu:951-982
+ *                            ("GV" = cuda_version/)
  *   - file readers:          CV/helper.cpp:31-96, CV/cosmology.cpp:10-72
  * GSL routines the reference calls are replaced by:
  *   - orc_qag15: QUADPACK QAG with the 15-point Gauss-Kronrod rule (Piessens et
@@ -466,6 +469,68 @@ struct GMRule {
     }
 };
 
+/* 15-point Gauss-Kronrod rule of one-dimensional regions (CV/hcubature.cpp:620-765, the rule the
+   driver takes for dim = 1).  Point order per region: center, the three Gauss pairs (+-xgk[1, 3, 5]),
+   the four Kronrod pairs (+-xgk[0, 2, 4, 6]); sums accumulate in that order.  The error estimate is
+   QUADPACK's rescaled |K15 - G7|. */
+struct GK15Rule {
+    unsigned dim = 1, fdim, npts = 15;
+    std::vector<double> pts, vals;
+    explicit GK15Rule(unsigned f) : fdim(f) {}
+    int eval(orc_integrand_v f, void *fdata, std::vector<Region> &R, size_t nR)
+    {
+        pts.assign(nR * npts, 0.0);
+        vals.assign(nR * npts * fdim, 0.0);
+        size_t n = 0;
+        for (size_t iR = 0; iR < nR; ++iR) {
+            const double c = R[iR].c[0], hw = R[iR].hw[0];
+            pts[n++] = c;
+            for (int j = 1; j < 7; j += 2) {
+                pts[n++] = c - hw * XGK15[j];
+                pts[n++] = c + hw * XGK15[j];
+            }
+            for (int j = 0; j < 8; j += 2) {
+                pts[n++] = c - hw * XGK15[j];
+                pts[n++] = c + hw * XGK15[j];
+            }
+            R[iR].split = 0;
+        }
+        if (f(1, nR * npts, pts.data(), fdata, fdim, vals.data())) return 1;
+        for (size_t iR = 0; iR < nR; ++iR) {
+            const double hw = R[iR].hw[0];
+            for (unsigned k = 0; k < fdim; ++k) {
+                const double *v = vals.data() + iR * npts * fdim + k;
+                auto V = [&](unsigned i) { return v[fdim * i]; };
+                double g = V(0) * WG7[3], kr = V(0) * WGK15[7], ra = std::fabs(kr);
+                unsigned i = 1;
+                for (int j = 0; j < 3; ++j, i += 2) {
+                    double s = V(i) + V(i + 1);
+                    g += WG7[j] * s;
+                    kr += WGK15[2 * j + 1] * s;
+                    ra += WGK15[2 * j + 1] * (std::fabs(V(i)) + std::fabs(V(i + 1)));
+                }
+                for (int j = 0; j < 4; ++j, i += 2) {
+                    kr += WGK15[2 * j] * (V(i) + V(i + 1));
+                    ra += WGK15[2 * j] * (std::fabs(V(i)) + std::fabs(V(i + 1)));
+                }
+                R[iR].val[k] = kr * hw;
+                double mean = kr * 0.5, rasc = WGK15[7] * std::fabs(V(0) - mean);
+                i = 1;
+                for (int j = 0; j < 3; ++j, i += 2)
+                    rasc += WGK15[2 * j + 1] * (std::fabs(V(i) - mean) + std::fabs(V(i + 1) - mean));
+                for (int j = 0; j < 4; ++j, i += 2)
+                    rasc += WGK15[2 * j] * (std::fabs(V(i) - mean) + std::fabs(V(i + 1) - mean));
+                R[iR].err[k] = rescale_error((kr - g) * hw, ra * hw, rasc * hw);
+            }
+            double em = 0;
+            for (unsigned k = 0; k < fdim; ++k)
+                if (R[iR].err[k] > em) em = R[iR].err[k];
+            R[iR].errmax = em;
+        }
+        return 0;
+    }
+};
+
 /* binary max-heap keyed on errmax, with running sums (CV/hcubature.cpp:772-890) */
 struct RegionHeap {
     std::vector<Region> items;
@@ -511,17 +576,11 @@ struct RegionHeap {
 
 thread_local long long g_last_neval = 0, g_last_nregions = 0;
 
-int hcubature_v_impl(unsigned fdim, orc_integrand_v f, void *fdata, unsigned dim,
-                     const double *xmin, const double *xmax, size_t maxEval, double reqAbs,
-                     double reqRel, int norm, double *val, double *err)
+template <class Rule>
+int hcubature_run(Rule &rule, unsigned fdim, orc_integrand_v f, void *fdata, unsigned dim, const double *xmin,
+                  const double *xmax, size_t maxEval, double reqAbs, double reqRel, int norm, double *val,
+                  double *err)
 { /* CV/hcubature.cpp:904-1068 (parallel / Gladwell variant used by hcubature_v) */
-    g_last_neval = 0;
-    g_last_nregions = 0;
-    if (fdim == 0) return 0;
-    if (dim < 2) return 1; /* the oracle covers the dim>=2 Genz-Malik path only */
-    if (fdim <= 1) norm = ORC_ERROR_INDIVIDUAL;
-    if (norm < 0 || norm > ORC_ERROR_LINF) return 1;
-    GMRule rule(dim, fdim);
     RegionHeap heap(fdim);
     std::vector<Region> R(1);
     R[0].c.resize(dim);
@@ -576,6 +635,24 @@ int hcubature_v_impl(unsigned fdim, orc_integrand_v f, void *fdata, unsigned dim
     g_last_neval = (long long)numEval;
     g_last_nregions = nreg;
     return 0;
+}
+
+int hcubature_v_impl(unsigned fdim, orc_integrand_v f, void *fdata, unsigned dim,
+                     const double *xmin, const double *xmax, size_t maxEval, double reqAbs,
+                     double reqRel, int norm, double *val, double *err)
+{ /* CV/hcubature.cpp:1030-1047: Gauss-Kronrod for dim = 1, Genz-Malik otherwise */
+    g_last_neval = 0;
+    g_last_nregions = 0;
+    if (fdim == 0) return 0;
+    if (dim < 1) return 1;
+    if (fdim <= 1) norm = ORC_ERROR_INDIVIDUAL;
+    if (norm < 0 || norm > ORC_ERROR_LINF) return 1;
+    if (dim == 1) {
+        GK15Rule rule(fdim);
+        return hcubature_run(rule, fdim, f, fdata, dim, xmin, xmax, maxEval, reqAbs, reqRel, norm, val, err);
+    }
+    GMRule rule(dim, fdim);
+    return hcubature_run(rule, fdim, f, fdata, dim, xmin, xmax, maxEval, reqAbs, reqRel, norm, val, err);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1100,6 +1177,114 @@ struct Calc {
         return std::sin(2 * psi) * (i1 + i2 + i3);
     }
 
+    /* ---------------- convergence power spectrum and <Map^2> ---------------- */
+    double om_m_of_z(double z) const
+    { /* CV/bispectrum.cpp:1022-1026 (no w: the flat-universe form) */
+        double aa = 1. / (1 + z);
+        return cosmo.om / (cosmo.om + aa * (aa * aa * cosmo.ow + (1 - cosmo.om - cosmo.ow)));
+    }
+    double om_v_of_z(double z) const
+    { /* CV/bispectrum.cpp:1028-1032 */
+        double aa = 1. / (1 + z);
+        return cosmo.ow * std::pow(aa, 3) / (cosmo.om + aa * (aa * aa * cosmo.ow + (1 - cosmo.om - cosmo.ow)));
+    }
+
+    double P_k_nonlinear(double k, double z) const
+    { /* CV/bispectrum.cpp:288-367: revised Halofit (Takahashi et al. 2012) with the tabulated
+         D1, r_sigma, n_eff and n_cur; the reference's flatness check cannot fail here (ow = 1 - om) */
+        int idx;
+        double didx;
+        z_index(z, &idx, &didx);
+        double r_sigma, n_eff, D1v;
+        coeffs(idx, didx, &D1v, &r_sigma, &n_eff);
+        double nsqr = n_eff * n_eff;
+        double nc = ncur[idx] * (1 - didx) + ncur[idx + 1] * didx;
+        double om_m = om_m_of_z(z), om_v = om_v_of_z(z);
+        double f1a = std::pow(om_m, -0.0732), f2a = std::pow(om_m, -0.1423), f3a = std::pow(om_m, 0.0725);
+        double f1b = std::pow(om_m, -0.0307), f2b = std::pow(om_m, -0.0585), f3b = std::pow(om_m, 0.0743);
+        double frac = om_v / (1. - om_m);
+        double f1 = frac * f1b + (1 - frac) * f1a;
+        double f2 = frac * f2b + (1 - frac) * f2a;
+        double f3 = frac * f3b + (1 - frac) * f3a;
+        double a = 1.5222 + 2.8553 * n_eff + 2.3706 * nsqr + 0.9903 * n_eff * nsqr + 0.2250 * nsqr * nsqr -
+                   0.6038 * nc + 0.1749 * om_v * (1.0 + cosmo.w);
+        a = std::pow(10.0, a);
+        double b = std::pow(10.0, -0.5642 + 0.5864 * n_eff + 0.5716 * nsqr - 1.5474 * nc +
+                                      0.2279 * om_v * (1.0 + cosmo.w));
+        double c = std::pow(10.0, 0.3698 + 2.0404 * n_eff + 0.8161 * nsqr + 0.5869 * nc);
+        double gam = 0.1971 - 0.0843 * n_eff + 0.8460 * nc;
+        double alpha = std::fabs(6.0835 + 1.3373 * n_eff - 0.1959 * nsqr - 5.5274 * nc);
+        double beta = 2.0379 - 0.7354 * n_eff + 0.3157 * nsqr + 1.2490 * n_eff * nsqr + 0.3980 * nsqr * nsqr -
+                      0.1682 * nc;
+        double xnu = std::pow(10.0, 5.2105 + 3.6902 * n_eff);
+        double plin = linear_pk(k) * D1v * D1v * k * k * k / (2 * M_PI * M_PI);
+        double y = k * r_sigma, ysqr = y * y;
+        double ph = a * std::pow(y, f1 * 3) / (1 + b * std::pow(y, f2) + std::pow(f3 * c * y, 3 - gam));
+        ph = ph / (1 + xnu / ysqr);
+        double pq = plin * std::pow(1 + plin, beta) / (1 + plin * alpha) * std::exp(-y / 4.0 - ysqr / 8.0);
+        return 2 * M_PI * M_PI * (pq + ph) / (k * k * k);
+    }
+
+    double linear_pk_at_z(double k, double z) const
+    { /* CV/bispectrum.cpp:369-380 */
+        int idx;
+        double didx;
+        z_index(z, &idx, &didx);
+        double r_sigma, n_eff, D1v;
+        coeffs(idx, didx, &D1v, &r_sigma, &n_eff);
+        return D1v * D1v * linear_pk(k);
+    }
+
+    static double t17_correction(double k)
+    { /* GV/bispectrum.cu:971-979: the ratio fitted to the T17 simulations */
+        const double c1 = 9.5171e-4, c2 = 5.1543e-3, a1 = 1.3063, a2 = 1.1475, a3 = 0.62793;
+        return std::pow(1 + c1 * std::pow(k, -1. * a1), a1) / std::pow(1 + c2 * std::pow(k, -1. * a2), a3);
+    }
+
+    double limber_integrand_power_spectrum(double ell, double z, bool nonlinear, bool t17 = false) const
+    { /* CV/bispectrum.cpp:543-577 (GV/bispectrum.cu:951-982 for the T17 factor); the reference's
+         f_K = 0 exit cannot trigger for z >= 1e-5 */
+        if (z < 1e-5) return 0;
+        int idx;
+        double didx;
+        z_index(z, &idx, &didx);
+        double g_value = g[idx] * (1 - didx) + g[idx + 1] * didx;
+        double f_K_value = fK[idx] * (1 - didx) + fK[idx + 1] * didx;
+        double prefactor = 9. / 4. * H0_over_c * H0_over_c * H0_over_c * cosmo.om * cosmo.om * (1. + z) * (1. + z) *
+                           g_value * g_value / E(z);
+        double k = ell / f_K_value;
+        double corr = t17 ? t17_correction(k) : 1.0;
+        return corr * prefactor * (nonlinear ? P_k_nonlinear(k, z) : linear_pk_at_z(k, z));
+    }
+
+    double GQ96_of_Pk(double a, double b, double ell, bool nonlinear = true, bool t17 = false) const
+    { /* CV/bispectrum.cpp:1034-1046 */
+        const GL96 &G = gl96();
+        double cx = (a + b) / 2, dx = (b - a) / 2, q = 0;
+        for (int i = 0; i < 48; i++)
+            q += G.W[i] * (limber_integrand_power_spectrum(ell, cx - dx * G.A[i], nonlinear, t17) +
+                           limber_integrand_power_spectrum(ell, cx + dx * G.A[i], nonlinear, t17));
+        return q * dx;
+    }
+
+    double Pell(double ell, double sigma, double n_gal, bool t17 = false) const
+    { /* CV/bispectrum.cpp:1048-1059: Limber-integrated P(k) plus the shape-noise power 0.5 sigma^2 / n */
+        double P_shapenoise = 0.5 * sigma * sigma / n_gal;
+        if (ell == 0) return P_shapenoise;
+        return GQ96_of_Pk(0, z_max, ell, true, t17) + P_shapenoise;
+    }
+
+    double map2_integrand(double ell, double z, double theta, bool nonlinear, bool t17, double sigma,
+                          double ngal_arcmin2) const
+    { /* GV/apertureStatistics.cu:29-57 for one source bin; sigma = 0 drops the shape-noise term */
+        double u = uHat(ell * theta);
+        double v = ell * std::pow(u, 2) * limber_integrand_power_spectrum(ell, z, nonlinear, t17);
+        if (t17) v /= 1 + std::pow(ell / (1.6 * 8192.), 2);
+        if (sigma != 0)
+            v += ell * std::pow(u * sigma, 2) * 0.5 / ngal_arcmin2 * std::pow(2.9088820866e-4, 2) / z_max;
+        return v;
+    }
+
     static double one_rotation_angle_otc(double x1, double x2, double x3)
     { /* CV/gamma.cpp:624-633 */
         double psi3 = interior_angle(x1, x2, x3);
@@ -1152,6 +1337,31 @@ int gamma_cb(unsigned ndim, size_t npts, const double *x, void *fdata, unsigned 
         fval[i * 2] = t.real();
         fval[i * 2 + 1] = t.imag();
     }
+    return 0;
+}
+
+struct PowerCtx {
+    const Calc *c;
+    double ell;   // P_kappa: the multipole; <Map^2>: the aperture radius [rad]
+    bool nonlinear, t17;
+    double sigma, ngal;
+};
+
+int pkappa_cb(unsigned ndim, size_t npts, const double *x, void *fdata, unsigned fdim, double *fval)
+{ /* CV/bispectrum.cpp:390-417 */
+    PowerCtx *p = (PowerCtx *)fdata;
+    if (fdim != 1 || ndim != 1) return 1;
+    for (size_t i = 0; i < npts; i++)
+        fval[i] = p->c->limber_integrand_power_spectrum(p->ell, x[i], p->nonlinear, p->t17);
+    return 0;
+}
+
+int map2_cb(unsigned ndim, size_t npts, const double *x, void *fdata, unsigned fdim, double *fval)
+{ /* GV/apertureStatistics.cu:59-108: points (ell, z) */
+    PowerCtx *p = (PowerCtx *)fdata;
+    if (fdim != 1 || ndim != 2) return 1;
+    for (size_t i = 0; i < npts; i++)
+        fval[i] = p->c->map2_integrand(x[2 * i], x[2 * i + 1], p->ell, p->nonlinear, p->t17, p->sigma, p->ngal);
     return 0;
 }
 
@@ -1410,6 +1620,57 @@ int orc_gamma(void *h, int which, double x1, double x2, double x3, double reltol
     r = r * std::exp(std::complex<double>(0, -2.) * (a1 + a2 + a3));
     *re = r.real();
     *im = r.imag();
+    if (neval) *neval = g_last_neval;
+    return rc;
+}
+
+double orc_pk_nonlinear(void *h, double k, double z) { return ((Calc *)h)->P_k_nonlinear(k, z); }
+double orc_linear_pk_at_z(void *h, double k, double z) { return ((Calc *)h)->linear_pk_at_z(k, z); }
+double orc_om_m_of_z(void *h, double z) { return ((Calc *)h)->om_m_of_z(z); }
+double orc_om_v_of_z(void *h, double z) { return ((Calc *)h)->om_v_of_z(z); }
+double orc_limber_pkappa(void *h, double ell, double z, int flags)
+{
+    return ((Calc *)h)->limber_integrand_power_spectrum(ell, z, !(flags & 1), (flags & 2) != 0);
+}
+double orc_gq96_pkappa(void *h, double ell, int flags)
+{
+    Calc *c = (Calc *)h;
+    return c->GQ96_of_Pk(0, c->z_max, ell, !(flags & 1), (flags & 2) != 0);
+}
+double orc_pell(void *h, double ell, double sigma, double n_gal, int flags)
+{ /* CV's Pell is nonlinear only; flags & 1 gives the linear spectrum for comparison */
+    Calc *c = (Calc *)h;
+    double P_shapenoise = 0.5 * sigma * sigma / n_gal;
+    if (!(flags & 1)) return c->Pell(ell, sigma, n_gal, (flags & 2) != 0);
+    if (ell == 0) return P_shapenoise;
+    return c->GQ96_of_Pk(0, c->z_max, ell, false, (flags & 2) != 0) + P_shapenoise;
+}
+double orc_map2_integrand(void *h, double ell, double z, double theta, int flags, double sigma, double ngal)
+{
+    return ((Calc *)h)->map2_integrand(ell, z, theta, !(flags & 1), (flags & 2) != 0, sigma, ngal);
+}
+
+int orc_convergence_power_spectrum(void *h, double ell, double reltol, int flags, double *val, double *err,
+                                   long long *neval)
+{ /* CV/bispectrum.cpp:382-388: 1-D hcubature over z in [0, z_max] (GK15 rule); flags as orc_pell */
+    Calc *c = (Calc *)h;
+    PowerCtx p{c, ell, !(flags & 1), (flags & 2) != 0, 0, 1};
+    double vmin[1] = {0}, vmax[1] = {c->z_max};
+    int rc = hcubature_v_impl(1, pkappa_cb, &p, 1, vmin, vmax, 0, 0, reltol, ORC_ERROR_L1, val, err);
+    if (neval) *neval = g_last_neval;
+    return rc;
+}
+
+int orc_map2(void *h, double theta, double reltol, int flags, double sigma, double ngal_arcmin2, double *val,
+             double *err, long long *neval)
+{ /* GV/apertureStatistics.cu:110-136: 2-D hcubature over ell in [1, 10/theta], z in [0, z_max], / 2 pi */
+    Calc *c = (Calc *)h;
+    PowerCtx p{c, theta, !(flags & 1), (flags & 2) != 0, sigma, ngal_arcmin2};
+    double vmin[2] = {1., 0}, vmax[2] = {10. / theta, c->z_max};
+    double result = 0, error = 0;
+    int rc = hcubature_v_impl(1, map2_cb, &p, 2, vmin, vmax, 0, 0, reltol, ORC_ERROR_L1, &result, &error);
+    *val = result / 2. / M_PI;
+    *err = error / 2. / M_PI;
     if (neval) *neval = g_last_neval;
     return rc;
 }

@@ -7,7 +7,7 @@
  *
  * It is a plain restatement of the reference's classed_version CPU path
  * (bispectrum.cpp, apertureStatistics.cpp, gamma.cpp, hcubature.cpp,
- * helper.cpp, cosmology.cpp), with the GSL pieces that path calls
+ * helper.cpp, cosmology.cpp; <Map^2> from the cuda_version, which alone has it), with the GSL pieces that path calls
  * (gsl_integration_qag, gsl_sf_bessel_Jn, gsl_sf_bessel_zero_Jnu) replaced
  * by project-written routines that follow the published algorithms.
  * Each function in tp_oracle.cpp cites the reference file:line it follows.
@@ -79,6 +79,23 @@ int orc_gamma_integrand_batch(void *h, int which, double x1, double x2, double x
 int orc_gamma(void *h, int which, double x1, double x2, double x3, double reltol,
               int nthreads, double *re, double *im, long long *neval);
 void orc_ggg_analytic(double r1, double r2, double r3, double *re, double *im);
+
+/* convergence power spectrum and <Map^2> (bispectrum.cpp:288-417, 543-577, 1022-1059;
+   the cuda_version's Map2 and T17 correction, apertureStatistics.cu:29-136, bispectrum.cu:951-982) */
+double orc_pk_nonlinear(void *h, double k, double z);
+double orc_linear_pk_at_z(void *h, double k, double z);
+double orc_om_m_of_z(void *h, double z);
+double orc_om_v_of_z(void *h, double z);
+/* flags: 1 = linear D1^2 P_lin instead of Halofit, 2 = T17 correction */
+double orc_limber_pkappa(void *h, double ell, double z, int flags);
+double orc_gq96_pkappa(void *h, double ell, int flags);
+double orc_pell(void *h, double ell, double sigma, double n_gal_rad2, int flags);
+double orc_map2_integrand(void *h, double ell, double z, double theta, int flags, double sigma,
+                          double ngal_arcmin2);
+int orc_convergence_power_spectrum(void *h, double ell, double reltol, int flags, double *val,
+                                   double *err, long long *neval);
+int orc_map2(void *h, double theta, double reltol, int flags, double sigma, double ngal_arcmin2,
+             double *val, double *err, long long *neval);
 
 #ifdef __cplusplus
 }

@@ -49,6 +49,20 @@ _SIGS = {
     "orc_gamma": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_double] * 4 + [ctypes.c_int, _dp, _dp,
                                                                                              _llp]),
     "orc_ggg_analytic": (None, [ctypes.c_double] * 3 + [_dp, _dp]),
+    "orc_pk_nonlinear": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    "orc_linear_pk_at_z": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    "orc_om_m_of_z": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double]),
+    "orc_om_v_of_z": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double]),
+    "orc_limber_pkappa": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int]),
+    "orc_gq96_pkappa": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double, ctypes.c_int]),
+    "orc_pell": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_int]),
+    "orc_map2_integrand": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    "orc_convergence_power_spectrum": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                                      ctypes.c_int, _dp, _dp, _llp]),
+    "orc_map2": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                ctypes.c_double, _dp, _dp, _llp]),
     "orc_read_cosmology": (ctypes.c_int, [ctypes.c_char_p, _dp]),
     "orc_read_n_of_z": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_double, _dp]),
 }
@@ -142,6 +156,20 @@ class Oracle:
         lib().orc_gamma(self.h, which, x1, x2, x3, reltol, nthreads, ctypes.byref(re), ctypes.byref(im),
                         ctypes.byref(n))
         return re.value + 1j * im.value, n.value
+
+    def convergence_power_spectrum(self, ell, reltol=1e-6, flags=0):
+        v, e, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        rc = lib().orc_convergence_power_spectrum(self.h, ell, reltol, flags, ctypes.byref(v), ctypes.byref(e),
+                                                  ctypes.byref(n))
+        assert rc == 0
+        return v.value, e.value, n.value
+
+    def map2(self, theta, reltol=1e-5, flags=0, sigma=0.0, ngal=1.0):
+        v, e, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        rc = lib().orc_map2(self.h, theta, reltol, flags, sigma, ngal, ctypes.byref(v), ctypes.byref(e),
+                            ctypes.byref(n))
+        assert rc == 0
+        return v.value, e.value, n.value
 
     def scalar(self, name, *args):
         return getattr(lib(), "orc_" + name)(self.h, *args)

@@ -63,6 +63,48 @@ def test_calculate_gamma_cli(tmp_path):
     assert np.allclose(rows[:, 9], 0.5) and np.allclose(rows[:, 10], [-0.5, 0.5])
 
 
+def test_calculate_powerspectrum_cli(tmp_path, fid_oracle):
+    """ell = 1.05^i < 5e4 and P_kappa(ell) with the T17 factor (the cuda_version's default) or without."""
+    cf, nf = _inputs(tmp_path)
+    for extra, flags in (([], 2), (["--no-t17"], 0)):
+        out = tmp_path / ("pk%d.dat" % flags)
+        r = subprocess.run([os.path.join(LIBDIR, "calculatePowerspectrum.x"), str(cf), str(out), str(nf)] + extra,
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        rows = np.loadtxt(out, ndmin=2)
+        n = int(np.ceil(np.log(5e4) / np.log(1.05)))
+        assert rows.shape == (n, 2)
+        assert abs(rows[0, 0] - 1) < 1e-12 and rows[-1, 0] < 5e4
+        for i in (0, 50, 150, n - 1):
+            ov = fid_oracle.convergence_power_spectrum(1.05 ** i, 1e-6, flags)[0]
+            assert abs(rows[i, 1] / ov - 1) < 1e-5  # six significant digits in the output
+
+
+def test_calculate_second_order_cli(tmp_path, fid_oracle):
+    cf, nf = _inputs(tmp_path)
+    tf = tmp_path / "thetas.dat"
+    tf.write_text("2\n4\n8\n16\n")
+    ngal_arcmin2 = 108000 / 0.017453 ** 2 * 2.9088820866e-4 ** 2  # per deg^2 -> rad^-2 -> arcmin^-2
+    for only in (0, 1):
+        cov = tmp_path / "cov.param"
+        cov.write_text("# survey\nthetaMax 7.87\nshapenoise_sigma 0.37\ngalaxy_density 108000\nunit deg\n"
+                       "shapenoiseOnly %d\narea 61.9369\n" % only)
+        out = tmp_path / "map2.dat"
+        r = subprocess.run([os.path.join(LIBDIR, "calculateSecondOrderApertureStatistics.x"), str(cf), str(cov),
+                            str(tf), str(out), str(nf)], capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        rows = np.loadtxt(out, ndmin=2)
+        assert rows.shape == (4, 2) and list(rows[:, 0]) == [2, 4, 8, 16]
+        for t, m in rows:
+            th = t * 2.9088820866e-4
+            if only:  # the noise term alone: sigma^2 / (2 n) / (8 pi theta^2)
+                ref = 0.37 ** 2 / (2 * ngal_arcmin2 / 2.9088820866e-4 ** 2) / (8 * np.pi * th ** 2)
+                assert abs(m / ref - 1) < 1e-4
+            else:
+                ref = fid_oracle.map2(th, 1e-5, 2, 0.37, ngal_arcmin2)[0]
+                assert abs(m / ref - 1) < 1e-5
+
+
 def test_c3_batch_subset_matches_oracle(gpu_ctx, fid_nz):
     """Latin-hypercube cosmologies with their own GPU FP64 tables, many triples in one batched run."""
     from _oracle import Oracle

@@ -1,0 +1,126 @@
+"""GPU parity of the convergence power spectrum and <Map^2> against the CPU FP64 oracle.
+
+Point values (P(k, z), the Limber integrand, Pell) agree to rounding (1e-12 relative); the adaptive
+integrals (P_kappa, <Map^2>) take the oracle's decision sequence, so their point counts are equal
+and their values agree to rounding too (1e-10).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import threepoint_amd as tp
+from threepoint_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+A = 2.9088820866e-4
+
+
+def rel(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-300)
+
+
+@pytest.mark.parametrize("flags", [0, tp.PK_LINEAR])
+def test_pk_at_z_matches_oracle(fid_tables, fid_oracle, flags):
+    rng = np.random.default_rng(7)
+    n = 400
+    kz = np.stack([np.exp(rng.uniform(np.log(1e-4), np.log(1e2), n)), rng.uniform(0, 3.0, n)], 1)
+    kz[:4, 1] = [0.0, 1e-6, 3.0, 2.9999999]
+    g = fid_tables.pk_at_z(kz, flags=flags)
+    name = "linear_pk_at_z" if flags & tp.PK_LINEAR else "pk_nonlinear"
+    o = np.array([fid_oracle.scalar(name, k, z) for k, z in kz])
+    r = rel(g, o)
+    print("P(k, z) flags %d max rel %.3e" % (flags, r.max()))
+    assert r.max() < 1e-12
+
+
+@pytest.mark.parametrize("flags", [0, tp.PK_LINEAR, tp.PK_T17])
+def test_limber_integrand_matches_oracle(fid_tables, fid_oracle, flags):
+    rng = np.random.default_rng(11)
+    n = 400
+    lz = np.stack([np.exp(rng.uniform(0, np.log(1e5), n)), rng.uniform(0, 3.0, n)], 1)
+    lz[:3, 1] = [0.0, 0.99e-5, 3.0]
+    g = fid_tables.limber_pkappa(lz, flags=flags)
+    o = np.array([fid_oracle.scalar("limber_pkappa", l, z, flags) for l, z in lz])
+    assert g[0] == 0 and g[1] == 0
+    r = rel(g[2:], o[2:])
+    print("limber flags %d max rel %.3e" % (flags, r.max()))
+    assert r.max() < 1e-12
+
+
+@pytest.mark.parametrize("flags,sigma", [(0, 0.0), (tp.PK_T17, 0.0), (0, 0.3)])
+def test_pell_matches_oracle(fid_tables, fid_oracle, flags, sigma):
+    """sigma = 0.3 with the reference's default density n (bispectrum.hpp:120-121, integer division
+    included): the shape noise then dominates; sigma = 0 tests the Limber sum itself."""
+    ell = np.concatenate([[0.0], np.geomspace(1, 5e4, 120)])
+    n = 4096 * 4096 // 10 // 10 * math.pi * math.pi / 180 / 180
+    g = fid_tables.pell(ell, flags=flags, sigma=sigma, n_gal=n)
+    o = np.array([fid_oracle.scalar("pell", l, sigma, n, flags) for l in ell])
+    assert g[0] == 0.5 * sigma * sigma / n
+    r = rel(g[1:], o[1:])
+    print("Pell flags %d sigma %g max rel %.3e" % (flags, sigma, r.max()))
+    assert r.max() < 1e-12
+
+
+def test_pkappa_batch_matches_oracle(gpu_ctx, fid_nz):
+    """Two cosmologies, multipoles interleaved: the batched cubature reproduces each integral's
+    one-at-a-time decision sequence (equal point counts)."""
+    c2 = tp.Cosmology(h=0.65, sigma8=0.9, omb=0.045, ns=0.95, w=-0.9, om=0.35)
+    tabs = tp.CosmologyTables([W.fiducial(), c2], fid_nz, 100, 3.0, gpu_ctx)
+    from _oracle import Oracle
+    orcs = [Oracle(W.fiducial(), fid_nz), Oracle(c2, fid_nz)]
+    ell = np.geomspace(1, 1e5, 24)
+    ic = np.arange(24) % 2
+    v, e, n = tabs.pkappa_batch(ell, ic, reltol=1e-6)
+    for i in range(24):
+        ov, oe, on = orcs[ic[i]].convergence_power_spectrum(ell[i], 1e-6)
+        assert n[i] == on, (i, n[i], on)
+        assert abs(v[i] / ov - 1) < 1e-10 and abs(e[i] / oe - 1) < 1e-6
+
+
+@pytest.mark.parametrize("flags,sigma", [(0, 0.0), (tp.PK_T17, 0.0), (0, 0.3)])
+def test_map2_batch_matches_oracle(fid_tables, fid_oracle, flags, sigma):
+    th = np.array([0.5, 1.0, 2.0, 4.0, 8.0, 16.0, 32.0]) * A
+    v, e, n = fid_tables.map2_batch(th, reltol=1e-5, flags=flags, sigma=sigma, ngal=30.0)
+    for i, t in enumerate(th):
+        ov, oe, on = fid_oracle.map2(t, 1e-5, flags, sigma, 30.0)
+        assert n[i] == on, (i, n[i], on)
+        assert abs(v[i] / ov - 1) < 1e-10
+
+
+def test_map2_shape_noise_closed_form(fid_tables):
+    """Size-independent property: <Map^2> with noise minus without = sigma^2 / (2 n) / (8 pi theta^2)."""
+    th = np.geomspace(0.5, 64, 25) * A
+    m0 = fid_tables.map2_batch(th, reltol=1e-7)[0]
+    m1 = fid_tables.map2_batch(th, reltol=1e-7, sigma=0.3, ngal=30.0)[0]
+    cf = 0.09 / (2 * 30.0 / A ** 2) / (8 * np.pi * th ** 2)
+    assert np.max(np.abs((m1 - m0) / cf - 1)) < 1e-5
+
+
+def test_power_with_tabulated_pk(gpu_ctx, fid_nz):
+    """The tabulated linear P(k) input feeds the power spectrum as it feeds the bispectrum."""
+    from _oracle import Oracle
+    c = W.fiducial()
+    eh = Oracle(c, fid_nz)
+    kh = np.geomspace(1e-5, 1e3, 300)
+    pk = np.array([eh.scalar("linear_pk", k) for k in kh]) * 1.05
+    tabs = tp.CosmologyTables([c], fid_nz, 100, 3.0, gpu_ctx, pk=(kh, pk))
+    orc = Oracle(c, fid_nz, pk=(kh, pk))
+    ell = np.geomspace(10, 1e4, 9)
+    g = tabs.pell(ell)
+    o = np.array([orc.scalar("pell", l, 0.0, 1.0, 0) for l in ell])
+    assert rel(g, o).max() < 1e-12
+    v, e, n = tabs.map2_batch(np.array([2.0, 8.0]) * A)
+    for i, t in enumerate([2.0, 8.0]):
+        ov, oe, on = orc.map2(t * A, 1e-5)
+        assert n[i] == on and abs(v[i] / ov - 1) < 1e-10
+
+
+def test_power_rejects_bad_arguments(fid_tables):
+    with pytest.raises(tp.TpError):
+        fid_tables.map2_batch(np.array([0.0]))
+    with pytest.raises(tp.TpError):
+        fid_tables.pell(np.array([10.0]), flags=8)
+    with pytest.raises(tp.TpError):
+        fid_tables.pkappa_batch(np.array([10.0]), icosmo=np.array([3]))

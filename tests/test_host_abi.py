@@ -53,7 +53,7 @@ def _integrand(fdim, dim):
 
 
 @pytest.mark.parametrize("dim,fdim,norm,tol", [(3, 1, 3, 1e-4), (3, 2, 3, 1e-4), (2, 2, 1, 1e-5), (4, 1, 0, 1e-3),
-                                               (1, 2, 2, 1e-8), (3, 2, 4, 1e-4), (2, 2, 0, 1e-6)])
+                                               (1, 2, 2, 1e-8), (1, 1, 3, 1e-10), (3, 2, 4, 1e-4), (2, 2, 0, 1e-6)])
 def test_product_cubature_bit_exact(dim, fdim, norm, tol):
     cb = _integrand(fdim, dim)
     lo, hi = np.zeros(dim), np.ones(dim)
@@ -68,10 +68,9 @@ def test_product_cubature_bit_exact(dim, fdim, norm, tol):
         v2, e2 = np.zeros(fdim), np.zeros(fdim)
         assert ref.hcubature_v(fdim, cb, None, dim, O.d(lo), O.d(hi), 0, 0.0, tol, norm, O.d(v2), O.d(e2)) == 0
         assert np.array_equal(v1, v2) and np.array_equal(e1, e2)
-    if dim >= 2:
-        v3, e3 = np.zeros(fdim), np.zeros(fdim)
-        O.lib().orc_hcubature_v(fdim, cb, None, dim, O.d(lo), O.d(hi), 0, 0.0, tol, norm, O.d(v3), O.d(e3))
-        assert np.array_equal(v1, v3) and np.array_equal(e1, e3)
+    v3, e3 = np.zeros(fdim), np.zeros(fdim)
+    assert O.lib().orc_hcubature_v(fdim, cb, None, dim, O.d(lo), O.d(hi), 0, 0.0, tol, norm, O.d(v3), O.d(e3)) == 0
+    assert np.array_equal(v1, v3) and np.array_equal(e1, e3)
 
 
 def test_product_cubature_maxeval_and_abort():

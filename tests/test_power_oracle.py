@@ -1,0 +1,98 @@
+"""Pins of the oracle's convergence power spectrum and <Map^2> (CPU only).
+
+What each check fixes, independently of the oracle's own formulas:
+  * om_m(z) + om_v(z) = 1 for a flat model, om_m(0) = Omega_m (the density parameters of Halofit);
+  * Halofit's linear limit: P_NL / (D1^2 P_lin) -> 1 as k -> 0, the leading deviation being the
+    exp(-y/4) damping, y = k r_sigma -- fixes the Delta^2 <-> P conversions around pq;
+  * the linear P_kappa scales exactly as sigma8^2 (g, f_K, D1 do not depend on sigma8);
+  * two quadratures of the same Limber integral agree: the adaptive Gauss-Kronrod cubature
+    (convergence_power_spectrum) and the fixed 96-node sum (Pell), to the accuracy the
+    piecewise-linear tables leave the 96-node rule (a few 1e-4);
+  * <Map^2> as the reference's 2-D cubature over (ell, z) equals an independent 1-D ell-integral
+    of ell uHat^2 P_kappa / 2 pi (scipy quad over the adaptive P_kappa) -- fixes the ell range,
+    the filter and the 1 / 2 pi;
+  * the shape-noise term of <Map^2> has the closed form sigma^2 / (2 n) / (8 pi theta^2)
+    (int_0^inf ell uHat(ell theta)^2 d ell = 1 / (4 theta^2)).
+The Halofit and T17 fit constants themselves are pinned by restatement review only (parity unpinned
+for those constants): the reference ships no power-spectrum fixture.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy.integrate import quad
+
+from _oracle import Oracle
+from threepoint_amd import workloads as W
+
+A = 2.9088820866e-4  # arcmin in rad (classed_version/helper.cpp:139)
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle(W.fiducial(), W.fiducial_nz())
+
+
+def test_density_parameters_flat(orc):
+    for z in (0.0, 0.3, 1.7, 3.0):
+        assert abs(orc.scalar("om_m_of_z", z) + orc.scalar("om_v_of_z", z) - 1) < 1e-15
+    assert abs(orc.scalar("om_m_of_z", 0.0) - 0.3) < 1e-15
+    assert orc.scalar("om_m_of_z", 2.0) > orc.scalar("om_m_of_z", 1.0) > 0.3
+
+
+def test_halofit_linear_limit(orc):
+    rs = orc.tables()["r_sigma"]
+    for z in (0.1, 1.0, 2.9):
+        assert abs(orc.scalar("pk_nonlinear", 1e-6, z) / orc.scalar("linear_pk_at_z", 1e-6, z) - 1) < 1e-5
+        # leading deviation: the damping exp(-y/4), y = k r_sigma(z), with r_sigma(z) inside the table's range
+        d = orc.scalar("pk_nonlinear", 1e-9, z) / orc.scalar("linear_pk_at_z", 1e-9, z) - 1
+        assert -rs.max() / 4 * 1.001 < d / 1e-9 < -rs.min() / 4 * 0.999
+    # and the nonlinear boost at small scales
+    assert orc.scalar("pk_nonlinear", 5.0, 0.5) > 10 * orc.scalar("linear_pk_at_z", 5.0, 0.5)
+
+
+def test_linear_pkappa_scales_as_sigma8_squared(orc):
+    c = W.fiducial()
+    c.sigma8 = 0.9
+    o2 = Oracle(c, W.fiducial_nz())
+    for ell in (10.0, 1000.0, 3e4):
+        r = orc.scalar("gq96_pkappa", ell, 1) / o2.scalar("gq96_pkappa", ell, 1)
+        assert abs(r / (0.82 / 0.9) ** 2 - 1) < 1e-13
+
+
+def test_pkappa_adaptive_vs_96_nodes(orc):
+    for ell in (1.0, 100.0, 1e4, 1e5):
+        v, e, n = orc.convergence_power_spectrum(ell, 1e-6)
+        g = orc.scalar("gq96_pkappa", ell, 0)
+        assert abs(v / g - 1) < 5e-4
+        assert e < 2e-6 * abs(v) and n % 15 == 0
+    assert orc.scalar("limber_pkappa", 100.0, 0.5e-5, 0) == 0.0  # z < 1e-5 guard
+    # Pell = the 96-node sum plus 0.5 sigma^2 / n; ell = 0 gives the shape noise alone
+    assert orc.scalar("pell", 0.0, 0.3, 50.0, 0) == 0.5 * 0.09 / 50.0
+    assert abs(orc.scalar("pell", 100.0, 0.3, 1e8, 0) - orc.scalar("gq96_pkappa", 100.0, 0) - 0.045 / 1e8) < 1e-24
+
+
+@pytest.mark.parametrize("th_arcmin", [2.0, 8.0])
+def test_map2_equals_ell_integral_of_pkappa(orc, th_arcmin):
+    t = th_arcmin * A
+
+    def f(lnl):
+        ell = math.exp(lnl)
+        eta = ell * t
+        u = 0.5 * eta * eta * math.exp(-0.5 * eta * eta)
+        return ell * ell * u * u * orc.convergence_power_spectrum(ell, 1e-9)[0]
+
+    I, _ = quad(f, 0.0, math.log(10 / t), epsrel=1e-9, limit=200)
+    m, e, n = orc.map2(t, reltol=1e-7)
+    assert abs(m / (I / (2 * math.pi)) - 1) < 1e-6
+    assert n % 17 == 0
+
+
+@pytest.mark.parametrize("th_arcmin", [1.0, 4.0, 16.0])
+def test_map2_shape_noise_closed_form(orc, th_arcmin):
+    t = th_arcmin * A
+    sigma, ngal = 0.3, 30.0
+    m0 = orc.map2(t, reltol=1e-7)[0]
+    m1 = orc.map2(t, reltol=1e-7, sigma=sigma, ngal=ngal)[0]
+    cf = sigma ** 2 / (2 * ngal / A ** 2) / (8 * math.pi * t ** 2)
+    assert abs((m1 - m0) / cf - 1) < 1e-5

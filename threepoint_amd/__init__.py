@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libthreepoint_amd.so")
 
 MODE_FP64, MODE_FP32, MODE_SURROGATE, MODE_ANALYTIC = 0, 1, 2, 3
+PK_LINEAR, PK_T17 = 1, 2  # flags of the power-spectrum calls
 ERROR_INDIVIDUAL, ERROR_PAIRED, ERROR_L2, ERROR_L1, ERROR_LINF = 0, 1, 2, 3, 4
 ARCMIN = 2.9088820866e-4  # classed_version/helper.cpp:139
 
@@ -85,6 +86,16 @@ _SIGS = {
     "tp_gamma_setup": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
     "tp_gamma_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_size_t, _dp, _dp]),
+    "tp_pk_at_z_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
+                                       ctypes.c_int, _dp]),
+    "tp_limber_pkappa_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
+                                             ctypes.c_int, _dp]),
+    "tp_pell": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp, ctypes.c_int,
+                               ctypes.c_double, ctypes.c_double, _dp]),
+    "tp_pkappa_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp, ctypes.c_double,
+                                       ctypes.c_int, _dp, _dp, _llp]),
+    "tp_map2_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp, ctypes.c_double,
+                                     ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _llp]),
     "tp_gamma_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
                                       ctypes.c_double, _dp, _dp, _llp]),
     "tp_hcubature_v": (ctypes.c_int, [ctypes.c_uint, INTEGRAND_V, ctypes.c_void_p, ctypes.c_uint, _dp, _dp,
@@ -314,6 +325,51 @@ class CosmologyTables:
         _check(lib().tp_gamma_batch(self.ctx.h, self.h, icosmo, n, _d(x), reltol, _d(re), _d(im),
                                     ne.ctypes.data_as(_llp)), "tp_gamma_batch")
         return (re + 1j * im).reshape(n, 4), ne.reshape(n, 4)
+
+
+    # ---------------- convergence power spectrum and <Map^2>
+    def pk_at_z(self, kz, icosmo=0, flags=0):
+        """P_NL(k, z) (or D1^2 P_lin with PK_LINEAR) at points kz[n, 2] = (k [h/Mpc], z)."""
+        x = np.ascontiguousarray(kz, dtype=np.float64).reshape(-1, 2)
+        out = np.zeros(x.shape[0])
+        _check(lib().tp_pk_at_z_eval(self.ctx.h, self.h, icosmo, x.shape[0], _d(x), flags, _d(out)),
+               "tp_pk_at_z_eval")
+        return out
+
+    def limber_pkappa(self, ellz, icosmo=0, flags=0):
+        """limber_integrand_power_spectrum at points ellz[n, 2] = (ell, z)."""
+        x = np.ascontiguousarray(ellz, dtype=np.float64).reshape(-1, 2)
+        out = np.zeros(x.shape[0])
+        _check(lib().tp_limber_pkappa_eval(self.ctx.h, self.h, icosmo, x.shape[0], _d(x), flags, _d(out)),
+               "tp_limber_pkappa_eval")
+        return out
+
+    def pell(self, ell, icosmo=0, flags=0, sigma=0.0, n_gal=1.0):
+        """Pell: 96-node Limber sum of P_NL plus 0.5 sigma^2 / n_gal (n_gal in rad^-2)."""
+        e = np.ascontiguousarray(ell, dtype=np.float64).reshape(-1)
+        out = np.zeros(e.size)
+        _check(lib().tp_pell(self.ctx.h, self.h, icosmo, e.size, _d(e), flags, sigma, n_gal, _d(out)), "tp_pell")
+        return out
+
+    def pkappa_batch(self, ell, icosmo=None, reltol=1e-6, flags=0):
+        """convergence_power_spectrum for many multipoles: (values, error estimates, points)."""
+        e = np.ascontiguousarray(ell, dtype=np.float64).reshape(-1)
+        n = e.size
+        ic = np.zeros(n, dtype=np.int32) if icosmo is None else np.ascontiguousarray(icosmo, dtype=np.int32)
+        val, err, ne = np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int64)
+        _check(lib().tp_pkappa_batch(self.ctx.h, self.h, n, ic.ctypes.data_as(_ip), _d(e), reltol, flags, _d(val),
+                                     _d(err), ne.ctypes.data_as(_llp)), "tp_pkappa_batch")
+        return val, err, ne
+
+    def map2_batch(self, theta, icosmo=None, reltol=1e-5, flags=0, sigma=0.0, ngal=1.0):
+        """<Map^2>(theta [rad]) for one source bin; sigma != 0 adds shape noise (ngal in arcmin^-2)."""
+        t = np.ascontiguousarray(theta, dtype=np.float64).reshape(-1)
+        n = t.size
+        ic = np.zeros(n, dtype=np.int32) if icosmo is None else np.ascontiguousarray(icosmo, dtype=np.int32)
+        val, err, ne = np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int64)
+        _check(lib().tp_map2_batch(self.ctx.h, self.h, n, ic.ctypes.data_as(_ip), _d(t), reltol, flags, sigma, ngal,
+                                   _d(val), _d(err), ne.ctypes.data_as(_llp)), "tp_map2_batch")
+        return val, err, ne
 
 
 def census_map3_point():

@@ -6,6 +6,9 @@ artefacts travel with the source tree:
   threepoint_amd/_lib/libthreepoint_amd.so      HIP kernels + C ABI + cubature
   threepoint_amd/_lib/calculateApertureStatistics.x
   threepoint_amd/_lib/calculateGamma.x
+  threepoint_amd/_lib/calculatePowerspectrum.x
+  threepoint_amd/_lib/calculateSecondOrderApertureStatistics.x
+  threepoint_amd/_lib/threepoint_cosmosis.so    the cosmosis module
 """
 import os
 import subprocess
@@ -17,9 +20,10 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(LIBDIR, "libthreepoint_amd.so")
 
-HIP_SOURCES = ["tp_tables.hip", "tp_map3.hip", "tp_runtime.hip", "tp_gamma.hip", "tp_mlp.hip"]
+HIP_SOURCES = ["tp_tables.hip", "tp_map3.hip", "tp_runtime.hip", "tp_gamma.hip", "tp_mlp.hip", "tp_power.hip"]
 CPP_SOURCES = ["tp_cubature.cpp", "tp_census.cpp", "tp_classes.cpp", "tp_io.cpp"]
-APPS = ["calculateApertureStatistics", "calculateGamma"]
+APPS = ["calculateApertureStatistics", "calculateGamma", "calculatePowerspectrum",
+        "calculateSecondOrderApertureStatistics"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [

@@ -92,7 +92,67 @@ double BispectrumCalculator::integrand_bkappa(double z, ell_params p)
     return out;
 }
 
+double BispectrumCalculator::P_k_nonlinear(double k, double z)
+{
+    double in[2] = {k, z}, out = 0;
+    check(tp_pk_at_z_eval(ctx_, tables_, 0, 1, in, 0, &out), "tp_pk_at_z_eval");
+    return out;
+}
+
+double BispectrumCalculator::linear_pk_at_z(double k, double z)
+{
+    double in[2] = {k, z}, out = 0;
+    check(tp_pk_at_z_eval(ctx_, tables_, 0, 1, in, TP_PK_LINEAR, &out), "tp_pk_at_z_eval");
+    return out;
+}
+
+double BispectrumCalculator::limber_integrand_power_spectrum(double ell, double z, bool nonlinear)
+{
+    double in[2] = {ell, z}, out = 0;
+    check(tp_limber_pkappa_eval(ctx_, tables_, 0, 1, in, nonlinear ? 0 : TP_PK_LINEAR, &out),
+          "tp_limber_pkappa_eval");
+    return out;
+}
+
+double BispectrumCalculator::convergence_power_spectrum(double ell)
+{
+    return convergence_power_spectrum(std::vector<double>{ell})[0];
+}
+
+std::vector<double> BispectrumCalculator::convergence_power_spectrum(const std::vector<double> &ell)
+{
+    // classed_version/bispectrum.cpp:382-388: relative tolerance 1e-6
+    std::vector<double> out(ell.size()), err(ell.size());
+    check(tp_pkappa_batch(ctx_, tables_, ell.size(), nullptr, ell.data(), 1e-6, 0, out.data(), err.data(), nullptr),
+          "tp_pkappa_batch");
+    return out;
+}
+
+double BispectrumCalculator::Pell(double ell) { return Pell(std::vector<double>{ell})[0]; }
+
+std::vector<double> BispectrumCalculator::Pell(const std::vector<double> &ell)
+{
+    std::vector<double> out(ell.size());
+    check(tp_pell(ctx_, tables_, 0, ell.size(), ell.data(), 0, sigma, n, out.data()), "tp_pell");
+    return out;
+}
+
 ApertureStatistics::ApertureStatistics(BispectrumCalculator *b) : Bispectrum_(b) {}
+
+double ApertureStatistics::Map2(double theta, double sigma_epsilon, double ngal, bool t17)
+{
+    return Map2(std::vector<double>{theta}, sigma_epsilon, ngal, t17)[0];
+}
+
+std::vector<double> ApertureStatistics::Map2(const std::vector<double> &thetas, double sigma_epsilon, double ngal,
+                                             bool t17)
+{
+    std::vector<double> out(thetas.size()), err(thetas.size());
+    check(tp_map2_batch(Bispectrum_->ctx(), Bispectrum_->tables(), thetas.size(), nullptr, thetas.data(), reltol_map2,
+                        t17 ? TP_PK_T17 : 0, sigma_epsilon, ngal, out.data(), err.data(), nullptr),
+          "tp_map2_batch");
+    return out;
+}
 
 double ApertureStatistics::uHat(const double &eta)
 {

@@ -179,6 +179,9 @@ int tp_set_error(int code, const std::string &msg);
             return tp_set_error(TP_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// n-point Gauss-Legendre nodes and weights on [-1, 1], ascending (Newton iteration on P_n).
+void tp_gauss_legendre(int n, double *x, double *w);
+
 // Launchers implemented in the .hip files (all asynchronous on ctx->stream).
 int tp_launch_map3_points(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const double *th, size_t npts,
                           const double *d_x, double *d_f, int mode, int *d_flag);

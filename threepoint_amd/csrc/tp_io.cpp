@@ -140,3 +140,41 @@ std::ostream &operator<<(std::ostream &out, const configGamma &c)
     out << "Setting the triangle center as Centroid" << std::endl;
     return out;
 }
+
+covarianceParameters::covarianceParameters(const std::string &fn)
+{
+    // cuda_version/cosmology.cu:92-170: "name value" lines, '#' comments; lengths and densities in `unit`
+    std::ifstream input(fn.c_str());
+    if (input.fail()) throw std::runtime_error("covarianceParameters: Could not open " + fn);
+    std::string line, unit;  // no default: a file without "unit" is rejected, as by the reference
+    while (std::getline(input, line)) {
+        if (line.empty() || line[0] == '#') continue;
+        std::istringstream iss(line);
+        std::string name, value;
+        iss >> name >> value;
+        if (name == "thetaMax") thetaMax = std::stod(value);
+        else if (name == "thetaMax_smaller") thetaMax_smaller = std::stod(value);
+        else if (name == "shapenoise_sigma") shapenoise_sigma = std::stod(value);
+        else if (name == "galaxy_density") galaxy_density = std::stod(value);
+        else if (name == "unit") unit = value;
+        else if (name == "shapenoiseOnly") shapenoiseOnly = std::stoi(value) != 0;
+        else if (name == "area") area = std::stod(value);
+        else throw std::runtime_error("covarianceParameters: parameter file is not in the right format (" + name + ")");
+    }
+    // to rad and rad^-2 (cosmology.cu:160-169)
+    const double r1 = convert_angle_to_rad(1, unit);
+    thetaMax = convert_angle_to_rad(thetaMax, unit);
+    thetaMax_smaller = convert_angle_to_rad(thetaMax_smaller, unit);
+    area = area * r1 * r1;
+    galaxy_density = galaxy_density / r1 / r1;
+}
+
+std::ostream &operator<<(std::ostream &out, const covarianceParameters &c)
+{
+    // cuda_version/cosmology.cu:172-179
+    out << "# thetaMax and galaxy density in rad and rad^-2" << std::endl;
+    out << "thetaMax " << c.thetaMax << std::endl;
+    out << "galaxy density " << c.galaxy_density << std::endl;
+    out << "shapenoise " << c.shapenoise_sigma << std::endl;
+    return out;
+}

@@ -27,23 +27,6 @@ namespace {
 
 constexpr double kPi = 3.14159265358979323846;
 
-// ----------------------------------------------------------------- EH P(k)
-__device__ __forceinline__ double pk_eh(const TpCosmoConst &C, double k, double norm)
-{
-    // classed_version/bispectrum.cpp:811-841, k in h/Mpc
-    k *= C.h;
-    double x = 0.43 * k * C.eh_sh;
-    double x2 = x * x;
-    double geff = C.eh_omh2 * (C.eh_sqrt_alnu + (1.0 - C.eh_sqrt_alnu) / (1.0 + x2 * x2));
-    double qeff = k / geff * C.eh_theta2;
-    double L = log(2.718281828 + C.eh_cL * qeff);
-    double Cq = 14.4 + 325.0 / (1.0 + 60.5 * pow(qeff, 1.11));
-    double r = L / (L + Cq * qeff * qeff);
-    double delk = norm * norm * pow(k * 2997.9 / C.h, 3. + C.ns) * r * r;
-    double pk = 2.0 * kPi * kPi / (k * k * k) * delk;
-    return C.h * C.h * C.h * pk;
-}
-
 __device__ __forceinline__ double window(double x, int i)
 {
     // classed_version/bispectrum.cpp:769-783
@@ -53,16 +36,10 @@ __device__ __forceinline__ double window(double x, int i)
     return x * x * (1 - x * x) * exp(-x * x);
 }
 
-// linear_pk (classed_version/bispectrum.cpp:785-809): the tabulated P(k) when given, else the fit
-__device__ __forceinline__ double pk_lin(const TpCosmoConst &C, double k, double norm)
-{
-    return C.tab.nk ? tp_pk_table(C.tab, k) : pk_eh(C, k, norm);
-}
-
 __device__ __forceinline__ double sigma_term(const TpCosmoConst &C, double k, double r, int j, double norm)
 {
     double w = window(k * r, j);
-    double kk = k * k * k * pk_lin(C, k, norm);
+    double kk = k * k * k * tp_pk_lin(C, k, norm);
     return (j < 3) ? kk * w * w : kk * w;
 }
 
@@ -546,7 +523,9 @@ __global__ void k_nodes(int ncosmo, int n_z, const TpCosmoConst *cc, const doubl
     }
 }
 
-void gauss_legendre_host(int n, double *x, double *w)
+}  // namespace
+
+void tp_gauss_legendre(int n, double *x, double *w)
 {
     for (int i = 0; i < (n + 1) / 2; ++i) {
         double z = cos(kPi * (i + 0.75) / (n + 0.5)), pp = 0;
@@ -569,15 +548,13 @@ void gauss_legendre_host(int n, double *x, double *w)
     }
 }
 
-}  // namespace
-
 // Host: upload GL-96 half nodes once per process and device.
 static int upload_gl96(int device)
 {
     static bool done[64] = {false};
     if (device >= 0 && device < 64 && done[device]) return 0;
     double x[96], w[96], hx[48], hw[48];
-    gauss_legendre_host(96, x, w);
+    tp_gauss_legendre(96, x, w);
     for (int i = 0; i < 48; ++i) {
         hx[i] = x[48 + i];
         hw[i] = w[48 + i];
@@ -585,7 +562,7 @@ static int upload_gl96(int device)
     TP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gl_x), hx, sizeof(hx)));
     TP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gl_w), hw, sizeof(hw)));
     double x8[8], w8[8];
-    gauss_legendre_host(8, x8, w8);
+    tp_gauss_legendre(8, x8, w8);
     TP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gl8_x), x8, sizeof(x8)));
     TP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_gl8_w), w8, sizeof(w8)));
     if (device >= 0 && device < 64) done[device] = true;
