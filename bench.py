@@ -239,6 +239,99 @@ def gamma_cpu_baseline(W, x, points_per_triangle, seconds_target):
                       "per triangle of the GPU run" % (done, cores, cpu_model(), dt, points_per_triangle)}
 
 
+def map2_bench(tp, W, ctx, args):
+    """<Map^2> of one source bin for 512 Latin-hypercube cosmologies x 10 aperture radii (0.5-32 arcmin):
+    5120 2-D cubatures over (ell, z), rel tol 1e-5, no shape noise, the classed_version spectrum."""
+    t0 = time.perf_counter()
+    cos = W.lhs_c3(512)
+    tabs = tp.CosmologyTables(cos, W.fiducial_nz(), 100, 3.0, ctx)
+    tables_ms = 1e3 * (time.perf_counter() - t0)
+    radii = W.to_rad(W.map3_c1_radii())
+    theta = np.tile(radii, 512)
+    icosmo = np.repeat(np.arange(512, dtype=np.int32), radii.size)
+    c = tp.census_power_point()
+    fe_point = c["map2_flops"] + FLOPS_PER_ELEM * c["map2_elem"]
+    results, values = {}, {}
+    for p in [q for q in args.paths.split(",") if q in ("plain_fp64", "batched_fp64")]:
+        print("bench: map2 path %s ..." % p, file=sys.stderr, flush=True)
+        steps = 1 if p == "plain_fp64" else args.steps
+        n = 20 if p == "plain_fp64" else theta.size
+        agg = {"points": 0, "integrand_ms": 0.0, "launches": 0, "rounds": 0}
+        if p == "batched_fp64":
+            for _ in range(args.warmup):
+                tabs.map2_batch(theta, icosmo)
+        tt = time.perf_counter()
+        for _ in range(steps):
+            if p == "batched_fp64":
+                v = tabs.map2_batch(theta, icosmo)[0]
+                st = ctx.stats()
+                for k in agg:
+                    agg[k] += st.get(k, 0)
+            else:  # one integral per call: the reference's one-at-a-time loop
+                v = []
+                for i in range(n):
+                    v.append(tabs.map2_batch(theta[i:i + 1], icosmo[i:i + 1])[0][0])
+                    st = ctx.stats()
+                    for k in agg:
+                        agg[k] += st.get(k, 0)
+                v = np.array(v)
+        dt = time.perf_counter() - tt
+        agg["ms_per_step"] = 1e3 * dt / steps
+        agg["integrals_per_s"] = n * steps / dt
+        agg["points_per_integral"] = agg["points"] / (steps * n)
+        if agg["integrand_ms"] > 0:
+            ach = fe_point * agg["points"] / (agg["integrand_ms"] * 1e-3)
+            agg["integrand_kernel"] = {"achieved_flops": ach, "peak": PEAK["fp64"], "frac": ach / PEAK["fp64"],
+                                       "avg_launch_ms": agg["integrand_ms"] / max(1, agg["launches"])}
+        results[p], values[p] = agg, v
+    if "plain_fp64" in values and "batched_fp64" in values:
+        a, b = values["batched_fp64"][:20], values["plain_fp64"]
+        results["plain_fp64"]["max_rel_diff_vs_batched"] = float(np.max(np.abs(a / b - 1)))
+    head = "batched_fp64" if "batched_fp64" in results else next(iter(results))
+    hr = results[head]
+    dom = hr["integrand_kernel"]
+    line = {
+        "metric": "<Map^2> integrals/s on 1 GPU (one source bin)",
+        "value": hr["integrals_per_s"], "unit": "integrals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": hr["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "<Map^2> of 512 Latin-hypercube wCDM cosmologies x 10 log radii 0.5-32 arcmin = "
+                               "5120 integrals, revised Halofit, 100 z-bins, 2-D hcubature (ell, z) rel tol 1e-5, "
+                               "no shape noise", "path": head, "tables_ms": tables_ms},
+        "paths": results,
+        "roofline": {"bound": "valu", "kernel": "k_power_points", "achieved": dom["achieved_flops"] / 1e12,
+                     "peak": dom["peak"] / 1e12, "unit": "TFLOP/s", "frac": dom["frac"], "traffic": None,
+                     "algorithmic_bytes_per_point": 16 + 4 + 8,
+                     "work_per_point_flop_eq": fe_point, "flops_per_elementary_call": FLOPS_PER_ELEM,
+                     "census": c},
+    }
+    if not args.no_cpu:
+        print("bench: cpu baseline ...", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = map2_cpu_baseline(W, cos, radii, args.cpu_seconds)
+    print(json.dumps(line))
+
+
+def map2_cpu_baseline(W, cos, radii, seconds_target):
+    """The FP64 CPU oracle's <Map^2> (the same cubature and integrand), one integral per thread."""
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _oracle import Oracle
+    cores = min(os.cpu_count() or 1, 64)
+    nz = W.fiducial_nz()
+    orcs = [Oracle(c, nz) for c in cos[:cores]]
+    jobs = [(orcs[i % len(orcs)], radii[i % radii.size]) for i in range(100000)]
+    done, t0 = 0, time.perf_counter()
+    with ThreadPoolExecutor(cores) as ex:  # the oracle's ctypes calls release the GIL
+        while time.perf_counter() - t0 < seconds_target:
+            batch = jobs[done:done + 4 * cores]
+            list(ex.map(lambda j: j[0].map2(j[1], 1e-5), batch))
+            done += len(batch)
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "integrals/s", "cores": cores, "kind": "port",
+            "sample": "%d <Map^2> integrals (the first %d cosmologies x the 10 radii) through the FP64 CPU oracle, "
+                      "%d threads, %s, %.1f s" % (done, len(orcs), cores, cpu_model(), dt)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,7 +341,7 @@ def main():
     ap.add_argument("--paths", default="plain_fp64,batched_fp64,batched_fp32,surrogate")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3"])
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "map2"])
     ap.add_argument("--c2-triangles", type=int, default=10,
                     help="Gamma workload: number of the 1000 TreeCorr-grid triangles (one per r bin at 10)")
     ap.add_argument("--surrogate", default=os.path.join(ROOT, "threepoint_amd", "models", "bkappa_mlp.bin"))
@@ -276,6 +369,9 @@ def main():
               file=sys.stderr, flush=True)
     if args.workload == "c2":
         gamma_bench(tp, W, ctx, args)
+        return
+    if args.workload == "map2":
+        map2_bench(tp, W, ctx, args)
         return
     t0 = time.perf_counter()
     icosmo = None

@@ -160,6 +160,9 @@ int tp_census_map3_point(double *out);
  * elementary calls per Ogata term (a cubature point has 3 (prec_k - 1) terms), out[2] flops
  * and out[3] elementary calls of the per-point triangle shape. */
 int tp_census_gamma_term(double *out);
+/* Census of the <Map^2> integrand (tp_map2_batch): out[0] flops and out[1] elementary calls per point
+ * (Halofit, no T17 factor, no shape noise); out[2], out[3] per point of the P_kappa Limber integrand. */
+int tp_census_power_point(double *out);
 
 /* ---------------------------------------------------------------- Gamma_i */
 /* Replaces GammaCalculator(bk, prec_h, prec_k, "centroid") + initialize_bessel

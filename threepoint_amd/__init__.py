@@ -103,6 +103,7 @@ _SIGS = {
     "tp_hcubature_last_neval": (ctypes.c_longlong, []),
     "tp_census_map3_point": (ctypes.c_int, [_dp]),
     "tp_census_gamma_term": (ctypes.c_int, [_dp]),
+    "tp_census_power_point": (ctypes.c_int, [_dp]),
     "tp_hbm_copy_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _dp]),
     "tp_set_region_lanes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "tp_region_lanes_table": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
@@ -383,6 +384,13 @@ def flop_equivalents_per_point(flops_per_elem=20.0):
     c = census_map3_point()
     return c["nodes"] * (c["node_flops"] + flops_per_elem * c["node_elem"]) + c["point_flops"] + \
         flops_per_elem * c["point_elem"]
+
+
+def census_power_point():
+    """Algorithmic op census of one <Map^2> and one P_kappa integrand point (see tp_census_power_point)."""
+    out = np.zeros(4)
+    _check(lib().tp_census_power_point(_d(out)), "tp_census_power_point")
+    return dict(map2_flops=out[0], map2_elem=out[1], pk_flops=out[2], pk_elem=out[3])
 
 
 def census_gamma_term():

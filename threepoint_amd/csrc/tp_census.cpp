@@ -6,7 +6,7 @@
 // is one elementary-function call.  Comparisons, selects and loads are not counted.
 #include <cmath>
 
-#include "tp_device.h"
+#include "tp_power_dev.h"
 
 namespace {
 struct OpCounts {
@@ -27,6 +27,13 @@ inline Cnt &operator+=(Cnt &a, Cnt b) { a = a + b; return a; }
 inline Cnt &operator*=(Cnt &a, Cnt b) { a = a * b; return a; }
 inline bool operator<(Cnt a, Cnt b) { return a.v < b.v; }
 inline bool operator>(Cnt a, Cnt b) { return a.v > b.v; }
+// elementary functions of the power-spectrum point (found by argument-dependent lookup)
+inline Cnt pow(Cnt a, Cnt b) { ++g_cnt.trans; return Cnt(std::pow(a.v, b.v)); }
+inline Cnt exp(Cnt a) { ++g_cnt.trans; return Cnt(std::exp(a.v)); }
+inline Cnt log(Cnt a) { ++g_cnt.trans; return Cnt(std::log(a.v)); }
+inline Cnt sqrt(Cnt a) { ++g_cnt.trans; return Cnt(std::sqrt(a.v)); }
+inline Cnt fabs(Cnt a) { return Cnt(std::fabs(a.v)); }
+inline double tp_dbl(Cnt a) { return a.v; }
 }  // namespace
 
 template <> struct TpMath<Cnt> {
@@ -109,5 +116,32 @@ extern "C" int tp_census_gamma_term(double *out)
     out[1] = (double)term.trans;
     out[2] = (double)shape.flops;
     out[3] = (double)shape.trans;
+    return 0;
+}
+
+// out: [0] flops and [1] elementary calls (pow, exp, log, sqrt) of one <Map^2> integrand point
+// (Halofit, no T17 factor, no shape noise); [2], [3] the same for one P_kappa Limber point; divisions
+// count as flops
+extern "C" int tp_census_power_point(double *out)
+{
+    TpCosmoConst C{};
+    C.h = 0.7; C.sigma8 = 0.82; C.omb = 0.047; C.omc = 0.253; C.ns = 0.97; C.w = -1; C.om = 0.3; C.ow = 0.7;
+    C.zmax = 3; C.norm = 1; C.eh_fb = 0.16; C.eh_sqrt_alnu = 0.9; C.eh_sh = 100; C.eh_omh2 = 0.15;
+    C.eh_theta2 = 1.02; C.eh_cL = 2.0; C.limber_pref = 1e-9; C.n_z = 4;
+    C.tab = TpPkTable{};  // the census counts the Eisenstein & Hu form
+    double tab[6 * 4];
+    for (int i = 0; i < 24; ++i) tab[i] = 1.0 + 0.1 * (i % 4);
+    g_cnt = OpCounts();
+    Cnt m = tp_map2_point<Cnt>(C, tab, 4, Cnt(2000.), Cnt(0.7), 5e-4, 0, 0.0, 1.0);
+    (void)m;
+    OpCounts map2 = g_cnt;
+    g_cnt = OpCounts();
+    Cnt p = tp_limber_pk<Cnt>(C, tab, 4, Cnt(2000.), Cnt(0.7), 0);
+    (void)p;
+    OpCounts pk = g_cnt;
+    out[0] = (double)map2.flops;
+    out[1] = (double)map2.trans;
+    out[2] = (double)pk.flops;
+    out[3] = (double)pk.trans;
     return 0;
 }

@@ -47,25 +47,30 @@ __host__ __device__ __forceinline__ double tp_pk_table(const TpPkTable &t, doubl
 
 // Eisenstein & Hu no-wiggle linear P(k), k in h/Mpc (classed_version/bispectrum.cpp:811-841), with the
 // k-independent constants folded into C; norm is C.norm except while the normalisation is computed.
-__host__ __device__ __forceinline__ double tp_pk_eh(const TpCosmoConst &C, double k, double norm)
+// T is double in the kernels (a counting type in the operation census).
+template <class T>
+__host__ __device__ __forceinline__ T tp_pk_eh(const TpCosmoConst &C, T k, double norm)
 {
-    k *= C.h;
-    double x = 0.43 * k * C.eh_sh;
-    double x2 = x * x;
-    double geff = C.eh_omh2 * (C.eh_sqrt_alnu + (1.0 - C.eh_sqrt_alnu) / (1.0 + x2 * x2));
-    double qeff = k / geff * C.eh_theta2;
-    double L = log(2.718281828 + C.eh_cL * qeff);
-    double Cq = 14.4 + 325.0 / (1.0 + 60.5 * pow(qeff, 1.11));
-    double r = L / (L + Cq * qeff * qeff);
-    double delk = norm * norm * pow(k * 2997.9 / C.h, 3. + C.ns) * r * r;
-    double pk = 2.0 * tpd::kPi * tpd::kPi / (k * k * k) * delk;
+    k = k * C.h;
+    T x = 0.43 * k * C.eh_sh;
+    T x2 = x * x;
+    T geff = C.eh_omh2 * (C.eh_sqrt_alnu + (1.0 - C.eh_sqrt_alnu) / (1.0 + x2 * x2));
+    T qeff = k / geff * C.eh_theta2;
+    T L = log(2.718281828 + C.eh_cL * qeff);
+    T Cq = 14.4 + 325.0 / (1.0 + 60.5 * pow(qeff, 1.11));
+    T r = L / (L + Cq * qeff * qeff);
+    T delk = norm * norm * pow(k * 2997.9 / C.h, 3. + C.ns) * r * r;
+    T pk = 2.0 * tpd::kPi * tpd::kPi / (k * k * k) * delk;
     return C.h * C.h * C.h * pk;
 }
 
 // linear_pk (classed_version/bispectrum.cpp:785-809): the tabulated P(k) when given, else the fit
-__host__ __device__ __forceinline__ double tp_pk_lin(const TpCosmoConst &C, double k, double norm)
+template <class T>
+__host__ __device__ __forceinline__ T tp_pk_lin(const TpCosmoConst &C, T k, double norm)
 {
-    return C.tab.nk ? tp_pk_table(C.tab, k) : tp_pk_eh(C, k, norm);
+    if constexpr (std::is_same<T, double>::value)
+        if (C.tab.nk) return tp_pk_table(C.tab, k);
+    return tp_pk_eh<T>(C, k, norm);
 }
 
 // Eisenstein & Hu no-wiggle P_lin at k = q / r_sigma (classed_version/bispectrum.cpp:811-841).

@@ -22,11 +22,9 @@
 #include <vector>
 
 #include "tp_cubature.h"
-#include "tp_device.h"
+#include "tp_power_dev.h"
 
 namespace {
-
-constexpr double kArcmin = 2.9088820866e-4;  // classed_version/helper.cpp:139, the shape-noise term's unit
 
 // What one integral (or one evaluation call) computes at its points.
 enum PowerKind { kPkappa = 0, kMap2 = 1, kPkAtZ = 2, kLimber = 3 };
@@ -40,114 +38,6 @@ struct TpPowerDesc {
     int pad;
 };
 
-// Table values at redshift z (classed_version/bispectrum.cpp:506-513 + compute_coefficients :914-920).
-struct TpZAt {
-    double fK, g, D1, rs, ne, nc;
-};
-
-__device__ __forceinline__ void z_at(const double *T, int n_z, double zmax, double z, TpZAt *o)
-{
-    double d = z / zmax * (n_z - 1);
-    int idx;
-    if (!(d >= 0)) {  // below the table (or NaN): its first node; the cubature domains start at 0
-        idx = 0;
-        d = 0;
-    } else if (d >= n_z - 1) {  // z = z_max: the reference's (n-2, 1.0); beyond it: clamped the same way
-        idx = n_z - 2;
-        d = 1.;
-    } else {
-        idx = (int)d;
-        d -= idx;
-    }
-    const double e = 1 - d;
-    o->fK = T[idx] * e + T[idx + 1] * d;
-    o->g = T[n_z + idx] * e + T[n_z + idx + 1] * d;
-    o->D1 = T[2 * n_z + idx] * e + T[2 * n_z + idx + 1] * d;
-    o->rs = T[3 * n_z + idx] * e + T[3 * n_z + idx + 1] * d;
-    o->ne = T[4 * n_z + idx] * e + T[4 * n_z + idx + 1] * d;
-    o->nc = T[5 * n_z + idx] * e + T[5 * n_z + idx + 1] * d;
-}
-
-// Revised Halofit, classed_version/bispectrum.cpp:288-367.  The reference's flatness check always
-// passes: ow = 1 - om by construction.
-__device__ double pnl(const TpCosmoConst &C, const TpZAt &Z, double k, double z)
-{
-    const double n_eff = Z.ne, nsqr = n_eff * n_eff, nc = Z.nc;
-    // om_m_of_z, om_v_of_z (bispectrum.cpp:1022-1032): the flat form, without w
-    const double aa = 1. / (1 + z);
-    const double den = C.om + aa * (aa * aa * C.ow + (1 - C.om - C.ow));
-    const double om_m = C.om / den;
-    const double om_v = C.ow * (aa * aa * aa) / den;
-    const double f1a = pow(om_m, -0.0732), f2a = pow(om_m, -0.1423), f3a = pow(om_m, 0.0725);
-    const double f1b = pow(om_m, -0.0307), f2b = pow(om_m, -0.0585), f3b = pow(om_m, 0.0743);
-    const double frac = om_v / (1. - om_m);
-    const double f1 = frac * f1b + (1 - frac) * f1a;
-    const double f2 = frac * f2b + (1 - frac) * f2a;
-    const double f3 = frac * f3b + (1 - frac) * f3a;
-    double a = 1.5222 + 2.8553 * n_eff + 2.3706 * nsqr + 0.9903 * n_eff * nsqr + 0.2250 * nsqr * nsqr -
-               0.6038 * nc + 0.1749 * om_v * (1.0 + C.w);
-    a = pow(10.0, a);
-    const double b = pow(10.0, -0.5642 + 0.5864 * n_eff + 0.5716 * nsqr - 1.5474 * nc + 0.2279 * om_v * (1.0 + C.w));
-    const double c = pow(10.0, 0.3698 + 2.0404 * n_eff + 0.8161 * nsqr + 0.5869 * nc);
-    const double gam = 0.1971 - 0.0843 * n_eff + 0.8460 * nc;
-    const double alpha = fabs(6.0835 + 1.3373 * n_eff - 0.1959 * nsqr - 5.5274 * nc);
-    const double beta = 2.0379 - 0.7354 * n_eff + 0.3157 * nsqr + 1.2490 * n_eff * nsqr + 0.3980 * nsqr * nsqr -
-                        0.1682 * nc;
-    const double xnu = pow(10.0, 5.2105 + 3.6902 * n_eff);
-    const double twopi2 = 2 * tpd::kPi * tpd::kPi;
-    const double plin = tp_pk_lin(C, k, C.norm) * Z.D1 * Z.D1 * k * k * k / twopi2;
-    const double y = k * Z.rs, ysqr = y * y;
-    double ph = a * pow(y, f1 * 3) / (1 + b * pow(y, f2) + pow(f3 * c * y, 3 - gam));
-    ph = ph / (1 + xnu / ysqr);
-    const double pq = plin * pow(1 + plin, beta) / (1 + plin * alpha) * exp(-y / 4.0 - ysqr / 8.0);
-    return twopi2 * (pq + ph) / (k * k * k);
-}
-
-// P(k, z): Halofit, or D1^2 P_lin with TP_PK_LINEAR (linear_pk_at_z, bispectrum.cpp:369-380).
-__device__ __forceinline__ double pk_at(const TpCosmoConst &C, const TpZAt &Z, double k, double z, int flags)
-{
-    return (flags & TP_PK_LINEAR) ? Z.D1 * Z.D1 * tp_pk_lin(C, k, C.norm) : pnl(C, Z, k, z);
-}
-
-// The ratio fitted to the T17 simulations, cuda_version/bispectrum.cu:971-979.
-__device__ __forceinline__ double t17_factor(double k)
-{
-    const double c1 = 9.5171e-4, c2 = 5.1543e-3, a1 = 1.3063, a2 = 1.1475, a3 = 0.62793;
-    return pow(1 + c1 * pow(k, -1. * a1), a1) / pow(1 + c2 * pow(k, -1. * a2), a3);
-}
-
-// limber_integrand_power_spectrum, classed_version/bispectrum.cpp:543-577.
-__device__ double limber_pk(const TpCosmoConst &C, const double *T, int n_z, double ell, double z, int flags)
-{
-    if (z < 1e-5) return 0;
-    TpZAt Z;
-    z_at(T, n_z, C.zmax, z, &Z);
-    const double H = 100. / 299792.;  // H0/c [h/Mpc], classed_version/bispectrum.hpp:148
-    const double zp = 1. + z;
-    const double E = sqrt(C.om * (zp * zp * zp) + C.ow);
-    const double pref = 9. / 4. * H * H * H * C.om * C.om * zp * zp * Z.g * Z.g / E;
-    const double k = ell / Z.fK;
-    const double corr = (flags & TP_PK_T17) ? t17_factor(k) : 1.0;
-    return corr * pref * pk_at(C, Z, k, z, flags);
-}
-
-// <Map^2> integrand at (ell, z) for one source bin, cuda_version/apertureStatistics.cu:29-57.
-__device__ double map2_f(const TpCosmoConst &C, const double *T, int n_z, double ell, double z, const TpPowerDesc &D)
-{
-    const double eta = ell * D.par, t = 0.5 * eta * eta;
-    const double u = t * exp(-t);  // uHat, classed_version/apertureStatistics.cpp:10-14
-    double v = ell * (u * u) * limber_pk(C, T, n_z, ell, z, D.flags);
-    if (D.flags & TP_PK_T17) {
-        const double r = ell / (1.6 * 8192.);
-        v /= 1 + r * r;
-    }
-    if (D.sigma != 0) {
-        const double us = u * D.sigma;
-        v += ell * (us * us) * 0.5 / D.ngal * (kArcmin * kArcmin) / C.zmax;
-    }
-    return v;
-}
-
 // One thread per point; pdesc[i] names the integral (desc entry) of point i.
 __global__ void k_power_points(const TpCosmoConst *__restrict__ cc, const double *__restrict__ tab, int n_z,
                                const TpPowerDesc *__restrict__ desc, const int *__restrict__ pdesc, size_t npts,
@@ -160,21 +50,108 @@ __global__ void k_power_points(const TpCosmoConst *__restrict__ cc, const double
     const double *T = tab + (size_t)D.icosmo * 6 * n_z;
     double v;
     if (D.kind == kPkappa) {
-        v = limber_pk(C, T, n_z, D.par, x[i], D.flags);
+        v = tp_limber_pk<double>(C, T, n_z, D.par, x[i], D.flags);
     } else {
         const double x0 = x[2 * i], x1 = x[2 * i + 1];
         if (D.kind == kMap2) {
-            v = map2_f(C, T, n_z, x0, x1, D);
+            v = tp_map2_point<double>(C, T, n_z, x0, x1, D.par, D.flags, D.sigma, D.ngal);
         } else if (D.kind == kLimber) {
-            v = limber_pk(C, T, n_z, x0, x1, D.flags);
+            v = tp_limber_pk<double>(C, T, n_z, x0, x1, D.flags);
         } else {
-            TpZAt Z;
-            z_at(T, n_z, C.zmax, x1, &Z);
-            v = pk_at(C, Z, x0, x1, D.flags);
+            TpZAtT<double> Z;
+            tp_z_at<double>(T, n_z, C.zmax, x1, &Z);
+            v = tp_pk_at<double>(C, Z, x0, x1, D.flags);
         }
     }
     if (!isfinite(v)) atomicOr(flag, 1);
     f[i] = v;
+}
+
+// ---------------------------------------------------------------- 2-D regions on the device
+// The 17 Genz-Malik points of a 2-D region in the reference's order (classed_version/hcubature.cpp:
+// 456-513): centre; -+lambda2, -+lambda4 along each axis; the four (lambda4, lambda4) sign pairs;
+// the four lambda5 corners in Gray-code order from (+, +).  Codes: 0 = centre, +-1 = lambda2,
+// +-2 = lambda4, +-3 = lambda5 (sign = side).
+__constant__ signed char c_gm2_code[17][2] = {{0, 0},   {-1, 0}, {1, 0},  {-2, 0}, {2, 0},  {0, -1},
+                                              {0, 1},   {0, -2}, {0, 2},  {-2, -2}, {2, -2}, {2, 2},
+                                              {-2, 2},  {3, 3},  {-3, 3}, {-3, -3}, {3, -3}};
+constexpr int kGm2Pts = 17;
+
+struct TpRegion2 {
+    double c[2], hw[2];
+    double vol;
+    int integral;  // index into the integral descriptors of the call
+    int pad;
+};
+struct TpRegion2Out {
+    double val, err;
+    int split;
+    int flag;  // nonzero: non-finite rule estimate
+};
+
+// One thread per (region, rule point): the integrand at the point.
+__global__ void __launch_bounds__(256) k_map2_region_points(const TpCosmoConst *__restrict__ cc,
+                                                            const double *__restrict__ tab, int n_z,
+                                                            const TpPowerDesc *__restrict__ desc,
+                                                            const TpRegion2 *__restrict__ reg, size_t nreg,
+                                                            double *__restrict__ f)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nreg * kGm2Pts) return;
+    const size_t r = i / kGm2Pts;
+    const int p = (int)(i - r * kGm2Pts);
+    const TpRegion2 R = reg[r];
+    const TpPowerDesc D = desc[R.integral];
+    const double ell = tp_gm_coord(R.c[0], R.hw[0], c_gm2_code[p][0]);
+    const double z = tp_gm_coord(R.c[1], R.hw[1], c_gm2_code[p][1]);
+    f[i] = tp_map2_point<double>(cc[D.icosmo], tab + (size_t)D.icosmo * 6 * n_z, n_z, ell, z, D.par, D.flags,
+                                 D.sigma, D.ngal);
+}
+
+// One thread per region: value, error and split axis from its 17 values, in the arithmetic order of
+// the host rule (classed_version/hcubature.cpp:515-575 with dim = 2, fdim = 1), contraction off.
+__global__ void __launch_bounds__(256) k_map2_region_rule(size_t nreg, const TpRegion2 *__restrict__ reg,
+                                                          const double *__restrict__ fv,
+                                                          TpRegion2Out *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    const size_t r = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nreg) return;
+    const double *v = fv + r * kGm2Pts;
+    const double l2 = 0.3585685828003180919906451539079374954541;  // sqrt(9/70)
+    const double l4 = 0.9486832980505137995996680633298155601160;  // sqrt(9/10)
+    const double ratio = (l2 * l2) / (l4 * l4);
+    const double w1 = (double)(12824 - 9120 * 2 + 400 * 4) / 19683.0;
+    const double w3 = (double)(1820 - 400 * 2) / 19683.0;
+    const double w5 = 6859.0 / 19683.0 / 4.0;
+    const double wE1 = (double)(729 - 950 * 2 + 50 * 4) / 729.0;
+    const double wE3 = (double)(265 - 100 * 2) / 1458.0;
+    const double wt2 = 980. / 6561., wt4 = 200. / 19683., wtE2 = 245. / 486., wtE4 = 25. / 729.;
+    const double val0 = v[0];
+    double sum2 = 0, sum3 = 0, sum4 = 0, sum5 = 0, diff[2];
+    for (int k = 0; k < 2; ++k) {
+        const double a0 = v[1 + 4 * k], a1 = v[2 + 4 * k], a2 = v[3 + 4 * k], a3 = v[4 + 4 * k];
+        sum2 += a0 + a1;
+        sum3 += a2 + a3;
+        diff[k] = fabs(a0 + a1 - 2 * val0 - ratio * (a2 + a3 - 2 * val0));
+    }
+    for (int k = 0; k < 4; ++k) sum4 += v[9 + k];
+    for (int k = 0; k < 4; ++k) sum5 += v[13 + k];
+    const double vol = reg[r].vol;
+    const double result = vol * (w1 * val0 + wt2 * sum2 + w3 * sum3 + wt4 * sum4 + w5 * sum5);
+    const double res5th = vol * (wE1 * val0 + wtE2 * sum2 + wE3 * sum3 + wtE4 * sum4);
+    TpRegion2Out o;
+    o.val = result;
+    o.err = fabs(res5th - result);
+    double md = 0;  // the first axis of largest difference, the host rule's scan
+    o.split = 0;
+    for (int i = 0; i < 2; ++i)
+        if (diff[i] > md) {
+            md = diff[i];
+            o.split = i;
+        }
+    o.flag = (isfinite(result) && isfinite(res5th)) ? 0 : 1;
+    out[r] = o;
 }
 
 // 96-point Gauss-Legendre half nodes/weights (own Newton solve, uploaded once per device)
@@ -196,8 +173,8 @@ __global__ void __launch_bounds__(64) k_pell(const TpCosmoConst *__restrict__ cc
     const double e = ell[i];
     const double cx = zmax / 2, dx = zmax / 2;
     if (j < 48 && e != 0)
-        part[j] = c_pgl_w[j] * (limber_pk(C, T, n_z, e, cx - dx * c_pgl_x[j], flags) +
-                                limber_pk(C, T, n_z, e, cx + dx * c_pgl_x[j], flags));
+        part[j] = c_pgl_w[j] * (tp_limber_pk<double>(C, T, n_z, e, cx - dx * c_pgl_x[j], flags) +
+                                tp_limber_pk<double>(C, T, n_z, e, cx + dx * c_pgl_x[j], flags));
     __syncthreads();
     if (j == 0) {
         double r = shot;
@@ -342,6 +319,113 @@ int power_batch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &
     return 0;
 }
 
+// <Map^2> integrals: as power_batch, with the rule points generated and the rule applied on the
+// device, so a round moves 48 B of region in and 24 B of result out per region (not 17 points and
+// values) and the host only keeps the heaps.
+int map2_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &desc, const double *lo,
+                 const double *hi, double reltol, double *out, double *err, long long *neval)
+{
+    auto t0 = std::chrono::steady_clock::now();
+    ctx->stats = tp_batch_stats{};
+    const size_t nint = desc.size();
+    std::vector<tpc::AdaptiveIntegralT<2, 1>> I(nint);
+    for (size_t k = 0; k < nint; ++k)
+        if (!I[k].init(2, 1, lo + 2 * k, hi + 2 * k, 0, 0, reltol, tpc::L1))
+            return tp_set_error(TP_EINVAL, "bad integration domain");
+    int rc;
+    const size_t bd = nint * sizeof(TpPowerDesc);
+    if ((rc = ctx->d_desc.ensure(std::max(bd, (size_t)64)))) return rc;
+    TP_HIP(hipMemcpyAsync(ctx->d_desc.p, desc.data(), bd, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<size_t> active(nint), offs;
+    for (size_t k = 0; k < nint; ++k) active[k] = k;
+    while (!active.empty()) {
+        const size_t na = active.size();
+        offs.assign(na + 1, 0);
+        for (size_t a = 0; a < na; ++a) offs[a + 1] = offs[a] + I[active[a]].pending().size();
+        const size_t nreg = offs[na];
+        const size_t br = nreg * sizeof(TpRegion2), bo = nreg * sizeof(TpRegion2Out);
+        const size_t bf = nreg * kGm2Pts * sizeof(double);
+        if ((rc = ctx->d_regions.ensure(br)) || (rc = ctx->d_rout.ensure(bo)) || (rc = ctx->d_fvals.ensure(bf)) ||
+            (rc = ctx->h_regions.ensure(br)) || (rc = ctx->h_rout.ensure(bo)))
+            return rc;
+        TpRegion2 *hr = (TpRegion2 *)ctx->h_regions.p;
+#pragma omp parallel for schedule(dynamic, 64) if (na > 256)
+        for (size_t a = 0; a < na; ++a) {
+            auto &A = I[active[a]];
+            const auto &pend = A.pending();
+            for (size_t q = 0; q < pend.size(); ++q) {
+                const auto &R = A.regions()[pend[q]];
+                TpRegion2 &o = hr[offs[a] + q];
+                o.c[0] = R.c[0];
+                o.c[1] = R.c[1];
+                o.hw[0] = R.hw[0];
+                o.hw[1] = R.hw[1];
+                o.vol = R.vol;
+                o.integral = (int)active[a];
+                o.pad = 0;
+            }
+        }
+        TP_HIP(hipMemcpyAsync(ctx->d_regions.p, hr, br, hipMemcpyHostToDevice, ctx->stream));
+        TP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+        const size_t npts = nreg * kGm2Pts;
+        hipLaunchKernelGGL(k_map2_region_points, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, ctx->stream,
+                           c->d_const, c->d_tab, c->n_z, (const TpPowerDesc *)ctx->d_desc.p,
+                           (const TpRegion2 *)ctx->d_regions.p, nreg, (double *)ctx->d_fvals.p);
+        TP_HIP(hipGetLastError());
+        TP_HIP(hipEventRecord(ctx->evm, ctx->stream));
+        hipLaunchKernelGGL(k_map2_region_rule, dim3((unsigned)((nreg + 255) / 256)), dim3(256), 0, ctx->stream, nreg,
+                           (const TpRegion2 *)ctx->d_regions.p, (const double *)ctx->d_fvals.p,
+                           (TpRegion2Out *)ctx->d_rout.p);
+        TP_HIP(hipGetLastError());
+        TP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+        TP_HIP(hipMemcpyAsync(ctx->h_rout.p, ctx->d_rout.p, bo, hipMemcpyDeviceToHost, ctx->stream));
+        TP_HIP(hipStreamSynchronize(ctx->stream));
+        float ms = 0, ms_pts = 0;
+        hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+        hipEventElapsedTime(&ms_pts, ctx->ev0, ctx->evm);
+        ctx->stats.kernel_ms += ms;
+        ctx->stats.integrand_ms += ms_pts;
+        ctx->stats.launches += 1;
+        ctx->stats.rounds += 1;
+        ctx->stats.points += (long long)npts;
+        const TpRegion2Out *ho = (const TpRegion2Out *)ctx->h_rout.p;
+        std::vector<char> fin(na, 0);
+        int bad = 0;
+#pragma omp parallel for schedule(dynamic, 64) if (na > 256) reduction(| : bad)
+        for (size_t a = 0; a < na; ++a) {
+            auto &A = I[active[a]];
+            const auto &pend = A.pending();
+            for (size_t q = 0; q < pend.size(); ++q) {
+                auto &R = A.regions()[pend[q]];
+                const TpRegion2Out &o = ho[offs[a] + q];
+                R.val[0] = o.val;
+                R.err[0] = o.err;
+                R.split = (unsigned)o.split;
+                R.errmax = o.err;
+                R.corr = 0.0;
+                R.cvar = 0.0;
+                bad |= o.flag;
+            }
+            A.absorb();
+            fin[a] = A.advance() ? 1 : 0;
+        }
+        if (bad) return tp_set_error(TP_ENONFINITE, "<Map^2> integrand: non-finite value");
+        std::vector<size_t> next;
+        for (size_t a = 0; a < na; ++a)
+            if (!fin[a]) next.push_back(active[a]);
+        active.swap(next);
+    }
+    for (size_t k = 0; k < nint; ++k) {
+        double v, e;
+        I[k].result(&v, &e);
+        out[k] = v / 2. / tpd::kPi;  // apertureStatistics.cu:135
+        if (err) err[k] = e / 2. / tpd::kPi;
+        if (neval) neval[k] = (long long)I[k].num_eval();
+    }
+    ctx->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
 int point_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, int kind, size_t npts, const double *in, int flags,
                double *out)
 {
@@ -446,5 +530,5 @@ extern "C" int tp_map2_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int
         lo[2 * i + 1] = 0;             // z range [0, z_max]
         hi[2 * i + 1] = c->z_max;
     }
-    return power_batch<2>(ctx, c, desc, lo.data(), hi.data(), reltol, true, out, err, neval);
+    return map2_regions(ctx, c, desc, lo.data(), hi.data(), reltol, out, err, neval);
 }

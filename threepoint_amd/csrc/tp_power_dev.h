@@ -1,0 +1,126 @@
+// tp_power_dev.h -- per-point functions of the convergence power spectrum and <Map^2>.
+//
+// Templates on the arithmetic type T: double in the kernels of tp_power.hip, a counting type in the
+// operation census (tp_census.cpp).  tp_dbl(x) gives the value of x as a double for table indexing.
+#pragma once
+#include "tp_device.h"
+
+__host__ __device__ __forceinline__ double tp_dbl(double x) { return x; }
+
+// Table values at redshift z (classed_version/bispectrum.cpp:506-513 + compute_coefficients :914-920).
+template <class T>
+struct TpZAtT {
+    T fK, g, D1, rs, ne, nc;
+};
+
+template <class T>
+__host__ __device__ __forceinline__ void tp_z_at(const double *Tab, int n_z, double zmax, T z, TpZAtT<T> *o)
+{
+    double d = tp_dbl(z) / zmax * (n_z - 1);
+    int idx;
+    if (!(d >= 0)) {  // below the table (or NaN): its first node; the cubature domains start at 0
+        idx = 0;
+        d = 0;
+    } else if (d >= n_z - 1) {  // z = z_max: the reference's (n-2, 1.0); beyond it: clamped the same way
+        idx = n_z - 2;
+        d = 1.;
+    } else {
+        idx = (int)d;
+        d -= idx;
+    }
+    const T dd = d, e = T(1.0) - dd;
+    o->fK = Tab[idx] * e + Tab[idx + 1] * dd;
+    o->g = Tab[n_z + idx] * e + Tab[n_z + idx + 1] * dd;
+    o->D1 = Tab[2 * n_z + idx] * e + Tab[2 * n_z + idx + 1] * dd;
+    o->rs = Tab[3 * n_z + idx] * e + Tab[3 * n_z + idx + 1] * dd;
+    o->ne = Tab[4 * n_z + idx] * e + Tab[4 * n_z + idx + 1] * dd;
+    o->nc = Tab[5 * n_z + idx] * e + Tab[5 * n_z + idx + 1] * dd;
+}
+
+// Revised Halofit (Takahashi et al. 2012), classed_version/bispectrum.cpp:288-367.  The reference's
+// flatness check always passes: ow = 1 - om by construction.
+template <class T>
+__host__ __device__ T tp_pnl(const TpCosmoConst &C, const TpZAtT<T> &Z, T k, T z)
+{
+    const T n_eff = Z.ne, nsqr = n_eff * n_eff, nc = Z.nc;
+    // om_m_of_z, om_v_of_z (bispectrum.cpp:1022-1032): the flat form, without w
+    const T aa = 1. / (1 + z);
+    const T den = C.om + aa * (aa * aa * C.ow + (1 - C.om - C.ow));
+    const T om_m = C.om / den;
+    const T om_v = C.ow * (aa * aa * aa) / den;
+    const T f1a = pow(om_m, -0.0732), f2a = pow(om_m, -0.1423), f3a = pow(om_m, 0.0725);
+    const T f1b = pow(om_m, -0.0307), f2b = pow(om_m, -0.0585), f3b = pow(om_m, 0.0743);
+    const T frac = om_v / (1. - om_m);
+    const T f1 = frac * f1b + (1 - frac) * f1a;
+    const T f2 = frac * f2b + (1 - frac) * f2a;
+    const T f3 = frac * f3b + (1 - frac) * f3a;
+    T a = 1.5222 + 2.8553 * n_eff + 2.3706 * nsqr + 0.9903 * n_eff * nsqr + 0.2250 * nsqr * nsqr - 0.6038 * nc +
+          0.1749 * om_v * (1.0 + C.w);
+    a = pow(10.0, a);
+    const T b = pow(10.0, -0.5642 + 0.5864 * n_eff + 0.5716 * nsqr - 1.5474 * nc + 0.2279 * om_v * (1.0 + C.w));
+    const T c = pow(10.0, 0.3698 + 2.0404 * n_eff + 0.8161 * nsqr + 0.5869 * nc);
+    const T gam = 0.1971 - 0.0843 * n_eff + 0.8460 * nc;
+    const T alpha = fabs(6.0835 + 1.3373 * n_eff - 0.1959 * nsqr - 5.5274 * nc);
+    const T beta = 2.0379 - 0.7354 * n_eff + 0.3157 * nsqr + 1.2490 * n_eff * nsqr + 0.3980 * nsqr * nsqr -
+                   0.1682 * nc;
+    const T xnu = pow(10.0, 5.2105 + 3.6902 * n_eff);
+    const double twopi2 = 2 * tpd::kPi * tpd::kPi;
+    const T plin = tp_pk_lin<T>(C, k, C.norm) * Z.D1 * Z.D1 * k * k * k / twopi2;
+    const T y = k * Z.rs, ysqr = y * y;
+    T ph = a * pow(y, f1 * 3) / (1 + b * pow(y, f2) + pow(f3 * c * y, 3 - gam));
+    ph = ph / (1 + xnu / ysqr);
+    const T pq = plin * pow(1 + plin, beta) / (1 + plin * alpha) * exp(-y / 4.0 - ysqr / 8.0);
+    return twopi2 * (pq + ph) / (k * k * k);
+}
+
+// P(k, z): Halofit, or D1^2 P_lin with TP_PK_LINEAR (linear_pk_at_z, bispectrum.cpp:369-380).
+template <class T>
+__host__ __device__ __forceinline__ T tp_pk_at(const TpCosmoConst &C, const TpZAtT<T> &Z, T k, T z, int flags)
+{
+    return (flags & TP_PK_LINEAR) ? Z.D1 * Z.D1 * tp_pk_lin<T>(C, k, C.norm) : tp_pnl<T>(C, Z, k, z);
+}
+
+// The ratio fitted to the T17 simulations, cuda_version/bispectrum.cu:971-979.
+template <class T>
+__host__ __device__ __forceinline__ T tp_t17_factor(T k)
+{
+    const double c1 = 9.5171e-4, c2 = 5.1543e-3, a1 = 1.3063, a2 = 1.1475, a3 = 0.62793;
+    return pow(1 + c1 * pow(k, -1. * a1), a1) / pow(1 + c2 * pow(k, -1. * a2), a3);
+}
+
+// limber_integrand_power_spectrum, classed_version/bispectrum.cpp:543-577.
+template <class T>
+__host__ __device__ T tp_limber_pk(const TpCosmoConst &C, const double *Tab, int n_z, T ell, T z, int flags)
+{
+    if (tp_dbl(z) < 1e-5) return T(0.0);
+    TpZAtT<T> Z;
+    tp_z_at<T>(Tab, n_z, C.zmax, z, &Z);
+    const double H = 100. / 299792.;  // H0/c [h/Mpc], classed_version/bispectrum.hpp:148
+    const T zp = 1. + z;
+    const T E = sqrt(C.om * (zp * zp * zp) + C.ow);
+    const T pref = 9. / 4. * H * H * H * C.om * C.om * zp * zp * Z.g * Z.g / E;
+    const T k = ell / Z.fK;
+    const T P = tp_pk_at<T>(C, Z, k, z, flags);
+    return (flags & TP_PK_T17) ? tp_t17_factor<T>(k) * pref * P : T(1.0) * pref * P;
+}
+
+// <Map^2> integrand at (ell, z) for one source bin, cuda_version/apertureStatistics.cu:29-57:
+// theta the aperture radius, sigma/ngal the shape noise (sigma = 0: none; ngal in arcmin^-2).
+template <class T>
+__host__ __device__ T tp_map2_point(const TpCosmoConst &C, const double *Tab, int n_z, T ell, T z, double theta,
+                                    int flags, double sigma, double ngal)
+{
+    const double arcmin = 2.9088820866e-4;  // classed_version/helper.cpp:139
+    const T eta = ell * theta, t = 0.5 * eta * eta;
+    const T u = t * exp(-t);  // uHat, classed_version/apertureStatistics.cpp:10-14
+    T v = ell * (u * u) * tp_limber_pk<T>(C, Tab, n_z, ell, z, flags);
+    if (flags & TP_PK_T17) {
+        const T r = ell / (1.6 * 8192.);
+        v = v / (1 + r * r);
+    }
+    if (sigma != 0) {
+        const T us = u * sigma;
+        v = v + ell * (us * us) * 0.5 / ngal * (arcmin * arcmin) / C.zmax;
+    }
+    return v;
+}
