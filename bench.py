@@ -332,6 +332,74 @@ def map2_cpu_baseline(W, cos, radii, seconds_target):
                       "%d threads, %s, %.1f s" % (done, len(orcs), cores, cpu_model(), dt)}
 
 
+def cov_bench(tp, W, ctx, args):
+    """T1 of the Gaussian <Map^3> covariance (infinite survey) for all 210 pairs of the 20 triples of
+    radii {2, 4, 8, 16} arcmin: the survey of the reference's example parameter file (61.94 deg^2,
+    sigma_eps 0.37, 108000 galaxies per deg^2), T17 spectrum, rel tol 1e-4."""
+    t0 = time.perf_counter()
+    tabs = tp.CosmologyTables([W.fiducial()], W.fiducial_nz(), 100, 3.0, ctx)
+    tables_ms = 1e3 * (time.perf_counter() - t0)
+    deg = 0.017453  # classed_version/helper.cpp:142 (the reference's degree constant)
+    area = 61.9369 * deg * deg
+    n_gal = 108000 / deg / deg
+    tri = W.to_rad(W.triples(W.map3_c0_radii()))
+    pairs = np.array([np.concatenate([tri[i], tri[j]]) for i in range(len(tri)) for j in range(i, len(tri))])
+    flags = tp.PK_T17
+    results = {}
+    print("bench: cov T1 ...", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        tabs.map3_cov_t1(pairs[:8], area, 0.0, 0.37, n_gal, flags)
+    tt = time.perf_counter()
+    for _ in range(args.steps):
+        v, ne = tabs.map3_cov_t1(pairs, area, 0.0, 0.37, n_gal, flags)
+    dt = time.perf_counter() - tt
+    st = ctx.stats()
+    pts = st.get("points", 0)
+    node_pairs = 3 * 48 * pts  # three Pell per point, 48 node pairs each
+    agg = {"ms_per_step": 1e3 * dt / args.steps, "pairs_per_s": pairs.shape[0] * args.steps / dt,
+           "points": pts, "integrand_ms": st.get("integrand_ms", 0.0), "launches": st.get("launches", 0),
+           "rounds": st.get("rounds", 0)}
+    c = tp.census_power_point()
+    # a T1 point: 3 Pell of 96 z-nodes, each node the k-dependent part of the Limber P (the z-only Halofit
+    # coefficients are kept per node); counted as the full Limber P point of the census, an upper bound
+    fe_point = 3 * 96 * (c["pk_flops"] + FLOPS_PER_ELEM * c["pk_elem"])
+    ach = fe_point * pts / (agg["integrand_ms"] * 1e-3) if agg["integrand_ms"] > 0 else 0.0
+    results["batched_fp64"] = agg
+    line = {
+        "metric": "Gaussian <Map^3> covariance T1 pairs/s on 1 GPU (infinite survey)",
+        "value": agg["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": agg["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "T1_total of 210 pairs of the 20 triples of radii 2-16 arcmin, infinite survey of "
+                               "61.94 deg^2, sigma_eps 0.37, 108000 / deg^2, T17 revised Halofit, 3-D hcubature "
+                               "rel tol 1e-4", "path": "batched_fp64", "tables_ms": tables_ms,
+                   "t1_integrals": int(sum({3: 6, 2: 3, 1: 1}[len(set(p[3:]))] for p in pairs))},
+        "paths": results,
+        "roofline": {"bound": "valu", "kernel": "k_power_points (T1)", "achieved": ach / 1e12,
+                     "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK["fp64"], "traffic": None,
+                     "work_per_point_flop_eq": fe_point, "flops_per_elementary_call": FLOPS_PER_ELEM,
+                     "node_evaluations_per_point": 288},
+    }
+    if not args.no_cpu:
+        print("bench: cpu baseline ...", file=sys.stderr, flush=True)
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from _oracle import Oracle
+        cores = min(os.cpu_count() or 1, 64)
+        o = Oracle(W.fiducial(), W.fiducial_nz())
+        done, pts_done, t1 = 0, 0, time.perf_counter()
+        while time.perf_counter() - t1 < args.cpu_seconds and done < pairs.shape[0]:
+            p = pairs[(done * 37) % pairs.shape[0]]
+            _, n = o.t1_total(p[:3], p[3:], area, 0.0, 0.37, n_gal, flags, 1e-4, cores)
+            done += 1
+            pts_done += n
+        d1 = time.perf_counter() - t1
+        line["cpu_baseline"] = {"value": done / d1, "unit": "pairs/s", "cores": cores, "kind": "port",
+                                "sample": "%d of the 210 pairs (every 37th) through the FP64 CPU oracle's T1_total, "
+                                          "OpenMP %d threads over each cubature round, %s, %.1f s"
+                                          % (done, cores, cpu_model(), d1)}
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -341,7 +409,7 @@ def main():
     ap.add_argument("--paths", default="plain_fp64,batched_fp64,batched_fp32,surrogate")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "map2"])
+    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "map2", "cov"])
     ap.add_argument("--c2-triangles", type=int, default=10,
                     help="Gamma workload: number of the 1000 TreeCorr-grid triangles (one per r bin at 10)")
     ap.add_argument("--surrogate", default=os.path.join(ROOT, "threepoint_amd", "models", "bkappa_mlp.bin"))
@@ -372,6 +440,9 @@ def main():
         return
     if args.workload == "map2":
         map2_bench(tp, W, ctx, args)
+        return
+    if args.workload == "cov":
+        cov_bench(tp, W, ctx, args)
         return
     t0 = time.perf_counter()
     icosmo = None

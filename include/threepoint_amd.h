@@ -184,8 +184,10 @@ int tp_gamma_batch(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t ntri, cons
 enum {
     TP_PK_LINEAR = 1, /* D1^2 P_lin instead of revised Halofit (limber_integrand_power_spectrum's
                          nonlinear = false, classed_version/bispectrum.cpp:569-572)               */
-    TP_PK_T17 = 2     /* the cuda_version's T17 correction of the Limber integrand and of <Map^2>
+    TP_PK_T17 = 2,    /* the cuda_version's T17 correction of the Limber integrand and of <Map^2>
                          (cuda_version/bispectrum.cu:971-979, apertureStatistics.cu:39-46)        */
+    TP_PK_SHAPENOISE_ONLY = 4 /* Pell is the shape noise alone: the cuda_version's constant spectrum
+                         (constant_powerspectrum, bispectrum.cu:515-516); tp_pell and T1 only     */
 };
 /* Replaces BispectrumCalculator::P_k_nonlinear(k, z) (classed_version/bispectrum.cpp:288-367) and,
  * with TP_PK_LINEAR, linear_pk_at_z (:369-380): kz is npts*2 (k [h/Mpc], z), out npts values
@@ -216,6 +218,27 @@ int tp_pkappa_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo,
 int tp_map2_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo, const double *theta,
                   double reltol, int flags, double sigma, double ngal_arcmin2, double *out, double *err,
                   long long *neval);
+
+/* ------------------------------------------ Gaussian <Map^3> covariance */
+/* Replaces ApertureStatistics::MapMapMap_covariance_Gauss (classed_version/apertureStatistics.cpp:
+ * 355-396) for n pairs of radius triples at once: theta6[6i..6i+5] = theta_123, theta_456 [rad];
+ * 4-D hcubature over ell1, ell2 in [1, 10 / max(min theta_123, min theta_456)], phi in [0, pi],
+ * z in [0, z_max] of uHat products x the single-z triple power spectrum (bispectrum.cpp:580-607),
+ * relative tolerance reltol (the reference's 1e-3); result 2 / area / (2 pi)^3. */
+int tp_map3_cov_gauss_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo, const double *theta6,
+                            double survey_area, double reltol, double *out, double *err, long long *neval);
+/* Replaces T1_total (cuda_version/apertureStatisticsCovariance.cu:272-326) for an infinite survey
+ * (type 2, T1 :1065-1106, integrand :1958-1985) for n pairs at once: each is a sum of T1 integrals
+ * over the distinct permutations of theta_456, each a 3-D hcubature over ell1, ell2 in
+ * [lmin, 10 / max(min theta_123, min theta_456)], phi in [0, pi] of Pell(l1) Pell(l2) Pell(l3)
+ * times both uHat products, relative tolerance reltol (the reference's 1e-4), times 2 / area /
+ * (2 pi)^3 (area in rad^2).  Pell carries the shape noise 0.5 sigma^2 / n_gal (n_gal in rad^-2);
+ * flags: TP_PK_T17 (the cuda_version's default), TP_PK_SHAPENOISE_ONLY.  T2 vanishes for an
+ * infinite survey (apertureStatisticsCovariance.cu:330-331); the finite geometries are not provided.
+ * neval receives each pair's integrand points. */
+int tp_map3_cov_t1_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo, const double *theta6,
+                         double area, double lmin, double sigma, double n_gal, int flags, double reltol,
+                         double *out, long long *neval);
 
 /* ------------------------------------------------------------- cubature */
 /* Replaces hcubature_v (classed_version/cubature.h:90-94, hcubature.cpp:1059-1068):

@@ -12,6 +12,9 @@
  *   - Gamma_i (Ogata):       CV/gamma.cpp:6-81, 121-219, 291-377, 462-563, 624-643
  *   - P_kappa, Halofit:      CV/bispectrum.cpp:288-417, 543-577, 1022-1059
  *   - <Map^2>, T17 factor:   GV/apertureStatistics.cu:29-136, GV/bispectrum.cu:951-982
+ *   - Gaussian covariance:   CV/apertureStatistics.cpp:50-159, 355-396, CV/bispectrum.cpp:580-607;
+ *                            T1 (infinite survey) GV/apertureStatisticsCovariance.cu:1065-1106,
+ *                            1958-1985
  *                            ("GV" = cuda_version/)
  *   - file readers:          CV/helper.cpp:31-96, CV/cosmology.cpp:10-72
  * GSL routines the reference calls are replaced by:
@@ -1274,6 +1277,65 @@ struct Calc {
         return GQ96_of_Pk(0, z_max, ell, true, t17) + P_shapenoise;
     }
 
+    /* ---------------- Gaussian <Map^3> covariance ---------------- */
+    double limber_integrand_triple_power_spectrum(double ell1, double ell2, double ell3, double z) const
+    { /* CV/bispectrum.cpp:580-607 (the f_K = 0 exit cannot trigger for z >= 1e-5) */
+        if (z < 1e-5) return 0;
+        int idx;
+        double didx;
+        z_index(z, &idx, &didx);
+        double g_value = g[idx] * (1 - didx) + g[idx + 1] * didx;
+        double f_K_value = fK[idx] * (1 - didx) + fK[idx + 1] * didx;
+        double prefactor = 9. / 4. * H0_over_c * H0_over_c * H0_over_c * cosmo.om * cosmo.om * (1. + z) * (1. + z) *
+                           g_value * g_value / E(z);
+        return std::pow(prefactor, 3) * P_k_nonlinear(ell1 / f_K_value, z) * P_k_nonlinear(ell2 / f_K_value, z) *
+               P_k_nonlinear(ell3 / f_K_value, z);
+    }
+
+    static double uhat_product(double l1, double l2, double l3, const double *th)
+    { /* CV/apertureStatistics.cpp:50-63 */
+        return uHat(l1 * th[0]) * uHat(l2 * th[1]) * uHat(l3 * th[2]);
+    }
+
+    double cov_gauss_integrand(double l1, double l2, double phi, double z, const double *th6) const
+    { /* CV/apertureStatistics.cpp:66-106: uHat_product(123) x the six permutations of (456), the
+         single-z triple power spectrum, l1 l2 */
+        double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * std::cos(phi));
+        double result = uhat_product(l1, l2, l3, th6);
+        double perm = uhat_product(l1, l2, l3, th6 + 3);
+        perm += uhat_product(l2, l3, l1, th6 + 3);
+        perm += uhat_product(l3, l1, l2, th6 + 3);
+        perm += uhat_product(l1, l3, l2, th6 + 3);
+        perm += uhat_product(l3, l2, l1, th6 + 3);
+        perm += uhat_product(l2, l1, l3, th6 + 3);
+        result *= perm;
+        result *= limber_integrand_triple_power_spectrum(l1, l2, l3, z);
+        result *= l1 * l2;
+        return result;
+    }
+
+    double pell_flags(double ell, double sigma, double n_gal, int flags) const
+    { /* the cuda_version's device Pell (GV/bispectrum.cu:511-529, __CUDA_ARCH__ branch): the
+         96-node Limber sum + shape noise, shape noise alone with the constant spectrum (flags & 4) */
+        double P_shapenoise = 0.5 * sigma * sigma / n_gal;
+        if (flags & 4) return P_shapenoise;
+        if (ell == 0) return P_shapenoise;
+        return GQ96_of_Pk(0, z_max, ell, !(flags & 1), (flags & 2) != 0) + P_shapenoise;
+    }
+
+    double t1_integrand(double l1, double l2, double phi, const double *th6, double lmin, double lmax, double sigma,
+                        double n_gal, int flags) const
+    { /* GV/apertureStatisticsCovariance.cu:1958-1985 (integrand_T1_infinite) */
+        double l3 = std::sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * std::cos(phi));
+        if (l1 <= lmin || l2 <= lmin || l3 <= lmin || l3 > lmax) return 0;
+        double result = pell_flags(l1, sigma, n_gal, flags) * pell_flags(l2, sigma, n_gal, flags) *
+                        pell_flags(l3, sigma, n_gal, flags);
+        result *= uHat(l1 * th6[0]) * uHat(l2 * th6[1]) * uHat(l3 * th6[2]);
+        result *= uHat(l1 * th6[3]) * uHat(l2 * th6[4]) * uHat(l3 * th6[5]);
+        result *= l1 * l2;
+        return result;
+    }
+
     double map2_integrand(double ell, double z, double theta, bool nonlinear, bool t17, double sigma,
                           double ngal_arcmin2) const
     { /* GV/apertureStatistics.cu:29-57 for one source bin; sigma = 0 drops the shape-noise term */
@@ -1362,6 +1424,34 @@ int map2_cb(unsigned ndim, size_t npts, const double *x, void *fdata, unsigned f
     if (fdim != 1 || ndim != 2) return 1;
     for (size_t i = 0; i < npts; i++)
         fval[i] = p->c->map2_integrand(x[2 * i], x[2 * i + 1], p->ell, p->nonlinear, p->t17, p->sigma, p->ngal);
+    return 0;
+}
+
+struct CovCtx {
+    const Calc *c;
+    double th[6];
+    double lmin, lmax, sigma, ngal;
+    int flags, nthreads;
+};
+
+int cov_gauss_cb(unsigned ndim, size_t npts, const double *x, void *fdata, unsigned fdim, double *fval)
+{ /* CV/apertureStatistics.cpp:108-159 */
+    CovCtx *p = (CovCtx *)fdata;
+    if (fdim != 1 || ndim != 4) return 1;
+#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+    for (long long i = 0; i < (long long)npts; i++)
+        fval[i] = p->c->cov_gauss_integrand(x[4 * i], x[4 * i + 1], x[4 * i + 2], x[4 * i + 3], p->th);
+    return 0;
+}
+
+int t1_cb(unsigned ndim, size_t npts, const double *x, void *fdata, unsigned fdim, double *fval)
+{ /* GV/apertureStatisticsCovariance.cu integrand_T1 with the infinite-survey kernel */
+    CovCtx *p = (CovCtx *)fdata;
+    if (fdim != 1 || ndim != 3) return 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(p->nthreads)
+    for (long long i = 0; i < (long long)npts; i++)
+        fval[i] = p->c->t1_integrand(x[3 * i], x[3 * i + 1], x[3 * i + 2], p->th, p->lmin, p->lmax, p->sigma, p->ngal,
+                                     p->flags);
     return 0;
 }
 
@@ -1672,6 +1762,100 @@ int orc_map2(void *h, double theta, double reltol, int flags, double sigma, doub
     *val = result / 2. / M_PI;
     *err = error / 2. / M_PI;
     if (neval) *neval = g_last_neval;
+    return rc;
+}
+
+double orc_limber_triple_pk(void *h, double l1, double l2, double l3, double z)
+{
+    return ((Calc *)h)->limber_integrand_triple_power_spectrum(l1, l2, l3, z);
+}
+double orc_cov_gauss_integrand(void *h, double l1, double l2, double phi, double z, const double *th6)
+{
+    return ((Calc *)h)->cov_gauss_integrand(l1, l2, phi, z, th6);
+}
+double orc_t1_integrand(void *h, double l1, double l2, double phi, const double *th6, double lmin, double lmax,
+                        double sigma, double n_gal, int flags)
+{
+    return ((Calc *)h)->t1_integrand(l1, l2, phi, th6, lmin, lmax, sigma, n_gal, flags);
+}
+
+int orc_map3_cov_gauss(void *h, const double *th6, double survey_area, double reltol, int nthreads, double *val,
+                       double *err, long long *neval)
+{ /* CV/apertureStatistics.cpp:355-396 (INTEGRATE4D): 4-D hcubature over (l1, l2, phi in [0, pi], z),
+     lMax = 10 / max(min theta_123, min theta_456), result 2 / area / (2 pi)^3 */
+    Calc *c = (Calc *)h;
+    double m123 = std::min({th6[0], th6[1], th6[2]}), m456 = std::min({th6[3], th6[4], th6[5]});
+    double lMax = 10. / std::max(m123, m456);
+    CovCtx p{c, {th6[0], th6[1], th6[2], th6[3], th6[4], th6[5]}, 1, lMax, 0, 1, 0, nthreads_or_default(nthreads)};
+    double vmin[4] = {1, 1, 0, 0}, vmax[4] = {lMax, lMax, 2 * M_PI / 2., c->z_max};
+    double result = 0, error = 0;
+    int rc = hcubature_v_impl(1, cov_gauss_cb, &p, 4, vmin, vmax, 0, 0, reltol, ORC_ERROR_L1, &result, &error);
+    *val = 2. * result / survey_area / 8 / M_PI / M_PI / M_PI;
+    *err = 2. * error / survey_area / 8 / M_PI / M_PI / M_PI;
+    if (neval) *neval = g_last_neval;
+    return rc;
+}
+
+int orc_t1(void *h, const double *th6, double area, double lmin, double sigma, double n_gal, int flags,
+           double reltol, int nthreads, double *val, double *err, long long *neval)
+{ /* GV/apertureStatisticsCovariance.cu:1065-1106, infinite survey (type 2): 3-D hcubature over
+     [lMin, lMax]^2 x [0, pi], lMax = 10 / max(min theta_123, min theta_456), x 2 / area / (2 pi)^3 */
+    Calc *c = (Calc *)h;
+    double m123 = std::min({th6[0], th6[1], th6[2]}), m456 = std::min({th6[3], th6[4], th6[5]});
+    double lMax = 10. / std::max(m123, m456);
+    CovCtx p{c, {th6[0], th6[1], th6[2], th6[3], th6[4], th6[5]}, lmin, lMax, sigma, n_gal, flags,
+             nthreads_or_default(nthreads)};
+    double vmin[3] = {lmin, lmin, 0}, vmax[3] = {lMax, lMax, M_PI};
+    double result = 0, error = 0;
+    int rc = hcubature_v_impl(1, t1_cb, &p, 3, vmin, vmax, 0, 0, reltol, ORC_ERROR_L1, &result, &error);
+    result *= 2 / area / std::pow(2 * M_PI, 3);
+    error *= 2 / area / std::pow(2 * M_PI, 3);
+    *val = result;
+    *err = error;
+    if (neval) *neval = g_last_neval;
+    return rc;
+}
+
+int orc_t1_total(void *h, const double *th123, const double *th456, double area, double lmin, double sigma,
+                 double n_gal, int flags, double reltol, int nthreads, double *val, long long *neval)
+{ /* GV/apertureStatisticsCovariance.cu:272-326: T1 summed over the distinct permutations of theta_456 */
+    const double th0 = th123[0], th1 = th123[1], th2 = th123[2], th3 = th456[0], th4 = th456[1], th5 = th456[2];
+    long long ntot = 0;
+    int rc = 0;
+    auto T1 = [&](double a, double b, double c) {
+        double th6[6] = {th0, th1, th2, a, b, c}, v = 0, e = 0;
+        long long n = 0;
+        rc |= orc_t1(h, th6, area, lmin, sigma, n_gal, flags, reltol, nthreads, &v, &e, &n);
+        ntot += n;
+        return v;
+    };
+    double result;
+    if (th3 == th4) {
+        if (th4 == th5) {
+            result = 6 * T1(th3, th3, th3);
+        } else {
+            result = 2 * T1(th3, th3, th5);
+            result += 2 * T1(th3, th5, th3);
+            result += 2 * T1(th5, th3, th3);
+        }
+    } else if (th3 == th5) {
+        result = 2 * T1(th3, th4, th3);
+        result += 2 * T1(th3, th3, th4);
+        result += 2 * T1(th4, th3, th3);
+    } else if (th4 == th5) {
+        result = 2 * T1(th3, th4, th4);
+        result += 2 * T1(th4, th3, th4);
+        result += 2 * T1(th4, th4, th3);
+    } else {
+        result = T1(th3, th4, th5);
+        result += T1(th3, th5, th4);
+        result += T1(th4, th3, th5);
+        result += T1(th4, th5, th3);
+        result += T1(th5, th3, th4);
+        result += T1(th5, th4, th3);
+    }
+    *val = result;
+    if (neval) *neval = ntot;
     return rc;
 }
 

@@ -97,6 +97,19 @@ int orc_convergence_power_spectrum(void *h, double ell, double reltol, int flags
 int orc_map2(void *h, double theta, double reltol, int flags, double sigma, double ngal_arcmin2,
              double *val, double *err, long long *neval);
 
+/* Gaussian <Map^3> covariance: the classed_version's 4-D form and the cuda_version's T1 for an
+   infinite survey (flags: 1 linear, 2 T17, 4 shape noise only; th6 = theta_123, theta_456 in rad) */
+double orc_limber_triple_pk(void *h, double l1, double l2, double l3, double z);
+double orc_cov_gauss_integrand(void *h, double l1, double l2, double phi, double z, const double *th6);
+double orc_t1_integrand(void *h, double l1, double l2, double phi, const double *th6, double lmin,
+                        double lmax, double sigma, double n_gal, int flags);
+int orc_map3_cov_gauss(void *h, const double *th6, double survey_area, double reltol, int nthreads,
+                       double *val, double *err, long long *neval);
+int orc_t1(void *h, const double *th6, double area, double lmin, double sigma, double n_gal, int flags,
+           double reltol, int nthreads, double *val, double *err, long long *neval);
+int orc_t1_total(void *h, const double *th123, const double *th456, double area, double lmin, double sigma,
+                 double n_gal, int flags, double reltol, int nthreads, double *val, long long *neval);
+
 #ifdef __cplusplus
 }
 #endif

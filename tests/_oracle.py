@@ -63,6 +63,16 @@ _SIGS = {
                                                       ctypes.c_int, _dp, _dp, _llp]),
     "orc_map2": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                 ctypes.c_double, _dp, _dp, _llp]),
+    "orc_limber_triple_pk": (ctypes.c_double, [ctypes.c_void_p] + [ctypes.c_double] * 4),
+    "orc_cov_gauss_integrand": (ctypes.c_double, [ctypes.c_void_p] + [ctypes.c_double] * 4 + [_dp]),
+    "orc_t1_integrand": (ctypes.c_double, [ctypes.c_void_p] + [ctypes.c_double] * 3 + [_dp] +
+                         [ctypes.c_double] * 4 + [ctypes.c_int]),
+    "orc_map3_cov_gauss": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp,
+                                          _dp, _llp]),
+    "orc_t1": (ctypes.c_int, [ctypes.c_void_p, _dp] + [ctypes.c_double] * 4 + [ctypes.c_int, ctypes.c_double,
+                                                                                ctypes.c_int, _dp, _dp, _llp]),
+    "orc_t1_total": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp] + [ctypes.c_double] * 4 +
+                     [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _llp]),
     "orc_read_cosmology": (ctypes.c_int, [ctypes.c_char_p, _dp]),
     "orc_read_n_of_z": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_double, _dp]),
 }
@@ -170,6 +180,28 @@ class Oracle:
                             ctypes.byref(n))
         assert rc == 0
         return v.value, e.value, n.value
+
+    def map3_cov_gauss(self, th6, area, reltol=1e-3, nthreads=0):
+        th = np.ascontiguousarray(th6, dtype=np.float64)
+        v, e, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        assert lib().orc_map3_cov_gauss(self.h, d(th), area, reltol, nthreads, ctypes.byref(v), ctypes.byref(e),
+                                        ctypes.byref(n)) == 0
+        return v.value, e.value, n.value
+
+    def t1(self, th6, area, lmin=0.0, sigma=0.3, n_gal=1.0, flags=0, reltol=1e-4, nthreads=0):
+        th = np.ascontiguousarray(th6, dtype=np.float64)
+        v, e, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
+        assert lib().orc_t1(self.h, d(th), area, lmin, sigma, n_gal, flags, reltol, nthreads, ctypes.byref(v),
+                            ctypes.byref(e), ctypes.byref(n)) == 0
+        return v.value, e.value, n.value
+
+    def t1_total(self, th123, th456, area, lmin=0.0, sigma=0.3, n_gal=1.0, flags=0, reltol=1e-4, nthreads=0):
+        a = np.ascontiguousarray(th123, dtype=np.float64)
+        b = np.ascontiguousarray(th456, dtype=np.float64)
+        v, n = ctypes.c_double(), ctypes.c_longlong()
+        assert lib().orc_t1_total(self.h, d(a), d(b), area, lmin, sigma, n_gal, flags, reltol, nthreads,
+                                  ctypes.byref(v), ctypes.byref(n)) == 0
+        return v.value, n.value
 
     def scalar(self, name, *args):
         return getattr(lib(), "orc_" + name)(self.h, *args)

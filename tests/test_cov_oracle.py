@@ -1,0 +1,122 @@
+"""Pins of the oracle's Gaussian <Map^3> covariance (CPU only).
+
+  * T1 of an infinite survey with the constant (shape-noise-only) spectrum and all six radii equal has
+    a closed form: T1 = P^3 pi^2 E0 / (192 theta^4 A (2 pi)^4), E0 = E[l1^4 l2^4 l3^4] for (l1, l2)
+    Gaussian 2-vectors with precision 2 M, M the quadratic form of l1^2 + l2^2 + |l1 + l2|^2
+    (Isserlis' theorem, exact rational arithmetic).  It fixes the cubature domain, the phi-symmetry
+    factor 2, the 2 / A / (2 pi)^3 prefactor, the filter products and the shape-noise Pell; T1_total
+    is 6 T1 for equal radii.
+  * The classed_version's 4-D form is symmetric under theta_123 <-> theta_456 up to its ell truncation
+    (the permutation sum makes the integrand symmetric on the full domain): swapped pairs agree to the
+    1e-3 tolerance.  Its single-z triple spectrum is pinned through P_NL (test_power_oracle.py) and the
+    restatement; the 4-D cubature driver is bit-exact against the reference's (test_host_abi.py).
+"""
+import math
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from _oracle import Oracle
+from threepoint_amd import workloads as W
+
+A = 2.9088820866e-4
+
+
+def _poly_mul(p, q):
+    r = {}
+    for ka, va in p.items():
+        for kb, vb in q.items():
+            k = tuple(x + y for x, y in zip(ka, kb))
+            r[k] = r.get(k, 0) + va * vb
+    return r
+
+
+def _bivariate_moment(i, j, s_aa, s_ab, s_bb, memo):
+    """E[a^i b^j] of a zero-mean bivariate normal (Isserlis: pair one factor a, or one b)."""
+    if (i + j) % 2:
+        return Fraction(0)
+    if i == 0 and j == 0:
+        return Fraction(1)
+    key = (i, j)
+    if key in memo:
+        return memo[key]
+    if i > 0:
+        r = (i - 1) * s_aa * _bivariate_moment(i - 2, j, s_aa, s_ab, s_bb, memo) if i >= 2 else Fraction(0)
+        r += j * s_ab * _bivariate_moment(i - 1, j - 1, s_aa, s_ab, s_bb, memo) if j >= 1 else 0
+    else:
+        r = (j - 1) * s_bb * _bivariate_moment(i, j - 2, s_aa, s_ab, s_bb, memo)
+    memo[key] = r
+    return r
+
+
+def t1_closed_form_factor():
+    """E0 = E[l1^4 l2^4 l3^4]: x-components (a, b) and y-components (c, d) are independent with
+    covariance (2 [[2, 1], [1, 2]])^-1 = [[1/3, -1/6], [-1/6, 1/3]]."""
+    # monomials in (a, b, c, d): l1^2 = a^2 + c^2, l2^2 = b^2 + d^2, l3^2 = (a+b)^2 + (c+d)^2
+    l1 = {(2, 0, 0, 0): 1, (0, 0, 2, 0): 1}
+    l2 = {(0, 2, 0, 0): 1, (0, 0, 0, 2): 1}
+    l3 = {(2, 0, 0, 0): 1, (1, 1, 0, 0): 2, (0, 2, 0, 0): 1, (0, 0, 2, 0): 1, (0, 0, 1, 1): 2, (0, 0, 0, 2): 1}
+    p = {(0, 0, 0, 0): 1}
+    for f in (l1, l1, l2, l2, l3, l3):
+        p = _poly_mul(p, f)
+    s_aa, s_ab = Fraction(1, 3), Fraction(-1, 6)
+    memo = {}
+    e0 = Fraction(0)
+    for (i, j, k, l), v in p.items():
+        e0 += v * _bivariate_moment(i, j, s_aa, s_ab, s_aa, memo) * _bivariate_moment(k, l, s_aa, s_ab, s_aa, memo)
+    return e0
+
+
+def t1_closed_form(theta, area, P):
+    return P ** 3 * math.pi ** 2 * float(t1_closed_form_factor()) / (192 * theta ** 4 * area * (2 * math.pi) ** 4)
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return Oracle(W.fiducial(), W.fiducial_nz())
+
+
+def test_isserlis_moments():
+    # E[a^2] = 1/3, E[a b] = -1/6, E[a^4] = 3/9, E[a^2 b^2] = 1/9 + 2/36
+    m = {}
+    assert _bivariate_moment(2, 0, Fraction(1, 3), Fraction(-1, 6), Fraction(1, 3), m) == Fraction(1, 3)
+    assert _bivariate_moment(1, 1, Fraction(1, 3), Fraction(-1, 6), Fraction(1, 3), m) == Fraction(-1, 6)
+    assert _bivariate_moment(4, 0, Fraction(1, 3), Fraction(-1, 6), Fraction(1, 3), m) == Fraction(1, 3)
+    assert _bivariate_moment(2, 2, Fraction(1, 3), Fraction(-1, 6), Fraction(1, 3), m) == Fraction(1, 9) + Fraction(2, 36)
+
+
+@pytest.mark.parametrize("th_arcmin", [4.0, 10.0])
+def test_t1_shape_noise_closed_form(orc, th_arcmin):
+    th = th_arcmin * A
+    area = (5 * 60 * A) ** 2
+    sigma, n_gal = 0.3, 30 / A ** 2
+    P = 0.5 * sigma * sigma / n_gal
+    v, e, n = orc.t1(np.full(6, th), area, 0.0, sigma, n_gal, 4, 1e-7)
+    cf = t1_closed_form(th, area, P)
+    assert abs(v / cf - 1) < 1e-6
+    tot, _ = orc.t1_total(np.full(3, th), np.full(3, th), area, 0.0, sigma, n_gal, 4, 1e-7)
+    assert tot == 6 * v
+
+
+def test_t1_total_permutations(orc):
+    """Distinct theta_456: T1_total is the sum over the six orderings (each a separate integral)."""
+    area = (5 * 60 * A) ** 2
+    a = np.array([4.0, 6.0, 8.0]) * A
+    b = np.array([5.0, 7.0, 9.0]) * A
+    n_gal = 30 / A ** 2
+    tot, _ = orc.t1_total(a, b, area, 0.0, 0.3, n_gal, 4, 1e-5)
+    s = 0.0
+    for i, perm in enumerate([(0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)]):
+        v = orc.t1(np.concatenate([a, b[list(perm)]]), area, 0.0, 0.3, n_gal, 4, 1e-5)[0]
+        s = v if i == 0 else s + v
+    assert tot == s
+
+
+def test_cov_gauss_symmetric_in_the_two_triples(orc):
+    area = (5 * 60 * A) ** 2
+    a = np.array([4.0, 8.0, 8.0]) * A
+    b = np.array([6.0, 6.0, 12.0]) * A
+    v1 = orc.map3_cov_gauss(np.concatenate([a, b]), area, 1e-4)[0]
+    v2 = orc.map3_cov_gauss(np.concatenate([b, a]), area, 1e-4)[0]
+    assert v1 > 0 and abs(v1 / v2 - 1) < 1e-3

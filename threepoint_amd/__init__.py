@@ -15,7 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libthreepoint_amd.so")
 
 MODE_FP64, MODE_FP32, MODE_SURROGATE, MODE_ANALYTIC = 0, 1, 2, 3
-PK_LINEAR, PK_T17 = 1, 2  # flags of the power-spectrum calls
+PK_LINEAR, PK_T17, PK_SHAPENOISE_ONLY = 1, 2, 4  # flags of the power-spectrum calls
 ERROR_INDIVIDUAL, ERROR_PAIRED, ERROR_L2, ERROR_L1, ERROR_LINF = 0, 1, 2, 3, 4
 ARCMIN = 2.9088820866e-4  # classed_version/helper.cpp:139
 
@@ -96,6 +96,11 @@ _SIGS = {
                                        ctypes.c_int, _dp, _dp, _llp]),
     "tp_map2_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_double, ctypes.c_double, _dp, _dp, _llp]),
+    "tp_map3_cov_gauss_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp,
+                                               ctypes.c_double, ctypes.c_double, _dp, _dp, _llp]),
+    "tp_map3_cov_t1_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_int, ctypes.c_double, _dp, _llp]),
     "tp_gamma_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
                                       ctypes.c_double, _dp, _dp, _llp]),
     "tp_hcubature_v": (ctypes.c_int, [ctypes.c_uint, INTEGRAND_V, ctypes.c_void_p, ctypes.c_uint, _dp, _dp,
@@ -371,6 +376,29 @@ class CosmologyTables:
         _check(lib().tp_map2_batch(self.ctx.h, self.h, n, ic.ctypes.data_as(_ip), _d(t), reltol, flags, sigma, ngal,
                                    _d(val), _d(err), ne.ctypes.data_as(_llp)), "tp_map2_batch")
         return val, err, ne
+
+
+    # ---------------- Gaussian <Map^3> covariance
+    def map3_cov_gauss(self, theta6, area, icosmo=None, reltol=1e-3):
+        """The classed_version's MapMapMap_covariance_Gauss for pairs theta6[n, 6] [rad]."""
+        t = np.ascontiguousarray(theta6, dtype=np.float64).reshape(-1, 6)
+        n = t.shape[0]
+        ic = np.zeros(n, dtype=np.int32) if icosmo is None else np.ascontiguousarray(icosmo, dtype=np.int32)
+        val, err, ne = np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int64)
+        _check(lib().tp_map3_cov_gauss_batch(self.ctx.h, self.h, n, ic.ctypes.data_as(_ip), _d(t), area, reltol,
+                                             _d(val), _d(err), ne.ctypes.data_as(_llp)), "tp_map3_cov_gauss_batch")
+        return val, err, ne
+
+    def map3_cov_t1(self, theta6, area, lmin=0.0, sigma=0.3, n_gal=1.0, flags=0, icosmo=None, reltol=1e-4):
+        """T1_total of the cuda_version's covariance for an infinite survey, pairs theta6[n, 6] [rad]."""
+        t = np.ascontiguousarray(theta6, dtype=np.float64).reshape(-1, 6)
+        n = t.shape[0]
+        ic = np.zeros(n, dtype=np.int32) if icosmo is None else np.ascontiguousarray(icosmo, dtype=np.int32)
+        val, ne = np.zeros(n), np.zeros(n, dtype=np.int64)
+        _check(lib().tp_map3_cov_t1_batch(self.ctx.h, self.h, n, ic.ctypes.data_as(_ip), _d(t), area, lmin, sigma,
+                                          n_gal, flags, reltol, _d(val), ne.ctypes.data_as(_llp)),
+               "tp_map3_cov_t1_batch")
+        return val, ne
 
 
 def census_map3_point():

@@ -95,6 +95,7 @@ struct tp_cosmo {
     TpNode32 *d_base32 = nullptr;     // [ncosmo][8]: 8-node FP32 Limber tables (surrogate baseline)
     int *d_err = nullptr;             // build error flag
     void *d_pk = nullptr;             // tabulated P(k): k, p [ncosmo][nk] then idx [ncosmo][ng]
+    void *d_pnode = nullptr;          // [ncosmo][96] TpPNode: the P Limber sum's z-nodes (tp_power.hip)
     int pk_nk = 0;                    // 0: Eisenstein & Hu
     std::vector<TpCosmoConst> h_const;
 };
@@ -181,6 +182,9 @@ int tp_set_error(int code, const std::string &msg);
 
 // n-point Gauss-Legendre nodes and weights on [-1, 1], ascending (Newton iteration on P_n).
 void tp_gauss_legendre(int n, double *x, double *w);
+
+// Builds c->d_pnode from c's tables (tp_power.hip); called at the end of every table build.
+int tp_build_pnodes(tp_ctx *ctx, tp_cosmo *c);
 
 // Launchers implemented in the .hip files (all asynchronous on ctx->stream).
 int tp_launch_map3_points(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const double *th, size_t npts,

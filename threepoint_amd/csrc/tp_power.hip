@@ -27,41 +27,124 @@
 namespace {
 
 // What one integral (or one evaluation call) computes at its points.
-enum PowerKind { kPkappa = 0, kMap2 = 1, kPkAtZ = 2, kLimber = 3 };
+enum PowerKind { kPkappa = 0, kMap2 = 1, kPkAtZ = 2, kLimber = 3, kCovGauss = 4, kT1 = 5 };
 
 struct TpPowerDesc {
     double par;          // kPkappa: ell; kMap2: theta [rad]
-    double sigma, ngal;  // kMap2 shape noise (sigma = 0: none), ngal [arcmin^-2]
+    double sigma, ngal;  // kMap2 shape noise (sigma = 0: none), ngal [arcmin^-2]; kT1: Pell's shape noise
     int icosmo;
     int kind;
-    int flags;           // TP_PK_LINEAR | TP_PK_T17
+    int flags;           // TP_PK_LINEAR | TP_PK_T17 (| TP_PK_SHAPENOISE_ONLY for kT1)
     int pad;
+    double th[6];        // kCovGauss, kT1: theta_123, theta_456 [rad]
+    double lmin, lmax;   // kT1: the integrand's ell cut
 };
+
+// 96-point Gauss-Legendre half nodes/weights (own Newton solve, uploaded once per device)
+__constant__ double c_pgl_x[48];
+__constant__ double c_pgl_w[48];
+
+__device__ __forceinline__ double uhat(double eta)
+{
+    const double t = 0.5 * eta * eta;  // classed_version/apertureStatistics.cpp:10-14
+    return t * exp(-t);
+}
+
+// Pell from the z-node records N[96] (pairs 2j, 2j+1 at cx -+ dx x_j): the cuda_version's device Pell
+// (bispectrum.cu:511-529), node pairs added in the reference's order (GQ96_of_Pk, bispectrum.cpp:1034-1046).
+__device__ double pell_nodes(const TpCosmoConst &C, const TpPNode *__restrict__ N, double ell, int flags, double dx,
+                             double shot)
+{
+#pragma clang fp contract(off)
+    if ((flags & TP_PK_SHAPENOISE_ONLY) || ell == 0) return shot;
+    double q = 0;
+    for (int j = 0; j < 48; ++j)
+        q += c_pgl_w[j] * (tp_limber_pk_node(C, N[2 * j], ell, flags) + tp_limber_pk_node(C, N[2 * j + 1], ell, flags));
+    return q * dx + shot;
+}
+
+// T1 of the Gaussian <Map^3> covariance for an infinite survey at (l1, l2, phi),
+// cuda_version/apertureStatisticsCovariance.cu:1958-1985.
+__device__ double t1_point(const TpCosmoConst &C, const TpPNode *__restrict__ N, double dx, double l1, double l2,
+                           double phi, const TpPowerDesc &D)
+{
+    double l3;
+    {
+#pragma clang fp contract(off)
+        l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * cos(phi));
+    }
+    if (l1 <= D.lmin || l2 <= D.lmin || l3 <= D.lmin || l3 > D.lmax) return 0;
+    const double shot = 0.5 * D.sigma * D.sigma / D.ngal;
+    double r = pell_nodes(C, N, l1, D.flags, dx, shot) * pell_nodes(C, N, l2, D.flags, dx, shot) *
+               pell_nodes(C, N, l3, D.flags, dx, shot);
+    r *= uhat(l1 * D.th[0]) * uhat(l2 * D.th[1]) * uhat(l3 * D.th[2]);
+    r *= uhat(l1 * D.th[3]) * uhat(l2 * D.th[4]) * uhat(l3 * D.th[5]);
+    r *= l1 * l2;
+    return r;
+}
+
+// The classed_version's Gaussian covariance integrand at (l1, l2, phi, z),
+// classed_version/apertureStatistics.cpp:66-106, with the single-z triple spectrum of bispectrum.cpp:580-607.
+__device__ double covg_point(const TpCosmoConst &C, const double *T, int n_z, double l1, double l2, double phi,
+                             double z, const double *th)
+{
+    double l3;
+    {
+#pragma clang fp contract(off)
+        l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * cos(phi));
+    }
+    auto up = [&](double a, double b, double c, const double *t) { return uhat(a * t[0]) * uhat(b * t[1]) * uhat(c * t[2]); };
+    double result = up(l1, l2, l3, th);
+    double perm = up(l1, l2, l3, th + 3);
+    perm += up(l2, l3, l1, th + 3);
+    perm += up(l3, l1, l2, th + 3);
+    perm += up(l1, l3, l2, th + 3);
+    perm += up(l3, l2, l1, th + 3);
+    perm += up(l2, l1, l3, th + 3);
+    result *= perm;
+    double tri = 0;
+    if (!(z < 1e-5)) {
+        TpZAtT<double> Z;
+        tp_z_at<double>(T, n_z, C.zmax, z, &Z);
+        const double pref = tp_limber_pref<double>(C, z, Z.g);
+        TpHaloZT<double> h;
+        tp_halofit_z<double>(C, Z, z, &h);
+        tri = pow(pref, 3) * tp_halofit_k<double>(C, Z.D1, Z.rs, h, l1 / Z.fK) *
+              tp_halofit_k<double>(C, Z.D1, Z.rs, h, l2 / Z.fK) * tp_halofit_k<double>(C, Z.D1, Z.rs, h, l3 / Z.fK);
+    }
+    result *= tri;
+    result *= l1 * l2;
+    return result;
+}
+
 
 // One thread per point; pdesc[i] names the integral (desc entry) of point i.
 __global__ void k_power_points(const TpCosmoConst *__restrict__ cc, const double *__restrict__ tab, int n_z,
-                               const TpPowerDesc *__restrict__ desc, const int *__restrict__ pdesc, size_t npts,
-                               int dim, const double *__restrict__ x, double *__restrict__ f, int *flag)
+                               const TpPNode *__restrict__ pnode, double pdx, const TpPowerDesc *__restrict__ desc,
+                               const int *__restrict__ pdesc, size_t npts, int dim, const double *__restrict__ x,
+                               double *__restrict__ f, int *flag)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npts) return;
     const TpPowerDesc D = desc[pdesc ? pdesc[i] : 0];
     const TpCosmoConst &C = cc[D.icosmo];
     const double *T = tab + (size_t)D.icosmo * 6 * n_z;
+    const double *xi = x + (size_t)dim * i;
     double v;
     if (D.kind == kPkappa) {
-        v = tp_limber_pk<double>(C, T, n_z, D.par, x[i], D.flags);
+        v = tp_limber_pk<double>(C, T, n_z, D.par, xi[0], D.flags);
+    } else if (D.kind == kMap2) {
+        v = tp_map2_point<double>(C, T, n_z, xi[0], xi[1], D.par, D.flags, D.sigma, D.ngal);
+    } else if (D.kind == kLimber) {
+        v = tp_limber_pk<double>(C, T, n_z, xi[0], xi[1], D.flags);
+    } else if (D.kind == kPkAtZ) {
+        TpZAtT<double> Z;
+        tp_z_at<double>(T, n_z, C.zmax, xi[1], &Z);
+        v = tp_pk_at<double>(C, Z, xi[0], xi[1], D.flags);
+    } else if (D.kind == kT1) {
+        v = t1_point(C, pnode + (size_t)D.icosmo * TP_NNODES, pdx, xi[0], xi[1], xi[2], D);
     } else {
-        const double x0 = x[2 * i], x1 = x[2 * i + 1];
-        if (D.kind == kMap2) {
-            v = tp_map2_point<double>(C, T, n_z, x0, x1, D.par, D.flags, D.sigma, D.ngal);
-        } else if (D.kind == kLimber) {
-            v = tp_limber_pk<double>(C, T, n_z, x0, x1, D.flags);
-        } else {
-            TpZAtT<double> Z;
-            tp_z_at<double>(T, n_z, C.zmax, x1, &Z);
-            v = tp_pk_at<double>(C, Z, x0, x1, D.flags);
-        }
+        v = covg_point(C, T, n_z, xi[0], xi[1], xi[2], xi[3], D.th);
     }
     if (!isfinite(v)) atomicOr(flag, 1);
     f[i] = v;
@@ -154,14 +237,10 @@ __global__ void __launch_bounds__(256) k_map2_region_rule(size_t nreg, const TpR
     out[r] = o;
 }
 
-// 96-point Gauss-Legendre half nodes/weights (own Newton solve, uploaded once per device)
-__constant__ double c_pgl_x[48];
-__constant__ double c_pgl_w[48];
-
 // Pell: one wave per multipole.  Lane j < 48 evaluates the node pair j (z = cx -+ dx x_j); lane 0
 // then adds the 48 pair sums in node order (the reference's loop order, GQ96_of_Pk).
-__global__ void __launch_bounds__(64) k_pell(const TpCosmoConst *__restrict__ cc, const double *__restrict__ tab,
-                                             int n_z, int icosmo, int flags, size_t n, const double *__restrict__ ell,
+__global__ void __launch_bounds__(64) k_pell(const TpCosmoConst *__restrict__ cc, const TpPNode *__restrict__ pnode,
+                                             int icosmo, int flags, size_t n, const double *__restrict__ ell,
                                              double zmax, double shot, double *__restrict__ out, int *flag)
 {
     __shared__ double part[48];
@@ -169,14 +248,16 @@ __global__ void __launch_bounds__(64) k_pell(const TpCosmoConst *__restrict__ cc
     const int j = threadIdx.x;
     if (i >= n) return;
     const TpCosmoConst &C = cc[icosmo];
-    const double *T = tab + (size_t)icosmo * 6 * n_z;
+    const TpPNode *N = pnode + (size_t)icosmo * TP_NNODES;
     const double e = ell[i];
-    const double cx = zmax / 2, dx = zmax / 2;
-    if (j < 48 && e != 0)
-        part[j] = c_pgl_w[j] * (tp_limber_pk<double>(C, T, n_z, e, cx - dx * c_pgl_x[j], flags) +
-                                tp_limber_pk<double>(C, T, n_z, e, cx + dx * c_pgl_x[j], flags));
+    const double dx = zmax / 2;
+    if (j < 48 && e != 0) {
+#pragma clang fp contract(off)
+        part[j] = c_pgl_w[j] * (tp_limber_pk_node(C, N[2 * j], e, flags) + tp_limber_pk_node(C, N[2 * j + 1], e, flags));
+    }
     __syncthreads();
     if (j == 0) {
+#pragma clang fp contract(off)
         double r = shot;
         if (e != 0) {
             double q = 0;
@@ -186,6 +267,19 @@ __global__ void __launch_bounds__(64) k_pell(const TpCosmoConst *__restrict__ cc
         if (!isfinite(r)) atomicOr(flag, 1);
         out[i] = r;
     }
+}
+
+// The P Limber sum's z-nodes of every cosmology: node 2j at cx - dx x_j, node 2j+1 at cx + dx x_j.
+__global__ void k_pnodes(int ncosmo, int n_z, const TpCosmoConst *__restrict__ cc, const double *__restrict__ tab,
+                         double zmax, TpPNode *__restrict__ out)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncosmo * TP_NNODES) return;
+    const int ic = i / TP_NNODES, nd = i - ic * TP_NNODES, j = nd >> 1;
+    const double cx = zmax / 2, dx = zmax / 2;
+    const double z = (nd & 1) ? cx + dx * c_pgl_x[j] : cx - dx * c_pgl_x[j];
+    tp_pnode_make(cc[ic], tab + (size_t)ic * 6 * n_z, n_z, z, &out[i]);
 }
 
 int upload_pgl(int device)
@@ -208,7 +302,8 @@ int upload_pgl(int device)
 int check_args(tp_ctx *ctx, const tp_cosmo *c, int flags)
 {
     if (!ctx || !c) return tp_set_error(TP_EINVAL, "NULL context or tables");
-    if (flags & ~(TP_PK_LINEAR | TP_PK_T17)) return tp_set_error(TP_EINVAL, "unknown power-spectrum flags");
+    if (flags & ~(TP_PK_LINEAR | TP_PK_T17 | TP_PK_SHAPENOISE_ONLY))
+        return tp_set_error(TP_EINVAL, "unknown power-spectrum flags");
     return 0;
 }
 
@@ -235,7 +330,8 @@ int power_launch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
     TP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     const unsigned blk = 256;
     hipLaunchKernelGGL(k_power_points, dim3((unsigned)((npts + blk - 1) / blk)), dim3(blk), 0, ctx->stream,
-                       c->d_const, c->d_tab, c->n_z, (const TpPowerDesc *)ctx->d_desc.p,
+                       c->d_const, c->d_tab, c->n_z, (const TpPNode *)c->d_pnode, c->z_max / 2,
+                       (const TpPowerDesc *)ctx->d_desc.p,
                        pdesc ? (const int *)ctx->d_misc.p : nullptr, npts, dim, (const double *)ctx->d_in.p,
                        (double *)ctx->d_out.p, (int *)ctx->d_flag.p);
     TP_HIP(hipGetLastError());
@@ -246,7 +342,7 @@ int power_launch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
     TP_HIP(hipStreamSynchronize(ctx->stream));
     std::memcpy(fval, ctx->h_out.p, bf);
     float ms = 0;
-    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     ctx->stats.kernel_ms += ms;
     ctx->stats.integrand_ms += ms;
     ctx->stats.launches += 1;
@@ -255,9 +351,9 @@ int power_launch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
 }
 
 // All integrals of one call advance together, one launch per round (see the file comment).
-template <unsigned D>
+template <unsigned D, class Post>
 int power_batch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &desc, const double *lo,
-                const double *hi, double reltol, bool per_2pi, double *out, double *err, long long *neval)
+                const double *hi, double reltol, Post post, double *out, double *err, long long *neval)
 {
     auto t0 = std::chrono::steady_clock::now();
     ctx->stats = tp_batch_stats{};
@@ -311,8 +407,8 @@ int power_batch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &
     for (size_t k = 0; k < nint; ++k) {
         double v, e;
         I[k].result(&v, &e);
-        out[k] = per_2pi ? v / 2. / tpd::kPi : v;  // apertureStatistics.cu:135
-        if (err) err[k] = per_2pi ? e / 2. / tpd::kPi : e;
+        out[k] = post(v);
+        if (err) err[k] = post(e);
         if (neval) neval[k] = (long long)I[k].num_eval();
     }
     ctx->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -381,8 +477,8 @@ int map2_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
         TP_HIP(hipMemcpyAsync(ctx->h_rout.p, ctx->d_rout.p, bo, hipMemcpyDeviceToHost, ctx->stream));
         TP_HIP(hipStreamSynchronize(ctx->stream));
         float ms = 0, ms_pts = 0;
-        hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-        hipEventElapsedTime(&ms_pts, ctx->ev0, ctx->evm);
+        (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+        (void)hipEventElapsedTime(&ms_pts, ctx->ev0, ctx->evm);
         ctx->stats.kernel_ms += ms;
         ctx->stats.integrand_ms += ms_pts;
         ctx->stats.launches += 1;
@@ -469,8 +565,8 @@ extern "C" int tp_pell(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t n, con
     TP_HIP(hipMemsetAsync(ctx->d_flag.p, 0, sizeof(int), ctx->stream));
     TP_HIP(hipMemcpyAsync(ctx->d_in.p, ell, b, hipMemcpyHostToDevice, ctx->stream));
     TP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    hipLaunchKernelGGL(k_pell, dim3((unsigned)n), dim3(64), 0, ctx->stream, c->d_const, c->d_tab, c->n_z, icosmo,
-                       flags, n, (const double *)ctx->d_in.p, c->z_max, shot, (double *)ctx->d_out.p,
+    hipLaunchKernelGGL(k_pell, dim3((unsigned)n), dim3(64), 0, ctx->stream, c->d_const, (const TpPNode *)c->d_pnode,
+                       icosmo, flags, n, (const double *)ctx->d_in.p, c->z_max, shot, (double *)ctx->d_out.p,
                        (int *)ctx->d_flag.p);
     TP_HIP(hipGetLastError());
     TP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
@@ -479,7 +575,7 @@ extern "C" int tp_pell(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t n, con
     TP_HIP(hipMemcpyAsync(&flag, ctx->d_flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     TP_HIP(hipStreamSynchronize(ctx->stream));
     float ms = 0;
-    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     ctx->stats = tp_batch_stats{};
     ctx->stats.kernel_ms = ctx->stats.integrand_ms = ms;
     ctx->stats.launches = 1;
@@ -504,7 +600,7 @@ extern "C" int tp_pkappa_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const i
         if (ic < 0 || ic >= c->ncosmo) return tp_set_error(TP_EINVAL, "tp_pkappa_batch: cosmology index out of range");
         desc[i] = TpPowerDesc{ell[i], 0, 1, ic, kPkappa, flags, 0};
     }
-    return power_batch<1>(ctx, c, desc, lo.data(), hi.data(), reltol, false, out, err, neval);
+    return power_batch<1>(ctx, c, desc, lo.data(), hi.data(), reltol, [](double v) { return v; }, out, err, neval);
 }
 
 extern "C" int tp_map2_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo, const double *theta,
@@ -531,4 +627,149 @@ extern "C" int tp_map2_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int
         hi[2 * i + 1] = c->z_max;
     }
     return map2_regions(ctx, c, desc, lo.data(), hi.data(), reltol, out, err, neval);
+}
+
+int tp_build_pnodes(tp_ctx *ctx, tp_cosmo *c)
+{
+    int rc = upload_pgl(ctx->device);
+    if (rc) return rc;
+    const size_t n = (size_t)c->ncosmo * TP_NNODES;
+    TP_HIP(hipMalloc(&c->d_pnode, n * sizeof(TpPNode)));
+    hipLaunchKernelGGL(k_pnodes, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, ctx->stream, c->ncosmo, c->n_z,
+                       c->d_const, c->d_tab, c->z_max, (TpPNode *)c->d_pnode);
+    TP_HIP(hipGetLastError());
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+extern "C" int tp_map3_cov_gauss_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo,
+                                       const double *theta6, double survey_area, double reltol, double *out,
+                                       double *err, long long *neval)
+{
+    int rc = check_args(ctx, c, 0);
+    if (rc) return rc;
+    if (n && (!theta6 || !out)) return tp_set_error(TP_EINVAL, "tp_map3_cov_gauss_batch: NULL array");
+    if (!(reltol > 0) || !(survey_area > 0))
+        return tp_set_error(TP_EINVAL, "tp_map3_cov_gauss_batch: reltol and survey_area must be positive");
+    if (n > 0x7fffffffu) return tp_set_error(TP_EINVAL, "tp_map3_cov_gauss_batch: too many integrals for one call");
+    TP_HIP(hipSetDevice(ctx->device));
+    std::vector<TpPowerDesc> desc(n);
+    std::vector<double> lo(4 * n), hi(4 * n);
+    for (size_t i = 0; i < n; ++i) {
+        const int ic = icosmo ? icosmo[i] : 0;
+        if (ic < 0 || ic >= c->ncosmo)
+            return tp_set_error(TP_EINVAL, "tp_map3_cov_gauss_batch: cosmology index out of range");
+        TpPowerDesc D{};
+        D.icosmo = ic;
+        D.kind = kCovGauss;
+        const double *t = theta6 + 6 * i;
+        for (int k = 0; k < 6; ++k) {
+            if (!(t[k] > 0)) return tp_set_error(TP_EINVAL, "tp_map3_cov_gauss_batch: aperture radii must be positive");
+            D.th[k] = t[k];
+        }
+        desc[i] = D;
+        // apertureStatistics.cpp:358-380: lMax = 10 / max(min theta_123, min theta_456), phi in [0, pi]
+        const double lmax = 10. / std::max(std::min({t[0], t[1], t[2]}), std::min({t[3], t[4], t[5]}));
+        const double l[4] = {1., 1., 0., 0.}, h[4] = {lmax, lmax, 2 * tpd::kPi / 2., c->z_max};
+        for (int k = 0; k < 4; ++k) {
+            lo[4 * i + k] = l[k];
+            hi[4 * i + k] = h[k];
+        }
+    }
+    const double A = survey_area;
+    return power_batch<4>(
+        ctx, c, desc, lo.data(), hi.data(), reltol,
+        [A](double v) { return 2. * v / A / 8 / tpd::kPi / tpd::kPi / tpd::kPi; },  // apertureStatistics.cpp:395
+        out, err, neval);
+}
+
+extern "C" int tp_map3_cov_t1_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo,
+                                    const double *theta6, double area, double lmin, double sigma, double n_gal,
+                                    int flags, double reltol, double *out, long long *neval)
+{
+    int rc = check_args(ctx, c, flags);
+    if (rc) return rc;
+    if (n && (!theta6 || !out)) return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch: NULL array");
+    if (!(reltol > 0) || !(area > 0) || !(n_gal > 0) || !(lmin >= 0))
+        return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch: reltol, area and n_gal must be positive, lmin >= 0");
+    if (n > 0x7fffffffu / 6) return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch: too many pairs for one call");
+    TP_HIP(hipSetDevice(ctx->device));
+    // T1_total (apertureStatisticsCovariance.cu:272-326): each pair is a sum of T1 integrals over the
+    // distinct permutations of theta_456, with multiplicities; terms[i] lists (coefficient, integral).
+    std::vector<TpPowerDesc> desc;
+    std::vector<double> lo, hi;
+    std::vector<std::vector<std::pair<int, size_t>>> terms(n);
+    for (size_t i = 0; i < n; ++i) {
+        const int ic = icosmo ? icosmo[i] : 0;
+        if (ic < 0 || ic >= c->ncosmo) return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch: cosmology index out of range");
+        const double *t = theta6 + 6 * i;
+        for (int k = 0; k < 6; ++k)
+            if (!(t[k] > 0)) return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch: aperture radii must be positive");
+        const double th3 = t[3], th4 = t[4], th5 = t[5];
+        auto add = [&](int coef, double a, double b, double d) {
+            TpPowerDesc D{};
+            D.icosmo = ic;
+            D.kind = kT1;
+            D.flags = flags;
+            D.sigma = sigma;
+            D.ngal = n_gal;
+            D.th[0] = t[0];
+            D.th[1] = t[1];
+            D.th[2] = t[2];
+            D.th[3] = a;
+            D.th[4] = b;
+            D.th[5] = d;
+            D.lmin = lmin;
+            // T1 (apertureStatisticsCovariance.cu:1065-1093): lMax = 10 / max(min theta_123, min theta_456)
+            D.lmax = 10. / std::max(std::min({t[0], t[1], t[2]}), std::min({a, b, d}));
+            terms[i].push_back({coef, desc.size()});
+            desc.push_back(D);
+            lo.insert(lo.end(), {lmin, lmin, 0.});
+            hi.insert(hi.end(), {D.lmax, D.lmax, tpd::kPi});
+        };
+        if (th3 == th4) {
+            if (th4 == th5) {
+                add(6, th3, th3, th3);
+            } else {
+                add(2, th3, th3, th5);
+                add(2, th3, th5, th3);
+                add(2, th5, th3, th3);
+            }
+        } else if (th3 == th5) {
+            add(2, th3, th4, th3);
+            add(2, th3, th3, th4);
+            add(2, th4, th3, th3);
+        } else if (th4 == th5) {
+            add(2, th3, th4, th4);
+            add(2, th4, th3, th4);
+            add(2, th4, th4, th3);
+        } else {
+            add(1, th3, th4, th5);
+            add(1, th3, th5, th4);
+            add(1, th4, th3, th5);
+            add(1, th4, th5, th3);
+            add(1, th5, th3, th4);
+            add(1, th5, th4, th3);
+        }
+    }
+    std::vector<double> v(desc.size()), e(desc.size());
+    std::vector<long long> ne(desc.size());
+    const double A = area;
+    rc = power_batch<3>(
+        ctx, c, desc, lo.data(), hi.data(), reltol,
+        [A](double x) { return x * (2 / A / pow(2 * tpd::kPi, 3)); },  // apertureStatisticsCovariance.cu:1089
+        v.data(), e.data(), ne.data());
+    if (rc) return rc;
+    for (size_t i = 0; i < n; ++i) {
+        double r = 0;
+        long long np = 0;
+        for (size_t q = 0; q < terms[i].size(); ++q) {
+            const double tq = terms[i][q].first == 1 ? v[terms[i][q].second] : terms[i][q].first * v[terms[i][q].second];
+            r = q == 0 ? tq : r + tq;
+            np += ne[terms[i][q].second];
+        }
+        out[i] = r;
+        if (neval) neval[i] = np;
+    }
+    return 0;
 }

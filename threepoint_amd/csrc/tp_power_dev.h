@@ -37,10 +37,16 @@ __host__ __device__ __forceinline__ void tp_z_at(const double *Tab, int n_z, dou
     o->nc = Tab[5 * n_z + idx] * e + Tab[5 * n_z + idx + 1] * dd;
 }
 
-// Revised Halofit (Takahashi et al. 2012), classed_version/bispectrum.cpp:288-367.  The reference's
-// flatness check always passes: ow = 1 - om by construction.
+// Revised Halofit (Takahashi et al. 2012), classed_version/bispectrum.cpp:288-367, split into its
+// z-only coefficients (tp_halofit_z) and the k-dependent part (tp_halofit_k), so that fixed z-nodes
+// can keep the former.  The reference's flatness check always passes: ow = 1 - om by construction.
 template <class T>
-__host__ __device__ T tp_pnl(const TpCosmoConst &C, const TpZAtT<T> &Z, T k, T z)
+struct TpHaloZT {
+    T f1, f2, f3, a, b, c, gam, alpha, beta, xnu;
+};
+
+template <class T>
+__host__ __device__ void tp_halofit_z(const TpCosmoConst &C, const TpZAtT<T> &Z, T z, TpHaloZT<T> *h)
 {
     const T n_eff = Z.ne, nsqr = n_eff * n_eff, nc = Z.nc;
     // om_m_of_z, om_v_of_z (bispectrum.cpp:1022-1032): the flat form, without w
@@ -51,26 +57,38 @@ __host__ __device__ T tp_pnl(const TpCosmoConst &C, const TpZAtT<T> &Z, T k, T z
     const T f1a = pow(om_m, -0.0732), f2a = pow(om_m, -0.1423), f3a = pow(om_m, 0.0725);
     const T f1b = pow(om_m, -0.0307), f2b = pow(om_m, -0.0585), f3b = pow(om_m, 0.0743);
     const T frac = om_v / (1. - om_m);
-    const T f1 = frac * f1b + (1 - frac) * f1a;
-    const T f2 = frac * f2b + (1 - frac) * f2a;
-    const T f3 = frac * f3b + (1 - frac) * f3a;
+    h->f1 = frac * f1b + (1 - frac) * f1a;
+    h->f2 = frac * f2b + (1 - frac) * f2a;
+    h->f3 = frac * f3b + (1 - frac) * f3a;
     T a = 1.5222 + 2.8553 * n_eff + 2.3706 * nsqr + 0.9903 * n_eff * nsqr + 0.2250 * nsqr * nsqr - 0.6038 * nc +
           0.1749 * om_v * (1.0 + C.w);
-    a = pow(10.0, a);
-    const T b = pow(10.0, -0.5642 + 0.5864 * n_eff + 0.5716 * nsqr - 1.5474 * nc + 0.2279 * om_v * (1.0 + C.w));
-    const T c = pow(10.0, 0.3698 + 2.0404 * n_eff + 0.8161 * nsqr + 0.5869 * nc);
-    const T gam = 0.1971 - 0.0843 * n_eff + 0.8460 * nc;
-    const T alpha = fabs(6.0835 + 1.3373 * n_eff - 0.1959 * nsqr - 5.5274 * nc);
-    const T beta = 2.0379 - 0.7354 * n_eff + 0.3157 * nsqr + 1.2490 * n_eff * nsqr + 0.3980 * nsqr * nsqr -
-                   0.1682 * nc;
-    const T xnu = pow(10.0, 5.2105 + 3.6902 * n_eff);
+    h->a = pow(10.0, a);
+    h->b = pow(10.0, -0.5642 + 0.5864 * n_eff + 0.5716 * nsqr - 1.5474 * nc + 0.2279 * om_v * (1.0 + C.w));
+    h->c = pow(10.0, 0.3698 + 2.0404 * n_eff + 0.8161 * nsqr + 0.5869 * nc);
+    h->gam = 0.1971 - 0.0843 * n_eff + 0.8460 * nc;
+    h->alpha = fabs(6.0835 + 1.3373 * n_eff - 0.1959 * nsqr - 5.5274 * nc);
+    h->beta = 2.0379 - 0.7354 * n_eff + 0.3157 * nsqr + 1.2490 * n_eff * nsqr + 0.3980 * nsqr * nsqr - 0.1682 * nc;
+    h->xnu = pow(10.0, 5.2105 + 3.6902 * n_eff);
+}
+
+template <class T>
+__host__ __device__ T tp_halofit_k(const TpCosmoConst &C, T D1, T rs, const TpHaloZT<T> &h, T k)
+{
     const double twopi2 = 2 * tpd::kPi * tpd::kPi;
-    const T plin = tp_pk_lin<T>(C, k, C.norm) * Z.D1 * Z.D1 * k * k * k / twopi2;
-    const T y = k * Z.rs, ysqr = y * y;
-    T ph = a * pow(y, f1 * 3) / (1 + b * pow(y, f2) + pow(f3 * c * y, 3 - gam));
-    ph = ph / (1 + xnu / ysqr);
-    const T pq = plin * pow(1 + plin, beta) / (1 + plin * alpha) * exp(-y / 4.0 - ysqr / 8.0);
+    const T plin = tp_pk_lin<T>(C, k, C.norm) * D1 * D1 * k * k * k / twopi2;
+    const T y = k * rs, ysqr = y * y;
+    T ph = h.a * pow(y, h.f1 * 3) / (1 + h.b * pow(y, h.f2) + pow(h.f3 * h.c * y, 3 - h.gam));
+    ph = ph / (1 + h.xnu / ysqr);
+    const T pq = plin * pow(1 + plin, h.beta) / (1 + plin * h.alpha) * exp(-y / 4.0 - ysqr / 8.0);
     return twopi2 * (pq + ph) / (k * k * k);
+}
+
+template <class T>
+__host__ __device__ T tp_pnl(const TpCosmoConst &C, const TpZAtT<T> &Z, T k, T z)
+{
+    TpHaloZT<T> h;
+    tp_halofit_z<T>(C, Z, z, &h);
+    return tp_halofit_k<T>(C, Z.D1, Z.rs, h, k);
 }
 
 // P(k, z): Halofit, or D1^2 P_lin with TP_PK_LINEAR (linear_pk_at_z, bispectrum.cpp:369-380).
@@ -88,6 +106,16 @@ __host__ __device__ __forceinline__ T tp_t17_factor(T k)
     return pow(1 + c1 * pow(k, -1. * a1), a1) / pow(1 + c2 * pow(k, -1. * a2), a3);
 }
 
+// The Limber prefactor 9/4 (H0/c)^3 om^2 (1+z)^2 g^2 / E(z) (classed_version/bispectrum.cpp:574-577).
+template <class T>
+__host__ __device__ __forceinline__ T tp_limber_pref(const TpCosmoConst &C, T z, T g)
+{
+    const double H = 100. / 299792.;  // H0/c [h/Mpc], classed_version/bispectrum.hpp:148
+    const T zp = 1. + z;
+    const T E = sqrt(C.om * (zp * zp * zp) + C.ow);
+    return 9. / 4. * H * H * H * C.om * C.om * zp * zp * g * g / E;
+}
+
 // limber_integrand_power_spectrum, classed_version/bispectrum.cpp:543-577.
 template <class T>
 __host__ __device__ T tp_limber_pk(const TpCosmoConst &C, const double *Tab, int n_z, T ell, T z, int flags)
@@ -95,13 +123,44 @@ __host__ __device__ T tp_limber_pk(const TpCosmoConst &C, const double *Tab, int
     if (tp_dbl(z) < 1e-5) return T(0.0);
     TpZAtT<T> Z;
     tp_z_at<T>(Tab, n_z, C.zmax, z, &Z);
-    const double H = 100. / 299792.;  // H0/c [h/Mpc], classed_version/bispectrum.hpp:148
-    const T zp = 1. + z;
-    const T E = sqrt(C.om * (zp * zp * zp) + C.ow);
-    const T pref = 9. / 4. * H * H * H * C.om * C.om * zp * zp * Z.g * Z.g / E;
+    const T pref = tp_limber_pref<T>(C, z, Z.g);
     const T k = ell / Z.fK;
     const T P = tp_pk_at<T>(C, Z, k, z, flags);
     return (flags & TP_PK_T17) ? tp_t17_factor<T>(k) * pref * P : T(1.0) * pref * P;
+}
+
+// One z-node of a fixed-node Limber sum of P: every z-only factor of tp_limber_pk, evaluated once
+// (same expressions, same bits).
+struct TpPNode {
+    double fK, D1, rs, pref;
+    TpHaloZT<double> h;
+    int zero;  // z < 1e-5: the integrand is 0 there
+    int pad;
+};
+
+__host__ __device__ __forceinline__ void tp_pnode_make(const TpCosmoConst &C, const double *Tab, int n_z, double z,
+                                                       TpPNode *o)
+{
+    TpZAtT<double> Z;
+    tp_z_at<double>(Tab, n_z, C.zmax, z, &Z);
+    o->fK = Z.fK;
+    o->D1 = Z.D1;
+    o->rs = Z.rs;
+    o->pref = tp_limber_pref<double>(C, z, Z.g);
+    tp_halofit_z<double>(C, Z, z, &o->h);
+    o->zero = z < 1e-5 ? 1 : 0;
+    o->pad = 0;
+}
+
+// tp_limber_pk at a precomputed node
+__host__ __device__ __forceinline__ double tp_limber_pk_node(const TpCosmoConst &C, const TpPNode &N, double ell,
+                                                             int flags)
+{
+    if (N.zero) return 0.0;
+    const double k = ell / N.fK;
+    const double P = (flags & TP_PK_LINEAR) ? N.D1 * N.D1 * tp_pk_lin<double>(C, k, C.norm)
+                                            : tp_halofit_k<double>(C, N.D1, N.rs, N.h, k);
+    return (flags & TP_PK_T17) ? tp_t17_factor<double>(k) * N.pref * P : 1.0 * N.pref * P;
 }
 
 // <Map^2> integrand at (ell, z) for one source bin, cuda_version/apertureStatistics.cu:29-57:

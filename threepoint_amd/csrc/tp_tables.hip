@@ -691,6 +691,8 @@ static int cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *params, cons
     TPC(hipStreamSynchronize(s));
     (void)hipFree(d_params);
     if (herr) return fail(tp_set_error(TP_ECAPACITY, "tp_cosmo_build: f_K quadrature exceeded its interval capacity"));
+    int prc = tp_build_pnodes(ctx, c);
+    if (prc) return fail(prc);
     *out = c;
     return 0;
 #undef TPC
@@ -718,6 +720,7 @@ extern "C" int tp_cosmo_destroy(tp_cosmo *c)
     (void)hipFree(c->d_tab);
     (void)hipFree(c->d_nz);
     (void)hipFree(c->d_node64);
+    (void)hipFree(c->d_pnode);
     (void)hipFree(c->d_node32);
     (void)hipFree(c->d_eh64);
     (void)hipFree(c->d_eh32);
