@@ -361,8 +361,10 @@ def cov_bench(tp, W, ctx, args):
            "rounds": st.get("rounds", 0)}
     c = tp.census_power_point()
     # a T1 point: 3 Pell of 96 z-nodes, each node the k-dependent part of the Limber P (the z-only Halofit
-    # coefficients are kept per node); counted as the full Limber P point of the census, an upper bound
-    fe_point = 3 * 96 * (c["pk_flops"] + FLOPS_PER_ELEM * c["pk_elem"])
+    # coefficients are kept per node), plus 3 flops per node pair and the point's filters (l3: 7 flops +
+    # cos + sqrt; 6 uHat: 4 flops + exp each; 8 products)
+    fe_node = c["node_flops"] + FLOPS_PER_ELEM * c["node_elem"]
+    fe_point = 3 * (96 * fe_node + 48 * 3 + 2) + (7 + 2 * FLOPS_PER_ELEM) + 6 * (4 + FLOPS_PER_ELEM) + 8
     ach = fe_point * pts / (agg["integrand_ms"] * 1e-3) if agg["integrand_ms"] > 0 else 0.0
     results["batched_fp64"] = agg
     line = {

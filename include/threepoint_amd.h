@@ -161,7 +161,8 @@ int tp_census_map3_point(double *out);
  * and out[3] elementary calls of the per-point triangle shape. */
 int tp_census_gamma_term(double *out);
 /* Census of the <Map^2> integrand (tp_map2_batch): out[0] flops and out[1] elementary calls per point
- * (Halofit, no T17 factor, no shape noise); out[2], out[3] per point of the P_kappa Limber integrand. */
+ * (Halofit, no T17 factor, no shape noise); out[2], out[3] per point of the P_kappa Limber integrand;
+ * out[4], out[5] per z-node of the fixed-node sums of tp_pell and the T1 covariance integrand. */
 int tp_census_power_point(double *out);
 
 /* ---------------------------------------------------------------- Gamma_i */

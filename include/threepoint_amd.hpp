@@ -150,6 +150,32 @@ class ApertureStatistics {
     std::vector<double> Map2(const std::vector<double> &thetas, double sigma_epsilon = 0, double ngal = 1,
                              bool t17 = false);
     double reltol_map2 = 1e-5; /* apertureStatistics.cu:133 */
+
+    /* classed_version/apertureStatistics.cpp:355-396 (4-D form, relative tolerance 1e-3) */
+    double MapMapMap_covariance_Gauss(const std::vector<double> &thetas_123, const std::vector<double> &thetas_456,
+                                      double survey_area);
+    double reltol_cov = 1e-3;
+};
+
+/* The Gaussian <Map^3> covariance terms of the cuda_version (apertureStatisticsCovariance.cu), for
+ * the survey of a covarianceParameters file.  Only the infinite geometry ("infinite") is provided:
+ * T1 as the reference integrates it, T2 = 0 as the reference returns for it.  Pell carries the
+ * parameters' shape noise; t17 selects the cuda_version's T17 spectrum (its default). */
+class ApertureStatisticsCovariance {
+  public:
+    ApertureStatisticsCovariance(BispectrumCalculator *Bispectrum, const covarianceParameters &covPar,
+                                 const std::string &type = "infinite", bool t17 = true);
+    double T1_total(const std::vector<double> &thetas_123, const std::vector<double> &thetas_456);
+    double T2_total(const std::vector<double> &thetas_123, const std::vector<double> &thetas_456);
+    /* many pairs in one batched run: pairs[i] = {theta_123, theta_456} */
+    std::vector<double> T1_total(const std::vector<std::array<double, 6>> &pairs);
+    double lMin = 0;        /* the reference's executable sets lMin = 0 */
+    double reltol = 1e-4;   /* apertureStatisticsCovariance.cu:1088 */
+
+  private:
+    BispectrumCalculator *Bispectrum_;
+    covarianceParameters covPar_;
+    int flags_;
 };
 
 class GammaCalculator {

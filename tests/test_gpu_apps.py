@@ -105,6 +105,35 @@ def test_calculate_second_order_cli(tmp_path, fid_oracle):
                 assert abs(m / ref - 1) < 1e-5
 
 
+def test_calculate_covariance_cli(tmp_path, fid_oracle):
+    """T1 (infinite survey) for the 4 triples of radii {4, 8}: the 4 x 4 matrix against the oracle's T1_total."""
+    cf, nf = _inputs(tmp_path)
+    tf = tmp_path / "thetas.dat"
+    tf.write_text("4\n8\n")
+    cov = tmp_path / "cov.param"
+    cov.write_text("# example survey\nthetaMax 7.87\nshapenoise_sigma 0.37\ngalaxy_density 108000\nunit deg\n"
+                   "shapenoiseOnly 0\narea 61.9369\n")
+    folder = str(tmp_path) + "/"
+    args = [os.path.join(LIBDIR, "calculateApertureStatisticsCovariance.x"), str(cf), str(tf), str(nf), folder,
+            str(cov), "1", "1", "0", "0", "0", "0", "0", "infinite", "--reltol", "1e-3"]
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    deg = 0.017453
+    n_deg = 108000 / deg / deg / 57.3 / 57.3
+    name = "cov_infinite_term%dNumerical_sigma_0.37_n_%.2f_thetaMax_%.2f_gpu.dat"
+    t1 = np.loadtxt(tmp_path / (name % (1, n_deg, 7.87 * deg * 57.3)))
+    t2 = np.loadtxt(tmp_path / (name % (2, n_deg, 7.87 * deg * 57.3)))
+    assert t1.shape == (4, 4) and np.array_equal(t1, t1.T) and np.all(t2 == 0)
+    A = 2.9088820866e-4
+    tri = [np.array(t) * A for t in ([4, 4, 4], [4, 4, 8], [4, 8, 8], [8, 8, 8])]
+    for i in range(4):
+        for j in range(i, 4):
+            ov, _ = fid_oracle.t1_total(tri[i], tri[j], 61.9369 * deg * deg, 0.0, 0.37, 108000 / deg / deg, 2, 1e-3)
+            assert abs(t1[i, j] / ov - 1) < 1e-5
+    r = subprocess.run(args[:9] + ["1"] + args[10:], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1  # T4 requested: not provided
+
+
 def test_c3_batch_subset_matches_oracle(gpu_ctx, fid_nz):
     """Latin-hypercube cosmologies with their own GPU FP64 tables, many triples in one batched run."""
     from _oracle import Oracle

@@ -416,9 +416,10 @@ def flop_equivalents_per_point(flops_per_elem=20.0):
 
 def census_power_point():
     """Algorithmic op census of one <Map^2> and one P_kappa integrand point (see tp_census_power_point)."""
-    out = np.zeros(4)
+    out = np.zeros(6)
     _check(lib().tp_census_power_point(_d(out)), "tp_census_power_point")
-    return dict(map2_flops=out[0], map2_elem=out[1], pk_flops=out[2], pk_elem=out[3])
+    return dict(map2_flops=out[0], map2_elem=out[1], pk_flops=out[2], pk_elem=out[3], node_flops=out[4],
+                node_elem=out[5])
 
 
 def census_gamma_term():

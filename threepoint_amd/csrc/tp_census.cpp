@@ -120,8 +120,9 @@ extern "C" int tp_census_gamma_term(double *out)
 }
 
 // out: [0] flops and [1] elementary calls (pow, exp, log, sqrt) of one <Map^2> integrand point
-// (Halofit, no T17 factor, no shape noise); [2], [3] the same for one P_kappa Limber point; divisions
-// count as flops
+// (Halofit, no T17 factor, no shape noise); [2], [3] the same for one P_kappa Limber point; [4], [5]
+// for one precomputed z-node of a fixed-node Limber sum (the k-dependent part only, as Pell and T1
+// evaluate it); divisions count as flops
 extern "C" int tp_census_power_point(double *out)
 {
     TpCosmoConst C{};
@@ -139,9 +140,20 @@ extern "C" int tp_census_power_point(double *out)
     Cnt p = tp_limber_pk<Cnt>(C, tab, 4, Cnt(2000.), Cnt(0.7), 0);
     (void)p;
     OpCounts pk = g_cnt;
+    // one z-node of a fixed-node Limber sum: k = ell / f_K, the k-dependent Halofit part, pref * P
+    g_cnt = OpCounts();
+    TpHaloZT<Cnt> h;
+    h.f1 = Cnt(1.0); h.f2 = Cnt(0.9); h.f3 = Cnt(1.1); h.a = Cnt(2.0); h.b = Cnt(0.5); h.c = Cnt(0.7);
+    h.gam = Cnt(0.3); h.alpha = Cnt(1.5); h.beta = Cnt(0.4); h.xnu = Cnt(1e5);
+    Cnt k = Cnt(2000.) / Cnt(1500.);
+    Cnt node = Cnt(1.0) * Cnt(2.0) * tp_halofit_k<Cnt>(C, Cnt(0.8), Cnt(2.5), h, k);
+    (void)node;
+    OpCounts nd = g_cnt;
     out[0] = (double)map2.flops;
     out[1] = (double)map2.trans;
     out[2] = (double)pk.flops;
     out[3] = (double)pk.trans;
+    out[4] = (double)nd.flops;
+    out[5] = (double)nd.trans;
     return 0;
 }

@@ -232,3 +232,49 @@ std::complex<double> GammaCalculator::gamma0(double x1, double x2, double x3) { 
 std::complex<double> GammaCalculator::gamma1(double x1, double x2, double x3) { return all(x1, x2, x3)[1]; }
 std::complex<double> GammaCalculator::gamma2(double x1, double x2, double x3) { return all(x1, x2, x3)[2]; }
 std::complex<double> GammaCalculator::gamma3(double x1, double x2, double x3) { return all(x1, x2, x3)[3]; }
+
+double ApertureStatistics::MapMapMap_covariance_Gauss(const std::vector<double> &thetas_123,
+                                                      const std::vector<double> &thetas_456, double survey_area)
+{
+    if (thetas_123.size() != 3 || thetas_456.size() != 3)
+        throw std::runtime_error("MapMapMap_covariance_Gauss: need two triples of aperture radii");
+    double th6[6] = {thetas_123[0], thetas_123[1], thetas_123[2], thetas_456[0], thetas_456[1], thetas_456[2]};
+    double v = 0, e = 0;
+    check(tp_map3_cov_gauss_batch(Bispectrum_->ctx(), Bispectrum_->tables(), 1, nullptr, th6, survey_area, reltol_cov,
+                                  &v, &e, nullptr),
+          "tp_map3_cov_gauss_batch");
+    return v;
+}
+
+ApertureStatisticsCovariance::ApertureStatisticsCovariance(BispectrumCalculator *b, const covarianceParameters &c,
+                                                           const std::string &type, bool t17)
+    : Bispectrum_(b), covPar_(c), flags_(t17 ? TP_PK_T17 : 0)
+{
+    if (type != "infinite")
+        throw std::runtime_error("ApertureStatisticsCovariance: only the infinite survey geometry is provided");
+    if (covPar_.shapenoiseOnly) flags_ |= TP_PK_SHAPENOISE_ONLY;
+}
+
+std::vector<double> ApertureStatisticsCovariance::T1_total(const std::vector<std::array<double, 6>> &pairs)
+{
+    std::vector<double> th, out(pairs.size());
+    for (auto &p : pairs) th.insert(th.end(), p.begin(), p.end());
+    check(tp_map3_cov_t1_batch(Bispectrum_->ctx(), Bispectrum_->tables(), pairs.size(), nullptr, th.data(),
+                               covPar_.area, lMin, covPar_.shapenoise_sigma, covPar_.galaxy_density, flags_, reltol,
+                               out.data(), nullptr),
+          "tp_map3_cov_t1_batch");
+    return out;
+}
+
+double ApertureStatisticsCovariance::T1_total(const std::vector<double> &a, const std::vector<double> &b)
+{
+    if (a.size() != 3 || b.size() != 3) throw std::invalid_argument("T1_total: Wrong number of aperture radii");
+    return T1_total(std::vector<std::array<double, 6>>{{a[0], a[1], a[2], b[0], b[1], b[2]}})[0];
+}
+
+double ApertureStatisticsCovariance::T2_total(const std::vector<double> &a, const std::vector<double> &b)
+{
+    // apertureStatisticsCovariance.cu:328-331: T2 vanishes for an infinite survey
+    if (a.size() != 3 || b.size() != 3) throw std::invalid_argument("T2_total: Wrong number of aperture radii");
+    return 0;
+}
