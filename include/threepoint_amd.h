@@ -241,6 +241,17 @@ int tp_map3_cov_t1_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *ic
                          double area, double lmin, double sigma, double n_gal, int flags, double reltol,
                          double *out, long long *neval);
 
+/* ---------------------------------------------------------- input files */
+/* Replaces read_n_of_z: averaging = 0 the classed_version's reader (helper.cpp:31-96, linear
+ * resampling to z_i = i z_max / n_bins), 1 the cuda_version's (helpers.cu:9-127, overlap-weighted
+ * bin averages when the file is coarser than the internal bins); out receives n_bins values
+ * normalised to sum n dz = 1. */
+int tp_read_n_of_z(const char *fn, int n_bins, double z_max, int averaging, double *out);
+/* Replaces read_triangle_configurations (classed_version/helper.cpp:5-29): the (r, u, v) of each
+ * "rind uind vind r u v" row into ruv (3 per row, at most cap rows); *n receives the row count
+ * (TP_ECAPACITY when it exceeds cap). */
+int tp_read_triangle_configurations(const char *fn, size_t cap, double *ruv, size_t *n);
+
 /* ------------------------------------------------------------- cubature */
 /* Replaces hcubature_v (classed_version/cubature.h:90-94, hcubature.cpp:1059-1068):
  * same arguments, same callback layout x[i*ndim+j], fval[i*fdim+k], same return

@@ -40,6 +40,15 @@ std::ostream &operator<<(std::ostream &out, const cosmology &cosmo);
 
 void read_n_of_z(const std::string &fn, const int &n_bins, const double &zMax, std::vector<double> &nz);
 void read_thetas(const std::string &fn, std::vector<double> &thetas);
+/* the cuda_version's n(z) reader (cuda_version/helpers.cu:9-127): interpolation when the internal bins
+ * are coarser than the file's spacing, overlap-weighted bin averages otherwise */
+void read_n_of_z_averaging(const std::string &fn, const int &n_bins, const double &zMax, std::vector<double> &nz);
+
+/* TreeCorr triangle bins, classed_version/helper.hpp:9-14 (helper.cpp:5-29): rows "rind uind vind r u v" */
+struct treecorr_bin {
+    double r, u, v;
+};
+int read_triangle_configurations(std::string &infile, std::vector<treecorr_bin> &triangle_config);
 double convert_angle_to_rad(const double &value, const std::string &unit = "arcmin");
 double convert_rad_to_angle(const double &value, const std::string &unit = "arcmin");
 

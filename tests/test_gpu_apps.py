@@ -105,6 +105,28 @@ def test_calculate_second_order_cli(tmp_path, fid_oracle):
                 assert abs(m / ref - 1) < 1e-5
 
 
+def test_calculate_gamma_triangle_list(tmp_path, fid_tables):
+    """The classed_version's mode: TreeCorr triangle-list rows in, "i j k Re/Im Gamma0..3 r u v" out."""
+    cf, nf = _inputs(tmp_path)
+    rows = [(0, 4, 5, 5.0, 0.45, 0.1), (1, 5, 15, 6.0, 0.55, 0.5), (2, 0, 0, 5.5, 0.0, 0.2)]
+    tfile = tmp_path / "triangles.dat"
+    tfile.write_text("".join("%d\t%d\t%d\t%g\t%g\t%g\n" % r for r in rows))
+    out = tmp_path / "gammas.dat"
+    r = subprocess.run([os.path.join(LIBDIR, "calculateGamma.x"), str(cf), "-", str(out), str(nf), "--prec-h", "0.1",
+                        "--prec-k", "2", "--reltol", "1e-2", "--triangles", str(tfile)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    got = np.loadtxt(out, ndmin=2)
+    assert got.shape == (3, 14) and np.array_equal(got[:, :3], [[0, 4, 5], [1, 5, 15], [2, 0, 0]])
+    assert np.all(got[2, 3:11] == 0)  # a zero side: left at 0, as the reference does
+    tp.gamma_setup(0.1, 2.0, fid_tables.ctx)
+    x = W.treecorr_sides([5.0, 6.0], [0.45, 0.55], [0.1, 0.5]) * 2.9088820866e-4
+    g, _ = fid_tables.gamma_batch(x, 1e-2)
+    for t in range(2):
+        want = np.ravel(np.column_stack([g[t].real, g[t].imag]))
+        assert np.allclose(got[t, 3:11], want, rtol=1e-6, atol=1e-6 * np.abs(want).max())
+
+
 def test_calculate_covariance_cli(tmp_path, fid_oracle):
     """T1 (infinite survey) for the 4 triples of radii {4, 8}: the 4 x 4 matrix against the oracle's T1_total."""
     cf, nf = _inputs(tmp_path)

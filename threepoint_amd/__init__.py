@@ -101,6 +101,9 @@ _SIGS = {
     "tp_map3_cov_t1_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_int, ctypes.c_double, _dp, _llp]),
+    "tp_read_n_of_z": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]),
+    "tp_read_triangle_configurations": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, _dp,
+                                                       ctypes.POINTER(ctypes.c_size_t)]),
     "tp_gamma_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
                                       ctypes.c_double, _dp, _dp, _llp]),
     "tp_hcubature_v": (ctypes.c_int, [ctypes.c_uint, INTEGRAND_V, ctypes.c_void_p, ctypes.c_uint, _dp, _dp,

@@ -1,7 +1,7 @@
 // calculateApertureStatistics.x -- drop-in for cuda_version/calculateApertureStatistics.cu:20-133.
 //
 //   calculateApertureStatistics.x <cosmology file> <thetas file [arcmin]> <output file> <n(z) file>
-//                                 [--nz-bins N] [--reltol X] [--mode fp64|fp32|surrogate]
+//                                 [--nz-bins N] [--nz-averaging] [--reltol X] [--mode fp64|fp32|surrogate]
 //                                 [--surrogate FILE] [--single]
 //
 // Computes <Map^3>(theta1, theta2, theta3) for all theta1 <= theta2 <= theta3 of the list and writes
@@ -20,7 +20,8 @@
 int main(int argc, char *argv[])
 {
     const char *usage =
-        "calculateApertureStatistics.x <cosmology> <thetas [arcmin]> <output> <n(z)> [--nz-bins N] [--reltol X]\n"
+        "calculateApertureStatistics.x <cosmology> <thetas [arcmin]> <output> <n(z)> [--nz-bins N] [--nz-averaging]\n"
+        "                              [--reltol X]\n"
         "                              [--mode fp64|fp32|surrogate] [--surrogate FILE] [--single]\n";
     if (argc < 5) {
         std::cerr << usage;
@@ -29,11 +30,12 @@ int main(int argc, char *argv[])
     int nbins = 100;
     double reltol = 1e-4;
     int mode = TP_MODE_FP64;
-    bool single = false;
+    bool single = false, nz_avg = false;
     std::string surrogate;
     for (int i = 5; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--nz-bins" && i + 1 < argc) nbins = std::atoi(argv[++i]);
+        else if (a == "--nz-averaging") nz_avg = true;
         else if (a == "--reltol" && i + 1 < argc) reltol = std::atof(argv[++i]);
         else if (a == "--mode" && i + 1 < argc) {
             std::string m = argv[++i];
@@ -48,7 +50,8 @@ int main(int argc, char *argv[])
     try {
         cosmology cosmo(argv[1]);
         std::vector<double> nz;
-        read_n_of_z(argv[4], nbins, cosmo.zmax, nz);
+        if (nz_avg) read_n_of_z_averaging(argv[4], nbins, cosmo.zmax, nz);  // cuda_version/helpers.cu:9-127
+        else read_n_of_z(argv[4], nbins, cosmo.zmax, nz);
         std::ofstream out(argv[3]);
         if (!out.is_open()) {
             std::cerr << "Couldn't open " << argv[3] << std::endl;

@@ -1,16 +1,25 @@
 // calculateGamma.x -- drop-in for cuda_version/calculateGamma.cu:21-152.
 //
 //   calculateGamma.x <cosmology file> <gamma config file> <output file> <n(z) file> [device]
-//                    [--nz-bins N] [--prec-h H] [--prec-k K] [--reltol X]
+//                    [--nz-bins N] [--nz-averaging] [--prec-h H] [--prec-k K] [--reltol X]
+//                    [--triangles FILE]
 //
 // Gamma_0..Gamma_3 (centroid convention) on the (r, u, v) grid of the config file, r2 = r pi/180/60,
 // r3 = r2 u, r1 = v r3 + r2 (the TreeCorr binning of cuda_version/calculateGamma.cu:107-121), one
 // output line per triangle: Re/Im Gamma0 ... Re/Im Gamma3, r, u, v.  All triangles of one r bin are
 // evaluated in one batched GPU pass and written before the next r bin starts.
+//
+// --triangles FILE: the classed_version's mode (calculateGamma.cpp:64-160): the (r, u, v) bins of a
+// TreeCorr triangle list ("rind uind vind r u v" rows, read_triangle_configurations), sides converted
+// with convert_angle_to_rad, bins with a zero side left at 0; rows "rind uind vind Re/Im Gamma0..3
+// r u v" in %e format.  The config argument is not read then.  --nz-averaging reads n(z) with the
+// cuda_version's reader (overlap-weighted bin averages for coarse files).
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <fstream>
+#include <sstream>
 #include <iostream>
 #include <vector>
 
@@ -19,16 +28,20 @@
 int main(int argc, char **argv)
 {
     const char *usage = "calculateGamma.x <cosmology> <gamma config> <output> <n(z)> [device] [--nz-bins N]\n"
-                        "                 [--prec-h H] [--prec-k K] [--reltol X]\n";
+                        "                 [--nz-averaging] [--prec-h H] [--prec-k K] [--reltol X] [--triangles FILE]\n";
     if (argc < 5) {
         std::cerr << usage;
         return 1;
     }
     int nbins = 100, device = 0;
     double prec_h = 1e-3, prec_k = 5.0, reltol = 1e-3;
+    bool nz_avg = false;
+    std::string trifile;
     for (int i = 5; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--nz-bins" && i + 1 < argc) nbins = std::atoi(argv[++i]);
+        else if (a == "--nz-averaging") nz_avg = true;
+        else if (a == "--triangles" && i + 1 < argc) trifile = argv[++i];
         else if (a == "--prec-h" && i + 1 < argc) prec_h = std::atof(argv[++i]);
         else if (a == "--prec-k" && i + 1 < argc) prec_k = std::atof(argv[++i]);
         else if (a == "--reltol" && i + 1 < argc) reltol = std::atof(argv[++i]);
@@ -42,11 +55,55 @@ int main(int argc, char **argv)
         if (device != 0) threepoint_default_context(device);
         cosmology cosmo(argv[1]);
         std::vector<double> nz;
-        read_n_of_z(argv[4], nbins, cosmo.zmax, nz);
+        if (nz_avg) read_n_of_z_averaging(argv[4], nbins, cosmo.zmax, nz);
+        else read_n_of_z(argv[4], nbins, cosmo.zmax, nz);
         std::ofstream out(argv[3]);
         if (!out.is_open()) {
             std::cerr << "Couldn't open " << argv[3] << std::endl;
             return 1;
+        }
+        if (!trifile.empty()) {
+            std::vector<treecorr_bin> bins;
+            std::vector<std::array<int, 3>> idx;
+            {  // the row indices travel with the bins
+                std::ifstream fin(trifile);
+                if (!fin.is_open()) throw std::runtime_error("Could not open " + trifile);
+                std::string line;
+                while (std::getline(fin, line)) {
+                    if (line.empty() || line[0] == '#') continue;
+                    std::istringstream iss(line);
+                    std::array<int, 3> ix;
+                    treecorr_bin b;
+                    if (iss >> ix[0] >> ix[1] >> ix[2] >> b.r >> b.u >> b.v) {
+                        bins.push_back(b);
+                        idx.push_back(ix);
+                    }
+                }
+            }
+            BispectrumCalculator bk(&cosmo, nz, nbins, cosmo.zmax);
+            GammaCalculator gc(&bk, prec_h, prec_k, "centroid");
+            gc.reltol = reltol;
+            std::vector<std::array<double, 3>> x;
+            std::vector<size_t> which;
+            for (size_t t = 0; t < bins.size(); ++t) {
+                double r2 = bins[t].r, r3 = r2 * bins[t].u, r1 = bins[t].v * r3 + r2;  // calculateGamma.cpp:101-103
+                if (r1 != 0 && r2 != 0 && r3 != 0) {
+                    x.push_back({convert_angle_to_rad(r1), convert_angle_to_rad(r2), convert_angle_to_rad(r3)});
+                    which.push_back(t);
+                }
+            }
+            std::vector<std::array<std::complex<double>, 4>> g(bins.size());
+            auto gs = gc.gammas(x);
+            for (size_t q = 0; q < which.size(); ++q) g[which[q]] = gs[q];
+            char buf[512];
+            for (size_t t = 0; t < bins.size(); ++t) {
+                std::snprintf(buf, sizeof(buf), "%d %d %d %e %e %e %e %e %e %e %e %e %e %e \n", idx[t][0], idx[t][1],
+                              idx[t][2], g[t][0].real(), g[t][0].imag(), g[t][1].real(), g[t][1].imag(), g[t][2].real(),
+                              g[t][2].imag(), g[t][3].real(), g[t][3].imag(), bins[t].r, bins[t].u, bins[t].v);
+                out << buf;
+            }
+            std::cerr << bins.size() << " triangle bins done" << std::endl;
+            return 0;
         }
         configGamma config;
         read_gamma_config(argv[2], config);

@@ -140,3 +140,17 @@ def bkappa_c1_points(n=1000000, seed=0xC1B0D5):
     phi = np.pi * u[:, 2]
     l3 = np.sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * np.cos(phi))
     return np.stack([l1, l2, phi], 1), np.stack([l1, l2, l3], 1)
+
+
+def read_triangle_configurations(path):
+    """(r, u, v) of a TreeCorr triangle list, rows "rind uind vind r u v" (classed_version/helper.cpp:5-29)."""
+    a = np.loadtxt(path, ndmin=2, comments="#")
+    return a[:, 3], a[:, 4], a[:, 5]
+
+
+def treecorr_sides(r, u, v):
+    """Sides (x1, x2, x3) of TreeCorr bins in the units of r: r2 = r, r3 = r u, r1 = v r3 + r2
+    (classed_version/calculateGamma.cpp:101-103)."""
+    r = np.asarray(r, dtype=np.float64)
+    r3 = r * np.asarray(u, dtype=np.float64)
+    return np.stack([np.asarray(v, dtype=np.float64) * r3 + r, r, r3], 1)
