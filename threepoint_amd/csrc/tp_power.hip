@@ -150,6 +150,51 @@ __global__ void k_power_points(const TpCosmoConst *__restrict__ cc, const double
     f[i] = v;
 }
 
+// T1 points in a kernel of their own: one thread per (l1, l2, phi); the three Pell of a point share one
+// pass over the 96 node records (each record read once for the three multipoles), and each Pell keeps
+// the reference's node-pair order, so the values equal pell_nodes' bit for bit.
+__global__ void __launch_bounds__(256) k_t1_points(const TpCosmoConst *__restrict__ cc,
+                                                   const TpPNode *__restrict__ pnode, double dx,
+                                                   const TpPowerDesc *__restrict__ desc,
+                                                   const int *__restrict__ pdesc, size_t npts,
+                                                   const double *__restrict__ x, double *__restrict__ f, int *flag)
+{
+#pragma clang fp contract(off)
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const TpPowerDesc &D = desc[pdesc[i]];
+    const int ic = D.icosmo, flags = D.flags;
+    const TpCosmoConst &C = cc[ic];
+    const TpPNode *N = pnode + (size_t)ic * TP_NNODES;
+    const double l1 = x[3 * i], l2 = x[3 * i + 1], phi = x[3 * i + 2];
+    const double l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * cos(phi));
+    const double lmin = D.lmin, lmax = D.lmax;
+    double v = 0;
+    if (!(l1 <= lmin || l2 <= lmin || l3 <= lmin || l3 > lmax)) {
+        const double sg = D.sigma, shot = 0.5 * sg * sg / D.ngal;
+        double p1 = shot, p2 = shot, p3 = shot;
+        if (!(flags & TP_PK_SHAPENOISE_ONLY)) {
+            double q1 = 0, q2 = 0, q3 = 0;
+            for (int j = 0; j < 48; ++j) {
+                const TpPNode &A = N[2 * j], &B = N[2 * j + 1];
+                const double w = c_pgl_w[j];
+                q1 += w * (tp_limber_pk_node(C, A, l1, flags) + tp_limber_pk_node(C, B, l1, flags));
+                q2 += w * (tp_limber_pk_node(C, A, l2, flags) + tp_limber_pk_node(C, B, l2, flags));
+                q3 += w * (tp_limber_pk_node(C, A, l3, flags) + tp_limber_pk_node(C, B, l3, flags));
+            }
+            if (l1 != 0) p1 = q1 * dx + shot;
+            if (l2 != 0) p2 = q2 * dx + shot;
+            if (l3 != 0) p3 = q3 * dx + shot;
+        }
+        v = p1 * p2 * p3;
+        v *= uhat(l1 * D.th[0]) * uhat(l2 * D.th[1]) * uhat(l3 * D.th[2]);
+        v *= uhat(l1 * D.th[3]) * uhat(l2 * D.th[4]) * uhat(l3 * D.th[5]);
+        v *= l1 * l2;
+    }
+    if (!isfinite(v)) atomicOr(flag, 1);
+    f[i] = v;
+}
+
 // ---------------------------------------------------------------- 2-D regions on the device
 // The 17 Genz-Malik points of a 2-D region in the reference's order (classed_version/hcubature.cpp:
 // 456-513): centre; -+lambda2, -+lambda4 along each axis; the four (lambda4, lambda4) sign pairs;
@@ -329,6 +374,12 @@ int power_launch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
     TP_HIP(hipMemcpyAsync(ctx->d_desc.p, desc.data(), bd, hipMemcpyHostToDevice, ctx->stream));
     TP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     const unsigned blk = 256;
+    if (!desc.empty() && desc[0].kind == kT1 && pdesc) {  // a call's integrals are all of one kind
+        hipLaunchKernelGGL(k_t1_points, dim3((unsigned)((npts + blk - 1) / blk)), dim3(blk), 0, ctx->stream,
+                           c->d_const, (const TpPNode *)c->d_pnode, c->z_max / 2, (const TpPowerDesc *)ctx->d_desc.p,
+                           (const int *)ctx->d_misc.p, npts, (const double *)ctx->d_in.p, (double *)ctx->d_out.p,
+                           (int *)ctx->d_flag.p);
+    } else
     hipLaunchKernelGGL(k_power_points, dim3((unsigned)((npts + blk - 1) / blk)), dim3(blk), 0, ctx->stream,
                        c->d_const, c->d_tab, c->n_z, (const TpPNode *)c->d_pnode, c->z_max / 2,
                        (const TpPowerDesc *)ctx->d_desc.p,
