@@ -159,14 +159,14 @@ def gamma_bench(tp, W, ctx, args):
     x = tri[sel, :3]
     fe_point = tp.flop_equivalents_per_gamma_point(prec_k, FLOPS_PER_ELEM)
     results, values = {}, {}
-    for p in [q for q in args.paths.split(",") if q in ("plain_fp64", "batched_fp64")]:
+    for p in [q for q in args.paths.split(",") if q in ("plain_fp64", "batched_fp64", "batched_fp32")]:
         print("bench: gamma path %s ..." % p, file=sys.stderr, flush=True)
         steps = 1
         agg = {"points": 0, "integrand_ms": 0.0, "launches": 0, "rounds": 0, "host_ms": 0.0}
         tt = time.perf_counter()
         for _ in range(steps):
-            if p == "batched_fp64":
-                g, ne = tabs.gamma_batch(x, 1e-3)
+            if p in ("batched_fp64", "batched_fp32"):
+                g, ne = tabs.gamma_batch(x, 1e-3, mode=tp.MODE_FP32 if p == "batched_fp32" else tp.MODE_FP64)
                 st = ctx.stats()
                 for k in agg:
                     agg[k] += st.get(k, 0)
@@ -185,20 +185,28 @@ def gamma_bench(tp, W, ctx, args):
         agg["points_per_triangle"] = agg["points"] / (steps * x.shape[0])
         if agg["integrand_ms"] > 0:
             ach = fe_point * agg["points"] / (agg["integrand_ms"] * 1e-3)
-            agg["integrand_kernel"] = {"achieved_flops": ach, "peak": PEAK["fp64"], "frac": ach / PEAK["fp64"],
+            pk = PEAK["fp32"] if p == "batched_fp32" else PEAK["fp64"]  # mixed form: priced at the FP32 peak
+            agg["integrand_kernel"] = {"achieved_flops": ach, "peak": pk, "frac": ach / pk,
                                        "avg_launch_ms": agg["integrand_ms"] / max(1, agg["launches"])}
         results[p], values[p] = agg, g
     if "plain_fp64" in values and "batched_fp64" in values:
         a, b = values["batched_fp64"], values["plain_fp64"]
         results["plain_fp64"]["max_abs_diff_vs_batched"] = float(np.max(np.abs(a - b)) / np.max(np.abs(a)))
-    head = "batched_fp64" if "batched_fp64" in results else next(iter(results))
+    if "batched_fp32" in values and "batched_fp64" in values:
+        a, b = values["batched_fp64"], values["batched_fp32"]
+        # per triangle and component, |difference| relative to the component's modulus
+        rel = np.abs(b - a) / np.maximum(np.abs(a), 1e-300)
+        results["batched_fp32"]["max_rel_err_vs_fp64"] = float(rel.max())
+        results["batched_fp32"]["rel_err_p50_p99"] = [float(np.percentile(rel, q)) for q in (50, 99)]
+    ok = [p for p in results if p != "batched_fp32" or results[p].get("max_rel_err_vs_fp64", 1) <= 1e-3]
+    head = max(ok, key=lambda p: results[p]["triangles_per_s"]) if ok else next(iter(results))
     hr = results[head]
     dom = hr["integrand_kernel"]
     line = {
         "metric": "Gamma_i triangles/s on 1 GPU (4 complex natural components per triangle)",
         "value": hr["triangles_per_s"], "unit": "triangles/s", "n_gpus": 1, "steps": 1, "warmup": 0,
         "ms_per_step": hr["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
+        "dtype": "f32" if head == "batched_fp32" else "f64", "data": "synthetic",
         "config": {"workload": "Gamma_0..3 of %d of the 1000 TreeCorr-grid triangles (r 0.1-100 arcmin log x 10, "
                                "u 0-1 x 10, v -1-1 x 10, centroid), fiducial flat LCDM, Ogata h 1e-3 with 4999 "
                                "zeros of J6 and J2, hcubature (z, phi, psi) rel tol 1e-3" % x.shape[0],
@@ -408,7 +416,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--reltol", type=float, default=1e-4)
-    ap.add_argument("--paths", default="plain_fp64,batched_fp64,batched_fp32,surrogate")
+    ap.add_argument("--paths", default="plain_fp64,batched_fp64,batched_fp32,surrogate",
+                    help="paths to time (c2 takes plain_fp64, batched_fp64, batched_fp32)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "map2", "cov"])

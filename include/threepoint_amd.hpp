@@ -197,6 +197,7 @@ class GammaCalculator {
     /* all four natural components of many triangles: out[t] = {Gamma0..Gamma3} */
     std::vector<std::array<std::complex<double>, 4>> gammas(const std::vector<std::array<double, 3>> &x123);
     double reltol = 1e-3;  /* hcubature relative tolerance (classed_version/gamma.cpp:375) */
+    int mode = TP_MODE_FP64; /* TP_MODE_FP32: FP32 Ogata terms in their window, FP64 accumulation */
 
   private:
     BispectrumCalculator *Bispectrum_;

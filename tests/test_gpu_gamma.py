@@ -53,3 +53,18 @@ def test_gamma_isosceles_symmetry(fid_tables, gpu_ctx):
     assert abs(g[0].imag) <= 5e-3 * abs(g[0])
     assert abs(g[3].imag) <= 5e-3 * abs(g[3])
     assert abs(g[1].imag + g[2].imag) <= 5e-3 * (abs(g[1]) + abs(g[2]))
+
+
+def test_gamma_fp32_form_within_1e3(fid_tables, fid_oracle, gpu_ctx):
+    """FP32 Ogata terms (inside the wavenumber window) with FP64 accumulation: Gamma_0..3 of two
+    triangles within 1e-3 of the FP64 path (itself within 1e-6 of the oracle), at the production Ogata grid."""
+    tp.gamma_setup(1e-3, 5.0, gpu_ctx)
+    rows = W.gamma_c2_triangles()
+    x = rows[[555, 231], :3]
+    g64, n64 = fid_tables.gamma_batch(x, reltol=1e-2)
+    g32, n32 = fid_tables.gamma_batch(x, reltol=1e-2, mode=tp.MODE_FP32)
+    rel = np.abs(g32 - g64) / np.abs(g64)
+    print("Gamma FP32 vs FP64 max rel %.3e, points %d / %d" % (rel.max(), n32.sum(), n64.sum()))
+    assert rel.max() < 1e-3
+    with pytest.raises(tp.TpError):
+        fid_tables.gamma_batch(x, reltol=1e-2, mode=tp.MODE_SURROGATE)

@@ -104,6 +104,9 @@ _SIGS = {
     "tp_read_n_of_z": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]),
     "tp_read_triangle_configurations": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, _dp,
                                                        ctypes.POINTER(ctypes.c_size_t)]),
+    "tp_gamma_batch_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
+                                           ctypes.c_double, ctypes.c_int, _dp, _dp, _llp]),
+    "tp_gamma_fp32_window": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
     "tp_gamma_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
                                       ctypes.c_double, _dp, _dp, _llp]),
     "tp_hcubature_v": (ctypes.c_int, [ctypes.c_uint, INTEGRAND_V, ctypes.c_void_p, ctypes.c_uint, _dp, _dp,
@@ -326,13 +329,13 @@ class CosmologyTables:
                                    _d(pts), _d(out)), "tp_gamma_eval")
         return out[:, 0] + 1j * out[:, 1]
 
-    def gamma_batch(self, x123, reltol=1e-3, icosmo=0):
+    def gamma_batch(self, x123, reltol=1e-3, icosmo=0, mode=MODE_FP64):
         x = np.ascontiguousarray(x123, dtype=np.float64).reshape(-1, 3)
         n = x.shape[0]
         re, im = np.zeros(4 * n), np.zeros(4 * n)
         ne = np.zeros(4 * n, dtype=np.int64)
-        _check(lib().tp_gamma_batch(self.ctx.h, self.h, icosmo, n, _d(x), reltol, _d(re), _d(im),
-                                    ne.ctypes.data_as(_llp)), "tp_gamma_batch")
+        _check(lib().tp_gamma_batch_mode(self.ctx.h, self.h, icosmo, n, _d(x), reltol, mode, _d(re), _d(im),
+                                         ne.ctypes.data_as(_llp)), "tp_gamma_batch")
         return (re + 1j * im).reshape(n, 4), ne.reshape(n, 4)
 
 

@@ -2,7 +2,7 @@
 //
 //   calculateGamma.x <cosmology file> <gamma config file> <output file> <n(z) file> [device]
 //                    [--nz-bins N] [--nz-averaging] [--prec-h H] [--prec-k K] [--reltol X]
-//                    [--triangles FILE]
+//                    [--triangles FILE] [--fp32]
 //
 // Gamma_0..Gamma_3 (centroid convention) on the (r, u, v) grid of the config file, r2 = r pi/180/60,
 // r3 = r2 u, r1 = v r3 + r2 (the TreeCorr binning of cuda_version/calculateGamma.cu:107-121), one
@@ -13,7 +13,8 @@
 // TreeCorr triangle list ("rind uind vind r u v" rows, read_triangle_configurations), sides converted
 // with convert_angle_to_rad, bins with a zero side left at 0; rows "rind uind vind Re/Im Gamma0..3
 // r u v" in %e format.  The config argument is not read then.  --nz-averaging reads n(z) with the
-// cuda_version's reader (overlap-weighted bin averages for coarse files).
+// cuda_version's reader (overlap-weighted bin averages for coarse files).  --fp32 selects the FP32 form
+// of the integrand (FP32 Ogata terms inside their wavenumber window, FP64 accumulation).
 #include <array>
 #include <chrono>
 #include <cmath>
@@ -28,19 +29,21 @@
 int main(int argc, char **argv)
 {
     const char *usage = "calculateGamma.x <cosmology> <gamma config> <output> <n(z)> [device] [--nz-bins N]\n"
-                        "                 [--nz-averaging] [--prec-h H] [--prec-k K] [--reltol X] [--triangles FILE]\n";
+                        "                 [--nz-averaging] [--prec-h H] [--prec-k K] [--reltol X] [--triangles FILE]\n"
+                        "                 [--fp32]\n";
     if (argc < 5) {
         std::cerr << usage;
         return 1;
     }
     int nbins = 100, device = 0;
     double prec_h = 1e-3, prec_k = 5.0, reltol = 1e-3;
-    bool nz_avg = false;
+    bool nz_avg = false, fp32 = false;
     std::string trifile;
     for (int i = 5; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--nz-bins" && i + 1 < argc) nbins = std::atoi(argv[++i]);
         else if (a == "--nz-averaging") nz_avg = true;
+        else if (a == "--fp32") fp32 = true;
         else if (a == "--triangles" && i + 1 < argc) trifile = argv[++i];
         else if (a == "--prec-h" && i + 1 < argc) prec_h = std::atof(argv[++i]);
         else if (a == "--prec-k" && i + 1 < argc) prec_k = std::atof(argv[++i]);
@@ -83,6 +86,7 @@ int main(int argc, char **argv)
             BispectrumCalculator bk(&cosmo, nz, nbins, cosmo.zmax);
             GammaCalculator gc(&bk, prec_h, prec_k, "centroid");
             gc.reltol = reltol;
+            if (fp32) gc.mode = TP_MODE_FP32;
             std::vector<std::array<double, 3>> x;
             std::vector<size_t> which;
             for (size_t t = 0; t < bins.size(); ++t) {
@@ -111,6 +115,7 @@ int main(int argc, char **argv)
         BispectrumCalculator bk(&cosmo, nz, nbins, cosmo.zmax);
         GammaCalculator gc(&bk, prec_h, prec_k, "centroid");
         gc.reltol = reltol;
+        if (fp32) gc.mode = TP_MODE_FP32;
         double lrmin = std::log(config.rmin), lrmax = std::log(config.rmax);
         auto t0 = std::chrono::steady_clock::now();
         for (int i = 0; i < config.rsteps; i++) {

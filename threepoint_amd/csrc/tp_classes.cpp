@@ -219,8 +219,8 @@ std::vector<std::array<std::complex<double>, 4>> GammaCalculator::gammas(const s
 {
     std::vector<double> in, re(4 * x.size()), im(4 * x.size());
     for (auto &t : x) in.insert(in.end(), t.begin(), t.end());
-    check(tp_gamma_batch(Bispectrum_->ctx(), Bispectrum_->tables(), 0, x.size(), in.data(), reltol, re.data(),
-                         im.data(), nullptr),
+    check(tp_gamma_batch_mode(Bispectrum_->ctx(), Bispectrum_->tables(), 0, x.size(), in.data(), reltol, mode,
+                              re.data(), im.data(), nullptr),
           "tp_gamma_batch");
     std::vector<std::array<std::complex<double>, 4>> out(x.size());
     for (size_t t = 0; t < x.size(); ++t)

@@ -121,15 +121,23 @@ __device__ __forceinline__ double ogata_term(const TpNode64 &N, const TpEH64 &E,
     return N.wz * b * prodk;
 }
 
+// FP32 form: an Ogata term whose wavenumbers q = s ks l_i all lie in the window [q32lo, q32hi]
+// (tp_ctx::gamma_q32, by default [1e-5, 1e6]: on the C2 triangles a wider window gains nothing and
+// a narrower one leaves speed, at the same 1e-5 agreement with FP64) is evaluated in FP32; the rest (small z, where f_K -> 0, the first Ogata nodes, where psi_k -> 0, the far nodes)
+// and any FP32 term that leaves the float range stay FP64.  Accumulation is FP64.
+
+template <bool F32>
 __global__ void __launch_bounds__(kGammaBlock) k_gamma_points(const TpCosmoConst *__restrict__ cc,
                                                               const double *__restrict__ tab, const TpEH64 *__restrict__ eh,
+                                                              const TpEH32 *__restrict__ eh32,
                                                               int n_z, const double *__restrict__ ogata, int prec_k,
                                                               const TpGammaDesc *__restrict__ desc,
                                                               const int *__restrict__ pdesc, size_t npts,
                                                               const double *__restrict__ x, double *__restrict__ f,
-                                                              int *flag)
+                                                              int *flag, double q32lo, double q32hi)
 {
     __shared__ TpNode64 sN;
+    __shared__ TpNode32 sN32;
     __shared__ int sok;
     __shared__ double red[3][kGammaBlock / 64];
     size_t p = blockIdx.x;
@@ -137,7 +145,14 @@ __global__ void __launch_bounds__(kGammaBlock) k_gamma_points(const TpCosmoConst
     const TpGammaDesc D = desc[pdesc[p]];
     const double z = x[3 * p], phi = x[3 * p + 1], psi = x[3 * p + 2];
     const TpCosmoConst &C = cc[D.icosmo];
-    if (threadIdx.x == 0) sok = z_record(C, tab + (size_t)D.icosmo * 6 * n_z, n_z, z, &sN) ? 1 : 0;
+    if (threadIdx.x == 0) {
+        sok = z_record(C, tab + (size_t)D.icosmo * 6 * n_z, n_z, z, &sN) ? 1 : 0;
+        if (F32) {  // the record's FP32 copy (TpNodeT<double> and TpNodeT<float> list the same fields)
+            const double *src = &sN.wz;
+            float *dst = &sN32.wz;
+            for (int k = 0; k < (int)(sizeof(TpNode64) / sizeof(double)); ++k) dst[k] = (float)src[k];
+        }
+    }
     __syncthreads();
     const double x1 = D.x[0], x2 = D.x[1], x3 = D.x[2];
     // term geometry: (A_j, phase_j) for j = 0, 1, 2
@@ -166,10 +181,45 @@ __global__ void __launch_bounds__(kGammaBlock) k_gamma_points(const TpCosmoConst
             TpPointPre P;
             tp_point_pre<double>(cpsi, spsi, sqrt(u3s), &P);
             const double lninvA[3] = {-log(A[0]), -log(A[1]), -log(A[2])};
-            for (int k = 1 + threadIdx.x; k < prec_k; k += kGammaBlock) {
-                const double pk = psik[k], wk = prodk[k], lpk = lnpsik[k];
+            if (F32) {
+                const TpEH32 E32 = eh32[D.icosmo];
+                TpPointPreT<float> P32;
+                for (int d = 0; d < 3; ++d) {
+                    P32.l[d] = (float)P.l[d];
+                    P32.ll[d] = (float)P.ll[d];
+                    P32.f2t[d] = (float)P.f2t[d];
+                }
+                P32.lr1 = (float)P.lr1;
+                P32.r2 = (float)P.r2;
+                const double lmax = fmax(P.l[0], fmax(P.l[1], P.l[2]));
+                const double lmin = fmin(P.l[0], fmin(P.l[1], P.l[2]));
+                // FP32 term for s in [smin32, smax32): every wavenumber of the term in [1e-3, 1e4], the
+                // range of the FP32 <Map^3> path (below it the one-halo denominators underflow in FP32)
+                const double smax32 = q32hi / (sN.ks * lmax);
+                const double smin32 = q32lo / (sN.ks * lmin);
+                for (int k = 1 + threadIdx.x; k < prec_k; k += kGammaBlock) {
+                    const double pk = psik[k], wk = prodk[k], lpk = lnpsik[k];
 #pragma unroll
-                for (int j = 0; j < 3; ++j) s[j] += tp_ogata_term<double>(sN, E, P, pk * invA[j], lpk + lninvA[j], wk);
+                    for (int j = 0; j < 3; ++j) {
+                        const double sj = pk * invA[j];
+                        float t32 = 0.f;
+                        const bool use32 = sj < smax32 && sj >= smin32;
+                        if (use32)
+                            t32 = tp_ogata_term<float>(sN32, E32, P32, (float)sj, (float)(lpk + lninvA[j]), (float)wk);
+                        // a term outside the FP32 window, or one whose FP32 form leaves the float range
+                        if (use32 && isfinite(t32))
+                            s[j] += (double)t32;
+                        else
+                            s[j] += tp_ogata_term<double>(sN, E, P, sj, lpk + lninvA[j], wk);
+                    }
+                }
+            } else {
+                for (int k = 1 + threadIdx.x; k < prec_k; k += kGammaBlock) {
+                    const double pk = psik[k], wk = prodk[k], lpk = lnpsik[k];
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        s[j] += tp_ogata_term<double>(sN, E, P, pk * invA[j], lpk + lninvA[j], wk);
+                }
             }
         } else {
             for (int k = 1 + threadIdx.x; k < prec_k; k += kGammaBlock) {
@@ -404,7 +454,8 @@ extern "C" int tp_gamma_setup(tp_ctx *ctx, double prec_h, double prec_k_arg)
 }
 
 static int gamma_launch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpGammaDesc> &desc,
-                        const std::vector<int> &pdesc, size_t npts, const double *x, double *fval)
+                        const std::vector<int> &pdesc, size_t npts, const double *x, double *fval,
+                        int mode = TP_MODE_FP64)
 {
     if (npts == 0) return 0;
     int rc;
@@ -419,17 +470,24 @@ static int gamma_launch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpGamm
     TP_HIP(hipMemcpyAsync(ctx->d_desc.p, desc.data(), bd, hipMemcpyHostToDevice, ctx->stream));
     TP_HIP(hipMemcpyAsync(ctx->d_misc.p, pdesc.data(), bp, hipMemcpyHostToDevice, ctx->stream));
     TP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    hipLaunchKernelGGL(k_gamma_points, dim3((unsigned)npts), dim3(kGammaBlock), 0, ctx->stream, c->d_const, c->d_tab,
-                       c->d_eh64, c->n_z, ctx->d_ogata, (int)ctx->gamma_prec_k, (const TpGammaDesc *)ctx->d_desc.p,
-                       (const int *)ctx->d_misc.p, npts, (const double *)ctx->d_in.p, (double *)ctx->d_out.p,
-                       (int *)ctx->d_flag.p);
+    if (mode == TP_MODE_FP32)
+        hipLaunchKernelGGL(k_gamma_points<true>, dim3((unsigned)npts), dim3(kGammaBlock), 0, ctx->stream, c->d_const,
+                           c->d_tab, c->d_eh64, c->d_eh32, c->n_z, ctx->d_ogata, (int)ctx->gamma_prec_k,
+                           (const TpGammaDesc *)ctx->d_desc.p, (const int *)ctx->d_misc.p, npts,
+                           (const double *)ctx->d_in.p, (double *)ctx->d_out.p, (int *)ctx->d_flag.p,
+                           ctx->gamma_q32[0], ctx->gamma_q32[1]);
+    else
+        hipLaunchKernelGGL(k_gamma_points<false>, dim3((unsigned)npts), dim3(kGammaBlock), 0, ctx->stream, c->d_const,
+                           c->d_tab, c->d_eh64, c->d_eh32, c->n_z, ctx->d_ogata, (int)ctx->gamma_prec_k,
+                           (const TpGammaDesc *)ctx->d_desc.p, (const int *)ctx->d_misc.p, npts,
+                           (const double *)ctx->d_in.p, (double *)ctx->d_out.p, (int *)ctx->d_flag.p, 0.0, 0.0);
     TP_HIP(hipGetLastError());
     TP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     TP_HIP(hipMemcpyAsync(ctx->h_out.p, ctx->d_out.p, bf, hipMemcpyDeviceToHost, ctx->stream));
     TP_HIP(hipStreamSynchronize(ctx->stream));
     std::memcpy(fval, ctx->h_out.p, bf);
     float ms = 0;
-    hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
+    (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     ctx->stats.kernel_ms += ms;
     ctx->stats.integrand_ms += ms;
     ctx->stats.launches += 1;
@@ -467,7 +525,15 @@ static double one_rotation_angle_otc(double x1, double x2, double x3)
 extern "C" int tp_gamma_batch(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t ntri, const double *x123,
                               double reltol, double *out_re, double *out_im, long long *neval)
 {
+    return tp_gamma_batch_mode(ctx, c, icosmo, ntri, x123, reltol, TP_MODE_FP64, out_re, out_im, neval);
+}
+
+extern "C" int tp_gamma_batch_mode(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t ntri, const double *x123,
+                                   double reltol, int mode, double *out_re, double *out_im, long long *neval)
+{
     if (!ctx || !c || !x123 || !out_re || !out_im) return tp_set_error(TP_EINVAL, "tp_gamma_batch: NULL");
+    if (mode != TP_MODE_FP64 && mode != TP_MODE_FP32)
+        return tp_set_error(TP_EINVAL, "tp_gamma_batch: mode must be TP_MODE_FP64 or TP_MODE_FP32");
     if (!ctx->d_ogata) return tp_set_error(TP_EINVAL, "tp_gamma_batch: call tp_gamma_setup first");
     if (icosmo < 0 || icosmo >= c->ncosmo) return tp_set_error(TP_EINVAL, "tp_gamma_batch: bad icosmo");
     TP_HIP(hipSetDevice(ctx->device));
@@ -515,7 +581,7 @@ extern "C" int tp_gamma_batch(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t
                 for (unsigned q = 0; q < np; ++q) pdesc[o + q] = (int)active[a];
             }
         }
-        int rc = gamma_launch(ctx, c, desc, pdesc, npts, pts.data(), vals.data());
+        int rc = gamma_launch(ctx, c, desc, pdesc, npts, pts.data(), vals.data(), mode);
         if (rc) return rc;
         ctx->stats.rounds += 1;
         ctx->stats.points += (long long)npts;
@@ -553,5 +619,13 @@ extern "C" int tp_gamma_batch(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t
         if (neval) neval[k] = (long long)G[k].I.num_eval();
     }
     ctx->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+extern "C" int tp_gamma_fp32_window(tp_ctx *ctx, double qmin, double qmax)
+{
+    if (!ctx || !(qmin > 0) || !(qmax > qmin)) return tp_set_error(TP_EINVAL, "tp_gamma_fp32_window: need 0 < qmin < qmax");
+    ctx->gamma_q32[0] = qmin;
+    ctx->gamma_q32[1] = qmax;
     return 0;
 }

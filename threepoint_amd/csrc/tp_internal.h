@@ -161,6 +161,7 @@ struct tp_ctx {
     double gamma_prec_h = 0;
     unsigned gamma_prec_k = 0;
     double *d_ogata = nullptr;  // [6][prec_k]: psi6, prod6, psi2, prod2, ln psi6, ln psi2
+    double gamma_q32[2] = {1e-5, 1e6};  // FP32 window of the Ogata terms' wavenumbers
     TpSurrogate *sur = nullptr;
     double sur_check_frac = 0.01, sur_tol = 1e-3;
     uint64_t sur_seed = 0x5eed;
