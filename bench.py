@@ -53,8 +53,11 @@ def cpu_model():
 def run_path(tabs, ctx, thetas, path, steps, warmup, reltol, icosmo=None):
     import threepoint_amd as tp
     mode = {"plain_fp64": tp.MODE_FP64, "batched_fp64": tp.MODE_FP64, "batched_fp32": tp.MODE_FP32,
+            "batched_fp64_graph": tp.MODE_FP64, "batched_fp32_graph": tp.MODE_FP32,
             "surrogate": tp.MODE_SURROGATE}[path]
     n = thetas.shape[0]
+    # *_graph: every cubature round replayed as one HIP graph per group (tp_set_graph_rounds)
+    tp._check(tp.lib().tp_set_graph_rounds(ctx.h, 1 if path.endswith("_graph") else 0), "tp_set_graph_rounds")
 
     def one_step():
         if path == "plain_fp64":
@@ -71,7 +74,7 @@ def run_path(tabs, ctx, thetas, path, steps, warmup, reltol, icosmo=None):
     for _ in range(warmup):
         one_step()
     agg = {"points": 0, "integrand_ms": 0.0, "kernel_ms": 0.0, "launches": 0, "rounds": 0, "host_ms": 0.0,
-           "wait_ms": 0.0}
+           "wait_ms": 0.0, "graph_rebuilds": 0}
     t0 = time.perf_counter()
     vals = None
     for _ in range(steps):
@@ -516,7 +519,7 @@ def main():
         r = results[p]
         if r.get("integrand_ms", 0) > 0:
             ach = fe_point * r["points"] / (r["integrand_ms"] * 1e-3)
-            peak = PEAK["fp32"] if p == "batched_fp32" else PEAK["fp64"]
+            peak = PEAK["fp32"] if p.startswith("batched_fp32") else PEAK["fp64"]
             r["integrand_kernel"] = {"achieved_flops": ach, "peak": peak, "frac": ach / peak,
                                      "avg_launch_ms": r["integrand_ms"] / max(1, r["launches"])}
     ok = [p for p in results if results[p].get("max_rel_err_vs_fp64", 1.0) <= 1e-3]
@@ -525,7 +528,7 @@ def main():
     head = max(ok, key=lambda p: results[p]["triples_per_s"]) if ok else "batched_fp64"
     hr = results[head]
     dom = hr.get("integrand_kernel") or next(r["integrand_kernel"] for r in results.values() if "integrand_kernel" in r)
-    prec = "float" if head == "batched_fp32" else "double"
+    prec = "float" if head.startswith("batched_fp32") else "double"
     kname = "k_region_points<%s, waves per point tuned>" % prec
     pts_per_launch = hr["points"] / max(1, hr["launches"])
     roof = {"bound": "valu", "kernel": kname,
@@ -540,7 +543,7 @@ def main():
         "value": hr["triples_per_s"], "unit": "triples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": hr["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if head == "batched_fp32" else ("bf16+f64" if head == "surrogate" else "f64"),
+        "dtype": "f32" if head.startswith("batched_fp32") else ("bf16+f64" if head == "surrogate" else "f64"),
         "data": "synthetic",
         "config": {"workload": wl,
                    "path": head, "triples_per_step": int(thetas.shape[0]), "tables_ms": tables_ms},

@@ -127,6 +127,11 @@ int tp_set_region_lanes(tp_ctx *ctx, int lanes);
 /* The tuned shapes of mode (TP_MODE_FP64 or TP_MODE_FP32) for launches of < 32k, < 160k,
  * < 640k and more points (0 = not tuned yet). */
 int tp_region_lanes_table(tp_ctx *ctx, int mode, int *lanes4);
+/* on != 0: tp_map3_batch replays each exact-mode cubature round as one HIP graph per group (copy in,
+ * points kernel, rule kernel, copy out, with the timing events), instantiated once and updated each
+ * round, instead of queuing those operations one by one.  Results are identical either way.  No
+ * counterpart in the reference. */
+int tp_set_graph_rounds(tp_ctx *ctx, int on);
 
 /* Statistics of the last tp_map3_batch call on this context. */
 typedef struct {
@@ -142,6 +147,7 @@ typedef struct {
                                     of their surrogate correction exceeded tol          */
     double host_ms;              /* host time in region gathering and heap decisions   */
     double wait_ms;              /* host time blocked waiting for the device           */
+    long long graph_rebuilds;    /* round graphs rebuilt because an update was refused */
 } tp_batch_stats;
 int tp_last_batch_stats(const tp_ctx *ctx, tp_batch_stats *out);
 

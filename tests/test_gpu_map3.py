@@ -211,3 +211,18 @@ def test_given_pk_matches_oracle(gpu_ctx, fid_nz):
         assert abs(v32[j] / vo - 1) < 1e-3
     with pytest.raises(RuntimeError):
         tabs.map3_batch(th, None, 1e-4, tp.MODE_SURROGATE)
+
+
+@pytest.mark.parametrize("mode", [tp.MODE_FP64, tp.MODE_FP32])
+def test_graph_rounds_bit_identical(fid_tables, gpu_ctx, mode):
+    """Rounds replayed as one HIP graph per group (instantiated once, updated every round, with the
+    host's decisions between replays) give the eager sequence's values, errors and point counts."""
+    th = W.to_rad(W.triples(W.map3_c0_radii()))
+    v0, e0, n0 = fid_tables.map3_batch(th, None, 1e-4, mode)
+    try:
+        tp._check(tp.lib().tp_set_graph_rounds(gpu_ctx.h, 1), "tp_set_graph_rounds")
+        for rep in range(2):
+            v1, e1, n1 = fid_tables.map3_batch(th, None, 1e-4, mode)
+            assert np.array_equal(v0, v1) and np.array_equal(e0, e1) and np.array_equal(n0, n1), rep
+    finally:
+        tp.lib().tp_set_graph_rounds(gpu_ctx.h, 0)

@@ -43,7 +43,7 @@ class tp_batch_stats(ctypes.Structure):
                 ("surrogate_points", ctypes.c_longlong), ("exact_checks", ctypes.c_longlong),
                 ("fallback_batches", ctypes.c_longlong), ("max_check_relerr", ctypes.c_double),
                 ("fallback_integrals", ctypes.c_longlong), ("host_ms", ctypes.c_double),
-                ("wait_ms", ctypes.c_double)]
+                ("wait_ms", ctypes.c_double), ("graph_rebuilds", ctypes.c_longlong)]
 
 
 class tp_train_config(ctypes.Structure):
@@ -107,6 +107,7 @@ _SIGS = {
     "tp_gamma_batch_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
                                            ctypes.c_double, ctypes.c_int, _dp, _dp, _llp]),
     "tp_gamma_fp32_window": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    "tp_set_graph_rounds": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "tp_gamma_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
                                       ctypes.c_double, _dp, _dp, _llp]),
     "tp_hcubature_v": (ctypes.c_int, [ctypes.c_uint, INTEGRAND_V, ctypes.c_void_p, ctypes.c_uint, _dp, _dp,

@@ -145,6 +145,11 @@ struct TpLane {
     DevBuf d_regions, d_rout, d_fvals, d_flag;
     HostPinned h_regions, h_rout, h_flag;
     hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr, done = nullptr;
+    // the round as one graph (tp_graph_round): copy in, ev0, points kernel, evm, rule kernel, ev1,
+    // copy out, done; instantiated once, its node parameters updated every round
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipGraphNode_t n_in = nullptr, n_pts = nullptr, n_rule = nullptr, n_out = nullptr;
     int create();
     void release();
 };
@@ -169,6 +174,7 @@ struct tp_ctx {
     int region_lpp[2][4] = {{1, 1, 1, 1}, {1, 1, 1, 1}};
     bool region_tuned[2] = {false, false};
     int region_lpp_force = 0;   // 0 = tuned; 1, 2, 4, 8 = fixed
+    bool graph_rounds = false;  // batched rounds replayed as one graph per lane (tp_set_graph_rounds)
     TpLane lane[TP_NLANES];
 };
 
@@ -195,5 +201,8 @@ int tp_launch_bkappa_points(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t n
 int tp_launch_regions(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegion *d_reg,
                       const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int mode,
                       int *d_flag, hipEvent_t evm);
+// One exact-mode round of lane L (nreg regions already in L.h_regions) as a graph replay: the same
+// work and events as the copy + tp_launch_regions + copy sequence, queued on ctx->stream.
+int tp_graph_round(tp_ctx *ctx, const tp_cosmo *c, TpLane &L, size_t nreg, int mode);
 int tp_launch_gamma_points(tp_ctx *ctx, const tp_cosmo *c, int icosmo, int which, double x1, double x2,
                            double x3, size_t npts, const double *d_x, double *d_f, int *d_flag);

@@ -208,6 +208,102 @@ int tp_launch_regions(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegio
     return 0;
 }
 
+// ------------------------------------------------------------------ graph rounds
+// The round's kernels as (function, grid, block) for graph kernel nodes, the same choice as
+// launch_region_points makes.
+template <class T>
+static void region_points_shape(int lpp, size_t nreg, void **func, dim3 *grid, dim3 *block)
+{
+    const size_t groups = (nreg * TP_NPTS_GM3 + 63) / 64;
+    switch (lpp) {
+    case 2:
+        *func = (void *)k_region_points<T, 2>;
+        *grid = dim3((unsigned)((groups + 1) / 2));
+        *block = dim3(256);
+        break;
+    case 4:
+        *func = (void *)k_region_points<T, 4>;
+        *grid = dim3((unsigned)groups);
+        *block = dim3(256);
+        break;
+    case 8:
+        *func = (void *)k_region_points<T, 8>;
+        *grid = dim3((unsigned)groups);
+        *block = dim3(512);
+        break;
+    default:
+        *func = (void *)k_region_points<T, 1>;
+        *grid = dim3((unsigned)((groups + 3) / 4));
+        *block = dim3(256);
+    }
+}
+
+int tp_graph_round(tp_ctx *ctx, const tp_cosmo *c, TpLane &L, size_t nreg, int mode)
+{
+    if (nreg == 0) return 0;
+    const int lpp = tp_region_lanes(ctx, c, mode, nreg * TP_NPTS_GM3);
+    if (lpp < 0) return TP_EHIP;
+    const bool f32 = mode == TP_MODE_FP32;
+    // kernel arguments (copied into the node at set-up / update)
+    const void *nodes = f32 ? (const void *)c->d_node32 : (const void *)c->d_node64;
+    const void *eh = f32 ? (const void *)c->d_eh32 : (const void *)c->d_eh64;
+    const TpRegion *reg = (const TpRegion *)L.d_regions.p;
+    const TpIntegralDesc *desc = (const TpIntegralDesc *)ctx->d_desc.p;
+    double *fv = (double *)L.d_fvals.p;
+    TpRegionOut *ro = (TpRegionOut *)L.d_rout.p;
+    int *flag = (int *)L.d_flag.p;
+    int md = mode;
+    size_t n = nreg;
+    void *pargs[] = {(void *)&nodes, (void *)&eh, (void *)&n, (void *)&reg, (void *)&desc, (void *)&fv, (void *)&md,
+                     (void *)&flag};
+    const int *cflag = flag;
+    void *rargs[] = {(void *)&n, (void *)&reg, (void *)&fv, (void *)&ro, (void *)&cflag};
+    hipKernelNodeParams kp{}, kr{};
+    if (f32) region_points_shape<float>(lpp, nreg, &kp.func, &kp.gridDim, &kp.blockDim);
+    else region_points_shape<double>(lpp, nreg, &kp.func, &kp.gridDim, &kp.blockDim);
+    kp.kernelParams = pargs;
+    kr.func = (void *)k_region_rule;
+    kr.gridDim = dim3(grid_for(nreg, 256));
+    kr.blockDim = dim3(256);
+    kr.kernelParams = rargs;
+    const size_t bin = sizeof(TpRegion) * nreg, bout = sizeof(TpRegionOut) * nreg;
+    bool fresh = false;
+    if (L.exec) {  // update the instantiated chain; an update the runtime refuses rebuilds it
+        hipError_t u = hipGraphExecMemcpyNodeSetParams1D(L.exec, L.n_in, L.d_regions.p, L.h_regions.p, bin,
+                                                         hipMemcpyHostToDevice);
+        if (u == hipSuccess) u = hipGraphExecKernelNodeSetParams(L.exec, L.n_pts, &kp);
+        if (u == hipSuccess) u = hipGraphExecKernelNodeSetParams(L.exec, L.n_rule, &kr);
+        if (u == hipSuccess)
+            u = hipGraphExecMemcpyNodeSetParams1D(L.exec, L.n_out, L.h_rout.p, L.d_rout.p, bout, hipMemcpyDeviceToHost);
+        if (u != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipGraphExecDestroy(L.exec);
+            (void)hipGraphDestroy(L.graph);
+            L.exec = nullptr;
+            L.graph = nullptr;
+            ctx->stats.graph_rebuilds += 1;
+        }
+    }
+    if (!L.exec) {  // build the chain
+        TP_HIP(hipGraphCreate(&L.graph, 0));
+        hipGraphNode_t e0, em, e1, ed;
+        TP_HIP(hipGraphAddMemcpyNode1D(&L.n_in, L.graph, nullptr, 0, L.d_regions.p, L.h_regions.p, bin,
+                                       hipMemcpyHostToDevice));
+        TP_HIP(hipGraphAddEventRecordNode(&e0, L.graph, &L.n_in, 1, L.ev0));
+        TP_HIP(hipGraphAddKernelNode(&L.n_pts, L.graph, &e0, 1, &kp));
+        TP_HIP(hipGraphAddEventRecordNode(&em, L.graph, &L.n_pts, 1, L.evm));
+        TP_HIP(hipGraphAddKernelNode(&L.n_rule, L.graph, &em, 1, &kr));
+        TP_HIP(hipGraphAddEventRecordNode(&e1, L.graph, &L.n_rule, 1, L.ev1));
+        TP_HIP(hipGraphAddMemcpyNode1D(&L.n_out, L.graph, &e1, 1, L.h_rout.p, L.d_rout.p, bout, hipMemcpyDeviceToHost));
+        TP_HIP(hipGraphAddEventRecordNode(&ed, L.graph, &L.n_out, 1, L.done));
+        TP_HIP(hipGraphInstantiate(&L.exec, L.graph, nullptr, nullptr, 0));
+        fresh = true;
+    }
+    (void)fresh;
+    TP_HIP(hipGraphLaunch(L.exec, ctx->stream));
+    return 0;
+}
+
 // ------------------------------------------------------------------ launch-shape autotuner
 // The waves-per-point split of k_region_points is chosen per launch-size bucket by timing every
 // shape on a synthetic batch of that size, once per context and precision, on the first batched

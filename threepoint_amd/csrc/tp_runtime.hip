@@ -104,6 +104,10 @@ int TpLane::create()
 }
 void TpLane::release()
 {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
     for (DevBuf *b : {&d_regions, &d_rout, &d_fvals, &d_flag}) b->release();
     for (HostPinned *b : {&h_regions, &h_rout, &h_flag}) b->release();
     for (hipEvent_t *e : {&ev0, &evm, &ev1, &done})
@@ -458,6 +462,15 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
             }
         }
         ctx->stats.host_ms += ms_since(th0);
+        if (ctx->graph_rounds && mode != TP_MODE_SURROGATE) {
+            if ((e = tp_graph_round(ctx, c, L, nreg, mode))) return e;
+            Gr.inflight = true;
+            Gr.rounds += 1;
+            ctx->stats.launches += 1;
+            ctx->stats.regions += (long long)nreg;
+            ctx->stats.points += (long long)nreg * TP_NPTS_GM3;
+            return 0;
+        }
         if (mode == TP_MODE_SURROGATE) TP_HIP(hipMemsetAsync(L.d_flag.p, 0, sizeof(int), ctx->stream));
         TP_HIP(hipMemcpyAsync(L.d_regions.p, hr, sizeof(TpRegion) * nreg, hipMemcpyHostToDevice, ctx->stream));
         TP_HIP(hipEventRecord(L.ev0, ctx->stream));
@@ -600,5 +613,12 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         ctx->stats = keep;
     }
     ctx->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+extern "C" int tp_set_graph_rounds(tp_ctx *ctx, int on)
+{
+    if (!ctx) return tp_set_error(TP_EINVAL, "tp_set_graph_rounds: NULL context");
+    ctx->graph_rounds = on != 0;
     return 0;
 }
