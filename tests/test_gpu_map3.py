@@ -226,3 +226,29 @@ def test_graph_rounds_bit_identical(fid_tables, gpu_ctx, mode):
             assert np.array_equal(v0, v1) and np.array_equal(e0, e1) and np.array_equal(n0, n1), rep
     finally:
         tp.lib().tp_set_graph_rounds(gpu_ctx.h, 0)
+
+
+def test_bkappa_1e6_dump_bit_stable_and_checked(fid_oracle, tmp_path):
+    """The 1e6-point B_kappa dump of the C1 workload: two runs write the same bytes; a seeded 4000-point
+    sample of it matches the CPU oracle (FP64 bound of test_bkappa_fp64_matches_oracle) and the FP32
+    dump is within 1e-3 where l_max / l_min <= 100."""
+    from threepoint_amd import dump_bkappa
+    a = dump_bkappa.dump(1000000, tp.MODE_FP64)
+    b = dump_bkappa.dump(1000000, tp.MODE_FP64)
+    assert a.shape == (1000000, 4) and a.tobytes() == b.tobytes()
+    p = tmp_path / "bk.f64"
+    a.tofile(p)
+    assert np.array_equal(np.fromfile(p, dtype="<f8").reshape(-1, 4), a)
+    idx = np.random.default_rng(3).choice(1000000, 4000, replace=False)
+    x = a[idx]
+    l3 = np.sqrt(x[:, 0] ** 2 + x[:, 1] ** 2 + 2 * x[:, 0] * x[:, 1] * np.cos(x[:, 2]))
+    ell = np.column_stack([x[:, 0], x[:, 1], l3])
+    o = fid_oracle.bkappa(ell)
+    sq = ell.max(1) / ell.min(1)
+    assert np.all(rel(x[:, 3], o) <= 1e-12 + 2e-14 * sq)
+    f = dump_bkappa.dump(1000000, tp.MODE_FP32)
+    r32 = rel(f[:, 3], a[:, 3])
+    l3a = np.sqrt(a[:, 0] ** 2 + a[:, 1] ** 2 + 2 * a[:, 0] * a[:, 1] * np.cos(a[:, 2]))
+    sqa = np.maximum(np.maximum(a[:, 0], a[:, 1]), l3a) / np.minimum(np.minimum(a[:, 0], a[:, 1]), l3a)
+    print("1e6 dump: FP32 max rel %.3e (ratio <= 100: %.3e)" % (r32.max(), r32[sqa <= 100].max()))
+    assert r32[sqa <= 100].max() < 1e-3
