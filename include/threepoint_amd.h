@@ -132,6 +132,14 @@ int tp_region_lanes_table(tp_ctx *ctx, int mode, int *lanes4);
  * round, instead of queuing those operations one by one.  Results are identical either way.  No
  * counterpart in the reference. */
 int tp_set_graph_rounds(tp_ctx *ctx, int on);
+/* Number of integral groups tp_map3_batch keeps in flight (1-4, default 2): while the device evaluates
+ * one group's regions the host runs the others' heap decisions.  Results do not depend on it.  No
+ * counterpart in the reference. */
+int tp_set_batch_groups(tp_ctx *ctx, int groups);
+/* Exact-mode rounds of at most max_regions regions read their regions from, and write their results
+ * to, pinned host memory directly instead of copying them (0 = always copy).  Results do not depend
+ * on it.  No counterpart in the reference. */
+int tp_set_zero_copy_regions(tp_ctx *ctx, long long max_regions);
 
 /* Statistics of the last tp_map3_batch call on this context. */
 typedef struct {

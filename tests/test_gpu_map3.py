@@ -252,3 +252,18 @@ def test_bkappa_1e6_dump_bit_stable_and_checked(fid_oracle, tmp_path):
     sqa = np.maximum(np.maximum(a[:, 0], a[:, 1]), l3a) / np.minimum(np.minimum(a[:, 0], a[:, 1]), l3a)
     print("1e6 dump: FP32 max rel %.3e (ratio <= 100: %.3e)" % (r32.max(), r32[sqa <= 100].max()))
     assert r32[sqa <= 100].max() < 1e-3
+
+
+@pytest.mark.parametrize("mode", [tp.MODE_FP64, tp.MODE_FP32])
+def test_zero_copy_rounds_bit_identical(fid_tables, gpu_ctx, mode):
+    """Rounds that read regions from and write results to pinned host memory directly give the copied
+    rounds' values, errors and point counts (C0, every round zero-copy, then none)."""
+    th = W.to_rad(W.triples(W.map3_c0_radii()))
+    try:
+        tp._check(tp.lib().tp_set_zero_copy_regions(gpu_ctx.h, 0), "zc")
+        v0, e0, n0 = fid_tables.map3_batch(th, None, 1e-4, mode)
+        tp._check(tp.lib().tp_set_zero_copy_regions(gpu_ctx.h, 1 << 40), "zc")
+        v1, e1, n1 = fid_tables.map3_batch(th, None, 1e-4, mode)
+        assert np.array_equal(v0, v1) and np.array_equal(e0, e1) and np.array_equal(n0, n1)
+    finally:
+        tp.lib().tp_set_zero_copy_regions(gpu_ctx.h, 0)

@@ -108,6 +108,8 @@ _SIGS = {
                                            ctypes.c_double, ctypes.c_int, _dp, _dp, _llp]),
     "tp_gamma_fp32_window": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
     "tp_set_graph_rounds": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "tp_set_batch_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "tp_set_zero_copy_regions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
     "tp_gamma_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
                                       ctypes.c_double, _dp, _dp, _llp]),
     "tp_hcubature_v": (ctypes.c_int, [ctypes.c_uint, INTEGRAND_V, ctypes.c_void_p, ctypes.c_uint, _dp, _dp,

@@ -131,6 +131,7 @@ struct DevBuf {
 };
 struct HostPinned {
     void *p = nullptr;
+    void *dp = nullptr;  // the same memory as seen by kernels (hipHostGetDevicePointer)
     size_t cap = 0;
     int ensure(size_t bytes);
     void release();
@@ -153,7 +154,7 @@ struct TpLane {
     int create();
     void release();
 };
-constexpr int TP_NLANES = 2;
+constexpr int TP_NLANES = 4;  // groups of a batched call (at most; tp_ctx::nlanes are used)
 
 struct tp_ctx {
     int device = 0;
@@ -175,6 +176,8 @@ struct tp_ctx {
     bool region_tuned[2] = {false, false};
     int region_lpp_force = 0;   // 0 = tuned; 1, 2, 4, 8 = fixed
     bool graph_rounds = false;  // batched rounds replayed as one graph per lane (tp_set_graph_rounds)
+    int nlanes = 2;             // groups in flight in tp_map3_batch (tp_set_batch_groups)
+    long long zero_copy_regions = 0;  // rounds up to this many regions skip the copies (pinned buffers)
     TpLane lane[TP_NLANES];
 };
 

@@ -48,13 +48,19 @@ int HostPinned::ensure(size_t bytes)
     cap = 0;
     hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
     if (e != hipSuccess) return tp_set_error(TP_EHIP, std::string("hipHostMalloc: ") + hipGetErrorString(e));
+    e = hipHostGetDevicePointer(&dp, p, 0);
+    if (e != hipSuccess) {
+        (void)hipHostFree(p);
+        p = dp = nullptr;
+        return tp_set_error(TP_EHIP, std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
+    }
     cap = want;
     return 0;
 }
 void HostPinned::release()
 {
     if (p) hipHostFree(p);
-    p = nullptr;
+    p = dp = nullptr;
     cap = 0;
 }
 
@@ -399,7 +405,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     // evaluates one group's regions, the host absorbs the other group's results and runs
     // its heap decisions.  Each integral's decision sequence is unchanged.  A surrogate
     // round synchronises inside (its fallback test reads the checks back), so it uses one group.
-    const int nl = (mode == TP_MODE_SURROGATE) ? 1 : TP_NLANES;
+    const int nl = (mode == TP_MODE_SURROGATE) ? 1 : std::max(1, std::min(ctx->nlanes, TP_NLANES));
     struct Group {
         std::vector<size_t> active, offs;
         size_t nreg = 0;
@@ -472,7 +478,11 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
             return 0;
         }
         if (mode == TP_MODE_SURROGATE) TP_HIP(hipMemsetAsync(L.d_flag.p, 0, sizeof(int), ctx->stream));
-        TP_HIP(hipMemcpyAsync(L.d_regions.p, hr, sizeof(TpRegion) * nreg, hipMemcpyHostToDevice, ctx->stream));
+        // a small exact round reads its regions from, and writes its results to, the pinned host
+        // buffers directly: no copy in or out (per-round latency is what the tail rounds cost)
+        const bool zc = mode != TP_MODE_SURROGATE && nreg <= (size_t)ctx->zero_copy_regions;
+        if (!zc)
+            TP_HIP(hipMemcpyAsync(L.d_regions.p, hr, sizeof(TpRegion) * nreg, hipMemcpyHostToDevice, ctx->stream));
         TP_HIP(hipEventRecord(L.ev0, ctx->stream));
         if (mode == TP_MODE_SURROGATE) {
             int fb = 0;
@@ -481,12 +491,16 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
             if (e) return e;
             ctx->stats.fallback_batches += (fb == 1);
         } else {
-            e = tp_launch_regions(ctx, c, nreg, (const TpRegion *)L.d_regions.p, (const TpIntegralDesc *)ctx->d_desc.p,
-                                  (double *)L.d_fvals.p, (TpRegionOut *)L.d_rout.p, mode, (int *)L.d_flag.p, L.evm);
+            const TpRegion *rin = zc ? (const TpRegion *)L.h_regions.dp : (const TpRegion *)L.d_regions.p;
+            TpRegionOut *rout = zc ? (TpRegionOut *)L.h_rout.dp : (TpRegionOut *)L.d_rout.p;
+            e = tp_launch_regions(ctx, c, nreg, rin, (const TpIntegralDesc *)ctx->d_desc.p, (double *)L.d_fvals.p,
+                                  rout, mode, (int *)L.d_flag.p, L.evm);
             if (e) return e;
         }
         TP_HIP(hipEventRecord(L.ev1, ctx->stream));
-        TP_HIP(hipMemcpyAsync(L.h_rout.p, L.d_rout.p, sizeof(TpRegionOut) * nreg, hipMemcpyDeviceToHost, ctx->stream));
+        if (!zc)
+            TP_HIP(hipMemcpyAsync(L.h_rout.p, L.d_rout.p, sizeof(TpRegionOut) * nreg, hipMemcpyDeviceToHost,
+                                  ctx->stream));
         if (mode == TP_MODE_SURROGATE)
             TP_HIP(hipMemcpyAsync(L.h_flag.p, L.d_flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         TP_HIP(hipEventRecord(L.done, ctx->stream));
@@ -620,5 +634,20 @@ extern "C" int tp_set_graph_rounds(tp_ctx *ctx, int on)
 {
     if (!ctx) return tp_set_error(TP_EINVAL, "tp_set_graph_rounds: NULL context");
     ctx->graph_rounds = on != 0;
+    return 0;
+}
+
+extern "C" int tp_set_batch_groups(tp_ctx *ctx, int groups)
+{
+    if (!ctx || groups < 1 || groups > TP_NLANES)
+        return tp_set_error(TP_EINVAL, "tp_set_batch_groups: groups must be 1.." + std::to_string(TP_NLANES));
+    ctx->nlanes = groups;
+    return 0;
+}
+
+extern "C" int tp_set_zero_copy_regions(tp_ctx *ctx, long long max_regions)
+{
+    if (!ctx || max_regions < 0) return tp_set_error(TP_EINVAL, "tp_set_zero_copy_regions: bad arguments");
+    ctx->zero_copy_regions = max_regions;
     return 0;
 }
