@@ -388,7 +388,7 @@ def cov_bench(tp, W, ctx, args):
                                "rel tol 1e-4", "path": "batched_fp64", "tables_ms": tables_ms,
                    "t1_integrals": int(sum({3: 6, 2: 3, 1: 1}[len(set(p[3:]))] for p in pairs))},
         "paths": results,
-        "roofline": {"bound": "valu", "kernel": "k_power_points (T1)", "achieved": ach / 1e12,
+        "roofline": {"bound": "valu", "kernel": "k_t1_points", "achieved": ach / 1e12,
                      "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK["fp64"], "traffic": None,
                      "work_per_point_flop_eq": fe_point, "flops_per_elementary_call": FLOPS_PER_ELEM,
                      "node_evaluations_per_point": 288},

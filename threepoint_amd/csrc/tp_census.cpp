@@ -32,6 +32,7 @@ inline Cnt pow(Cnt a, Cnt b) { ++g_cnt.trans; return Cnt(std::pow(a.v, b.v)); }
 inline Cnt exp(Cnt a) { ++g_cnt.trans; return Cnt(std::exp(a.v)); }
 inline Cnt log(Cnt a) { ++g_cnt.trans; return Cnt(std::log(a.v)); }
 inline Cnt sqrt(Cnt a) { ++g_cnt.trans; return Cnt(std::sqrt(a.v)); }
+inline Cnt log1p(Cnt a) { ++g_cnt.trans; return Cnt(std::log1p(a.v)); }
 inline Cnt fabs(Cnt a) { return Cnt(std::fabs(a.v)); }
 inline double tp_dbl(Cnt a) { return a.v; }
 }  // namespace
@@ -144,9 +145,9 @@ extern "C" int tp_census_power_point(double *out)
     g_cnt = OpCounts();
     TpHaloZT<Cnt> h;
     h.f1 = Cnt(1.0); h.f2 = Cnt(0.9); h.f3 = Cnt(1.1); h.a = Cnt(2.0); h.b = Cnt(0.5); h.c = Cnt(0.7);
-    h.gam = Cnt(0.3); h.alpha = Cnt(1.5); h.beta = Cnt(0.4); h.xnu = Cnt(1e5);
-    Cnt k = Cnt(2000.) / Cnt(1500.);
-    Cnt node = Cnt(1.0) * Cnt(2.0) * tp_halofit_k<Cnt>(C, Cnt(0.8), Cnt(2.5), h, k);
+    h.gam = Cnt(0.3); h.alpha = Cnt(1.5); h.beta = Cnt(0.4); h.xnu = Cnt(1e5); h.lf3c = Cnt(0.5);
+    Cnt k = Cnt(2000.) / Cnt(1500.), lk = Cnt(7.6) - Cnt(7.3);
+    Cnt node = Cnt(1.0) * Cnt(2.0) * tp_halofit_k<Cnt>(C, Cnt(0.8), Cnt(2.5), Cnt(0.9), h, k, lk);
     (void)node;
     OpCounts nd = g_cnt;
     out[0] = (double)map2.flops;

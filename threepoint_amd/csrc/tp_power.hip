@@ -57,9 +57,11 @@ __device__ double pell_nodes(const TpCosmoConst &C, const TpPNode *__restrict__ 
 {
 #pragma clang fp contract(off)
     if ((flags & TP_PK_SHAPENOISE_ONLY) || ell == 0) return shot;
+    const double lell = log(ell);
     double q = 0;
     for (int j = 0; j < 48; ++j)
-        q += c_pgl_w[j] * (tp_limber_pk_node(C, N[2 * j], ell, flags) + tp_limber_pk_node(C, N[2 * j + 1], ell, flags));
+        q += c_pgl_w[j] * (tp_limber_pk_node(C, N[2 * j], ell, lell, flags) +
+                           tp_limber_pk_node(C, N[2 * j + 1], ell, lell, flags));
     return q * dx + shot;
 }
 
@@ -109,8 +111,10 @@ __device__ double covg_point(const TpCosmoConst &C, const double *T, int n_z, do
         const double pref = tp_limber_pref<double>(C, z, Z.g);
         TpHaloZT<double> h;
         tp_halofit_z<double>(C, Z, z, &h);
-        tri = pow(pref, 3) * tp_halofit_k<double>(C, Z.D1, Z.rs, h, l1 / Z.fK) *
-              tp_halofit_k<double>(C, Z.D1, Z.rs, h, l2 / Z.fK) * tp_halofit_k<double>(C, Z.D1, Z.rs, h, l3 / Z.fK);
+        const double lrs = log(Z.rs), k1 = l1 / Z.fK, k2 = l2 / Z.fK, k3 = l3 / Z.fK;
+        tri = pow(pref, 3) * tp_halofit_k<double>(C, Z.D1, Z.rs, lrs, h, k1, log(k1)) *
+              tp_halofit_k<double>(C, Z.D1, Z.rs, lrs, h, k2, log(k2)) *
+              tp_halofit_k<double>(C, Z.D1, Z.rs, lrs, h, k3, log(k3));
     }
     result *= tri;
     result *= l1 * l2;
@@ -175,12 +179,13 @@ __global__ void __launch_bounds__(256) k_t1_points(const TpCosmoConst *__restric
         double p1 = shot, p2 = shot, p3 = shot;
         if (!(flags & TP_PK_SHAPENOISE_ONLY)) {
             double q1 = 0, q2 = 0, q3 = 0;
+            const double g1 = log(l1), g2 = log(l2), g3 = log(l3);
             for (int j = 0; j < 48; ++j) {
                 const TpPNode &A = N[2 * j], &B = N[2 * j + 1];
                 const double w = c_pgl_w[j];
-                q1 += w * (tp_limber_pk_node(C, A, l1, flags) + tp_limber_pk_node(C, B, l1, flags));
-                q2 += w * (tp_limber_pk_node(C, A, l2, flags) + tp_limber_pk_node(C, B, l2, flags));
-                q3 += w * (tp_limber_pk_node(C, A, l3, flags) + tp_limber_pk_node(C, B, l3, flags));
+                q1 += w * (tp_limber_pk_node(C, A, l1, g1, flags) + tp_limber_pk_node(C, B, l1, g1, flags));
+                q2 += w * (tp_limber_pk_node(C, A, l2, g2, flags) + tp_limber_pk_node(C, B, l2, g2, flags));
+                q3 += w * (tp_limber_pk_node(C, A, l3, g3, flags) + tp_limber_pk_node(C, B, l3, g3, flags));
             }
             if (l1 != 0) p1 = q1 * dx + shot;
             if (l2 != 0) p2 = q2 * dx + shot;
@@ -298,7 +303,9 @@ __global__ void __launch_bounds__(64) k_pell(const TpCosmoConst *__restrict__ cc
     const double dx = zmax / 2;
     if (j < 48 && e != 0) {
 #pragma clang fp contract(off)
-        part[j] = c_pgl_w[j] * (tp_limber_pk_node(C, N[2 * j], e, flags) + tp_limber_pk_node(C, N[2 * j + 1], e, flags));
+        const double le = log(e);
+        part[j] = c_pgl_w[j] * (tp_limber_pk_node(C, N[2 * j], e, le, flags) +
+                                tp_limber_pk_node(C, N[2 * j + 1], e, le, flags));
     }
     __syncthreads();
     if (j == 0) {

@@ -43,6 +43,7 @@ __host__ __device__ __forceinline__ void tp_z_at(const double *Tab, int n_z, dou
 template <class T>
 struct TpHaloZT {
     T f1, f2, f3, a, b, c, gam, alpha, beta, xnu;
+    T lf3c;  // ln(f3 c): the k-part takes (f3 c y)^(3 - gam) as exp((3 - gam)(ln(f3 c) + ln y))
 };
 
 template <class T>
@@ -69,17 +70,47 @@ __host__ __device__ void tp_halofit_z(const TpCosmoConst &C, const TpZAtT<T> &Z,
     h->alpha = fabs(6.0835 + 1.3373 * n_eff - 0.1959 * nsqr - 5.5274 * nc);
     h->beta = 2.0379 - 0.7354 * n_eff + 0.3157 * nsqr + 1.2490 * n_eff * nsqr + 0.3980 * nsqr * nsqr - 0.1682 * nc;
     h->xnu = pow(10.0, 5.2105 + 3.6902 * n_eff);
+    h->lf3c = log(h->f3 * h->c);
+}
+
+// The k-dependent part in log space: every power of k is exp(e ln k), with ln k supplied by the caller
+// (shared by the nodes of a fixed-node sum) -- 7 exp and 3 log per evaluation instead of 6 pow, an exp
+// and a log; the rounding differs from the pow forms by ~|e ln k| ulp (< 1e-14 relative).
+// Eisenstein & Hu P_lin (tp_pk_eh): (k h 2997.9 / h)^(3 + n_s) = exp((3 + n_s)(ln k + ln 2997.9)).
+template <class T>
+__host__ __device__ T tp_pk_eh_log(const TpCosmoConst &C, T k, T lk, double norm)
+{
+    const double ln29979 = 8.005667322535853;  // ln 2997.9
+    const T kh = k * C.h;
+    const T x = 0.43 * kh * C.eh_sh;
+    const T x2 = x * x;
+    const T geff = C.eh_omh2 * (C.eh_sqrt_alnu + (1.0 - C.eh_sqrt_alnu) / (1.0 + x2 * x2));
+    const T qeff = kh / geff * C.eh_theta2;
+    const T L = log(2.718281828 + C.eh_cL * qeff);
+    const T Cq = 14.4 + 325.0 / (1.0 + 60.5 * exp(1.11 * log(qeff)));
+    const T r = L / (L + Cq * qeff * qeff);
+    const T delk = norm * norm * exp((3. + C.ns) * (lk + ln29979)) * r * r;
+    const T pk = 2.0 * tpd::kPi * tpd::kPi / (kh * kh * kh) * delk;
+    return C.h * C.h * C.h * pk;
 }
 
 template <class T>
-__host__ __device__ T tp_halofit_k(const TpCosmoConst &C, T D1, T rs, const TpHaloZT<T> &h, T k)
+__host__ __device__ __forceinline__ T tp_pk_lin_log(const TpCosmoConst &C, T k, T lk)
+{
+    if constexpr (std::is_same<T, double>::value)
+        if (C.tab.nk) return tp_pk_table(C.tab, k);
+    return tp_pk_eh_log<T>(C, k, lk, C.norm);
+}
+
+template <class T>
+__host__ __device__ T tp_halofit_k(const TpCosmoConst &C, T D1, T rs, T lrs, const TpHaloZT<T> &h, T k, T lk)
 {
     const double twopi2 = 2 * tpd::kPi * tpd::kPi;
-    const T plin = tp_pk_lin<T>(C, k, C.norm) * D1 * D1 * k * k * k / twopi2;
-    const T y = k * rs, ysqr = y * y;
-    T ph = h.a * pow(y, h.f1 * 3) / (1 + h.b * pow(y, h.f2) + pow(h.f3 * h.c * y, 3 - h.gam));
+    const T plin = tp_pk_lin_log<T>(C, k, lk) * D1 * D1 * k * k * k / twopi2;
+    const T y = k * rs, ly = lk + lrs, ysqr = y * y;
+    T ph = h.a * exp(h.f1 * 3 * ly) / (1 + h.b * exp(h.f2 * ly) + exp((3 - h.gam) * (h.lf3c + ly)));
     ph = ph / (1 + h.xnu / ysqr);
-    const T pq = plin * pow(1 + plin, h.beta) / (1 + plin * h.alpha) * exp(-y / 4.0 - ysqr / 8.0);
+    const T pq = plin * exp(h.beta * log1p(plin)) / (1 + plin * h.alpha) * exp(-y / 4.0 - ysqr / 8.0);
     return twopi2 * (pq + ph) / (k * k * k);
 }
 
@@ -88,22 +119,23 @@ __host__ __device__ T tp_pnl(const TpCosmoConst &C, const TpZAtT<T> &Z, T k, T z
 {
     TpHaloZT<T> h;
     tp_halofit_z<T>(C, Z, z, &h);
-    return tp_halofit_k<T>(C, Z.D1, Z.rs, h, k);
+    return tp_halofit_k<T>(C, Z.D1, Z.rs, log(Z.rs), h, k, log(k));
 }
 
 // P(k, z): Halofit, or D1^2 P_lin with TP_PK_LINEAR (linear_pk_at_z, bispectrum.cpp:369-380).
 template <class T>
 __host__ __device__ __forceinline__ T tp_pk_at(const TpCosmoConst &C, const TpZAtT<T> &Z, T k, T z, int flags)
 {
-    return (flags & TP_PK_LINEAR) ? Z.D1 * Z.D1 * tp_pk_lin<T>(C, k, C.norm) : tp_pnl<T>(C, Z, k, z);
+    return (flags & TP_PK_LINEAR) ? Z.D1 * Z.D1 * tp_pk_lin_log<T>(C, k, log(k)) : tp_pnl<T>(C, Z, k, z);
 }
 
 // The ratio fitted to the T17 simulations, cuda_version/bispectrum.cu:971-979.
+// (1 + c1 k^-a1)^a1 / (1 + c2 k^-a2)^a3 evaluated as exp(a1 ln(1 + c1 e^(-a1 ln k)) - a3 ln(1 + c2 e^(-a2 ln k))).
 template <class T>
-__host__ __device__ __forceinline__ T tp_t17_factor(T k)
+__host__ __device__ __forceinline__ T tp_t17_factor(T lk)
 {
     const double c1 = 9.5171e-4, c2 = 5.1543e-3, a1 = 1.3063, a2 = 1.1475, a3 = 0.62793;
-    return pow(1 + c1 * pow(k, -1. * a1), a1) / pow(1 + c2 * pow(k, -1. * a2), a3);
+    return exp(a1 * log1p(c1 * exp(-1. * a1 * lk)) - a3 * log1p(c2 * exp(-1. * a2 * lk)));
 }
 
 // The Limber prefactor 9/4 (H0/c)^3 om^2 (1+z)^2 g^2 / E(z) (classed_version/bispectrum.cpp:574-577).
@@ -124,15 +156,23 @@ __host__ __device__ T tp_limber_pk(const TpCosmoConst &C, const double *Tab, int
     TpZAtT<T> Z;
     tp_z_at<T>(Tab, n_z, C.zmax, z, &Z);
     const T pref = tp_limber_pref<T>(C, z, Z.g);
-    const T k = ell / Z.fK;
-    const T P = tp_pk_at<T>(C, Z, k, z, flags);
-    return (flags & TP_PK_T17) ? tp_t17_factor<T>(k) * pref * P : T(1.0) * pref * P;
+    const T k = ell / Z.fK, lk = log(k);
+    T P;
+    if (flags & TP_PK_LINEAR) {
+        P = Z.D1 * Z.D1 * tp_pk_lin_log<T>(C, k, lk);
+    } else {
+        TpHaloZT<T> h;
+        tp_halofit_z<T>(C, Z, z, &h);
+        P = tp_halofit_k<T>(C, Z.D1, Z.rs, log(Z.rs), h, k, lk);
+    }
+    return (flags & TP_PK_T17) ? tp_t17_factor<T>(lk) * pref * P : T(1.0) * pref * P;
 }
 
 // One z-node of a fixed-node Limber sum of P: every z-only factor of tp_limber_pk, evaluated once
 // (same expressions, same bits).
 struct TpPNode {
     double fK, D1, rs, pref;
+    double lfK, lrs;  // ln f_K, ln r_sigma: ln k = ln ell - ln f_K, ln y = ln k + ln r_sigma
     TpHaloZT<double> h;
     int zero;  // z < 1e-5: the integrand is 0 there
     int pad;
@@ -146,6 +186,8 @@ __host__ __device__ __forceinline__ void tp_pnode_make(const TpCosmoConst &C, co
     o->fK = Z.fK;
     o->D1 = Z.D1;
     o->rs = Z.rs;
+    o->lfK = log(Z.fK);
+    o->lrs = log(Z.rs);
     o->pref = tp_limber_pref<double>(C, z, Z.g);
     tp_halofit_z<double>(C, Z, z, &o->h);
     o->zero = z < 1e-5 ? 1 : 0;
@@ -153,14 +195,15 @@ __host__ __device__ __forceinline__ void tp_pnode_make(const TpCosmoConst &C, co
 }
 
 // tp_limber_pk at a precomputed node
+// (lell = ln ell, computed once per multipole)
 __host__ __device__ __forceinline__ double tp_limber_pk_node(const TpCosmoConst &C, const TpPNode &N, double ell,
-                                                             int flags)
+                                                             double lell, int flags)
 {
     if (N.zero) return 0.0;
-    const double k = ell / N.fK;
-    const double P = (flags & TP_PK_LINEAR) ? N.D1 * N.D1 * tp_pk_lin<double>(C, k, C.norm)
-                                            : tp_halofit_k<double>(C, N.D1, N.rs, N.h, k);
-    return (flags & TP_PK_T17) ? tp_t17_factor<double>(k) * N.pref * P : 1.0 * N.pref * P;
+    const double k = ell / N.fK, lk = lell - N.lfK;
+    const double P = (flags & TP_PK_LINEAR) ? N.D1 * N.D1 * tp_pk_lin_log<double>(C, k, lk)
+                                            : tp_halofit_k<double>(C, N.D1, N.rs, N.lrs, N.h, k, lk);
+    return (flags & TP_PK_T17) ? tp_t17_factor<double>(lk) * N.pref * P : 1.0 * N.pref * P;
 }
 
 // <Map^2> integrand at (ell, z) for one source bin, cuda_version/apertureStatistics.cu:29-57:
