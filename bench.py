@@ -365,18 +365,19 @@ def cov_bench(tp, W, ctx, args):
         v, ne = tabs.map3_cov_t1(pairs, area, 0.0, 0.37, n_gal, flags)
     dt = time.perf_counter() - tt
     st = ctx.stats()
-    pts = st.get("points", 0)
-    node_pairs = 3 * 48 * pts  # three Pell per point, 48 node pairs each
+    pts, nreg = st.get("points", 0), st.get("regions", 0)
     agg = {"ms_per_step": 1e3 * dt / args.steps, "pairs_per_s": pairs.shape[0] * args.steps / dt,
-           "points": pts, "integrand_ms": st.get("integrand_ms", 0.0), "launches": st.get("launches", 0),
-           "rounds": st.get("rounds", 0)}
+           "points": pts, "regions": nreg, "integrand_ms": st.get("integrand_ms", 0.0),
+           "launches": st.get("launches", 0), "rounds": st.get("rounds", 0)}
     c = tp.census_power_point()
-    # a T1 point: 3 Pell of 96 z-nodes, each node the k-dependent part of the Limber P (the z-only Halofit
-    # coefficients are kept per node), plus 3 flops per node pair and the point's filters (l3: 7 flops +
-    # cos + sqrt; 6 uHat: 4 flops + exp each; 8 products)
+    # a T1 region (k_t1_regions): 47 Pell (the 7 distinct l1, 7 distinct l2 and 33 l3 of its 33 points) of
+    # 96 z-nodes, each node the k-dependent part of the Limber P (the z-only Halofit coefficients are kept
+    # per node), plus 3 flops per node pair; per point l3 (7 flops + cos + sqrt), 6 uHat (4 flops + exp
+    # each) and 8 products; the 3-D Genz-Malik rule (~90 flops)
     fe_node = c["node_flops"] + FLOPS_PER_ELEM * c["node_elem"]
-    fe_point = 3 * (96 * fe_node + 48 * 3 + 2) + (7 + 2 * FLOPS_PER_ELEM) + 6 * (4 + FLOPS_PER_ELEM) + 8
-    ach = fe_point * pts / (agg["integrand_ms"] * 1e-3) if agg["integrand_ms"] > 0 else 0.0
+    fe_region = (47 * (96 * fe_node + 48 * 3 + 2) + 33 * ((7 + 2 * FLOPS_PER_ELEM) + 6 * (4 + FLOPS_PER_ELEM) + 8)
+                 + 90)
+    ach = fe_region * nreg / (agg["integrand_ms"] * 1e-3) if agg["integrand_ms"] > 0 else 0.0
     results["batched_fp64"] = agg
     line = {
         "metric": "Gaussian <Map^3> covariance T1 pairs/s on 1 GPU (infinite survey)",
@@ -388,10 +389,10 @@ def cov_bench(tp, W, ctx, args):
                                "rel tol 1e-4", "path": "batched_fp64", "tables_ms": tables_ms,
                    "t1_integrals": int(sum({3: 6, 2: 3, 1: 1}[len(set(p[3:]))] for p in pairs))},
         "paths": results,
-        "roofline": {"bound": "valu", "kernel": "k_t1_points", "achieved": ach / 1e12,
+        "roofline": {"bound": "valu", "kernel": "k_t1_regions", "achieved": ach / 1e12,
                      "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK["fp64"], "traffic": None,
-                     "work_per_point_flop_eq": fe_point, "flops_per_elementary_call": FLOPS_PER_ELEM,
-                     "node_evaluations_per_point": 288},
+                     "work_per_region_flop_eq": fe_region, "flops_per_elementary_call": FLOPS_PER_ELEM,
+                     "node_evaluations_per_region": 47 * 96},
     }
     if not args.no_cpu:
         print("bench: cpu baseline ...", file=sys.stderr, flush=True)

@@ -24,12 +24,14 @@ def test_cov_gauss_matches_oracle(fid_tables, fid_oracle):
         assert abs(v[i] / ov - 1) < 1e-10
 
 
-@pytest.mark.parametrize("flags", [0, tp.PK_T17, tp.PK_SHAPENOISE_ONLY])
-def test_t1_total_matches_oracle(fid_tables, fid_oracle, flags):
+@pytest.mark.parametrize("flags,lmin", [(0, 0.0), (tp.PK_T17, 0.0), (tp.PK_SHAPENOISE_ONLY, 0.0), (tp.PK_T17, 20.0)])
+def test_t1_total_matches_oracle(fid_tables, fid_oracle, flags, lmin):
+    """lmin = 20 cuts regions partly (points with l3 <= lmin are zero): the device's per-region
+    Pell cache skips those points' Pell and must still give the oracle's values and point counts."""
     pairs = np.array([[4, 8, 8, 8, 8, 8], [6, 6, 12, 4, 8, 16], [8, 8, 8, 6, 12, 6]], dtype=float) * A
-    v, n = fid_tables.map3_cov_t1(pairs, AREA, 0.0, 0.3, NGAL, flags, reltol=1e-3)
+    v, n = fid_tables.map3_cov_t1(pairs, AREA, lmin, 0.3, NGAL, flags, reltol=1e-3)
     for i, p in enumerate(pairs):
-        ov, on = fid_oracle.t1_total(p[:3], p[3:], AREA, 0.0, 0.3, NGAL, flags, 1e-3)
+        ov, on = fid_oracle.t1_total(p[:3], p[3:], AREA, lmin, 0.3, NGAL, flags, 1e-3)
         assert n[i] == on, (i, n[i], on)
         assert abs(v[i] / ov - 1) < 1e-10
 

@@ -7,14 +7,17 @@
 //                                (classed_version/bispectrum.cpp:1034-1059), one wave per ell;
 //   * convergence_power_spectrum(ell): adaptive 1-D cubature over z (bispectrum.cpp:382-417);
 //   * <Map^2>(theta)           : adaptive 2-D cubature over (ell, z) (cuda_version/apertureStatistics.cu:29-136).
+// and the Gaussian <Map^3> covariance integrands (the 4-D single-z form of the classed_version and
+// T1 of the cuda_version for an infinite survey).
 // The adaptive integrals of one call run together: every round gathers the pending regions of all
-// unfinished integrals, one launch evaluates all their rule points (15 Gauss-Kronrod points in 1-D,
-// 17 Genz-Malik points in 2-D), and the host applies the rules and advances each integral's heap
-// with the decision sequence of the reference's one-at-a-time hcubature.
+// unfinished integrals and one launch evaluates all their rule points (15 Gauss-Kronrod points in 1-D,
+// 17 / 33 / 57 Genz-Malik points in 2-D / 3-D / 4-D); each integral's heap advances with the decision
+// sequence of the reference's one-at-a-time hcubature.  P_kappa and the 4-D covariance apply the rules
+// on the host; <Map^2> and T1 apply them on the device (bit-identical arithmetic order).
 //
-// Per point the integrand is one table interpolation, the Halofit coefficients (13 pow), the
-// linear P(k) (E&H fit: 2 pow, 1 log) and the filter: it is transcendental-bound FP64 work on a
-// few doubles of input, so a launch is bounded by its FP64 issue rate, and the rounds by the host.
+// Per point the integrand is one table interpolation, the Halofit z-coefficients (pow), the k-part in
+// log space and the filter: transcendental-bound FP64 work on a few doubles of input, so a launch is
+// bounded by its FP64 issue rate, and the rounds by the host.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -65,24 +68,11 @@ __device__ double pell_nodes(const TpCosmoConst &C, const TpPNode *__restrict__ 
     return q * dx + shot;
 }
 
-// T1 of the Gaussian <Map^3> covariance for an infinite survey at (l1, l2, phi),
-// cuda_version/apertureStatisticsCovariance.cu:1958-1985.
-__device__ double t1_point(const TpCosmoConst &C, const TpPNode *__restrict__ N, double dx, double l1, double l2,
-                           double phi, const TpPowerDesc &D)
+// l3 = |l1 + l2| of the T1 integrand at (l1, l2, phi), cuda_version/apertureStatisticsCovariance.cu:1962
+__device__ __forceinline__ double t1_l3(double l1, double l2, double phi)
 {
-    double l3;
-    {
 #pragma clang fp contract(off)
-        l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * cos(phi));
-    }
-    if (l1 <= D.lmin || l2 <= D.lmin || l3 <= D.lmin || l3 > D.lmax) return 0;
-    const double shot = 0.5 * D.sigma * D.sigma / D.ngal;
-    double r = pell_nodes(C, N, l1, D.flags, dx, shot) * pell_nodes(C, N, l2, D.flags, dx, shot) *
-               pell_nodes(C, N, l3, D.flags, dx, shot);
-    r *= uhat(l1 * D.th[0]) * uhat(l2 * D.th[1]) * uhat(l3 * D.th[2]);
-    r *= uhat(l1 * D.th[3]) * uhat(l2 * D.th[4]) * uhat(l3 * D.th[5]);
-    r *= l1 * l2;
-    return r;
+    return sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * cos(phi));
 }
 
 // The classed_version's Gaussian covariance integrand at (l1, l2, phi, z),
@@ -124,7 +114,7 @@ __device__ double covg_point(const TpCosmoConst &C, const double *T, int n_z, do
 
 // One thread per point; pdesc[i] names the integral (desc entry) of point i.
 __global__ void k_power_points(const TpCosmoConst *__restrict__ cc, const double *__restrict__ tab, int n_z,
-                               const TpPNode *__restrict__ pnode, double pdx, const TpPowerDesc *__restrict__ desc,
+                               const TpPowerDesc *__restrict__ desc,
                                const int *__restrict__ pdesc, size_t npts, int dim, const double *__restrict__ x,
                                double *__restrict__ f, int *flag)
 {
@@ -145,8 +135,6 @@ __global__ void k_power_points(const TpCosmoConst *__restrict__ cc, const double
         TpZAtT<double> Z;
         tp_z_at<double>(T, n_z, C.zmax, xi[1], &Z);
         v = tp_pk_at<double>(C, Z, xi[0], xi[1], D.flags);
-    } else if (D.kind == kT1) {
-        v = t1_point(C, pnode + (size_t)D.icosmo * TP_NNODES, pdx, xi[0], xi[1], xi[2], D);
     } else {
         v = covg_point(C, T, n_z, xi[0], xi[1], xi[2], xi[3], D.th);
     }
@@ -154,50 +142,81 @@ __global__ void k_power_points(const TpCosmoConst *__restrict__ cc, const double
     f[i] = v;
 }
 
-// T1 points in a kernel of their own: one thread per (l1, l2, phi); the three Pell of a point share one
-// pass over the 96 node records (each record read once for the three multipoles), and each Pell keeps
-// the reference's node-pair order, so the values equal pell_nodes' bit for bit.
-__global__ void __launch_bounds__(256) k_t1_points(const TpCosmoConst *__restrict__ cc,
-                                                   const TpPNode *__restrict__ pnode, double dx,
-                                                   const TpPowerDesc *__restrict__ desc,
-                                                   const int *__restrict__ pdesc, size_t npts,
-                                                   const double *__restrict__ x, double *__restrict__ f, int *flag)
+// T1 of the Gaussian <Map^3> covariance for an infinite survey (cuda_version/
+// apertureStatisticsCovariance.cu:1958-1985) over whole 3-D regions (l1, l2, phi): the rule points are
+// generated and the Genz-Malik rule applied on the device, so a round moves one region record in and
+// one result out per region.  Of a region's 33 points only 7 distinct l1 (the axis-0 codes -3..3) and
+// 7 distinct l2 occur, so a region needs 7 + 7 + 33 = 47 Pell, not 99.  A workgroup takes kT1Reg
+// regions in three phases:
+//   1. one thread per Pell (7 l1, 7 l2, 33 l3 per region; those of cut points skipped), the node order
+//      of pell_nodes, so each value has the bits of a per-point evaluation;
+//   2. one thread per point: the integrand from the cached Pell, in the reference's product order;
+//   3. one thread per region: the rule (gm_rule3, the host rule's arithmetic order).
+constexpr int kT1Reg = 5;                     // regions per workgroup: 235 Pell on 256 threads
+constexpr int kT1Pell = 14 + TP_NPTS_GM3;     // Pell per region
+
+__global__ void __launch_bounds__(256) k_t1_regions(const TpCosmoConst *__restrict__ cc,
+                                                    const TpPNode *__restrict__ pnode, double dx,
+                                                    const TpPowerDesc *__restrict__ desc,
+                                                    const TpRegion *__restrict__ reg, size_t nreg,
+                                                    TpRegionOut *__restrict__ out)
 {
 #pragma clang fp contract(off)
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npts) return;
-    const TpPowerDesc &D = desc[pdesc[i]];
-    const int ic = D.icosmo, flags = D.flags;
-    const TpCosmoConst &C = cc[ic];
-    const TpPNode *N = pnode + (size_t)ic * TP_NNODES;
-    const double l1 = x[3 * i], l2 = x[3 * i + 1], phi = x[3 * i + 2];
-    const double l3 = sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * cos(phi));
-    const double lmin = D.lmin, lmax = D.lmax;
-    double v = 0;
-    if (!(l1 <= lmin || l2 <= lmin || l3 <= lmin || l3 > lmax)) {
-        const double sg = D.sigma, shot = 0.5 * sg * sg / D.ngal;
-        double p1 = shot, p2 = shot, p3 = shot;
-        if (!(flags & TP_PK_SHAPENOISE_ONLY)) {
-            double q1 = 0, q2 = 0, q3 = 0;
-            const double g1 = log(l1), g2 = log(l2), g3 = log(l3);
-            for (int j = 0; j < 48; ++j) {
-                const TpPNode &A = N[2 * j], &B = N[2 * j + 1];
-                const double w = c_pgl_w[j];
-                q1 += w * (tp_limber_pk_node(C, A, l1, g1, flags) + tp_limber_pk_node(C, B, l1, g1, flags));
-                q2 += w * (tp_limber_pk_node(C, A, l2, g2, flags) + tp_limber_pk_node(C, B, l2, g2, flags));
-                q3 += w * (tp_limber_pk_node(C, A, l3, g3, flags) + tp_limber_pk_node(C, B, l3, g3, flags));
+    __shared__ double pl[kT1Reg][kT1Pell];
+    __shared__ double fv[kT1Reg][TP_NPTS_GM3];
+    const size_t r0 = (size_t)blockIdx.x * kT1Reg;
+    const int t = threadIdx.x;
+    if (t < kT1Reg * kT1Pell) {
+        const int rr = t / kT1Pell, k = t - rr * kT1Pell;
+        const size_t r = r0 + rr;
+        double v = 0;
+        if (r < nreg) {
+            const TpRegion R = reg[r];
+            const TpPowerDesc &D = desc[R.integral];
+            double ell;
+            bool need;
+            if (k < 14) {  // k = 7 axis + code + 3
+                const int ax = k / 7;  // selected, not indexed: the region record stays in registers
+                ell = tp_gm_coord(ax ? R.c[1] : R.c[0], ax ? R.hw[1] : R.hw[0], k - 7 * ax - 3);
+                need = ell > D.lmin;
+            } else {
+                double l1, l2, phi;
+                gm_point3(R, k - 14, &l1, &l2, &phi);
+                ell = t1_l3(l1, l2, phi);
+                need = !(l1 <= D.lmin || l2 <= D.lmin || ell <= D.lmin || ell > D.lmax);
             }
-            if (l1 != 0) p1 = q1 * dx + shot;
-            if (l2 != 0) p2 = q2 * dx + shot;
-            if (l3 != 0) p3 = q3 * dx + shot;
+            if (need)
+                v = pell_nodes(cc[D.icosmo], pnode + (size_t)D.icosmo * TP_NNODES, ell, D.flags, dx,
+                               0.5 * D.sigma * D.sigma / D.ngal);
         }
-        v = p1 * p2 * p3;
-        v *= uhat(l1 * D.th[0]) * uhat(l2 * D.th[1]) * uhat(l3 * D.th[2]);
-        v *= uhat(l1 * D.th[3]) * uhat(l2 * D.th[4]) * uhat(l3 * D.th[5]);
-        v *= l1 * l2;
+        pl[rr][k] = v;
     }
-    if (!isfinite(v)) atomicOr(flag, 1);
-    f[i] = v;
+    __syncthreads();
+    if (t < kT1Reg * TP_NPTS_GM3) {
+        const int rr = t / TP_NPTS_GM3, p = t - rr * TP_NPTS_GM3;
+        const size_t r = r0 + rr;
+        double v = 0;
+        if (r < nreg) {
+            const TpRegion R = reg[r];
+            const TpPowerDesc &D = desc[R.integral];
+            double l1, l2, phi;
+            gm_point3(R, p, &l1, &l2, &phi);
+            const double l3 = t1_l3(l1, l2, phi);
+            if (!(l1 <= D.lmin || l2 <= D.lmin || l3 <= D.lmin || l3 > D.lmax)) {
+                v = pl[rr][c_gm3_code[p][0] + 3] * pl[rr][c_gm3_code[p][1] + 10] * pl[rr][14 + p];
+                v *= uhat(l1 * D.th[0]) * uhat(l2 * D.th[1]) * uhat(l3 * D.th[2]);
+                v *= uhat(l1 * D.th[3]) * uhat(l2 * D.th[4]) * uhat(l3 * D.th[5]);
+                v *= l1 * l2;
+            }
+        }
+        fv[rr][p] = v;
+    }
+    __syncthreads();
+    if (t < kT1Reg && r0 + t < nreg) {
+        TpRegionOut o;
+        gm_rule3(reg[r0 + t].vol, fv[t], &o);
+        out[r0 + t] = o;
+    }
 }
 
 // ---------------------------------------------------------------- 2-D regions on the device
@@ -381,15 +400,8 @@ int power_launch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
     TP_HIP(hipMemcpyAsync(ctx->d_desc.p, desc.data(), bd, hipMemcpyHostToDevice, ctx->stream));
     TP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
     const unsigned blk = 256;
-    if (!desc.empty() && desc[0].kind == kT1 && pdesc) {  // a call's integrals are all of one kind
-        hipLaunchKernelGGL(k_t1_points, dim3((unsigned)((npts + blk - 1) / blk)), dim3(blk), 0, ctx->stream,
-                           c->d_const, (const TpPNode *)c->d_pnode, c->z_max / 2, (const TpPowerDesc *)ctx->d_desc.p,
-                           (const int *)ctx->d_misc.p, npts, (const double *)ctx->d_in.p, (double *)ctx->d_out.p,
-                           (int *)ctx->d_flag.p);
-    } else
     hipLaunchKernelGGL(k_power_points, dim3((unsigned)((npts + blk - 1) / blk)), dim3(blk), 0, ctx->stream,
-                       c->d_const, c->d_tab, c->n_z, (const TpPNode *)c->d_pnode, c->z_max / 2,
-                       (const TpPowerDesc *)ctx->d_desc.p,
+                       c->d_const, c->d_tab, c->n_z, (const TpPowerDesc *)ctx->d_desc.p,
                        pdesc ? (const int *)ctx->d_misc.p : nullptr, npts, dim, (const double *)ctx->d_in.p,
                        (double *)ctx->d_out.p, (int *)ctx->d_flag.p);
     TP_HIP(hipGetLastError());
@@ -473,18 +485,40 @@ int power_batch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &
     return 0;
 }
 
-// <Map^2> integrals: as power_batch, with the rule points generated and the rule applied on the
-// device, so a round moves 48 B of region in and 24 B of result out per region (not 17 points and
-// values) and the host only keeps the heaps.
-int map2_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &desc, const double *lo,
-                 const double *hi, double reltol, double *out, double *err, long long *neval)
+// Region records of the device-rule batches: the 2-D form for <Map^2>, the engine's 3-D form for T1.
+template <unsigned D>
+struct RegionIO;
+template <>
+struct RegionIO<2> {
+    using In = TpRegion2;
+    using Out = TpRegion2Out;
+    static constexpr int kPts = kGm2Pts;
+};
+template <>
+struct RegionIO<3> {
+    using In = TpRegion;
+    using Out = TpRegionOut;
+    static constexpr int kPts = TP_NPTS_GM3;
+};
+
+// As power_batch, with the rule points generated and the rule applied on the device, so a round moves
+// one region record in and one result out per region (not the rule's points and values) and the host
+// only keeps the heaps.  launch(nreg, d_regions, d_out) queues the round's kernels on ctx->stream and
+// records ctx->evm after the integrand; fv_per_region doubles of ctx->d_fvals per region are reserved
+// for kernels that pass the point values between launches.
+template <unsigned D, class Launch, class Post>
+int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const double *lo, const double *hi,
+                  double reltol, size_t fv_per_region, Launch launch, Post post, const char *what, double *out,
+                  double *err, long long *neval)
 {
+    using In = typename RegionIO<D>::In;
+    using Out = typename RegionIO<D>::Out;
     auto t0 = std::chrono::steady_clock::now();
     ctx->stats = tp_batch_stats{};
     const size_t nint = desc.size();
-    std::vector<tpc::AdaptiveIntegralT<2, 1>> I(nint);
+    std::vector<tpc::AdaptiveIntegralT<D, 1>> I(nint);
     for (size_t k = 0; k < nint; ++k)
-        if (!I[k].init(2, 1, lo + 2 * k, hi + 2 * k, 0, 0, reltol, tpc::L1))
+        if (!I[k].init(D, 1, lo + D * k, hi + D * k, 0, 0, reltol, tpc::L1))
             return tp_set_error(TP_EINVAL, "bad integration domain");
     int rc;
     const size_t bd = nint * sizeof(TpPowerDesc);
@@ -497,23 +531,23 @@ int map2_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
         offs.assign(na + 1, 0);
         for (size_t a = 0; a < na; ++a) offs[a + 1] = offs[a] + I[active[a]].pending().size();
         const size_t nreg = offs[na];
-        const size_t br = nreg * sizeof(TpRegion2), bo = nreg * sizeof(TpRegion2Out);
-        const size_t bf = nreg * kGm2Pts * sizeof(double);
-        if ((rc = ctx->d_regions.ensure(br)) || (rc = ctx->d_rout.ensure(bo)) || (rc = ctx->d_fvals.ensure(bf)) ||
+        const size_t br = nreg * sizeof(In), bo = nreg * sizeof(Out);
+        if ((rc = ctx->d_regions.ensure(br)) || (rc = ctx->d_rout.ensure(bo)) ||
+            (rc = ctx->d_fvals.ensure(std::max(nreg * fv_per_region * sizeof(double), (size_t)64))) ||
             (rc = ctx->h_regions.ensure(br)) || (rc = ctx->h_rout.ensure(bo)))
             return rc;
-        TpRegion2 *hr = (TpRegion2 *)ctx->h_regions.p;
+        In *hr = (In *)ctx->h_regions.p;
 #pragma omp parallel for schedule(dynamic, 64) if (na > 256)
         for (size_t a = 0; a < na; ++a) {
             auto &A = I[active[a]];
             const auto &pend = A.pending();
             for (size_t q = 0; q < pend.size(); ++q) {
                 const auto &R = A.regions()[pend[q]];
-                TpRegion2 &o = hr[offs[a] + q];
-                o.c[0] = R.c[0];
-                o.c[1] = R.c[1];
-                o.hw[0] = R.hw[0];
-                o.hw[1] = R.hw[1];
+                In &o = hr[offs[a] + q];
+                for (unsigned d = 0; d < D; ++d) {
+                    o.c[d] = R.c[d];
+                    o.hw[d] = R.hw[d];
+                }
                 o.vol = R.vol;
                 o.integral = (int)active[a];
                 o.pad = 0;
@@ -521,16 +555,7 @@ int map2_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
         }
         TP_HIP(hipMemcpyAsync(ctx->d_regions.p, hr, br, hipMemcpyHostToDevice, ctx->stream));
         TP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-        const size_t npts = nreg * kGm2Pts;
-        hipLaunchKernelGGL(k_map2_region_points, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, ctx->stream,
-                           c->d_const, c->d_tab, c->n_z, (const TpPowerDesc *)ctx->d_desc.p,
-                           (const TpRegion2 *)ctx->d_regions.p, nreg, (double *)ctx->d_fvals.p);
-        TP_HIP(hipGetLastError());
-        TP_HIP(hipEventRecord(ctx->evm, ctx->stream));
-        hipLaunchKernelGGL(k_map2_region_rule, dim3((unsigned)((nreg + 255) / 256)), dim3(256), 0, ctx->stream, nreg,
-                           (const TpRegion2 *)ctx->d_regions.p, (const double *)ctx->d_fvals.p,
-                           (TpRegion2Out *)ctx->d_rout.p);
-        TP_HIP(hipGetLastError());
+        if ((rc = launch(nreg, (const In *)ctx->d_regions.p, (Out *)ctx->d_rout.p))) return rc;
         TP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
         TP_HIP(hipMemcpyAsync(ctx->h_rout.p, ctx->d_rout.p, bo, hipMemcpyDeviceToHost, ctx->stream));
         TP_HIP(hipStreamSynchronize(ctx->stream));
@@ -541,8 +566,9 @@ int map2_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
         ctx->stats.integrand_ms += ms_pts;
         ctx->stats.launches += 1;
         ctx->stats.rounds += 1;
-        ctx->stats.points += (long long)npts;
-        const TpRegion2Out *ho = (const TpRegion2Out *)ctx->h_rout.p;
+        ctx->stats.regions += (long long)nreg;
+        ctx->stats.points += (long long)nreg * RegionIO<D>::kPts;
+        const Out *ho = (const Out *)ctx->h_rout.p;
         std::vector<char> fin(na, 0);
         int bad = 0;
 #pragma omp parallel for schedule(dynamic, 64) if (na > 256) reduction(| : bad)
@@ -551,7 +577,7 @@ int map2_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
             const auto &pend = A.pending();
             for (size_t q = 0; q < pend.size(); ++q) {
                 auto &R = A.regions()[pend[q]];
-                const TpRegion2Out &o = ho[offs[a] + q];
+                const Out &o = ho[offs[a] + q];
                 R.val[0] = o.val;
                 R.err[0] = o.err;
                 R.split = (unsigned)o.split;
@@ -563,7 +589,7 @@ int map2_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
             A.absorb();
             fin[a] = A.advance() ? 1 : 0;
         }
-        if (bad) return tp_set_error(TP_ENONFINITE, "<Map^2> integrand: non-finite value");
+        if (bad) return tp_set_error(TP_ENONFINITE, std::string(what) + ": non-finite value");
         std::vector<size_t> next;
         for (size_t a = 0; a < na; ++a)
             if (!fin[a]) next.push_back(active[a]);
@@ -572,12 +598,54 @@ int map2_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
     for (size_t k = 0; k < nint; ++k) {
         double v, e;
         I[k].result(&v, &e);
-        out[k] = v / 2. / tpd::kPi;  // apertureStatistics.cu:135
-        if (err) err[k] = e / 2. / tpd::kPi;
+        out[k] = post(v);
+        if (err) err[k] = post(e);
         if (neval) neval[k] = (long long)I[k].num_eval();
     }
     ctx->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;
+}
+
+// <Map^2> rounds: one thread per (region, point), then one thread per region for the rule.
+int map2_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &desc, const double *lo,
+                 const double *hi, double reltol, double *out, double *err, long long *neval)
+{
+    auto launch = [&](size_t nreg, const TpRegion2 *d_reg, TpRegion2Out *d_out) -> int {
+        const size_t npts = nreg * kGm2Pts;
+        hipLaunchKernelGGL(k_map2_region_points, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, ctx->stream,
+                           c->d_const, c->d_tab, c->n_z, (const TpPowerDesc *)ctx->d_desc.p, d_reg, nreg,
+                           (double *)ctx->d_fvals.p);
+        TP_HIP(hipGetLastError());
+        TP_HIP(hipEventRecord(ctx->evm, ctx->stream));
+        hipLaunchKernelGGL(k_map2_region_rule, dim3((unsigned)((nreg + 255) / 256)), dim3(256), 0, ctx->stream, nreg,
+                           d_reg, (const double *)ctx->d_fvals.p, d_out);
+        TP_HIP(hipGetLastError());
+        return 0;
+    };
+    return regions_batch<2>(ctx, desc, lo, hi, reltol, kGm2Pts, launch,
+                            [](double v) { return v / 2. / tpd::kPi; },  // apertureStatistics.cu:135
+                            "<Map^2> integrand", out, err, neval);
+}
+
+// T1 rounds: one launch of k_t1_regions (points, cached Pell and rule in one kernel).
+int t1_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &desc, const double *lo,
+               const double *hi, double reltol, double area, double *out, long long *neval)
+{
+    auto launch = [&](size_t nreg, const TpRegion *d_reg, TpRegionOut *d_out) -> int {
+        const size_t nblk = (nreg + kT1Reg - 1) / kT1Reg;
+        if (nblk > 0x7fffffffu) return tp_set_error(TP_ECAPACITY, "T1: too many regions for one launch");
+        hipLaunchKernelGGL(k_t1_regions, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, c->d_const,
+                           (const TpPNode *)c->d_pnode, c->z_max / 2, (const TpPowerDesc *)ctx->d_desc.p, d_reg,
+                           nreg, d_out);
+        TP_HIP(hipGetLastError());
+        TP_HIP(hipEventRecord(ctx->evm, ctx->stream));
+        return 0;
+    };
+    std::vector<double> e(desc.size());
+    return regions_batch<3>(
+        ctx, desc, lo, hi, reltol, 0, launch,
+        [area](double x) { return x * (2 / area / pow(2 * tpd::kPi, 3)); },  // apertureStatisticsCovariance.cu:1089
+        "T1 integrand", out, e.data(), neval);
 }
 
 int point_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, int kind, size_t npts, const double *in, int flags,
@@ -810,13 +878,9 @@ extern "C" int tp_map3_cov_t1_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, co
             add(1, th5, th4, th3);
         }
     }
-    std::vector<double> v(desc.size()), e(desc.size());
+    std::vector<double> v(desc.size());
     std::vector<long long> ne(desc.size());
-    const double A = area;
-    rc = power_batch<3>(
-        ctx, c, desc, lo.data(), hi.data(), reltol,
-        [A](double x) { return x * (2 / A / pow(2 * tpd::kPi, 3)); },  // apertureStatisticsCovariance.cu:1089
-        v.data(), e.data(), ne.data());
+    rc = t1_regions(ctx, c, desc, lo.data(), hi.data(), reltol, area, v.data(), ne.data());
     if (rc) return rc;
     for (size_t i = 0; i < n; ++i) {
         double r = 0;
