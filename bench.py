@@ -530,14 +530,21 @@ def main():
     hr = results[head]
     dom = hr.get("integrand_kernel") or next(r["integrand_kernel"] for r in results.values() if "integrand_kernel" in r)
     prec = "float" if head.startswith("batched_fp32") else "double"
-    kname = "k_region_points<%s, waves per point tuned>" % prec
+    kname = "k_region_points<%s, 1/2/4/8 waves per point> | k_region_cached<%s> (shape tuned per launch size)" % (
+        prec, prec)
     pts_per_launch = hr["points"] / max(1, hr["launches"])
+    # the region-cached shape evaluates the P_E terms (3 per node and point) once per distinct multipole of
+    # a region: 47 per 33 points instead of 99, so it executes less than the algorithmic work per point
+    fe_pse = census["pse_flops"] + FLOPS_PER_ELEM * census["pse_elem"]
+    fe_exec_cached = fe_point - census["nodes"] * fe_pse * (99 - 47) / 33
     roof = {"bound": "valu", "kernel": kname,
             "achieved": dom["achieved_flops"] / 1e12, "peak": dom["peak"] / 1e12, "unit": "TFLOP/s",
             "frac": dom["frac"], "traffic": HBM_BYTES_PER_POINT[prec] * pts_per_launch,
             "traffic_unit": "HBM bytes per launch (per-point bytes from the PMC passes x points per launch)",
             "algorithmic_bytes_per_point": 64 / 33 + 8,
             "work_per_point_flop_eq": fe_point, "flops_per_elementary_call": FLOPS_PER_ELEM,
+            "executed_work_per_point_flop_eq_region_cached": fe_exec_cached,
+            "executed_frac_if_all_region_cached": dom["frac"] * fe_exec_cached / fe_point,
             "census": census}
     line = {
         "metric": "Map3 triples/s on 1 GPU: plain FP64 vs surrogate form at <=1e-3 rel err",

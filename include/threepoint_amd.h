@@ -119,10 +119,12 @@ int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, const int *ic
                   long long *neval);
 
 /* Launch shape of the batched region kernel: a cubature point's 96 z-nodes are summed by 1, 2, 4
- * or 8 waves.  By default the shape is tuned per launch-size bucket on the first batched call of
- * a context and precision (all shapes give bit-identical results).  lanes = 0 restores tuning;
- * 1, 2, 4 or 8 forces that shape.  No counterpart in the reference (its GPU build has one
- * fixed launch shape). */
+ * or 8 waves, or (16, 32, 64) by the region-cached kernel with 1, 2 or 4 groups of 256 threads per
+ * workgroup, which evaluates the one-wavenumber terms of BiHalofit once per distinct multipole of
+ * a region (47 per 33 points) in LDS.  By default the shape is tuned per launch-size bucket on the
+ * first batched call of a context and precision (all shapes give bit-identical results).
+ * lanes = 0 restores tuning; 1, 2, 4, 8, 16, 32 or 64 forces that shape.  No counterpart in the
+ * reference (its GPU build has one fixed launch shape). */
 int tp_set_region_lanes(tp_ctx *ctx, int lanes);
 /* The tuned shapes of mode (TP_MODE_FP64 or TP_MODE_FP32) for launches of < 32k, < 160k,
  * < 640k and more points (0 = not tuned yet). */
@@ -136,6 +138,20 @@ int tp_set_graph_rounds(tp_ctx *ctx, int on);
  * one group's regions the host runs the others' heap decisions.  Results do not depend on it.  No
  * counterpart in the reference. */
 int tp_set_batch_groups(tp_ctx *ctx, int groups);
+/* Per-round trace of tp_map3_batch (diagnostics, off by default).  While on, every cubature round
+ * of a call appends 7 doubles: group, regions, host launch begin and end, completion begin (after
+ * the wait) and end [ms since the call began], and the time waited for the device [ms].  The trace
+ * is cleared at the start of each call; tp_round_log copies up to cap doubles and sets *n to the
+ * number available.  No counterpart in the reference. */
+int tp_set_round_log(tp_ctx *ctx, int on);
+
+/* OpenMP threads of the host side of the batched cubatures (region records and heap decisions
+ * between device rounds): n = 0 restores the defaults (8 for tp_map3_batch, whose host work
+ * overlaps the device; 16 for the synchronous batches); the count is capped by
+ * omp_get_max_threads().  The results do not depend on it.  No counterpart in the reference. */
+int tp_set_host_threads(tp_ctx *ctx, int n);
+int tp_round_log(const tp_ctx *ctx, double *out, size_t cap, size_t *n);
+
 /* Exact-mode rounds of at most max_regions regions read their regions from, and write their results
  * to, pinned host memory directly instead of copying them (0 = always copy).  Results do not depend
  * on it.  No counterpart in the reference. */
@@ -170,6 +186,11 @@ int tp_hbm_copy_probe(tp_ctx *ctx, size_t nbytes, double *gbps);
  * elementary calls per point outside the node loop, out[4] z-nodes per point.  Counted by
  * instantiating the kernel's own device functions with a counting scalar on the host. */
 int tp_census_map3_point(double *out);
+
+/* Census of one P_E term of BiHalofit (eq. B6: the linear P(k) and the one-halo-free part of one
+ * wavenumber): out[0] flops, out[1] elementary calls.  A z-node evaluation makes three; the
+ * region-cached kernel makes 47 per 33 points.  No counterpart in the reference. */
+int tp_census_map3_pse(double *out);
 /* Census of the Gamma_i integrand (tp_gamma_eval, tp_gamma_batch): out[0] flops and out[1]
  * elementary calls per Ogata term (a cubature point has 3 (prec_k - 1) terms), out[2] flops
  * and out[3] elementary calls of the per-point triangle shape. */

@@ -160,14 +160,15 @@ def test_nonfinite_theta_rejected(fid_tables):
 
 
 def test_launch_shapes_bit_identical(gpu_ctx, fid_tables):
-    """Every launch shape of the region kernel (1, 2, 4, 8 waves per point) gives the same bits,
-    so the autotuner's choice never changes a result."""
+    """Every launch shape of the region kernel (1, 2, 4, 8 waves per point; 16, 32, 64 = the
+    region-cached kernel with 1, 2, 4 groups) gives the same bits, so the autotuner's choice never
+    changes a result."""
     import ctypes
     th = W.to_rad(W.triples(W.map3_c0_radii()))
     try:
         for mode in (tp.MODE_FP64, tp.MODE_FP32):
             ref = None
-            for lanes in (1, 2, 4, 8):
+            for lanes in (1, 2, 4, 8, 16, 32, 64):
                 tp._check(tp.lib().tp_set_region_lanes(gpu_ctx.h, lanes), "tp_set_region_lanes")
                 v, e, n = fid_tables.map3_batch(th, None, 1e-4, mode)
                 if ref is None:
@@ -181,7 +182,7 @@ def test_launch_shapes_bit_identical(gpu_ctx, fid_tables):
     tab = (ctypes.c_int * 4)()
     tp._check(tp.lib().tp_region_lanes_table(gpu_ctx.h, tp.MODE_FP32, tab), "tp_region_lanes_table")
     print("tuned waves per point (FP32, by launch size):", list(tab))
-    assert all(x in (1, 2, 4, 8) for x in tab)
+    assert all(x in (1, 2, 4, 8, 16, 32, 64) for x in tab)
 
 
 def test_given_pk_matches_oracle(gpu_ctx, fid_nz):

@@ -110,12 +110,16 @@ _SIGS = {
     "tp_set_graph_rounds": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "tp_set_batch_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "tp_set_zero_copy_regions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
+    "tp_set_round_log": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "tp_set_host_threads": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "tp_round_log": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "tp_gamma_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
                                       ctypes.c_double, _dp, _dp, _llp]),
     "tp_hcubature_v": (ctypes.c_int, [ctypes.c_uint, INTEGRAND_V, ctypes.c_void_p, ctypes.c_uint, _dp, _dp,
                                       ctypes.c_size_t, ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp, _dp]),
     "tp_hcubature_last_neval": (ctypes.c_longlong, []),
     "tp_census_map3_point": (ctypes.c_int, [_dp]),
+    "tp_census_map3_pse": (ctypes.c_int, [_dp]),
     "tp_census_gamma_term": (ctypes.c_int, [_dp]),
     "tp_census_power_point": (ctypes.c_int, [_dp]),
     "tp_hbm_copy_probe": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _dp]),
@@ -194,6 +198,22 @@ class Context:
         s = tp_batch_stats()
         _check(lib().tp_last_batch_stats(self.h, ctypes.byref(s)), "tp_last_batch_stats")
         return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def set_host_threads(self, n=0):
+        """OpenMP threads of the host side of the batched cubatures (0: the default)."""
+        _check(lib().tp_set_host_threads(self.h, int(n)), "tp_set_host_threads")
+
+    def set_round_log(self, on=True):
+        _check(lib().tp_set_round_log(self.h, 1 if on else 0), "tp_set_round_log")
+
+    def round_log(self):
+        """Per-round trace of the last batched <Map^3> call: rows (group, regions, launch begin, launch end,
+        completion begin, completion end [ms since the call began], wait [ms])."""
+        n = ctypes.c_size_t()
+        _check(lib().tp_round_log(self.h, None, 0, ctypes.byref(n)), "tp_round_log")
+        out = np.zeros(n.value)
+        _check(lib().tp_round_log(self.h, _d(out), n.value, ctypes.byref(n)), "tp_round_log")
+        return out.reshape(-1, 7)
 
 
 _default_ctx = None
@@ -414,7 +434,10 @@ def census_map3_point():
     """Algorithmic op census of one <Map^3> integrand point (see tp_census_map3_point)."""
     out = np.zeros(5)
     _check(lib().tp_census_map3_point(_d(out)), "tp_census_map3_point")
-    return dict(node_flops=out[0], node_elem=out[1], point_flops=out[2], point_elem=out[3], nodes=out[4])
+    pse = np.zeros(2)
+    _check(lib().tp_census_map3_pse(_d(pse)), "tp_census_map3_pse")
+    return dict(node_flops=out[0], node_elem=out[1], point_flops=out[2], point_elem=out[3], nodes=out[4],
+                pse_flops=pse[0], pse_elem=pse[1])
 
 
 def flop_equivalents_per_point(flops_per_elem=20.0):

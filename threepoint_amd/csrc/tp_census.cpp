@@ -89,6 +89,26 @@ extern "C" int tp_census_map3_point(double *out)
     return 0;
 }
 
+// out: [0] flops and [1] elementary calls of one P_E term of BiHalofit (tp_pse_node, a function of one
+// wavenumber; a node evaluation of tp_bispec_node makes three)
+extern "C" int tp_census_map3_pse(double *out)
+{
+    Cnt vals[sizeof(TpNodeT<Cnt>) / sizeof(Cnt)];
+    for (auto &v : vals) v = Cnt(0.5);
+    TpNodeT<Cnt> N = *reinterpret_cast<TpNodeT<Cnt> *>(vals);
+    N.acap = Cnt(10.0);
+    Cnt ev[sizeof(TpEHT<Cnt>) / sizeof(Cnt)];
+    for (auto &v : ev) v = Cnt(0.7);
+    TpEHT<Cnt> E = *reinterpret_cast<TpEHT<Cnt> *>(ev);
+    E.tab = TpPkTable{};
+    g_cnt = OpCounts();
+    Cnt pse = tp_pse_node<Cnt>(N, E, Cnt(0.4), Cnt(-0.9));
+    (void)pse;
+    out[0] = (double)g_cnt.flops;
+    out[1] = (double)g_cnt.trans;
+    return 0;
+}
+
 // out: [0] flops per Ogata term, [1] elementary calls per term (a Gamma cubature point has
 // 3 (prec_k - 1) terms), [2] flops and [3] elementary calls of the per-point shape
 extern "C" int tp_census_gamma_term(double *out)

@@ -136,15 +136,29 @@ __host__ __device__ __forceinline__ void tp_point_pre(T l1, T l2, T l3, TpPointP
 //   1 / (I_1h (1 + 1/(c_n q)))  = c_n q / (I_1h (c_n q + 1))                    (eq. B4, B7)
 //   (1 + f_n q^2) / (1 + g_n q + h_n q^2) D^2 P_lin + 1 / (M (1 + 1/(p_n q)^3))
 //     = (A D^2 P_lin C + B (p_n q)^3) / (B C),  C = M ((p_n q)^3 + 1)            (eq. B6)
+// The node value splits into the P_E term of eq. B6, a function of one wavenumber (tp_pse_node),
+// and the rest, which depends on the triangle shape through r1, r2 (tp_bispec_joint); kernels that
+// share wavenumbers between points evaluate the P_E terms once per distinct wavenumber.
 template <class T>
-__host__ __device__ __forceinline__ T tp_bispec_node(const TpNodeT<T> &N, const TpEHT<T> &E, const T q[3], const T lq[3],
-                                            const T f2t[3], T lr1, T r2)
+__host__ __device__ __forceinline__ T tp_pse_node(const TpNodeT<T> &N, const TpEHT<T> &E, T qi, T lqi)
+{
+    T pl = tp_plin_q(N, E, qi, lqi);
+    T pq = N.pn * qi;
+    T pq3 = pq * pq * pq;
+    T A = T(1) + N.fn * qi * qi;
+    T B = T(1) + N.gn * qi + N.hn * qi * qi;
+    T C = (N.mn * TpMath<T>::ex(N.mun * lqi) + N.nn * TpMath<T>::ex(N.nun * lqi)) * (pq3 + T(1));
+    return (A * N.D1sq * pl * C + B * pq3) * TpMath<T>::rcp(B * C);  // eq. B6
+}
+
+template <class T>
+__host__ __device__ __forceinline__ T tp_bispec_joint(const TpNodeT<T> &N, const T q[3], const T lq[3], const T f2t[3],
+                                                      T lr1, T r2, const T pse[3])
 {
     T an = TpMath<T>::ex(N.lnA0 + N.m031ln10 * TpMath<T>::ex(N.c1 * lr1));   // eq. B7
     T alphan = TpMath<T>::ex(N.a0ln + N.a1ln * r2 * r2);
     if (alphan > N.acap) alphan = N.acap;
     T betan = TpMath<T>::ex(N.b0ln + N.b1ln * r2);
-    T pse[3];
     T bs1h = T(1), damp = T(1);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -152,18 +166,21 @@ __host__ __device__ __forceinline__ T tp_bispec_node(const TpNodeT<T> &N, const 
         T oneh = an * TpMath<T>::ex(alphan * lqi) + N.bn * TpMath<T>::ex(betan * lqi);   // eq. B4
         T cq = N.cn * qi;
         bs1h *= cq * TpMath<T>::rcp(oneh * (cq + T(1)));
-        T pl = tp_plin_q(N, E, qi, lqi);
-        T pq = N.pn * qi;
-        T pq3 = pq * pq * pq;
-        T A = T(1) + N.fn * qi * qi;
-        T B = T(1) + N.gn * qi + N.hn * qi * qi;
-        T C = (N.mn * TpMath<T>::ex(N.mun * lqi) + N.nn * TpMath<T>::ex(N.nun * lqi)) * (pq3 + T(1));
-        pse[i] = (A * N.D1sq * pl * C + B * pq3) * TpMath<T>::rcp(B * C);  // eq. B6
         damp *= T(1) + N.en * qi;
     }
     T bs3h = T(2) * ((f2t[0] + N.dn * q[2]) * pse[0] * pse[1] + (f2t[1] + N.dn * q[0]) * pse[1] * pse[2] +
                      (f2t[2] + N.dn * q[1]) * pse[2] * pse[0]);                            // eq. B5 with F2 (:614-627)
     return bs1h + bs3h * TpMath<T>::rcp(damp);
+}
+
+template <class T>
+__host__ __device__ __forceinline__ T tp_bispec_node(const TpNodeT<T> &N, const TpEHT<T> &E, const T q[3], const T lq[3],
+                                            const T f2t[3], T lr1, T r2)
+{
+    T pse[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) pse[i] = tp_pse_node<T>(N, E, q[i], lq[i]);
+    return tp_bispec_joint<T>(N, q, lq, f2t, lr1, r2, pse);
 }
 
 // Same bispectrum from raw wavenumbers (used where the per-point hoisting does not apply)
@@ -224,6 +241,29 @@ __host__ __device__ __forceinline__ T tp_ogata_term(const TpNodeT<T> &N, const T
 // 1, 2, 4 or 8 waves forms the same tree, so every launch shape gives the same bits.
 constexpr int TP_NPART = 8, TP_PART_NODES = TP_NNODES / TP_NPART;
 
+// Adds partial k (0-based, in order) to the pairwise tree: s1, s2, s4 hold the pending left operands
+// of the levels (p0 + p1), ((p0 + p1) + (p2 + p3)), ...; returns the running subtree value, after
+// partial 7 the whole tree ((p0 + p1) + (p2 + p3)) + ((p4 + p5) + (p6 + p7)).
+__device__ __forceinline__ double tp_tree8_push(int k, double p, double &s1, double &s2, double &s4)
+{
+    double v = p;
+    if (k & 1) {
+        v = s1 + v;
+        if (k & 2) {
+            v = s2 + v;
+            if (k & 4)
+                v = s4 + v;
+            else
+                s4 = v;
+        } else {
+            s2 = v;
+        }
+    } else {
+        s1 = v;
+    }
+    return v;
+}
+
 // partials [part0, part0 + NP) of the tree, combined pairwise (NP a power of two)
 template <class T, int NP>
 __device__ __forceinline__ double tp_bkappa_parts(const TpNodeT<T> *__restrict__ nodes, const TpEHT<T> &E,
@@ -233,9 +273,7 @@ __device__ __forceinline__ double tp_bkappa_parts(const TpNodeT<T> *__restrict__
     const T LL[3] = {(T)P.ll[0], (T)P.ll[1], (T)P.ll[2]};
     const T F2T[3] = {(T)P.f2t[0], (T)P.f2t[1], (T)P.f2t[2]};
     const T LR1 = (T)P.lr1, R2 = (T)P.r2;
-    // one pass over the nodes; at the end of each partial its sum joins a pairwise tree built as
-    // the partials arrive (s1, s2, s4 hold the pending left operands of the levels (p0 + p1),
-    // ((p0 + p1) + (p2 + p3)), ...)
+    // one pass over the nodes; at the end of each partial its sum joins the pairwise tree
     double s1 = 0.0, s2 = 0.0, s4 = 0.0, v = 0.0, acc = 0.0;
     const int n0 = part0 * TP_PART_NODES, n1 = n0 + NP * TP_PART_NODES;
 #pragma unroll 2
@@ -248,23 +286,8 @@ __device__ __forceinline__ double tp_bkappa_parts(const TpNodeT<T> *__restrict__
         acc += (double)(N.wz * b);
         const int m = n - n0 + 1;
         if (m % TP_PART_NODES == 0) {  // partial k = m / 12 - 1 complete
-            const int k = m / TP_PART_NODES - 1;
-            v = acc;
+            v = tp_tree8_push(m / TP_PART_NODES - 1, acc, s1, s2, s4);
             acc = 0.0;
-            if (k & 1) {
-                v = s1 + v;
-                if (k & 2) {
-                    v = s2 + v;
-                    if (k & 4)
-                        v = s4 + v;
-                    else
-                        s4 = v;
-                } else {
-                    s2 = v;
-                }
-            } else {
-                s1 = v;
-            }
         }
     }
     return v;

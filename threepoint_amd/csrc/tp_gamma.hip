@@ -428,7 +428,7 @@ extern "C" int tp_gamma_setup(tp_ctx *ctx, double prec_h, double prec_k_arg)
     auto psip = [](double t) {
         return (std::sinh(M_PI * std::sinh(t)) + M_PI * t * std::cosh(t)) / (std::cosh(M_PI * std::sinh(t)) + 1);
     };
-#pragma omp parallel for schedule(dynamic, 64)
+#pragma omp parallel for schedule(dynamic, 64) num_threads(tp_host_threads(ctx, kTpHostThreadsSync))
     for (long long i = 1; i < (long long)pk; ++i) {
         for (int which = 0; which < 2; ++which) {
             int nu = which == 0 ? 6 : 2;
@@ -539,6 +539,7 @@ extern "C" int tp_gamma_batch_mode(tp_ctx *ctx, const tp_cosmo *c, int icosmo, s
     TP_HIP(hipSetDevice(ctx->device));
     auto t0 = std::chrono::steady_clock::now();
     ctx->stats = tp_batch_stats{};
+    const int nt = tp_host_threads(ctx, kTpHostThreadsSync);
     const size_t nint = 4 * ntri;
     std::vector<GammaIntegral> G(nint);
     std::vector<TpGammaDesc> desc(nint);
@@ -571,7 +572,7 @@ extern "C" int tp_gamma_batch_mode(tp_ctx *ctx, const tp_cosmo *c, int icosmo, s
         pts.resize(npts * 3);
         vals.resize(npts * 2);
         pdesc.resize(npts);
-#pragma omp parallel for schedule(dynamic, 1) if (na > 1 && npts > 16384)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt) if (na > 1 && npts > 16384)
         for (size_t a = 0; a < na; ++a) {
             auto &A = G[active[a]].I;
             const auto &pend = A.pending();
@@ -586,7 +587,7 @@ extern "C" int tp_gamma_batch_mode(tp_ctx *ctx, const tp_cosmo *c, int icosmo, s
         ctx->stats.rounds += 1;
         ctx->stats.points += (long long)npts;
         std::vector<char> fin(na, 0);
-#pragma omp parallel for schedule(dynamic, 1) if (na > 1 && npts > 16384)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt) if (na > 1 && npts > 16384)
         for (size_t a = 0; a < na; ++a) {
             auto &A = G[active[a]].I;
             const auto &pend = A.pending();

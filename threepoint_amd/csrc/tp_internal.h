@@ -178,8 +178,21 @@ struct tp_ctx {
     bool graph_rounds = false;  // batched rounds replayed as one graph per lane (tp_set_graph_rounds)
     int nlanes = 2;             // groups in flight in tp_map3_batch (tp_set_batch_groups)
     long long zero_copy_regions = 0;  // rounds up to this many regions skip the copies (pinned buffers)
+    int host_threads = 0;             // OpenMP threads of the host-side cubature work (0: default)
+    bool round_log_on = false;        // tp_set_round_log: per-round trace of tp_map3_batch
+    std::vector<double> round_log;    // 7 doubles per round (tp_round_log)
     TpLane lane[TP_NLANES];
 };
+
+// OpenMP threads for the host side of the batched cubatures (region records, heaps):
+// ctx->host_threads when set, else dflt, at most omp_get_max_threads().  The <Map^3> engine, whose
+// host work overlaps the device, defaults to kTpHostThreadsOverlap: that leaves CPU time to the
+// calling thread and the HIP runtime (with as many spinning OpenMP threads as the process's CPU
+// quota, a whole-process throttle of several ms was seen now and then between two rounds, with the
+// device idle).  The synchronous batches (host work on the critical path) default to
+// kTpHostThreadsSync.
+constexpr int kTpHostThreadsOverlap = 8, kTpHostThreadsSync = 16;
+int tp_host_threads(const tp_ctx *ctx, int dflt);
 
 // Error helpers (thread-local last error string).
 int tp_set_error(int code, const std::string &msg);

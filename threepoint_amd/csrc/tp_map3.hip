@@ -102,6 +102,153 @@ __global__ void __launch_bounds__(64 * (LPP < 4 ? 4 : LPP)) k_region_points(
     if (live) fvals[t] = tp_map3_end(P, kind, v, D.th[0], D.th[1], D.th[2], flag);
 }
 
+// Region-cached form of k_region_points (launch shapes 16, 32, 64).  Of a region's 33 points only 7
+// distinct l1 (axis-0 codes -3..3) and 7 distinct l2 occur, and the P_E term of BiHalofit
+// (tp_pse_node: the linear P(k), its exp/log and reciprocals) depends on one wavenumber only.  A
+// workgroup takes kCacheReg regions with G groups of 256 threads; group g owns the partials
+// [g 8/G, (g+1) 8/G) of the summation tree and, for each partial (12 z-nodes),
+//   A. evaluates the P_E terms of the 47 multipoles per region (7 l1, 7 l2, 33 l3) at the 12 nodes
+//      into its LDS block (work items of one node and 64 multipoles, so node reads are wave-uniform);
+//   B. lets each point add its 12 node values, the shape-dependent rest (tp_bispec_joint) with the
+//      three P_E terms read from LDS, and push the partial into the group's subtree.
+// The groups' subtrees are then joined in tree order.  Every value is formed by the same operations
+// as in k_region_points, so every shape gives the same bits.  More groups shorten a thread's share
+// of the work, which keeps launches of a few thousand regions from ending on a half-empty wave of
+// workgroups.
+constexpr int kCacheReg = 7;                       // regions per workgroup: 231 points per group
+constexpr int kCacheEll = 14 + TP_NPTS_GM3;        // multipoles per region: 7 l1, 7 l2, 33 l3
+constexpr int kCacheN = kCacheReg * kCacheEll;     // 329
+
+template <class T, int G>
+__global__ void __launch_bounds__(256 * G) k_region_cached(const TpNodeT<T> *__restrict__ nodes,
+                                                           const TpEHT<T> *__restrict__ eh, size_t nreg,
+                                                           const TpRegion *__restrict__ reg,
+                                                           const TpIntegralDesc *__restrict__ desc,
+                                                           double *__restrict__ fvals, int mode, int *flag)
+{
+    constexpr int PPG = TP_NPART / G;                 // partials per group
+    __shared__ T lv[kCacheN], llv[kCacheN];           // multipole and its log, as the points see them
+    __shared__ T pse[G][TP_PART_NODES][kCacheN];      // P_E at (group, node of the partial, multipole)
+    __shared__ double sub[G > 1 ? G : 1][kCacheReg * TP_NPTS_GM3];  // the groups' subtree values
+    __shared__ int ics[kCacheReg];                    // cosmology of each region
+    __shared__ int one_cosmo;                         // all regions of the workgroup share ics[0]
+    const int t = threadIdx.x & 255, lane = t & 63;
+    const int g = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 8);
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const size_t r0 = (size_t)blockIdx.x * kCacheReg;
+    const int nr = (int)((nreg - r0) < (size_t)kCacheReg ? (nreg - r0) : (size_t)kCacheReg);  // >= 1
+    const int ne = nr * kCacheEll;
+    // set-up: every group's point threads form their point; group 0 stores the l3 entries and
+    // its threads < 14 nr the l1 / l2 entries
+    const bool pt = t < nr * TP_NPTS_GM3;
+    const int prr = t / TP_NPTS_GM3, pp = t - prr * TP_NPTS_GM3;
+    TpPointPre P;
+    int kind = TP_PT_DONE;
+    TpIntegralDesc D{};
+    if (pt) {
+        const TpRegion R = reg[r0 + prr];
+        D = desc[R.integral];
+        double x0, x1, x2;
+        gm_point3(R, pp, &x0, &x1, &x2);
+        kind = tp_map3_begin(x0, x1, x2, mode, flag, &P);  // every group flags the same points: same bit
+        if (g == 0) {
+            const int e = prr * kCacheEll + 14 + pp;
+            lv[e] = kind == TP_PT_SUM ? (T)P.l[2] : T(1);
+            llv[e] = kind == TP_PT_SUM ? (T)P.ll[2] : T(0);
+            if (pp == 0) ics[prr] = D.icosmo;
+        }
+    }
+    if (g == 0 && t < nr * 14) {
+        const int rr = t / 14, k = t - rr * 14, ax = k / 7;  // k = 7 axis + code + 3
+        const TpRegion R = reg[r0 + rr];
+        const double x = tp_gm_coord(ax ? R.c[1] : R.c[0], ax ? R.hw[1] : R.hw[0], k - 7 * ax - 3);
+        lv[rr * kCacheEll + k] = (T)x;
+        llv[rr * kCacheEll + k] = x > 0 ? (T)TpMath<double>::lg(x) : T(0);  // non-positive: no summed point uses it
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int u = 1;
+        for (int rr = 1; rr < nr; ++rr) u &= ics[rr] == ics[0];
+        one_cosmo = u;
+    }
+    __syncthreads();
+    // block-uniform: with one cosmology the node and E&H records are read through scalar loads
+    const int uni = __builtin_amdgcn_readfirstlane(one_cosmo);
+    const int ic0 = __builtin_amdgcn_readfirstlane(ics[0]);
+    const TpNodeT<T> *__restrict__ nodes0 = nodes + (size_t)ic0 * TP_NNODES;
+    const bool sum = pt && kind == TP_PT_SUM;
+    T L[3] = {T(1), T(1), T(1)}, LL[3] = {T(0), T(0), T(0)}, F2T[3] = {T(0), T(0), T(0)}, LR1 = T(0), R2 = T(0);
+    int e0 = 0, e1 = 0, e2 = 0;
+    const TpNodeT<T> *Np = nodes;
+    if (sum) {
+        for (int i = 0; i < 3; ++i) {
+            L[i] = (T)P.l[i];
+            LL[i] = (T)P.ll[i];
+            F2T[i] = (T)P.f2t[i];
+        }
+        LR1 = (T)P.lr1;
+        R2 = (T)P.r2;
+        const int base = prr * kCacheEll;
+        e0 = base + c_gm3_code[pp][0] + 3;
+        e1 = base + c_gm3_code[pp][1] + 10;
+        e2 = base + 14 + pp;
+        Np = nodes + (size_t)D.icosmo * TP_NNODES;
+    }
+    T(*ps_g)[kCacheN] = pse[g];
+    // phase B body: the point's 12 node values of the partial starting at node n0, from records Nb
+    auto node_block = [&](const TpNodeT<T> *__restrict__ Nb, int n0) {
+        double acc = 0.0;
+        for (int j = 0; j < TP_PART_NODES; ++j) {
+            const TpNodeT<T> &N = Nb[n0 + j];
+            const T ks = N.ks, lks = N.lnks;
+            const T q[3] = {L[0] * ks, L[1] * ks, L[2] * ks};
+            const T lq[3] = {LL[0] + lks, LL[1] + lks, LL[2] + lks};
+            const T ps[3] = {ps_g[j][e0], ps_g[j][e1], ps_g[j][e2]};
+            const T b = tp_bispec_joint<T>(N, q, lq, F2T, LR1, R2, ps);
+            acc += (double)(N.wz * b);
+        }
+        return acc;
+    };
+    // phase A work items: (node j, 64 consecutive multipoles), wave-uniform j
+    const int nblk = (ne + 63) >> 6, items = TP_PART_NODES * nblk;
+    double s1 = 0.0, s2 = 0.0, s4 = 0.0, v = 0.0;
+    for (int k = 0; k < PPG; ++k) {
+        const int n0 = (g * PPG + k) * TP_PART_NODES;
+        for (int it = wv; it < items; it += 4) {
+            const int j = it / nblk, e = (it - j * nblk) * 64 + lane;
+            if (e < ne) {
+                if (uni) {
+                    const TpNodeT<T> &N = nodes0[n0 + j];
+                    ps_g[j][e] = tp_pse_node<T>(N, eh[ic0], lv[e] * N.ks, llv[e] + N.lnks);
+                } else {
+                    const int ic = ics[e / kCacheEll];
+                    const TpNodeT<T> &N = nodes[(size_t)ic * TP_NNODES + n0 + j];
+                    ps_g[j][e] = tp_pse_node<T>(N, eh[ic], lv[e] * N.ks, llv[e] + N.lnks);
+                }
+            }
+        }
+        __syncthreads();
+        if (sum) v = tp_tree8_push(k, uni ? node_block(nodes0, n0) : node_block(Np, n0), s1, s2, s4);
+        __syncthreads();
+    }
+    if constexpr (G > 1) {  // join the groups' subtrees: v0 + v1, or (v0 + v1) + (v2 + v3)
+        if (pt) sub[g][t] = v;
+        __syncthreads();
+        if (g != 0) return;
+        if (pt) {
+            double a[G];
+#pragma unroll
+            for (int i = 0; i < G; ++i) a[i] = sub[i][t];
+#pragma unroll
+            for (int st = 1; st < G; st <<= 1)
+#pragma unroll
+                for (int i = 0; i < G; i += 2 * st) a[i] = a[i] + a[i + st];
+            v = a[0];
+        }
+    }
+    if (pt) fvals[(r0 + prr) * TP_NPTS_GM3 + pp] = tp_map3_end(P, kind, v, D.th[0], D.th[1], D.th[2], flag);
+}
+
 // Region r's flag: bit 0 = non-finite rule estimate.  Region 0 also carries the point
 // kernel's flag word (non-finite ell: bit 1, non-finite B_kappa: bit 2), so one copy of
 // the region outputs returns every error of the round.
@@ -174,6 +321,18 @@ static void launch_region_points(hipStream_t st, int lpp, const TpNodeT<T> *node
         hipLaunchKernelGGL((k_region_points<T, 8>), dim3((unsigned)groups), dim3(512), 0, st, nodes, eh, nreg, d_reg,
                            d_desc, d_fvals, mode, d_flag);
         break;
+    case 16:
+        hipLaunchKernelGGL((k_region_cached<T, 1>), dim3((unsigned)((nreg + kCacheReg - 1) / kCacheReg)), dim3(256),
+                           0, st, nodes, eh, nreg, d_reg, d_desc, d_fvals, mode, d_flag);
+        break;
+    case 32:
+        hipLaunchKernelGGL((k_region_cached<T, 2>), dim3((unsigned)((nreg + kCacheReg - 1) / kCacheReg)), dim3(512),
+                           0, st, nodes, eh, nreg, d_reg, d_desc, d_fvals, mode, d_flag);
+        break;
+    case 64:
+        hipLaunchKernelGGL((k_region_cached<T, 4>), dim3((unsigned)((nreg + kCacheReg - 1) / kCacheReg)), dim3(1024),
+                           0, st, nodes, eh, nreg, d_reg, d_desc, d_fvals, mode, d_flag);
+        break;
     default:
         hipLaunchKernelGGL((k_region_points<T, 1>), dim3((unsigned)((groups + 3) / 4)), dim3(256), 0, st, nodes, eh,
                            nreg, d_reg, d_desc, d_fvals, mode, d_flag);
@@ -230,6 +389,14 @@ static void region_points_shape(int lpp, size_t nreg, void **func, dim3 *grid, d
         *func = (void *)k_region_points<T, 8>;
         *grid = dim3((unsigned)groups);
         *block = dim3(512);
+        break;
+    case 16:
+    case 32:
+    case 64:
+        *func = lpp == 16 ? (void *)k_region_cached<T, 1> : lpp == 32 ? (void *)k_region_cached<T, 2>
+                                                                       : (void *)k_region_cached<T, 4>;
+        *grid = dim3((unsigned)((nreg + kCacheReg - 1) / kCacheReg));
+        *block = dim3(16 * lpp);
         break;
     default:
         *func = (void *)k_region_points<T, 1>;
@@ -353,10 +520,10 @@ int tp_region_lanes(tp_ctx *ctx, const tp_cosmo *c, int mode, size_t np)
                  hipMemcpy(d_desc, &hd, sizeof(hd), hipMemcpyHostToDevice) == hipSuccess &&
                  hipMemset(d_fl, 0, sizeof(int)) == hipSuccess;
         }
-        const int shapes[4] = {1, 2, 4, 8};
+        const int shapes[7] = {1, 2, 4, 8, 16, 32, 64};
         for (int b = 0; ok && b < 4; ++b) {
             float best = 0;
-            for (int s = 0; ok && s < 4; ++s) {
+            for (int s = 0; ok && s < 7; ++s) {
                 float ms = 0;
                 for (int rep = 0; rep < 3; ++rep) {  // first launch warms up
                     ok = hipEventRecord(e0, ctx->stream) == hipSuccess;
@@ -390,8 +557,10 @@ int tp_region_lanes(tp_ctx *ctx, const tp_cosmo *c, int mode, size_t np)
 
 extern "C" int tp_set_region_lanes(tp_ctx *ctx, int lanes)
 {
-    if (!ctx || !(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8))
-        return tp_set_error(TP_EINVAL, "tp_set_region_lanes: lanes must be 0 (tuned), 1, 2, 4 or 8");
+    if (!ctx || !(lanes == 0 || lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32 ||
+                  lanes == 64))
+        return tp_set_error(TP_EINVAL,
+                            "tp_set_region_lanes: lanes must be 0 (tuned), 1, 2, 4, 8, or 16, 32, 64 (region-cached)");
     ctx->region_lpp_force = lanes;
     return 0;
 }

@@ -427,6 +427,7 @@ int power_batch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &
 {
     auto t0 = std::chrono::steady_clock::now();
     ctx->stats = tp_batch_stats{};
+    const int nt = tp_host_threads(ctx, kTpHostThreadsSync);
     const size_t nint = desc.size();
     std::vector<tpc::AdaptiveIntegralT<D, 1>> I(nint);
     for (size_t k = 0; k < nint; ++k)
@@ -445,7 +446,7 @@ int power_batch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &
         pts.resize(npts * D);
         vals.resize(npts);
         pdesc.resize(npts);
-#pragma omp parallel for schedule(dynamic, 16) if (na > 64)
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt) if (na > 64)
         for (size_t a = 0; a < na; ++a) {
             auto &A = I[active[a]];
             const auto &pend = A.pending();
@@ -460,7 +461,7 @@ int power_batch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &
         ctx->stats.rounds += 1;
         ctx->stats.points += (long long)npts;
         std::vector<char> fin(na, 0);
-#pragma omp parallel for schedule(dynamic, 16) if (na > 64)
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt) if (na > 64)
         for (size_t a = 0; a < na; ++a) {
             auto &A = I[active[a]];
             const auto &pend = A.pending();
@@ -515,6 +516,7 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
     using Out = typename RegionIO<D>::Out;
     auto t0 = std::chrono::steady_clock::now();
     ctx->stats = tp_batch_stats{};
+    const int nt = tp_host_threads(ctx, kTpHostThreadsSync);
     const size_t nint = desc.size();
     std::vector<tpc::AdaptiveIntegralT<D, 1>> I(nint);
     for (size_t k = 0; k < nint; ++k)
@@ -537,7 +539,7 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
             (rc = ctx->h_regions.ensure(br)) || (rc = ctx->h_rout.ensure(bo)))
             return rc;
         In *hr = (In *)ctx->h_regions.p;
-#pragma omp parallel for schedule(dynamic, 64) if (na > 256)
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt) if (na > 256)
         for (size_t a = 0; a < na; ++a) {
             auto &A = I[active[a]];
             const auto &pend = A.pending();
@@ -571,7 +573,7 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
         const Out *ho = (const Out *)ctx->h_rout.p;
         std::vector<char> fin(na, 0);
         int bad = 0;
-#pragma omp parallel for schedule(dynamic, 64) if (na > 256) reduction(| : bad)
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt) if (na > 256) reduction(| : bad)
         for (size_t a = 0; a < na; ++a) {
             auto &A = I[active[a]];
             const auto &pend = A.pending();

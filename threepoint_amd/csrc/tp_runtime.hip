@@ -411,6 +411,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         size_t nreg = 0;
         long long rounds = 0;
         bool inflight = false;
+        double tl0 = 0, tl1 = 0;  // host time of the round's launch [ms since the call began] (round log)
     };
     Group G[TP_NLANES];
     for (size_t t = 0; t < ntriples; ++t) G[nl == 1 ? 0 : (int)(t % nl)].active.push_back(t);
@@ -422,16 +423,18 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         // the flag word is only ever raised, and a raised flag ends the batch: clear it once per call
         TP_HIP(hipMemsetAsync(ctx->lane[g].d_flag.p, 0, sizeof(int), ctx->stream));
     }
-    const int nthreads = std::max(1, std::min(omp_get_max_threads(), 16));
+    const int nthreads = tp_host_threads(ctx, kTpHostThreadsOverlap);
     long long total_regions = 0;
     using clk = std::chrono::steady_clock;
     auto ms_since = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
+    ctx->round_log.clear();
 
     // gathers the group's pending regions and queues their evaluation
     auto launch = [&](int g) -> int {
         Group &Gr = G[g];
         TpLane &L = ctx->lane[g];
         auto th0 = clk::now();
+        Gr.tl0 = ms_since(t0);
         if (Gr.rounds >= kMaxRounds)
             return tp_set_error(TP_ECAPACITY, "tp_map3_batch: round limit reached (integrals not converging)");
         const size_t na = Gr.active.size();
@@ -468,6 +471,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
             }
         }
         ctx->stats.host_ms += ms_since(th0);
+        Gr.tl1 = ms_since(t0);
         if (ctx->graph_rounds && mode != TP_MODE_SURROGATE) {
             if ((e = tp_graph_round(ctx, c, L, nreg, mode))) return e;
             Gr.inflight = true;
@@ -519,7 +523,9 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         TpLane &L = ctx->lane[g];
         auto tw = clk::now();
         TP_HIP(hipEventSynchronize(L.done));
-        ctx->stats.wait_ms += ms_since(tw);
+        const double waited = ms_since(tw);
+        ctx->stats.wait_ms += waited;
+        const double tc0 = ms_since(t0);
         Gr.inflight = false;
         float ms = 0, msp = 0;
         TP_HIP(hipEventElapsedTime(&ms, L.ev0, L.ev1));
@@ -561,6 +567,9 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
             if (!finished[a]) next.push_back(act[a]);
         Gr.active.swap(next);
         ctx->stats.host_ms += ms_since(th0);
+        if (ctx->round_log_on)
+            ctx->round_log.insert(ctx->round_log.end(),
+                                  {(double)g, (double)Gr.nreg, Gr.tl0, Gr.tl1, tc0, ms_since(t0), waited});
         return 0;
     };
 
@@ -642,6 +651,36 @@ extern "C" int tp_set_batch_groups(tp_ctx *ctx, int groups)
     if (!ctx || groups < 1 || groups > TP_NLANES)
         return tp_set_error(TP_EINVAL, "tp_set_batch_groups: groups must be 1.." + std::to_string(TP_NLANES));
     ctx->nlanes = groups;
+    return 0;
+}
+
+int tp_host_threads(const tp_ctx *ctx, int dflt)
+{
+    const int want = ctx && ctx->host_threads > 0 ? ctx->host_threads : dflt;
+    return std::max(1, std::min(omp_get_max_threads(), want));
+}
+
+extern "C" int tp_set_host_threads(tp_ctx *ctx, int n)
+{
+    if (!ctx || n < 0 || n > 256) return tp_set_error(TP_EINVAL, "tp_set_host_threads: n must be 0 (default) .. 256");
+    ctx->host_threads = n;
+    return 0;
+}
+
+extern "C" int tp_set_round_log(tp_ctx *ctx, int on)
+{
+    if (!ctx) return tp_set_error(TP_EINVAL, "tp_set_round_log: NULL context");
+    ctx->round_log_on = on != 0;
+    if (!on) ctx->round_log.clear();
+    return 0;
+}
+
+extern "C" int tp_round_log(const tp_ctx *ctx, double *out, size_t cap, size_t *n)
+{
+    if (!ctx || !n || (cap && !out)) return tp_set_error(TP_EINVAL, "tp_round_log: NULL argument");
+    *n = ctx->round_log.size();
+    if (cap && !ctx->round_log.empty())
+        std::memcpy(out, ctx->round_log.data(), sizeof(double) * std::min(cap, ctx->round_log.size()));
     return 0;
 }
 
