@@ -33,10 +33,11 @@ sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
 # HBM bytes per integrand point of the region kernels: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the
-# C1 batched paths (the launch-shape tuner's probe launches excluded), each counter scaled by its reading
-# for k_copy16 of 256 MiB (FETCH_SIZE x2.000, WRITE_SIZE x1.000).  Algorithmic: 64/33 B of region in +
-# 8 B of integrand value out.
-HBM_BYTES_PER_POINT = {"double": 14.00, "float": 13.23}
+# C1 batched paths (the launch-shape tuner's probe launches excluded; tuned shapes: 8 waves per point for
+# small rounds, the region-cached kernel for large ones), each counter scaled by its reading for k_copy16
+# of 256 MiB (FETCH_SIZE x2.000, WRITE_SIZE x1.000).  Algorithmic: 64/33 B of region in + 8 B of
+# integrand value out.
+HBM_BYTES_PER_POINT = {"double": 13.29, "float": 12.12}
 FLOPS_PER_ELEM = 20.0
 
 
