@@ -39,6 +39,22 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
 # integrand value out.
 HBM_BYTES_PER_POINT = {"double": 13.29, "float": 12.12}
 FLOPS_PER_ELEM = 20.0
+CENSUS_FILE = os.path.join("tests", "golden", "census.json")
+
+
+def census_fixture(tp):
+    """The committed op census (tests/golden/census.json, written by tests/golden/make_golden.py from the
+    library's op-counting instantiation of its device functions); the library's live census must agree."""
+    fx = json.load(open(os.path.join(ROOT, CENSUS_FILE)))
+    live = {"map3_point": tp.census_map3_point(), "power_point": tp.census_power_point(),
+            "gamma_term": tp.census_gamma_term()}
+    for k, d in live.items():
+        for f, v in d.items():
+            if float(v) != fx[k][f]:
+                raise RuntimeError("op census of %s.%s (%g) differs from %s (%g)" % (k, f, v, CENSUS_FILE, fx[k][f]))
+    if fx["flops_per_elementary_call"] != FLOPS_PER_ELEM:
+        raise RuntimeError("%s uses another flops-per-elementary-call convention" % CENSUS_FILE)
+    return fx
 
 
 def cpu_model():
@@ -161,6 +177,7 @@ def gamma_bench(tp, W, ctx, args):
     prec_k = 5000
     sel, tri = c2_sample(W, args.c2_triangles)
     x = tri[sel, :3]
+    fx = census_fixture(tp)
     fe_point = tp.flop_equivalents_per_gamma_point(prec_k, FLOPS_PER_ELEM)
     results, values = {}, {}
     for p in [q for q in args.paths.split(",") if q in ("plain_fp64", "batched_fp64", "batched_fp32")]:
@@ -219,7 +236,7 @@ def gamma_bench(tp, W, ctx, args):
         "roofline": {"bound": "valu", "kernel": "k_gamma_points", "achieved": dom["achieved_flops"] / 1e12,
                      "peak": dom["peak"] / 1e12, "unit": "TFLOP/s", "frac": dom["frac"], "traffic": None,
                      "work_per_point_flop_eq": fe_point, "flops_per_elementary_call": FLOPS_PER_ELEM,
-                     "census": tp.census_gamma_term()},
+                     "census": fx["gamma_term"], "census_file": CENSUS_FILE},
     }
     if not args.no_cpu:
         print("bench: cpu baseline ...", file=sys.stderr, flush=True)
@@ -261,8 +278,9 @@ def map2_bench(tp, W, ctx, args):
     radii = W.to_rad(W.map3_c1_radii())
     theta = np.tile(radii, 512)
     icosmo = np.repeat(np.arange(512, dtype=np.int32), radii.size)
-    c = tp.census_power_point()
-    fe_point = c["map2_flops"] + FLOPS_PER_ELEM * c["map2_elem"]
+    fx = census_fixture(tp)
+    c = fx["power_point"]
+    fe_point = fx["flop_eq"]["map2_point"]
     results, values = {}, {}
     for p in [q for q in args.paths.split(",") if q in ("plain_fp64", "batched_fp64")]:
         print("bench: map2 path %s ..." % p, file=sys.stderr, flush=True)
@@ -315,7 +333,7 @@ def map2_bench(tp, W, ctx, args):
                      "peak": dom["peak"] / 1e12, "unit": "TFLOP/s", "frac": dom["frac"], "traffic": None,
                      "algorithmic_bytes_per_point": 16 + 4 + 8,
                      "work_per_point_flop_eq": fe_point, "flops_per_elementary_call": FLOPS_PER_ELEM,
-                     "census": c},
+                     "census": c, "census_file": CENSUS_FILE},
     }
     if not args.no_cpu:
         print("bench: cpu baseline ...", file=sys.stderr, flush=True)
@@ -370,14 +388,12 @@ def cov_bench(tp, W, ctx, args):
     agg = {"ms_per_step": 1e3 * dt / args.steps, "pairs_per_s": pairs.shape[0] * args.steps / dt,
            "points": pts, "regions": nreg, "integrand_ms": st.get("integrand_ms", 0.0),
            "launches": st.get("launches", 0), "rounds": st.get("rounds", 0)}
-    c = tp.census_power_point()
+    fx = census_fixture(tp)
     # a T1 region (k_t1_regions): 47 Pell (the 7 distinct l1, 7 distinct l2 and 33 l3 of its 33 points) of
     # 96 z-nodes, each node the k-dependent part of the Limber P (the z-only Halofit coefficients are kept
     # per node), plus 3 flops per node pair; per point l3 (7 flops + cos + sqrt), 6 uHat (4 flops + exp
     # each) and 8 products; the 3-D Genz-Malik rule (~90 flops)
-    fe_node = c["node_flops"] + FLOPS_PER_ELEM * c["node_elem"]
-    fe_region = (47 * (96 * fe_node + 48 * 3 + 2) + 33 * ((7 + 2 * FLOPS_PER_ELEM) + 6 * (4 + FLOPS_PER_ELEM) + 8)
-                 + 90)
+    fe_region = fx["flop_eq"]["t1_region"]
     ach = fe_region * nreg / (agg["integrand_ms"] * 1e-3) if agg["integrand_ms"] > 0 else 0.0
     results["batched_fp64"] = agg
     line = {
@@ -393,6 +409,7 @@ def cov_bench(tp, W, ctx, args):
         "roofline": {"bound": "valu", "kernel": "k_t1_regions", "achieved": ach / 1e12,
                      "peak": PEAK["fp64"] / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK["fp64"], "traffic": None,
                      "work_per_region_flop_eq": fe_region, "flops_per_elementary_call": FLOPS_PER_ELEM,
+                     "census_file": CENSUS_FILE,
                      "node_evaluations_per_region": 47 * 96},
     }
     if not args.no_cpu:
@@ -488,8 +505,9 @@ def main():
         accuracy_study(tabs, ctx, thetas, icosmo, paths, args.reltol)
         return
 
-    census = tp.census_map3_point()
-    fe_point = tp.flop_equivalents_per_point(FLOPS_PER_ELEM)
+    fx = census_fixture(tp)
+    census = fx["map3_point"]
+    fe_point = fx["flop_eq"]["map3_point"]
     results, values = {}, {}
     for p in paths:
         print("bench: path %s ..." % p, file=sys.stderr, flush=True)
@@ -536,8 +554,7 @@ def main():
     pts_per_launch = hr["points"] / max(1, hr["launches"])
     # the region-cached shape evaluates the P_E terms (3 per node and point) once per distinct multipole of
     # a region: 47 per 33 points instead of 99, so it executes less than the algorithmic work per point
-    fe_pse = census["pse_flops"] + FLOPS_PER_ELEM * census["pse_elem"]
-    fe_exec_cached = fe_point - census["nodes"] * fe_pse * (99 - 47) / 33
+    fe_exec_cached = fx["flop_eq"]["map3_point_region_cached_executed"]
     roof = {"bound": "valu", "kernel": kname,
             "achieved": dom["achieved_flops"] / 1e12, "peak": dom["peak"] / 1e12, "unit": "TFLOP/s",
             "frac": dom["frac"], "traffic": HBM_BYTES_PER_POINT[prec] * pts_per_launch,
@@ -546,7 +563,7 @@ def main():
             "work_per_point_flop_eq": fe_point, "flops_per_elementary_call": FLOPS_PER_ELEM,
             "executed_work_per_point_flop_eq_region_cached": fe_exec_cached,
             "executed_frac_if_all_region_cached": dom["frac"] * fe_exec_cached / fe_point,
-            "census": census}
+            "census": census, "census_file": CENSUS_FILE}
     line = {
         "metric": "Map3 triples/s on 1 GPU: plain FP64 vs surrogate form at <=1e-3 rel err",
         "value": hr["triples_per_s"], "unit": "triples/s", "n_gpus": 1,

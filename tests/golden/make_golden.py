@@ -5,7 +5,12 @@ GPU parity tests do not have to rerun minutes of CPU cubature on the GPU host:
   map3_c0_fp64.json   <Map^3> of the 20 C0 triples (rel tol 1e-4) + points used
   tables_fid.json     f_K, g, D1, r_sigma, n_eff, ncur, norm of the fiducial cosmology
   bkappa_c1_2k.npz    B_kappa at the first 2000 C1 dump points
-Usage: python tests/golden/make_golden.py [map3|tables|bkappa|all]
+  census.json         the op census of the integrand points (flops and elementary calls per
+                      z-node, per point, per P_E term, per Ogata term, per <Map^2> / Limber /
+                      node-record point), counted by instantiating the kernels' device functions
+                      with an op-counting scalar on the host, and the flop-equivalents per unit at
+                      20 flops per elementary call; bench.py's roofline reads it
+Usage: python tests/golden/make_golden.py [map3|tables|bkappa|census|all]
 """
 import json
 import os
@@ -33,6 +38,8 @@ def main(what):
         ell = ell[:2000]
         b = o.bkappa(ell)
         np.savez(os.path.join(HERE, "bkappa_c1_2k.npz"), ell=ell, bkappa=b)
+    if what in ("census", "all"):
+        json.dump(census(), open(os.path.join(HERE, "census.json"), "w"), indent=1)
     if what in ("map3", "all"):
         tr = W.triples(W.map3_c0_radii())
         rows = []
@@ -42,6 +49,30 @@ def main(what):
             rows.append({"theta_arcmin": th.tolist(), "map3": v, "err": e, "neval": n})
             print(th, v, e, n, "%.1fs" % (time.time() - t0), flush=True)
         json.dump({"reltol": 1e-4, "rows": rows}, open(os.path.join(HERE, "map3_c0_fp64.json"), "w"), indent=1)
+
+
+def census(flops_per_elem=20.0):
+    """The op census of the library's integrand device functions (tp_census_*) and the work per unit."""
+    import threepoint_amd as tp
+    m3 = {k: float(v) for k, v in tp.census_map3_point().items()}
+    pw = {k: float(v) for k, v in tp.census_power_point().items()}
+    gt = {k: float(v) for k, v in tp.census_gamma_term().items()}
+    fe_node = m3["node_flops"] + flops_per_elem * m3["node_elem"]
+    fe_pse = m3["pse_flops"] + flops_per_elem * m3["pse_elem"]
+    fe_point = m3["nodes"] * fe_node + m3["point_flops"] + flops_per_elem * m3["point_elem"]
+    fe_pnode = pw["node_flops"] + flops_per_elem * pw["node_elem"]
+    return {
+        "flops_per_elementary_call": flops_per_elem,
+        "map3_point": m3, "power_point": pw, "gamma_term": gt,
+        "flop_eq": {
+            "map3_point": fe_point,
+            "map3_point_region_cached_executed": fe_point - m3["nodes"] * fe_pse * (99 - 47) / 33,
+            "map2_point": pw["map2_flops"] + flops_per_elem * pw["map2_elem"],
+            "t1_region": 47 * (96 * fe_pnode + 48 * 3 + 2) + 33 * ((7 + 2 * flops_per_elem) +
+                                                                   6 * (4 + flops_per_elem) + 8) + 90,
+            "gamma_term": gt["term_flops"] + flops_per_elem * gt["term_elem"],
+        },
+    }
 
 
 if __name__ == "__main__":

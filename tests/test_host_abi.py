@@ -40,6 +40,23 @@ def test_census_is_positive():
     assert c["nodes"] == 96 and c["node_flops"] > 100 and c["node_elem"] > 10
 
 
+def test_census_matches_committed_fixture():
+    """bench.py's roofline reads tests/golden/census.json: the library's op census must still give it
+    (a change to a device function that changes the work per point must regenerate the fixture)."""
+    import json
+    import os
+    sys_path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_golden", os.path.join(sys_path, "make_golden.py"))
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    fx = json.load(open(os.path.join(sys_path, "census.json")))
+    assert mg.census() == fx
+    # the split the region-cached kernel relies on: a node is three P_E terms plus the rest
+    c = fx["map3_point"]
+    assert 3 * c["pse_flops"] < c["node_flops"] and 3 * c["pse_elem"] < c["node_elem"]
+
+
 def _integrand(fdim, dim):
     def f(ndim, npt, x, fdata, fd, fval):
         for i in range(npt):

@@ -400,14 +400,18 @@ bool AdaptiveIntegralT<MD, MF>::advance()
 template <unsigned MD, unsigned MF>
 void AdaptiveIntegralT<MD, MF>::result(double *val, double *err) const
 {
-    // final re-sum in heap order, classed_version/hcubature.cpp:1000-1008
+    // final re-sum in heap order, classed_version/hcubature.cpp:1000-1008; the leaf corrections are
+    // summed in the same pass (each sum keeps its own order, so the bits equal separate passes)
     for (unsigned j = 0; j < fdim_; ++j) val[j] = err[j] = 0;
-    for (const HeapEnt &h : heap_)
+    double c = 0;
+    for (const HeapEnt &h : heap_) {
+        const RegionType &r = pool_[h.idx];
         for (unsigned j = 0; j < fdim_; ++j) {
-            val[j] += pool_[h.idx].val[j];
-            err[j] += pool_[h.idx].err[j];
+            val[j] += r.val[j];
+            err[j] += r.err[j];
         }
-    double c = correction();
+        c += r.corr;
+    }
     if (c != 0.0) val[0] += c;
 }
 

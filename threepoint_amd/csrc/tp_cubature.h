@@ -66,6 +66,12 @@ class AdaptiveIntegralT {
     long long rounds() const { return rounds_; }
     bool converged_flag() const { return converged_; }
     size_t heap_size() const { return heap_.size(); }
+    // host bytes held by the region pool, heap and pending list (capacity, kept across init())
+    size_t bytes_reserved() const
+    {
+        return pool_.capacity() * sizeof(RegionType) + heap_.capacity() * sizeof(HeapEnt) +
+               pending_.capacity() * sizeof(uint32_t);
+    }
 
   private:
     void heap_push(uint32_t idx);

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "threepoint_amd.h"
+#include "tp_cubature.h"
 
 #define TP_NNODES 96      // Gauss-Legendre z-nodes of the Limber integral
 #define TP_NZ_MAX 1024    // max redshift bins of the tables
@@ -146,6 +147,9 @@ struct TpLane {
     DevBuf d_regions, d_rout, d_fvals, d_flag;
     HostPinned h_regions, h_rout, h_flag;
     hipEvent_t ev0 = nullptr, evm = nullptr, ev1 = nullptr, done = nullptr;
+    // the group's own stream (default priority): the groups' rounds run concurrently on the device,
+    // so a small round of one group does not wait behind the other group's kernel
+    hipStream_t stream = nullptr;
     // the round as one graph (tp_graph_round): copy in, ev0, points kernel, evm, rule kernel, ev1,
     // copy out, done; instantiated once, its node parameters updated every round
     hipGraph_t graph = nullptr;
@@ -179,6 +183,10 @@ struct tp_ctx {
     int nlanes = 2;             // groups in flight in tp_map3_batch (tp_set_batch_groups)
     long long zero_copy_regions = 0;  // rounds up to this many regions skip the copies (pinned buffers)
     int host_threads = 0;             // OpenMP threads of the host-side cubature work (0: default)
+    // tp_map3_batch's integrators of the last call, reused by the next (their region pools keep
+    // their capacity, so a call of the same size allocates and page-faults nothing); dropped when
+    // they hold more than kTpKeepPoolBytes
+    std::vector<tpc::AdaptiveIntegral31> map3_keep;
     bool round_log_on = false;        // tp_set_round_log: per-round trace of tp_map3_batch
     std::vector<double> round_log;    // 7 doubles per round (tp_round_log)
     TpLane lane[TP_NLANES];
@@ -192,6 +200,7 @@ struct tp_ctx {
 // device idle).  The synchronous batches (host work on the critical path) default to
 // kTpHostThreadsSync.
 constexpr int kTpHostThreadsOverlap = 8, kTpHostThreadsSync = 16;
+constexpr size_t kTpKeepPoolBytes = size_t(512) << 20;
 int tp_host_threads(const tp_ctx *ctx, int dflt);
 
 // Error helpers (thread-local last error string).
@@ -216,9 +225,9 @@ int tp_launch_bkappa_points(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t n
                             double *d_out, int mode, int *d_flag);
 int tp_launch_regions(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegion *d_reg,
                       const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int mode,
-                      int *d_flag, hipEvent_t evm);
+                      int *d_flag, hipEvent_t evm, hipStream_t st);
 // One exact-mode round of lane L (nreg regions already in L.h_regions) as a graph replay: the same
 // work and events as the copy + tp_launch_regions + copy sequence, queued on ctx->stream.
-int tp_graph_round(tp_ctx *ctx, const tp_cosmo *c, TpLane &L, size_t nreg, int mode);
+int tp_graph_round(tp_ctx *ctx, const tp_cosmo *c, TpLane &L, size_t nreg, int mode, hipStream_t st);
 int tp_launch_gamma_points(tp_ctx *ctx, const tp_cosmo *c, int icosmo, int which, double x1, double x2,
                            double x3, size_t npts, const double *d_x, double *d_f, int *d_flag);

@@ -346,7 +346,7 @@ int tp_region_lanes(tp_ctx *ctx, const tp_cosmo *c, int mode, size_t np);  // tp
 
 int tp_launch_regions(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegion *d_reg,
                       const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int mode, int *d_flag,
-                      hipEvent_t evm)
+                      hipEvent_t evm, hipStream_t st)
 {
     if (nreg == 0) return 0;
     const int B = 256;
@@ -354,14 +354,12 @@ int tp_launch_regions(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegio
     const int lpp = tp_region_lanes(ctx, c, mode, np);
     if (lpp < 0) return TP_EHIP;
     if (mode == TP_MODE_FP32)
-        launch_region_points<float>(ctx->stream, lpp, c->d_node32, c->d_eh32, nreg, d_reg, d_desc, d_fvals, mode,
-                                    d_flag);
+        launch_region_points<float>(st, lpp, c->d_node32, c->d_eh32, nreg, d_reg, d_desc, d_fvals, mode, d_flag);
     else
-        launch_region_points<double>(ctx->stream, lpp, c->d_node64, c->d_eh64, nreg, d_reg, d_desc, d_fvals, mode,
-                                     d_flag);
+        launch_region_points<double>(st, lpp, c->d_node64, c->d_eh64, nreg, d_reg, d_desc, d_fvals, mode, d_flag);
     TP_HIP(hipGetLastError());
-    TP_HIP(hipEventRecord(evm, ctx->stream));
-    hipLaunchKernelGGL(k_region_rule, dim3(grid_for(nreg, B)), dim3(B), 0, ctx->stream, nreg, d_reg, d_fvals, d_out,
+    TP_HIP(hipEventRecord(evm, st));
+    hipLaunchKernelGGL(k_region_rule, dim3(grid_for(nreg, B)), dim3(B), 0, st, nreg, d_reg, d_fvals, d_out,
                        d_flag);
     TP_HIP(hipGetLastError());
     return 0;
@@ -405,7 +403,7 @@ static void region_points_shape(int lpp, size_t nreg, void **func, dim3 *grid, d
     }
 }
 
-int tp_graph_round(tp_ctx *ctx, const tp_cosmo *c, TpLane &L, size_t nreg, int mode)
+int tp_graph_round(tp_ctx *ctx, const tp_cosmo *c, TpLane &L, size_t nreg, int mode, hipStream_t st)
 {
     if (nreg == 0) return 0;
     const int lpp = tp_region_lanes(ctx, c, mode, nreg * TP_NPTS_GM3);
@@ -467,7 +465,7 @@ int tp_graph_round(tp_ctx *ctx, const tp_cosmo *c, TpLane &L, size_t nreg, int m
         fresh = true;
     }
     (void)fresh;
-    TP_HIP(hipGraphLaunch(L.exec, ctx->stream));
+    TP_HIP(hipGraphLaunch(L.exec, st));
     return 0;
 }
 

@@ -1379,7 +1379,8 @@ int tp_surrogate_round(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegi
     // rounds too small to carry enough checked points run exactly (they are cheap)
     if ((double)nreg * TP_NPTS_GM3 * ctx->sur_check_frac < 64.0) {
         *fell_back = 2;
-        return tp_launch_regions(ctx, c, nreg, d_reg, d_desc, d_fvals, d_out, TP_MODE_FP64, d_flag, ctx->evm);
+        return tp_launch_regions(ctx, c, nreg, d_reg, d_desc, d_fvals, d_out, TP_MODE_FP64, d_flag, ctx->evm,
+                                 ctx->stream);
     }
     const double frac = ctx->sur_check_frac;
     const unsigned long long key = ctx->sur_seed * 0x9E3779B97F4A7C15ULL + (unsigned long long)ctx->stats.rounds * 7919ULL;
@@ -1421,7 +1422,8 @@ int tp_surrogate_round(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegi
     if (relse > ctx->sur_tol) {
         // the surrogate misses the tolerance on this round's checked points: exact re-evaluation
         *fell_back = 1;
-        return tp_launch_regions(ctx, c, nreg, d_reg, d_desc, d_fvals, d_out, TP_MODE_FP64, d_flag, ctx->evm);
+        return tp_launch_regions(ctx, c, nreg, d_reg, d_desc, d_fvals, d_out, TP_MODE_FP64, d_flag, ctx->evm,
+                                 ctx->stream);
     }
     ctx->stats.surrogate_points += (long long)(nreg * TP_NPTS_GM3);
     return 0;

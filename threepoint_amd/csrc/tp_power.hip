@@ -475,6 +475,7 @@ int power_batch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &
             if (!fin[a]) next.push_back(active[a]);
         active.swap(next);
     }
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt) if (nint > 256)
     for (size_t k = 0; k < nint; ++k) {
         double v, e;
         I[k].result(&v, &e);
@@ -597,6 +598,7 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
             if (!fin[a]) next.push_back(active[a]);
         active.swap(next);
     }
+#pragma omp parallel for schedule(dynamic, 64) num_threads(nt) if (nint > 256)
     for (size_t k = 0; k < nint; ++k) {
         double v, e;
         I[k].result(&v, &e);

@@ -104,6 +104,7 @@ extern "C" int tp_ctx_create(int device, tp_ctx **out)
 int TpLane::create()
 {
     if (done) return 0;
+    TP_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (hipEvent_t *e : {&ev0, &evm, &ev1, &done}) TP_HIP(hipEventCreate(e));
     if (h_flag.ensure(64) || d_flag.ensure(256)) return TP_EHIP;
     return 0;
@@ -121,6 +122,8 @@ void TpLane::release()
             (void)hipEventDestroy(*e);
             *e = nullptr;
         }
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
 }
 
 void tp_surrogate_free(tp_ctx *ctx);  // tp_mlp.hip
@@ -385,7 +388,9 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     auto t0 = std::chrono::steady_clock::now();
     ctx->stats = tp_batch_stats{};
     std::vector<TpIntegralDesc> desc(ntriples);
-    std::vector<tpc::AdaptiveIntegral31> I(ntriples);
+    std::vector<tpc::AdaptiveIntegral31> I;
+    I.swap(ctx->map3_keep);  // the last call's integrators (their pools keep capacity), if kept
+    if (I.size() < ntriples) I.resize(ntriples);
     for (size_t t = 0; t < ntriples; ++t) {
         const double *th = theta_rad + 3 * t;
         if (icosmo[t] < 0 || icosmo[t] >= c->ncosmo) return tp_set_error(TP_EINVAL, "tp_map3_batch: bad icosmo");
@@ -418,10 +423,16 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     // launch-shape tuning (first batched call of this precision) before any round is timed
     if (mode == TP_MODE_FP64 || mode == TP_MODE_FP32 || mode == TP_MODE_SURROGATE)
         if (tp_region_lanes(ctx, c, mode == TP_MODE_FP32 ? TP_MODE_FP32 : TP_MODE_FP64, 0) < 0) return TP_EHIP;
+    // each group's rounds go to its own stream (one group: the context's stream), ordered after the
+    // descriptor upload above
+    hipStream_t gst[TP_NLANES];
+    if (nl > 1) TP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     for (int g = 0; g < nl; ++g) {
         if ((rc = ctx->lane[g].create())) return rc;
+        gst[g] = nl == 1 ? ctx->stream : ctx->lane[g].stream;
+        if (nl > 1) TP_HIP(hipStreamWaitEvent(gst[g], ctx->ev1, 0));
         // the flag word is only ever raised, and a raised flag ends the batch: clear it once per call
-        TP_HIP(hipMemsetAsync(ctx->lane[g].d_flag.p, 0, sizeof(int), ctx->stream));
+        TP_HIP(hipMemsetAsync(ctx->lane[g].d_flag.p, 0, sizeof(int), gst[g]));
     }
     const int nthreads = tp_host_threads(ctx, kTpHostThreadsOverlap);
     long long total_regions = 0;
@@ -473,7 +484,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         ctx->stats.host_ms += ms_since(th0);
         Gr.tl1 = ms_since(t0);
         if (ctx->graph_rounds && mode != TP_MODE_SURROGATE) {
-            if ((e = tp_graph_round(ctx, c, L, nreg, mode))) return e;
+            if ((e = tp_graph_round(ctx, c, L, nreg, mode, gst[g]))) return e;
             Gr.inflight = true;
             Gr.rounds += 1;
             ctx->stats.launches += 1;
@@ -481,13 +492,13 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
             ctx->stats.points += (long long)nreg * TP_NPTS_GM3;
             return 0;
         }
-        if (mode == TP_MODE_SURROGATE) TP_HIP(hipMemsetAsync(L.d_flag.p, 0, sizeof(int), ctx->stream));
+        if (mode == TP_MODE_SURROGATE) TP_HIP(hipMemsetAsync(L.d_flag.p, 0, sizeof(int), gst[g]));
         // a small exact round reads its regions from, and writes its results to, the pinned host
         // buffers directly: no copy in or out (per-round latency is what the tail rounds cost)
         const bool zc = mode != TP_MODE_SURROGATE && nreg <= (size_t)ctx->zero_copy_regions;
         if (!zc)
-            TP_HIP(hipMemcpyAsync(L.d_regions.p, hr, sizeof(TpRegion) * nreg, hipMemcpyHostToDevice, ctx->stream));
-        TP_HIP(hipEventRecord(L.ev0, ctx->stream));
+            TP_HIP(hipMemcpyAsync(L.d_regions.p, hr, sizeof(TpRegion) * nreg, hipMemcpyHostToDevice, gst[g]));
+        TP_HIP(hipEventRecord(L.ev0, gst[g]));
         if (mode == TP_MODE_SURROGATE) {
             int fb = 0;
             e = tp_surrogate_round(ctx, c, nreg, (const TpRegion *)L.d_regions.p, (const TpIntegralDesc *)ctx->d_desc.p,
@@ -498,16 +509,16 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
             const TpRegion *rin = zc ? (const TpRegion *)L.h_regions.dp : (const TpRegion *)L.d_regions.p;
             TpRegionOut *rout = zc ? (TpRegionOut *)L.h_rout.dp : (TpRegionOut *)L.d_rout.p;
             e = tp_launch_regions(ctx, c, nreg, rin, (const TpIntegralDesc *)ctx->d_desc.p, (double *)L.d_fvals.p,
-                                  rout, mode, (int *)L.d_flag.p, L.evm);
+                                  rout, mode, (int *)L.d_flag.p, L.evm, gst[g]);
             if (e) return e;
         }
-        TP_HIP(hipEventRecord(L.ev1, ctx->stream));
+        TP_HIP(hipEventRecord(L.ev1, gst[g]));
         if (!zc)
             TP_HIP(hipMemcpyAsync(L.h_rout.p, L.d_rout.p, sizeof(TpRegionOut) * nreg, hipMemcpyDeviceToHost,
-                                  ctx->stream));
+                                  gst[g]));
         if (mode == TP_MODE_SURROGATE)
-            TP_HIP(hipMemcpyAsync(L.h_flag.p, L.d_flag.p, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        TP_HIP(hipEventRecord(L.done, ctx->stream));
+            TP_HIP(hipMemcpyAsync(L.h_flag.p, L.d_flag.p, sizeof(int), hipMemcpyDeviceToHost, gst[g]));
+        TP_HIP(hipEventRecord(L.done, gst[g]));
         Gr.inflight = true;
         Gr.rounds += 1;
         ctx->stats.launches += 1;
@@ -576,6 +587,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     // on an error, let queued work drain before the buffers can be reused
     auto fail = [&](int e) {
         (void)hipStreamSynchronize(ctx->stream);
+        for (int g = 0; g < nl; ++g) (void)hipStreamSynchronize(gst[g]);
         return e;
     };
     for (int g = 0; g < nl; ++g)
@@ -595,7 +607,9 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         for (int g = 0; g < nl; ++g) r = std::max(r, G[g].rounds);
         ctx->stats.rounds = r;
     }
-    std::vector<size_t> redo;
+    // the final re-sums walk every integral's heap: spread them over the host threads
+    std::vector<char> redo_flag(ntriples, 0);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads) if (ntriples > 16)
     for (size_t t = 0; t < ntriples; ++t) {
         double r, e;
         I[t].result(&r, &e);
@@ -605,7 +619,15 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         // surrogate mode: a triple whose correction has a standard error above tol is redone exactly
         if (mode == TP_MODE_SURROGATE &&
             (!I[t].converged_flag() || !(std::sqrt(I[t].correction_var()) <= ctx->sur_tol * std::fabs(r))))
-            redo.push_back(t);
+            redo_flag[t] = 1;
+    }
+    std::vector<size_t> redo;
+    for (size_t t = 0; t < ntriples; ++t)
+        if (redo_flag[t]) redo.push_back(t);
+    {
+        size_t held = 0;
+        for (const auto &A : I) held += A.bytes_reserved();
+        if (held <= kTpKeepPoolBytes) ctx->map3_keep.swap(I);
     }
     if (!redo.empty()) {
         tp_batch_stats keep = ctx->stats;
