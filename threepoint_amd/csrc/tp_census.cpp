@@ -110,7 +110,8 @@ extern "C" int tp_census_map3_pse(double *out)
 }
 
 // out: [0] flops per Ogata term, [1] elementary calls per term (a Gamma cubature point has
-// 3 (prec_k - 1) terms), [2] flops and [3] elementary calls of the per-point shape
+// 3 (prec_k - 1) terms), [2] flops and [3] elementary calls of the per-point shape (including the
+// node-and-shape part of the one-halo term, formed once per point since all its terms share both)
 extern "C" int tp_census_gamma_term(double *out)
 {
     Cnt vals[sizeof(TpNodeT<Cnt>) / sizeof(Cnt)];
@@ -128,10 +129,12 @@ extern "C" int tp_census_gamma_term(double *out)
     TpPointPreT<Cnt> P;
     tp_point_pre<Cnt>(cps, sps, u3, &P);
     for (int j = 0; j < 3; ++j) (void)TpMath<Cnt>::lg(Cnt(2.0));
+    TpShapeNodeT<Cnt> S;
+    tp_shape_node<Cnt>(N, P.lr1, P.r2, &S);
     OpCounts shape = g_cnt;
     g_cnt = OpCounts();
     Cnt acc(0.0);
-    acc += tp_ogata_term<Cnt>(N, E, P, Cnt(30.0), Cnt(3.4), Cnt(0.1));
+    acc += tp_ogata_term<Cnt>(N, E, P, S, Cnt(30.0), Cnt(3.4), Cnt(0.1));
     OpCounts term = g_cnt;
     out[0] = (double)term.flops;
     out[1] = (double)term.trans;

@@ -151,19 +151,33 @@ __host__ __device__ __forceinline__ T tp_pse_node(const TpNodeT<T> &N, const TpE
     return (A * N.D1sq * pl * C + B * pq3) * TpMath<T>::rcp(B * C);  // eq. B6
 }
 
+// The parts of the one-halo term that depend on the node and the triangle shape only (eq. B7:
+// a_n from r1, alpha_n and beta_n from r2); a kernel whose terms share node and shape (the Ogata sums
+// of Gamma) forms them once.
 template <class T>
-__host__ __device__ __forceinline__ T tp_bispec_joint(const TpNodeT<T> &N, const T q[3], const T lq[3], const T f2t[3],
-                                                      T lr1, T r2, const T pse[3])
+struct TpShapeNodeT {
+    T an, alphan, betan;
+};
+
+template <class T>
+__host__ __device__ __forceinline__ void tp_shape_node(const TpNodeT<T> &N, T lr1, T r2, TpShapeNodeT<T> *S)
 {
-    T an = TpMath<T>::ex(N.lnA0 + N.m031ln10 * TpMath<T>::ex(N.c1 * lr1));   // eq. B7
+    S->an = TpMath<T>::ex(N.lnA0 + N.m031ln10 * TpMath<T>::ex(N.c1 * lr1));   // eq. B7
     T alphan = TpMath<T>::ex(N.a0ln + N.a1ln * r2 * r2);
     if (alphan > N.acap) alphan = N.acap;
-    T betan = TpMath<T>::ex(N.b0ln + N.b1ln * r2);
+    S->alphan = alphan;
+    S->betan = TpMath<T>::ex(N.b0ln + N.b1ln * r2);
+}
+
+template <class T>
+__host__ __device__ __forceinline__ T tp_bispec_joint_s(const TpNodeT<T> &N, const TpShapeNodeT<T> &S, const T q[3],
+                                                        const T lq[3], const T f2t[3], const T pse[3])
+{
     T bs1h = T(1), damp = T(1);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         const T qi = q[i], lqi = lq[i];
-        T oneh = an * TpMath<T>::ex(alphan * lqi) + N.bn * TpMath<T>::ex(betan * lqi);   // eq. B4
+        T oneh = S.an * TpMath<T>::ex(S.alphan * lqi) + N.bn * TpMath<T>::ex(S.betan * lqi);   // eq. B4
         T cq = N.cn * qi;
         bs1h *= cq * TpMath<T>::rcp(oneh * (cq + T(1)));
         damp *= T(1) + N.en * qi;
@@ -171,6 +185,15 @@ __host__ __device__ __forceinline__ T tp_bispec_joint(const TpNodeT<T> &N, const
     T bs3h = T(2) * ((f2t[0] + N.dn * q[2]) * pse[0] * pse[1] + (f2t[1] + N.dn * q[0]) * pse[1] * pse[2] +
                      (f2t[2] + N.dn * q[1]) * pse[2] * pse[0]);                            // eq. B5 with F2 (:614-627)
     return bs1h + bs3h * TpMath<T>::rcp(damp);
+}
+
+template <class T>
+__host__ __device__ __forceinline__ T tp_bispec_joint(const TpNodeT<T> &N, const T q[3], const T lq[3], const T f2t[3],
+                                                      T lr1, T r2, const T pse[3])
+{
+    TpShapeNodeT<T> S;
+    tp_shape_node<T>(N, lr1, r2, &S);
+    return tp_bispec_joint_s<T>(N, S, q, lq, f2t, pse);
 }
 
 template <class T>
@@ -225,14 +248,18 @@ __device__ __forceinline__ double tp_bkappa_sum_n(const TpNodeT<T> *__restrict__
 // hoisted: every term of a cubature point (z, phi, psi) has the sides
 //   l = (psi_k / A) (cos psi, sin psi, sqrt(1 + sin(2 psi) cos phi)),
 // i.e. one triangle shape P (unit scale) at the scale s = psi_k / A.  ln_s = ln psi_k - ln A.
+// S: tp_shape_node(N, P.lr1, P.r2), the same for every term of a point.
 template <class T>
 __host__ __device__ __forceinline__ T tp_ogata_term(const TpNodeT<T> &N, const TpEHT<T> &E, const TpPointPreT<T> &P,
-                                                     T s, T ln_s, T prodk)
+                                                     const TpShapeNodeT<T> &S, T s, T ln_s, T prodk)
 {
     const T sk = s * N.ks, lsk = ln_s + N.lnks;
     T q[3] = {P.l[0] * sk, P.l[1] * sk, P.l[2] * sk};
     T lq[3] = {P.ll[0] + lsk, P.ll[1] + lsk, P.ll[2] + lsk};
-    return N.wz * tp_bispec_node<T>(N, E, q, lq, P.f2t, P.lr1, P.r2) * prodk;
+    T pse[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) pse[i] = tp_pse_node<T>(N, E, q[i], lq[i]);
+    return N.wz * tp_bispec_joint_s<T>(N, S, q, lq, P.f2t, pse) * prodk;
 }
 
 // The 96-node Limber sum of the reference's bkappa (classed_version/bispectrum.cpp:238-286),

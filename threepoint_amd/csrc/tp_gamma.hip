@@ -181,6 +181,9 @@ __global__ void __launch_bounds__(kGammaBlock) k_gamma_points(const TpCosmoConst
             TpPointPre P;
             tp_point_pre<double>(cpsi, spsi, sqrt(u3s), &P);
             const double lninvA[3] = {-log(A[0]), -log(A[1]), -log(A[2])};
+            // the node (one z per point) and the shape are those of every term: a_n, alpha_n, beta_n once
+            TpShapeNodeT<double> S;
+            tp_shape_node<double>(sN, P.lr1, P.r2, &S);
             if (F32) {
                 const TpEH32 E32 = eh32[D.icosmo];
                 TpPointPreT<float> P32;
@@ -191,6 +194,8 @@ __global__ void __launch_bounds__(kGammaBlock) k_gamma_points(const TpCosmoConst
                 }
                 P32.lr1 = (float)P.lr1;
                 P32.r2 = (float)P.r2;
+                TpShapeNodeT<float> S32;
+                tp_shape_node<float>(sN32, P32.lr1, P32.r2, &S32);
                 const double lmax = fmax(P.l[0], fmax(P.l[1], P.l[2]));
                 const double lmin = fmin(P.l[0], fmin(P.l[1], P.l[2]));
                 // FP32 term for s in [smin32, smax32): every wavenumber of the term in [1e-3, 1e4], the
@@ -205,12 +210,13 @@ __global__ void __launch_bounds__(kGammaBlock) k_gamma_points(const TpCosmoConst
                         float t32 = 0.f;
                         const bool use32 = sj < smax32 && sj >= smin32;
                         if (use32)
-                            t32 = tp_ogata_term<float>(sN32, E32, P32, (float)sj, (float)(lpk + lninvA[j]), (float)wk);
+                            t32 = tp_ogata_term<float>(sN32, E32, P32, S32, (float)sj, (float)(lpk + lninvA[j]),
+                                                       (float)wk);
                         // a term outside the FP32 window, or one whose FP32 form leaves the float range
                         if (use32 && isfinite(t32))
                             s[j] += (double)t32;
                         else
-                            s[j] += tp_ogata_term<double>(sN, E, P, sj, lpk + lninvA[j], wk);
+                            s[j] += tp_ogata_term<double>(sN, E, P, S, sj, lpk + lninvA[j], wk);
                     }
                 }
             } else {
@@ -218,7 +224,7 @@ __global__ void __launch_bounds__(kGammaBlock) k_gamma_points(const TpCosmoConst
                     const double pk = psik[k], wk = prodk[k], lpk = lnpsik[k];
 #pragma unroll
                     for (int j = 0; j < 3; ++j)
-                        s[j] += tp_ogata_term<double>(sN, E, P, pk * invA[j], lpk + lninvA[j], wk);
+                        s[j] += tp_ogata_term<double>(sN, E, P, S, pk * invA[j], lpk + lninvA[j], wk);
                 }
             }
         } else {

@@ -186,7 +186,8 @@ struct tp_ctx {
     // tp_map3_batch's integrators of the last call, reused by the next (their region pools keep
     // their capacity, so a call of the same size allocates and page-faults nothing); dropped when
     // they hold more than kTpKeepPoolBytes
-    std::vector<tpc::AdaptiveIntegral31> map3_keep;
+    std::vector<tpc::AdaptiveIntegral31> map3_keep;              // also T1's (3-D) regions batches
+    std::vector<tpc::AdaptiveIntegralT<2, 1>> map2_keep;        // <Map^2>'s (2-D)
     bool round_log_on = false;        // tp_set_round_log: per-round trace of tp_map3_batch
     std::vector<double> round_log;    // 7 doubles per round (tp_round_log)
     TpLane lane[TP_NLANES];

@@ -487,6 +487,20 @@ int power_batch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &
     return 0;
 }
 
+// The context's kept integrators of a dimension (their pools keep capacity across calls).
+template <unsigned D>
+std::vector<tpc::AdaptiveIntegralT<D, 1>> &kept_integrals(tp_ctx *ctx);
+template <>
+std::vector<tpc::AdaptiveIntegralT<2, 1>> &kept_integrals<2>(tp_ctx *ctx)
+{
+    return ctx->map2_keep;
+}
+template <>
+std::vector<tpc::AdaptiveIntegralT<3, 1>> &kept_integrals<3>(tp_ctx *ctx)
+{
+    return ctx->map3_keep;
+}
+
 // Region records of the device-rule batches: the 2-D form for <Map^2>, the engine's 3-D form for T1.
 template <unsigned D>
 struct RegionIO;
@@ -519,7 +533,9 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
     ctx->stats = tp_batch_stats{};
     const int nt = tp_host_threads(ctx, kTpHostThreadsSync);
     const size_t nint = desc.size();
-    std::vector<tpc::AdaptiveIntegralT<D, 1>> I(nint);
+    std::vector<tpc::AdaptiveIntegralT<D, 1>> I;
+    I.swap(kept_integrals<D>(ctx));  // the last call's integrators, if kept
+    if (I.size() < nint) I.resize(nint);
     for (size_t k = 0; k < nint; ++k)
         if (!I[k].init(D, 1, lo + D * k, hi + D * k, 0, 0, reltol, tpc::L1))
             return tp_set_error(TP_EINVAL, "bad integration domain");
@@ -605,6 +621,11 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
         out[k] = post(v);
         if (err) err[k] = post(e);
         if (neval) neval[k] = (long long)I[k].num_eval();
+    }
+    {
+        size_t held = 0;
+        for (const auto &A : I) held += A.bytes_reserved();
+        if (held <= kTpKeepPoolBytes) kept_integrals<D>(ctx).swap(I);
     }
     ctx->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;
