@@ -145,6 +145,12 @@ int tp_set_batch_groups(tp_ctx *ctx, int groups);
  * number available.  No counterpart in the reference. */
 int tp_set_round_log(tp_ctx *ctx, int on);
 
+/* Host memory kept between batched calls: the integrators of the last tp_map3_batch (and of the
+ * last <Map^2> and T1 batch) stay in the context when their region pools hold at most `bytes`
+ * (default 16 GiB), so the next call of a similar size allocates and page-faults nothing.
+ * bytes = 0 releases what is kept and keeps nothing.  No counterpart in the reference. */
+int tp_set_keep_pool_bytes(tp_ctx *ctx, unsigned long long bytes);
+
 /* OpenMP threads of the host side of the batched cubatures (region records and heap decisions
  * between device rounds): n = 0 restores the defaults (8 for tp_map3_batch, whose host work
  * overlaps the device; 16 for the synchronous batches); the count is capped by
@@ -160,8 +166,9 @@ int tp_set_zero_copy_regions(tp_ctx *ctx, long long max_regions);
 /* Statistics of the last tp_map3_batch call on this context. */
 typedef struct {
     long long rounds, regions, points, launches;
-    double kernel_ms;            /* summed duration of the integrand/rule kernels */
-    double integrand_ms;         /* summed duration of the integrand (point) kernel only */
+    double kernel_ms;            /* device time of the integrand/rule kernels (tp_map3_batch: the union
+                                    of the rounds' intervals, since the groups' rounds can overlap) */
+    double integrand_ms;         /* the same for the integrand (point) kernel only */
     double wall_ms;              /* whole call                                     */
     long long surrogate_points;  /* points evaluated by the surrogate              */
     long long exact_checks;      /* points re-evaluated exactly                    */

@@ -112,6 +112,7 @@ _SIGS = {
     "tp_set_zero_copy_regions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
     "tp_set_round_log": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "tp_set_host_threads": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "tp_set_keep_pool_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_ulonglong]),
     "tp_round_log": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "tp_gamma_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
                                       ctypes.c_double, _dp, _dp, _llp]),
@@ -202,6 +203,10 @@ class Context:
     def set_host_threads(self, n=0):
         """OpenMP threads of the host side of the batched cubatures (0: the default)."""
         _check(lib().tp_set_host_threads(self.h, int(n)), "tp_set_host_threads")
+
+    def set_keep_pool_bytes(self, nbytes):
+        """Cap on the host region pools kept between batched calls (0: release and keep nothing)."""
+        _check(lib().tp_set_keep_pool_bytes(self.h, int(nbytes)), "tp_set_keep_pool_bytes")
 
     def set_round_log(self, on=True):
         _check(lib().tp_set_round_log(self.h, 1 if on else 0), "tp_set_round_log")

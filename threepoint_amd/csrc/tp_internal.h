@@ -160,10 +160,14 @@ struct TpLane {
 };
 constexpr int TP_NLANES = 4;  // groups of a batched call (at most; tp_ctx::nlanes are used)
 
+constexpr size_t kTpKeepPoolBytes = size_t(16) << 30;  // default cap on the host region pools kept
+
 struct tp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evm = nullptr;  // evm: after the integrand kernel
+    hipEvent_t ev_batch = nullptr;  // start of a tp_map3_batch call: the groups' streams wait on it,
+                                    // and their rounds are timed against it
     DevBuf d_in, d_out, d_regions, d_rout, d_fvals, d_desc, d_flag, d_misc;
     HostPinned h_in, h_out, h_regions, h_rout;
     tp_batch_stats stats{};
@@ -185,9 +189,10 @@ struct tp_ctx {
     int host_threads = 0;             // OpenMP threads of the host-side cubature work (0: default)
     // tp_map3_batch's integrators of the last call, reused by the next (their region pools keep
     // their capacity, so a call of the same size allocates and page-faults nothing); dropped when
-    // they hold more than kTpKeepPoolBytes
+    // they hold more than keep_pool_bytes (tp_set_keep_pool_bytes)
     std::vector<tpc::AdaptiveIntegral31> map3_keep;              // also T1's (3-D) regions batches
     std::vector<tpc::AdaptiveIntegralT<2, 1>> map2_keep;        // <Map^2>'s (2-D)
+    size_t keep_pool_bytes = kTpKeepPoolBytes;
     bool round_log_on = false;        // tp_set_round_log: per-round trace of tp_map3_batch
     std::vector<double> round_log;    // 7 doubles per round (tp_round_log)
     TpLane lane[TP_NLANES];
@@ -201,7 +206,6 @@ struct tp_ctx {
 // device idle).  The synchronous batches (host work on the critical path) default to
 // kTpHostThreadsSync.
 constexpr int kTpHostThreadsOverlap = 8, kTpHostThreadsSync = 16;
-constexpr size_t kTpKeepPoolBytes = size_t(512) << 20;
 int tp_host_threads(const tp_ctx *ctx, int dflt);
 
 // Error helpers (thread-local last error string).

@@ -625,7 +625,7 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
     {
         size_t held = 0;
         for (const auto &A : I) held += A.bytes_reserved();
-        if (held <= kTpKeepPoolBytes) kept_integrals<D>(ctx).swap(I);
+        if (held <= ctx->keep_pool_bytes) kept_integrals<D>(ctx).swap(I);
     }
     ctx->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return 0;

@@ -93,6 +93,7 @@ extern "C" int tp_ctx_create(int device, tp_ctx **out)
     hipEventCreate(&c->ev0);
     hipEventCreate(&c->ev1);
     hipEventCreate(&c->evm);
+    hipEventCreate(&c->ev_batch);
     if (c->d_flag.ensure(256)) {
         delete c;
         return TP_EHIP;
@@ -143,6 +144,7 @@ extern "C" int tp_ctx_destroy(tp_ctx *ctx)
     hipEventDestroy(ctx->ev0);
     hipEventDestroy(ctx->ev1);
     hipEventDestroy(ctx->evm);
+    hipEventDestroy(ctx->ev_batch);
     hipStreamDestroy(ctx->stream);
     delete ctx;
     return 0;
@@ -426,11 +428,11 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     // each group's rounds go to its own stream (one group: the context's stream), ordered after the
     // descriptor upload above
     hipStream_t gst[TP_NLANES];
-    if (nl > 1) TP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    TP_HIP(hipEventRecord(ctx->ev_batch, ctx->stream));
     for (int g = 0; g < nl; ++g) {
         if ((rc = ctx->lane[g].create())) return rc;
         gst[g] = nl == 1 ? ctx->stream : ctx->lane[g].stream;
-        if (nl > 1) TP_HIP(hipStreamWaitEvent(gst[g], ctx->ev1, 0));
+        if (nl > 1) TP_HIP(hipStreamWaitEvent(gst[g], ctx->ev_batch, 0));
         // the flag word is only ever raised, and a raised flag ends the batch: clear it once per call
         TP_HIP(hipMemsetAsync(ctx->lane[g].d_flag.p, 0, sizeof(int), gst[g]));
     }
@@ -528,6 +530,23 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         return 0;
     };
 
+    std::vector<std::pair<float, float>> iv_pts, iv_ker;  // device intervals of the rounds [ms]
+    auto union_ms = [](std::vector<std::pair<float, float>> &iv) {
+        std::sort(iv.begin(), iv.end());
+        double tot = 0, a = 0, b = -1;
+        for (const auto &x : iv) {
+            if (x.first > b) {
+                if (b > a) tot += b - a;
+                a = x.first;
+                b = x.second;
+            } else if (x.second > b) {
+                b = x.second;
+            }
+        }
+        if (b > a) tot += b - a;
+        return tot;
+    };
+
     // waits for the group's evaluation, absorbs it and selects the next regions
     auto complete = [&](int g) -> int {
         Group &Gr = G[g];
@@ -538,11 +557,16 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         ctx->stats.wait_ms += waited;
         const double tc0 = ms_since(t0);
         Gr.inflight = false;
-        float ms = 0, msp = 0;
-        TP_HIP(hipEventElapsedTime(&ms, L.ev0, L.ev1));
-        if (mode != TP_MODE_SURROGATE) TP_HIP(hipEventElapsedTime(&msp, L.ev0, L.evm));
-        ctx->stats.kernel_ms += ms;
-        ctx->stats.integrand_ms += msp;
+        // the round's device intervals against the call's start; the groups' rounds can overlap on
+        // the device, so the times reported are the unions of these intervals
+        float ta = 0, tm = 0, tb = 0;
+        TP_HIP(hipEventElapsedTime(&ta, ctx->ev_batch, L.ev0));
+        TP_HIP(hipEventElapsedTime(&tb, ctx->ev_batch, L.ev1));
+        iv_ker.push_back({ta, tb});
+        if (mode != TP_MODE_SURROGATE) {
+            TP_HIP(hipEventElapsedTime(&tm, ctx->ev_batch, L.evm));
+            iv_pts.push_back({ta, tm});
+        }
         const TpRegionOut *ro = (const TpRegionOut *)L.h_rout.p;
         // the point kernels' flag word: in region 0's flag (exact rounds) or copied back (surrogate rounds)
         const int pflag = mode == TP_MODE_SURROGATE ? *(const int *)L.h_flag.p : (Gr.nreg ? ro[0].flag >> 1 : 0);
@@ -607,6 +631,8 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         for (int g = 0; g < nl; ++g) r = std::max(r, G[g].rounds);
         ctx->stats.rounds = r;
     }
+    ctx->stats.kernel_ms = union_ms(iv_ker);
+    ctx->stats.integrand_ms = union_ms(iv_pts);
     // the final re-sums walk every integral's heap: spread them over the host threads
     std::vector<char> redo_flag(ntriples, 0);
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads) if (ntriples > 16)
@@ -627,7 +653,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     {
         size_t held = 0;
         for (const auto &A : I) held += A.bytes_reserved();
-        if (held <= kTpKeepPoolBytes) ctx->map3_keep.swap(I);
+        if (held <= ctx->keep_pool_bytes) ctx->map3_keep.swap(I);
     }
     if (!redo.empty()) {
         tp_batch_stats keep = ctx->stats;
@@ -686,6 +712,17 @@ extern "C" int tp_set_host_threads(tp_ctx *ctx, int n)
 {
     if (!ctx || n < 0 || n > 256) return tp_set_error(TP_EINVAL, "tp_set_host_threads: n must be 0 (default) .. 256");
     ctx->host_threads = n;
+    return 0;
+}
+
+extern "C" int tp_set_keep_pool_bytes(tp_ctx *ctx, unsigned long long bytes)
+{
+    if (!ctx) return tp_set_error(TP_EINVAL, "tp_set_keep_pool_bytes: NULL context");
+    ctx->keep_pool_bytes = (size_t)bytes;
+    if (bytes == 0) {  // release what is kept now
+        std::vector<tpc::AdaptiveIntegral31>().swap(ctx->map3_keep);
+        std::vector<tpc::AdaptiveIntegralT<2, 1>>().swap(ctx->map2_keep);
+    }
     return 0;
 }
 
