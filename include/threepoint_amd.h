@@ -179,6 +179,8 @@ typedef struct {
     double host_ms;              /* host time in region gathering and heap decisions   */
     double wait_ms;              /* host time blocked waiting for the device           */
     long long graph_rebuilds;    /* round graphs rebuilt because an update was refused */
+    double integrand_sum_ms;     /* sum over launches of the integrand kernel's event-timed duration
+                                    (launches x a per-launch profiler's average; >= integrand_ms) */
 } tp_batch_stats;
 int tp_last_batch_stats(const tp_ctx *ctx, tp_batch_stats *out);
 

@@ -43,7 +43,8 @@ class tp_batch_stats(ctypes.Structure):
                 ("surrogate_points", ctypes.c_longlong), ("exact_checks", ctypes.c_longlong),
                 ("fallback_batches", ctypes.c_longlong), ("max_check_relerr", ctypes.c_double),
                 ("fallback_integrals", ctypes.c_longlong), ("host_ms", ctypes.c_double),
-                ("wait_ms", ctypes.c_double), ("graph_rebuilds", ctypes.c_longlong)]
+                ("wait_ms", ctypes.c_double), ("graph_rebuilds", ctypes.c_longlong),
+                ("integrand_sum_ms", ctypes.c_double)]
 
 
 class tp_train_config(ctypes.Structure):

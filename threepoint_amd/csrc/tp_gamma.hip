@@ -496,6 +496,7 @@ static int gamma_launch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpGamm
     (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     ctx->stats.kernel_ms += ms;
     ctx->stats.integrand_ms += ms;
+    ctx->stats.integrand_sum_ms += ms;
     ctx->stats.launches += 1;
     int flag = 0;
     TP_HIP(hipMemcpy(&flag, ctx->d_flag.p, sizeof(int), hipMemcpyDeviceToHost));

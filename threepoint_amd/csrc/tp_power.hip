@@ -415,6 +415,7 @@ int power_launch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
     (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     ctx->stats.kernel_ms += ms;
     ctx->stats.integrand_ms += ms;
+    ctx->stats.integrand_sum_ms += ms;
     ctx->stats.launches += 1;
     if (flag) return tp_set_error(TP_ENONFINITE, "power spectrum integrand: non-finite value");
     return 0;
@@ -583,6 +584,7 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
         (void)hipEventElapsedTime(&ms_pts, ctx->ev0, ctx->evm);
         ctx->stats.kernel_ms += ms;
         ctx->stats.integrand_ms += ms_pts;
+        ctx->stats.integrand_sum_ms += ms_pts;
         ctx->stats.launches += 1;
         ctx->stats.rounds += 1;
         ctx->stats.regions += (long long)nreg;
@@ -728,7 +730,7 @@ extern "C" int tp_pell(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t n, con
     float ms = 0;
     (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
     ctx->stats = tp_batch_stats{};
-    ctx->stats.kernel_ms = ctx->stats.integrand_ms = ms;
+    ctx->stats.kernel_ms = ctx->stats.integrand_ms = ctx->stats.integrand_sum_ms = ms;
     ctx->stats.launches = 1;
     ctx->stats.points = (long long)n * 96;
     if (flag) return tp_set_error(TP_ENONFINITE, "tp_pell: non-finite value");

@@ -566,6 +566,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         if (mode != TP_MODE_SURROGATE) {
             TP_HIP(hipEventElapsedTime(&tm, ctx->ev_batch, L.evm));
             iv_pts.push_back({ta, tm});
+            ctx->stats.integrand_sum_ms += tm - ta;
         }
         const TpRegionOut *ro = (const TpRegionOut *)L.h_rout.p;
         // the point kernels' flag word: in region 0's flag (exact rounds) or copied back (surrogate rounds)
@@ -678,6 +679,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         keep.launches += ctx->stats.launches;
         keep.kernel_ms += ctx->stats.kernel_ms;
         keep.integrand_ms += ctx->stats.integrand_ms;
+        keep.integrand_sum_ms += ctx->stats.integrand_sum_ms;
         keep.host_ms += ctx->stats.host_ms;
         keep.wait_ms += ctx->stats.wait_ms;
         keep.fallback_integrals += (long long)redo.size();
