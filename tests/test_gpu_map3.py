@@ -268,3 +268,46 @@ def test_zero_copy_rounds_bit_identical(fid_tables, gpu_ctx, mode):
         assert np.array_equal(v0, v1) and np.array_equal(e0, e1) and np.array_equal(n0, n1)
     finally:
         tp.lib().tp_set_zero_copy_regions(gpu_ctx.h, 0)
+
+
+def test_host_settings_bit_identical(gpu_ctx, fid_tables):
+    """Host threads, kept or fresh region pools, and a reused integrator set change no bit of a result."""
+    th = W.to_rad(W.triples(W.map3_c0_radii()))
+    try:
+        for mode in (tp.MODE_FP64, tp.MODE_FP32):
+            ref = fid_tables.map3_batch(th, None, 1e-4, mode)
+            again = fid_tables.map3_batch(th, None, 1e-4, mode)  # the kept integrators reused
+            gpu_ctx.set_host_threads(1)
+            one = fid_tables.map3_batch(th, None, 1e-4, mode)
+            gpu_ctx.set_host_threads(0)
+            gpu_ctx.set_keep_pool_bytes(0)  # fresh pools every call
+            fresh = fid_tables.map3_batch(th, None, 1e-4, mode)
+            small = fid_tables.map3_batch(th[:3], None, 1e-4, mode)  # then a smaller batch
+            gpu_ctx.set_keep_pool_bytes(1 << 34)
+            for r in (again, one, fresh):
+                for a, b in zip(ref, r):
+                    assert np.array_equal(a, b), mode
+            for a, b in zip(ref, small):
+                assert np.array_equal(a[:3], b), mode
+    finally:
+        gpu_ctx.set_host_threads(0)
+        gpu_ctx.set_keep_pool_bytes(1 << 34)
+
+
+def test_round_log_and_timing_stats(gpu_ctx, fid_tables):
+    """The per-round trace accounts for every launch and region, and the device-busy time (union of the
+    two groups' intervals) never exceeds the summed launch durations."""
+    th = W.to_rad(W.triples(W.map3_c0_radii()))
+    gpu_ctx.set_round_log(True)
+    try:
+        fid_tables.map3_batch(th, None, 1e-4, tp.MODE_FP32)
+        st = gpu_ctx.stats()
+        L = gpu_ctx.round_log()
+    finally:
+        gpu_ctx.set_round_log(False)
+    assert L.shape == (st["launches"], 7)
+    assert int(L[:, 1].sum()) == st["regions"] and st["points"] == 33 * st["regions"]
+    assert set(np.unique(L[:, 0]).astype(int)) <= {0, 1}
+    assert np.all(L[:, 3] >= L[:, 2]) and np.all(L[:, 5] >= L[:, 4]) and np.all(L[:, 4] >= L[:, 3])
+    assert 0 < st["integrand_ms"] <= st["integrand_sum_ms"] * (1 + 1e-6) + 1e-3
+    assert st["kernel_ms"] >= st["integrand_ms"] - 1e-3

@@ -124,3 +124,26 @@ def test_power_rejects_bad_arguments(fid_tables):
         fid_tables.pell(np.array([10.0]), flags=8)
     with pytest.raises(tp.TpError):
         fid_tables.pkappa_batch(np.array([10.0]), icosmo=np.array([3]))
+
+
+def test_map2_and_t1_kept_pools_bit_identical(gpu_ctx, fid_tables):
+    """<Map^2> and T1 batches give the same bits with their integrators reused or freshly allocated."""
+    th = np.array([0.5, 2.0, 8.0, 32.0]) * A
+    pairs = np.array([[4, 8, 8, 6, 6, 12], [8, 8, 8, 8, 8, 8]], dtype=float) * A
+    area, ngal = (5 * 60 * A) ** 2, 30 / A ** 2
+    try:
+        m_ref = fid_tables.map2_batch(th, reltol=1e-5)
+        t_ref = fid_tables.map3_cov_t1(pairs, area, 0.0, 0.3, ngal, tp.PK_T17, reltol=1e-3)
+        m_again = fid_tables.map2_batch(th, reltol=1e-5)
+        t_again = fid_tables.map3_cov_t1(pairs, area, 0.0, 0.3, ngal, tp.PK_T17, reltol=1e-3)
+        gpu_ctx.set_keep_pool_bytes(0)
+        m_fresh = fid_tables.map2_batch(th, reltol=1e-5)
+        t_fresh = fid_tables.map3_cov_t1(pairs, area, 0.0, 0.3, ngal, tp.PK_T17, reltol=1e-3)
+    finally:
+        gpu_ctx.set_keep_pool_bytes(1 << 34)
+    for r in (m_again, m_fresh):
+        for a, b in zip(m_ref, r):
+            assert np.array_equal(a, b)
+    for r in (t_again, t_fresh):
+        for a, b in zip(t_ref, r):
+            assert np.array_equal(a, b)

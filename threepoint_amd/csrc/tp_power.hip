@@ -520,9 +520,12 @@ struct RegionIO<3> {
 
 // As power_batch, with the rule points generated and the rule applied on the device, so a round moves
 // one region record in and one result out per region (not the rule's points and values) and the host
-// only keeps the heaps.  launch(nreg, d_regions, d_out) queues the round's kernels on ctx->stream and
-// records ctx->evm after the integrand; fv_per_region doubles of ctx->d_fvals per region are reserved
-// for kernels that pass the point values between launches.
+// only keeps the heaps.  The integrals are split into two groups, each with its own stream and buffers
+// (the context's lanes), that alternate as in tp_map3_batch: while the device evaluates one group's
+// regions and copies them over, the host absorbs the other group's results.  Each integral's decision
+// sequence is unchanged.  launch(nreg, d_regions, d_out, d_fvals, stream, evm) queues a round's kernels
+// on `stream` and records evm after the integrand; fv_per_region doubles of d_fvals per region are
+// reserved for kernels that pass the point values between launches.
 template <unsigned D, class Launch, class Post>
 int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const double *lo, const double *hi,
                   double reltol, size_t fv_per_region, Launch launch, Post post, const char *what, double *out,
@@ -530,7 +533,9 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
 {
     using In = typename RegionIO<D>::In;
     using Out = typename RegionIO<D>::Out;
-    auto t0 = std::chrono::steady_clock::now();
+    using clk = std::chrono::steady_clock;
+    auto t0 = clk::now();
+    auto ms_since = [](clk::time_point a) { return std::chrono::duration<double, std::milli>(clk::now() - a).count(); };
     ctx->stats = tp_batch_stats{};
     const int nt = tp_host_threads(ctx, kTpHostThreadsSync);
     const size_t nint = desc.size();
@@ -544,22 +549,44 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
     const size_t bd = nint * sizeof(TpPowerDesc);
     if ((rc = ctx->d_desc.ensure(std::max(bd, (size_t)64)))) return rc;
     TP_HIP(hipMemcpyAsync(ctx->d_desc.p, desc.data(), bd, hipMemcpyHostToDevice, ctx->stream));
-    std::vector<size_t> active(nint), offs;
-    for (size_t k = 0; k < nint; ++k) active[k] = k;
-    while (!active.empty()) {
-        const size_t na = active.size();
-        offs.assign(na + 1, 0);
-        for (size_t a = 0; a < na; ++a) offs[a + 1] = offs[a] + I[active[a]].pending().size();
-        const size_t nreg = offs[na];
+    const int nl = nint > 1 ? 2 : 1;
+    struct Group {
+        std::vector<size_t> active, offs;
+        size_t nreg = 0;
+        bool inflight = false;
+    };
+    Group G[2];
+    for (size_t k = 0; k < nint; ++k) G[nl == 1 ? 0 : (int)(k & 1)].active.push_back(k);
+    hipStream_t gst[2];
+    TP_HIP(hipEventRecord(ctx->ev_batch, ctx->stream));
+    for (int g = 0; g < nl; ++g) {
+        if ((rc = ctx->lane[g].create())) return rc;
+        gst[g] = nl == 1 ? ctx->stream : ctx->lane[g].stream;
+        if (nl > 1) TP_HIP(hipStreamWaitEvent(gst[g], ctx->ev_batch, 0));
+    }
+    std::vector<std::pair<float, float>> iv_pts, iv_ker;  // device intervals of the rounds [ms]
+
+    auto issue = [&](int g) -> int {
+        Group &Gr = G[g];
+        TpLane &L = ctx->lane[g];
+        auto th0 = clk::now();
+        const std::vector<size_t> &act = Gr.active;
+        const size_t na = act.size();
+        Gr.offs.assign(na + 1, 0);
+        for (size_t a = 0; a < na; ++a) Gr.offs[a + 1] = Gr.offs[a] + I[act[a]].pending().size();
+        const size_t nreg = Gr.offs[na];
+        Gr.nreg = nreg;
         const size_t br = nreg * sizeof(In), bo = nreg * sizeof(Out);
-        if ((rc = ctx->d_regions.ensure(br)) || (rc = ctx->d_rout.ensure(bo)) ||
-            (rc = ctx->d_fvals.ensure(std::max(nreg * fv_per_region * sizeof(double), (size_t)64))) ||
-            (rc = ctx->h_regions.ensure(br)) || (rc = ctx->h_rout.ensure(bo)))
-            return rc;
-        In *hr = (In *)ctx->h_regions.p;
+        int e;
+        if ((e = L.d_regions.ensure(br)) || (e = L.d_rout.ensure(bo)) ||
+            (e = L.d_fvals.ensure(std::max(nreg * fv_per_region * sizeof(double), (size_t)64))) ||
+            (e = L.h_regions.ensure(br)) || (e = L.h_rout.ensure(bo)))
+            return e;
+        In *hr = (In *)L.h_regions.p;
+        const std::vector<size_t> &offs = Gr.offs;
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nt) if (na > 256)
         for (size_t a = 0; a < na; ++a) {
-            auto &A = I[active[a]];
+            auto &A = I[act[a]];
             const auto &pend = A.pending();
             for (size_t q = 0; q < pend.size(); ++q) {
                 const auto &R = A.regions()[pend[q]];
@@ -569,32 +596,48 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
                     o.hw[d] = R.hw[d];
                 }
                 o.vol = R.vol;
-                o.integral = (int)active[a];
+                o.integral = (int)act[a];
                 o.pad = 0;
             }
         }
-        TP_HIP(hipMemcpyAsync(ctx->d_regions.p, hr, br, hipMemcpyHostToDevice, ctx->stream));
-        TP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-        if ((rc = launch(nreg, (const In *)ctx->d_regions.p, (Out *)ctx->d_rout.p))) return rc;
-        TP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-        TP_HIP(hipMemcpyAsync(ctx->h_rout.p, ctx->d_rout.p, bo, hipMemcpyDeviceToHost, ctx->stream));
-        TP_HIP(hipStreamSynchronize(ctx->stream));
-        float ms = 0, ms_pts = 0;
-        (void)hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1);
-        (void)hipEventElapsedTime(&ms_pts, ctx->ev0, ctx->evm);
-        ctx->stats.kernel_ms += ms;
-        ctx->stats.integrand_ms += ms_pts;
-        ctx->stats.integrand_sum_ms += ms_pts;
+        ctx->stats.host_ms += ms_since(th0);
+        TP_HIP(hipMemcpyAsync(L.d_regions.p, hr, br, hipMemcpyHostToDevice, gst[g]));
+        TP_HIP(hipEventRecord(L.ev0, gst[g]));
+        if ((e = launch(nreg, (const In *)L.d_regions.p, (Out *)L.d_rout.p, (double *)L.d_fvals.p, gst[g], L.evm)))
+            return e;
+        TP_HIP(hipEventRecord(L.ev1, gst[g]));
+        TP_HIP(hipMemcpyAsync(L.h_rout.p, L.d_rout.p, bo, hipMemcpyDeviceToHost, gst[g]));
+        TP_HIP(hipEventRecord(L.done, gst[g]));
+        Gr.inflight = true;
         ctx->stats.launches += 1;
-        ctx->stats.rounds += 1;
         ctx->stats.regions += (long long)nreg;
         ctx->stats.points += (long long)nreg * RegionIO<D>::kPts;
-        const Out *ho = (const Out *)ctx->h_rout.p;
+        return 0;
+    };
+    auto complete = [&](int g) -> int {
+        Group &Gr = G[g];
+        TpLane &L = ctx->lane[g];
+        auto tw = clk::now();
+        TP_HIP(hipEventSynchronize(L.done));
+        ctx->stats.wait_ms += ms_since(tw);
+        Gr.inflight = false;
+        float ta = 0, tm = 0, tb = 0;
+        TP_HIP(hipEventElapsedTime(&ta, ctx->ev_batch, L.ev0));
+        TP_HIP(hipEventElapsedTime(&tm, ctx->ev_batch, L.evm));
+        TP_HIP(hipEventElapsedTime(&tb, ctx->ev_batch, L.ev1));
+        iv_pts.push_back({ta, tm});
+        iv_ker.push_back({ta, tb});
+        ctx->stats.integrand_sum_ms += tm - ta;
+        auto th0 = clk::now();
+        const Out *ho = (const Out *)L.h_rout.p;
+        const std::vector<size_t> &act = Gr.active;
+        const std::vector<size_t> &offs = Gr.offs;
+        const size_t na = act.size();
         std::vector<char> fin(na, 0);
         int bad = 0;
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nt) if (na > 256) reduction(| : bad)
         for (size_t a = 0; a < na; ++a) {
-            auto &A = I[active[a]];
+            auto &A = I[act[a]];
             const auto &pend = A.pending();
             for (size_t q = 0; q < pend.size(); ++q) {
                 auto &R = A.regions()[pend[q]];
@@ -612,10 +655,55 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
         }
         if (bad) return tp_set_error(TP_ENONFINITE, std::string(what) + ": non-finite value");
         std::vector<size_t> next;
+        next.reserve(na);
         for (size_t a = 0; a < na; ++a)
-            if (!fin[a]) next.push_back(active[a]);
-        active.swap(next);
+            if (!fin[a]) next.push_back(act[a]);
+        Gr.active.swap(next);
+        ctx->stats.host_ms += ms_since(th0);
+        return 0;
+    };
+    // on an error, let queued work drain before the buffers can be reused
+    auto fail = [&](int e) {
+        for (int g = 0; g < nl; ++g) (void)hipStreamSynchronize(gst[g]);
+        return e;
+    };
+    long long rounds[2] = {0, 0};
+    for (int g = 0; g < nl; ++g)
+        if (!G[g].active.empty()) {
+            if ((rc = issue(g))) return fail(rc);
+            ++rounds[g];
+        }
+    for (;;) {
+        bool any = false;
+        for (int g = 0; g < nl; ++g) {
+            if (!G[g].inflight) continue;
+            any = true;
+            if ((rc = complete(g))) return fail(rc);
+            if (!G[g].active.empty()) {
+                if ((rc = issue(g))) return fail(rc);
+                ++rounds[g];
+            }
+        }
+        if (!any) break;
     }
+    ctx->stats.rounds = std::max(rounds[0], rounds[1]);
+    auto union_ms = [](std::vector<std::pair<float, float>> &iv) {
+        std::sort(iv.begin(), iv.end());
+        double tot = 0, ua = 0, ub = -1;
+        for (const auto &x : iv) {
+            if (x.first > ub) {
+                if (ub > ua) tot += ub - ua;
+                ua = x.first;
+                ub = x.second;
+            } else if (x.second > ub) {
+                ub = x.second;
+            }
+        }
+        if (ub > ua) tot += ub - ua;
+        return tot;
+    };
+    ctx->stats.kernel_ms = union_ms(iv_ker);
+    ctx->stats.integrand_ms = union_ms(iv_pts);
 #pragma omp parallel for schedule(dynamic, 64) num_threads(nt) if (nint > 256)
     for (size_t k = 0; k < nint; ++k) {
         double v, e;
@@ -629,7 +717,7 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
         for (const auto &A : I) held += A.bytes_reserved();
         if (held <= ctx->keep_pool_bytes) kept_integrals<D>(ctx).swap(I);
     }
-    ctx->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ctx->stats.wall_ms = ms_since(t0);
     return 0;
 }
 
@@ -637,15 +725,15 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
 int map2_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &desc, const double *lo,
                  const double *hi, double reltol, double *out, double *err, long long *neval)
 {
-    auto launch = [&](size_t nreg, const TpRegion2 *d_reg, TpRegion2Out *d_out) -> int {
+    auto launch = [&](size_t nreg, const TpRegion2 *d_reg, TpRegion2Out *d_out, double *d_fv, hipStream_t st,
+                      hipEvent_t evm) -> int {
         const size_t npts = nreg * kGm2Pts;
-        hipLaunchKernelGGL(k_map2_region_points, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, ctx->stream,
-                           c->d_const, c->d_tab, c->n_z, (const TpPowerDesc *)ctx->d_desc.p, d_reg, nreg,
-                           (double *)ctx->d_fvals.p);
+        hipLaunchKernelGGL(k_map2_region_points, dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, st, c->d_const,
+                           c->d_tab, c->n_z, (const TpPowerDesc *)ctx->d_desc.p, d_reg, nreg, d_fv);
         TP_HIP(hipGetLastError());
-        TP_HIP(hipEventRecord(ctx->evm, ctx->stream));
-        hipLaunchKernelGGL(k_map2_region_rule, dim3((unsigned)((nreg + 255) / 256)), dim3(256), 0, ctx->stream, nreg,
-                           d_reg, (const double *)ctx->d_fvals.p, d_out);
+        TP_HIP(hipEventRecord(evm, st));
+        hipLaunchKernelGGL(k_map2_region_rule, dim3((unsigned)((nreg + 255) / 256)), dim3(256), 0, st, nreg, d_reg,
+                           (const double *)d_fv, d_out);
         TP_HIP(hipGetLastError());
         return 0;
     };
@@ -658,14 +746,15 @@ int map2_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
 int t1_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &desc, const double *lo,
                const double *hi, double reltol, double area, double *out, long long *neval)
 {
-    auto launch = [&](size_t nreg, const TpRegion *d_reg, TpRegionOut *d_out) -> int {
+    auto launch = [&](size_t nreg, const TpRegion *d_reg, TpRegionOut *d_out, double *, hipStream_t st,
+                      hipEvent_t evm) -> int {
         const size_t nblk = (nreg + kT1Reg - 1) / kT1Reg;
         if (nblk > 0x7fffffffu) return tp_set_error(TP_ECAPACITY, "T1: too many regions for one launch");
-        hipLaunchKernelGGL(k_t1_regions, dim3((unsigned)nblk), dim3(256), 0, ctx->stream, c->d_const,
+        hipLaunchKernelGGL(k_t1_regions, dim3((unsigned)nblk), dim3(256), 0, st, c->d_const,
                            (const TpPNode *)c->d_pnode, c->z_max / 2, (const TpPowerDesc *)ctx->d_desc.p, d_reg,
                            nreg, d_out);
         TP_HIP(hipGetLastError());
-        TP_HIP(hipEventRecord(ctx->evm, ctx->stream));
+        TP_HIP(hipEventRecord(evm, st));
         return 0;
     };
     std::vector<double> e(desc.size());
