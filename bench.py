@@ -15,6 +15,9 @@ Paths timed in the same run:
   batched_fp64  all triples advance together, one integrand + rule launch per round.
   batched_fp32  same, FP32 bispectrum with FP64 z-sum and cubature accumulation.
   surrogate     batched, learned B_kappa with exact checks (when a model is present).
+The paths run in the order given (by default FP32 before FP64: for about two calls after FP64
+calls an FP32 call runs ~15% slower, its two groups' kernels no longer overlapping, so each path is
+timed in its own steady state); errors are compared after all paths ran.
 
 value = triples/s of the fastest path whose max relative error against the
 batched FP64 result (itself within 1e-6 of the CPU FP64 oracle) is <= 1e-3.
@@ -102,6 +105,11 @@ def run_path(tabs, ctx, thetas, path, steps, warmup, reltol, icosmo=None):
     dt = time.perf_counter() - t0
     agg["ms_per_step"] = 1e3 * dt / steps
     agg["triples_per_s"] = n * steps / dt
+    if path.startswith("batched_fp"):  # the launch shapes the tuner chose, by launch-size bucket
+        import ctypes
+        tab = (ctypes.c_int * 4)()
+        tp._check(tp.lib().tp_region_lanes_table(ctx.h, mode, tab), "tp_region_lanes_table")
+        agg["tuned_shapes"] = list(tab)
     return vals, agg
 
 
@@ -438,7 +446,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--reltol", type=float, default=1e-4)
-    ap.add_argument("--paths", default="plain_fp64,batched_fp64,batched_fp32,surrogate",
+    ap.add_argument("--paths", default="plain_fp64,batched_fp32,batched_fp64,surrogate",
                     help="paths to time (c2 takes plain_fp64, batched_fp64, batched_fp32)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")

@@ -366,6 +366,8 @@ extern "C" int tp_map3_single(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const 
 // regions than these ends with TP_ECAPACITY instead of growing without bound.  The adaptive
 // rounds of the measured workloads stay below 300 and 1e8 regions.
 static const long long kMaxRounds = 20000;
+// adaptive grouping of tp_map3_batch (two groups): split above, merge below these round sizes
+static const size_t kSplitRegions = 8192, kMergeRegions = 2048;
 static const long long kMaxRegions = 400000000LL;
 static const size_t kSurrogateMaxEval = 4000000;  // points per triple before an exact redo
 int tp_region_lanes(tp_ctx *ctx, const tp_cosmo *c, int mode, size_t np);  // tp_map3.hip
@@ -615,6 +617,22 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         for (int g = 0; g < nl; ++g) (void)hipStreamSynchronize(gst[g]);
         return e;
     };
+    // With two groups the grouping adapts to the round size: while a round is small (the first rounds
+    // and the tail, tens of us of kernel) the two groups' host turnarounds would only queue behind each
+    // other, so the integrals run as one group; a round above kSplitRegions splits them into two
+    // groups, which merge again below kMergeRegions.  Each integral's decision sequence is unchanged.
+    const bool adaptive = nl == 2;
+    auto pending_of = [&](int g) {
+        size_t n = 0;
+        for (size_t a : G[g].active) n += I[a].pending().size();
+        return n;
+    };
+    int ng = nl;
+    if (adaptive && pending_of(0) + pending_of(1) < kSplitRegions) {
+        G[0].active.insert(G[0].active.end(), G[1].active.begin(), G[1].active.end());
+        G[1].active.clear();
+        ng = 1;
+    }
     for (int g = 0; g < nl; ++g)
         if (!G[g].active.empty() && (rc = launch(g))) return fail(rc);
     for (;;) {
@@ -623,13 +641,31 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
             if (!G[g].inflight) continue;
             any = true;
             if ((rc = complete(g))) return fail(rc);
+            if (adaptive) {
+                const int o = 1 - g;
+                const size_t next = pending_of(g);
+                if (ng == 2 && next + (G[o].inflight ? G[o].nreg : pending_of(o)) < kMergeRegions) {
+                    if (G[o].inflight && (rc = complete(o))) return fail(rc);
+                    G[g].active.insert(G[g].active.end(), G[o].active.begin(), G[o].active.end());
+                    G[o].active.clear();
+                    G[g].rounds = std::max(G[g].rounds, G[o].rounds);
+                    ng = 1;
+                } else if (ng == 1 && next > kSplitRegions && G[g].active.size() > 1) {
+                    const size_t h = G[g].active.size() / 2;
+                    G[o].active.assign(G[g].active.begin() + h, G[g].active.end());
+                    G[g].active.resize(h);
+                    G[o].rounds = G[g].rounds;
+                    ng = 2;
+                    if ((rc = launch(o))) return fail(rc);
+                }
+            }
             if (!G[g].active.empty() && (rc = launch(g))) return fail(rc);
         }
         if (!any) break;
     }
-    if (nl > 1) {
+    if (nl > 1) {  // evaluation rounds of the integral that took the most
         long long r = 0;
-        for (int g = 0; g < nl; ++g) r = std::max(r, G[g].rounds);
+        for (size_t t = 0; t < ntriples; ++t) r = std::max(r, I[t].rounds() + 1);
         ctx->stats.rounds = r;
     }
     ctx->stats.kernel_ms = union_ms(iv_ker);
