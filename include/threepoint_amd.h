@@ -225,7 +225,8 @@ int tp_gamma_batch(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t ntri, cons
                    double reltol, double *out_re, double *out_im, long long *neval);
 /* tp_gamma_batch with the integrand's precision: TP_MODE_FP64, or TP_MODE_FP32 (Ogata terms whose
  * wavenumbers lie in the FP32 window in FP32, the others and any that leave the float range in FP64;
- * FP64 accumulation, rule and heap).  No counterpart in the reference (its integrand is FP64). */
+ * FP64 accumulation, rule and heap; the FP32 form needs prec_k <= 10753, TP_EINVAL otherwise).  No
+ * counterpart in the reference (its integrand is FP64). */
 int tp_gamma_batch_mode(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t ntri, const double *x123,
                         double reltol, int mode, double *out_re, double *out_im, long long *neval);
 /* The FP32 form's window: Ogata terms whose wavenumbers all lie in [qmin, qmax] run in FP32

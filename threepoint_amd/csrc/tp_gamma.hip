@@ -23,6 +23,8 @@
 namespace {
 
 constexpr int kGammaBlock = 256;
+// the FP32 form's miss mask holds 3 bits per k slot of a thread in 128 bits: 42 slots of 256 k
+constexpr int kGammaMaxK32 = 42 * kGammaBlock + 1;
 
 struct TpGammaDesc {
     double x[3];
@@ -127,7 +129,7 @@ __device__ __forceinline__ double ogata_term(const TpNode64 &N, const TpEH64 &E,
 // and any FP32 term that leaves the float range stay FP64.  Accumulation is FP64.
 
 template <bool F32>
-__global__ void __launch_bounds__(kGammaBlock) k_gamma_points(const TpCosmoConst *__restrict__ cc,
+__global__ void __launch_bounds__(kGammaBlock) __attribute__((amdgpu_waves_per_eu(F32 ? 3 : 1))) k_gamma_points(const TpCosmoConst *__restrict__ cc,
                                                               const double *__restrict__ tab, const TpEH64 *__restrict__ eh,
                                                               const TpEH32 *__restrict__ eh32,
                                                               int n_z, const double *__restrict__ ogata, int prec_k,
@@ -138,6 +140,8 @@ __global__ void __launch_bounds__(kGammaBlock) k_gamma_points(const TpCosmoConst
 {
     __shared__ TpNode64 sN;
     __shared__ TpNode32 sN32;
+    __shared__ TpPointPre sP;              // the point's shape (FP32 form: read back by the FP64 pass)
+    __shared__ TpShapeNodeT<double> sS;
     __shared__ int sok;
     __shared__ double red[3][kGammaBlock / 64];
     size_t p = blockIdx.x;
@@ -171,7 +175,7 @@ __global__ void __launch_bounds__(kGammaBlock) k_gamma_points(const TpCosmoConst
     const double *lnpsik = ogata + (D.which == 0 ? 4 : 5) * (size_t)prec_k;
     double s[3] = {0, 0, 0};
     if (sok) {
-        const TpEH64 E = eh[D.icosmo];
+        const TpEH64 &E = eh[D.icosmo];
         const double cpsi = cos(psi), spsi = sin(psi), cphi = cos(phi);
         double invA[3] = {1.0 / A[0], 1.0 / A[1], 1.0 / A[2]};
         // every term's triangle has the shape (cos psi, sin psi, u3); away from the degenerate
@@ -185,6 +189,9 @@ __global__ void __launch_bounds__(kGammaBlock) k_gamma_points(const TpCosmoConst
             TpShapeNodeT<double> S;
             tp_shape_node<double>(sN, P.lr1, P.r2, &S);
             if (F32) {
+                // every thread of the block evaluates the same point: each stores the same shape
+                sP = P;
+                sS = S;
                 const TpEH32 E32 = eh32[D.icosmo];
                 TpPointPreT<float> P32;
                 for (int d = 0; d < 3; ++d) {
@@ -202,23 +209,62 @@ __global__ void __launch_bounds__(kGammaBlock) k_gamma_points(const TpCosmoConst
                 // range of the FP32 <Map^3> path (below it the one-halo denominators underflow in FP32)
                 const double smax32 = q32hi / (sN.ks * lmax);
                 const double smin32 = q32lo / (sN.ks * lmin);
-                for (int k = 1 + threadIdx.x; k < prec_k; k += kGammaBlock) {
+                // Two passes, so that the rare FP64 terms do not set the register budget of the FP32 loop:
+                // 1. the FP32 terms inside the window; a term whose FP32 form leaves the float range sets
+                //    its bit in `miss` (slot m of this thread's k = 1 + tid + 256 m, component j: bit
+                //    3 m + j of the two words; prec_k <= kGammaMaxK32, checked by the host);
+                // 2. the FP64 terms: outside the window, and the missed ones.
+                // Each component is the FP64 sum of its FP32 terms, then of its FP64 terms, in k order.
+                uint64_t miss0 = 0, miss1 = 0;
+                int m = 0;
+                for (int k = 1 + threadIdx.x; k < prec_k; k += kGammaBlock, ++m) {
                     const double pk = psik[k], wk = prodk[k], lpk = lnpsik[k];
 #pragma unroll
                     for (int j = 0; j < 3; ++j) {
                         const double sj = pk * invA[j];
-                        float t32 = 0.f;
-                        const bool use32 = sj < smax32 && sj >= smin32;
-                        if (use32)
-                            t32 = tp_ogata_term<float>(sN32, E32, P32, S32, (float)sj, (float)(lpk + lninvA[j]),
-                                                       (float)wk);
-                        // a term outside the FP32 window, or one whose FP32 form leaves the float range
-                        if (use32 && isfinite(t32))
-                            s[j] += (double)t32;
-                        else
-                            s[j] += tp_ogata_term<double>(sN, E, P, S, sj, lpk + lninvA[j], wk);
+                        if (sj < smax32 && sj >= smin32) {
+                            const float t32 = tp_ogata_term<float>(sN32, E32, P32, S32, (float)sj,
+                                                                   (float)(lpk + lninvA[j]), (float)wk);
+                            const int bit = 3 * m + j;
+                            if (isfinite(t32))
+                                s[j] += (double)t32;
+                            else if (bit < 64)
+                                miss0 |= 1ull << bit;
+                            else
+                                miss1 |= 1ull << (bit - 64);
+                        }
                     }
                 }
+                // the FP64 pass reads the point's shape and the E&H constants from memory, so they do not
+                // stay in registers through the FP32 loop (the empty asm: memory may have changed)
+                __asm__ __volatile__("" ::: "memory");
+                const TpPointPre &Pm = sP;
+                const TpShapeNodeT<double> &Sm = sS;
+                const TpEH64 &Em = eh[D.icosmo];
+                m = 0;
+                double s0 = 0, s1 = 0, s2 = 0;
+                for (int k = 1 + threadIdx.x; k < prec_k; k += kGammaBlock, ++m) {
+                    const double pk = psik[k], wk = prodk[k], lpk = lnpsik[k];
+#pragma unroll 1
+                    for (int j = 0; j < 3; ++j) {
+                        const double ia = j == 0 ? invA[0] : (j == 1 ? invA[1] : invA[2]);
+                        const double la = j == 0 ? lninvA[0] : (j == 1 ? lninvA[1] : lninvA[2]);
+                        const double sj = pk * ia;
+                        const int bit = 3 * m + j;
+                        const bool missed = bit < 64 ? ((miss0 >> bit) & 1) : ((miss1 >> (bit - 64)) & 1);
+                        if (sj < smax32 && sj >= smin32 && !missed) continue;
+                        const double t = tp_ogata_term<double>(sN, Em, Pm, Sm, sj, lpk + la, wk);
+                        if (j == 0)
+                            s0 += t;
+                        else if (j == 1)
+                            s1 += t;
+                        else
+                            s2 += t;
+                    }
+                }
+                s[0] += s0;
+                s[1] += s1;
+                s[2] += s2;
             } else {
                 for (int k = 1 + threadIdx.x; k < prec_k; k += kGammaBlock) {
                     const double pk = psik[k], wk = prodk[k], lpk = lnpsik[k];
@@ -227,10 +273,10 @@ __global__ void __launch_bounds__(kGammaBlock) k_gamma_points(const TpCosmoConst
                         s[j] += tp_ogata_term<double>(sN, E, P, S, pk * invA[j], lpk + lninvA[j], wk);
                 }
             }
-        } else {
+        } else {  // the degenerate shape (rare): one term at a time
             for (int k = 1 + threadIdx.x; k < prec_k; k += kGammaBlock) {
                 const double pk = psik[k], wk = prodk[k];
-#pragma unroll
+#pragma unroll 1
                 for (int j = 0; j < 3; ++j) s[j] += ogata_term(sN, E, pk, wk, invA[j], cpsi, spsi, cphi);
             }
         }
@@ -464,6 +510,9 @@ static int gamma_launch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpGamm
                         int mode = TP_MODE_FP64)
 {
     if (npts == 0) return 0;
+    if (mode == TP_MODE_FP32 && ctx->gamma_prec_k > (unsigned)kGammaMaxK32)
+        return tp_set_error(TP_EINVAL, "Gamma FP32 form: prec_k above " + std::to_string(kGammaMaxK32) +
+                                           " (set up with tp_gamma_setup); use the FP64 form");
     int rc;
     size_t bx = npts * 3 * sizeof(double), bf = npts * 2 * sizeof(double);
     size_t bd = desc.size() * sizeof(TpGammaDesc), bp = npts * sizeof(int);

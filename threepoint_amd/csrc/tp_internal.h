@@ -185,7 +185,7 @@ struct tp_ctx {
     int region_lpp_force = 0;   // 0 = tuned; 1, 2, 4, 8 = fixed
     bool graph_rounds = false;  // batched rounds replayed as one graph per lane (tp_set_graph_rounds)
     int nlanes = 2;             // groups in flight in tp_map3_batch (tp_set_batch_groups)
-    long long zero_copy_regions = 0;  // rounds up to this many regions skip the copies (pinned buffers)
+    long long zero_copy_regions = 2048;  // rounds up to this many regions skip the copies (pinned buffers)
     int host_threads = 0;             // OpenMP threads of the host-side cubature work (0: default)
     // tp_map3_batch's integrators of the last call, reused by the next (their region pools keep
     // their capacity, so a call of the same size allocates and page-faults nothing); dropped when
