@@ -293,6 +293,12 @@ int tp_map3_cov_gauss_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int 
 int tp_map3_cov_t1_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo, const double *theta6,
                          double area, double lmin, double sigma, double n_gal, int flags, double reltol,
                          double *out, long long *neval);
+/* tp_map3_cov_t1_batch with the Pell's precision: TP_MODE_FP64, or TP_MODE_FP32 (FP32 node values of the
+ * 96-node Limber sum with native exp/log, added in FP64; FP64 filters, rule and heaps; Eisenstein & Hu
+ * P(k) only).  No counterpart in the reference (its integrand is FP64). */
+int tp_map3_cov_t1_batch_mode(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo, const double *theta6,
+                              double area, double lmin, double sigma, double n_gal, int flags, double reltol,
+                              int mode, double *out, long long *neval);
 
 /* ---------------------------------------------------------- input files */
 /* Replaces read_n_of_z: averaging = 0 the classed_version's reader (helper.cpp:31-96, linear

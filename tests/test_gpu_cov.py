@@ -50,3 +50,34 @@ def test_cov_rejects_bad_arguments(fid_tables):
         fid_tables.map3_cov_gauss(np.zeros((1, 6)), AREA)
     with pytest.raises(tp.TpError):
         fid_tables.map3_cov_t1(np.full((1, 6), 4 * A), -1.0)
+
+
+@pytest.mark.parametrize("flags", [0, tp.PK_T17, tp.PK_LINEAR])
+def test_t1_fp32_form_within_1e3(fid_tables, fid_oracle, flags):
+    """The FP32 form of T1 (FP32 Pell node values, FP64 sums, rule and heaps) against the FP64 path and
+    the FP64 oracle, at the 1e-3 bar of the FP32 forms."""
+    pairs = np.array([[4, 8, 8, 8, 8, 8], [6, 6, 12, 4, 8, 16], [2, 4, 16, 2, 2, 2]], dtype=float) * A
+    v64, n64 = fid_tables.map3_cov_t1(pairs, AREA, 0.0, 0.3, NGAL, flags, reltol=1e-4)
+    v32, n32 = fid_tables.map3_cov_t1(pairs, AREA, 0.0, 0.3, NGAL, flags, reltol=1e-4, mode=tp.MODE_FP32)
+    r = np.abs(v32 / v64 - 1)
+    print("T1 FP32 vs FP64 (flags %d): max rel %.3e, points %s vs %s" % (flags, r.max(), n32, n64))
+    assert r.max() < 1e-3
+    ov, _ = fid_oracle.t1_total(pairs[0, :3], pairs[0, 3:], AREA, 0.0, 0.3, NGAL, flags, 1e-4)
+    assert abs(v32[0] / ov - 1) < 1e-3
+
+
+def test_t1_fp32_shape_noise_only_is_exact(fid_tables):
+    """With shape noise only no node is evaluated: the FP32 form gives the FP64 path's bits."""
+    pairs = np.array([[3, 6, 12, 3, 6, 12], [4, 4, 4, 8, 8, 8]], dtype=float) * A
+    a = fid_tables.map3_cov_t1(pairs, AREA, 0.0, 0.3, NGAL, tp.PK_SHAPENOISE_ONLY, reltol=1e-5)
+    b = fid_tables.map3_cov_t1(pairs, AREA, 0.0, 0.3, NGAL, tp.PK_SHAPENOISE_ONLY, reltol=1e-5, mode=tp.MODE_FP32)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_t1_fp32_rejects_tabulated_pk(gpu_ctx, fid_nz):
+    from threepoint_amd import workloads as W
+    c = W.fiducial()
+    kh = np.geomspace(1e-4, 1e2, 50)
+    tabs = tp.CosmologyTables([c], fid_nz, 100, 3.0, gpu_ctx, pk=(kh, 1e4 * kh / (1 + (kh / 0.02) ** 3)))
+    with pytest.raises(tp.TpError):
+        tabs.map3_cov_t1(np.full((1, 6), 4 * A), AREA, mode=tp.MODE_FP32)

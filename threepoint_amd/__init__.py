@@ -102,6 +102,9 @@ _SIGS = {
     "tp_map3_cov_t1_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_int, ctypes.c_double, _dp, _llp]),
+    "tp_map3_cov_t1_batch_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp,
+                                                 ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                 ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _llp]),
     "tp_read_n_of_z": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp]),
     "tp_read_triangle_configurations": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t, _dp,
                                                        ctypes.POINTER(ctypes.c_size_t)]),
@@ -424,15 +427,17 @@ class CosmologyTables:
                                              _d(val), _d(err), ne.ctypes.data_as(_llp)), "tp_map3_cov_gauss_batch")
         return val, err, ne
 
-    def map3_cov_t1(self, theta6, area, lmin=0.0, sigma=0.3, n_gal=1.0, flags=0, icosmo=None, reltol=1e-4):
-        """T1_total of the cuda_version's covariance for an infinite survey, pairs theta6[n, 6] [rad]."""
+    def map3_cov_t1(self, theta6, area, lmin=0.0, sigma=0.3, n_gal=1.0, flags=0, icosmo=None, reltol=1e-4,
+                    mode=MODE_FP64):
+        """T1_total of the cuda_version's covariance for an infinite survey, pairs theta6[n, 6] [rad];
+        mode MODE_FP64 or MODE_FP32 (FP32 Pell node values, FP64 sums)."""
         t = np.ascontiguousarray(theta6, dtype=np.float64).reshape(-1, 6)
         n = t.shape[0]
         ic = np.zeros(n, dtype=np.int32) if icosmo is None else np.ascontiguousarray(icosmo, dtype=np.int32)
         val, ne = np.zeros(n), np.zeros(n, dtype=np.int64)
-        _check(lib().tp_map3_cov_t1_batch(self.ctx.h, self.h, n, ic.ctypes.data_as(_ip), _d(t), area, lmin, sigma,
-                                          n_gal, flags, reltol, _d(val), ne.ctypes.data_as(_llp)),
-               "tp_map3_cov_t1_batch")
+        _check(lib().tp_map3_cov_t1_batch_mode(self.ctx.h, self.h, n, ic.ctypes.data_as(_ip), _d(t), area, lmin,
+                                               sigma, n_gal, flags, reltol, mode, _d(val), ne.ctypes.data_as(_llp)),
+               "tp_map3_cov_t1_batch_mode")
         return val, ne
 
 

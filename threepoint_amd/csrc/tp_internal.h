@@ -97,6 +97,7 @@ struct tp_cosmo {
     int *d_err = nullptr;             // build error flag
     void *d_pk = nullptr;             // tabulated P(k): k, p [ncosmo][nk] then idx [ncosmo][ng]
     void *d_pnode = nullptr;          // [ncosmo][96] TpPNode: the P Limber sum's z-nodes (tp_power.hip)
+    void *d_pnode32 = nullptr;        // [ncosmo][96] TpPNode32: their FP32 form (same allocation)
     int pk_nk = 0;                    // 0: Eisenstein & Hu
     std::vector<TpCosmoConst> h_const;
 };

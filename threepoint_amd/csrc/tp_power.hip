@@ -68,6 +68,18 @@ __device__ double pell_nodes(const TpCosmoConst &C, const TpPNode *__restrict__ 
     return q * dx + shot;
 }
 
+// FP32 form of pell_nodes: FP32 node values (tp_limber_pk_node32), added in FP64 in the same order.
+__device__ double pell_nodes32(const TpPNode32 *__restrict__ N, double ell, int flags, double dx, double shot)
+{
+    if ((flags & TP_PK_SHAPENOISE_ONLY) || ell == 0) return shot;
+    const float e = (float)ell, le = (float)log(ell);
+    double q = 0;
+    for (int j = 0; j < 48; ++j)
+        q += c_pgl_w[j] * ((double)tp_limber_pk_node32(N[2 * j], e, le, flags) +
+                           (double)tp_limber_pk_node32(N[2 * j + 1], e, le, flags));
+    return q * dx + shot;
+}
+
 // l3 = |l1 + l2| of the T1 integrand at (l1, l2, phi), cuda_version/apertureStatisticsCovariance.cu:1962
 __device__ __forceinline__ double t1_l3(double l1, double l2, double phi)
 {
@@ -155,8 +167,10 @@ __global__ void k_power_points(const TpCosmoConst *__restrict__ cc, const double
 constexpr int kT1Reg = 5;                     // regions per workgroup: 235 Pell on 256 threads
 constexpr int kT1Pell = 14 + TP_NPTS_GM3;     // Pell per region
 
+template <bool F32>  // F32: the Pell from the FP32 node records (pell_nodes32)
 __global__ void __launch_bounds__(256) k_t1_regions(const TpCosmoConst *__restrict__ cc,
-                                                    const TpPNode *__restrict__ pnode, double dx,
+                                                    const TpPNode *__restrict__ pnode,
+                                                    const TpPNode32 *__restrict__ pnode32, double dx,
                                                     const TpPowerDesc *__restrict__ desc,
                                                     const TpRegion *__restrict__ reg, size_t nreg,
                                                     TpRegionOut *__restrict__ out)
@@ -185,9 +199,13 @@ __global__ void __launch_bounds__(256) k_t1_regions(const TpCosmoConst *__restri
                 ell = t1_l3(l1, l2, phi);
                 need = !(l1 <= D.lmin || l2 <= D.lmin || ell <= D.lmin || ell > D.lmax);
             }
-            if (need)
-                v = pell_nodes(cc[D.icosmo], pnode + (size_t)D.icosmo * TP_NNODES, ell, D.flags, dx,
-                               0.5 * D.sigma * D.sigma / D.ngal);
+            if (need) {
+                const double shot = 0.5 * D.sigma * D.sigma / D.ngal;
+                if (F32)
+                    v = pell_nodes32(pnode32 + (size_t)D.icosmo * TP_NNODES, ell, D.flags, dx, shot);
+                else
+                    v = pell_nodes(cc[D.icosmo], pnode + (size_t)D.icosmo * TP_NNODES, ell, D.flags, dx, shot);
+            }
         }
         pl[rr][k] = v;
     }
@@ -342,7 +360,7 @@ __global__ void __launch_bounds__(64) k_pell(const TpCosmoConst *__restrict__ cc
 
 // The P Limber sum's z-nodes of every cosmology: node 2j at cx - dx x_j, node 2j+1 at cx + dx x_j.
 __global__ void k_pnodes(int ncosmo, int n_z, const TpCosmoConst *__restrict__ cc, const double *__restrict__ tab,
-                         double zmax, TpPNode *__restrict__ out)
+                         double zmax, TpPNode *__restrict__ out, TpPNode32 *__restrict__ out32)
 {
 #pragma clang fp contract(off)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -351,6 +369,7 @@ __global__ void k_pnodes(int ncosmo, int n_z, const TpCosmoConst *__restrict__ c
     const double cx = zmax / 2, dx = zmax / 2;
     const double z = (nd & 1) ? cx + dx * c_pgl_x[j] : cx - dx * c_pgl_x[j];
     tp_pnode_make(cc[ic], tab + (size_t)ic * 6 * n_z, n_z, z, &out[i]);
+    tp_pnode32_make(cc[ic], out[i], &out32[i]);
 }
 
 int upload_pgl(int device)
@@ -744,14 +763,15 @@ int map2_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
 
 // T1 rounds: one launch of k_t1_regions (points, cached Pell and rule in one kernel).
 int t1_regions(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &desc, const double *lo,
-               const double *hi, double reltol, double area, double *out, long long *neval)
+               const double *hi, double reltol, double area, int mode, double *out, long long *neval)
 {
     auto launch = [&](size_t nreg, const TpRegion *d_reg, TpRegionOut *d_out, double *, hipStream_t st,
                       hipEvent_t evm) -> int {
         const size_t nblk = (nreg + kT1Reg - 1) / kT1Reg;
         if (nblk > 0x7fffffffu) return tp_set_error(TP_ECAPACITY, "T1: too many regions for one launch");
-        hipLaunchKernelGGL(k_t1_regions, dim3((unsigned)nblk), dim3(256), 0, st, c->d_const,
-                           (const TpPNode *)c->d_pnode, c->z_max / 2, (const TpPowerDesc *)ctx->d_desc.p, d_reg,
+        hipLaunchKernelGGL(mode == TP_MODE_FP32 ? k_t1_regions<true> : k_t1_regions<false>, dim3((unsigned)nblk),
+                           dim3(256), 0, st, c->d_const, (const TpPNode *)c->d_pnode,
+                           (const TpPNode32 *)c->d_pnode32, c->z_max / 2, (const TpPowerDesc *)ctx->d_desc.p, d_reg,
                            nreg, d_out);
         TP_HIP(hipGetLastError());
         TP_HIP(hipEventRecord(evm, st));
@@ -876,9 +896,11 @@ int tp_build_pnodes(tp_ctx *ctx, tp_cosmo *c)
     int rc = upload_pgl(ctx->device);
     if (rc) return rc;
     const size_t n = (size_t)c->ncosmo * TP_NNODES;
-    TP_HIP(hipMalloc(&c->d_pnode, n * sizeof(TpPNode)));
+    const size_t b64 = (n * sizeof(TpPNode) + 255) / 256 * 256;
+    TP_HIP(hipMalloc(&c->d_pnode, b64 + n * sizeof(TpPNode32)));
+    c->d_pnode32 = (char *)c->d_pnode + b64;
     hipLaunchKernelGGL(k_pnodes, dim3((unsigned)((n + 127) / 128)), dim3(128), 0, ctx->stream, c->ncosmo, c->n_z,
-                       c->d_const, c->d_tab, c->z_max, (TpPNode *)c->d_pnode);
+                       c->d_const, c->d_tab, c->z_max, (TpPNode *)c->d_pnode, (TpPNode32 *)c->d_pnode32);
     TP_HIP(hipGetLastError());
     TP_HIP(hipStreamSynchronize(ctx->stream));
     return 0;
@@ -929,8 +951,21 @@ extern "C" int tp_map3_cov_t1_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, co
                                     const double *theta6, double area, double lmin, double sigma, double n_gal,
                                     int flags, double reltol, double *out, long long *neval)
 {
+    return tp_map3_cov_t1_batch_mode(ctx, c, n, icosmo, theta6, area, lmin, sigma, n_gal, flags, reltol,
+                                     TP_MODE_FP64, out, neval);
+}
+
+extern "C" int tp_map3_cov_t1_batch_mode(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo,
+                                         const double *theta6, double area, double lmin, double sigma, double n_gal,
+                                         int flags, double reltol, int mode, double *out, long long *neval)
+{
     int rc = check_args(ctx, c, flags);
     if (rc) return rc;
+    if (mode != TP_MODE_FP64 && mode != TP_MODE_FP32)
+        return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch_mode: mode must be TP_MODE_FP64 or TP_MODE_FP32");
+    if (mode == TP_MODE_FP32 && c->pk_nk)
+        return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch_mode: the FP32 form uses the Eisenstein & Hu P(k); "
+                                       "tables built from a tabulated P(k) need TP_MODE_FP64");
     if (n && (!theta6 || !out)) return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch: NULL array");
     if (!(reltol > 0) || !(area > 0) || !(n_gal > 0) || !(lmin >= 0))
         return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch: reltol, area and n_gal must be positive, lmin >= 0");
@@ -996,7 +1031,7 @@ extern "C" int tp_map3_cov_t1_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, co
     }
     std::vector<double> v(desc.size());
     std::vector<long long> ne(desc.size());
-    rc = t1_regions(ctx, c, desc, lo.data(), hi.data(), reltol, area, v.data(), ne.data());
+    rc = t1_regions(ctx, c, desc, lo.data(), hi.data(), reltol, area, mode, v.data(), ne.data());
     if (rc) return rc;
     for (size_t i = 0; i < n; ++i) {
         double r = 0;

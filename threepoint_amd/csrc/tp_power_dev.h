@@ -194,6 +194,79 @@ __host__ __device__ __forceinline__ void tp_pnode_make(const TpCosmoConst &C, co
     o->pad = 0;
 }
 
+// FP32 form of a node record, for the FP32 Pell of T1: the node's constants folded so that a node value
+// is a few native exp/log.  With Delta^2_lin(k) = D1^2 P_lin k^3 / (2 pi^2) = r_EH(k)^2 exp((3 + n_s) ln k
+// + ld), ld = ln(D1^2 norm^2) + (3 + n_s) ln 2997.9, and the Limber factor pref 2 pi^2 / k^3 =
+// exp(lp - 3 ln k), lp = ln(2 pi^2 pref), the node value is (pq + ph) exp(lp - 3 ln k) [x T17].
+// The Eisenstein & Hu fit only (a tabulated P(k) keeps the FP64 form).
+struct TpPNode32 {
+    float inv_fK, lfK, lrs, rs;
+    float lp, ld, ns3;                       // see above; ns3 = 3 + n_s
+    float h, sh043, omh2, sqa, theta2, cL;   // E&H: h, 0.43 s h, om h^2, sqrt(alpha_nu), theta^2, c_L
+    float f1x3, f2, a, b, gam3, lf3c, alpha, beta, xnu;  // Halofit z-part (f1x3 = 3 f1, gam3 = 3 - gam)
+    int zero;
+};
+
+__host__ __device__ inline void tp_pnode32_make(const TpCosmoConst &C, const TpPNode &N, TpPNode32 *o)
+{
+    const double ln29979 = 8.005667322535853;  // ln 2997.9
+    o->inv_fK = (float)(1.0 / N.fK);
+    o->lfK = (float)N.lfK;
+    o->lrs = (float)N.lrs;
+    o->rs = (float)N.rs;
+    o->lp = (float)(N.pref > 0 ? log(2 * tpd::kPi * tpd::kPi * N.pref) : -HUGE_VAL);
+    o->ld = (float)(log(N.D1 * N.D1 * C.norm * C.norm) + (3. + C.ns) * ln29979);
+    o->ns3 = (float)(3. + C.ns);
+    o->h = (float)C.h;
+    o->sh043 = (float)(0.43 * C.eh_sh);
+    o->omh2 = (float)C.eh_omh2;
+    o->sqa = (float)C.eh_sqrt_alnu;
+    o->theta2 = (float)C.eh_theta2;
+    o->cL = (float)C.eh_cL;
+    o->f1x3 = (float)(3 * N.h.f1);
+    o->f2 = (float)N.h.f2;
+    o->a = (float)N.h.a;
+    o->b = (float)N.h.b;
+    o->gam3 = (float)(3 - N.h.gam);
+    o->lf3c = (float)N.h.lf3c;
+    o->alpha = (float)N.h.alpha;
+    o->beta = (float)N.h.beta;
+    o->xnu = (float)N.h.xnu;
+    o->zero = N.zero;
+}
+
+// The FP32 node value (tp_limber_pk_node's arithmetic with the folded constants; native exp, log and
+// reciprocal).  lell = ln ell.
+__device__ __forceinline__ float tp_limber_pk_node32(const TpPNode32 &N, float ell, float lell, int flags)
+{
+    if (N.zero) return 0.f;
+    const float k = ell * N.inv_fK, lk = lell - N.lfK;
+    // Eisenstein & Hu transfer function r (tp_pk_eh_log)
+    const float kh = k * N.h, x = N.sh043 * kh, x2 = x * x;
+    const float geff = N.omh2 * (N.sqa + (1.f - N.sqa) * __builtin_amdgcn_rcpf(1.f + x2 * x2));
+    const float qeff = kh * __builtin_amdgcn_rcpf(geff) * N.theta2;
+    const float L = __logf(2.718281828f + N.cL * qeff);
+    const float Cq = 14.4f + 325.f * __builtin_amdgcn_rcpf(1.f + 60.5f * __expf(1.11f * __logf(qeff)));
+    const float r = L * __builtin_amdgcn_rcpf(L + Cq * qeff * qeff);
+    const float plin = r * r * __expf(N.ns3 * lk + N.ld);  // Delta^2_lin
+    float P = plin;
+    if (!(flags & TP_PK_LINEAR)) {  // revised Halofit k-part (tp_halofit_k)
+        const float y = k * N.rs, ly = lk + N.lrs, ysqr = y * y;
+        float ph = N.a * __expf(N.f1x3 * ly) *
+                   __builtin_amdgcn_rcpf(1.f + N.b * __expf(N.f2 * ly) + __expf(N.gam3 * (N.lf3c + ly)));
+        ph = ph * __builtin_amdgcn_rcpf(1.f + N.xnu * __builtin_amdgcn_rcpf(ysqr));
+        const float pq = plin * __expf(N.beta * __logf(1.f + plin)) * __builtin_amdgcn_rcpf(1.f + plin * N.alpha) *
+                         __expf(-0.25f * y - 0.125f * ysqr);
+        P = pq + ph;
+    }
+    float v = P * __expf(N.lp - 3.f * lk);
+    if (flags & TP_PK_T17) {  // tp_t17_factor
+        const float c1 = 9.5171e-4f, c2 = 5.1543e-3f, a1 = 1.3063f, a2 = 1.1475f, a3 = 0.62793f;
+        v *= __expf(a1 * __logf(1.f + c1 * __expf(-a1 * lk)) - a3 * __logf(1.f + c2 * __expf(-a2 * lk)));
+    }
+    return v;
+}
+
 // tp_limber_pk at a precomputed node
 // (lell = ln ell, computed once per multipole)
 __host__ __device__ __forceinline__ double tp_limber_pk_node(const TpCosmoConst &C, const TpPNode &N, double ell,
