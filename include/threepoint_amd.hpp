@@ -180,6 +180,7 @@ class ApertureStatisticsCovariance {
     std::vector<double> T1_total(const std::vector<std::array<double, 6>> &pairs);
     double lMin = 0;        /* the reference's executable sets lMin = 0 */
     double reltol = 1e-4;   /* apertureStatisticsCovariance.cu:1088 */
+    int mode = TP_MODE_FP64;/* or TP_MODE_FP32: FP32 Pell node values, FP64 sums (not in the reference) */
 
   private:
     BispectrumCalculator *Bispectrum_;

@@ -154,6 +154,14 @@ def test_calculate_covariance_cli(tmp_path, fid_oracle):
             assert abs(t1[i, j] / ov - 1) < 1e-5
     r = subprocess.run(args[:9] + ["1"] + args[10:], capture_output=True, text=True, timeout=60)
     assert r.returncode == 1  # T4 requested: not provided
+    # --fp32: the FP32 form of T1 writes the same file, within 1e-3 of the FP64 matrix
+    fp32_dir = tmp_path / "fp32"
+    fp32_dir.mkdir()
+    r = subprocess.run(args[:4] + [str(fp32_dir) + "/"] + args[5:] + ["--fp32"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    t1f = np.loadtxt(fp32_dir / (name % (1, n_deg, 7.87 * deg * 57.3)))
+    assert t1f.shape == (4, 4) and np.max(np.abs(t1f / t1 - 1)) < 1e-3
 
 
 def test_c3_batch_subset_matches_oracle(gpu_ctx, fid_nz):

@@ -3,7 +3,7 @@
 //
 //   calculateApertureStatisticsCovariance.x <cosmology> <thetas [arcmin]> <n(z)> <output folder>
 //       <covariance parameters> <T1 0|1> <T2 0|1> <T4 0|1> <T5 0|1> <T6 0|1> <T7 0|1> <T7_2h 0|1> <geometry>
-//       [--nz-bins N] [--reltol X] [--no-t17]
+//       [--nz-bins N] [--reltol X] [--no-t17] [--fp32]
 //
 // The same 13 arguments as the reference.  Provided: T1 and T2 for the infinite survey geometry (T2 is
 // 0 there, as in the reference); requesting T4-T7 or another geometry is an error (exit status 1).
@@ -41,19 +41,21 @@ int main(int argc, char *argv[])
     const char *usage =
         "calculateApertureStatisticsCovariance.x <cosmology> <thetas [arcmin]> <n(z)> <output folder> <covariance "
         "parameters>\n    <T1> <T2> <T4> <T5> <T6> <T7> <T7_2h> <circle|square|infinite|rectangular> [--nz-bins N] "
-        "[--reltol X] [--no-t17]\n";
+        "[--reltol X] [--no-t17] [--fp32]\n";
     if (argc < 14) {
         std::cerr << usage;
         return 1;
     }
     int nbins = 100;
     double reltol = 1e-4;
+    bool fp32 = false;  // T1 with FP32 Pell node values (FP64 sums), within 1e-3 of FP64
     bool t17 = true;
     for (int i = 14; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--nz-bins" && i + 1 < argc) nbins = std::atoi(argv[++i]);
         else if (a == "--reltol" && i + 1 < argc) reltol = std::atof(argv[++i]);
         else if (a == "--no-t17") t17 = false;
+        else if (a == "--fp32") fp32 = true;
         else {
             std::cerr << usage;
             return 1;
@@ -81,6 +83,7 @@ int main(int argc, char *argv[])
         BispectrumCalculator bk(&cosmo, nz, nbins, cosmo.zmax);
         ApertureStatisticsCovariance cov(&bk, covPar, type, t17);
         cov.reltol = reltol;
+        if (fp32) cov.mode = TP_MODE_FP32;
         std::vector<std::array<double, 3>> tri;
         const size_t N = thetas.size();
         for (size_t i = 0; i < N; i++)

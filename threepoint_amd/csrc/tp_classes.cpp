@@ -259,10 +259,10 @@ std::vector<double> ApertureStatisticsCovariance::T1_total(const std::vector<std
 {
     std::vector<double> th, out(pairs.size());
     for (auto &p : pairs) th.insert(th.end(), p.begin(), p.end());
-    check(tp_map3_cov_t1_batch(Bispectrum_->ctx(), Bispectrum_->tables(), pairs.size(), nullptr, th.data(),
-                               covPar_.area, lMin, covPar_.shapenoise_sigma, covPar_.galaxy_density, flags_, reltol,
-                               out.data(), nullptr),
-          "tp_map3_cov_t1_batch");
+    check(tp_map3_cov_t1_batch_mode(Bispectrum_->ctx(), Bispectrum_->tables(), pairs.size(), nullptr, th.data(),
+                                    covPar_.area, lMin, covPar_.shapenoise_sigma, covPar_.galaxy_density, flags_,
+                                    reltol, mode, out.data(), nullptr),
+          "tp_map3_cov_t1_batch_mode");
     return out;
 }
 
