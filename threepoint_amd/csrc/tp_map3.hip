@@ -108,7 +108,8 @@ __global__ void __launch_bounds__(64 * (LPP < 4 ? 4 : LPP)) k_region_points(
 // workgroup takes kCacheReg regions with G groups of 256 threads; group g owns the partials
 // [g 8/G, (g+1) 8/G) of the summation tree and, for each partial (12 z-nodes),
 //   A. evaluates the P_E terms of the 47 multipoles per region (7 l1, 7 l2, 33 l3) at the 12 nodes
-//      into its LDS block (work items of one node and 64 multipoles, so node reads are wave-uniform);
+//      into its LDS block (work items of 64 consecutive (node, multipole) entries, whose at most two
+//      node records are read through wave-uniform loads);
 //   B. lets each point add its 12 node values, the shape-dependent rest (tp_bispec_joint) with the
 //      three P_E terms read from LDS, and push the partial into the group's subtree.
 // The groups' subtrees are then joined in tree order.  Every value is formed by the same operations
@@ -209,21 +210,39 @@ __global__ void __launch_bounds__(256 * G) k_region_cached(const TpNodeT<T> *__r
         }
         return acc;
     };
-    // phase A work items: (node j, 64 consecutive multipoles), wave-uniform j
-    const int nblk = (ne + 63) >> 6, items = TP_PART_NODES * nblk;
+    // phase A work items.  With one cosmology and at least two regions (ne >= 94): 64 consecutive
+    // entries of the flattened (node j, multipole e) space of the partial, so that every lane has
+    // work; such an item spans at most two nodes, read as two wave-uniform records and selected per
+    // lane.  Otherwise: one node and 64 consecutive multipoles.
+    const bool flat = uni && ne >= 64;
+    const int nflat = TP_PART_NODES * ne;
+    const int nblk = (ne + 63) >> 6, items = flat ? (nflat + 63) >> 6 : TP_PART_NODES * nblk;
     double s1 = 0.0, s2 = 0.0, s4 = 0.0, v = 0.0;
     for (int k = 0; k < PPG; ++k) {
         const int n0 = (g * PPG + k) * TP_PART_NODES;
-        for (int it = wv; it < items; it += 4) {
-            const int j = it / nblk, e = (it - j * nblk) * 64 + lane;
-            if (e < ne) {
-                if (uni) {
-                    const TpNodeT<T> &N = nodes0[n0 + j];
+        if (flat) {
+            for (int it = wv; it < items; it += 4) {
+                const int f = it * 64 + lane;
+                const int ja = (it * 64) / ne, jb = min(ja + 1, TP_PART_NODES - 1);  // wave-uniform
+                const TpNodeT<T> &Na = nodes0[n0 + ja], &Nb = nodes0[n0 + jb];
+                if (f < nflat) {
+                    const int j = f / ne, e = f - j * ne;
+                    const TpNodeT<T> N = j == ja ? Na : Nb;
                     ps_g[j][e] = tp_pse_node<T>(N, eh[ic0], lv[e] * N.ks, llv[e] + N.lnks);
-                } else {
-                    const int ic = ics[e / kCacheEll];
-                    const TpNodeT<T> &N = nodes[(size_t)ic * TP_NNODES + n0 + j];
-                    ps_g[j][e] = tp_pse_node<T>(N, eh[ic], lv[e] * N.ks, llv[e] + N.lnks);
+                }
+            }
+        } else {
+            for (int it = wv; it < items; it += 4) {
+                const int j = it / nblk, e = (it - j * nblk) * 64 + lane;
+                if (e < ne) {
+                    if (uni) {
+                        const TpNodeT<T> &N = nodes0[n0 + j];
+                        ps_g[j][e] = tp_pse_node<T>(N, eh[ic0], lv[e] * N.ks, llv[e] + N.lnks);
+                    } else {
+                        const int ic = ics[e / kCacheEll];
+                        const TpNodeT<T> &N = nodes[(size_t)ic * TP_NNODES + n0 + j];
+                        ps_g[j][e] = tp_pse_node<T>(N, eh[ic], lv[e] * N.ks, llv[e] + N.lnks);
+                    }
                 }
             }
         }
