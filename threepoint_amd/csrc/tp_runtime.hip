@@ -410,10 +410,11 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     if ((rc = ctx->d_desc.ensure(sizeof(TpIntegralDesc) * ntriples))) return rc;
     TP_HIP(hipMemcpyAsync(ctx->d_desc.p, desc.data(), sizeof(TpIntegralDesc) * ntriples, hipMemcpyHostToDevice,
                           ctx->stream));
-    // Integrals are split into groups that alternate on the stream: while the device
-    // evaluates one group's regions, the host absorbs the other group's results and runs
-    // its heap decisions.  Each integral's decision sequence is unchanged.  A surrogate
-    // round synchronises inside (its fallback test reads the checks back), so it uses one group.
+    // Integrals are split into groups, each with its own stream: while the device evaluates
+    // one group's regions, the host absorbs the other group's results and runs its heap
+    // decisions (with two groups the split adapts to the round size, see below).  Each
+    // integral's decision sequence is unchanged.  A surrogate round synchronises inside (its
+    // fallback test reads the checks back), so it uses one group on the context's stream.
     const int nl = (mode == TP_MODE_SURROGATE) ? 1 : std::max(1, std::min(ctx->nlanes, TP_NLANES));
     struct Group {
         std::vector<size_t> active, offs;
