@@ -605,7 +605,7 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
         const std::vector<size_t> &offs = Gr.offs;
         // static schedules in both loops: a thread keeps the same integrals from loop to loop and round to
         // round, so their region pools stay in its caches (dynamic chunks: ~28% slower on 5 120 <Map^2>)
-#pragma omp parallel for schedule(static) num_threads(nt) if (na > 256)
+#pragma omp parallel for schedule(static) num_threads(nt) if (na > 1 && (na > 256 || nreg > 2048))
         for (size_t a = 0; a < na; ++a) {
             auto &A = I[act[a]];
             const auto &pend = A.pending();
@@ -656,7 +656,7 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
         const size_t na = act.size();
         std::vector<char> fin(na, 0);
         int bad = 0;
-#pragma omp parallel for schedule(static) num_threads(nt) if (na > 256) reduction(| : bad)
+#pragma omp parallel for schedule(static) num_threads(nt) if (na > 1 && (na > 256 || Gr.nreg > 2048)) reduction(| : bad)
         for (size_t a = 0; a < na; ++a) {
             auto &A = I[act[a]];
             const auto &pend = A.pending();
