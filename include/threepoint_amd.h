@@ -139,8 +139,9 @@ int tp_set_graph_rounds(tp_ctx *ctx, int on);
  * counterpart in the reference. */
 int tp_set_batch_groups(tp_ctx *ctx, int groups);
 /* Per-round trace of tp_map3_batch (diagnostics, off by default).  While on, every cubature round
- * of a call appends 7 doubles: group, regions, host launch begin and end, completion begin (after
- * the wait) and end [ms since the call began], and the time waited for the device [ms].  The trace
+ * of a call appends 9 doubles: group, regions, host launch begin and end, completion begin (after
+ * the wait) and end [ms since the call began], the time waited for the device [ms], and the
+ * round's device begin and end [ms since an event recorded when the call began].  The trace
  * is cleared at the start of each call; tp_round_log copies up to cap doubles and sets *n to the
  * number available.  No counterpart in the reference. */
 int tp_set_round_log(tp_ctx *ctx, int on);

@@ -305,7 +305,8 @@ def test_round_log_and_timing_stats(gpu_ctx, fid_tables):
         L = gpu_ctx.round_log()
     finally:
         gpu_ctx.set_round_log(False)
-    assert L.shape == (st["launches"], 7)
+    assert L.shape == (st["launches"], 9)
+    assert np.all(L[:, 8] >= L[:, 7]) and np.all(L[:, 7] >= 0)
     assert int(L[:, 1].sum()) == st["regions"] and st["points"] == 33 * st["regions"]
     assert set(np.unique(L[:, 0]).astype(int)) <= {0, 1}
     assert np.all(L[:, 3] >= L[:, 2]) and np.all(L[:, 5] >= L[:, 4]) and np.all(L[:, 4] >= L[:, 3])

@@ -217,12 +217,13 @@ class Context:
 
     def round_log(self):
         """Per-round trace of the last batched <Map^3> call: rows (group, regions, launch begin, launch end,
-        completion begin, completion end [ms since the call began], wait [ms])."""
+        completion begin, completion end [ms since the call began], wait [ms], device begin, device end [ms
+        since an event recorded when the call began])."""
         n = ctypes.c_size_t()
         _check(lib().tp_round_log(self.h, None, 0, ctypes.byref(n)), "tp_round_log")
         out = np.zeros(n.value)
         _check(lib().tp_round_log(self.h, _d(out), n.value, ctypes.byref(n)), "tp_round_log")
-        return out.reshape(-1, 7)
+        return out.reshape(-1, 9)
 
 
 _default_ctx = None

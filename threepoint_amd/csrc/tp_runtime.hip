@@ -608,7 +608,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         ctx->stats.host_ms += ms_since(th0);
         if (ctx->round_log_on)
             ctx->round_log.insert(ctx->round_log.end(),
-                                  {(double)g, (double)Gr.nreg, Gr.tl0, Gr.tl1, tc0, ms_since(t0), waited});
+                                  {(double)g, (double)Gr.nreg, Gr.tl0, Gr.tl1, tc0, ms_since(t0), waited, (double)ta, (double)tb});
         return 0;
     };
 
