@@ -289,7 +289,8 @@ int tp_map3_cov_gauss_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int 
  * times both uHat products, relative tolerance reltol (the reference's 1e-4), times 2 / area /
  * (2 pi)^3 (area in rad^2).  Pell carries the shape noise 0.5 sigma^2 / n_gal (n_gal in rad^-2);
  * flags: TP_PK_T17 (the cuda_version's default), TP_PK_SHAPENOISE_ONLY.  T2 vanishes for an
- * infinite survey (apertureStatisticsCovariance.cu:330-331); the finite geometries are not provided.
+ * infinite survey (apertureStatisticsCovariance.cu:330-331); T1 of the finite geometries is not
+ * provided (T2 of them: tp_map3_cov_t2_batch).
  * neval receives each pair's integrand points. */
 int tp_map3_cov_t1_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo, const double *theta6,
                          double area, double lmin, double sigma, double n_gal, int flags, double reltol,
@@ -300,6 +301,21 @@ int tp_map3_cov_t1_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *ic
 int tp_map3_cov_t1_batch_mode(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo, const double *theta6,
                               double area, double lmin, double sigma, double n_gal, int flags, double reltol,
                               int mode, double *out, long long *neval);
+
+/* Replaces T2_total (cuda_version/apertureStatisticsCovariance.cu:328-538) for n pairs at once:
+ * geometry 0 circle (diameter theta_max), 1 square (side theta_max), 3 rectangle (sides theta_max,
+ * theta_max2), all in rad; 2 infinite, where T2 = 0 (:330-331).  Each pair sums T2 over the pairings
+ * of theta_123 with theta_456, with the reference's multiplicities; each T2 (:1108-1163) is the
+ * product A1 A2 B of the 1-D integrals A of ell Pell uHat uHat / 2 pi (integrand :2014-2033, twice)
+ * and B, the window-weighted Pell uHat uHat integral: 1-D over ell for the circle (x 4 J1^2 / x^2,
+ * :2035-2058), 2-D over (ell_x, ell_y) for the square and the rectangle (x sinc^2 sinc^2,
+ * :2060-2110); every integral at relative tolerance reltol (the reference's 1e-4) with the
+ * reference's domains.  Pell, sigma, n_gal (rad^-2), lmin and flags as tp_map3_cov_t1_batch.  Equal
+ * integrals within the call are evaluated once; neval receives each pair's integrand points as the
+ * reference would evaluate them (every T2 call's three integrals). */
+int tp_map3_cov_t2_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo, const double *theta6,
+                         int geometry, double theta_max, double theta_max2, double lmin, double sigma, double n_gal,
+                         int flags, double reltol, double *out, long long *neval);
 
 /* ---------------------------------------------------------- input files */
 /* Replaces read_n_of_z: averaging = 0 the classed_version's reader (helper.cpp:31-96, linear

@@ -14,7 +14,8 @@
  *   - <Map^2>, T17 factor:   GV/apertureStatistics.cu:29-136, GV/bispectrum.cu:951-982
  *   - Gaussian covariance:   CV/apertureStatistics.cpp:50-159, 355-396, CV/bispectrum.cpp:580-607;
  *                            T1 (infinite survey) GV/apertureStatisticsCovariance.cu:1065-1106,
- *                            1958-1985
+ *                            1958-1985; T2 (circle, square, rectangle) :117-177, 328-538,
+ *                            1108-1163, 2014-2110 (J1 of the circle window: the C library's j1)
  *                            ("GV" = cuda_version/)
  *   - file readers:          CV/helper.cpp:31-96, CV/cosmology.cpp:10-72
  * GSL routines the reference calls are replaced by:
@@ -1455,6 +1456,112 @@ int t1_cb(unsigned ndim, size_t npts, const double *x, void *fdata, unsigned fdi
     return 0;
 }
 
+/* Survey window factors of the finite geometries (GV/apertureStatisticsCovariance.cu:117-177):
+   circle 4 J1(theta_max l)^2 / (theta_max l)^2; square and rectangle the product of the two squared
+   sinc factors of the half side lengths, 1 below |argument| 1e-6 */
+double g_circle(double ell, double tmax)
+{
+    double t = tmax * ell;
+    double r = j1(t);
+    r *= r;
+    r *= 4 / t / t;
+    return r;
+}
+
+double sinc_cut(double t) { return std::fabs(t) <= 1e-6 ? 1 : std::sin(t) / t; }
+
+double g_box(double lx, double ly, double tx, double ty)
+{
+    double a = sinc_cut(0.5 * lx * tx), b = sinc_cut(0.5 * ly * ty);
+    return a * a * b * b;
+}
+
+/* The four T2 integrands (GV/apertureStatisticsCovariance.cu:2014-2110): kind 0 integrand_T2_part1
+   (ell Pell uHat uHat, 1-D), 1 integrand_T2_part2_circle (1-D, x G_circle), 2 the square and 3 the
+   rectangle (2-D over (ell_x, ell_y), x G) */
+struct T2Ctx {
+    const Calc *c;
+    int kind;
+    double ta, tb, tmax, tmax2, lmin, sigma, ngal;
+    int flags, nthreads;
+};
+
+double t2_point(const T2Ctx &p, const double *x)
+{
+    if (p.kind <= 1) {
+        double ell = x[0];
+        if (ell < p.lmin) return 0;
+        if (p.kind == 0) {
+            double r = ell * p.c->pell_flags(ell, p.sigma, p.ngal, p.flags);
+            r *= Calc::uHat(ell * p.ta) * Calc::uHat(ell * p.tb);
+            return r;
+        }
+        double G = g_circle(ell, p.tmax);
+        double r = p.c->pell_flags(ell, p.sigma, p.ngal, p.flags);
+        r *= ell * Calc::uHat(ell * p.ta) * Calc::uHat(ell * p.tb);
+        r *= G;
+        return r;
+    }
+    double lx = x[0], ly = x[1];
+    double G = g_box(lx, ly, p.tmax, p.kind == 2 ? p.tmax : p.tmax2);
+    double ell = std::sqrt(lx * lx + ly * ly);
+    if (ell < p.lmin) return 0;
+    double r = p.c->pell_flags(ell, p.sigma, p.ngal, p.flags);
+    r *= Calc::uHat(ell * p.ta) * Calc::uHat(ell * p.tb);
+    r *= G;
+    return r;
+}
+
+int t2_cb(unsigned ndim, size_t npts, const double *x, void *fdata, unsigned fdim, double *fval)
+{
+    T2Ctx *p = (T2Ctx *)fdata;
+    if (fdim != 1 || ndim != (p->kind <= 1 ? 1u : 2u)) return 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(p->nthreads)
+    for (long long i = 0; i < (long long)npts; i++) fval[i] = t2_point(*p, x + ndim * i);
+    return 0;
+}
+
+/* The pairings of T2_total (GV/apertureStatisticsCovariance.cu:328-538), one side at a time: each
+   entry is (multiplicity, three indices into the side's radii).  For theta_123 the indices are
+   T2's arguments 1-3 (the two radii of one A factor, then the radius of B); for theta_456 its
+   arguments 4-6 (the radius of B, then the two of the other A factor).  The reference's table is
+   the product of the two lists, with the equality tests in its order.  The theta_1 == theta_3 list is
+   the reference's (:451-488): (theta_2, theta_1, theta_1) where the pairing sum has (theta_1,
+   theta_1, theta_2). */
+int t2_side_123(const double *t, int mult[3], int idx[3][3])
+{
+    static const int all[1][3] = {{0, 0, 0}}, e01[2][3] = {{0, 0, 2}, {0, 2, 0}},
+                     e12[2][3] = {{0, 1, 1}, {1, 1, 0}}, e02[2][3] = {{0, 1, 0}, {1, 0, 0}},
+                     dif[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}};
+    const int(*L)[3];
+    int n;
+    if (t[0] == t[1] && t[1] == t[2]) { L = all; n = 1; mult[0] = 3; }
+    else if (t[0] == t[1]) { L = e01; n = 2; mult[0] = 1; mult[1] = 2; }
+    else if (t[1] == t[2]) { L = e12; n = 2; mult[0] = 2; mult[1] = 1; }
+    else if (t[0] == t[2]) { L = e02; n = 2; mult[0] = 2; mult[1] = 1; }
+    else { L = dif; n = 3; mult[0] = mult[1] = mult[2] = 1; }
+    for (int k = 0; k < n; ++k)
+        for (int d = 0; d < 3; ++d) idx[k][d] = L[k][d];
+    return n;
+}
+
+int t2_side_456(const double *t, int mult[3], int idx[3][3])
+{
+    static const int all[1][3] = {{0, 0, 0}}, e01[2][3] = {{0, 0, 2}, {2, 0, 0}},
+                     e02[2][3] = {{0, 1, 0}, {1, 0, 0}}, e12[2][3] = {{1, 0, 1}, {0, 1, 1}},
+                     dif[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 0, 1}};
+    const int(*L)[3];
+    int n;
+    if (t[0] == t[1] && t[1] == t[2]) { L = all; n = 1; mult[0] = 3; }
+    else if (t[0] == t[1]) { L = e01; n = 2; mult[0] = 2; mult[1] = 1; }
+    else if (t[0] == t[2]) { L = e02; n = 2; mult[0] = 2; mult[1] = 1; }
+    else if (t[1] == t[2]) { L = e12; n = 2; mult[0] = 2; mult[1] = 1; }
+    else { L = dif; n = 3; mult[0] = mult[1] = mult[2] = 1; }
+    for (int k = 0; k < n; ++k)
+        for (int d = 0; d < 3; ++d) idx[k][d] = L[k][d];
+    return n;
+}
+
 int nthreads_or_default(int nt)
 {
 #ifdef _OPENMP
@@ -1854,6 +1961,78 @@ int orc_t1_total(void *h, const double *th123, const double *th456, double area,
         result += T1(th5, th3, th4);
         result += T1(th5, th4, th3);
     }
+    *val = result;
+    if (neval) *neval = ntot;
+    return rc;
+}
+
+int orc_t2(void *h, const double *th6, int geometry, double theta_max, double theta_max2, double lmin,
+           double sigma, double n_gal, int flags, double reltol, int nthreads, double *val, long long *neval)
+{ /* GV/apertureStatisticsCovariance.cu:1108-1163: A1 A2 B, each a hcubature at reltol; the second A
+     integral and the circle's B reuse the first integral's domain [lMin, 10 / min(theta_1..theta_5)];
+     the square's and the rectangle's B run over [-lMax, lMax]^2, lMax = 10 / min(theta_3, theta_4) */
+    Calc *c = (Calc *)h;
+    *val = 0;
+    if (neval) *neval = 0;
+    if (geometry == 2) return 0;
+    if (geometry != 0 && geometry != 1 && geometry != 3) return 1;
+    const int nt = nthreads_or_default(nthreads);
+    double lMax = 10. / std::min({th6[0], th6[1], th6[2], th6[3], th6[4]});
+    double vmin1[1] = {lmin}, vmax1[1] = {lMax};
+    long long ntot = 0;
+    int rc = 0;
+    auto run = [&](int kind, double ta, double tb, unsigned dim, const double *lo, const double *hi) {
+        T2Ctx p{c, kind, ta, tb, theta_max, theta_max2, lmin, sigma, n_gal, flags, nt};
+        double v = 0, e = 0;
+        rc |= hcubature_v_impl(1, t2_cb, &p, dim, lo, hi, 0, 0, reltol, ORC_ERROR_L1, &v, &e);
+        ntot += g_last_neval;
+        return v;
+    };
+    double A1 = run(0, th6[0], th6[1], 1, vmin1, vmax1);
+    A1 /= 2 * M_PI;
+    double A2 = run(0, th6[4], th6[5], 1, vmin1, vmax1);
+    A2 /= 2 * M_PI;
+    double B;
+    if (geometry == 0) {
+        B = run(1, th6[2], th6[3], 1, vmin1, vmax1);
+        B /= 2 * M_PI;
+    } else {
+        double l3 = 10. / std::min(th6[2], th6[3]);
+        double vmin2[2] = {-l3, -l3}, vmax2[2] = {l3, l3};
+        B = run(geometry == 1 ? 2 : 3, th6[2], th6[3], 2, vmin2, vmax2);
+        B /= std::pow(2 * M_PI, 2);
+    }
+    *val = A1 * A2 * B;
+    if (neval) *neval = ntot;
+    return rc;
+}
+
+int orc_t2_total(void *h, const double *th123, const double *th456, int geometry, double theta_max,
+                 double theta_max2, double lmin, double sigma, double n_gal, int flags, double reltol, int nthreads,
+                 double *val, long long *neval)
+{ /* GV/apertureStatisticsCovariance.cu:328-538 as the product of the two sides' pairing lists */
+    *val = 0;
+    if (neval) *neval = 0;
+    if (geometry == 2) return 0;
+    int ma[3], mb[3], ia[3][3], ib[3][3];
+    const int na = t2_side_123(th123, ma, ia), nb = t2_side_456(th456, mb, ib);
+    double result = 0;
+    long long ntot = 0;
+    int rc = 0;
+    bool first = true;
+    for (int a = 0; a < na; ++a)
+        for (int b = 0; b < nb; ++b) {
+            double th6[6] = {th123[ia[a][0]], th123[ia[a][1]], th123[ia[a][2]],
+                             th456[ib[b][0]], th456[ib[b][1]], th456[ib[b][2]]};
+            double v = 0;
+            long long n = 0;
+            rc |= orc_t2(h, th6, geometry, theta_max, theta_max2, lmin, sigma, n_gal, flags, reltol, nthreads, &v, &n);
+            const int m = ma[a] * mb[b];
+            const double tv = m == 1 ? v : m * v;
+            result = first ? tv : result + tv;
+            first = false;
+            ntot += n;
+        }
     *val = result;
     if (neval) *neval = ntot;
     return rc;

@@ -110,6 +110,15 @@ int orc_t1(void *h, const double *th6, double area, double lmin, double sigma, d
 int orc_t1_total(void *h, const double *th123, const double *th456, double area, double lmin, double sigma,
                  double n_gal, int flags, double reltol, int nthreads, double *val, long long *neval);
 
+/* T2 of the Gaussian <Map^3> covariance for the survey geometries of the cuda_version
+   (geometry 0 circle, 1 square, 2 infinite (T2 = 0), 3 rectangle; theta_max and theta_max2 the
+   side lengths or the diameter [rad]; flags as orc_t1) */
+int orc_t2(void *h, const double *th6, int geometry, double theta_max, double theta_max2, double lmin,
+           double sigma, double n_gal, int flags, double reltol, int nthreads, double *val, long long *neval);
+int orc_t2_total(void *h, const double *th123, const double *th456, int geometry, double theta_max,
+                 double theta_max2, double lmin, double sigma, double n_gal, int flags, double reltol, int nthreads,
+                 double *val, long long *neval);
+
 #ifdef __cplusplus
 }
 #endif

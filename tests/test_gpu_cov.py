@@ -1,4 +1,4 @@
-"""GPU parity of the Gaussian <Map^3> covariance against the CPU FP64 oracle.
+"""GPU parity of the Gaussian <Map^3> covariance (4-D form, T1, T2) against the CPU FP64 oracle.
 
 The adaptive integrals take the oracle's decision sequence (equal point counts); values agree to
 rounding (1e-10).  The shape-noise-only T1 is also checked against its closed form.
@@ -43,6 +43,32 @@ def test_t1_shape_noise_closed_form_on_gpu(fid_tables):
     P = 0.5 * 0.09 / NGAL
     for i, t in enumerate(th):
         assert abs(v[i] / (6 * t1_closed_form(t, AREA, P)) - 1) < 1e-6
+
+
+@pytest.mark.parametrize("flags", [tp.PK_T17, tp.PK_SHAPENOISE_ONLY])
+@pytest.mark.parametrize("geometry,tx,ty", [(tp.GEOMETRY_CIRCLE, 300 * A, 0.0), (tp.GEOMETRY_SQUARE, 300 * A, 0.0),
+                                            (tp.GEOMETRY_RECTANGLE, 300 * A, 180 * A)])
+def test_t2_total_matches_oracle(fid_tables, fid_oracle, geometry, tx, ty, flags):
+    """T2 of the finite geometries: the pairings (the third pair takes the reference's theta_1 == theta_3
+    branch), the 1-D and 2-D integrals and their domains; equal point counts, values to rounding."""
+    pairs = np.array([[4, 8, 8, 8, 8, 8], [6, 6, 12, 4, 8, 16], [8, 4, 8, 6, 12, 6]], dtype=float) * A
+    v, n = fid_tables.map3_cov_t2(pairs, geometry, tx, ty, 1.0, 0.3, NGAL, flags, reltol=1e-4)
+    for i, p in enumerate(pairs):
+        ov, on = fid_oracle.t2_total(p[:3], p[3:], geometry, tx, ty, 1.0, 0.3, NGAL, flags, 1e-4)
+        assert n[i] == on, (i, n[i], on)
+        assert ov > 0 and abs(v[i] / ov - 1) < 1e-10, (i, v[i], ov)
+
+
+def test_t2_infinite_survey_is_zero_and_bad_arguments(fid_tables):
+    pairs = np.full((2, 6), 4 * A)
+    v, n = fid_tables.map3_cov_t2(pairs, tp.GEOMETRY_INFINITE, 0.0, 0.0, 1.0, 0.3, NGAL, tp.PK_T17)
+    assert np.all(v == 0) and np.all(n == 0)
+    with pytest.raises(tp.TpError):
+        fid_tables.map3_cov_t2(pairs, 5, 300 * A)
+    with pytest.raises(tp.TpError):
+        fid_tables.map3_cov_t2(pairs, tp.GEOMETRY_RECTANGLE, 300 * A, 0.0)
+    with pytest.raises(tp.TpError):
+        fid_tables.map3_cov_t2(np.zeros((1, 6)), tp.GEOMETRY_CIRCLE, 300 * A)
 
 
 def test_cov_rejects_bad_arguments(fid_tables):

@@ -15,6 +15,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libthreepoint_amd.so")
 
 MODE_FP64, MODE_FP32, MODE_SURROGATE, MODE_ANALYTIC = 0, 1, 2, 3
+# survey geometries of the covariance (the cuda_version's type codes)
+GEOMETRY_CIRCLE, GEOMETRY_SQUARE, GEOMETRY_INFINITE, GEOMETRY_RECTANGLE = 0, 1, 2, 3
 PK_LINEAR, PK_T17, PK_SHAPENOISE_ONLY = 1, 2, 4  # flags of the power-spectrum calls
 ERROR_INDIVIDUAL, ERROR_PAIRED, ERROR_L2, ERROR_L1, ERROR_LINF = 0, 1, 2, 3, 4
 ARCMIN = 2.9088820866e-4  # classed_version/helper.cpp:139
@@ -102,6 +104,8 @@ _SIGS = {
     "tp_map3_cov_t1_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_int, ctypes.c_double, _dp, _llp]),
+    "tp_map3_cov_t2_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp, ctypes.c_int] +
+                             [ctypes.c_double] * 5 + [ctypes.c_int, ctypes.c_double, _dp, _llp]),
     "tp_map3_cov_t1_batch_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp,
                                                  ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                                  ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _llp]),
@@ -439,6 +443,20 @@ class CosmologyTables:
         _check(lib().tp_map3_cov_t1_batch_mode(self.ctx.h, self.h, n, ic.ctypes.data_as(_ip), _d(t), area, lmin,
                                                sigma, n_gal, flags, reltol, mode, _d(val), ne.ctypes.data_as(_llp)),
                "tp_map3_cov_t1_batch_mode")
+        return val, ne
+
+    def map3_cov_t2(self, theta6, geometry, theta_max, theta_max2=0.0, lmin=0.0, sigma=0.3, n_gal=1.0, flags=0,
+                    icosmo=None, reltol=1e-4):
+        """T2_total of the cuda_version's covariance, pairs theta6[n, 6] [rad]; geometry GEOMETRY_CIRCLE
+        (theta_max the diameter), GEOMETRY_SQUARE (the side), GEOMETRY_RECTANGLE (sides theta_max,
+        theta_max2) or GEOMETRY_INFINITE (T2 = 0)."""
+        t = np.ascontiguousarray(theta6, dtype=np.float64).reshape(-1, 6)
+        n = t.shape[0]
+        ic = np.zeros(n, dtype=np.int32) if icosmo is None else np.ascontiguousarray(icosmo, dtype=np.int32)
+        val, ne = np.zeros(n), np.zeros(n, dtype=np.int64)
+        _check(lib().tp_map3_cov_t2_batch(self.ctx.h, self.h, n, ic.ctypes.data_as(_ip), _d(t), int(geometry),
+                                          theta_max, theta_max2, lmin, sigma, n_gal, flags, reltol, _d(val),
+                                          ne.ctypes.data_as(_llp)), "tp_map3_cov_t2_batch")
         return val, ne
 
 

@@ -22,6 +22,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "tp_cubature.h"
@@ -30,7 +32,11 @@
 namespace {
 
 // What one integral (or one evaluation call) computes at its points.
-enum PowerKind { kPkappa = 0, kMap2 = 1, kPkAtZ = 2, kLimber = 3, kCovGauss = 4, kT1 = 5 };
+enum PowerKind {
+    kPkappa = 0, kMap2 = 1, kPkAtZ = 2, kLimber = 3, kCovGauss = 4, kT1 = 5,
+    // the T2 integrals of a finite survey (1-D: A and the circle's B; 2-D: the square's and rectangle's B)
+    kT2A = 6, kT2Circle = 7, kT2Square = 8, kT2Rect = 9
+};
 
 struct TpPowerDesc {
     double par;          // kPkappa: ell; kMap2: theta [rad]
@@ -39,8 +45,9 @@ struct TpPowerDesc {
     int kind;
     int flags;           // TP_PK_LINEAR | TP_PK_T17 (| TP_PK_SHAPENOISE_ONLY for kT1)
     int pad;
-    double th[6];        // kCovGauss, kT1: theta_123, theta_456 [rad]
-    double lmin, lmax;   // kT1: the integrand's ell cut
+    double th[6];        // kCovGauss, kT1: theta_123, theta_456 [rad]; kT2*: the two radii, the survey's
+                         // side lengths or diameter (theta_max, theta_max_smaller) [rad]
+    double lmin, lmax;   // kT1, kT2*: the integrand's ell cut
 };
 
 // 96-point Gauss-Legendre half nodes/weights (own Newton solve, uploaded once per device)
@@ -124,11 +131,59 @@ __device__ double covg_point(const TpCosmoConst &C, const double *T, int n_z, do
 }
 
 
+// Survey window factors (cuda_version/apertureStatisticsCovariance.cu:117-177): the circle's
+// 4 J1(theta_max l)^2 / (theta_max l)^2, and the squared sinc factors of the half side lengths of the
+// square and the rectangle (1 below |argument| 1e-6).
+__device__ __forceinline__ double g_circle(double ell, double tmax)
+{
+#pragma clang fp contract(off)
+    const double t = tmax * ell;
+    double r = j1(t);
+    r *= r;
+    r *= 4 / t / t;
+    return r;
+}
+
+__device__ __forceinline__ double sinc_cut(double t) { return fabs(t) <= 1e-6 ? 1.0 : sin(t) / t; }
+
+// The T2 integrands at one point (apertureStatisticsCovariance.cu:2014-2110), in the reference's
+// product order: kT2A integrand_T2_part1 (ell Pell uHat uHat), kT2Circle integrand_T2_part2_circle,
+// kT2Square / kT2Rect integrand_T2_part2_square / _rectangle over (ell_x, ell_y).
+__device__ double t2_point(const TpCosmoConst &C, const TpPNode *__restrict__ N, const TpPowerDesc &D,
+                           const double *x, double dx)
+{
+#pragma clang fp contract(off)
+    const double shot = 0.5 * D.sigma * D.sigma / D.ngal;
+    if (D.kind == kT2A || D.kind == kT2Circle) {
+        const double ell = x[0];
+        if (ell < D.lmin) return 0;
+        if (D.kind == kT2A) {
+            double r = ell * pell_nodes(C, N, ell, D.flags, dx, shot);
+            r *= uhat(ell * D.th[0]) * uhat(ell * D.th[1]);
+            return r;
+        }
+        const double G = g_circle(ell, D.th[2]);
+        double r = pell_nodes(C, N, ell, D.flags, dx, shot);
+        r *= ell * uhat(ell * D.th[0]) * uhat(ell * D.th[1]);
+        r *= G;
+        return r;
+    }
+    const double lx = x[0], ly = x[1];
+    const double a = sinc_cut(0.5 * lx * D.th[2]), b = sinc_cut(0.5 * ly * (D.kind == kT2Square ? D.th[2] : D.th[3]));
+    const double G = a * a * b * b;
+    const double ell = sqrt(lx * lx + ly * ly);
+    if (ell < D.lmin) return 0;
+    double r = pell_nodes(C, N, ell, D.flags, dx, shot);
+    r *= uhat(ell * D.th[0]) * uhat(ell * D.th[1]);
+    r *= G;
+    return r;
+}
+
 // One thread per point; pdesc[i] names the integral (desc entry) of point i.
 __global__ void k_power_points(const TpCosmoConst *__restrict__ cc, const double *__restrict__ tab, int n_z,
                                const TpPowerDesc *__restrict__ desc,
                                const int *__restrict__ pdesc, size_t npts, int dim, const double *__restrict__ x,
-                               double *__restrict__ f, int *flag)
+                               double *__restrict__ f, int *flag, const TpPNode *__restrict__ pnode, double dx)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npts) return;
@@ -147,6 +202,8 @@ __global__ void k_power_points(const TpCosmoConst *__restrict__ cc, const double
         TpZAtT<double> Z;
         tp_z_at<double>(T, n_z, C.zmax, xi[1], &Z);
         v = tp_pk_at<double>(C, Z, xi[0], xi[1], D.flags);
+    } else if (D.kind >= kT2A) {
+        v = t2_point(C, pnode + (size_t)D.icosmo * TP_NNODES, D, xi, dx);
     } else {
         v = covg_point(C, T, n_z, xi[0], xi[1], xi[2], xi[3], D.th);
     }
@@ -422,7 +479,7 @@ int power_launch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> 
     hipLaunchKernelGGL(k_power_points, dim3((unsigned)((npts + blk - 1) / blk)), dim3(blk), 0, ctx->stream,
                        c->d_const, c->d_tab, c->n_z, (const TpPowerDesc *)ctx->d_desc.p,
                        pdesc ? (const int *)ctx->d_misc.p : nullptr, npts, dim, (const double *)ctx->d_in.p,
-                       (double *)ctx->d_out.p, (int *)ctx->d_flag.p);
+                       (double *)ctx->d_out.p, (int *)ctx->d_flag.p, (const TpPNode *)c->d_pnode, c->z_max / 2);
     TP_HIP(hipGetLastError());
     TP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     TP_HIP(hipMemcpyAsync(ctx->h_out.p, ctx->d_out.p, bf, hipMemcpyDeviceToHost, ctx->stream));
@@ -1042,6 +1099,166 @@ extern "C" int tp_map3_cov_t1_batch_mode(tp_ctx *ctx, const tp_cosmo *c, size_t 
             const double tq = terms[i][q].first == 1 ? v[terms[i][q].second] : terms[i][q].first * v[terms[i][q].second];
             r = q == 0 ? tq : r + tq;
             np += ne[terms[i][q].second];
+        }
+        out[i] = r;
+        if (neval) neval[i] = np;
+    }
+    return 0;
+}
+
+// The pairings of T2_total (apertureStatisticsCovariance.cu:328-538), one side at a time: (multiplicity,
+// three indices into the side's radii).  For theta_123 the indices give T2's arguments 1-3 (the radii of
+// one A factor, then the radius of B), for theta_456 its arguments 4-6 (the radius of B, then the radii
+// of the other A factor); the reference's table is the product of the two lists, with the equality
+// tests in its order.  Its theta_1 == theta_3 list (:451-488) is kept as it is: (theta_1, theta_2,
+// theta_1) twice and (theta_2, theta_1, theta_1) once, so theta_2 never enters B, where the pairing
+// sum has (theta_1, theta_1, theta_2) for the second entry.
+static int t2_side(const double *t, bool first, int mult[3], int idx[3][3])
+{
+    static const int all[1][3] = {{0, 0, 0}};
+    static const int a01[2][3] = {{0, 0, 2}, {0, 2, 0}}, a12[2][3] = {{0, 1, 1}, {1, 1, 0}},
+                     a02[2][3] = {{0, 1, 0}, {1, 0, 0}}, adif[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}};
+    static const int b01[2][3] = {{0, 0, 2}, {2, 0, 0}}, b02[2][3] = {{0, 1, 0}, {1, 0, 0}},
+                     b12[2][3] = {{1, 0, 1}, {0, 1, 1}}, bdif[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 0, 1}};
+    const int(*L)[3];
+    int n;
+    if (t[0] == t[1] && t[1] == t[2]) {
+        L = all, n = 1, mult[0] = 3;
+    } else if (t[0] == t[1]) {
+        L = first ? a01 : b01, n = 2;
+        mult[0] = first ? 1 : 2, mult[1] = first ? 2 : 1;
+    } else if (first ? t[1] == t[2] : t[0] == t[2]) {
+        L = first ? a12 : b02, n = 2, mult[0] = 2, mult[1] = 1;
+    } else if (first ? t[0] == t[2] : t[1] == t[2]) {
+        L = first ? a02 : b12, n = 2, mult[0] = 2, mult[1] = 1;
+    } else {
+        L = first ? adif : bdif, n = 3, mult[0] = mult[1] = mult[2] = 1;
+    }
+    for (int k = 0; k < n; ++k)
+        for (int d = 0; d < 3; ++d) idx[k][d] = L[k][d];
+    return n;
+}
+
+extern "C" int tp_map3_cov_t2_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo,
+                                    const double *theta6, int geometry, double theta_max, double theta_max2,
+                                    double lmin, double sigma, double n_gal, int flags, double reltol, double *out,
+                                    long long *neval)
+{
+    int rc = check_args(ctx, c, flags);
+    if (rc) return rc;
+    if (n && (!theta6 || !out)) return tp_set_error(TP_EINVAL, "tp_map3_cov_t2_batch: NULL array");
+    if (geometry < 0 || geometry > 3)
+        return tp_set_error(TP_EINVAL, "tp_map3_cov_t2_batch: geometry must be 0 (circle), 1 (square), 2 (infinite) "
+                                       "or 3 (rectangle)");
+    if (!(reltol > 0) || !(n_gal > 0) || !(lmin >= 0))
+        return tp_set_error(TP_EINVAL, "tp_map3_cov_t2_batch: reltol and n_gal must be positive, lmin >= 0");
+    if (geometry != 2 && (!(theta_max > 0) || (geometry == 3 && !(theta_max2 > 0))))
+        return tp_set_error(TP_EINVAL, "tp_map3_cov_t2_batch: the survey's side lengths or diameter must be positive");
+    if (n > 0x7fffffffu / 81) return tp_set_error(TP_EINVAL, "tp_map3_cov_t2_batch: too many pairs for one call");
+    for (size_t i = 0; i < n; ++i) {
+        const int ic = icosmo ? icosmo[i] : 0;
+        if (ic < 0 || ic >= c->ncosmo) return tp_set_error(TP_EINVAL, "tp_map3_cov_t2_batch: cosmology index out of range");
+        for (int k = 0; k < 6; ++k)
+            if (!(theta6[6 * i + k] > 0)) return tp_set_error(TP_EINVAL, "tp_map3_cov_t2_batch: aperture radii must be positive");
+    }
+    if (geometry == 2) {  // T2 vanishes for an infinite survey (apertureStatisticsCovariance.cu:330-331)
+        for (size_t i = 0; i < n; ++i) {
+            out[i] = 0;
+            if (neval) neval[i] = 0;
+        }
+        return 0;
+    }
+    TP_HIP(hipSetDevice(ctx->device));
+    // Every T2 call of the reference (apertureStatisticsCovariance.cu:1108-1163) is A1 A2 B: A1 and A2 over
+    // [lMin, lMax], lMax = 10 / min(theta_1..theta_5) (the second A reuses the first one's domain), B over
+    // the same domain for the circle, over [-lMax', lMax']^2, lMax' = 10 / min(theta_3, theta_4), for the
+    // square and the rectangle.  Equal integrals (same kind, radii and domain) are evaluated once.
+    using Key = std::tuple<int, int, double, double, double, double>;
+    std::map<Key, size_t> ids[2];
+    std::vector<TpPowerDesc> desc[2];
+    std::vector<double> lo[2], hi[2];
+    auto integral = [&](int kind, int ic, double ta, double tb, double l, double h) -> size_t {
+        const int d = kind == kT2Square || kind == kT2Rect;
+        const Key k{kind, ic, ta, tb, l, h};
+        auto it = ids[d].find(k);
+        if (it != ids[d].end()) return it->second;
+        TpPowerDesc D{};
+        D.icosmo = ic;
+        D.kind = kind;
+        D.flags = flags;
+        D.sigma = sigma;
+        D.ngal = n_gal;
+        D.th[0] = ta;
+        D.th[1] = tb;
+        D.th[2] = theta_max;
+        D.th[3] = theta_max2;
+        D.lmin = lmin;
+        const size_t id = desc[d].size();
+        desc[d].push_back(D);
+        for (int q = 0; q <= d; ++q) {
+            lo[d].push_back(l);
+            hi[d].push_back(h);
+        }
+        ids[d].emplace(k, id);
+        return id;
+    };
+    struct Term {
+        int mult;
+        size_t a1, a2, b;  // a1, a2 index the 1-D integrals; b the 1-D (circle) or the 2-D ones
+    };
+    std::vector<std::vector<Term>> terms(n);
+    const int kb = geometry == 0 ? kT2Circle : geometry == 1 ? kT2Square : kT2Rect;
+    for (size_t i = 0; i < n; ++i) {
+        const int ic = icosmo ? icosmo[i] : 0;
+        const double *t = theta6 + 6 * i;
+        int ma[3], mb[3], ia[3][3], ib[3][3];
+        const int na = t2_side(t, true, ma, ia), nb = t2_side(t + 3, false, mb, ib);
+        for (int a = 0; a < na; ++a)
+            for (int b = 0; b < nb; ++b) {
+                const double u[6] = {t[ia[a][0]], t[ia[a][1]], t[ia[a][2]], t[3 + ib[b][0]], t[3 + ib[b][1]],
+                                     t[3 + ib[b][2]]};
+                const double lmax = 10. / std::min({u[0], u[1], u[2], u[3], u[4]});
+                Term T;
+                T.mult = ma[a] * mb[b];
+                T.a1 = integral(kT2A, ic, u[0], u[1], lmin, lmax);
+                T.a2 = integral(kT2A, ic, u[4], u[5], lmin, lmax);
+                if (kb == kT2Circle) {
+                    T.b = integral(kb, ic, u[2], u[3], lmin, lmax);
+                } else {
+                    const double l3 = 10. / std::min(u[2], u[3]);
+                    T.b = integral(kb, ic, u[2], u[3], -l3, l3);
+                }
+                terms[i].push_back(T);
+            }
+    }
+    std::vector<double> v1(desc[0].size()), v2(desc[1].size());
+    std::vector<long long> n1(desc[0].size()), n2(desc[1].size());
+    auto ident = [](double v) { return v; };
+    if ((rc = power_batch<1>(ctx, c, desc[0], lo[0].data(), hi[0].data(), reltol, ident, v1.data(), nullptr,
+                             n1.data())))
+        return rc;
+    if (!desc[1].empty() && (rc = power_batch<2>(ctx, c, desc[1], lo[1].data(), hi[1].data(), reltol, ident,
+                                                 v2.data(), nullptr, n2.data())))
+        return rc;
+    for (size_t i = 0; i < n; ++i) {
+        double r = 0;
+        long long np = 0;
+        for (size_t q = 0; q < terms[i].size(); ++q) {
+            const Term &T = terms[i][q];
+            double A1 = v1[T.a1], A2 = v1[T.a2], B;
+            A1 /= 2 * tpd::kPi;
+            A2 /= 2 * tpd::kPi;
+            if (kb == kT2Circle) {
+                B = v1[T.b];
+                B /= 2 * tpd::kPi;
+            } else {
+                B = v2[T.b];
+                B /= std::pow(2 * tpd::kPi, 2);
+            }
+            const double t2 = A1 * A2 * B;
+            const double tq = T.mult == 1 ? t2 : T.mult * t2;
+            r = q == 0 ? tq : r + tq;
+            np += n1[T.a1] + n1[T.a2] + (kb == kT2Circle ? n1[T.b] : n2[T.b]);
         }
         out[i] = r;
         if (neval) neval[i] = np;
