@@ -167,9 +167,11 @@ class ApertureStatistics {
 };
 
 /* The Gaussian <Map^3> covariance terms of the cuda_version (apertureStatisticsCovariance.cu), for
- * the survey of a covarianceParameters file.  Only the infinite geometry ("infinite") is provided:
- * T1 as the reference integrates it, T2 = 0 as the reference returns for it.  Pell carries the
- * parameters' shape noise; t17 selects the cuda_version's T17 spectrum (its default). */
+ * the survey of a covarianceParameters file and a geometry "circle", "square", "infinite" or
+ * "rectangular" (:109-124 of its executable).  T1: the infinite survey (T1_total of another
+ * geometry throws std::runtime_error).  T2: every geometry (0 for the infinite one, as the reference
+ * returns), with the survey's thetaMax and thetaMax_smaller.  Pell carries the parameters' shape
+ * noise; t17 selects the cuda_version's T17 spectrum (its default). */
 class ApertureStatisticsCovariance {
   public:
     ApertureStatisticsCovariance(BispectrumCalculator *Bispectrum, const covarianceParameters &covPar,
@@ -178,6 +180,7 @@ class ApertureStatisticsCovariance {
     double T2_total(const std::vector<double> &thetas_123, const std::vector<double> &thetas_456);
     /* many pairs in one batched run: pairs[i] = {theta_123, theta_456} */
     std::vector<double> T1_total(const std::vector<std::array<double, 6>> &pairs);
+    std::vector<double> T2_total(const std::vector<std::array<double, 6>> &pairs);
     double lMin = 0;        /* the reference's executable sets lMin = 0 */
     double reltol = 1e-4;   /* apertureStatisticsCovariance.cu:1088 */
     int mode = TP_MODE_FP64;/* or TP_MODE_FP32: FP32 Pell node values, FP64 sums (not in the reference) */
@@ -186,6 +189,7 @@ class ApertureStatisticsCovariance {
     BispectrumCalculator *Bispectrum_;
     covarianceParameters covPar_;
     int flags_;
+    int geometry_;  /* the reference's type code: 0 circle, 1 square, 2 infinite, 3 rectangular */
 };
 
 class GammaCalculator {

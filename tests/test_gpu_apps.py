@@ -128,7 +128,8 @@ def test_calculate_gamma_triangle_list(tmp_path, fid_tables):
 
 
 def test_calculate_covariance_cli(tmp_path, fid_oracle):
-    """T1 (infinite survey) for the 4 triples of radii {4, 8}: the 4 x 4 matrix against the oracle's T1_total."""
+    """T1 (infinite survey) for the 4 triples of radii {4, 8}: the 4 x 4 matrix against the oracle's T1_total;
+    T2 of a circular survey against the oracle's T2_total."""
     cf, nf = _inputs(tmp_path)
     tf = tmp_path / "thetas.dat"
     tf.write_text("4\n8\n")
@@ -162,6 +163,20 @@ def test_calculate_covariance_cli(tmp_path, fid_oracle):
     assert r.returncode == 0, r.stderr
     t1f = np.loadtxt(fp32_dir / (name % (1, n_deg, 7.87 * deg * 57.3)))
     assert t1f.shape == (4, 4) and np.max(np.abs(t1f / t1 - 1)) < 1e-3
+    # a circular survey (diameter thetaMax): T2 against the oracle's T2_total; T1 of it is not provided
+    circ = tmp_path / "circle"
+    circ.mkdir()
+    argc = args[:4] + [str(circ) + "/"] + args[5:6] + ["0", "1"] + args[8:13] + ["circle"] + args[14:]
+    r = subprocess.run(argc, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    t2c = np.loadtxt(circ / (name.replace("infinite", "circle") % (2, n_deg, 7.87 * deg * 57.3)))
+    assert t2c.shape == (4, 4) and np.array_equal(t2c, t2c.T)
+    for i in range(4):
+        for j in range(i, 4):
+            ov, _ = fid_oracle.t2_total(tri[i], tri[j], 0, 7.87 * deg, 0.0, 0.0, 0.37, 108000 / deg / deg, 2, 1e-3)
+            assert ov > 0 and abs(t2c[i, j] / ov - 1) < 1e-5
+    r = subprocess.run(argc[:6] + ["1"] + argc[7:], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1  # T1 of a circular survey: not provided
 
 
 def test_c3_batch_subset_matches_oracle(gpu_ctx, fid_nz):

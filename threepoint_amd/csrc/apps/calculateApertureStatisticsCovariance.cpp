@@ -5,11 +5,12 @@
 //       <covariance parameters> <T1 0|1> <T2 0|1> <T4 0|1> <T5 0|1> <T6 0|1> <T7 0|1> <T7_2h 0|1> <geometry>
 //       [--nz-bins N] [--reltol X] [--no-t17] [--fp32]
 //
-// The same 13 arguments as the reference.  Provided: T1 and T2 for the infinite survey geometry (T2 is
-// 0 there, as in the reference); requesting T4-T7 or another geometry is an error (exit status 1).
-// For every pair i <= j of the theta_1 <= theta_2 <= theta_3 triples of the list it computes
-// T1_total, all pairs in one batched GPU pass, and writes the symmetric N x N matrix to
-// <folder>cov_<geometry>_term1Numerical_sigma_<s>_n_<n>_thetaMax_<t>_gpu.dat as the reference does
+// The same 13 arguments as the reference.  Provided: T1 for the infinite survey geometry, T2 for every
+// geometry (circle, square, rectangular; 0 for the infinite one, as in the reference); requesting
+// T4-T7, or T1 of a finite geometry, is an error (exit status 1).  For every pair i <= j of the
+// theta_1 <= theta_2 <= theta_3 triples of the list it computes T1_total and T2_total, all pairs of a
+// term in one batched GPU pass, and writes each symmetric N x N matrix to
+// <folder>cov_<geometry>_term<k>Numerical_sigma_<s>_n_<n>_thetaMax_<t>_gpu.dat as the reference does
 // (apertureStatisticsCovariance.cu:46-86; the folder string is prefixed as given).
 #include <array>
 #include <cstdio>
@@ -68,8 +69,13 @@ int main(int argc, char *argv[])
             return 1;
         }
     const std::string type = argv[13];
-    if (type != "infinite") {
-        std::cerr << "survey geometry " << type << " is not provided by this build (infinite only)" << std::endl;
+    if (type != "circle" && type != "square" && type != "infinite" && type != "rectangular") {
+        std::cerr << "Cov type not correctly specified" << std::endl;
+        return 1;
+    }
+    if (do_t1 && type != "infinite") {
+        std::cerr << "T1 of the survey geometry " << type << " is not provided by this build (infinite only)"
+                  << std::endl;
         return 1;
     }
     try {
@@ -109,7 +115,7 @@ int main(int argc, char *argv[])
             write_cov(t1, N_ind, folder + fn);
         }
         if (do_t2) {
-            std::vector<double> t2(pairs.size(), 0.0);  // infinite survey: T2_total = 0
+            std::vector<double> t2 = cov.T2_total(pairs);
             std::snprintf(fn, sizeof(fn), "cov_%s_term2Numerical_sigma_%.2f_n_%.2f_thetaMax_%.2f_gpu.dat", type.c_str(),
                           covPar.shapenoise_sigma, n_deg, thetaMax_deg);
             std::cerr << "Writing Term2 to " << folder + fn << std::endl;

@@ -250,13 +250,19 @@ ApertureStatisticsCovariance::ApertureStatisticsCovariance(BispectrumCalculator 
                                                            const std::string &type, bool t17)
     : Bispectrum_(b), covPar_(c), flags_(t17 ? TP_PK_T17 : 0)
 {
-    if (type != "infinite")
-        throw std::runtime_error("ApertureStatisticsCovariance: only the infinite survey geometry is provided");
+    // cuda_version/calculateApertureStatisticsCovariance.cu:109-129
+    if (type == "circle") geometry_ = 0;
+    else if (type == "square") geometry_ = 1;
+    else if (type == "infinite") geometry_ = 2;
+    else if (type == "rectangular") geometry_ = 3;
+    else throw std::runtime_error("ApertureStatisticsCovariance: Cov type not correctly specified");
     if (covPar_.shapenoiseOnly) flags_ |= TP_PK_SHAPENOISE_ONLY;
 }
 
 std::vector<double> ApertureStatisticsCovariance::T1_total(const std::vector<std::array<double, 6>> &pairs)
 {
+    if (geometry_ != 2)
+        throw std::runtime_error("T1_total: only the infinite survey geometry is provided for T1");
     std::vector<double> th, out(pairs.size());
     for (auto &p : pairs) th.insert(th.end(), p.begin(), p.end());
     check(tp_map3_cov_t1_batch_mode(Bispectrum_->ctx(), Bispectrum_->tables(), pairs.size(), nullptr, th.data(),
@@ -272,9 +278,19 @@ double ApertureStatisticsCovariance::T1_total(const std::vector<double> &a, cons
     return T1_total(std::vector<std::array<double, 6>>{{a[0], a[1], a[2], b[0], b[1], b[2]}})[0];
 }
 
+std::vector<double> ApertureStatisticsCovariance::T2_total(const std::vector<std::array<double, 6>> &pairs)
+{
+    std::vector<double> th, out(pairs.size());
+    for (auto &p : pairs) th.insert(th.end(), p.begin(), p.end());
+    check(tp_map3_cov_t2_batch(Bispectrum_->ctx(), Bispectrum_->tables(), pairs.size(), nullptr, th.data(), geometry_,
+                               covPar_.thetaMax, covPar_.thetaMax_smaller, lMin, covPar_.shapenoise_sigma,
+                               covPar_.galaxy_density, flags_, reltol, out.data(), nullptr),
+          "tp_map3_cov_t2_batch");
+    return out;
+}
+
 double ApertureStatisticsCovariance::T2_total(const std::vector<double> &a, const std::vector<double> &b)
 {
-    // apertureStatisticsCovariance.cu:328-331: T2 vanishes for an infinite survey
     if (a.size() != 3 || b.size() != 3) throw std::invalid_argument("T2_total: Wrong number of aperture radii");
-    return 0;
+    return T2_total(std::vector<std::array<double, 6>>{{a[0], a[1], a[2], b[0], b[1], b[2]}})[0];
 }
