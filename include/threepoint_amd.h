@@ -289,8 +289,8 @@ int tp_map3_cov_gauss_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int 
  * times both uHat products, relative tolerance reltol (the reference's 1e-4), times 2 / area /
  * (2 pi)^3 (area in rad^2).  Pell carries the shape noise 0.5 sigma^2 / n_gal (n_gal in rad^-2);
  * flags: TP_PK_T17 (the cuda_version's default), TP_PK_SHAPENOISE_ONLY.  T2 vanishes for an
- * infinite survey (apertureStatisticsCovariance.cu:330-331); T1 of the finite geometries is not
- * provided (T2 of them: tp_map3_cov_t2_batch).
+ * infinite survey (apertureStatisticsCovariance.cu:330-331).  The finite geometries:
+ * tp_map3_cov_t1_batch_geometry and tp_map3_cov_t2_batch.
  * neval receives each pair's integrand points. */
 int tp_map3_cov_t1_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo, const double *theta6,
                          double area, double lmin, double sigma, double n_gal, int flags, double reltol,
@@ -302,8 +302,20 @@ int tp_map3_cov_t1_batch_mode(tp_ctx *ctx, const tp_cosmo *c, size_t n, const in
                               double area, double lmin, double sigma, double n_gal, int flags, double reltol,
                               int mode, double *out, long long *neval);
 
+/* Replaces T1_total (cuda_version/apertureStatisticsCovariance.cu:272-326) for every survey geometry
+ * (the reference's type codes: 0 circle of radius theta_max, 1 square of side theta_max, 2 infinite,
+ * 3 rectangle of sides theta_max, theta_max2; rad).  Geometry 2 is tp_map3_cov_t1_batch (area in
+ * rad^2).  The finite ones (T1 :1065-1102) are 6-D hcubatures over [-lMax, 1.02 lMax]^6, lMax = 10 /
+ * max(min theta_123, min theta_456), of Pell(l1) Pell(l2) Pell(l3), both uHat products and the
+ * window G (integrands :1886-1955, 1984-2012), relative tolerance reltol (the reference's 1e-2),
+ * divided by (2 pi)^6; area is not used.  Pell, sigma, n_gal, lmin and flags as
+ * tp_map3_cov_t1_batch; neval receives each pair's integrand points. */
+int tp_map3_cov_t1_batch_geometry(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo, const double *theta6,
+                                  int geometry, double theta_max, double theta_max2, double area, double lmin,
+                                  double sigma, double n_gal, int flags, double reltol, double *out,
+                                  long long *neval);
 /* Replaces T2_total (cuda_version/apertureStatisticsCovariance.cu:328-538) for n pairs at once:
- * geometry 0 circle (diameter theta_max), 1 square (side theta_max), 3 rectangle (sides theta_max,
+ * geometry 0 circle (radius theta_max), 1 square (side theta_max), 3 rectangle (sides theta_max,
  * theta_max2), all in rad; 2 infinite, where T2 = 0 (:330-331).  Each pair sums T2 over the pairings
  * of theta_123 with theta_456, with the reference's multiplicities; each T2 (:1108-1163) is the
  * product A1 A2 B of the 1-D integrals A of ell Pell uHat uHat / 2 pi (integrand :2014-2033, twice)

@@ -168,9 +168,9 @@ class ApertureStatistics {
 
 /* The Gaussian <Map^3> covariance terms of the cuda_version (apertureStatisticsCovariance.cu), for
  * the survey of a covarianceParameters file and a geometry "circle", "square", "infinite" or
- * "rectangular" (:109-124 of its executable).  T1: the infinite survey (T1_total of another
- * geometry throws std::runtime_error).  T2: every geometry (0 for the infinite one, as the reference
- * returns), with the survey's thetaMax and thetaMax_smaller.  Pell carries the parameters' shape
+ * "rectangular" (:109-124 of its executable), with the survey's thetaMax and thetaMax_smaller.  T1:
+ * 3-D for the infinite survey, 6-D with the window for the others; T2: 0 for the infinite one, as
+ * the reference returns, the window integrals for the others.  Pell carries the parameters' shape
  * noise; t17 selects the cuda_version's T17 spectrum (its default). */
 class ApertureStatisticsCovariance {
   public:
@@ -182,7 +182,8 @@ class ApertureStatisticsCovariance {
     std::vector<double> T1_total(const std::vector<std::array<double, 6>> &pairs);
     std::vector<double> T2_total(const std::vector<std::array<double, 6>> &pairs);
     double lMin = 0;        /* the reference's executable sets lMin = 0 */
-    double reltol = 1e-4;   /* apertureStatisticsCovariance.cu:1088 */
+    double reltol = 1e-4;   /* apertureStatisticsCovariance.cu:1088 (T1 infinite, T2) */
+    double reltol_finite = 1e-2; /* :1096 (T1 of the finite geometries) */
     int mode = TP_MODE_FP64;/* or TP_MODE_FP32: FP32 Pell node values, FP64 sums (not in the reference) */
 
   private:

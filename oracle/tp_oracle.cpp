@@ -14,8 +14,9 @@
  *   - <Map^2>, T17 factor:   GV/apertureStatistics.cu:29-136, GV/bispectrum.cu:951-982
  *   - Gaussian covariance:   CV/apertureStatistics.cpp:50-159, 355-396, CV/bispectrum.cpp:580-607;
  *                            T1 (infinite survey) GV/apertureStatisticsCovariance.cu:1065-1106,
- *                            1958-1985; T2 (circle, square, rectangle) :117-177, 328-538,
- *                            1108-1163, 2014-2110 (J1 of the circle window: the C library's j1)
+ *                            1958-1985; T1 and T2 (circle, square, rectangle) :117-177, 328-538,
+ *                            1065-1163, 1886-1955, 1984-2110 (J1 of the circle window: the C
+ *                            library's j1)
  *                            ("GV" = cuda_version/)
  *   - file readers:          CV/helper.cpp:31-96, CV/cosmology.cpp:10-72
  * GSL routines the reference calls are replaced by:
@@ -1562,6 +1563,49 @@ int t2_side_456(const double *t, int mult[3], int idx[3][3])
     return n;
 }
 
+/* T1 of a finite survey (GV/apertureStatisticsCovariance.cu:1886-1955, 1984-2012): 6-D over
+   (a, b) the window's wave vector and (c, d), (e, f) two of the three Pell wave vectors,
+   l1 = |(a - c - e, b - d - f)|, l2 = |(c, d)|, l3 = |(e, f)|; the circle cuts l_i <= lMin, the square
+   and the rectangle l_i <= 0 and l3 > lMax */
+struct T1FinCtx {
+    const Calc *c;
+    int geometry;
+    double th[6], tmax, tmax2, lmin, lmax, sigma, ngal;
+    int flags, nthreads;
+};
+
+double t1fin_point(const T1FinCtx &p, const double *x)
+{
+    double a = x[0], b = x[1], c = x[2], d = x[3], e = x[4], f = x[5];
+    double G;
+    if (p.geometry == 0) G = g_circle(std::sqrt(a * a + b * b), p.tmax);
+    else G = g_box(a, b, p.tmax, p.geometry == 1 ? p.tmax : p.tmax2);
+    double ell1 = std::sqrt((a - c - e) * (a - c - e) + (b - d - f) * (b - d - f));
+    double ell2 = std::sqrt(c * c + d * d);
+    double ell3 = std::sqrt(e * e + f * f);
+    if (p.geometry == 0) {
+        if (ell1 <= p.lmin || ell2 <= p.lmin || ell3 <= p.lmin) return 0;
+    } else if (ell1 <= 0 || ell2 <= 0 || ell3 <= 0 || ell3 > p.lmax) {
+        return 0;
+    }
+    const Calc &C = *p.c;
+    double r = C.pell_flags(ell1, p.sigma, p.ngal, p.flags) * C.pell_flags(ell2, p.sigma, p.ngal, p.flags) *
+               C.pell_flags(ell3, p.sigma, p.ngal, p.flags);
+    r *= Calc::uHat(ell1 * p.th[0]) * Calc::uHat(ell2 * p.th[1]) * Calc::uHat(ell3 * p.th[2]);
+    r *= Calc::uHat(ell1 * p.th[3]) * Calc::uHat(ell2 * p.th[4]) * Calc::uHat(ell3 * p.th[5]);
+    r *= G;
+    return r;
+}
+
+int t1fin_cb(unsigned ndim, size_t npts, const double *x, void *fdata, unsigned fdim, double *fval)
+{
+    T1FinCtx *p = (T1FinCtx *)fdata;
+    if (fdim != 1 || ndim != 6) return 1;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(p->nthreads)
+    for (long long i = 0; i < (long long)npts; i++) fval[i] = t1fin_point(*p, x + 6 * i);
+    return 0;
+}
+
 int nthreads_or_default(int nt)
 {
 #ifdef _OPENMP
@@ -1933,6 +1977,74 @@ int orc_t1_total(void *h, const double *th123, const double *th456, double area,
         double th6[6] = {th0, th1, th2, a, b, c}, v = 0, e = 0;
         long long n = 0;
         rc |= orc_t1(h, th6, area, lmin, sigma, n_gal, flags, reltol, nthreads, &v, &e, &n);
+        ntot += n;
+        return v;
+    };
+    double result;
+    if (th3 == th4) {
+        if (th4 == th5) {
+            result = 6 * T1(th3, th3, th3);
+        } else {
+            result = 2 * T1(th3, th3, th5);
+            result += 2 * T1(th3, th5, th3);
+            result += 2 * T1(th5, th3, th3);
+        }
+    } else if (th3 == th5) {
+        result = 2 * T1(th3, th4, th3);
+        result += 2 * T1(th3, th3, th4);
+        result += 2 * T1(th4, th3, th3);
+    } else if (th4 == th5) {
+        result = 2 * T1(th3, th4, th4);
+        result += 2 * T1(th4, th3, th4);
+        result += 2 * T1(th4, th4, th3);
+    } else {
+        result = T1(th3, th4, th5);
+        result += T1(th3, th5, th4);
+        result += T1(th4, th3, th5);
+        result += T1(th4, th5, th3);
+        result += T1(th5, th3, th4);
+        result += T1(th5, th4, th3);
+    }
+    *val = result;
+    if (neval) *neval = ntot;
+    return rc;
+}
+
+int orc_t1_finite(void *h, const double *th6, int geometry, double theta_max, double theta_max2, double lmin,
+                  double sigma, double n_gal, int flags, double reltol, int nthreads, double *val, long long *neval)
+{ /* GV/apertureStatisticsCovariance.cu:1065-1106, types 0, 1, 3: 6-D hcubature over
+     [-lMax, 1.02 lMax]^6, lMax = 10 / max(min theta_123, min theta_456), result / (2 pi)^6 */
+    Calc *c = (Calc *)h;
+    *val = 0;
+    if (neval) *neval = 0;
+    if (geometry != 0 && geometry != 1 && geometry != 3) return 1;
+    double m123 = std::min({th6[0], th6[1], th6[2]}), m456 = std::min({th6[3], th6[4], th6[5]});
+    double lMax = 10. / std::max(m123, m456);
+    double vMax = lMax;
+    T1FinCtx p{c, geometry, {th6[0], th6[1], th6[2], th6[3], th6[4], th6[5]}, theta_max, theta_max2, lmin, lMax,
+               sigma, n_gal, flags, nthreads_or_default(nthreads)};
+    double vmin[6] = {-vMax, -vMax, -lMax, -lMax, -lMax, -lMax};
+    double vmax[6] = {1.02 * vMax, 1.02 * vMax, 1.02 * lMax, 1.02 * lMax, 1.02 * lMax, 1.02 * lMax};
+    double result = 0, error = 0;
+    int rc = hcubature_v_impl(1, t1fin_cb, &p, 6, vmin, vmax, 0, 0, reltol, ORC_ERROR_L1, &result, &error);
+    result /= std::pow(2 * M_PI, 6);
+    *val = result;
+    if (neval) *neval = g_last_neval;
+    return rc;
+}
+
+int orc_t1_total_finite(void *h, const double *th123, const double *th456, int geometry, double theta_max,
+                        double theta_max2, double lmin, double sigma, double n_gal, int flags, double reltol,
+                        int nthreads, double *val, long long *neval)
+{ /* GV/apertureStatisticsCovariance.cu:272-326 with the finite-survey T1 */
+    const double th3 = th456[0], th4 = th456[1], th5 = th456[2];
+    long long ntot = 0;
+    int rc = 0;
+    auto T1 = [&](double a, double b, double d) {
+        double th6[6] = {th123[0], th123[1], th123[2], a, b, d}, v = 0;
+        long long n = 0;
+        rc |= orc_t1_finite(h, th6, geometry, theta_max, theta_max2, lmin, sigma, n_gal, flags, reltol, nthreads,
+                            &v, &n);
         ntot += n;
         return v;
     };

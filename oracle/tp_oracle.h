@@ -112,7 +112,13 @@ int orc_t1_total(void *h, const double *th123, const double *th456, double area,
 
 /* T2 of the Gaussian <Map^3> covariance for the survey geometries of the cuda_version
    (geometry 0 circle, 1 square, 2 infinite (T2 = 0), 3 rectangle; theta_max and theta_max2 the
-   side lengths or the diameter [rad]; flags as orc_t1) */
+   side lengths or the circle's radius [rad]; flags as orc_t1) */
+/* T1 of the finite geometries (0 circle, 1 square, 3 rectangle): 6-D, no area factor */
+int orc_t1_finite(void *h, const double *th6, int geometry, double theta_max, double theta_max2, double lmin,
+                  double sigma, double n_gal, int flags, double reltol, int nthreads, double *val, long long *neval);
+int orc_t1_total_finite(void *h, const double *th123, const double *th456, int geometry, double theta_max,
+                        double theta_max2, double lmin, double sigma, double n_gal, int flags, double reltol,
+                        int nthreads, double *val, long long *neval);
 int orc_t2(void *h, const double *th6, int geometry, double theta_max, double theta_max2, double lmin,
            double sigma, double n_gal, int flags, double reltol, int nthreads, double *val, long long *neval);
 int orc_t2_total(void *h, const double *th123, const double *th456, int geometry, double theta_max,

@@ -73,6 +73,8 @@ _SIGS = {
                                                                                 ctypes.c_int, _dp, _dp, _llp]),
     "orc_t1_total": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp] + [ctypes.c_double] * 4 +
                      [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _llp]),
+    "orc_t1_total_finite": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.c_int] + [ctypes.c_double] * 5 +
+                            [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _llp]),
     "orc_t2": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_int] + [ctypes.c_double] * 5 +
                [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _llp]),
     "orc_t2_total": (ctypes.c_int, [ctypes.c_void_p, _dp, _dp, ctypes.c_int] + [ctypes.c_double] * 5 +
@@ -205,6 +207,15 @@ class Oracle:
         v, n = ctypes.c_double(), ctypes.c_longlong()
         assert lib().orc_t1_total(self.h, d(a), d(b), area, lmin, sigma, n_gal, flags, reltol, nthreads,
                                   ctypes.byref(v), ctypes.byref(n)) == 0
+        return v.value, n.value
+
+    def t1_total_finite(self, th123, th456, geometry, theta_max, theta_max2=0.0, lmin=0.0, sigma=0.3, n_gal=1.0,
+                        flags=0, reltol=1e-2, nthreads=0):
+        a = np.ascontiguousarray(th123, dtype=np.float64)
+        b = np.ascontiguousarray(th456, dtype=np.float64)
+        v, n = ctypes.c_double(), ctypes.c_longlong()
+        assert lib().orc_t1_total_finite(self.h, d(a), d(b), geometry, theta_max, theta_max2, lmin, sigma, n_gal,
+                                         flags, reltol, nthreads, ctypes.byref(v), ctypes.byref(n)) == 0
         return v.value, n.value
 
     def t2(self, th6, geometry, theta_max, theta_max2=0.0, lmin=0.0, sigma=0.3, n_gal=1.0, flags=0, reltol=1e-4,

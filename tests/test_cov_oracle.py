@@ -6,6 +6,12 @@
     (Isserlis' theorem, exact rational arithmetic).  It fixes the cubature domain, the phi-symmetry
     factor 2, the 2 / A / (2 pi)^3 prefactor, the filter products and the shape-noise Pell; T1_total
     is 6 T1 for equal radii.
+  * T1 of a finite survey (circle, square, rectangle; 6-D with the window G): by Parseval
+    int d^2l G(l) / (2 pi)^2 = 1 / area for the three windows (the circle's is a disc of radius
+    theta_max), so for a survey much larger than the apertures T1 tends to the infinite-survey T1 of
+    that area.  The windows' heavy tails make the approach first order in theta / theta_max (~1% at
+    60 theta).  At the reference's tolerance 1e-2 the 6-D cubature itself can stop early on the narrow
+    window (up to 45% low on these cases), so the limit is checked at 1e-3.
   * The classed_version's 4-D form is symmetric under theta_123 <-> theta_456 up to its ell truncation
     (the permutation sum makes the integrand symmetric on the full domain): swapped pairs agree to the
     1e-3 tolerance.  Its single-z triple spectrum is pinned through P_NL (test_power_oracle.py) and the
@@ -120,3 +126,15 @@ def test_cov_gauss_symmetric_in_the_two_triples(orc):
     v1 = orc.map3_cov_gauss(np.concatenate([a, b]), area, 1e-4)[0]
     v2 = orc.map3_cov_gauss(np.concatenate([b, a]), area, 1e-4)[0]
     assert v1 > 0 and abs(v1 / v2 - 1) < 1e-3
+
+
+@pytest.mark.parametrize("geometry,ratio2,bar", [(1, 0.0, 0.02), (3, 0.6, 0.03), (0, 0.0, 0.02)])
+def test_t1_finite_tends_to_infinite_survey(orc, geometry, ratio2, bar):
+    th = np.array([8.0, 8.0, 8.0]) * A
+    tx = 60 * th[0]
+    ty = ratio2 * tx
+    area = math.pi * tx * tx if geometry == 0 else (tx * tx if geometry == 1 else tx * ty)
+    ngal = 30 / A ** 2
+    vf, nf = orc.t1_total_finite(th, th, geometry, tx, ty, 0.0, 0.3, ngal, 4, 1e-3)
+    vi, _ = orc.t1_total(th, th, area, 0.0, 0.3, ngal, 4, 1e-5)
+    assert nf > 0 and abs(vf / vi - 1) < bar, (geometry, vf / vi)

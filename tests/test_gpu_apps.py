@@ -163,7 +163,7 @@ def test_calculate_covariance_cli(tmp_path, fid_oracle):
     assert r.returncode == 0, r.stderr
     t1f = np.loadtxt(fp32_dir / (name % (1, n_deg, 7.87 * deg * 57.3)))
     assert t1f.shape == (4, 4) and np.max(np.abs(t1f / t1 - 1)) < 1e-3
-    # a circular survey (diameter thetaMax): T2 against the oracle's T2_total; T1 of it is not provided
+    # a circular survey (radius thetaMax): T2 against the oracle's T2_total
     circ = tmp_path / "circle"
     circ.mkdir()
     argc = args[:4] + [str(circ) + "/"] + args[5:6] + ["0", "1"] + args[8:13] + ["circle"] + args[14:]
@@ -175,8 +175,14 @@ def test_calculate_covariance_cli(tmp_path, fid_oracle):
         for j in range(i, 4):
             ov, _ = fid_oracle.t2_total(tri[i], tri[j], 0, 7.87 * deg, 0.0, 0.0, 0.37, 108000 / deg / deg, 2, 1e-3)
             assert ov > 0 and abs(t2c[i, j] / ov - 1) < 1e-5
-    r = subprocess.run(argc[:6] + ["1"] + argc[7:], capture_output=True, text=True, timeout=60)
-    assert r.returncode == 1  # T1 of a circular survey: not provided
+    # T1 of a circular survey (6-D, the reference's tolerance 1e-2), constant spectrum: a finite, positive,
+    # symmetric matrix (its values: test_gpu_cov.py::test_t1_finite_geometries_match_oracle)
+    cov_sn = tmp_path / "cov_sn.param"
+    cov_sn.write_text(cov.read_text().replace("shapenoiseOnly 0", "shapenoiseOnly 1"))
+    r = subprocess.run(argc[:5] + [str(cov_sn), "1", "0"] + argc[8:], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    t1c = np.loadtxt(circ / (name.replace("infinite", "circle") % (1, n_deg, 7.87 * deg * 57.3)))
+    assert t1c.shape == (4, 4) and np.array_equal(t1c, t1c.T) and np.all(np.isfinite(t1c)) and np.all(t1c > 0)
 
 
 def test_c3_batch_subset_matches_oracle(gpu_ctx, fid_nz):

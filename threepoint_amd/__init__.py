@@ -104,6 +104,9 @@ _SIGS = {
     "tp_map3_cov_t1_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                             ctypes.c_int, ctypes.c_double, _dp, _llp]),
+    "tp_map3_cov_t1_batch_geometry": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp,
+                                                     ctypes.c_int] + [ctypes.c_double] * 6 +
+                                      [ctypes.c_int, ctypes.c_double, _dp, _llp]),
     "tp_map3_cov_t2_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp, ctypes.c_int] +
                              [ctypes.c_double] * 5 + [ctypes.c_int, ctypes.c_double, _dp, _llp]),
     "tp_map3_cov_t1_batch_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _ip, _dp,
@@ -445,10 +448,24 @@ class CosmologyTables:
                "tp_map3_cov_t1_batch_mode")
         return val, ne
 
+    def map3_cov_t1_geometry(self, theta6, geometry, theta_max, theta_max2=0.0, area=1.0, lmin=0.0, sigma=0.3,
+                             n_gal=1.0, flags=0, icosmo=None, reltol=1e-2):
+        """T1_total of the cuda_version's covariance for any survey geometry (GEOMETRY_*; the finite ones
+        are 6-D integrals at the reference's tolerance 1e-2, GEOMETRY_INFINITE is map3_cov_t1)."""
+        t = np.ascontiguousarray(theta6, dtype=np.float64).reshape(-1, 6)
+        n = t.shape[0]
+        ic = np.zeros(n, dtype=np.int32) if icosmo is None else np.ascontiguousarray(icosmo, dtype=np.int32)
+        val, ne = np.zeros(n), np.zeros(n, dtype=np.int64)
+        _check(lib().tp_map3_cov_t1_batch_geometry(self.ctx.h, self.h, n, ic.ctypes.data_as(_ip), _d(t),
+                                                   int(geometry), theta_max, theta_max2, area, lmin, sigma, n_gal,
+                                                   flags, reltol, _d(val), ne.ctypes.data_as(_llp)),
+               "tp_map3_cov_t1_batch_geometry")
+        return val, ne
+
     def map3_cov_t2(self, theta6, geometry, theta_max, theta_max2=0.0, lmin=0.0, sigma=0.3, n_gal=1.0, flags=0,
                     icosmo=None, reltol=1e-4):
         """T2_total of the cuda_version's covariance, pairs theta6[n, 6] [rad]; geometry GEOMETRY_CIRCLE
-        (theta_max the diameter), GEOMETRY_SQUARE (the side), GEOMETRY_RECTANGLE (sides theta_max,
+        (theta_max the radius), GEOMETRY_SQUARE (the side), GEOMETRY_RECTANGLE (sides theta_max,
         theta_max2) or GEOMETRY_INFINITE (T2 = 0)."""
         t = np.ascontiguousarray(theta6, dtype=np.float64).reshape(-1, 6)
         n = t.shape[0]

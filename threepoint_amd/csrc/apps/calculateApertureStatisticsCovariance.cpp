@@ -5,9 +5,10 @@
 //       <covariance parameters> <T1 0|1> <T2 0|1> <T4 0|1> <T5 0|1> <T6 0|1> <T7 0|1> <T7_2h 0|1> <geometry>
 //       [--nz-bins N] [--reltol X] [--no-t17] [--fp32]
 //
-// The same 13 arguments as the reference.  Provided: T1 for the infinite survey geometry, T2 for every
-// geometry (circle, square, rectangular; 0 for the infinite one, as in the reference); requesting
-// T4-T7, or T1 of a finite geometry, is an error (exit status 1).  For every pair i <= j of the
+// The same 13 arguments as the reference.  Provided: T1 and T2 for every geometry (circle, square,
+// infinite, rectangular; T2 is 0 for the infinite one, as in the reference; T1 of a finite one is the
+// reference's 6-D integral at its tolerance 1e-2, --reltol sets the others); requesting T4-T7 is an
+// error (exit status 1).  For every pair i <= j of the
 // theta_1 <= theta_2 <= theta_3 triples of the list it computes T1_total and T2_total, all pairs of a
 // term in one batched GPU pass, and writes each symmetric N x N matrix to
 // <folder>cov_<geometry>_term<k>Numerical_sigma_<s>_n_<n>_thetaMax_<t>_gpu.dat as the reference does
@@ -73,11 +74,7 @@ int main(int argc, char *argv[])
         std::cerr << "Cov type not correctly specified" << std::endl;
         return 1;
     }
-    if (do_t1 && type != "infinite") {
-        std::cerr << "T1 of the survey geometry " << type << " is not provided by this build (infinite only)"
-                  << std::endl;
-        return 1;
-    }
+
     try {
         cosmology cosmo(argv[1]);
         covarianceParameters covPar(argv[5]);

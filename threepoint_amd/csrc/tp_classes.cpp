@@ -261,10 +261,16 @@ ApertureStatisticsCovariance::ApertureStatisticsCovariance(BispectrumCalculator 
 
 std::vector<double> ApertureStatisticsCovariance::T1_total(const std::vector<std::array<double, 6>> &pairs)
 {
-    if (geometry_ != 2)
-        throw std::runtime_error("T1_total: only the infinite survey geometry is provided for T1");
     std::vector<double> th, out(pairs.size());
     for (auto &p : pairs) th.insert(th.end(), p.begin(), p.end());
+    if (geometry_ != 2) {  // the finite geometries: 6-D, FP64, the reference's tolerance 1e-2
+        check(tp_map3_cov_t1_batch_geometry(Bispectrum_->ctx(), Bispectrum_->tables(), pairs.size(), nullptr,
+                                            th.data(), geometry_, covPar_.thetaMax, covPar_.thetaMax_smaller,
+                                            covPar_.area, lMin, covPar_.shapenoise_sigma, covPar_.galaxy_density,
+                                            flags_, reltol_finite, out.data(), nullptr),
+              "tp_map3_cov_t1_batch_geometry");
+        return out;
+    }
     check(tp_map3_cov_t1_batch_mode(Bispectrum_->ctx(), Bispectrum_->tables(), pairs.size(), nullptr, th.data(),
                                     covPar_.area, lMin, covPar_.shapenoise_sigma, covPar_.galaxy_density, flags_,
                                     reltol, mode, out.data(), nullptr),

@@ -437,17 +437,20 @@ template class AdaptiveIntegralT<3, 2>;
 template class AdaptiveIntegralT<1, 1>;
 template class AdaptiveIntegralT<2, 1>;
 template class AdaptiveIntegralT<4, 1>;
+template class AdaptiveIntegralT<6, 1>;
 template void rule_points_of<kMaxDim, kMaxFdim>(unsigned, const RegionT<kMaxDim, kMaxFdim> &, double *);
 template void rule_points_of<3, 1>(unsigned, const RegionT<3, 1> &, double *);
 template void rule_points_of<3, 2>(unsigned, const RegionT<3, 2> &, double *);
 template void rule_points_of<1, 1>(unsigned, const RegionT<1, 1> &, double *);
 template void rule_points_of<2, 1>(unsigned, const RegionT<2, 1> &, double *);
 template void rule_points_of<4, 1>(unsigned, const RegionT<4, 1> &, double *);
+template void rule_points_of<6, 1>(unsigned, const RegionT<6, 1> &, double *);
 template void rule_apply<kMaxDim, kMaxFdim>(unsigned, unsigned, RegionT<kMaxDim, kMaxFdim> &, const double *);
 template void rule_apply<3, 1>(unsigned, unsigned, RegionT<3, 1> &, const double *);
 template void rule_apply<3, 2>(unsigned, unsigned, RegionT<3, 2> &, const double *);
 template void rule_apply<1, 1>(unsigned, unsigned, RegionT<1, 1> &, const double *);
 template void rule_apply<2, 1>(unsigned, unsigned, RegionT<2, 1> &, const double *);
 template void rule_apply<4, 1>(unsigned, unsigned, RegionT<4, 1> &, const double *);
+template void rule_apply<6, 1>(unsigned, unsigned, RegionT<6, 1> &, const double *);
 
 }  // namespace tpc

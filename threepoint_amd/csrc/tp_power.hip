@@ -35,7 +35,9 @@ namespace {
 enum PowerKind {
     kPkappa = 0, kMap2 = 1, kPkAtZ = 2, kLimber = 3, kCovGauss = 4, kT1 = 5,
     // the T2 integrals of a finite survey (1-D: A and the circle's B; 2-D: the square's and rectangle's B)
-    kT2A = 6, kT2Circle = 7, kT2Square = 8, kT2Rect = 9
+    kT2A = 6, kT2Circle = 7, kT2Square = 8, kT2Rect = 9,
+    // T1 of a finite survey (6-D)
+    kT1Circle = 10, kT1Square = 11, kT1Rect = 12
 };
 
 struct TpPowerDesc {
@@ -46,8 +48,9 @@ struct TpPowerDesc {
     int flags;           // TP_PK_LINEAR | TP_PK_T17 (| TP_PK_SHAPENOISE_ONLY for kT1)
     int pad;
     double th[6];        // kCovGauss, kT1: theta_123, theta_456 [rad]; kT2*: the two radii, the survey's
-                         // side lengths or diameter (theta_max, theta_max_smaller) [rad]
-    double lmin, lmax;   // kT1, kT2*: the integrand's ell cut
+                         // side lengths or radius (theta_max, theta_max_smaller) [rad]
+    double lmin, lmax;   // kT1, kT2*, kT1Circle..: the integrand's ell cut
+    double wx, wy;       // kT1Circle..kT1Rect: the survey's radius or side lengths [rad]
 };
 
 // 96-point Gauss-Legendre half nodes/weights (own Newton solve, uploaded once per device)
@@ -179,6 +182,38 @@ __device__ double t2_point(const TpCosmoConst &C, const TpPNode *__restrict__ N,
     return r;
 }
 
+// T1 of a finite survey at one point (apertureStatisticsCovariance.cu:1886-1955, 1984-2012): x = (a, b)
+// the window's wave vector, (c, d) and (e, f) two of the Pell wave vectors, l1 = |(a - c - e, b - d - f)|;
+// the circle cuts l_i <= lMin, the square and the rectangle l_i <= 0 and l3 > lMax; the reference's
+// product order.
+__device__ double t1fin_point(const TpCosmoConst &C, const TpPNode *__restrict__ N, const TpPowerDesc &D,
+                              const double *x, double dx)
+{
+#pragma clang fp contract(off)
+    const double a = x[0], b = x[1], c = x[2], d = x[3], e = x[4], f = x[5];
+    double G;
+    if (D.kind == kT1Circle) {
+        G = g_circle(sqrt(a * a + b * b), D.wx);
+    } else {
+        const double p = sinc_cut(0.5 * a * D.wx), q = sinc_cut(0.5 * b * (D.kind == kT1Square ? D.wx : D.wy));
+        G = p * p * q * q;
+    }
+    const double ell1 = sqrt((a - c - e) * (a - c - e) + (b - d - f) * (b - d - f));
+    const double ell2 = sqrt(c * c + d * d), ell3 = sqrt(e * e + f * f);
+    if (D.kind == kT1Circle) {
+        if (ell1 <= D.lmin || ell2 <= D.lmin || ell3 <= D.lmin) return 0;
+    } else if (ell1 <= 0 || ell2 <= 0 || ell3 <= 0 || ell3 > D.lmax) {
+        return 0;
+    }
+    const double shot = 0.5 * D.sigma * D.sigma / D.ngal;
+    double r = pell_nodes(C, N, ell1, D.flags, dx, shot) * pell_nodes(C, N, ell2, D.flags, dx, shot) *
+               pell_nodes(C, N, ell3, D.flags, dx, shot);
+    r *= uhat(ell1 * D.th[0]) * uhat(ell2 * D.th[1]) * uhat(ell3 * D.th[2]);
+    r *= uhat(ell1 * D.th[3]) * uhat(ell2 * D.th[4]) * uhat(ell3 * D.th[5]);
+    r *= G;
+    return r;
+}
+
 // One thread per point; pdesc[i] names the integral (desc entry) of point i.
 __global__ void k_power_points(const TpCosmoConst *__restrict__ cc, const double *__restrict__ tab, int n_z,
                                const TpPowerDesc *__restrict__ desc,
@@ -202,6 +237,8 @@ __global__ void k_power_points(const TpCosmoConst *__restrict__ cc, const double
         TpZAtT<double> Z;
         tp_z_at<double>(T, n_z, C.zmax, xi[1], &Z);
         v = tp_pk_at<double>(C, Z, xi[0], xi[1], D.flags);
+    } else if (D.kind >= kT1Circle) {
+        v = t1fin_point(C, pnode + (size_t)D.icosmo * TP_NNODES, D, xi, dx);
     } else if (D.kind >= kT2A) {
         v = t2_point(C, pnode + (size_t)D.icosmo * TP_NNODES, D, xi, dx);
     } else {
@@ -1106,6 +1143,111 @@ extern "C" int tp_map3_cov_t1_batch_mode(tp_ctx *ctx, const tp_cosmo *c, size_t 
     return 0;
 }
 
+extern "C" int tp_map3_cov_t1_batch_geometry(tp_ctx *ctx, const tp_cosmo *c, size_t n, const int *icosmo,
+                                             const double *theta6, int geometry, double theta_max,
+                                             double theta_max2, double area, double lmin, double sigma,
+                                             double n_gal, int flags, double reltol, double *out,
+                                             long long *neval)
+{
+    if (geometry == 2)
+        return tp_map3_cov_t1_batch(ctx, c, n, icosmo, theta6, area, lmin, sigma, n_gal, flags, reltol, out, neval);
+    int rc = check_args(ctx, c, flags);
+    if (rc) return rc;
+    if (geometry != 0 && geometry != 1 && geometry != 3)
+        return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch_geometry: geometry must be 0 (circle), 1 (square), "
+                                       "2 (infinite) or 3 (rectangle)");
+    if (n && (!theta6 || !out)) return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch_geometry: NULL array");
+    if (!(reltol > 0) || !(n_gal > 0) || !(lmin >= 0))
+        return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch_geometry: reltol and n_gal must be positive, lmin >= 0");
+    if (!(theta_max > 0) || (geometry == 3 && !(theta_max2 > 0)))
+        return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch_geometry: the survey's side lengths or radius must "
+                                       "be positive");
+    if (n > 0x7fffffffu / 6) return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch_geometry: too many pairs");
+    TP_HIP(hipSetDevice(ctx->device));
+    const int kind = geometry == 0 ? kT1Circle : geometry == 1 ? kT1Square : kT1Rect;
+    std::vector<TpPowerDesc> desc;
+    std::vector<double> lo, hi;
+    std::vector<std::vector<std::pair<int, size_t>>> terms(n);
+    for (size_t i = 0; i < n; ++i) {
+        const int ic = icosmo ? icosmo[i] : 0;
+        if (ic < 0 || ic >= c->ncosmo)
+            return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch_geometry: cosmology index out of range");
+        const double *t = theta6 + 6 * i;
+        for (int k = 0; k < 6; ++k)
+            if (!(t[k] > 0)) return tp_set_error(TP_EINVAL, "tp_map3_cov_t1_batch_geometry: aperture radii must be positive");
+        const double th3 = t[3], th4 = t[4], th5 = t[5];
+        auto add = [&](int coef, double a, double b, double d) {
+            TpPowerDesc D{};
+            D.icosmo = ic;
+            D.kind = kind;
+            D.flags = flags;
+            D.sigma = sigma;
+            D.ngal = n_gal;
+            D.th[0] = t[0];
+            D.th[1] = t[1];
+            D.th[2] = t[2];
+            D.th[3] = a;
+            D.th[4] = b;
+            D.th[5] = d;
+            D.lmin = lmin;
+            D.wx = theta_max;
+            D.wy = theta_max2;
+            // T1 (apertureStatisticsCovariance.cu:1065-1102): lMax = 10 / max(min theta_123, min theta_456),
+            // every axis over [-lMax, 1.02 lMax]
+            D.lmax = 10. / std::max(std::min({t[0], t[1], t[2]}), std::min({a, b, d}));
+            terms[i].push_back({coef, desc.size()});
+            desc.push_back(D);
+            for (int q = 0; q < 6; ++q) {
+                lo.push_back(-D.lmax);
+                hi.push_back(1.02 * D.lmax);
+            }
+        };
+        // T1_total's permutations of theta_456 (apertureStatisticsCovariance.cu:272-326)
+        if (th3 == th4) {
+            if (th4 == th5) {
+                add(6, th3, th3, th3);
+            } else {
+                add(2, th3, th3, th5);
+                add(2, th3, th5, th3);
+                add(2, th5, th3, th3);
+            }
+        } else if (th3 == th5) {
+            add(2, th3, th4, th3);
+            add(2, th3, th3, th4);
+            add(2, th4, th3, th3);
+        } else if (th4 == th5) {
+            add(2, th3, th4, th4);
+            add(2, th4, th3, th4);
+            add(2, th4, th4, th3);
+        } else {
+            add(1, th3, th4, th5);
+            add(1, th3, th5, th4);
+            add(1, th4, th3, th5);
+            add(1, th4, th5, th3);
+            add(1, th5, th3, th4);
+            add(1, th5, th4, th3);
+        }
+    }
+    std::vector<double> v(desc.size());
+    std::vector<long long> ne(desc.size());
+    const double norm = std::pow(2 * tpd::kPi, 6);
+    rc = power_batch<6>(ctx, c, desc, lo.data(), hi.data(), reltol, [norm](double x) { return x / norm; }, v.data(),
+                        nullptr, ne.data());
+    if (rc) return rc;
+    for (size_t i = 0; i < n; ++i) {
+        double r = 0;
+        long long np = 0;
+        for (size_t q = 0; q < terms[i].size(); ++q) {
+            const double tq = terms[i][q].first == 1 ? v[terms[i][q].second] : terms[i][q].first * v[terms[i][q].second];
+            r = q == 0 ? tq : r + tq;
+            np += ne[terms[i][q].second];
+        }
+        out[i] = r;
+        if (neval) neval[i] = np;
+    }
+    return 0;
+}
+
 // The pairings of T2_total (apertureStatisticsCovariance.cu:328-538), one side at a time: (multiplicity,
 // three indices into the side's radii).  For theta_123 the indices give T2's arguments 1-3 (the radii of
 // one A factor, then the radius of B), for theta_456 its arguments 4-6 (the radius of B, then the radii
@@ -1153,7 +1295,7 @@ extern "C" int tp_map3_cov_t2_batch(tp_ctx *ctx, const tp_cosmo *c, size_t n, co
     if (!(reltol > 0) || !(n_gal > 0) || !(lmin >= 0))
         return tp_set_error(TP_EINVAL, "tp_map3_cov_t2_batch: reltol and n_gal must be positive, lmin >= 0");
     if (geometry != 2 && (!(theta_max > 0) || (geometry == 3 && !(theta_max2 > 0))))
-        return tp_set_error(TP_EINVAL, "tp_map3_cov_t2_batch: the survey's side lengths or diameter must be positive");
+        return tp_set_error(TP_EINVAL, "tp_map3_cov_t2_batch: the survey's side lengths or radius must be positive");
     if (n > 0x7fffffffu / 81) return tp_set_error(TP_EINVAL, "tp_map3_cov_t2_batch: too many pairs for one call");
     for (size_t i = 0; i < n; ++i) {
         const int ic = icosmo ? icosmo[i] : 0;
