@@ -1536,11 +1536,17 @@ int t2_side_123(const double *t, int mult[3], int idx[3][3])
                      dif[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}};
     const int(*L)[3];
     int n;
-    if (t[0] == t[1] && t[1] == t[2]) { L = all; n = 1; mult[0] = 3; }
-    else if (t[0] == t[1]) { L = e01; n = 2; mult[0] = 1; mult[1] = 2; }
-    else if (t[1] == t[2]) { L = e12; n = 2; mult[0] = 2; mult[1] = 1; }
-    else if (t[0] == t[2]) { L = e02; n = 2; mult[0] = 2; mult[1] = 1; }
-    else { L = dif; n = 3; mult[0] = mult[1] = mult[2] = 1; }
+    if (t[0] == t[1] && t[1] == t[2]) {
+        L = all; n = 1; mult[0] = 3;
+    } else if (t[0] == t[1]) {
+        L = e01; n = 2; mult[0] = 1; mult[1] = 2;
+    } else if (t[1] == t[2]) {
+        L = e12; n = 2; mult[0] = 2; mult[1] = 1;
+    } else if (t[0] == t[2]) {
+        L = e02; n = 2; mult[0] = 2; mult[1] = 1;
+    } else {
+        L = dif; n = 3; mult[0] = mult[1] = mult[2] = 1;
+    }
     for (int k = 0; k < n; ++k)
         for (int d = 0; d < 3; ++d) idx[k][d] = L[k][d];
     return n;
@@ -1553,11 +1559,17 @@ int t2_side_456(const double *t, int mult[3], int idx[3][3])
                      dif[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 0, 1}};
     const int(*L)[3];
     int n;
-    if (t[0] == t[1] && t[1] == t[2]) { L = all; n = 1; mult[0] = 3; }
-    else if (t[0] == t[1]) { L = e01; n = 2; mult[0] = 2; mult[1] = 1; }
-    else if (t[0] == t[2]) { L = e02; n = 2; mult[0] = 2; mult[1] = 1; }
-    else if (t[1] == t[2]) { L = e12; n = 2; mult[0] = 2; mult[1] = 1; }
-    else { L = dif; n = 3; mult[0] = mult[1] = mult[2] = 1; }
+    if (t[0] == t[1] && t[1] == t[2]) {
+        L = all; n = 1; mult[0] = 3;
+    } else if (t[0] == t[1]) {
+        L = e01; n = 2; mult[0] = 2; mult[1] = 1;
+    } else if (t[0] == t[2]) {
+        L = e02; n = 2; mult[0] = 2; mult[1] = 1;
+    } else if (t[1] == t[2]) {
+        L = e12; n = 2; mult[0] = 2; mult[1] = 1;
+    } else {
+        L = dif; n = 3; mult[0] = mult[1] = mult[2] = 1;
+    }
     for (int k = 0; k < n; ++k)
         for (int d = 0; d < 3; ++d) idx[k][d] = L[k][d];
     return n;

@@ -1262,19 +1262,26 @@ static int t2_side(const double *t, bool first, int mult[3], int idx[3][3])
                      a02[2][3] = {{0, 1, 0}, {1, 0, 0}}, adif[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}};
     static const int b01[2][3] = {{0, 0, 2}, {2, 0, 0}}, b02[2][3] = {{0, 1, 0}, {1, 0, 0}},
                      b12[2][3] = {{1, 0, 1}, {0, 1, 1}}, bdif[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 0, 1}};
-    const int(*L)[3];
-    int n;
+    const int(*L)[3] = first ? adif : bdif;
+    int n = 3;
+    mult[0] = mult[1] = mult[2] = 1;
     if (t[0] == t[1] && t[1] == t[2]) {
-        L = all, n = 1, mult[0] = 3;
+        L = all;
+        n = 1;
+        mult[0] = 3;
     } else if (t[0] == t[1]) {
-        L = first ? a01 : b01, n = 2;
-        mult[0] = first ? 1 : 2, mult[1] = first ? 2 : 1;
+        L = first ? a01 : b01;
+        n = 2;
+        mult[0] = first ? 1 : 2;
+        mult[1] = first ? 2 : 1;
     } else if (first ? t[1] == t[2] : t[0] == t[2]) {
-        L = first ? a12 : b02, n = 2, mult[0] = 2, mult[1] = 1;
+        L = first ? a12 : b02;
+        n = 2;
+        mult[0] = 2;
     } else if (first ? t[0] == t[2] : t[1] == t[2]) {
-        L = first ? a02 : b12, n = 2, mult[0] = 2, mult[1] = 1;
-    } else {
-        L = first ? adif : bdif, n = 3, mult[0] = mult[1] = mult[2] = 1;
+        L = first ? a02 : b12;
+        n = 2;
+        mult[0] = 2;
     }
     for (int k = 0; k < n; ++k)
         for (int d = 0; d < 3; ++d) idx[k][d] = L[k][d];
