@@ -85,8 +85,17 @@ int tp_bkappa_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, cons
                    double *out, int mode);
 
 /* Replaces BispectrumCalculator::bispec (classed_version/bispectrum.cpp:166-236) with the
- * table interpolation of :506-513: in is npts*4 (k1, k2, k3 [h/Mpc], z); out is B [(Mpc/h)^6]. */
+ * table interpolation of :506-513: in is npts*4 (k1, k2, k3 [h/Mpc], z); out is B [(Mpc/h)^6];
+ * z > 10 gives the tree-level bispectrum (:174-175, 609-612). */
 int tp_bispec_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *k123z, double *out);
+
+/* Replaces BispectrumCalculator::bispec(k1, k2, k3, z, idx, didx) (classed_version/bispectrum.hpp:247,
+ * bispectrum.cpp:166-236) with the caller's table interval: in is npts*6 (k1, k2, k3 [h/Mpc], z, idx,
+ * didx); D1, r_sigma and n_eff are the tables lerped at (idx, didx), z enters Omega_m(z) of F2; z > 10
+ * gives the tree-level bispectrum (:174-175, 609-612).  idx must be an integer in [0, n_z - 2]
+ * (TP_EINVAL otherwise: the reference would read past its tables). */
+int tp_bispec_eval_idx(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *k123z_idx,
+                       double *out);
 
 /* Replaces BispectrumCalculator::integrand_bkappa (classed_version/bispectrum.cpp:471-541):
  * in is npts*4 (z, ell1, ell2, ell3); out = g^3 (1+z)^3 B(ell/f_K, z) / (f_K E(z)), 0 at z = 0
@@ -164,6 +173,14 @@ int tp_round_log(const tp_ctx *ctx, double *out, size_t cap, size_t *n);
  * on it.  No counterpart in the reference. */
 int tp_set_zero_copy_regions(tp_ctx *ctx, long long max_regions);
 
+/* Guards of every batched cubature on this context (tp_map3_batch, tp_gamma_batch, tp_map2_batch,
+ * tp_pkappa_batch and the covariance batches): a call whose integrals need more than max_rounds
+ * rounds, or evaluate more than max_regions regions in total, ends with TP_ECAPACITY instead of
+ * growing without bound; the batches that apply the rule on the host evaluate a round in slices of at
+ * most round_points points.  0 restores a default (20000 rounds, 4e8 regions, 2^23 points).  Results
+ * do not depend on round_points.  No counterpart in the reference (its integrals have no bound). */
+int tp_set_batch_limits(tp_ctx *ctx, long long max_rounds, long long max_regions, long long round_points);
+
 /* Statistics of the last tp_map3_batch call on this context. */
 typedef struct {
     long long rounds, regions, points, launches;
@@ -230,6 +247,13 @@ int tp_gamma_batch(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t ntri, cons
  * counterpart in the reference (its integrand is FP64). */
 int tp_gamma_batch_mode(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t ntri, const double *x123,
                         double reltol, int mode, double *out_re, double *out_im, long long *neval);
+/* Triangle centers of GammaCalculator (classed_version/gamma.cpp:12-16) */
+enum { TP_CENTER_CENTROID = 0, TP_CENTER_ORTHOCENTER = 1 };
+/* tp_gamma_batch_mode with the reference's triangle_center choice (GammaCalculator's constructor argument,
+ * classed_version/gamma.cpp:6-23): TP_CENTER_CENTROID rotates the natural components from the orthocenter
+ * to the centroid (:44-45, 67-68, 624-643); TP_CENTER_ORTHOCENTER returns them unrotated. */
+int tp_gamma_batch_center(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t ntri, const double *x123,
+                          double reltol, int mode, int center, double *out_re, double *out_im, long long *neval);
 /* The FP32 form's window: Ogata terms whose wavenumbers all lie in [qmin, qmax] run in FP32
  * (default [1e-5, 1e6]); no counterpart in the reference. */
 int tp_gamma_fp32_window(tp_ctx *ctx, double qmin, double qmax);

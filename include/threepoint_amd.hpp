@@ -94,6 +94,9 @@ class BispectrumCalculator {
 
     void set_cosmology(cosmology *cosmo);  // rebuilds the GPU tables
     double bispec(double k1, double k2, double k3, double z);
+    /* classed_version/bispectrum.hpp:247: the table interval (idx, didx) given by the caller,
+     * idx = int(z / z_max * (n_z - 1)), didx the remainder */
+    double bispec(double k1, double k2, double k3, double z, int idx, double didx);
     double bkappa(double ell1, double ell2, double ell3);
     double integrand_bkappa(double z, ell_params p);
 
@@ -207,6 +210,7 @@ class GammaCalculator {
 
   private:
     BispectrumCalculator *Bispectrum_;
+    int center_ = TP_CENTER_CENTROID; /* triangle_center: "centroid" or "orthocenter" */
     std::array<std::complex<double>, 4> all(double x1, double x2, double x3);
 };
 

@@ -1783,6 +1783,10 @@ double orc_bispec(void *h, double k1, double k2, double k3, double z)
     c->z_index(z, &idx, &didx);
     return c->bispec(k1, k2, k3, z, idx, didx);
 }
+double orc_bispec_idx(void *h, double k1, double k2, double k3, double z, int idx, double didx)
+{
+    return ((Calc *)h)->bispec(k1, k2, k3, z, idx, didx);  // CV/bispectrum.cpp:166 (the 6-argument form)
+}
 double orc_integrand_bkappa(void *h, double z, double l1, double l2, double l3)
 {
     return ((Calc *)h)->integrand_bkappa(z, l1, l2, l3);

@@ -59,6 +59,7 @@ double orc_sigmam(void *h, double r, int j);
 double orc_lgr(void *h, double z);
 double orc_f_K_at_z(void *h, double z);
 double orc_bispec(void *h, double k1, double k2, double k3, double z);
+double orc_bispec_idx(void *h, double k1, double k2, double k3, double z, int idx, double didx);
 double orc_integrand_bkappa(void *h, double z, double l1, double l2, double l3);
 double orc_bkappa(void *h, double l1, double l2, double l3);
 void orc_bkappa_batch(void *h, long long n, const double *l123, double *out, int nthreads);

@@ -35,6 +35,8 @@ _SIGS = {
     "orc_f_K_at_z": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double]),
     "orc_bispec": (ctypes.c_double, [ctypes.c_void_p] + [ctypes.c_double] * 4),
     "orc_integrand_bkappa": (ctypes.c_double, [ctypes.c_void_p] + [ctypes.c_double] * 4),
+    "orc_bispec_idx": (ctypes.c_double, [ctypes.c_void_p] + [ctypes.c_double] * 4 + [ctypes.c_int,
+                                                                                      ctypes.c_double]),
     "orc_bkappa": (ctypes.c_double, [ctypes.c_void_p] + [ctypes.c_double] * 3),
     "orc_bkappa_batch": (None, [ctypes.c_void_p, ctypes.c_longlong, _dp, _dp, ctypes.c_int]),
     "orc_map3_integrand": (ctypes.c_double, [ctypes.c_void_p] + [ctypes.c_double] * 3 + [_dp]),

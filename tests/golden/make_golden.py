@@ -3,6 +3,7 @@
 These are outputs of the CPU FP64 oracle (oracle/tp_oracle.cpp), stored so the
 GPU parity tests do not have to rerun minutes of CPU cubature on the GPU host:
   map3_c0_fp64.json   <Map^3> of the 20 C0 triples (rel tol 1e-4) + points used
+  map3_c1_fp64.json   <Map^3> of the 220 triples of 10 log radii 0.5-32 arcmin (rel tol 1e-4) + points used
   tables_fid.json     f_K, g, D1, r_sigma, n_eff, ncur, norm of the fiducial cosmology
   bkappa_c1_2k.npz    B_kappa at the first 2000 C1 dump points
   census.json         the op census of the integrand points (flops and elementary calls per
@@ -10,7 +11,7 @@ GPU parity tests do not have to rerun minutes of CPU cubature on the GPU host:
                       node-record point), counted by instantiating the kernels' device functions
                       with an op-counting scalar on the host, and the flop-equivalents per unit at
                       20 flops per elementary call; bench.py's roofline reads it
-Usage: python tests/golden/make_golden.py [map3|tables|bkappa|census|all]
+Usage: python tests/golden/make_golden.py [map3|map3_c1|tables|bkappa|census|all]
 """
 import json
 import os
@@ -49,6 +50,15 @@ def main(what):
             rows.append({"theta_arcmin": th.tolist(), "map3": v, "err": e, "neval": n})
             print(th, v, e, n, "%.1fs" % (time.time() - t0), flush=True)
         json.dump({"reltol": 1e-4, "rows": rows}, open(os.path.join(HERE, "map3_c0_fp64.json"), "w"), indent=1)
+    if what == "map3_c1":
+        tr = W.triples(W.map3_c1_radii())
+        rows = []
+        t0 = time.time()
+        for th in tr:
+            v, e, n = o.map3(W.to_rad(th), 1e-4)
+            rows.append({"theta_arcmin": th.tolist(), "map3": v, "err": e, "neval": n})
+            print(th, v, e, n, "%.1fs" % (time.time() - t0), flush=True)
+        json.dump({"reltol": 1e-4, "rows": rows}, open(os.path.join(HERE, "map3_c1_fp64.json"), "w"), indent=1)
 
 
 def census(flops_per_elem=20.0):

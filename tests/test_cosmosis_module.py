@@ -43,14 +43,15 @@ def _load():
     return mock, mod
 
 
-def _options(mock, tmp_path, nbins=3, diag=False, output=True):
+def _options(mock, tmp_path, nbins=3, diag=False, output=True, zbins=100):
     z, n = W.nz_file_rows()
     nf = tmp_path / "nz.dat"
     nf.write_text("".join("%.17g %.17g\n" % (a, b) for a, b in zip(z, n)))
     o = mock.mock_block_new()
     s = b"threepoint"
     mock.mock_put_string(o, s, b"nz_file", str(nf).encode())
-    mock.mock_put_int(o, s, b"zBins", 100)
+    if zbins is not None:
+        mock.mock_put_int(o, s, b"zBins", zbins)
     mock.mock_put_double(o, s, b"zMax", 3.0)
     mock.mock_put_string(o, s, b"theta_unit", b"arcmin")
     mock.mock_put_double(o, s, b"theta_min", 4.0)
@@ -119,6 +120,40 @@ def test_module_execute_matches_batched_engine(gpu_ctx, tmp_path):
     assert np.max(np.abs(v / v_eh - 1)) < 5e-3
     files = list(tmp_path.glob("om_*.dat"))
     assert len(files) == 1 and len(files[0].read_text().splitlines()) == 11
+    assert mod.cleanup(ctypes.c_void_p(cfg)) == 0
+    mock.mock_block_free(blk)
+    mock.mock_block_free(opts)
+
+
+@pytest.mark.gpu
+def test_module_setup_without_zbins_uses_256_bins(gpu_ctx, tmp_path):
+    """The shipped pipeline (cosmosis/pipeline_gpu.ini, the cuda_version interface) sets no zBins: setup
+    still succeeds and uses the cuda_version's 256 redshift bins (cuda_version/bispectrum.cuh:35)."""
+    mock, mod = _load()
+    opts = _options(mock, tmp_path, nbins=1, diag=True, output=False, zbins=None)
+    cfg = mod.setup(opts)
+    assert cfg
+    c = W.fiducial()
+    blk = mock.mock_block_new()
+    cp = b"cosmological_parameters"
+    for k, v in ((b"omega_m", c.om), (b"sigma_8", c.sigma8), (b"h0", c.h), (b"omega_b", c.omb), (b"n_s", c.ns),
+                 (b"w", c.w)):
+        mock.mock_put_double(blk, cp, k, v)
+    kh = np.logspace(-4.5, 3.5, 200)
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _oracle import Oracle
+    p0 = np.array([Oracle(c, W.fiducial_nz()).scalar("linear_pk", k) for k in kh])
+    _put_array(mock, blk, b"matter_power_lin", b"k_h", kh)
+    _put_array(mock, blk, b"matter_power_lin", b"p_k", p0[None, :])
+    assert mod.execute(blk, cfg) == 0
+    out = np.zeros(4)
+    assert mock.mock_get_array(blk, b"threepoint", b"Map3s", out.ctypes.data_as(_dp), 4) == 1
+    z, n = W.nz_file_rows()
+    nz256 = W.resample_n_of_z(z, n, n_bins=256, z_max=3.0)
+    th = np.full((1, 3), tp.ARCMIN * 4.0)
+    v256, _, _ = tp.CosmologyTables([c], nz256, 256, 3.0, gpu_ctx, pk=(kh, p0)).map3_batch(th, None, 1e-4,
+                                                                                              tp.MODE_FP64)
+    assert abs(out[0] / v256[0] - 1) < 1e-12
     assert mod.cleanup(ctypes.c_void_p(cfg)) == 0
     mock.mock_block_free(blk)
     mock.mock_block_free(opts)

@@ -158,8 +158,8 @@ def test_calculate_covariance_cli(tmp_path, fid_oracle):
     # --fp32: the FP32 form of T1 writes the same file, within 1e-3 of the FP64 matrix
     fp32_dir = tmp_path / "fp32"
     fp32_dir.mkdir()
-    r = subprocess.run(args[:4] + [str(fp32_dir) + "/"] + args[5:] + ["--fp32"], capture_output=True, text=True,
-                       timeout=300)
+    r = subprocess.run(args[:4] + [str(fp32_dir) + "/"] + args[5:7] + ["0"] + args[8:] + ["--fp32"],
+                       capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     t1f = np.loadtxt(fp32_dir / (name % (1, n_deg, 7.87 * deg * 57.3)))
     assert t1f.shape == (4, 4) and np.max(np.abs(t1f / t1 - 1)) < 1e-3
@@ -167,6 +167,10 @@ def test_calculate_covariance_cli(tmp_path, fid_oracle):
     circ = tmp_path / "circle"
     circ.mkdir()
     argc = args[:4] + [str(circ) + "/"] + args[5:6] + ["0", "1"] + args[8:13] + ["circle"] + args[14:]
+    # --fp32 exists for T1 of the infinite survey only: with T2 or a finite geometry it is refused
+    for bad in (args + ["--fp32"], argc[:6] + ["1", "0"] + argc[8:] + ["--fp32"]):
+        r = subprocess.run(bad, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1 and "--fp32" in r.stderr, r.stderr
     r = subprocess.run(argc, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     t2c = np.loadtxt(circ / (name.replace("infinite", "circle") % (2, n_deg, 7.87 * deg * 57.3)))

@@ -123,3 +123,29 @@ def test_t1_fp32_rejects_tabulated_pk(gpu_ctx, fid_nz):
     tabs = tp.CosmologyTables([c], fid_nz, 100, 3.0, gpu_ctx, pk=(kh, 1e4 * kh / (1 + (kh / 0.02) ** 3)))
     with pytest.raises(tp.TpError):
         tabs.map3_cov_t1(np.full((1, 6), 4 * A), AREA, mode=tp.MODE_FP32)
+
+
+def test_batch_limits_end_runaway_integrals(gpu_ctx, fid_tables):
+    """The batched cubatures' guards: a tiny region or round cap ends a call with TP_ECAPACITY (code 4)
+    instead of letting it grow; slicing the host-rule rounds into small launches changes no bit."""
+    pairs = np.array([[8, 8, 8, 8, 8, 8]], dtype=float) * A
+    args = (pairs, tp.GEOMETRY_SQUARE, 60 * A, 0.0, 1.0, 0.0, 0.3, NGAL, tp.PK_SHAPENOISE_ONLY)
+    v0, n0 = fid_tables.map3_cov_t1_geometry(*args)
+    try:
+        gpu_ctx.set_batch_limits(round_points=10000)
+        v1, n1 = fid_tables.map3_cov_t1_geometry(*args)
+        assert np.array_equal(v0, v1) and np.array_equal(n0, n1)
+        gpu_ctx.set_batch_limits(max_regions=50)
+        with pytest.raises(tp.TpError, match=r"code 4\).*region limit"):
+            fid_tables.map3_cov_t1_geometry(*args)
+        with pytest.raises(tp.TpError, match=r"code 4\).*region limit"):
+            fid_tables.map2_batch(np.array([4 * A]))
+        gpu_ctx.set_batch_limits(max_rounds=3)
+        with pytest.raises(tp.TpError, match=r"code 4\).*round limit"):
+            fid_tables.map3_batch(np.array([[4 * A, 4 * A, 4 * A]]))
+        with pytest.raises(tp.TpError, match=r"code 4\).*round limit"):
+            fid_tables.pkappa_batch(np.array([100.0]), reltol=1e-12)
+    finally:
+        gpu_ctx.set_batch_limits()
+    v2, n2 = fid_tables.map3_cov_t1_geometry(*args)
+    assert np.array_equal(v0, v2) and np.array_equal(n0, n2)

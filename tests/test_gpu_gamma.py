@@ -68,3 +68,36 @@ def test_gamma_fp32_form_within_1e3(fid_tables, fid_oracle, gpu_ctx):
     assert rel.max() < 1e-3
     with pytest.raises(tp.TpError):
         fid_tables.gamma_batch(x, reltol=1e-2, mode=tp.MODE_SURROGATE)
+
+
+def test_gamma_orthocenter_convention(fid_tables, fid_oracle, gpu_ctx):
+    """triangle_center "orthocenter" (classed_version/gamma.cpp:12-13) returns the natural components
+    unrotated: the same integrals (equal point counts) and moduli as the centroid convention, and the
+    phase that the rotation of gamma.cpp:624-643 removes (gamma1..3 conjugate the first angle)."""
+    prec = (0.1, 2.0)
+    tp.gamma_setup(prec[0], prec[1], gpu_ctx)
+    x = _tri()
+    gc, nc = fid_tables.gamma_batch(x[None, :], reltol=1e-2, center=tp.CENTER_CENTROID)
+    go, no = fid_tables.gamma_batch(x[None, :], reltol=1e-2, center=tp.CENTER_ORTHOCENTER)
+    assert np.array_equal(nc, no)
+    assert np.allclose(np.abs(go), np.abs(gc), rtol=1e-13, atol=0)
+
+    def angle(a, b, c):  # one_rotation_angle_otc(a, b, c)
+        psi3 = np.arccos((a * a + b * b - c * c) / (2 * a * b))
+        h3 = 0.5 * np.sqrt(2 * a * a + 2 * b * b - c * c)
+        c2 = ((b * b - a * a) ** 2 - 4 * (a * b * np.sin(psi3)) ** 2) / (2 * h3 * c) ** 2
+        s2 = (b * b - a * a) * a * b * np.sin(psi3) / (h3 * c) ** 2
+        return np.arctan2(s2, c2) / 2
+
+    x1, x2, x3 = x
+    perms = [(x1, x2, x3), (x1, x2, x3), (x2, x3, x1), (x3, x1, x2)]
+    for i, (a, b, c) in enumerate(perms):
+        a1, a2, a3 = angle(b, c, a), angle(c, a, b), angle(a, b, c)
+        if i > 0:
+            a1 = -a1
+        want = go[0, i] * np.exp(-2j * (a1 + a2 + a3))
+        assert abs(want - gc[0, i]) <= 1e-12 * abs(gc[0, i]), (i, want, gc[0, i])
+    with pytest.raises(tp.TpError):
+        fid_tables.gamma_batch(x[None, :], reltol=1e-2, center=7)
+    with pytest.raises(ValueError):
+        tp.GammaCalculator(None, 0.1, 2.0, "incenter")

@@ -312,3 +312,65 @@ def test_round_log_and_timing_stats(gpu_ctx, fid_tables):
     assert np.all(L[:, 3] >= L[:, 2]) and np.all(L[:, 5] >= L[:, 4]) and np.all(L[:, 4] >= L[:, 3])
     assert 0 < st["integrand_ms"] <= st["integrand_sum_ms"] * (1 + 1e-6) + 1e-3
     assert st["kernel_ms"] >= st["integrand_ms"] - 1e-3
+
+
+def _single_z_points(n, seed):
+    """Random (k1, k2, k3) triangles [h/Mpc] over 1e-3..1e2 with random redshifts in (0, z_max]."""
+    rng = np.random.default_rng(seed)
+    k1 = np.exp(rng.uniform(np.log(1e-3), np.log(1e2), n))
+    k2 = np.exp(rng.uniform(np.log(1e-3), np.log(1e2), n))
+    c = rng.uniform(-1, 1, n)
+    k3 = np.sqrt(k1 * k1 + k2 * k2 + 2 * k1 * k2 * c)
+    z = rng.uniform(0, 3.0, n)
+    z[:4] = [3.0, 1e-6, 0.03, 2.97]  # the last table interval, the first one, and grid points
+    return np.stack([k1, k2, k3, z], 1)
+
+
+def test_bispec_eval_matches_oracle(fid_tables, fid_oracle):
+    """tp_bispec_eval (bispec at z through the reference's table-index rule) against the oracle's bispec at
+    the same interval, in the FP64 bound of the B_kappa test (1e-12 + 2e-14 k_max/k_min)."""
+    x = _single_z_points(400, 11)
+    g = fid_tables.bispec(x)
+    o = np.array([fid_oracle.scalar("bispec", *row) for row in x])
+    r = rel(g, o)
+    print("bispec max rel err %.3e" % r.max())
+    assert np.all(r <= 1e-12 + 2e-14 * squeeze(x[:, :3]))
+
+
+def test_bispec_eval_idx_matches_oracle(fid_tables, fid_oracle):
+    """The reference's 6-argument bispec(k1, k2, k3, z, idx, didx): the caller's table interval (also one
+    that does not match z), and z > 10, where the reference returns the tree-level bispectrum."""
+    from _oracle import lib
+    x = _single_z_points(300, 12)
+    rng = np.random.default_rng(13)
+    idx = rng.integers(0, 99, x.shape[0]).astype(np.float64)  # n_z = 100: idx in [0, 98]
+    didx = rng.uniform(0, 1, x.shape[0])
+    didx[:3] = [0.0, 1.0, 0.5]
+    x[-20:, 3] = rng.uniform(10.5, 20.0, 20)  # the tree-level branch
+    rows = np.column_stack([x, idx, didx])
+    g = fid_tables.bispec_idx(rows)
+    o = np.array([lib().orc_bispec_idx(fid_oracle.h, *r[:4], int(r[4]), r[5]) for r in rows])
+    r = rel(g, o)
+    print("bispec(idx) max rel err %.3e (tree branch %.3e)" % (r.max(), r[-20:].max()))
+    assert np.all(r <= 1e-12 + 2e-14 * squeeze(x[:, :3]))
+    for bad in (-1.0, 99.0, 2.5):
+        with pytest.raises(tp.TpError):
+            fid_tables.bispec_idx(np.array([[1.0, 1.0, 1.0, 1.0, bad, 0.5]]))
+
+
+def test_integrand_bkappa_eval_matches_oracle(fid_tables, fid_oracle):
+    """tp_integrand_bkappa_eval against the oracle's integrand_bkappa, with its zero guards (z = 0, l3 = 0)."""
+    rng = np.random.default_rng(14)
+    n = 400
+    l1 = np.exp(rng.uniform(1, 11, n))
+    l2 = np.exp(rng.uniform(1, 11, n))
+    l3 = np.sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * rng.uniform(-1, 1, n))
+    z = rng.uniform(0, 3.0, n)
+    z[0], l3[1], z[2] = 0.0, 0.0, 3.0
+    x = np.stack([z, l1, l2, l3], 1)
+    g = fid_tables.integrand_bkappa(x)
+    o = np.array([fid_oracle.scalar("integrand_bkappa", *row) for row in x])
+    assert g[0] == 0 and g[1] == 0 and o[0] == 0 and o[1] == 0
+    r = rel(g[2:], o[2:])
+    print("integrand_bkappa max rel err %.3e" % r.max())
+    assert np.all(r <= 1e-12 + 2e-14 * squeeze(x[2:, 1:]))

@@ -140,3 +140,52 @@ def test_lhs_is_latin():
     om = np.array([c.om for c in cs])
     strata = np.floor((om - 0.2) / 0.2 * 64).astype(int)
     assert sorted(strata) == list(range(64))
+
+
+CV_CALLER = r"""
+// A caller written against the classed_version class API (bispectrum.hpp:207-263,
+// apertureStatistics.hpp:223-232, gamma.hpp:33-77, cubature.h:55-94)
+#include <complex>
+#include <vector>
+#include "threepoint_amd.hpp"
+
+static int f(unsigned ndim, size_t npt, const double *x, void *fdata, unsigned fdim, double *fval)
+{
+    (void)ndim; (void)x; (void)fdata;
+    for (size_t i = 0; i < npt * fdim; ++i) fval[i] = 1.0;
+    return 0;
+}
+
+double caller(cosmology *cosmo, const std::vector<double> &nz)
+{
+    BispectrumCalculator bk(cosmo, nz, 100, 3.0);
+    bk.set_cosmology(cosmo);
+    double b = bk.bispec(0.1, 0.2, 0.25, 0.5, 16, 0.5) + bk.bispec(0.1, 0.2, 0.25, 0.5);
+    b += bk.bkappa(100., 200., 250.);
+    ell_params p{100., 200., 250.};
+    b += bk.integrand_bkappa(0.5, p) + bk.z_max;
+    ApertureStatistics as(&bk);
+    std::vector<double> th{1e-3, 2e-3, 4e-3};
+    b += as.MapMapMap(th) + as.integrand(10., 20., 0.5, th);
+    double in[3] = {10., 20., 0.5}, out = 0;
+    ApertureStatisticsContainer cont{&as, th};
+    ApertureStatistics::integrand(3, 1, in, &cont, 1, &out);
+    GammaCalculator g(&bk, 1e-3, 5., "orthocenter"), g2(&bk, 1e-3, 5., "centroid");
+    std::complex<double> c = g.gamma0(1e-3, 2e-3, 2.5e-3) + g.gamma1(1e-3, 2e-3, 2.5e-3) +
+                             g2.gamma2(1e-3, 2e-3, 2.5e-3) + g2.gamma3(1e-3, 2e-3, 2.5e-3);
+    double lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1}, v = 0, e = 0;
+    hcubature_v(1, f, nullptr, 3, lo, hi, 0, 0, 1e-4, ERROR_L1, &v, &e);
+    return b + c.real() + v;
+}
+"""
+
+
+def test_cv_style_caller_compiles(tmp_path):
+    """A C++ caller using the classed_version class signatures (including the 6-argument bispec and the
+    orthocenter triangle center) compiles against include/threepoint_amd.hpp unchanged."""
+    import subprocess
+    src = tmp_path / "cv_caller.cpp"
+    src.write_text(CV_CALLER)
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-I" + os.path.join(ROOT, "include"),
+                        str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr

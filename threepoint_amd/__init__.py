@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libthreepoint_amd.so")
 
 MODE_FP64, MODE_FP32, MODE_SURROGATE, MODE_ANALYTIC = 0, 1, 2, 3
+CENTER_CENTROID, CENTER_ORTHOCENTER = 0, 1  # triangle centers of the Gamma_i (gamma.cpp:12-16)
 # survey geometries of the covariance (the cuda_version's type codes)
 GEOMETRY_CIRCLE, GEOMETRY_SQUARE, GEOMETRY_INFINITE, GEOMETRY_RECTANGLE = 0, 1, 2, 3
 PK_LINEAR, PK_T17, PK_SHAPENOISE_ONLY = 1, 2, 4  # flags of the power-spectrum calls
@@ -118,12 +119,18 @@ _SIGS = {
     "tp_gamma_batch_mode": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
                                            ctypes.c_double, ctypes.c_int, _dp, _dp, _llp]),
     "tp_gamma_fp32_window": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
+    "tp_gamma_batch_center": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
+                                             ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _llp]),
+    "tp_bispec_eval_idx": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
+                                          _dp]),
     "tp_set_graph_rounds": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "tp_set_batch_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "tp_set_zero_copy_regions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
     "tp_set_round_log": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "tp_set_host_threads": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "tp_set_keep_pool_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_ulonglong]),
+    "tp_set_batch_limits": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong,
+                                           ctypes.c_longlong]),
     "tp_round_log": (ctypes.c_int, [ctypes.c_void_p, _dp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "tp_gamma_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
                                       ctypes.c_double, _dp, _dp, _llp]),
@@ -218,6 +225,12 @@ class Context:
     def set_keep_pool_bytes(self, nbytes):
         """Cap on the host region pools kept between batched calls (0: release and keep nothing)."""
         _check(lib().tp_set_keep_pool_bytes(self.h, int(nbytes)), "tp_set_keep_pool_bytes")
+
+    def set_batch_limits(self, max_rounds=0, max_regions=0, round_points=0):
+        """Guards of the batched cubatures (TP_ECAPACITY beyond them) and the host-rule batches' slice size;
+        0 restores a default."""
+        _check(lib().tp_set_batch_limits(self.h, int(max_rounds), int(max_regions), int(round_points)),
+               "tp_set_batch_limits")
 
     def set_round_log(self, on=True):
         _check(lib().tp_set_round_log(self.h, 1 if on else 0), "tp_set_round_log")
@@ -369,14 +382,37 @@ class CosmologyTables:
                                    _d(pts), _d(out)), "tp_gamma_eval")
         return out[:, 0] + 1j * out[:, 1]
 
-    def gamma_batch(self, x123, reltol=1e-3, icosmo=0, mode=MODE_FP64):
+    def gamma_batch(self, x123, reltol=1e-3, icosmo=0, mode=MODE_FP64, center=CENTER_CENTROID):
         x = np.ascontiguousarray(x123, dtype=np.float64).reshape(-1, 3)
         n = x.shape[0]
         re, im = np.zeros(4 * n), np.zeros(4 * n)
         ne = np.zeros(4 * n, dtype=np.int64)
-        _check(lib().tp_gamma_batch_mode(self.ctx.h, self.h, icosmo, n, _d(x), reltol, mode, _d(re), _d(im),
-                                         ne.ctypes.data_as(_llp)), "tp_gamma_batch")
+        _check(lib().tp_gamma_batch_center(self.ctx.h, self.h, icosmo, n, _d(x), reltol, mode, center, _d(re),
+                                           _d(im), ne.ctypes.data_as(_llp)), "tp_gamma_batch")
         return (re + 1j * im).reshape(n, 4), ne.reshape(n, 4)
+
+    def bispec(self, k123z, icosmo=0):
+        """bispec(k1, k2, k3, z) at rows (k1, k2, k3 [h/Mpc], z): the table interval of z."""
+        x = np.ascontiguousarray(k123z, dtype=np.float64).reshape(-1, 4)
+        out = np.zeros(x.shape[0])
+        _check(lib().tp_bispec_eval(self.ctx.h, self.h, icosmo, x.shape[0], _d(x), _d(out)), "tp_bispec_eval")
+        return out
+
+    def bispec_idx(self, k123z_idx, icosmo=0):
+        """bispec(k1, k2, k3, z, idx, didx) at rows of six (the reference's 6-argument form)."""
+        x = np.ascontiguousarray(k123z_idx, dtype=np.float64).reshape(-1, 6)
+        out = np.zeros(x.shape[0])
+        _check(lib().tp_bispec_eval_idx(self.ctx.h, self.h, icosmo, x.shape[0], _d(x), _d(out)),
+               "tp_bispec_eval_idx")
+        return out
+
+    def integrand_bkappa(self, zell, icosmo=0):
+        """integrand_bkappa at rows (z, ell1, ell2, ell3)."""
+        x = np.ascontiguousarray(zell, dtype=np.float64).reshape(-1, 4)
+        out = np.zeros(x.shape[0])
+        _check(lib().tp_integrand_bkappa_eval(self.ctx.h, self.h, icosmo, x.shape[0], _d(x), _d(out)),
+               "tp_integrand_bkappa_eval")
+        return out
 
 
     # ---------------- convergence power spectrum and <Map^2>
@@ -550,6 +586,15 @@ class BispectrumCalculator:
     def bkappa(self, ell1, ell2, ell3, mode=MODE_FP64):
         return float(self.tables_.bkappa(np.array([[ell1, ell2, ell3]]), 0, mode)[0])
 
+    def bispec(self, k1, k2, k3, z, idx=None, didx=None):
+        """bispec(k1, k2, k3, z, idx, didx) (bispectrum.hpp:247); without idx/didx the interval of z."""
+        if idx is None:
+            return float(self.tables_.bispec(np.array([[k1, k2, k3, z]]))[0])
+        return float(self.tables_.bispec_idx(np.array([[k1, k2, k3, z, idx, didx]]))[0])
+
+    def integrand_bkappa(self, z, ell1, ell2, ell3):
+        return float(self.tables_.integrand_bkappa(np.array([[z, ell1, ell2, ell3]]))[0])
+
 
 class ApertureStatistics:
     """classed_version/apertureStatistics.hpp:18-322: MapMapMap over (l1, l2, phi) with hcubature."""
@@ -564,17 +609,20 @@ class ApertureStatistics:
 
 
 class GammaCalculator:
-    """classed_version/gamma.hpp: Gamma_i via cubature + Ogata (centroid convention)."""
+    """classed_version/gamma.hpp: Gamma_i via cubature + Ogata; triangle_center "centroid" or "orthocenter"
+    (gamma.cpp:6-23)."""
 
     def __init__(self, bispectrum, prec_h, prec_k, triangle_center="centroid", reltol=1e-3):
-        if triangle_center != "centroid":
-            raise ValueError("only the centroid convention is provided")
+        centers = {"centroid": CENTER_CENTROID, "orthocenter": CENTER_ORTHOCENTER}
+        if triangle_center not in centers:
+            raise ValueError("no valid triangle center given; possible choices: orthocenter, centroid")
+        self.center = centers[triangle_center]
         self.bk = bispectrum
         self.reltol = reltol
         gamma_setup(prec_h, prec_k, self.bk.tables_.ctx)
 
     def gammas(self, x1, x2, x3):
-        g, _ = self.bk.tables_.gamma_batch(np.array([[x1, x2, x3]]), self.reltol)
+        g, _ = self.bk.tables_.gamma_batch(np.array([[x1, x2, x3]]), self.reltol, center=self.center)
         return g[0]
 
     def gamma0(self, x1, x2, x3):

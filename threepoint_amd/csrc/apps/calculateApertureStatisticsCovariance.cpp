@@ -8,7 +8,9 @@
 // The same 13 arguments as the reference.  Provided: T1 and T2 for every geometry (circle, square,
 // infinite, rectangular; T2 is 0 for the infinite one, as in the reference; T1 of a finite one is the
 // reference's 6-D integral at its tolerance 1e-2, --reltol sets the others); requesting T4-T7 is an
-// error (exit status 1).  For every pair i <= j of the
+// error (exit status 1).  --fp32 (FP32 Pell node values) exists for T1 of the infinite survey only;
+// with a finite geometry or with T2 it is an error (exit status 1), so no run silently writes FP64
+// numbers under that flag.  For every pair i <= j of the
 // theta_1 <= theta_2 <= theta_3 triples of the list it computes T1_total and T2_total, all pairs of a
 // term in one batched GPU pass, and writes each symmetric N x N matrix to
 // <folder>cov_<geometry>_term<k>Numerical_sigma_<s>_n_<n>_thetaMax_<t>_gpu.dat as the reference does
@@ -43,7 +45,8 @@ int main(int argc, char *argv[])
     const char *usage =
         "calculateApertureStatisticsCovariance.x <cosmology> <thetas [arcmin]> <n(z)> <output folder> <covariance "
         "parameters>\n    <T1> <T2> <T4> <T5> <T6> <T7> <T7_2h> <circle|square|infinite|rectangular> [--nz-bins N] "
-        "[--reltol X] [--no-t17] [--fp32]\n";
+        "[--reltol X] [--no-t17] [--fp32]\n"
+        "    --fp32: T1 of the infinite geometry only (FP32 Pell node values, FP64 sums)\n";
     if (argc < 14) {
         std::cerr << usage;
         return 1;
@@ -72,6 +75,11 @@ int main(int argc, char *argv[])
     const std::string type = argv[13];
     if (type != "circle" && type != "square" && type != "infinite" && type != "rectangular") {
         std::cerr << "Cov type not correctly specified" << std::endl;
+        return 1;
+    }
+    if (fp32 && (type != "infinite" || do_t2)) {
+        std::cerr << "--fp32 applies to T1 of the infinite geometry only (requested: geometry " << type
+                  << (do_t2 ? ", T2" : "") << ")" << std::endl;
         return 1;
     }
 

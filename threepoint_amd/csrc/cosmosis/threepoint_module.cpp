@@ -10,9 +10,13 @@
 // Written against cosmosis's C datablock API, so the library carries no cosmosis headers; the
 // datablock functions are resolved when cosmosis (libcosmosis) loads the module.
 //
-// Options ("threepoint" section), as the reference: nz_file, zBins, zMax, theta_unit, theta_min,
-// theta_max, theta_bins, diag_only, output_Map3, output_dir.  Additional, optional: reltol
-// (default 1e-4, the reference's tolerance) and precision ("fp64" default, or "fp32").
+// Options ("threepoint" section), as the reference: nz_file, zMax, theta_unit, theta_min,
+// theta_max, theta_bins, diag_only, output_Map3, output_dir, and zBins.  zBins is read by the
+// classed_version interface (interface.cpp:76); the cuda_version interface, which the shipped
+// pipeline (cosmosis/pipeline_gpu.ini) loads, has no such option and uses its fixed
+// n_redshift_bins = 256 (cuda_version/bispectrum.cuh:35, interface.cu:81), so an absent zBins
+// means 256 here.  Additional, optional: reltol (default 1e-4, the reference's tolerance) and
+// precision ("fp64" default, or "fp32").
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -44,6 +48,7 @@ DATABLOCK_STATUS c_datablock_put_double_array_1d(c_datablock *s, const char *sec
 namespace {
 
 constexpr int kModuleError = 1;  // status returned for failures outside the datablock
+constexpr int kDefaultZBins = 256;  // cuda_version/bispectrum.cuh:35 (n_redshift_bins)
 
 struct ModuleConfig {
     int zBins = 0;
@@ -82,7 +87,8 @@ void *setup(c_datablock *options)
     char *s = nullptr;
     bool ok = report(c_datablock_get_string(options, sec, "nz_file", &s), "nz_file");
     const std::string fn_nz = take_string(s);
-    ok = ok && report(c_datablock_get_int(options, sec, "zBins", &cfg->zBins), "zBins");
+    // zBins is optional: without it, the cuda_version's fixed bin count
+    if (c_datablock_get_int(options, sec, "zBins", &cfg->zBins) != 0) cfg->zBins = kDefaultZBins;
     ok = ok && report(c_datablock_get_double(options, sec, "zMax", &cfg->zMax), "zMax");
     s = nullptr;
     ok = ok && report(c_datablock_get_string(options, sec, "theta_unit", &s), "theta_unit");

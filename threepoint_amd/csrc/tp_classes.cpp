@@ -72,6 +72,13 @@ double BispectrumCalculator::bispec(double k1, double k2, double k3, double z)
     return out;
 }
 
+double BispectrumCalculator::bispec(double k1, double k2, double k3, double z, int idx, double didx)
+{
+    double in[6] = {k1, k2, k3, z, (double)idx, didx}, out = 0;
+    check(tp_bispec_eval_idx(ctx_, tables_, 0, 1, in, &out), "tp_bispec_eval_idx");
+    return out;
+}
+
 double BispectrumCalculator::bkappa(double ell1, double ell2, double ell3)
 {
     double in[3] = {ell1, ell2, ell3}, out = 0;
@@ -204,9 +211,14 @@ std::vector<double> ApertureStatistics::MapMapMap(const std::vector<std::array<d
 GammaCalculator::GammaCalculator(BispectrumCalculator *b, double prec_h, double prec_k, std::string center)
     : Bispectrum_(b)
 {
-    // classed_version/gamma.cpp:6-23
-    if (center != "centroid")
-        throw std::runtime_error("GammaCalculator: only the centroid triangle center is provided");
+    // classed_version/gamma.cpp:6-23 (the reference exits on an unknown center; this throws)
+    if (center == "orthocenter")
+        center_ = TP_CENTER_ORTHOCENTER;
+    else if (center == "centroid")
+        center_ = TP_CENTER_CENTROID;
+    else
+        throw std::runtime_error("GammaCalculator: no valid triangle center given; possible choices: "
+                                 "orthocenter, centroid");
     check(tp_gamma_setup(b->ctx(), prec_h, prec_k), "tp_gamma_setup");
 }
 
@@ -219,8 +231,8 @@ std::vector<std::array<std::complex<double>, 4>> GammaCalculator::gammas(const s
 {
     std::vector<double> in, re(4 * x.size()), im(4 * x.size());
     for (auto &t : x) in.insert(in.end(), t.begin(), t.end());
-    check(tp_gamma_batch_mode(Bispectrum_->ctx(), Bispectrum_->tables(), 0, x.size(), in.data(), reltol, mode,
-                              re.data(), im.data(), nullptr),
+    check(tp_gamma_batch_center(Bispectrum_->ctx(), Bispectrum_->tables(), 0, x.size(), in.data(), reltol, mode,
+                                center_, re.data(), im.data(), nullptr),
           "tp_gamma_batch");
     std::vector<std::array<std::complex<double>, 4>> out(x.size());
     for (size_t t = 0; t < x.size(); ++t)
@@ -264,6 +276,8 @@ std::vector<double> ApertureStatisticsCovariance::T1_total(const std::vector<std
     std::vector<double> th, out(pairs.size());
     for (auto &p : pairs) th.insert(th.end(), p.begin(), p.end());
     if (geometry_ != 2) {  // the finite geometries: 6-D, FP64, the reference's tolerance 1e-2
+        if (mode != TP_MODE_FP64)
+            throw std::invalid_argument("T1_total: the FP32 form exists for the infinite survey only");
         check(tp_map3_cov_t1_batch_geometry(Bispectrum_->ctx(), Bispectrum_->tables(), pairs.size(), nullptr,
                                             th.data(), geometry_, covPar_.thetaMax, covPar_.thetaMax_smaller,
                                             covPar_.area, lMin, covPar_.shapenoise_sigma, covPar_.galaxy_density,
@@ -286,6 +300,7 @@ double ApertureStatisticsCovariance::T1_total(const std::vector<double> &a, cons
 
 std::vector<double> ApertureStatisticsCovariance::T2_total(const std::vector<std::array<double, 6>> &pairs)
 {
+    if (mode != TP_MODE_FP64) throw std::invalid_argument("T2_total: the FP32 form exists for T1 only");
     std::vector<double> th, out(pairs.size());
     for (auto &p : pairs) th.insert(th.end(), p.begin(), p.end());
     check(tp_map3_cov_t2_batch(Bispectrum_->ctx(), Bispectrum_->tables(), pairs.size(), nullptr, th.data(), geometry_,

@@ -4,7 +4,8 @@
 // Follows classed_version/gamma.cpp: node tables (:133-188), geometry
 // (:121-219), the J6 / J2 Ogata sums (:291-318, :462-488), the Gamma0 / Gamma1
 // integrands (:320-336, :490-520), gamma0..3 with their prefactors (:27-81) and
-// the orthocenter -> centroid rotation (:624-643).
+// the orthocenter -> centroid rotation (:624-643) of the centroid convention (the
+// orthocenter convention returns the components unrotated, :12-13).
 //
 // Device layout: one 256-thread workgroup per cubature point.  The z-dependent
 // BiHalofit coefficients are computed once per point into LDS; each thread then
@@ -35,20 +36,10 @@ struct TpGammaDesc {
 // z-node record at an arbitrary redshift (the per-point analogue of k_nodes), without
 // the GL weight and Limber prefactor: integrand_bkappa = wz * B (classed_version/
 // bispectrum.cpp:471-541).  Returns false at z == 0 (the reference's zero guard).
-__device__ bool z_record(const TpCosmoConst &C, const double *T, int n_z, double z, TpNode64 *N)
+// The record from the table interval idx and the fraction d within it (compute_coefficients and
+// g/f_K_interpolated, bispectrum.cpp:914-920, 977-987); z enters E(z), (1 + z) and Omega_m(z).
+__device__ void z_record_at(const TpCosmoConst &C, const double *T, int n_z, double z, int idx, double d, TpNode64 *N)
 {
-    if (z == 0) return false;
-    double d = z / C.zmax * (n_z - 1);
-    int idx = (int)d;
-    d = d - idx;
-    if (idx == n_z - 1) {
-        idx = n_z - 2;
-        d = 1.;
-    }
-    if (idx < 0) {  // z outside the tables cannot come from the cubature domain [0, z_max]
-        idx = 0;
-        d = 0;
-    }
     double fK = T[idx] * (1 - d) + T[idx + 1] * d;
     double g = T[n_z + idx] * (1 - d) + T[n_z + idx + 1] * d;
     double D1 = T[2 * n_z + idx] * (1 - d) + T[2 * n_z + idx + 1] * d;
@@ -88,7 +79,50 @@ __device__ bool z_record(const TpCosmoConst &C, const double *T, int n_z, double
     double hr = C.h / rs;
     N->pk_pref = C.h * C.h * C.h * 2.0 * tpd::kPi * tpd::kPi * C.norm * C.norm / (hr * hr * hr);
     N->lnks = log(N->ks);
+}
+
+// The table interval of z as the reference's integrand_bkappa finds it (:506-513): idx = int(z / z_max *
+// (n - 1)), the last interval with fraction 1 at z = z_max.  A z outside [0, z_max], which the cubature
+// domains never produce, reads the nearest interval (extrapolating linearly) instead of past the tables.
+__device__ void z_interval(const TpCosmoConst &C, int n_z, double z, int *idx_out, double *d_out)
+{
+    double d = z / C.zmax * (n_z - 1);
+    int idx = (int)d;
+    d = d - idx;
+    if (idx > n_z - 2) {  // z >= z_max (at z = z_max: the last interval, fraction 1)
+        d += idx - (n_z - 2);
+        idx = n_z - 2;
+    }
+    if (idx < 0) {
+        idx = 0;
+        d = 0;
+    }
+    *idx_out = idx;
+    *d_out = d;
+}
+__device__ bool z_record(const TpCosmoConst &C, const double *T, int n_z, double z, TpNode64 *N)
+{
+    if (z == 0) return false;
+    int idx;
+    double d;
+    z_interval(C, n_z, z, &idx, &d);
+    z_record_at(C, T, n_z, z, idx, d, N);
     return true;
+}
+
+// Tree-level bispectrum, bispectrum.cpp:609-612 (its z > 10 branch, :174-175): D1^4 2 sum F2 P P with
+// the linear P(k) (the fit or the given table).
+__device__ double tree_bispec(const TpCosmoConst &C, double D1, double k1, double k2, double k3)
+{
+    const double p1 = tp_pk_lin<double>(C, k1, C.norm), p2 = tp_pk_lin<double>(C, k2, C.norm),
+                 p3 = tp_pk_lin<double>(C, k3, C.norm);
+    return pow(D1, 4) * 2. *
+           (tp_f2_tree<double>(k1, k2, k3) * p1 * p2 + tp_f2_tree<double>(k2, k3, k1) * p2 * p3 +
+            tp_f2_tree<double>(k3, k1, k2) * p3 * p1);
+}
+__device__ double table_lerp(const double *T, int n_z, int row, int idx, double d)
+{
+    return T[row * n_z + idx] * (1 - d) + T[row * n_z + idx + 1] * d;
 }
 
 __device__ __forceinline__ double interior_angle(double an1, double an2, double opp)
@@ -340,13 +374,28 @@ __global__ void __launch_bounds__(128) k_zpoint(const TpCosmoConst *__restrict__
     const TpCosmoConst &C = cc[icosmo];
     double v = 0.0;
     TpNode64 N;
-    if (what == 0) {  // bispec(k1, k2, k3, z): B at wavenumbers k [h/Mpc]
+    const double *T = tab + (size_t)icosmo * 6 * n_z;
+    if (what == 2) {  // bispec(k1, k2, k3, z, idx, didx): the caller's table interval (bispectrum.cpp:166-175)
+        const double *q = in + 6 * i;
+        const int idx = (int)q[4];
+        const double d = q[5];
+        if (q[3] > 10.) {
+            v = tree_bispec(C, table_lerp(T, n_z, 2, idx, d), q[0], q[1], q[2]);
+        } else {
+            z_record_at(C, T, n_z, q[3], idx, d, &N);
+            const double rs = 1.0 / N.inv_rs;
+            v = tp_bispec_q<double>(N, eh[icosmo], q[0] * rs, q[1] * rs, q[2] * rs);
+        }
+    } else if (what == 0) {  // bispec(k1, k2, k3, z): B at wavenumbers k [h/Mpc]
         double z = p[3];
-        if (z_record(C, tab + (size_t)icosmo * 6 * n_z, n_z, z, &N)) {
-            double rs = 1.0 / N.inv_rs;
-            v = tp_bispec_q<double>(N, eh[icosmo], p[0] * rs, p[1] * rs, p[2] * rs);
-        } else {  // z == 0: tables at z = 0 (the reference has no z guard in bispec)
-            z_record(C, tab + (size_t)icosmo * 6 * n_z, n_z, 1e-300, &N);
+        if (z > 10.) {
+            int idx;
+            double d;
+            z_interval(C, n_z, z, &idx, &d);
+            v = tree_bispec(C, table_lerp(T, n_z, 2, idx, d), p[0], p[1], p[2]);
+        } else {
+            if (!z_record(C, T, n_z, z, &N))  // z == 0: tables at z = 0 (the reference has no z guard in bispec)
+                z_record(C, T, n_z, 1e-300, &N);
             double rs = 1.0 / N.inv_rs;
             v = tp_bispec_q<double>(N, eh[icosmo], p[0] * rs, p[1] * rs, p[2] * rs);
         }
@@ -442,7 +491,14 @@ static int zpoint_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, int what, siz
     if (npts == 0) return 0;
     TP_HIP(hipSetDevice(ctx->device));
     int rc;
-    size_t bi = npts * 4 * sizeof(double), bo = npts * sizeof(double);
+    const size_t stride = what == 2 ? 6 : 4;
+    if (what == 2)
+        for (size_t i = 0; i < npts; ++i) {
+            const double ix = in[6 * i + 4];
+            if (!(ix >= 0 && ix <= c->n_z - 2) || ix != (double)(int)ix)
+                return tp_set_error(TP_EINVAL, "tp_bispec_eval_idx: idx must be an integer in [0, n_z - 2]");
+        }
+    size_t bi = npts * stride * sizeof(double), bo = npts * sizeof(double);
     if ((rc = ctx->d_in.ensure(bi)) || (rc = ctx->d_out.ensure(bo))) return rc;
     TP_HIP(hipMemsetAsync(ctx->d_flag.p, 0, sizeof(int), ctx->stream));
     TP_HIP(hipMemcpyAsync(ctx->d_in.p, in, bi, hipMemcpyHostToDevice, ctx->stream));
@@ -461,6 +517,12 @@ static int zpoint_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, int what, siz
 extern "C" int tp_bispec_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *k123z, double *out)
 {
     return zpoint_eval(ctx, c, icosmo, 0, npts, k123z, out);
+}
+
+extern "C" int tp_bispec_eval_idx(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *k123z_idx,
+                                  double *out)
+{
+    return zpoint_eval(ctx, c, icosmo, 2, npts, k123z_idx, out);
 }
 
 extern "C" int tp_integrand_bkappa_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t npts, const double *zell,
@@ -587,7 +649,16 @@ extern "C" int tp_gamma_batch(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t
 extern "C" int tp_gamma_batch_mode(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t ntri, const double *x123,
                                    double reltol, int mode, double *out_re, double *out_im, long long *neval)
 {
+    return tp_gamma_batch_center(ctx, c, icosmo, ntri, x123, reltol, mode, TP_CENTER_CENTROID, out_re, out_im, neval);
+}
+
+extern "C" int tp_gamma_batch_center(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t ntri, const double *x123,
+                                     double reltol, int mode, int center, double *out_re, double *out_im,
+                                     long long *neval)
+{
     if (!ctx || !c || !x123 || !out_re || !out_im) return tp_set_error(TP_EINVAL, "tp_gamma_batch: NULL");
+    if (center != TP_CENTER_CENTROID && center != TP_CENTER_ORTHOCENTER)
+        return tp_set_error(TP_EINVAL, "tp_gamma_batch: center must be TP_CENTER_CENTROID or TP_CENTER_ORTHOCENTER");
     if (mode != TP_MODE_FP64 && mode != TP_MODE_FP32)
         return tp_set_error(TP_EINVAL, "tp_gamma_batch: mode must be TP_MODE_FP64 or TP_MODE_FP32");
     if (!ctx->d_ogata) return tp_set_error(TP_EINVAL, "tp_gamma_batch: call tp_gamma_setup first");
@@ -620,11 +691,17 @@ extern "C" int tp_gamma_batch_mode(tp_ctx *ctx, const tp_cosmo *c, int icosmo, s
     std::vector<double> pts, vals;
     std::vector<int> pdesc;
     std::vector<size_t> offs;
+    long long regions = 0;
     while (!active.empty()) {
+        if (ctx->stats.rounds >= ctx->max_rounds)
+            return tp_set_error(TP_ECAPACITY, "tp_gamma_batch: round limit reached (integrals not converging)");
         size_t na = active.size();
         offs.assign(na + 1, 0);
         for (size_t a = 0; a < na; ++a) offs[a + 1] = offs[a] + G[active[a]].I.pending().size() * np;
         size_t npts = offs[na];
+        regions += (long long)(npts / np);
+        if (regions > ctx->max_regions)
+            return tp_set_error(TP_ECAPACITY, "tp_gamma_batch: region limit reached (integrals not converging)");
         pts.resize(npts * 3);
         vals.resize(npts * 2);
         pdesc.resize(npts);
@@ -665,12 +742,16 @@ extern "C" int tp_gamma_batch_mode(tp_ctx *ctx, const tp_cosmo *c, int icosmo, s
         r *= 27. / 8. * std::pow(C.om, 3) * std::pow(100. / 299792., 5);  // gamma.cpp:40-42
         r /= 3.;
         r /= (2 * std::pow(2 * M_PI, 3));
-        const TpGammaDesc &D = desc[k];
-        double a1 = one_rotation_angle_otc(D.x[1], D.x[2], D.x[0]);
-        double a2 = one_rotation_angle_otc(D.x[2], D.x[0], D.x[1]);
-        double a3 = one_rotation_angle_otc(D.x[0], D.x[1], D.x[2]);
-        if (G[k].comp != 0) a1 *= -1.;  // conjugate_phi1 for gamma1..3 (gamma.cpp:635-643)
-        r = r * std::exp(std::complex<double>(0, -2.) * (a1 + a2 + a3));
+        // the integrand's natural components refer to the orthocenter; the centroid convention rotates them
+        // (gamma.cpp:44-45, 67-68), the orthocenter convention keeps them (gamma.cpp:12-13)
+        if (center == TP_CENTER_CENTROID) {
+            const TpGammaDesc &D = desc[k];
+            double a1 = one_rotation_angle_otc(D.x[1], D.x[2], D.x[0]);
+            double a2 = one_rotation_angle_otc(D.x[2], D.x[0], D.x[1]);
+            double a3 = one_rotation_angle_otc(D.x[0], D.x[1], D.x[2]);
+            if (G[k].comp != 0) a1 *= -1.;  // conjugate_phi1 for gamma1..3 (gamma.cpp:635-643)
+            r = r * std::exp(std::complex<double>(0, -2.) * (a1 + a2 + a3));
+        }
         out_re[k] = r.real();
         out_im[k] = r.imag();
         if (neval) neval[k] = (long long)G[k].I.num_eval();

@@ -163,6 +163,14 @@ constexpr int TP_NLANES = 4;  // groups of a batched call (at most; tp_ctx::nlan
 
 constexpr size_t kTpKeepPoolBytes = size_t(16) << 30;  // default cap on the host region pools kept
 
+// Guards of the batched cubatures (tp_set_batch_limits): a call whose integrals need more rounds or
+// evaluate more regions than these ends with TP_ECAPACITY instead of growing without bound (the
+// measured workloads stay below 300 rounds and 1e8 regions); the host-rule batches also evaluate a
+// round in slices of at most kTpRoundPoints points, so the per-round point buffers stay bounded.
+constexpr long long kTpMaxRounds = 20000;
+constexpr long long kTpMaxRegions = 400000000LL;
+constexpr long long kTpRoundPoints = 1LL << 23;
+
 struct tp_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -194,6 +202,9 @@ struct tp_ctx {
     std::vector<tpc::AdaptiveIntegral31> map3_keep;              // also T1's (3-D) regions batches
     std::vector<tpc::AdaptiveIntegralT<2, 1>> map2_keep;        // <Map^2>'s (2-D)
     size_t keep_pool_bytes = kTpKeepPoolBytes;
+    long long max_rounds = kTpMaxRounds;     // tp_set_batch_limits
+    long long max_regions = kTpMaxRegions;
+    long long round_points = kTpRoundPoints;
     bool round_log_on = false;        // tp_set_round_log: per-round trace of tp_map3_batch
     std::vector<double> round_log;    // 7 doubles per round (tp_round_log)
     TpLane lane[TP_NLANES];

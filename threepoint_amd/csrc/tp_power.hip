@@ -552,38 +552,52 @@ int power_batch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpPowerDesc> &
     for (size_t k = 0; k < nint; ++k) active[k] = k;
     std::vector<double> pts, vals;
     std::vector<int> pdesc;
+    long long regions = 0;
+    const size_t budget = (size_t)std::max(1LL, ctx->round_points);
     while (!active.empty()) {
+        if (ctx->stats.rounds >= ctx->max_rounds)
+            return tp_set_error(TP_ECAPACITY, "batched cubature: round limit reached (integrals not converging)");
         const size_t na = active.size();
         offs.assign(na + 1, 0);
         for (size_t a = 0; a < na; ++a) offs[a + 1] = offs[a] + I[active[a]].pending().size() * np;
-        const size_t npts = offs[na];
-        pts.resize(npts * D);
-        vals.resize(npts);
-        pdesc.resize(npts);
-#pragma omp parallel for schedule(dynamic, 16) num_threads(nt) if (na > 64)
-        for (size_t a = 0; a < na; ++a) {
-            auto &A = I[active[a]];
-            const auto &pend = A.pending();
-            for (size_t r = 0; r < pend.size(); ++r) {
-                const size_t o = offs[a] + r * np;
-                tpc::rule_points_of(D, A.regions()[pend[r]], pts.data() + D * o);
-                for (unsigned q = 0; q < np; ++q) pdesc[o + q] = (int)active[a];
-            }
-        }
-        int rc = power_launch(ctx, c, desc, pdesc.data(), npts, (int)D, pts.data(), vals.data());
-        if (rc) return rc;
-        ctx->stats.rounds += 1;
-        ctx->stats.points += (long long)npts;
+        regions += (long long)(offs[na] / np);
+        if (regions > ctx->max_regions)
+            return tp_set_error(TP_ECAPACITY, "batched cubature: region limit reached (integrals not converging)");
         std::vector<char> fin(na, 0);
-#pragma omp parallel for schedule(dynamic, 16) num_threads(nt) if (na > 64)
-        for (size_t a = 0; a < na; ++a) {
-            auto &A = I[active[a]];
-            const auto &pend = A.pending();
-            for (size_t r = 0; r < pend.size(); ++r)
-                tpc::rule_apply(D, 1, A.regions()[pend[r]], vals.data() + offs[a] + r * np);
-            A.absorb();
-            fin[a] = A.advance() ? 1 : 0;
+        // the round in slices of whole integrals of at most `budget` points (at least one integral), so
+        // the point and value buffers stay bounded however many regions the integrals hold
+        for (size_t a0 = 0; a0 < na;) {
+            size_t a1 = a0 + 1;
+            while (a1 < na && offs[a1 + 1] - offs[a0] <= budget) ++a1;
+            const size_t base = offs[a0], npts = offs[a1] - base;
+            pts.resize(npts * D);
+            vals.resize(npts);
+            pdesc.resize(npts);
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt) if (a1 - a0 > 64)
+            for (size_t a = a0; a < a1; ++a) {
+                auto &A = I[active[a]];
+                const auto &pend = A.pending();
+                for (size_t r = 0; r < pend.size(); ++r) {
+                    const size_t o = offs[a] - base + r * np;
+                    tpc::rule_points_of(D, A.regions()[pend[r]], pts.data() + D * o);
+                    for (unsigned q = 0; q < np; ++q) pdesc[o + q] = (int)active[a];
+                }
+            }
+            int rc = power_launch(ctx, c, desc, pdesc.data(), npts, (int)D, pts.data(), vals.data());
+            if (rc) return rc;
+            ctx->stats.points += (long long)npts;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt) if (a1 - a0 > 64)
+            for (size_t a = a0; a < a1; ++a) {
+                auto &A = I[active[a]];
+                const auto &pend = A.pending();
+                for (size_t r = 0; r < pend.size(); ++r)
+                    tpc::rule_apply(D, 1, A.regions()[pend[r]], vals.data() + offs[a] - base + r * np);
+                A.absorb();
+                fin[a] = A.advance() ? 1 : 0;
+            }
+            a0 = a1;
         }
+        ctx->stats.rounds += 1;
         std::vector<size_t> next;
         for (size_t a = 0; a < na; ++a)
             if (!fin[a]) next.push_back(active[a]);
@@ -679,6 +693,7 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
     }
     std::vector<std::pair<float, float>> iv_pts, iv_ker;  // device intervals of the rounds [ms]
 
+    long long total_regions = 0;
     auto issue = [&](int g) -> int {
         Group &Gr = G[g];
         TpLane &L = ctx->lane[g];
@@ -689,6 +704,9 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
         for (size_t a = 0; a < na; ++a) Gr.offs[a + 1] = Gr.offs[a] + I[act[a]].pending().size();
         const size_t nreg = Gr.offs[na];
         Gr.nreg = nreg;
+        total_regions += (long long)nreg;
+        if (total_regions > ctx->max_regions)
+            return tp_set_error(TP_ECAPACITY, std::string(what) + ": region limit reached (integrals not converging)");
         const size_t br = nreg * sizeof(In), bo = nreg * sizeof(Out);
         int e;
         if ((e = L.d_regions.ensure(br)) || (e = L.d_rout.ensure(bo)) ||
@@ -795,6 +813,9 @@ int regions_batch(tp_ctx *ctx, const std::vector<TpPowerDesc> &desc, const doubl
             any = true;
             if ((rc = complete(g))) return fail(rc);
             if (!G[g].active.empty()) {
+                if (rounds[g] >= ctx->max_rounds)
+                    return fail(tp_set_error(TP_ECAPACITY, std::string(what) +
+                                                               ": round limit reached (integrals not converging)"));
                 if ((rc = issue(g))) return fail(rc);
                 ++rounds[g];
             }
@@ -1231,9 +1252,25 @@ extern "C" int tp_map3_cov_t1_batch_geometry(tp_ctx *ctx, const tp_cosmo *c, siz
     std::vector<double> v(desc.size());
     std::vector<long long> ne(desc.size());
     const double norm = std::pow(2 * tpd::kPi, 6);
-    rc = power_batch<6>(ctx, c, desc, lo.data(), hi.data(), reltol, [norm](double x) { return x / norm; }, v.data(),
-                        nullptr, ne.data());
-    if (rc) return rc;
+    // the 6-D integrals hold ~1e4-1e5 regions each: a chunk of kChunk integrals at a time bounds the
+    // host heaps (each chunk's integrators are released before the next one starts)
+    const size_t kChunk = 256;
+    tp_batch_stats tot{};
+    for (size_t k0 = 0; k0 < desc.size(); k0 += kChunk) {
+        const size_t k1 = std::min(desc.size(), k0 + kChunk);
+        const std::vector<TpPowerDesc> dk(desc.begin() + k0, desc.begin() + k1);
+        rc = power_batch<6>(ctx, c, dk, lo.data() + 6 * k0, hi.data() + 6 * k0, reltol,
+                            [norm](double x) { return x / norm; }, v.data() + k0, nullptr, ne.data() + k0);
+        if (rc) return rc;
+        tot.rounds += ctx->stats.rounds;
+        tot.points += ctx->stats.points;
+        tot.launches += ctx->stats.launches;
+        tot.kernel_ms += ctx->stats.kernel_ms;
+        tot.integrand_ms += ctx->stats.integrand_ms;
+        tot.integrand_sum_ms += ctx->stats.integrand_sum_ms;
+        tot.wall_ms += ctx->stats.wall_ms;
+    }
+    ctx->stats = tot;
     for (size_t i = 0; i < n; ++i) {
         double r = 0;
         long long np = 0;

@@ -362,13 +362,10 @@ extern "C" int tp_map3_single(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const 
 }
 
 // ----------------------------------------------------------- batched engine
-// Guards against integrals that stop converging (every mode): a run that needs more rounds or
-// regions than these ends with TP_ECAPACITY instead of growing without bound.  The adaptive
-// rounds of the measured workloads stay below 300 and 1e8 regions.
-static const long long kMaxRounds = 20000;
+// Guards against integrals that stop converging (every mode): ctx->max_rounds and ctx->max_regions
+// (tp_set_batch_limits) end a run that needs more with TP_ECAPACITY.
 // adaptive grouping of tp_map3_batch (two groups): split above, merge below these round sizes
 static const size_t kSplitRegions = 8192, kMergeRegions = 2048;
-static const long long kMaxRegions = 400000000LL;
 static const size_t kSurrogateMaxEval = 4000000;  // points per triple before an exact redo
 int tp_region_lanes(tp_ctx *ctx, const tp_cosmo *c, int mode, size_t np);  // tp_map3.hip
 int tp_surrogate_round(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegion *d_reg,
@@ -451,7 +448,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         TpLane &L = ctx->lane[g];
         auto th0 = clk::now();
         Gr.tl0 = ms_since(t0);
-        if (Gr.rounds >= kMaxRounds)
+        if (Gr.rounds >= ctx->max_rounds)
             return tp_set_error(TP_ECAPACITY, "tp_map3_batch: round limit reached (integrals not converging)");
         const size_t na = Gr.active.size();
         Gr.offs.assign(na + 1, 0);
@@ -459,7 +456,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         const size_t nreg = Gr.offs[na];
         Gr.nreg = nreg;
         total_regions += (long long)nreg;
-        if (total_regions > kMaxRegions)
+        if (total_regions > ctx->max_regions)
             return tp_set_error(TP_ECAPACITY, "tp_map3_batch: region limit reached (integrals not converging)");
         int e;
         if ((e = L.h_regions.ensure(sizeof(TpRegion) * nreg)) || (e = L.d_regions.ensure(sizeof(TpRegion) * nreg)) ||
@@ -723,6 +720,16 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         ctx->stats = keep;
     }
     ctx->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+extern "C" int tp_set_batch_limits(tp_ctx *ctx, long long max_rounds, long long max_regions, long long round_points)
+{
+    if (!ctx || max_rounds < 0 || max_regions < 0 || round_points < 0)
+        return tp_set_error(TP_EINVAL, "tp_set_batch_limits: limits must be >= 0 (0 = default)");
+    ctx->max_rounds = max_rounds ? max_rounds : kTpMaxRounds;
+    ctx->max_regions = max_regions ? max_regions : kTpMaxRegions;
+    ctx->round_points = round_points ? round_points : kTpRoundPoints;
     return 0;
 }
 
