@@ -109,6 +109,16 @@ int tp_integrand_bkappa_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t 
 int tp_map3_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const double theta[3], size_t npts,
                  const double *x, double *fval, int mode);
 
+/* tp_map3_eval for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): no counterpart in the reference.  Same arguments,
+ * kernels and results as tp_map3_eval, with at most TP_NOALLOC_MAX_POINTS points per call
+ * (TP_ECAPACITY beyond); the points and values are staged through one static device array per GPU
+ * (2 MiB, held until the process ends), so the call allocates nothing.  Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU (from several threads or contexts) race. */
+#define TP_NOALLOC_MAX_POINTS 65536
+int tp_map3_eval_noalloc(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const double theta[3], size_t npts,
+                         const double *x, double *fval, int mode);
+
 /* Replaces ApertureStatistics::MapMapMap (classed_version/apertureStatistics.cpp:321-353)
  * exactly as the reference drives it: tp_hcubature_v over [1, 10/theta_min]^2 x [0, 2pi]
  * (ERROR_L1, relative tolerance reltol) whose integrand callback sends each round's

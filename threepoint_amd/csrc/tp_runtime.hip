@@ -76,6 +76,38 @@ extern "C" int tp_device_count(int *n)
     return 0;
 }
 
+// ------------------------------------------------ staging of the entry point that allocates nothing
+// tp_map3_eval_noalloc stages its points through `scratch`, a file-scope device array (one copy per
+// GPU, held until the process ends): bytes [0, 256) hold the point kernel's flag word, then
+// TP_NOALLOC_MAX_POINTS x 3 doubles of points, then TP_NOALLOC_MAX_POINTS doubles of values.
+constexpr size_t kNoAllocIn = 256;
+constexpr size_t kNoAllocOut = kNoAllocIn + (size_t)TP_NOALLOC_MAX_POINTS * 3 * sizeof(double);
+constexpr size_t kNoAllocBytes = kNoAllocOut + (size_t)TP_NOALLOC_MAX_POINTS * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+// The array's device address on each GPU, looked up once per device with that device current: in host
+// code `scratch` names a host-side stand-in, never the device memory, so only these addresses are used.
+constexpr int kNoAllocMaxDevices = 64;
+static void *g_scratch_dev[kNoAllocMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_lookup(int device)  // device is current
+{
+    if (device < 0 || device >= kNoAllocMaxDevices)
+        return tp_set_error(TP_EINVAL, "device index beyond the static staging table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return 0;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        return tp_set_error(TP_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = p;
+    return 0;
+}
+
+namespace {
+__global__ void k_clear_word(int *w) { *w = 0; }
+}  // namespace
+
 extern "C" int tp_ctx_create(int device, tp_ctx **out)
 {
     if (!out) return tp_set_error(TP_EINVAL, "tp_ctx_create: NULL out");
@@ -97,6 +129,11 @@ extern "C" int tp_ctx_create(int device, tp_ctx **out)
     if (c->d_flag.ensure(256)) {
         delete c;
         return TP_EHIP;
+    }
+    int rc = scratch_lookup(device);  // once per device: the static staging of tp_map3_eval_noalloc
+    if (rc) {
+        tp_ctx_destroy(c);
+        return rc;
     }
     *out = c;
     return 0;
@@ -265,6 +302,37 @@ extern "C" int tp_map3_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const do
             return tp_launch_map3_points(ctx, c, icosmo, theta, npts, dx, dy, mode, fl);
         },
         "tp_map3_eval");
+}
+
+extern "C" int tp_map3_eval_noalloc(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const double theta[3], size_t npts,
+                                    const double *x, double *fval, int mode)
+{
+    int rc = check_cosmo(ctx, c, icosmo);
+    if (rc) return rc;
+    if (!theta || (npts && (!x || !fval))) return tp_set_error(TP_EINVAL, "tp_map3_eval_noalloc: NULL buffer");
+    if (mode != TP_MODE_FP64 && mode != TP_MODE_FP32 && mode != TP_MODE_ANALYTIC)
+        return tp_set_error(TP_EINVAL, "tp_map3_eval_noalloc: mode must be FP64, FP32 or ANALYTIC");
+    if (npts > (size_t)TP_NOALLOC_MAX_POINTS)
+        return tp_set_error(TP_ECAPACITY, "tp_map3_eval_noalloc: npts = " + std::to_string(npts) +
+                                              " exceeds TP_NOALLOC_MAX_POINTS = " +
+                                              std::to_string(TP_NOALLOC_MAX_POINTS) + " (split the batch)");
+    if (npts == 0) return 0;
+    TP_HIP(hipSetDevice(ctx->device));
+    unsigned char *base = ctx->device < kNoAllocMaxDevices ? (unsigned char *)g_scratch_dev[ctx->device] : nullptr;
+    if (!base) return tp_set_error(TP_EINVAL, "tp_map3_eval_noalloc: context without static staging");
+    int *d_flag = (int *)base;
+    double *d_in = (double *)(base + kNoAllocIn), *d_out = (double *)(base + kNoAllocOut);
+    // the flag word keeps the last call's contents: cleared by a kernel (no memset node if captured)
+    hipLaunchKernelGGL(k_clear_word, dim3(1), dim3(1), 0, ctx->stream, d_flag);
+    TP_HIP(hipGetLastError());
+    TP_HIP(hipMemcpyAsync(d_in, x, npts * 3 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    if ((rc = tp_launch_map3_points(ctx, c, icosmo, theta, npts, d_in, d_out, mode, d_flag))) return rc;
+    TP_HIP(hipMemcpyAsync(fval, d_out, npts * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    int flag = 0;
+    TP_HIP(hipMemcpyAsync(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    TP_HIP(hipStreamSynchronize(ctx->stream));
+    if (flag) return tp_set_error(TP_ENONFINITE, "tp_map3_eval_noalloc: non-finite ell or B_kappa value");
+    return 0;
 }
 
 // ------------------------------------------------------------- hcubature_v
