@@ -1,26 +1,25 @@
 #!/usr/bin/env python
-"""<Map^3> throughput on one MI355X: plain FP64 path vs batched / reduced-precision forms.
+"""<Map^3> throughput on one MI355X: plain FP64 path vs the surrogate form, with the batched exact forms and the
+CPU FP64 oracle beside it.
 
-Workload (one "step"): the 220 aperture-radius triples of 10 log-spaced radii
-0.5 ... 32 arcmin, flat wCDM h=0.7, Omega_m=0.3, Omega_b=0.047, sigma8=0.82,
-n_s=0.97, w=-1, n(z) ~ z^2 exp(-(z/0.5)^1.5) on [0, 3] (100 bins), each
-<Map^3> an h-adaptive cubature over (l1, l2, phi) with relative tolerance 1e-4.
-Synthetic inputs (no data files); the cosmology tables are built on the GPU once,
-before the timed region, and stay resident in HBM.
+Default workload (one "step"): 512 Latin-hypercube wCDM cosmologies (Omega_m 0.2-0.4, sigma8 0.7-0.95,
+h 0.6-0.8, w -1.2 - -0.8, n_s 0.92-1.0, Omega_b 0.047) x the 35 triples of radii {2, 4, 8, 16, 32} arcmin =
+17 920 integrals; n(z) ~ z^2 exp(-(z/0.5)^1.5) on [0, 3] (100 bins); each <Map^3> an h-adaptive cubature
+over (l1, l2, phi) with relative tolerance 1e-4.  A step builds the 512 cosmologies' FP64 tables on the
+GPU (timed) and then integrates.  --workload c1: the 220 triples of 10 log radii 0.5-32 arcmin of the
+fiducial flat LCDM cosmology.  Synthetic inputs, no data files.
 
 Paths timed in the same run:
-  plain_fp64    one triple after another through tp_map3_single: the reference's
-                hcubature_v driving an integrand callback that crosses to the GPU
-                every cubature round (cuda_version/apertureStatistics.cu:155-229).
-  batched_fp64  all triples advance together, one integrand + rule launch per round.
-  batched_fp32  same, FP32 bispectrum with FP64 z-sum and cubature accumulation.
-  surrogate     batched, learned B_kappa with exact checks (when a model is present).
-The paths run in the order given (by default FP32 before FP64: for about two calls after FP64
-calls an FP32 call runs ~15% slower, its two groups' kernels no longer overlapping, so each path is
-timed in its own steady state); errors are compared after all paths ran.
-
-value = triples/s of the fastest path whose max relative error against the
-batched FP64 result (itself within 1e-6 of the CPU FP64 oracle) is <= 1e-3.
+  plain_fp64    one triple after another through tp_map3_single: the reference's hcubature_v driving an
+                integrand callback that crosses to the GPU every cubature round
+                (cuda_version/apertureStatistics.cu:155-229); on the C3 workload one cosmology's 35 triples.
+  surrogate     batched; an integral's regions run the exact FP32 form until it holds 256 regions, then
+                the learned B_kappa (BF16 MFMA network) with a seeded 1% of points checked exactly.
+  batched_fp32  all integrals advance together; FP32 bispectrum, FP64 z-sum and cubature accumulation.
+  batched_fp64  the same in FP64: the reference's arithmetic, the parity anchor of the other paths.
+The headline (value) is the surrogate form when its max relative error against batched FP64 (itself
+within 1e-6 of the CPU FP64 oracle) is <= 1e-3 on every integral, else batched FP64; it runs --steps
+timed steps, the comparison paths --compare-steps.  Errors are compared after all paths ran.
 """
 import argparse
 import json
@@ -41,6 +40,16 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
 # of 256 MiB (FETCH_SIZE x2.000, WRITE_SIZE x1.000).  Algorithmic: 64/33 B of region in + 8 B of
 # integrand value out.
 HBM_BYTES_PER_POINT = {"double": 13.29, "float": 12.12}
+# The surrogate network 8 -> 4 x 256 -> 1: algorithmic flops per point 2 (8 x 256 + 3 x 256 x 256 + 256); the
+# inference kernel executes the three 256 x 256 layers on hi + lo BF16 activations (two MFMAs per weight
+# fragment) and layer 1 on the FP32 MFMA.
+MLP_FLOPS_PER_POINT = 2 * (8 * 256 + 3 * 256 * 256 + 256)
+MLP_MFMA_FLOPS_EXECUTED = 2 * (8 * 256 + 2 * 3 * 256 * 256)
+# HBM bytes per point of k_mlp_reg and its counters per point (rocprofv3 --pmc over the C3 surrogate step,
+# profiles/mlp_c3_counters.md); None until measured
+MLP_HBM_BYTES_PER_POINT = 40.0
+MLP_COUNTERS = None
+REGION_COUNTERS = {}
 FLOPS_PER_ELEM = 20.0
 CENSUS_FILE = os.path.join("tests", "golden", "census.json")
 
@@ -70,7 +79,10 @@ def cpu_model():
     return platform.processor()
 
 
-def run_path(tabs, ctx, thetas, path, steps, warmup, reltol, icosmo=None):
+def run_path(tabs, ctx, thetas, path, steps, warmup, reltol, icosmo=None, make_tables=None):
+    """Times `steps` steps of one path after `warmup` untimed ones.  With make_tables, every step first
+    builds the cosmologies' tables on the GPU (a new set of cosmologies, as a sampler step has) and the
+    step's time includes that build."""
     import threepoint_amd as tp
     mode = {"plain_fp64": tp.MODE_FP64, "batched_fp64": tp.MODE_FP64, "batched_fp32": tp.MODE_FP32,
             "batched_fp64_graph": tp.MODE_FP64, "batched_fp32_graph": tp.MODE_FP32,
@@ -80,21 +92,34 @@ def run_path(tabs, ctx, thetas, path, steps, warmup, reltol, icosmo=None):
     tp._check(tp.lib().tp_set_graph_rounds(ctx.h, 1 if path.endswith("_graph") else 0), "tp_set_graph_rounds")
 
     def one_step():
+        tb = tabs
+        t_tab = 0.0
+        if make_tables is not None:
+            t_tab = time.perf_counter()
+            tb = make_tables()
+            t_tab = time.perf_counter() - t_tab
         if path == "plain_fp64":
             vals = np.zeros(n)
             pts = 0
             for i in range(n):
-                v, e, ne = tabs.map3_single(thetas[i], reltol, 0 if icosmo is None else int(icosmo[i]), mode)
+                v, e, ne = tb.map3_single(thetas[i], reltol, 0 if icosmo is None else int(icosmo[i]), mode)
                 vals[i] = v
                 pts += ne
-            return vals, {"points": pts}
-        v, e, ne = tabs.map3_batch(thetas, icosmo, reltol, mode)
-        return v, ctx.stats()
+            st = {"points": pts}
+        else:
+            vals, e, ne = tb.map3_batch(thetas, icosmo, reltol, mode)
+            st = ctx.stats()
+        st["tables_ms"] = 1e3 * t_tab
+        if make_tables is not None:
+            tb.close()
+        return vals, st
 
     for _ in range(warmup):
         one_step()
     agg = {"points": 0, "integrand_ms": 0.0, "integrand_sum_ms": 0.0, "kernel_ms": 0.0, "launches": 0, "rounds": 0,
-           "host_ms": 0.0, "wait_ms": 0.0, "graph_rebuilds": 0}
+           "host_ms": 0.0, "wait_ms": 0.0, "graph_rebuilds": 0, "tables_ms": 0.0, "mlp_ms": 0.0, "mlp_launches": 0,
+           "mlp_points": 0, "surrogate_points": 0, "exact_checks": 0, "fallback_batches": 0,
+           "fallback_integrals": 0}
     t0 = time.perf_counter()
     vals = None
     for _ in range(steps):
@@ -103,6 +128,8 @@ def run_path(tabs, ctx, thetas, path, steps, warmup, reltol, icosmo=None):
             if k in st:
                 agg[k] += st[k]
     dt = time.perf_counter() - t0
+    agg["steps"] = steps
+    agg["warmup"] = warmup
     agg["ms_per_step"] = 1e3 * dt / steps
     agg["triples_per_s"] = n * steps / dt
     if path.startswith("batched_fp"):  # the launch shapes the tuner chose, by launch-size bucket
@@ -113,27 +140,35 @@ def run_path(tabs, ctx, thetas, path, steps, warmup, reltol, icosmo=None):
     return vals, agg
 
 
-def cpu_baseline(thetas, seconds_target, reltol):
-    """The CPU FP64 oracle (OpenMP over each integrand batch) on a bounded sample of the triples."""
+def cpu_baseline(cos, thetas, icosmo, seconds_target, reltol):
+    """The CPU FP64 oracle (OpenMP over each integrand batch) on a bounded sample of the step's triples."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from _oracle import Oracle
     from threepoint_amd import workloads as W
     cores = os.cpu_count() or 1
     cores = min(cores, 64)
-    o = Oracle(W.fiducial(), W.fiducial_nz())
-    # sample: every 11th triple of the 220 (20 triples spread over all radius combinations),
-    # stopped once the time budget is spent
-    order = list(range(0, thetas.shape[0], 11))
+    if icosmo is None:
+        # every 11th of the 220 triples (20 spread over all radius combinations)
+        order = list(range(0, thetas.shape[0], 11))
+        what = "%d of the 220 triples (every 11th, in order)"
+    else:
+        # the labelled subsample: the first 8 cosmologies x 35 triples, every 7th (40 integrals)
+        order = list(range(0, 8 * 35, 7))
+        what = "%d of the 280 integrals of the first 8 Latin-hypercube cosmologies x 35 triples (every 7th)"
+    oracles = {}
     done, t0 = 0, time.perf_counter()
     for i in order:
-        o.map3(thetas[i], reltol, cores)
+        ic = 0 if icosmo is None else int(icosmo[i])
+        if ic not in oracles:  # the cosmology's CPU tables are part of the CPU work
+            oracles[ic] = Oracle(cos[ic], W.fiducial_nz())
+        oracles[ic].map3(thetas[i], reltol, cores)
         done += 1
         if time.perf_counter() - t0 > seconds_target:
             break
     dt = time.perf_counter() - t0
     return {"value": done / dt, "unit": "triples/s", "cores": cores, "kind": "port",
-            "sample": "%d of the 220 triples (every 11th, in order) through the FP64 CPU oracle, OpenMP %d threads, "
-                      "%s, %.1f s" % (done, cores, cpu_model(), dt)}
+            "sample": (what + " through the FP64 CPU oracle incl. its table builds, OpenMP %d threads, %s, %.1f s")
+            % (done, cores, cpu_model(), dt)}
 
 
 def accuracy_study(tabs, ctx, thetas, icosmo, paths, reltol):
@@ -459,14 +494,17 @@ def cov_bench(tp, W, ctx, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--compare-steps", type=int, default=2,
+                    help="timed steps of the comparison paths (the headline path runs --steps)")
+    ap.add_argument("--compare-warmup", type=int, default=1)
     ap.add_argument("--reltol", type=float, default=1e-4)
-    ap.add_argument("--paths", default="plain_fp64,batched_fp32,batched_fp64,surrogate",
+    ap.add_argument("--paths", default="plain_fp64,surrogate,batched_fp32,batched_fp64",
                     help="paths to time (c2 takes plain_fp64, batched_fp64, batched_fp32)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", default="c1", choices=["c1", "c2", "c3", "map2", "cov"])
+    ap.add_argument("--workload", default="c3", choices=["c1", "c2", "c3", "map2", "cov"])
     ap.add_argument("--c2-triangles", type=int, default=10,
                     help="Gamma workload: number of the 1000 TreeCorr-grid triangles (one per r bin at 10)")
     ap.add_argument("--surrogate", default=os.path.join(ROOT, "threepoint_amd", "models", "bkappa_mlp.bin"))
@@ -501,21 +539,31 @@ def main():
     if args.workload == "cov":
         cov_bench(tp, W, ctx, args)
         return
-    t0 = time.perf_counter()
     icosmo = None
+    nz = W.fiducial_nz()
     if args.workload == "c1":
-        tabs = tp.CosmologyTables([W.fiducial()], W.fiducial_nz(), 100, 3.0, ctx)
+        cos = [W.fiducial()]
         thetas = W.to_rad(W.triples(W.map3_c1_radii()))
         wl = ("Map3 of 220 triples (10 log radii 0.5-32 arcmin), fiducial flat LCDM, 100 z-bins, "
-              "hcubature rel tol %g" % args.reltol)
+              "hcubature rel tol %g; a step builds the cosmology's FP64 tables on the GPU, then integrates"
+              % args.reltol)
     else:
         cos = W.lhs_c3(512)
-        tabs = tp.CosmologyTables(cos, W.fiducial_nz(), 100, 3.0, ctx)
         tri = W.to_rad(W.triples(W.map3_c3_radii()))
         thetas = np.tile(tri, (512, 1))
         icosmo = np.repeat(np.arange(512, dtype=np.int32), tri.shape[0])
         wl = ("Map3 of 512 Latin-hypercube wCDM cosmologies x 35 triples (radii 2-32 arcmin) = 17920 integrals, "
-              "per-cosmology FP64 GPU tables, 100 z-bins, hcubature rel tol %g" % args.reltol)
+              "100 z-bins, hcubature rel tol %g; a step builds the 512 cosmologies' FP64 tables on the GPU, "
+              "then integrates" % args.reltol)
+
+    def make_tables():
+        return tp.CosmologyTables(cos, nz, 100, 3.0, ctx)
+
+    def make_tables_first():  # the plain path's step: the first cosmology only
+        return tp.CosmologyTables(cos[:1], nz, 100, 3.0, ctx)
+
+    t0 = time.perf_counter()
+    tabs = make_tables()
     tables_ms = 1e3 * (time.perf_counter() - t0)
 
     paths = [p for p in args.paths.split(",") if p]
@@ -532,14 +580,22 @@ def main():
     fx = census_fixture(tp)
     census = fx["map3_point"]
     fe_point = fx["flop_eq"]["map3_point"]
+    # the headline is the surrogate form (the metric's target form) when a model is present, else batched FP64;
+    # it runs the requested steps, the comparison paths --compare-steps (their errors and rates are reported)
+    head_want = "surrogate" if have_sur else "batched_fp64"
     results, values = {}, {}
     for p in paths:
         print("bench: path %s ..." % p, file=sys.stderr, flush=True)
-        steps = 1 if p == "plain_fp64" else args.steps
-        th_p, ic_p = thetas, icosmo
-        if p == "plain_fp64" and args.workload == "c3":  # one cosmology's 35 triples
-            th_p, ic_p = thetas[:35], icosmo[:35]
-        v, agg = run_path(tabs, ctx, th_p, p, steps, 0 if p == "plain_fp64" else args.warmup, args.reltol, ic_p)
+        th_p, ic_p, mk = thetas, icosmo, make_tables
+        if p == "plain_fp64":
+            steps, warm = 1, 0
+            if args.workload == "c3":  # one cosmology's 35 triples
+                th_p, ic_p, mk = thetas[:35], icosmo[:35], make_tables_first
+        elif p == head_want:
+            steps, warm = args.steps, args.warmup
+        else:
+            steps, warm = args.compare_steps, args.compare_warmup
+        v, agg = run_path(tabs, ctx, th_p, p, steps, warm, args.reltol, ic_p, mk)
         values[p] = v
         results[p] = agg
     ref = values.get("batched_fp64")
@@ -547,13 +603,13 @@ def main():
         if ref is not None and values[p].shape == ref.shape:
             # triples whose first cubature region integrates to exactly 0 end at 0 in every path
             # (the reference algorithm's behaviour); they must stay exactly 0
-            nz = ref != 0
-            err = np.abs(values[p][nz] - ref[nz]) / np.abs(ref[nz])
-            zero_ok = bool(np.all(values[p][~nz] == 0))
+            nz_ = ref != 0
+            err = np.abs(values[p][nz_] - ref[nz_]) / np.abs(ref[nz_])
+            zero_ok = bool(np.all(values[p][~nz_] == 0))
             results[p]["max_rel_err_vs_fp64"] = float(err.max()) if zero_ok else float("inf")
-            results[p]["zero_triples"] = int((~nz).sum())
+            results[p]["zero_triples"] = int((~nz_).sum())
             if p != "batched_fp64":
-                idx = np.nonzero(nz)[0]
+                idx = np.nonzero(nz_)[0]
                 results[p]["rel_err_p50_p99_p999"] = [float(np.percentile(err, q)) for q in (50, 99, 99.9)]
                 results[p]["triples_over_1e-3"] = int((err > 1e-3).sum())
                 w = int(idx[np.argmax(err)])
@@ -561,53 +617,84 @@ def main():
                                               "icosmo": int(icosmo[w]) if icosmo is not None else 0,
                                               "fp64": float(ref[w]), "this_path": float(values[p][w])}
         r = results[p]
+        exact_pts = r["points"] - r.get("surrogate_points", 0)
         if r.get("integrand_sum_ms", 0) > 0:
             # per launch: work / event-timed duration (the sum over launches); the two groups' launches can
             # overlap on the device, so the union of their intervals (integrand_ms) is the device-busy time
-            ach = fe_point * r["points"] / (r["integrand_sum_ms"] * 1e-3)
-            peak = PEAK["fp32"] if p.startswith("batched_fp32") else PEAK["fp64"]
-            r["integrand_kernel"] = {"achieved_flops": ach, "peak": peak, "frac": ach / peak,
+            ach = fe_point * exact_pts / (r["integrand_sum_ms"] * 1e-3)
+            peak = PEAK["fp64"] if p.startswith("batched_fp64") or p == "plain_fp64" else PEAK["fp32"]
+            r["integrand_kernel"] = {"achieved_flops": ach, "peak": peak, "frac": ach / peak, "points": exact_pts,
                                      "avg_launch_ms": r["integrand_sum_ms"] / max(1, r["launches"]),
-                                     "device_busy_frac": fe_point * r["points"] / (r["integrand_ms"] * 1e-3) / peak}
+                                     "device_busy_frac": fe_point * exact_pts / (r["integrand_ms"] * 1e-3) / peak}
+        if r.get("mlp_ms", 0) > 0:
+            ach = MLP_FLOPS_PER_POINT * r["mlp_points"] / (r["mlp_ms"] * 1e-3)
+            r["mlp_kernel"] = {"achieved_flops": ach, "peak": PEAK["bf16"], "frac": ach / PEAK["bf16"],
+                               "points": r["mlp_points"], "launches": r["mlp_launches"],
+                               "avg_launch_ms": r["mlp_ms"] / max(1, r["mlp_launches"])}
     ok = [p for p in results if results[p].get("max_rel_err_vs_fp64", 1.0) <= 1e-3]
-    if ref is None:  # no FP64 reference in this run: the fastest path, unchecked
+    if ref is None:  # no FP64 reference in this run: the headline path, unchecked
         ok = list(results)
-    head = max(ok, key=lambda p: results[p]["triples_per_s"]) if ok else "batched_fp64"
+    head = head_want if head_want in ok else ("batched_fp64" if "batched_fp64" in results else
+                                              max(ok, key=lambda p: results[p]["triples_per_s"]))
     hr = results[head]
-    dom = hr.get("integrand_kernel") or next(r["integrand_kernel"] for r in results.values() if "integrand_kernel" in r)
-    prec = "float" if head.startswith("batched_fp32") else "double"
-    kname = "k_region_points<%s, 1/2/4/8 waves per point> | k_region_cached<%s> (shape tuned per launch size)" % (
-        prec, prec)
-    pts_per_launch = hr["points"] / max(1, hr["launches"])
-    # the region-cached shape evaluates the P_E terms (3 per node and point) once per distinct multipole of
-    # a region: 47 per 33 points instead of 99, so it executes less than the algorithmic work per point
-    fe_exec_cached = fx["flop_eq"]["map3_point_region_cached_executed"]
-    roof = {"bound": "valu", "kernel": kname,
-            "achieved": dom["achieved_flops"] / 1e12, "peak": dom["peak"] / 1e12, "unit": "TFLOP/s",
-            "frac": dom["frac"], "traffic": HBM_BYTES_PER_POINT[prec] * pts_per_launch,
-            "traffic_unit": "HBM bytes per launch (per-point bytes from the PMC passes x points per launch)",
-            "algorithmic_bytes_per_point": 64 / 33 + 8,
-            "work_per_point_flop_eq": fe_point, "flops_per_elementary_call": FLOPS_PER_ELEM,
-            "executed_work_per_point_flop_eq_region_cached": fe_exec_cached,
-            "executed_frac_if_all_region_cached": dom["frac"] * fe_exec_cached / fe_point,
-            "census": census, "census_file": CENSUS_FILE}
+    # the dominant kernel of the headline path: the network (MFMA) or the exact region kernel (VALU)
+    use_mlp = "mlp_kernel" in hr and ("integrand_kernel" not in hr or hr["mlp_ms"] >= hr["integrand_sum_ms"])
+    if use_mlp:
+        mk_ = hr["mlp_kernel"]
+        roof = {"bound": "mfma", "kernel": "k_mlp_reg (BF16 MFMA, hi/lo activations, FP32 accumulation)",
+                "achieved": mk_["achieved_flops"] / 1e12, "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s",
+                "frac": mk_["frac"],
+                "traffic": MLP_HBM_BYTES_PER_POINT * mk_["points"] / max(1, mk_["launches"]),
+                "traffic_unit": "HBM bytes per launch (per-point bytes from the PMC passes x points per launch)",
+                "algorithmic_flops_per_point": MLP_FLOPS_PER_POINT,
+                "executed_mfma_flops_per_point": MLP_MFMA_FLOPS_EXECUTED,
+                "counters": MLP_COUNTERS}
+        if "integrand_kernel" in hr:
+            roof["secondary"] = {"kernel": "exact FP32 region kernels of the surrogate form",
+                                 "achieved": hr["integrand_kernel"]["achieved_flops"] / 1e12,
+                                 "peak": hr["integrand_kernel"]["peak"] / 1e12,
+                                 "frac": hr["integrand_kernel"]["frac"], "unit": "TFLOP/s",
+                                 "ms": hr["integrand_sum_ms"], "mlp_ms": hr["mlp_ms"]}
+    else:
+        dom = hr.get("integrand_kernel") or next(r["integrand_kernel"] for r in results.values()
+                                                 if "integrand_kernel" in r)
+        prec = "double" if dom["peak"] == PEAK["fp64"] else "float"
+        kname = "k_region_points<%s, 1/2/4/8 waves per point> | k_region_cached<%s> (shape tuned per launch size)" % (
+            prec, prec)
+        pts_per_launch = dom["points"] / max(1, hr["launches"])
+        # the region-cached shape evaluates the P_E terms (3 per node and point) once per distinct multipole of
+        # a region: 47 per 33 points instead of 99, so it executes less than the algorithmic work per point
+        fe_exec_cached = fx["flop_eq"]["map3_point_region_cached_executed"]
+        roof = {"bound": "valu", "kernel": kname,
+                "achieved": dom["achieved_flops"] / 1e12, "peak": dom["peak"] / 1e12, "unit": "TFLOP/s",
+                "frac": dom["frac"], "traffic": HBM_BYTES_PER_POINT[prec] * pts_per_launch,
+                "traffic_unit": "HBM bytes per launch (per-point bytes from the PMC passes x points per launch)",
+                "algorithmic_bytes_per_point": 64 / 33 + 8,
+                "work_per_point_flop_eq": fe_point, "flops_per_elementary_call": FLOPS_PER_ELEM,
+                "executed_work_per_point_flop_eq_region_cached": fe_exec_cached,
+                "executed_frac_if_all_region_cached": dom["frac"] * fe_exec_cached / fe_point,
+                "census": census, "census_file": CENSUS_FILE, "counters": REGION_COUNTERS.get(prec)}
     line = {
         "metric": "Map3 triples/s on 1 GPU: plain FP64 vs surrogate form at <=1e-3 rel err",
         "value": hr["triples_per_s"], "unit": "triples/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": hr["ms_per_step"],
+        "steps": hr["steps"], "warmup": hr["warmup"], "ms_per_step": hr["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if head.startswith("batched_fp32") else ("bf16+f64" if head == "surrogate" else "f64"),
+        "dtype": {"batched_fp32": "f32", "surrogate": "bf16 network + f32 exact regions, f64 accumulation"}.get(
+            head, "f64"),
         "data": "synthetic",
-        "config": {"workload": wl,
-                   "path": head, "triples_per_step": int(thetas.shape[0]), "tables_ms": tables_ms},
+        "config": {"workload": wl, "path": head, "triples_per_step": int(thetas.shape[0]),
+                   "tables_ms_per_step": hr["tables_ms"] / max(1, hr["steps"]), "tables_ms_first_build": tables_ms,
+                   "comparison_paths_steps": {"steps": args.compare_steps, "warmup": args.compare_warmup}},
         "paths": results,
         "speedup_vs_plain_fp64": (hr["triples_per_s"] / results["plain_fp64"]["triples_per_s"]
                                   if "plain_fp64" in results else None),
         "roofline": roof,
     }
-    if not args.no_cpu and args.workload == "c1":
+    if "batched_fp32" in results and head == "surrogate":
+        line["surrogate_vs_exact_fp32"] = hr["triples_per_s"] / results["batched_fp32"]["triples_per_s"]
+    if not args.no_cpu:
         print("bench: cpu baseline ...", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(thetas, args.cpu_seconds, args.reltol)
+        line["cpu_baseline"] = cpu_baseline(cos, thetas, icosmo, args.cpu_seconds, args.reltol)
     print(json.dumps(line))
 
 

@@ -209,6 +209,9 @@ typedef struct {
     long long graph_rebuilds;    /* round graphs rebuilt because an update was refused */
     double integrand_sum_ms;     /* sum over launches of the integrand kernel's event-timed duration
                                     (launches x a per-launch profiler's average; >= integrand_ms) */
+    double mlp_ms;               /* surrogate mode: summed event-timed duration of the network's launches */
+    long long mlp_launches;      /* surrogate mode: launches of the network inference kernel           */
+    long long mlp_points;        /* surrogate mode: points those launches evaluated (padded to 64)     */
 } tp_batch_stats;
 int tp_last_batch_stats(const tp_ctx *ctx, tp_batch_stats *out);
 
@@ -417,6 +420,11 @@ int tp_surrogate_load(tp_ctx *ctx, const char *path);
  * are re-evaluated exactly; if the surrogate's error on them exceeds tol the batch
  * falls back to exact B_kappa. */
 int tp_surrogate_config(tp_ctx *ctx, double check_frac, double tol, uint64_t seed);
+/* Surrogate mode evaluates an integral's regions exactly (the FP32 form) until the integral holds
+ * nregions regions (default 256; 0: the surrogate from the first round).  An integral that the
+ * reference algorithm settles within that many regions -- where its early stopping decides the value --
+ * then takes the exact paths' decisions.  No counterpart in the reference. */
+int tp_surrogate_exact_regions(tp_ctx *ctx, long long nregions);
 /* y[i] = the network's output at in[8*i..] = (l1, l2, l3, Omega_m, sigma8, h, w, n_s): the learned
  * ln B_kappa - ln B_8, where B_8 is the 8-node FP32 Limber baseline of the same cosmology. */
 int tp_surrogate_predict(tp_ctx *ctx, size_t n, const double *in8, double *y);

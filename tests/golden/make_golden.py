@@ -4,6 +4,9 @@ These are outputs of the CPU FP64 oracle (oracle/tp_oracle.cpp), stored so the
 GPU parity tests do not have to rerun minutes of CPU cubature on the GPU host:
   map3_c0_fp64.json   <Map^3> of the 20 C0 triples (rel tol 1e-4) + points used
   map3_c1_fp64.json   <Map^3> of the 220 triples of 10 log radii 0.5-32 arcmin (rel tol 1e-4) + points used
+  gamma_c2_fp64.json  Gamma_0..3 of TreeCorr-grid triangles at the production Ogata grid (h 1e-3, 4999
+                      Bessel zeros), centroid convention, at the relative tolerance given (argument 2),
+                      + points used per component
   tables_fid.json     f_K, g, D1, r_sigma, n_eff, ncur, norm of the fiducial cosmology
   bkappa_c1_2k.npz    B_kappa at the first 2000 C1 dump points
   census.json         the op census of the integrand points (flops and elementary calls per
@@ -12,6 +15,7 @@ GPU parity tests do not have to rerun minutes of CPU cubature on the GPU host:
                       with an op-counting scalar on the host, and the flop-equivalents per unit at
                       20 flops per elementary call; bench.py's roofline reads it
 Usage: python tests/golden/make_golden.py [map3|map3_c1|tables|bkappa|census|all]
+       python tests/golden/make_golden.py gamma_c2 RELTOL TRIANGLE_INDEX...
 """
 import json
 import os
@@ -61,6 +65,28 @@ def main(what):
         json.dump({"reltol": 1e-4, "rows": rows}, open(os.path.join(HERE, "map3_c1_fp64.json"), "w"), indent=1)
 
 
+def gamma_c2(reltol, indices):
+    """Gamma_0..3 of the given TreeCorr-grid triangles (rows of workloads.gamma_c2_triangles()), appended to
+    gamma_c2_fp64.json (one entry per triangle and tolerance)."""
+    o = Oracle(W.fiducial(), W.fiducial_nz())
+    o.gamma_init(1e-3, 5.0)
+    fn = os.path.join(HERE, "gamma_c2_fp64.json")
+    out = json.load(open(fn)) if os.path.exists(fn) else {"prec_h": 1e-3, "prec_k_arg": 5.0, "rows": []}
+    tri = W.gamma_c2_triangles()
+    for i in indices:
+        x = tri[i, :3]
+        g, n = [], []
+        t0 = time.time()
+        for c in range(4):
+            v, ne = o.gamma(c, *x, reltol=reltol)
+            g.append([v.real, v.imag])
+            n.append(ne)
+            print(i, c, v, ne, "%.1fs" % (time.time() - t0), flush=True)
+        out["rows"].append({"triangle": int(i), "x123_rad": [float(a) for a in x], "reltol": reltol,
+                            "gamma_re_im": g, "neval": n, "cpu_seconds": time.time() - t0})
+        json.dump(out, open(fn, "w"), indent=1)
+
+
 def census(flops_per_elem=20.0):
     """The op census of the library's integrand device functions (tp_census_*) and the work per unit."""
     import threepoint_amd as tp
@@ -86,4 +112,7 @@ def census(flops_per_elem=20.0):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "all")
+    if len(sys.argv) > 1 and sys.argv[1] == "gamma_c2":
+        gamma_c2(float(sys.argv[2]), [int(a) for a in sys.argv[3:]])
+    else:
+        main(sys.argv[1] if len(sys.argv) > 1 else "all")

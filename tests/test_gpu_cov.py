@@ -128,7 +128,7 @@ def test_t1_fp32_rejects_tabulated_pk(gpu_ctx, fid_nz):
 def test_batch_limits_end_runaway_integrals(gpu_ctx, fid_tables):
     """The batched cubatures' guards: a tiny region or round cap ends a call with TP_ECAPACITY (code 4)
     instead of letting it grow; slicing the host-rule rounds into small launches changes no bit."""
-    pairs = np.array([[8, 8, 8, 8, 8, 8]], dtype=float) * A
+    pairs = np.array([[4, 8, 8, 6, 6, 12]], dtype=float) * A  # three 6-D integrals (permutations of 6, 6, 12)
     args = (pairs, tp.GEOMETRY_SQUARE, 60 * A, 0.0, 1.0, 0.0, 0.3, NGAL, tp.PK_SHAPENOISE_ONLY)
     v0, n0 = fid_tables.map3_cov_t1_geometry(*args)
     try:

@@ -314,6 +314,15 @@ def test_round_log_and_timing_stats(gpu_ctx, fid_tables):
     assert st["kernel_ms"] >= st["integrand_ms"] - 1e-3
 
 
+def _single_z_ok(r, sq):
+    """Single-z bispectra: rounding-level agreement (1e-12) where the triangle is not squeezed; in squeezed
+    triangles the three F2 P P terms of eq. B5 cancel to ~1/squeeze of their size, so the rounding
+    differences of the per-wavenumber factors (the device's log-space P_lin against pow, its merged
+    reciprocals) are amplified by the squeeze: the B_kappa bound 1e-12 + 2e-14 k_max/k_min."""
+    print("  max rel err / squeeze = %.3e" % np.max(r / sq))
+    return np.all(r[sq <= 100] <= 1e-12) and np.all(r <= 1e-12 + 2e-14 * sq)
+
+
 def _single_z_points(n, seed):
     """Random (k1, k2, k3) triangles [h/Mpc] over 1e-3..1e2 with random redshifts in (0, z_max]."""
     rng = np.random.default_rng(seed)
@@ -333,8 +342,9 @@ def test_bispec_eval_matches_oracle(fid_tables, fid_oracle):
     g = fid_tables.bispec(x)
     o = np.array([fid_oracle.scalar("bispec", *row) for row in x])
     r = rel(g, o)
-    print("bispec max rel err %.3e" % r.max())
-    assert np.all(r <= 1e-12 + 2e-14 * squeeze(x[:, :3]))
+    sq = squeeze(x[:, :3])
+    print("bispec max rel err %.3e (squeeze <= 100: %.3e)" % (r.max(), r[sq <= 100].max()))
+    assert _single_z_ok(r, sq)
 
 
 def test_bispec_eval_idx_matches_oracle(fid_tables, fid_oracle):
@@ -352,7 +362,7 @@ def test_bispec_eval_idx_matches_oracle(fid_tables, fid_oracle):
     o = np.array([lib().orc_bispec_idx(fid_oracle.h, *r[:4], int(r[4]), r[5]) for r in rows])
     r = rel(g, o)
     print("bispec(idx) max rel err %.3e (tree branch %.3e)" % (r.max(), r[-20:].max()))
-    assert np.all(r <= 1e-12 + 2e-14 * squeeze(x[:, :3]))
+    assert _single_z_ok(r, squeeze(x[:, :3]))
     for bad in (-1.0, 99.0, 2.5):
         with pytest.raises(tp.TpError):
             fid_tables.bispec_idx(np.array([[1.0, 1.0, 1.0, 1.0, bad, 0.5]]))
@@ -373,4 +383,24 @@ def test_integrand_bkappa_eval_matches_oracle(fid_tables, fid_oracle):
     assert g[0] == 0 and g[1] == 0 and o[0] == 0 and o[1] == 0
     r = rel(g[2:], o[2:])
     print("integrand_bkappa max rel err %.3e" % r.max())
-    assert np.all(r <= 1e-12 + 2e-14 * squeeze(x[2:, 1:]))
+    assert _single_z_ok(r, squeeze(x[2:, 1:]))
+
+
+def test_map3_c1_all_triples_match_oracle_fixture(fid_tables):
+    """Every one of the 220 triples of 10 log radii 0.5-32' (tests/golden/map3_c1_fp64.json: the CPU FP64
+    oracle at rel tol 1e-4) through the batched engine: FP64 within 1e-6 with the oracle's point counts,
+    the FP32 form within 1e-3, and the exact zeros (first region underflowing) kept."""
+    fx = json.load(open(os.path.join(GOLD, "map3_c1_fp64.json")))
+    th = W.to_rad(np.array([r["theta_arcmin"] for r in fx["rows"]]))
+    ref = np.array([r["map3"] for r in fx["rows"]])
+    nref = np.array([r["neval"] for r in fx["rows"]])
+    assert th.shape == (220, 3)
+    nz = ref != 0
+    v64, _, n64 = fid_tables.map3_batch(th, reltol=1e-4, mode=tp.MODE_FP64)
+    v32, _, _ = fid_tables.map3_batch(th, reltol=1e-4, mode=tp.MODE_FP32)
+    r64, r32 = rel(v64[nz], ref[nz]), rel(v32[nz], ref[nz])
+    print("C1 vs oracle: fp64 max rel %.3e (points equal: %d/220), fp32 max rel %.3e, zeros %d"
+          % (r64.max(), int((n64 == nref).sum()), r32.max(), int((~nz).sum())))
+    assert np.all(v64[~nz] == 0) and np.all(v32[~nz] == 0)
+    assert r64.max() < 1e-6 and np.array_equal(n64, nref)
+    assert r32.max() < 1e-3

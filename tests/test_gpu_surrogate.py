@@ -151,3 +151,35 @@ def test_surrogate_mode_gain_in_map3(gpu_ctx, fid_tables):
     print("untrained: surrogate share %.3f, fallback rounds %d, max rel err %.2e" % (frac2, st2["fallback_batches"], err2))
     _load_model(gpu_ctx)
     assert frac2 < 0.2 and st2["fallback_batches"] > 0 and err2 < 1e-3
+
+
+@pytest.mark.parametrize("workload", ["c1", "c3_subset"])
+def test_surrogate_form_within_1e3_of_fp64(gpu_ctx, fid_nz, workload):
+    """The surrogate form (exact FP32 regions until an integral holds 256 regions, then the network with
+    1% exact checks) against batched FP64 at the same tolerance 1e-4, on the 220 triples of 10 log radii
+    0.5-32' and on the labelled Latin-hypercube subset (the first 8 cosmologies and cosmology 14, whose
+    (4', 16', 16') integral the reference algorithm stops after 7 regions) x 35 triples of radii 2-32':
+    every integral within 1e-3, zeros kept."""
+    _load_model(gpu_ctx)
+    if workload == "c1":
+        tabs = tp.CosmologyTables([W.fiducial()], fid_nz, 100, 3.0, gpu_ctx)
+        th = W.to_rad(W.triples(W.map3_c1_radii()))
+        ic = None
+    else:
+        cos = W.lhs_c3(512)
+        pick = list(range(8)) + [14]
+        tabs = tp.CosmologyTables([cos[i] for i in pick], fid_nz, 100, 3.0, gpu_ctx)
+        tri = W.to_rad(W.triples(W.map3_c3_radii()))
+        th = np.tile(tri, (len(pick), 1))
+        ic = np.repeat(np.arange(len(pick), dtype=np.int32), tri.shape[0])
+    ref, _, nref = tabs.map3_batch(th, ic, 1e-4, tp.MODE_FP64)
+    v, _, n = tabs.map3_batch(th, ic, 1e-4, tp.MODE_SURROGATE)
+    st = gpu_ctx.stats()
+    nz = ref != 0
+    err = np.abs(v[nz] / ref[nz] - 1)
+    print("%s: surrogate share %.3f, fallback rounds %d, redone integrals %d, max rel err %.2e (p99 %.2e), points "
+          "%d vs %d" % (workload, st["surrogate_points"] / max(1, st["points"]), st["fallback_batches"],
+                        st["fallback_integrals"], err.max(), np.percentile(err, 99), n.sum(), nref.sum()))
+    assert np.all(v[~nz] == 0)
+    assert err.max() <= 1e-3
+    assert st["surrogate_points"] > 0.5 * st["points"]

@@ -47,7 +47,8 @@ class tp_batch_stats(ctypes.Structure):
                 ("fallback_batches", ctypes.c_longlong), ("max_check_relerr", ctypes.c_double),
                 ("fallback_integrals", ctypes.c_longlong), ("host_ms", ctypes.c_double),
                 ("wait_ms", ctypes.c_double), ("graph_rebuilds", ctypes.c_longlong),
-                ("integrand_sum_ms", ctypes.c_double)]
+                ("integrand_sum_ms", ctypes.c_double), ("mlp_ms", ctypes.c_double),
+                ("mlp_launches", ctypes.c_longlong), ("mlp_points", ctypes.c_longlong)]
 
 
 class tp_train_config(ctypes.Structure):
@@ -149,6 +150,7 @@ _SIGS = {
     "tp_surrogate_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "tp_surrogate_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "tp_surrogate_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]),
+    "tp_surrogate_exact_regions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
     "tp_surrogate_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _dp, _dp]),
     "tp_surrogate_bkappa": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
                                            _dp]),
@@ -231,6 +233,10 @@ class Context:
         0 restores a default."""
         _check(lib().tp_set_batch_limits(self.h, int(max_rounds), int(max_regions), int(round_points)),
                "tp_set_batch_limits")
+
+    def set_surrogate_exact_regions(self, n=256):
+        """Surrogate mode: integrals are evaluated exactly (FP32 form) until they hold n regions."""
+        _check(lib().tp_surrogate_exact_regions(self.h, int(n)), "tp_surrogate_exact_regions")
 
     def set_round_log(self, on=True):
         _check(lib().tp_set_round_log(self.h, 1 if on else 0), "tp_set_round_log")

@@ -110,15 +110,40 @@ __device__ bool z_record(const TpCosmoConst &C, const double *T, int n_z, double
     return true;
 }
 
+// F2 tree kernel from the wavenumbers as the reference forms it (bispectrum.cpp:629-633), contraction
+// off.  In squeezed triangles the three F2 P P terms of eq. B5 cancel to ~1/squeeze of their size, and a
+// rounding that breaks the consistency of the three cosines (an FMA in one of them, or wavenumbers
+// rescaled one by one) is amplified ~squeeze^2; formed this way the single-z entry points keep the
+// reference's cosines.
+__device__ double f2_tree_k(double k1, double k2, double k3)
+{
+#pragma clang fp contract(off)
+    double c12 = 0.5 * (k3 * k3 - k1 * k1 - k2 * k2) / (k1 * k2);
+    return (5. / 7.) + 0.5 * c12 * (k1 / k2 + k2 / k1) + (2. / 7.) * c12 * c12;
+}
+
+// BiHalofit at wavenumbers k [h/Mpc] (bispectrum.cpp:166-236): q = k r_sigma with the tables' r_sigma
+// (:177), r1 and r2 of eq. B8 from the sorted q (:179-192), the F2 kernels from k.
+__device__ double bispec_k(const TpNode64 &N, const TpEH64 &E, double rs, double k1, double k2, double k3)
+{
+    const double q[3] = {k1 * rs, k2 * rs, k3 * rs};
+    double a = q[0], b = q[1], c = q[2], t;
+    if (a < b) { t = a; a = b; b = t; }
+    if (b < c) { t = b; b = c; c = t; }
+    if (a < b) { t = a; a = b; b = t; }
+    const double lq[3] = {log(q[0]), log(q[1]), log(q[2])};
+    const double f2t[3] = {f2_tree_k(k1, k2, k3), f2_tree_k(k2, k3, k1), f2_tree_k(k3, k1, k2)};
+    return tp_bispec_node<double>(N, E, q, lq, f2t, log(c / a), (b + c - a) / a);
+}
+
 // Tree-level bispectrum, bispectrum.cpp:609-612 (its z > 10 branch, :174-175): D1^4 2 sum F2 P P with
 // the linear P(k) (the fit or the given table).
 __device__ double tree_bispec(const TpCosmoConst &C, double D1, double k1, double k2, double k3)
 {
     const double p1 = tp_pk_lin<double>(C, k1, C.norm), p2 = tp_pk_lin<double>(C, k2, C.norm),
                  p3 = tp_pk_lin<double>(C, k3, C.norm);
-    return pow(D1, 4) * 2. *
-           (tp_f2_tree<double>(k1, k2, k3) * p1 * p2 + tp_f2_tree<double>(k2, k3, k1) * p2 * p3 +
-            tp_f2_tree<double>(k3, k1, k2) * p3 * p1);
+    return pow(D1, 4) * 2. * (f2_tree_k(k1, k2, k3) * p1 * p2 + f2_tree_k(k2, k3, k1) * p2 * p3 +
+                              f2_tree_k(k3, k1, k2) * p3 * p1);
 }
 __device__ double table_lerp(const double *T, int n_z, int row, int idx, double d)
 {
@@ -375,34 +400,35 @@ __global__ void __launch_bounds__(128) k_zpoint(const TpCosmoConst *__restrict__
     double v = 0.0;
     TpNode64 N;
     const double *T = tab + (size_t)icosmo * 6 * n_z;
-    if (what == 2) {  // bispec(k1, k2, k3, z, idx, didx): the caller's table interval (bispectrum.cpp:166-175)
-        const double *q = in + 6 * i;
-        const int idx = (int)q[4];
-        const double d = q[5];
-        if (q[3] > 10.) {
+    if (what == 0 || what == 2) {  // bispec(k1, k2, k3, z[, idx, didx]): B at wavenumbers k [h/Mpc]
+        const double *q = what == 2 ? in + 6 * i : p;
+        const double z = q[3];
+        int idx;
+        double d;
+        if (what == 2) {  // the caller's table interval (bispectrum.cpp:166-175)
+            idx = (int)q[4];
+            d = q[5];
+        } else {          // the interval of z (:506-513); the reference has no z guard in bispec
+            z_interval(C, n_z, z, &idx, &d);
+        }
+        if (z > 10.) {
             v = tree_bispec(C, table_lerp(T, n_z, 2, idx, d), q[0], q[1], q[2]);
         } else {
-            z_record_at(C, T, n_z, q[3], idx, d, &N);
-            const double rs = 1.0 / N.inv_rs;
-            v = tp_bispec_q<double>(N, eh[icosmo], q[0] * rs, q[1] * rs, q[2] * rs);
+            z_record_at(C, T, n_z, z, idx, d, &N);
+            v = bispec_k(N, eh[icosmo], table_lerp(T, n_z, 3, idx, d), q[0], q[1], q[2]);
         }
-    } else if (what == 0) {  // bispec(k1, k2, k3, z): B at wavenumbers k [h/Mpc]
-        double z = p[3];
-        if (z > 10.) {
+    } else {  // integrand_bkappa(z, ell1, ell2, ell3), bispectrum.cpp:502-524: 0 at z = 0 or ell3 = 0
+        const double z = p[0];
+        if (z != 0.0 && p[3] != 0.0) {
             int idx;
             double d;
             z_interval(C, n_z, z, &idx, &d);
-            v = tree_bispec(C, table_lerp(T, n_z, 2, idx, d), p[0], p[1], p[2]);
-        } else {
-            if (!z_record(C, T, n_z, z, &N))  // z == 0: tables at z = 0 (the reference has no z guard in bispec)
-                z_record(C, T, n_z, 1e-300, &N);
-            double rs = 1.0 / N.inv_rs;
-            v = tp_bispec_q<double>(N, eh[icosmo], p[0] * rs, p[1] * rs, p[2] * rs);
+            z_record_at(C, T, n_z, z, idx, d, &N);
+            const double gv = table_lerp(T, n_z, 1, idx, d), fK = table_lerp(T, n_z, 0, idx, d);
+            const double E = sqrt(C.om * pow(1 + z, 3) + C.ow);  // E(z), bispectrum.cpp:967-970
+            const double b = bispec_k(N, eh[icosmo], table_lerp(T, n_z, 3, idx, d), p[1] / fK, p[2] / fK, p[3] / fK);
+            v = pow(gv * (1. + z), 3) * b / fK / E;
         }
-    } else {          // integrand_bkappa(z, ell1, ell2, ell3)
-        double z = p[0];
-        if (p[3] != 0.0 && z_record(C, tab + (size_t)icosmo * 6 * n_z, n_z, z, &N))
-            v = N.wz * tp_bispec_q<double>(N, eh[icosmo], p[1] * N.ks, p[2] * N.ks, p[3] * N.ks);
     }
     if (!isfinite(v)) atomicOr(flag, 8);
     out[i] = v;

@@ -188,6 +188,9 @@ struct tp_ctx {
     TpSurrogate *sur = nullptr;
     double sur_check_frac = 0.01, sur_tol = 1e-3;
     uint64_t sur_seed = 0x5eed;
+    // surrogate mode: an integral's regions are evaluated exactly (FP32 form) until the integral holds this
+    // many regions (tp_surrogate_exact_regions)
+    long long sur_exact_regions = 256;
     // launch shape of the region-point kernel: waves per point for [FP64, FP32][size bucket]
     int region_lpp[2][4] = {{1, 1, 1, 1}, {1, 1, 1, 1}};
     bool region_tuned[2] = {false, false};

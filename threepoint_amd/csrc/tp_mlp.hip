@@ -391,7 +391,28 @@ int grid(size_t n, int b) { return (int)((n + b - 1) / b); }
 }  // namespace
 
 // ================================================================ host side
+// Buffers of one group's surrogate rounds (each group of tp_map3_batch has its own, so the groups'
+// rounds can be in flight together on their streams).
+struct TpSurRoundBufs {
+    DevBuf X, F, Bb, flag, logb, Bex, resid, list, misc;
+    HostPinned stats;  // the round's check statistics, copied back at the end of the round
+    // events around the round's network launches (one pair per chunk) and those launches' points
+    std::vector<hipEvent_t> ev;
+    std::vector<long long> ev_points;
+    int nev = 0;
+    void release()
+    {
+        for (DevBuf *b : {&X, &F, &Bb, &flag, &logb, &Bex, &resid, &list, &misc}) b->release();
+        stats.release();
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+        ev.clear();
+        ev_points.clear();
+        nev = 0;
+    }
+};
+
 struct TpSurrogate {
+    TpSurRoundBufs rb[TP_NLANES];
     float *P = nullptr, *G = nullptr, *Am = nullptr, *Av = nullptr;
     unsigned short *Wb = nullptr;  // [3][H][H]
     uint4 *Wf = nullptr;           // [NFRAG] hidden weights, BF16, in k_mlp_reg's fragment order
@@ -454,6 +475,7 @@ static void sur_free(TpSurrogate *s)
     F(s->P); F(s->G); F(s->Am); F(s->Av); F(s->Wb); F(s->Wf); F(s->d_loss);
     F(s->X); F(s->T); F(s->Y); F(s->dy); F(s->parts);
     for (int i = 0; i < 4; ++i) { F(s->Z[i]); F(s->A[i]); F(s->dZ[i]); }
+    for (auto &b : s->rb) b.release();
     delete s;
 }
 
@@ -1348,43 +1370,42 @@ __global__ void __launch_bounds__(256) k_sur_rule(const TpRegion *__restrict__ r
 
 }  // namespace
 
-struct TpSurRoundBufs {
-    DevBuf X, F, Bb, flag, logb, Bex, resid, list, b0, b1, misc;
-};
-static TpSurRoundBufs g_rb;  // per process; tp_map3_batch calls on one context at a time
 
-int tp_surrogate_round(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegion *d_reg,
-                       const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int *d_flag, int *fell_back)
+// layout of misc / stats: [0] sum w r, [1] sum |w F B|, [2] sum (w r)^2, [3] checks, [4] exactly evaluated
+// points (as a double), then the worst checked relative error (float) and the chunk's list length (int)
+constexpr int kSurStatsD = 5;
+
+namespace {
+__global__ void k_sur_count(const int *n, double *total) { *total += (double)*n; }
+}  // namespace
+
+int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, const TpRegion *d_reg,
+                       const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int *d_flag, hipStream_t st,
+                       unsigned long long round_key)
 {
     TpSurrogate *s = ctx->sur;
     if (!s || !s->trained) return tp_set_error(TP_EINVAL, "surrogate mode without a trained model");
-    *fell_back = 0;
+    if (lane < 0 || lane >= TP_NLANES) return tp_set_error(TP_EINVAL, "tp_surrogate_queue: bad group");
     if (nreg == 0) return 0;
-    hipStream_t st = ctx->stream;
-    const size_t RC = 32768;  // regions per chunk (1 081 344 points)
+    const size_t RC = 32768;  // regions per chunk (1 081 344 points): bounds the per-point buffers
     const size_t PC = RC * TP_NPTS_GM3, PCP = (PC + 63) / 64 * 64;
     int rc;
-    TpSurRoundBufs &B = g_rb;
+    TpSurRoundBufs &B = s->rb[lane];
     if ((rc = B.X.ensure(PCP * NIN * sizeof(float))) || (rc = B.F.ensure(PC * sizeof(double))) ||
-        (rc = B.Bb.ensure(PC * sizeof(double))) ||
-        (rc = B.flag.ensure(PC * sizeof(int))) || (rc = B.logb.ensure(PCP * sizeof(double))) ||
-        (rc = B.Bex.ensure(PC * sizeof(double))) || (rc = B.resid.ensure(PC * sizeof(double))) ||
-        (rc = B.list.ensure(PC * sizeof(int))) || (rc = B.b0.ensure(PCP * H * 2)) || (rc = B.b1.ensure(PCP * H * 2)) ||
-        (rc = B.misc.ensure(64)))
+        (rc = B.Bb.ensure(PC * sizeof(double))) || (rc = B.flag.ensure(PC * sizeof(int))) ||
+        (rc = B.logb.ensure(PCP * sizeof(double))) || (rc = B.Bex.ensure(PC * sizeof(double))) ||
+        (rc = B.resid.ensure(PC * sizeof(double))) || (rc = B.list.ensure(PC * sizeof(int))) ||
+        (rc = B.misc.ensure(64)) || (rc = B.stats.ensure(64)))
         return rc;
-    double *dstats = (double *)B.misc.p;  // [0] sum w r, [1] sum |w F B|, [2] sum (w r)^2, [3] checks
-    float *dmaxrel = (float *)(dstats + 4);
-    int *dn = (int *)(dstats + 5);
+    double *dstats = (double *)B.misc.p;
+    float *dmaxrel = (float *)(dstats + kSurStatsD);
+    int *dn = (int *)(dmaxrel + 1);
     TP_HIP(hipMemsetAsync(B.misc.p, 0, 64, st));
-    // rounds too small to carry enough checked points run exactly (they are cheap)
-    if ((double)nreg * TP_NPTS_GM3 * ctx->sur_check_frac < 64.0) {
-        *fell_back = 2;
-        return tp_launch_regions(ctx, c, nreg, d_reg, d_desc, d_fvals, d_out, TP_MODE_FP64, d_flag, ctx->evm,
-                                 ctx->stream);
-    }
     const double frac = ctx->sur_check_frac;
-    const unsigned long long key = ctx->sur_seed * 0x9E3779B97F4A7C15ULL + (unsigned long long)ctx->stats.rounds * 7919ULL;
-    long long nexact = 0;
+    const unsigned long long key = ctx->sur_seed * 0x9E3779B97F4A7C15ULL + round_key * 7919ULL;
+    // the chunks follow each other on the stream; nothing here waits for the device
+    B.nev = 0;
+    B.ev_points.clear();
     for (size_t r0 = 0; r0 < nreg; r0 += RC) {
         size_t nr = std::min(RC, nreg - r0), np = nr * TP_NPTS_GM3, npp = (np + 63) / 64 * 64;
         TP_HIP(hipMemsetAsync(dn, 0, sizeof(int), st));
@@ -1392,9 +1413,16 @@ int tp_surrogate_round(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegi
                            c->d_base32, c->d_eh32, s->box, frac, key, (float *)B.X.p, (double *)B.F.p,
                            (double *)B.Bb.p, (int *)B.flag.p, (int *)B.list.p, dn, d_flag);
         if (npp > np) TP_HIP(hipMemsetAsync((float *)B.X.p + np * NIN, 0, (npp - np) * NIN * sizeof(float), st));
-        if ((rc = sur_infer(st, s, (const float *)B.X.p, (int)npp, (unsigned short *)B.b0.p, (unsigned short *)B.b1.p,
-                            (double *)B.logb.p)))
-            return rc;
+        while ((int)B.ev.size() < 2 * (B.nev + 1)) {
+            hipEvent_t e;
+            TP_HIP(hipEventCreate(&e));
+            B.ev.push_back(e);
+        }
+        TP_HIP(hipEventRecord(B.ev[2 * B.nev], st));
+        if ((rc = sur_infer(st, s, (const float *)B.X.p, (int)npp, nullptr, nullptr, (double *)B.logb.p))) return rc;
+        TP_HIP(hipEventRecord(B.ev[2 * B.nev + 1], st));
+        B.ev_points.push_back((long long)npp);
+        B.nev += 1;
         hipLaunchKernelGGL(k_sur_exact, dim3(grid(np * XL, 256)), dim3(256), 0, st, (const int *)B.list.p, dn, d_reg, r0,
                            d_desc, c->d_node64, c->d_eh64, (double *)B.Bex.p, d_flag);
         hipLaunchKernelGGL(k_sur_combine, dim3(grid(np, 256)), dim3(256), 0, st, np, (const int *)B.flag.p,
@@ -1404,28 +1432,33 @@ int tp_surrogate_round(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegi
                            (const double *)(d_fvals + r0 * TP_NPTS_GM3), (const double *)B.resid.p,
                            (const int *)B.flag.p, (const double *)B.F.p, (const double *)B.Bex.p, 1.0 / frac, d_out,
                            dstats);
+        hipLaunchKernelGGL(k_sur_count, dim3(1), dim3(1), 0, st, (const int *)dn, dstats + 4);
         TP_HIP(hipGetLastError());
-        int hn = 0;
-        TP_HIP(hipMemcpyAsync(&hn, dn, sizeof(int), hipMemcpyDeviceToHost, st));
-        TP_HIP(hipStreamSynchronize(st));
-        nexact += hn;
     }
-    double hs[4];
-    float hmax = 0;
-    TP_HIP(hipMemcpyAsync(hs, dstats, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
-    TP_HIP(hipMemcpyAsync(&hmax, dmaxrel, sizeof(float), hipMemcpyDeviceToHost, st));
-    TP_HIP(hipStreamSynchronize(st));
-    ctx->stats.exact_checks += nexact;
+    TP_HIP(hipMemcpyAsync(B.stats.p, B.misc.p, 64, hipMemcpyDeviceToHost, st));
+    return 0;
+}
+
+int tp_surrogate_result(tp_ctx *ctx, int lane, size_t nreg, bool *fallback)
+{
+    *fallback = false;
+    if (nreg == 0) return 0;
+    TpSurRoundBufs &B = ctx->sur->rb[lane];
+    for (int i = 0; i < B.nev; ++i) {  // the network's launches of the round (event-timed on their stream)
+        float ms = 0;
+        TP_HIP(hipEventElapsedTime(&ms, B.ev[2 * i], B.ev[2 * i + 1]));
+        ctx->stats.mlp_ms += ms;
+        ctx->stats.mlp_launches += 1;
+        ctx->stats.mlp_points += B.ev_points[i];
+    }
+    const double *hs = (const double *)B.stats.p;
+    const float hmax = *(const float *)(hs + kSurStatsD);
+    ctx->stats.exact_checks += (long long)hs[4];
     ctx->stats.max_check_relerr = std::max(ctx->stats.max_check_relerr, (double)hmax);
     // relative standard error of this round's correction: sqrt((1 - pi) sum (w r)^2) / sum |w F B|
-    double relse = hs[1] > 0 ? std::sqrt((1.0 - frac) * hs[2]) / hs[1] : 0.0;
-    if (relse > ctx->sur_tol) {
-        // the surrogate misses the tolerance on this round's checked points: exact re-evaluation
-        *fell_back = 1;
-        return tp_launch_regions(ctx, c, nreg, d_reg, d_desc, d_fvals, d_out, TP_MODE_FP64, d_flag, ctx->evm,
-                                 ctx->stream);
-    }
-    ctx->stats.surrogate_points += (long long)(nreg * TP_NPTS_GM3);
+    const double relse = hs[1] > 0 ? std::sqrt((1.0 - ctx->sur_check_frac) * hs[2]) / hs[1] : 0.0;
+    *fallback = relse > ctx->sur_tol;
+    if (!*fallback) ctx->stats.surrogate_points += (long long)(nreg * TP_NPTS_GM3);
     return 0;
 }
 

@@ -436,9 +436,11 @@ extern "C" int tp_map3_single(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const 
 static const size_t kSplitRegions = 8192, kMergeRegions = 2048;
 static const size_t kSurrogateMaxEval = 4000000;  // points per triple before an exact redo
 int tp_region_lanes(tp_ctx *ctx, const tp_cosmo *c, int mode, size_t np);  // tp_map3.hip
-int tp_surrogate_round(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegion *d_reg,
-                       const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int *d_flag,
-                       int *fell_back);  // tp_mlp.hip
+// tp_mlp.hip: a surrogate round queued on a group's stream, and its check statistics once it ran
+int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, const TpRegion *d_reg,
+                       const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int *d_flag, hipStream_t st,
+                       unsigned long long round_key);
+int tp_surrogate_result(tp_ctx *ctx, int lane, size_t nreg, bool *fallback);
 
 extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, const int *icosmo,
                              const double *theta_rad, double reltol, int mode, double *val, double *err,
@@ -478,12 +480,18 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     // Integrals are split into groups, each with its own stream: while the device evaluates
     // one group's regions, the host absorbs the other group's results and runs its heap
     // decisions (with two groups the split adapts to the round size, see below).  Each
-    // integral's decision sequence is unchanged.  A surrogate round synchronises inside (its
-    // fallback test reads the checks back), so it uses one group on the context's stream.
-    const int nl = (mode == TP_MODE_SURROGATE) ? 1 : std::max(1, std::min(ctx->nlanes, TP_NLANES));
+    // integral's decision sequence is unchanged.
+    // Surrogate mode: the regions of integrals that hold fewer than ctx->sur_exact_regions regions are
+    // evaluated exactly (the FP32 form), so an integral that the reference algorithm settles in a few
+    // rounds takes the same decisions as the exact paths; the others go through the surrogate with its
+    // checks, whose statistics are read when the round completes (a round that misses the tolerance is
+    // then re-evaluated exactly before its results are absorbed).
+    const int nl = std::max(1, std::min(ctx->nlanes, TP_NLANES));
+    const int exact_mode = mode == TP_MODE_SURROGATE ? TP_MODE_FP32 : mode;
     struct Group {
         std::vector<size_t> active, offs;
         size_t nreg = 0;
+        size_t nexact = 0;  // surrogate mode: the round's first nexact regions are evaluated exactly
         long long rounds = 0;
         bool inflight = false;
         double tl0 = 0, tl1 = 0;  // host time of the round's launch [ms since the call began] (round log)
@@ -491,8 +499,8 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     Group G[TP_NLANES];
     for (size_t t = 0; t < ntriples; ++t) G[nl == 1 ? 0 : (int)(t % nl)].active.push_back(t);
     // launch-shape tuning (first batched call of this precision) before any round is timed
-    if (mode == TP_MODE_FP64 || mode == TP_MODE_FP32 || mode == TP_MODE_SURROGATE)
-        if (tp_region_lanes(ctx, c, mode == TP_MODE_FP32 ? TP_MODE_FP32 : TP_MODE_FP64, 0) < 0) return TP_EHIP;
+    if (exact_mode == TP_MODE_FP64 || exact_mode == TP_MODE_FP32)
+        if (tp_region_lanes(ctx, c, exact_mode, 0) < 0) return TP_EHIP;
     // each group's rounds go to its own stream (one group: the context's stream), ordered after the
     // descriptor upload above
     hipStream_t gst[TP_NLANES];
@@ -519,10 +527,21 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         if (Gr.rounds >= ctx->max_rounds)
             return tp_set_error(TP_ECAPACITY, "tp_map3_batch: round limit reached (integrals not converging)");
         const size_t na = Gr.active.size();
+        size_t na_exact = 0;
+        if (mode == TP_MODE_SURROGATE) {  // integrals still small enough for exact evaluation come first
+            auto mid = std::stable_partition(Gr.active.begin(), Gr.active.end(), [&](size_t t) {
+                return (long long)I[t].num_regions() < ctx->sur_exact_regions;
+            });
+            na_exact = (size_t)(mid - Gr.active.begin());
+        }
         Gr.offs.assign(na + 1, 0);
         for (size_t a = 0; a < na; ++a) Gr.offs[a + 1] = Gr.offs[a] + I[Gr.active[a]].pending().size();
         const size_t nreg = Gr.offs[na];
         Gr.nreg = nreg;
+        // the surrogate part of a round needs enough checked points; a smaller part runs exactly too
+        Gr.nexact = mode != TP_MODE_SURROGATE ? nreg : Gr.offs[na_exact];
+        if (mode == TP_MODE_SURROGATE && (double)(nreg - Gr.nexact) * TP_NPTS_GM3 * ctx->sur_check_frac < 64.0)
+            Gr.nexact = nreg;
         total_regions += (long long)nreg;
         if (total_regions > ctx->max_regions)
             return tp_set_error(TP_ECAPACITY, "tp_map3_batch: region limit reached (integrals not converging)");
@@ -562,7 +581,6 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
             ctx->stats.points += (long long)nreg * TP_NPTS_GM3;
             return 0;
         }
-        if (mode == TP_MODE_SURROGATE) TP_HIP(hipMemsetAsync(L.d_flag.p, 0, sizeof(int), gst[g]));
         // a small exact round reads its regions from, and writes its results to, the pinned host
         // buffers directly: no copy in or out (per-round latency is what the tail rounds cost)
         const bool zc = mode != TP_MODE_SURROGATE && nreg <= (size_t)ctx->zero_copy_regions;
@@ -570,11 +588,18 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
             TP_HIP(hipMemcpyAsync(L.d_regions.p, hr, sizeof(TpRegion) * nreg, hipMemcpyHostToDevice, gst[g]));
         TP_HIP(hipEventRecord(L.ev0, gst[g]));
         if (mode == TP_MODE_SURROGATE) {
-            int fb = 0;
-            e = tp_surrogate_round(ctx, c, nreg, (const TpRegion *)L.d_regions.p, (const TpIntegralDesc *)ctx->d_desc.p,
-                                   (double *)L.d_fvals.p, (TpRegionOut *)L.d_rout.p, (int *)L.d_flag.p, &fb);
-            if (e) return e;
-            ctx->stats.fallback_batches += (fb == 1);
+            const TpRegion *dr = (const TpRegion *)L.d_regions.p;
+            TpRegionOut *dout = (TpRegionOut *)L.d_rout.p;
+            double *dfv = (double *)L.d_fvals.p;
+            if (Gr.nexact > 0 && (e = tp_launch_regions(ctx, c, Gr.nexact, dr, (const TpIntegralDesc *)ctx->d_desc.p,
+                                                         dfv, dout, exact_mode, (int *)L.d_flag.p, L.evm, gst[g])))
+                return e;
+            if (nreg > Gr.nexact &&
+                (e = tp_surrogate_queue(ctx, c, g, nreg - Gr.nexact, dr + Gr.nexact,
+                                        (const TpIntegralDesc *)ctx->d_desc.p, dfv + Gr.nexact * TP_NPTS_GM3,
+                                        dout + Gr.nexact, (int *)L.d_flag.p, gst[g],
+                                        (unsigned long long)(Gr.rounds * TP_NLANES + g))))
+                return e;
         } else {
             const TpRegion *rin = zc ? (const TpRegion *)L.h_regions.dp : (const TpRegion *)L.d_regions.p;
             TpRegionOut *rout = zc ? (TpRegionOut *)L.h_rout.dp : (TpRegionOut *)L.d_rout.p;
@@ -594,7 +619,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         ctx->stats.launches += 1;
         ctx->stats.regions += (long long)nreg;
         ctx->stats.points += (long long)nreg * TP_NPTS_GM3;
-        if (nl == 1) ctx->stats.rounds += 1;  // the surrogate's check sampling is keyed on it
+        if (nl == 1) ctx->stats.rounds += 1;
         return 0;
     };
 
@@ -631,7 +656,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         TP_HIP(hipEventElapsedTime(&ta, ctx->ev_batch, L.ev0));
         TP_HIP(hipEventElapsedTime(&tb, ctx->ev_batch, L.ev1));
         iv_ker.push_back({ta, tb});
-        if (mode != TP_MODE_SURROGATE) {
+        if (mode != TP_MODE_SURROGATE || Gr.nexact > 0) {  // the exact points kernel ran (evm follows it)
             TP_HIP(hipEventElapsedTime(&tm, ctx->ev_batch, L.evm));
             iv_pts.push_back({ta, tm});
             ctx->stats.integrand_sum_ms += tm - ta;
@@ -640,6 +665,26 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         // the point kernels' flag word: in region 0's flag (exact rounds) or copied back (surrogate rounds)
         const int pflag = mode == TP_MODE_SURROGATE ? *(const int *)L.h_flag.p : (Gr.nreg ? ro[0].flag >> 1 : 0);
         if (pflag) return tp_set_error(TP_ENONFINITE, "tp_map3_batch: non-finite ell or B_kappa value");
+        if (mode == TP_MODE_SURROGATE && Gr.nreg > Gr.nexact) {
+            bool fb = false;
+            const size_t ns = Gr.nreg - Gr.nexact;
+            int e = tp_surrogate_result(ctx, g, ns, &fb);
+            if (e) return e;
+            if (fb) {  // the surrogate missed the tolerance on this round's checks: exact re-evaluation
+                ctx->stats.fallback_batches += 1;
+                TpRegionOut *dout = (TpRegionOut *)L.d_rout.p + Gr.nexact;
+                if ((e = tp_launch_regions(ctx, c, ns, (const TpRegion *)L.d_regions.p + Gr.nexact,
+                                           (const TpIntegralDesc *)ctx->d_desc.p,
+                                           (double *)L.d_fvals.p + Gr.nexact * TP_NPTS_GM3, dout, exact_mode,
+                                           (int *)L.d_flag.p, L.evm, gst[g])))
+                    return e;
+                TP_HIP(hipMemcpyAsync((TpRegionOut *)L.h_rout.p + Gr.nexact, dout, sizeof(TpRegionOut) * ns,
+                                      hipMemcpyDeviceToHost, gst[g]));
+                TP_HIP(hipStreamSynchronize(gst[g]));
+                if (ro[Gr.nexact].flag >> 1)
+                    return tp_set_error(TP_ENONFINITE, "tp_map3_batch: non-finite ell or B_kappa value");
+            }
+        }
         auto th0 = clk::now();
         const std::vector<size_t> &act = Gr.active;
         const std::vector<size_t> &offs = Gr.offs;
@@ -745,7 +790,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         val[t] = r / 8 / M_PI / M_PI / M_PI;
         if (err) err[t] = e / 8 / M_PI / M_PI / M_PI;
         if (neval) neval[t] = (long long)I[t].num_eval();
-        // surrogate mode: a triple whose correction has a standard error above tol is redone exactly
+        // surrogate mode: a triple whose correction has a standard error above tol is redone exactly (FP32 form)
         if (mode == TP_MODE_SURROGATE &&
             (!I[t].converged_flag() || !(std::sqrt(I[t].correction_var()) <= ctx->sur_tol * std::fabs(r))))
             redo_flag[t] = 1;
@@ -767,7 +812,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
             ic[k] = icosmo[redo[k]];
             for (int d = 0; d < 3; ++d) th[3 * k + d] = theta_rad[3 * redo[k] + d];
         }
-        rc = tp_map3_batch(ctx, c, redo.size(), ic.data(), th.data(), reltol, TP_MODE_FP64, v.data(), e.data(),
+        rc = tp_map3_batch(ctx, c, redo.size(), ic.data(), th.data(), reltol, exact_mode, v.data(), e.data(),
                            ne.data());
         if (rc) return rc;
         for (size_t k = 0; k < redo.size(); ++k) {
@@ -788,6 +833,13 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         ctx->stats = keep;
     }
     ctx->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+extern "C" int tp_surrogate_exact_regions(tp_ctx *ctx, long long nregions)
+{
+    if (!ctx || nregions < 0) return tp_set_error(TP_EINVAL, "tp_surrogate_exact_regions: nregions must be >= 0");
+    ctx->sur_exact_regions = nregions;
     return 0;
 }
 
