@@ -171,6 +171,29 @@ def cpu_baseline(cos, thetas, icosmo, seconds_target, reltol):
             % (done, cores, cpu_model(), dt)}
 
 
+def write_inputs(W, workload, out_dir):
+    """The C1 or C3 workload as the text inputs of benchMap3.x: cosmologies.txt (h sigma8 Omega_b n_s w Omega_m
+    z_max), triples.txt (theta1 theta2 theta3 [arcmin] icosmo) and nz.txt (the n(z) file rows)."""
+    os.makedirs(out_dir, exist_ok=True)
+    if workload == "c1":
+        cos, tri = [W.fiducial()], W.triples(W.map3_c1_radii())
+        rows = [(t, 0) for t in tri]
+    else:
+        cos, tri = W.lhs_c3(512), W.triples(W.map3_c3_radii())
+        rows = [(t, i) for i in range(len(cos)) for t in tri]
+    with open(os.path.join(out_dir, "cosmologies.txt"), "w") as f:
+        for c in cos:
+            f.write("%.17g %.17g %.17g %.17g %.17g %.17g %.17g\n" % (c.h, c.sigma8, c.omb, c.ns, c.w, c.om, c.zmax))
+    with open(os.path.join(out_dir, "triples.txt"), "w") as f:
+        for t, i in rows:
+            f.write("%.17g %.17g %.17g %d\n" % (t[0], t[1], t[2], i))
+    z, n = W.nz_file_rows()
+    with open(os.path.join(out_dir, "nz.txt"), "w") as f:
+        for a, b in zip(z, n):
+            f.write("%.17g %.17g\n" % (a, b))
+    print("bench: wrote %d cosmologies and %d triples to %s" % (len(cos), len(rows), out_dir), file=sys.stderr)
+
+
 def accuracy_study(tabs, ctx, thetas, icosmo, paths, reltol):
     """Each batched path at reltol against batched FP64 at reltol/10 (the converged value)."""
     import threepoint_amd as tp
@@ -511,6 +534,9 @@ def main():
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first copy this many MiB device-to-device (a kernel of known traffic for "
                          "calibrating HBM byte counters)")
+    ap.add_argument("--write-inputs", metavar="DIR",
+                    help="write the workload's cosmologies, triples and n(z) as text files for benchMap3.x "
+                         "(the same workload driven from C++, for profilers), then exit")
     ap.add_argument("--accuracy-study", action="store_true",
                     help="compare every path at the tolerance against FP64 at a 10x tighter tolerance, then exit")
     args = ap.parse_args()
@@ -522,6 +548,9 @@ def main():
     import threepoint_amd as tp
     from threepoint_amd import workloads as W
 
+    if args.write_inputs:
+        write_inputs(W, args.workload, args.write_inputs)
+        return
     ctx = tp.default_context()
     if args.copy_calibration_mib > 0:
         import ctypes

@@ -254,7 +254,9 @@ int tp_gamma_eval(tp_ctx *ctx, const tp_cosmo *c, int icosmo, int which, double 
  * convention, hcubature over (z, phi, psi) with relative tolerance reltol. */
 int tp_gamma_batch(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t ntri, const double *x123,
                    double reltol, double *out_re, double *out_im, long long *neval);
-/* tp_gamma_batch with the integrand's precision: TP_MODE_FP64, or TP_MODE_FP32 (Ogata terms whose
+/* tp_gamma_batch with the integrand's precision: TP_MODE_FP64, TP_MODE_ANALYTIC (the reference's analytic
+ * test bispectrum, integrand_bkappa's test_analytical branch, classed_version/bispectrum.cpp:477-499: zero
+ * above z = 1), or TP_MODE_FP32 (Ogata terms whose
  * wavenumbers lie in the FP32 window in FP32, the others and any that leave the float range in FP64;
  * FP64 accumulation, rule and heap; the FP32 form needs prec_k <= 10753, TP_EINVAL otherwise).  No
  * counterpart in the reference (its integrand is FP64). */
@@ -421,10 +423,16 @@ int tp_surrogate_load(tp_ctx *ctx, const char *path);
  * falls back to exact B_kappa. */
 int tp_surrogate_config(tp_ctx *ctx, double check_frac, double tol, uint64_t seed);
 /* Surrogate mode evaluates an integral's regions exactly (the FP32 form) until the integral holds
- * nregions regions (default 256; 0: the surrogate from the first round).  An integral that the
+ * nregions regions (default 1024; 0: the surrogate from the first round).  An integral that the
  * reference algorithm settles within that many regions -- where its early stopping decides the value --
  * then takes the exact paths' decisions.  No counterpart in the reference. */
 int tp_surrogate_exact_regions(tp_ctx *ctx, long long nregions);
+/* Surrogate mode evaluates a triple exactly (the FP32 form) throughout when its largest radius is at
+ * least ratio times its smallest (default 16; 0: no triple).  Such a triple puts its weight on a ridge
+ * near phi = pi of relative width ~theta_min / theta_max, which the reference algorithm finds only after
+ * many splits; its decision sequence there, and so its value at the reference's tolerance, moves by
+ * more than 1e-3 under 1e-4-level changes of the integrand.  No counterpart in the reference. */
+int tp_surrogate_exact_ratio(tp_ctx *ctx, double ratio);
 /* y[i] = the network's output at in[8*i..] = (l1, l2, l3, Omega_m, sigma8, h, w, n_s): the learned
  * ln B_kappa - ln B_8, where B_8 is the 8-node FP32 Limber baseline of the same cosmology. */
 int tp_surrogate_predict(tp_ctx *ctx, size_t n, const double *in8, double *y);

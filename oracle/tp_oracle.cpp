@@ -1855,9 +1855,10 @@ int orc_gamma_integrand_batch(void *h, int which, double x1, double x2, double x
     return gamma_cb(3, npts, x, &g, 2, fval);
 }
 
-int orc_gamma(void *h, int which, double x1, double x2, double x3, double reltol, int nthreads, double *re,
-              double *im, long long *neval)
-{ /* CV/gamma.cpp:27-81 (gamma0..3), 361-377 / 545-563 (cubature + Ogata), centroid rotation 624-643 */
+int orc_gamma_center(void *h, int which, double x1, double x2, double x3, double reltol, int nthreads, int center,
+                     double *re, double *im, long long *neval)
+{ /* CV/gamma.cpp:27-81 (gamma0..3), 361-377 / 545-563 (cubature + Ogata); center 0 = centroid (rotation
+     624-643), 1 = orthocenter (no rotation, gamma.cpp:12-13) */
     Calc *c = (Calc *)h;
     bool conj = which != 0;
     if (which == 2) { double t = x1; x1 = x2; x2 = x3; x3 = t; }           /* gamma1(x2, x3, x1) */
@@ -1870,15 +1871,23 @@ int orc_gamma(void *h, int which, double x1, double x2, double x3, double reltol
     r *= 27. / 8. * std::pow(c->cosmo.om, 3) * std::pow(100. / 299792., 5);
     r /= 3.;
     r /= (2 * std::pow(2 * M_PI, 3));
-    double a1 = Calc::one_rotation_angle_otc(x2, x3, x1);
-    double a2 = Calc::one_rotation_angle_otc(x3, x1, x2);
-    double a3 = Calc::one_rotation_angle_otc(x1, x2, x3);
-    if (conj) a1 *= -1.;
-    r = r * std::exp(std::complex<double>(0, -2.) * (a1 + a2 + a3));
+    if (center == 0) {
+        double a1 = Calc::one_rotation_angle_otc(x2, x3, x1);
+        double a2 = Calc::one_rotation_angle_otc(x3, x1, x2);
+        double a3 = Calc::one_rotation_angle_otc(x1, x2, x3);
+        if (conj) a1 *= -1.;
+        r = r * std::exp(std::complex<double>(0, -2.) * (a1 + a2 + a3));
+    }
     *re = r.real();
     *im = r.imag();
     if (neval) *neval = g_last_neval;
     return rc;
+}
+
+int orc_gamma(void *h, int which, double x1, double x2, double x3, double reltol, int nthreads, double *re,
+              double *im, long long *neval)
+{
+    return orc_gamma_center(h, which, x1, x2, x3, reltol, nthreads, 0, re, im, neval);
 }
 
 double orc_pk_nonlinear(void *h, double k, double z) { return ((Calc *)h)->P_k_nonlinear(k, z); }

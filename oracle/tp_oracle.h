@@ -80,6 +80,8 @@ int orc_gamma_integrand_batch(void *h, int which, double x1, double x2, double x
 int orc_gamma(void *h, int which, double x1, double x2, double x3, double reltol,
               int nthreads, double *re, double *im, long long *neval);
 void orc_ggg_analytic(double r1, double r2, double r3, double *re, double *im);
+int orc_gamma_center(void *h, int which, double x1, double x2, double x3, double reltol, int nthreads, int center,
+                     double *re, double *im, long long *neval);
 
 /* convergence power spectrum and <Map^2> (bispectrum.cpp:288-417, 543-577, 1022-1059;
    the cuda_version's Map2 and T17 correction, apertureStatistics.cu:29-136, bispectrum.cu:951-982) */

@@ -51,6 +51,8 @@ _SIGS = {
     "orc_gamma": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_double] * 4 + [ctypes.c_int, _dp, _dp,
                                                                                              _llp]),
     "orc_ggg_analytic": (None, [ctypes.c_double] * 3 + [_dp, _dp]),
+    "orc_gamma_center": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_double] * 4 +
+                         [ctypes.c_int, ctypes.c_int, _dp, _dp, _llp]),
     "orc_pk_nonlinear": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
     "orc_linear_pk_at_z": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double]),
     "orc_om_m_of_z": (ctypes.c_double, [ctypes.c_void_p, ctypes.c_double]),
@@ -169,11 +171,19 @@ class Oracle:
                                         nthreads)
         return out[:, 0] + 1j * out[:, 1]
 
-    def gamma(self, which, x1, x2, x3, reltol=1e-3, nthreads=0):
+    def gamma(self, which, x1, x2, x3, reltol=1e-3, nthreads=0, center=0):
+        """Gamma_which of the triangle; center 0 = centroid (the reference's rotation), 1 = orthocenter."""
         re, im, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()
-        lib().orc_gamma(self.h, which, x1, x2, x3, reltol, nthreads, ctypes.byref(re), ctypes.byref(im),
-                        ctypes.byref(n))
+        lib().orc_gamma_center(self.h, which, x1, x2, x3, reltol, nthreads, center, ctypes.byref(re),
+                               ctypes.byref(im), ctypes.byref(n))
         return re.value + 1j * im.value, n.value
+
+    @staticmethod
+    def ggg_analytic(r1, r2, r3):
+        """The closed-form Gamma_0 of the analytic test bispectrum (CV/gamma.cpp:83-94, 567-586)."""
+        re, im = ctypes.c_double(), ctypes.c_double()
+        lib().orc_ggg_analytic(r1, r2, r3, ctypes.byref(re), ctypes.byref(im))
+        return re.value + 1j * im.value
 
     def convergence_power_spectrum(self, ell, reltol=1e-6, flags=0):
         v, e, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_longlong()

@@ -101,3 +101,58 @@ def test_gamma_orthocenter_convention(fid_tables, fid_oracle, gpu_ctx):
         fid_tables.gamma_batch(x[None, :], reltol=1e-2, center=7)
     with pytest.raises(ValueError):
         tp.GammaCalculator(None, 0.1, 2.0, "incenter")
+
+
+def test_gamma0_analytic_closed_form_on_gpu(gpu_ctx):
+    """The device's Gamma_0 integrand with the reference's analytic test bispectrum (TP_MODE_ANALYTIC, z_max = 1,
+    the grid of classed_version/testGamma_analytical_bispectrum.cpp) in the orthocenter convention: equal to
+    the closed form ggg (gamma.cpp:83-94, 567-586) at the agreement the oracle reaches (tests/test_gamma_oracle.py),
+    and to the oracle's value with the same cubature decisions."""
+    from _oracle import Oracle
+    from test_gamma_oracle import grid_triangle
+    c = W.fiducial()
+    c.zmax = 1.0
+    nz = np.full(10, 1.0)
+    tabs = tp.CosmologyTables([c], nz, 10, 1.0, gpu_ctx)
+    tp.gamma_setup(0.005, 5.0, gpu_ctx)
+    o = Oracle(c, nz, 10, 1.0)
+    o.set_analytic(True)
+    o.gamma_init(0.005, 5.0)
+    for ijk, tol in (((8, 10, 2), 1e-6), ((7, 3, 10), 5e-4)):
+        x = np.array(grid_triangle(*ijk))
+        g, n = tabs.gamma_batch(x[None, :], reltol=1e-5, mode=tp.MODE_ANALYTIC, center=tp.CENTER_ORTHOCENTER)
+        ggg = Oracle.ggg_analytic(*x)
+        go, no = o.gamma(0, *x, reltol=1e-5, center=1)
+        print("bin %s: GPU %s, closed form %s (rel %.2e), oracle rel %.2e, points %d / %d"
+              % (ijk, g[0, 0], ggg, abs(g[0, 0] - ggg) / abs(ggg), abs(g[0, 0] - go) / abs(go), n[0, 0], no))
+        assert abs(g[0, 0] - ggg) < tol * abs(ggg)
+        assert abs(g[0, 0] - go) < 1e-8 * abs(go) and n[0, 0] == no
+
+
+def test_gamma_production_grid_matches_oracle_fixture(fid_tables, gpu_ctx):
+    """Gamma_0..3 of TreeCorr-grid triangles at the production Ogata grid (step 1e-3, 4999 zeros of J6 and J2)
+    against the CPU FP64 oracle (tests/golden/gamma_c2_fp64.json, written offline: the oracle needs ~10 CPU
+    minutes per triangle there): FP64 within 1e-4 of each component's modulus, the FP32 form within 1e-3.
+
+    1e-4, not 1e-6: at this grid the integrand is ill-conditioned near phi = pi (ell3 -> 0), where one ulp of
+    (z, phi, psi) moves the oracle's own integrand by up to 1e-1 relative, and the adaptive cubature's
+    decisions follow those digits.  Moving x1 of triangle 555 by one ulp moves the oracle's own Gamma_0 by
+    3.9e-5 and its point count from 62007 to 63855; the integrands themselves agree to 1e-8
+    (test_gamma_integrand_matches_oracle) and whole Gamma_i at a coarser grid to 1e-6
+    (test_gamma_batch_small_matches_oracle)."""
+    import json
+    import os
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "gamma_c2_fp64.json")))
+    assert fx["prec_h"] == 1e-3 and fx["prec_k_arg"] == 5.0 and fx["rows"]
+    tp.gamma_setup(1e-3, 5.0, gpu_ctx)
+    for row in fx["rows"]:
+        x = np.array(row["x123_rad"])
+        want = np.array([complex(a, b) for a, b in row["gamma_re_im"]])
+        g64, n64 = fid_tables.gamma_batch(x[None, :], reltol=row["reltol"])
+        g32, _ = fid_tables.gamma_batch(x[None, :], reltol=row["reltol"], mode=tp.MODE_FP32)
+        e64 = np.abs(g64[0] - want) / np.abs(want)
+        e32 = np.abs(g32[0] - want) / np.abs(want)
+        print("triangle %d (rel tol %g): fp64 max rel %.2e, fp32 max rel %.2e, points %s vs %s"
+              % (row["triangle"], row["reltol"], e64.max(), e32.max(), list(n64[0]), row["neval"]))
+        assert e64.max() < 1e-4
+        assert e32.max() < 1e-3

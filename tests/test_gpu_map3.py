@@ -154,6 +154,18 @@ def test_map3_analytic_matches_oracle(fid_tables, fid_oracle):
         fid_oracle.set_analytic(False)
 
 
+def test_map3_analytic_matches_closed_form(fid_tables):
+    """The device's <Map^3> with the analytic test bispectrum against its closed form (exact Gaussian moments,
+    tests/test_map3_closed_form.py) at the cubature's tolerance."""
+    from test_map3_closed_form import TRIPLES, map3_analytic_closed_form
+    th = W.to_rad(np.array(TRIPLES))
+    v, _, _ = fid_tables.map3_batch(th, reltol=1e-6, mode=tp.MODE_ANALYTIC)
+    for i in range(len(TRIPLES)):
+        c = map3_analytic_closed_form(th[i])
+        print("theta %s: GPU %.12e closed form %.12e (rel %.2e)" % (TRIPLES[i], v[i], c, abs(v[i] - c) / abs(c)))
+        assert abs(v[i] - c) <= 1e-6 * abs(c)
+
+
 def test_nonfinite_theta_rejected(fid_tables):
     with pytest.raises(tp.TpError):
         fid_tables.map3_batch(np.array([[0.0, 1e-3, 1e-3]]))

@@ -151,6 +151,7 @@ _SIGS = {
     "tp_surrogate_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     "tp_surrogate_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]),
     "tp_surrogate_exact_regions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
+    "tp_surrogate_exact_ratio": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     "tp_surrogate_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _dp, _dp]),
     "tp_surrogate_bkappa": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
                                            _dp]),
@@ -234,9 +235,13 @@ class Context:
         _check(lib().tp_set_batch_limits(self.h, int(max_rounds), int(max_regions), int(round_points)),
                "tp_set_batch_limits")
 
-    def set_surrogate_exact_regions(self, n=256):
+    def set_surrogate_exact_regions(self, n=1024):
         """Surrogate mode: integrals are evaluated exactly (FP32 form) until they hold n regions."""
         _check(lib().tp_surrogate_exact_regions(self.h, int(n)), "tp_surrogate_exact_regions")
+
+    def set_surrogate_exact_ratio(self, ratio=16.0):
+        """Surrogate mode: triples with max(theta) >= ratio * min(theta) are evaluated exactly (0: none)."""
+        _check(lib().tp_surrogate_exact_ratio(self.h, float(ratio)), "tp_surrogate_exact_ratio")
 
     def set_round_log(self, on=True):
         _check(lib().tp_set_round_log(self.h, 1 if on else 0), "tp_set_round_log")

@@ -9,6 +9,7 @@ artefacts travel with the source tree:
   threepoint_amd/_lib/calculatePowerspectrum.x
   threepoint_amd/_lib/calculateSecondOrderApertureStatistics.x
   threepoint_amd/_lib/calculateApertureStatisticsCovariance.x
+  threepoint_amd/_lib/benchMap3.x                the batched <Map^3> engine from C++ (for profilers)
   threepoint_amd/_lib/threepoint_cosmosis.so    the cosmosis module
 """
 import os
@@ -24,7 +25,7 @@ LIB = os.path.join(LIBDIR, "libthreepoint_amd.so")
 HIP_SOURCES = ["tp_tables.hip", "tp_map3.hip", "tp_runtime.hip", "tp_gamma.hip", "tp_mlp.hip", "tp_power.hip"]
 CPP_SOURCES = ["tp_cubature.cpp", "tp_census.cpp", "tp_classes.cpp", "tp_io.cpp"]
 APPS = ["calculateApertureStatistics", "calculateGamma", "calculatePowerspectrum",
-        "calculateSecondOrderApertureStatistics", "calculateApertureStatisticsCovariance"]
+        "calculateSecondOrderApertureStatistics", "calculateApertureStatisticsCovariance", "benchMap3"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 COMMON = [

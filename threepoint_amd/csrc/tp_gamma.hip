@@ -170,6 +170,27 @@ __device__ __forceinline__ double A_f(double psi, double x1, double x2, double p
     return sqrt(a * a + b * b + sin(2 * psi) * x1 * x2 * cos(phi + vp));
 }
 
+// integrand_bkappa of the reference's analytic test bispectrum (bispectrum.cpp:477-499, 989-996): zero above
+// z = 1; phi = pi - acos(x) of the triangle, 0 where acos gives NaN (as the reference sets it)
+__device__ double integrand_bkappa_analytic(double z, double l1, double l2, double l3, double limber_pref)
+{
+    if (z > 1) return 0.0;
+    double phi = tpd::kPi - acos((l1 * l1 + l2 * l2 - l3 * l3) / (2 * l1 * l2));
+    if (isnan(phi)) phi = 0;
+    const double wts[9] = {0, 1.0e+8, 1.0e+6, 1.0e+4, 1.0e+2, 1.0e+0, 5.0e-3, 1.0e-5, 1.0e-6};
+    const double avs[9] = {3.0e+3, 1.0e+4, 3.0e+4, 1.0e+5, 3.0e+5, 1.0e+6, 3.0e+6, 1.0e+7, 3.0e+7};
+    const double cphi = cos(phi);
+    double temp = 0;
+    for (int i = 0; i < 9; i++) {
+        const double a = avs[i];
+        const double bis_pref = -pow(tpd::kPi, 2) / (8 * 72 * pow(a, 8));
+        const double pref = pow(l1, 2) * pow(l2, 2) * (pow(l1, 2) + pow(l2, 2) + 2 * l1 * l2 * cphi);
+        const double expon = exp(-(pow(l1, 2) + pow(l2, 2) + l1 * l2 * cphi) / (6 * a));
+        temp += wts[i] * (bis_pref * pref * expon);
+    }
+    return temp / limber_pref * 3.;
+}
+
 __device__ __forceinline__ double ogata_term(const TpNode64 &N, const TpEH64 &E, double psik, double prodk,
                                              double invA, double cpsi, double spsi, double cphi)
 {
@@ -187,7 +208,7 @@ __device__ __forceinline__ double ogata_term(const TpNode64 &N, const TpEH64 &E,
 // a narrower one leaves speed, at the same 1e-5 agreement with FP64) is evaluated in FP32; the rest (small z, where f_K -> 0, the first Ogata nodes, where psi_k -> 0, the far nodes)
 // and any FP32 term that leaves the float range stay FP64.  Accumulation is FP64.
 
-template <bool F32>
+template <bool F32, bool ANA = false>
 __global__ void __launch_bounds__(kGammaBlock) __attribute__((amdgpu_waves_per_eu(F32 ? 3 : 1))) k_gamma_points(const TpCosmoConst *__restrict__ cc,
                                                               const double *__restrict__ tab, const TpEH64 *__restrict__ eh,
                                                               const TpEH32 *__restrict__ eh32,
@@ -233,7 +254,19 @@ __global__ void __launch_bounds__(kGammaBlock) __attribute__((amdgpu_waves_per_e
     const double *prodk = psik + prec_k;
     const double *lnpsik = ogata + (D.which == 0 ? 4 : 5) * (size_t)prec_k;
     double s[3] = {0, 0, 0};
-    if (sok) {
+    if (ANA) {  // the analytic test bispectrum (no tables): every Ogata term of integrated_bdelta_times_rcubed
+        const double cpsi = cos(psi), spsi = sin(psi), cphi = cos(phi);
+        for (int k = 1 + threadIdx.x; k < prec_k; k += kGammaBlock) {
+            const double pk = psik[k], wk = prodk[k];
+#pragma unroll 1
+            for (int j = 0; j < 3; ++j) {
+                const double l1 = pk / A[j] * cpsi, l2 = pk / A[j] * spsi;  // gamma.cpp:300-307
+                double l3s = l1 * l1 + l2 * l2 + 2 * l1 * l2 * cphi;
+                const double l3 = l3s > 0 ? sqrt(l3s) : 0.0;
+                s[j] += integrand_bkappa_analytic(z, l1, l2, l3, C.limber_pref) * wk;
+            }
+        }
+    } else if (sok) {
         const TpEH64 &E = eh[D.icosmo];
         const double cpsi = cos(psi), spsi = sin(psi), cphi = cos(phi);
         double invA[3] = {1.0 / A[0], 1.0 / A[1], 1.0 / A[2]};
@@ -619,6 +652,11 @@ static int gamma_launch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpGamm
                            (const TpGammaDesc *)ctx->d_desc.p, (const int *)ctx->d_misc.p, npts,
                            (const double *)ctx->d_in.p, (double *)ctx->d_out.p, (int *)ctx->d_flag.p,
                            ctx->gamma_q32[0], ctx->gamma_q32[1]);
+    else if (mode == TP_MODE_ANALYTIC)
+        hipLaunchKernelGGL((k_gamma_points<false, true>), dim3((unsigned)npts), dim3(kGammaBlock), 0, ctx->stream,
+                           c->d_const, c->d_tab, c->d_eh64, c->d_eh32, c->n_z, ctx->d_ogata, (int)ctx->gamma_prec_k,
+                           (const TpGammaDesc *)ctx->d_desc.p, (const int *)ctx->d_misc.p, npts,
+                           (const double *)ctx->d_in.p, (double *)ctx->d_out.p, (int *)ctx->d_flag.p, 0.0, 0.0);
     else
         hipLaunchKernelGGL(k_gamma_points<false>, dim3((unsigned)npts), dim3(kGammaBlock), 0, ctx->stream, c->d_const,
                            c->d_tab, c->d_eh64, c->d_eh32, c->n_z, ctx->d_ogata, (int)ctx->gamma_prec_k,
@@ -685,8 +723,8 @@ extern "C" int tp_gamma_batch_center(tp_ctx *ctx, const tp_cosmo *c, int icosmo,
     if (!ctx || !c || !x123 || !out_re || !out_im) return tp_set_error(TP_EINVAL, "tp_gamma_batch: NULL");
     if (center != TP_CENTER_CENTROID && center != TP_CENTER_ORTHOCENTER)
         return tp_set_error(TP_EINVAL, "tp_gamma_batch: center must be TP_CENTER_CENTROID or TP_CENTER_ORTHOCENTER");
-    if (mode != TP_MODE_FP64 && mode != TP_MODE_FP32)
-        return tp_set_error(TP_EINVAL, "tp_gamma_batch: mode must be TP_MODE_FP64 or TP_MODE_FP32");
+    if (mode != TP_MODE_FP64 && mode != TP_MODE_FP32 && mode != TP_MODE_ANALYTIC)
+        return tp_set_error(TP_EINVAL, "tp_gamma_batch: mode must be TP_MODE_FP64, TP_MODE_FP32 or TP_MODE_ANALYTIC");
     if (!ctx->d_ogata) return tp_set_error(TP_EINVAL, "tp_gamma_batch: call tp_gamma_setup first");
     if (icosmo < 0 || icosmo >= c->ncosmo) return tp_set_error(TP_EINVAL, "tp_gamma_batch: bad icosmo");
     TP_HIP(hipSetDevice(ctx->device));

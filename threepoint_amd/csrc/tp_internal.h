@@ -190,7 +190,10 @@ struct tp_ctx {
     uint64_t sur_seed = 0x5eed;
     // surrogate mode: an integral's regions are evaluated exactly (FP32 form) until the integral holds this
     // many regions (tp_surrogate_exact_regions)
-    long long sur_exact_regions = 256;
+    long long sur_exact_regions = 1024;
+    // surrogate mode: a triple whose largest radius is at least this many times its smallest is evaluated
+    // exactly throughout (tp_surrogate_exact_ratio; 0 = none)
+    double sur_exact_ratio = 16.0;
     // launch shape of the region-point kernel: waves per point for [FP64, FP32][size bucket]
     int region_lpp[2][4] = {{1, 1, 1, 1}, {1, 1, 1, 1}};
     bool region_tuned[2] = {false, false};

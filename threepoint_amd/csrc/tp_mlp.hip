@@ -1272,43 +1272,44 @@ __global__ void __launch_bounds__(256) k_sur_points(const TpRegion *__restrict__
 }
 
 // exact B_kappa of the listed points: 16 lanes per point, 6 of the 96 z-nodes each, so a round's
-// few thousand checked points still fill the device
+// few thousand checked points still fill the device.  The list's length is known on the device only: the
+// grid is sized for the expected checks (kSurExactBlocks blocks of 16 points) and strides over the list.
 constexpr int XL = 16;
+constexpr int kSurExactBlocks = 1024;
 __global__ void __launch_bounds__(256) k_sur_exact(const int *list, const int *nlist, const TpRegion *__restrict__ reg,
                                                    size_t r0, const TpIntegralDesc *__restrict__ desc,
                                                    const TpNode64 *__restrict__ nodes, const TpEH64 *__restrict__ eh,
                                                    double *Bex, int *err)
 {
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = gt / XL, sub = gt % XL;
     const int n = *nlist;
-    const bool active = i < n;
-    double part = 0.0;
-    int t = 0;
-    if (active) {
-        t = list[i];
-        size_t r = r0 + t / TP_NPTS_GM3;
-        int p = t % TP_NPTS_GM3;
+    const int sub = threadIdx.x % XL;
+    // the XL lanes of a group share i, so a group runs the loop (and the shuffle below) with all its lanes
+    const int stride = gridDim.x * (blockDim.x / XL);
+    for (int i = (blockIdx.x * blockDim.x + threadIdx.x) / XL; i < n; i += stride) {
+        const int t = list[i];
+        const size_t r = r0 + t / TP_NPTS_GM3;
+        const int p = t % TP_NPTS_GM3;
         const TpRegion R = reg[r];
         const TpIntegralDesc D = desc[R.integral];
         double l1, l2, phi;
         gm_point3(R, p, &l1, &l2, &phi);
-        double l3 = tp_ell3(l1, l2, phi);
+        const double l3 = tp_ell3(l1, l2, phi);
         TpPointPre P;
         tp_point_pre<double>(l1, l2, l3, &P);
         const TpNode64 *nd = nodes + (size_t)D.icosmo * TP_NNODES;
         const TpEH64 E = eh[D.icosmo];
+        double part = 0.0;
         for (int k = sub; k < TP_NNODES; k += XL) {
             const TpNode64 &N = nd[k];
             double q[3] = {l1 * N.ks, l2 * N.ks, l3 * N.ks};
             double lq[3] = {P.ll[0] + N.lnks, P.ll[1] + N.lnks, P.ll[2] + N.lnks};
             part += N.wz * tp_bispec_node<double>(N, E, q, lq, P.f2t, P.lr1, P.r2);
         }
-    }
-    for (int o = XL / 2; o >= 1; o >>= 1) part += __shfl_xor(part, o, XL);
-    if (active && sub == 0) {
-        if (!isfinite(part)) atomicOr(err, 2);
-        Bex[t] = part;
+        for (int o = XL / 2; o >= 1; o >>= 1) part += __shfl_xor(part, o, XL);
+        if (sub == 0) {
+            if (!isfinite(part)) atomicOr(err, 2);
+            Bex[t] = part;
+        }
     }
 }
 
@@ -1423,7 +1424,7 @@ int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, co
         TP_HIP(hipEventRecord(B.ev[2 * B.nev + 1], st));
         B.ev_points.push_back((long long)npp);
         B.nev += 1;
-        hipLaunchKernelGGL(k_sur_exact, dim3(grid(np * XL, 256)), dim3(256), 0, st, (const int *)B.list.p, dn, d_reg, r0,
+        hipLaunchKernelGGL(k_sur_exact, dim3(kSurExactBlocks), dim3(256), 0, st, (const int *)B.list.p, dn, d_reg, r0,
                            d_desc, c->d_node64, c->d_eh64, (double *)B.Bex.p, d_flag);
         hipLaunchKernelGGL(k_sur_combine, dim3(grid(np, 256)), dim3(256), 0, st, np, (const int *)B.flag.p,
                            (const double *)B.F.p, (const double *)B.logb.p, (const double *)B.Bb.p,

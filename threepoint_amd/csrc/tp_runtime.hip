@@ -459,6 +459,7 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     auto t0 = std::chrono::steady_clock::now();
     ctx->stats = tp_batch_stats{};
     std::vector<TpIntegralDesc> desc(ntriples);
+    std::vector<char> wide(ntriples, 0);  // surrogate mode: triples evaluated exactly throughout
     std::vector<tpc::AdaptiveIntegral31> I;
     I.swap(ctx->map3_keep);  // the last call's integrators (their pools keep capacity), if kept
     if (I.size() < ntriples) I.resize(ntriples);
@@ -468,6 +469,8 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         double tmin = std::min(th[0], std::min(th[1], th[2]));
         if (!(tmin > 0)) return tp_set_error(TP_EINVAL, "tp_map3_batch: theta must be positive");
         desc[t] = TpIntegralDesc{{th[0], th[1], th[2]}, icosmo[t], 0};
+        const double tmax = std::max(th[0], std::max(th[1], th[2]));
+        wide[t] = ctx->sur_exact_ratio > 0 && tmax >= ctx->sur_exact_ratio * tmin;
         double lMax = 10. / tmin;  // classed_version/apertureStatistics.cpp:325-326
         double vmin[3] = {1., 1., 0.}, vmax[3] = {lMax, lMax, 2 * M_PI};
         // surrogate mode: a per-triple point budget; a triple that exhausts it is redone exactly
@@ -528,9 +531,9 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
             return tp_set_error(TP_ECAPACITY, "tp_map3_batch: round limit reached (integrals not converging)");
         const size_t na = Gr.active.size();
         size_t na_exact = 0;
-        if (mode == TP_MODE_SURROGATE) {  // integrals still small enough for exact evaluation come first
+        if (mode == TP_MODE_SURROGATE) {  // integrals evaluated exactly (wide, or still small) come first
             auto mid = std::stable_partition(Gr.active.begin(), Gr.active.end(), [&](size_t t) {
-                return (long long)I[t].num_regions() < ctx->sur_exact_regions;
+                return wide[t] || (long long)I[t].num_regions() < ctx->sur_exact_regions;
             });
             na_exact = (size_t)(mid - Gr.active.begin());
         }
@@ -840,6 +843,13 @@ extern "C" int tp_surrogate_exact_regions(tp_ctx *ctx, long long nregions)
 {
     if (!ctx || nregions < 0) return tp_set_error(TP_EINVAL, "tp_surrogate_exact_regions: nregions must be >= 0");
     ctx->sur_exact_regions = nregions;
+    return 0;
+}
+
+extern "C" int tp_surrogate_exact_ratio(tp_ctx *ctx, double ratio)
+{
+    if (!ctx || !(ratio >= 0)) return tp_set_error(TP_EINVAL, "tp_surrogate_exact_ratio: ratio must be >= 0");
+    ctx->sur_exact_ratio = ratio;
     return 0;
 }
 
