@@ -422,6 +422,12 @@ int tp_surrogate_load(tp_ctx *ctx, const char *path);
  * are re-evaluated exactly; if the surrogate's error on them exceeds tol the batch
  * falls back to exact B_kappa. */
 int tp_surrogate_config(tp_ctx *ctx, double check_frac, double tol, uint64_t seed);
+/* The surrogate's inference kernel: TP_SUR_INFER_FP16 (default) rounds the hidden activations once to
+ * FP16 (one FP16 MFMA per weight fragment); TP_SUR_INFER_HILO_BF16 carries them as BF16 hi + lo pairs
+ * (~16 significant bits, two BF16 MFMAs per fragment).  The weights are the trained BF16 values in
+ * both.  No counterpart in the reference. */
+enum { TP_SUR_INFER_HILO_BF16 = 0, TP_SUR_INFER_FP16 = 1 };
+int tp_surrogate_inference(tp_ctx *ctx, int kind);
 /* Surrogate mode evaluates an integral's regions exactly (the FP32 form) until the integral holds
  * nregions regions (default 1024; 0: the surrogate from the first round).  An integral that the
  * reference algorithm settles within that many regions -- where its early stopping decides the value --

@@ -88,6 +88,21 @@ def forward_hilo(P, X):
     return a @ P["W5"] + P["b5"][0]
 
 
+def fp16(x):
+    """round to FP16 (nearest even) and back"""
+    return np.asarray(x, dtype=np.float32).astype(np.float16).astype(np.float64)
+
+
+def forward_fp16(P, X):
+    """the FP16 inference kernel's arithmetic: FP16 activations (one rounding), the BF16 weights of layers 2-4
+    (exact in FP16 within its range)"""
+    a = fp16(silu(X @ P["W1"].T + P["b1"]))
+    for k in (2, 3, 4):
+        z = a @ fp16(bf16(P["W%d" % k])).T + P["b%d" % k]
+        a = fp16(silu(z)) if k < 4 else silu(z)
+    return a @ P["W5"] + P["b5"][0]
+
+
 MODEL_MAGIC = b"TPMLP002"
 
 

@@ -55,8 +55,10 @@ def test_training_learns(gpu_ctx):
     assert r["val_rmse_lnB"] < 0.2
 
 
-def test_fused_inference_matches_cpu_oracle(gpu_ctx, tmp_path):
-    """tp_surrogate_predict (fused hi/lo-BF16 MFMA kernel) against the numpy forward pass."""
+@pytest.mark.parametrize("kind", [tp.SUR_INFER_HILO_BF16, tp.SUR_INFER_FP16])
+def test_fused_inference_matches_cpu_oracle(gpu_ctx, tmp_path, kind):
+    """tp_surrogate_predict (fused MFMA kernels: hi/lo-BF16 or FP16 activations) against the numpy forward
+    pass with the same roundings."""
     rng = np.random.default_rng(3)
     d = {}
     for name, shp in MO.SHAPES:
@@ -67,18 +69,49 @@ def test_fused_inference_matches_cpu_oracle(gpu_ctx, tmp_path):
     fn = str(tmp_path / "m.bin")
     MO.write_model(fn, (2.0, 11.0), lo, hi, -30.0, 4.0, p)
     tp._check(tp.lib().tp_surrogate_load(gpu_ctx.h, fn.encode()), "load")
+    gpu_ctx.set_surrogate_inference(kind)
     n = 1000
     l1, l2 = np.exp(rng.uniform(2, 11, n)), np.exp(rng.uniform(2, 11, n))
     l3 = np.sqrt(l1 ** 2 + l2 ** 2 + 2 * l1 * l2 * np.cos(rng.uniform(0, np.pi, n)))
     cp = np.stack([rng.uniform(a, b, n) for a, b in zip(lo, hi)], 1)
     raw = np.ascontiguousarray(np.concatenate([np.stack([l1, l2, l3], 1), cp], 1))
     y = np.zeros(n)
-    tp._check(tp.lib().tp_surrogate_predict(gpu_ctx.h, n, tp._d(raw), tp._d(y)), "predict")
+    try:
+        tp._check(tp.lib().tp_surrogate_predict(gpu_ctx.h, n, tp._d(raw), tp._d(y)), "predict")
+    finally:
+        gpu_ctx.set_surrogate_inference(tp.SUR_INFER_FP16)
     m = MO.read_model(fn)
-    ref = MO.forward_hilo(MO.unpack(m["p"]), MO.encode(m, l1, l2, l3, cp)) * m["sd"] + m["mean"]
+    fwd = MO.forward_hilo if kind == tp.SUR_INFER_HILO_BF16 else MO.forward_fp16
+    ref = fwd(MO.unpack(m["p"]), MO.encode(m, l1, l2, l3, cp)) * m["sd"] + m["mean"]
     err = np.abs(y - ref)
-    print("fused inference max abs err (ln B) %.3e" % err.max())
+    print("fused inference (kind %d) max abs err (ln B) %.3e" % (kind, err.max()))
     assert err.max() < 2e-3
+
+
+def test_fp16_and_hilo_inference_agree_on_the_trained_model(gpu_ctx):
+    """The shipped network through both inference kernels: the FP16 activations' single rounding moves ln B by
+    about 1e-4 at most against the ~16-bit hi/lo activations."""
+    _load_model(gpu_ctx)
+    rng = np.random.default_rng(11)
+    c = W.fiducial()
+    l1, l2 = np.exp(rng.uniform(2.5, 10.5, 8192)), np.exp(rng.uniform(2.5, 10.5, 8192))
+    l3 = np.sqrt(l1 ** 2 + l2 ** 2 + 2 * l1 * l2 * np.cos(rng.uniform(0, np.pi, 8192)))
+    keep = (l3 > np.exp(2.2)) & (l3 < np.exp(10.8))  # inside the training box
+    l1, l2, l3 = l1[keep], l2[keep], l3[keep]
+    n = l1.shape[0]
+    cp = np.tile([c.om, c.sigma8, c.h, c.w, c.ns], (n, 1))
+    raw = np.ascontiguousarray(np.concatenate([np.stack([l1, l2, l3], 1), cp], 1))
+    y = {}
+    try:
+        for kind in (tp.SUR_INFER_HILO_BF16, tp.SUR_INFER_FP16):
+            gpu_ctx.set_surrogate_inference(kind)
+            y[kind] = np.zeros(n)
+            tp._check(tp.lib().tp_surrogate_predict(gpu_ctx.h, n, tp._d(raw), tp._d(y[kind])), "predict")
+    finally:
+        gpu_ctx.set_surrogate_inference(tp.SUR_INFER_FP16)
+    d = np.abs(y[tp.SUR_INFER_FP16] - y[tp.SUR_INFER_HILO_BF16])
+    print("FP16 vs hi/lo inference: max |d ln B| %.2e, rms %.2e" % (d.max(), np.sqrt((d ** 2).mean())))
+    assert d.max() < 1e-3
 
 
 def test_corrupted_model_rejected(gpu_ctx, tmp_path):
@@ -128,9 +161,10 @@ def test_trained_network_gain_over_baseline(gpu_ctx, fid_tables):
 
 
 def test_surrogate_mode_gain_in_map3(gpu_ctx, fid_tables):
-    """<Map^3> of the 20 C0 triples: with the trained network most points are served by it and no
-    round falls back; with an untrained network of the same shape the checks reject it and the
-    rounds fall back to exact B_kappa.  Both stay within 1e-3 of the FP64 path."""
+    """<Map^3> of the 20 C0 triples: with the trained network more than half of the points are served by it
+    (each integral's first 1024 regions run exact: about a third of C0's ~3000 regions per integral) and no
+    round falls back; with an untrained network of the same shape the checks reject it and the rounds fall
+    back to exact B_kappa.  Both stay within 1e-3 of the FP64 path."""
     th = W.to_rad(W.triples(W.map3_c0_radii()))
     ref, _, _ = fid_tables.map3_batch(th, None, 1e-4, tp.MODE_FP64)
     nz = ref != 0
@@ -140,7 +174,7 @@ def test_surrogate_mode_gain_in_map3(gpu_ctx, fid_tables):
     frac = st["surrogate_points"] / st["points"]
     err = np.abs(v[nz] / ref[nz] - 1).max()
     print("trained: surrogate share %.3f, fallback rounds %d, max rel err %.2e" % (frac, st["fallback_batches"], err))
-    assert frac > 0.8 and st["fallback_batches"] == 0 and err < 1e-3
+    assert frac > 0.5 and st["fallback_batches"] == 0 and err < 1e-3
     rng = np.random.default_rng(7)
     p = (rng.standard_normal(MO.NPARAM) * 0.05).astype(np.float32)
     tp._check(tp.lib().tp_mlp_set_params(gpu_ctx.h, p.ctypes.data_as(_fp)), "set")
@@ -155,8 +189,9 @@ def test_surrogate_mode_gain_in_map3(gpu_ctx, fid_tables):
 
 @pytest.mark.parametrize("workload", ["c1", "c3_subset"])
 def test_surrogate_form_within_1e3_of_fp64(gpu_ctx, fid_nz, workload):
-    """The surrogate form (exact FP32 regions until an integral holds 256 regions, then the network with
-    1% exact checks) against batched FP64 at the same tolerance 1e-4, on the 220 triples of 10 log radii
+    """The surrogate form (triples with max/min radius >= 16 exact throughout; the others exact FP32 regions
+    until an integral holds 1024 regions, then the network with 1% exact checks) against batched FP64 at the
+    same tolerance 1e-4, on the 220 triples of 10 log radii
     0.5-32' and on the labelled Latin-hypercube subset (the first 8 cosmologies and cosmology 14, whose
     (4', 16', 16') integral the reference algorithm stops after 7 regions) x 35 triples of radii 2-32':
     every integral within 1e-3, zeros kept."""
@@ -182,4 +217,5 @@ def test_surrogate_form_within_1e3_of_fp64(gpu_ctx, fid_nz, workload):
                         st["fallback_integrals"], err.max(), np.percentile(err, 99), n.sum(), nref.sum()))
     assert np.all(v[~nz] == 0)
     assert err.max() <= 1e-3
-    assert st["surrogate_points"] > 0.5 * st["points"]
+    # the network serves what the exact rules leave: ~60% of C1's points, ~47% of the C3 subset's
+    assert st["surrogate_points"] > 0.4 * st["points"]

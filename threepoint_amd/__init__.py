@@ -15,6 +15,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "libthreepoint_amd.so")
 
 MODE_FP64, MODE_FP32, MODE_SURROGATE, MODE_ANALYTIC = 0, 1, 2, 3
+SUR_INFER_HILO_BF16, SUR_INFER_FP16 = 0, 1  # surrogate inference kernels (tp_surrogate_inference)
 CENTER_CENTROID, CENTER_ORTHOCENTER = 0, 1  # triangle centers of the Gamma_i (gamma.cpp:12-16)
 # survey geometries of the covariance (the cuda_version's type codes)
 GEOMETRY_CIRCLE, GEOMETRY_SQUARE, GEOMETRY_INFINITE, GEOMETRY_RECTANGLE = 0, 1, 2, 3
@@ -152,6 +153,7 @@ _SIGS = {
     "tp_surrogate_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_uint64]),
     "tp_surrogate_exact_regions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
     "tp_surrogate_exact_ratio": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "tp_surrogate_inference": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "tp_surrogate_predict": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t, _dp, _dp]),
     "tp_surrogate_bkappa": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t, _dp,
                                            _dp]),
@@ -238,6 +240,10 @@ class Context:
     def set_surrogate_exact_regions(self, n=1024):
         """Surrogate mode: integrals are evaluated exactly (FP32 form) until they hold n regions."""
         _check(lib().tp_surrogate_exact_regions(self.h, int(n)), "tp_surrogate_exact_regions")
+
+    def set_surrogate_inference(self, kind=1):
+        """Surrogate inference kernel: SUR_INFER_FP16 (default) or SUR_INFER_HILO_BF16."""
+        _check(lib().tp_surrogate_inference(self.h, int(kind)), "tp_surrogate_inference")
 
     def set_surrogate_exact_ratio(self, ratio=16.0):
         """Surrogate mode: triples with max(theta) >= ratio * min(theta) are evaluated exactly (0: none)."""

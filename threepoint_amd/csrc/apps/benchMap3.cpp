@@ -1,8 +1,10 @@
 // benchMap3.x -- the batched <Map^3> engine driven from C++ through the C ABI, for profilers.
 //
 //   benchMap3.x <cosmologies> <triples> <n(z) file> <fp64|fp32|surrogate> <steps> [--model FILE]
-//               [--reltol X] [--nz-bins N] [--warmup N]
+//               [--reltol X] [--nz-bins N] [--warmup N] [--inference fp16|hilo] [--copy-probe MIB]
 //
+// --copy-probe MIB first copies MIB MiB device to device with the library's plain copy kernel (k_copy16):
+// a kernel of known traffic, whose HBM counters calibrate those of the other kernels in the same run.
 // <cosmologies>: one cosmology per line, "h sigma8 Omega_b n_s w Omega_m z_max" ('#' comments);
 // <triples>: one integral per line, "theta1 theta2 theta3 [arcmin] icosmo"; the n(z) file is read as
 // the classed_version reads it (helper.cpp:31-96).  Every step builds all cosmologies' FP64 tables on the
@@ -51,7 +53,8 @@ int main(int argc, char **argv)
 {
     if (argc < 6) {
         std::fprintf(stderr, "usage: benchMap3.x <cosmologies> <triples> <n(z)> <fp64|fp32|surrogate> <steps> "
-                             "[--model FILE] [--reltol X] [--nz-bins N] [--warmup N]\n");
+                             "[--model FILE] [--reltol X] [--nz-bins N] [--warmup N] [--inference fp16|hilo] "
+                             "[--copy-probe MIB]\n");
         return 1;
     }
     const std::string mode_s = argv[4];
@@ -64,13 +67,26 @@ int main(int argc, char **argv)
     const int steps = std::atoi(argv[5]);
     std::string model;
     double reltol = 1e-4;
-    int nbins = 100, warmup = 1;
+    int nbins = 100, warmup = 1, inference = TP_SUR_INFER_FP16;
+    long long copy_mib = 0;
     for (int i = 6; i + 1 < argc; i += 2) {
         const std::string a = argv[i];
         if (a == "--model") model = argv[i + 1];
         else if (a == "--reltol") reltol = std::atof(argv[i + 1]);
         else if (a == "--nz-bins") nbins = std::atoi(argv[i + 1]);
         else if (a == "--warmup") warmup = std::atoi(argv[i + 1]);
+        else if (a == "--copy-probe") copy_mib = std::atoll(argv[i + 1]);
+        else if (a == "--inference") {
+            const std::string k = argv[i + 1];
+            if (k != "fp16" && k != "hilo") {
+                std::fprintf(stderr, "benchMap3: --inference must be fp16 or hilo\n");
+                return 1;
+            }
+            inference = k == "fp16" ? TP_SUR_INFER_FP16 : TP_SUR_INFER_HILO_BF16;
+        } else {
+            std::fprintf(stderr, "benchMap3: unknown option %s\n", a.c_str());
+            return 1;
+        }
     }
     const auto cos_rows = read_rows(argv[1], 7), tri_rows = read_rows(argv[2], 4);
     if (cos_rows.empty() || tri_rows.empty()) {
@@ -100,7 +116,16 @@ int main(int argc, char **argv)
     }
     tp_ctx *ctx = nullptr;
     check(tp_ctx_create(0, &ctx), "tp_ctx_create");
-    if (mode == TP_MODE_SURROGATE) check(tp_surrogate_load(ctx, model.c_str()), "tp_surrogate_load");
+    if (mode == TP_MODE_SURROGATE) {
+        check(tp_surrogate_load(ctx, model.c_str()), "tp_surrogate_load");
+        check(tp_surrogate_inference(ctx, inference), "tp_surrogate_inference");
+    }
+    if (copy_mib > 0) {
+        double gbps = 0;
+        check(tp_hbm_copy_probe(ctx, (size_t)copy_mib << 20, &gbps), "tp_hbm_copy_probe");
+        std::printf("copy probe: %lld MiB device to device at %.0f GB/s\n", copy_mib, gbps);
+        std::fflush(stdout);
+    }
     using clk = std::chrono::steady_clock;
     double total_ms = 0, tables_ms = 0;
     long long points = 0;

@@ -416,6 +416,8 @@ struct TpSurrogate {
     float *P = nullptr, *G = nullptr, *Am = nullptr, *Av = nullptr;
     unsigned short *Wb = nullptr;  // [3][H][H]
     uint4 *Wf = nullptr;           // [NFRAG] hidden weights, BF16, in k_mlp_reg's fragment order
+    uint4 *Wh = nullptr;           // [NFRAG] the same BF16 weights as FP16 (exact), for k_mlp_h
+    int infer = TP_SUR_INFER_FP16; // inference kernel (tp_surrogate_inference)
     // activations / gradients for a batch of capacity cap rows
     int cap = 0;
     float *X = nullptr, *T = nullptr, *Y = nullptr, *dy = nullptr;
@@ -439,6 +441,7 @@ static int sur_alloc(TpSurrogate *s)
     TP_HIP(hipMalloc(&s->Av, NPARAM * sizeof(float)));
     TP_HIP(hipMalloc(&s->Wb, 3 * H * H * sizeof(unsigned short)));
     TP_HIP(hipMalloc(&s->Wf, NFRAG * sizeof(uint4)));
+    TP_HIP(hipMalloc(&s->Wh, NFRAG * sizeof(uint4)));
     TP_HIP(hipMalloc(&s->d_loss, sizeof(double)));
     TP_HIP(hipMemset(s->G, 0, NPARAM * sizeof(float)));
     TP_HIP(hipMemset(s->Am, 0, NPARAM * sizeof(float)));
@@ -472,7 +475,7 @@ static void sur_free(TpSurrogate *s)
 {
     if (!s) return;
     auto F = [](void *p) { if (p) (void)hipFree(p); };
-    F(s->P); F(s->G); F(s->Am); F(s->Av); F(s->Wb); F(s->Wf); F(s->d_loss);
+    F(s->P); F(s->G); F(s->Am); F(s->Av); F(s->Wb); F(s->Wf); F(s->Wh); F(s->d_loss);
     F(s->X); F(s->T); F(s->Y); F(s->dy); F(s->parts);
     for (int i = 0; i < 4; ++i) { F(s->Z[i]); F(s->A[i]); F(s->dZ[i]); }
     for (auto &b : s->rb) b.release();
@@ -574,15 +577,20 @@ static int sur_backward(hipStream_t st, TpSurrogate *s, int M)
 
 namespace {
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+template <bool FP16>
 __global__ void k_weights_frag(const float *__restrict__ P, u32x4 *__restrict__ Wf);
 }
 // BF16 copies of the hidden weights: Wb for the training GEMMs and, unless only training
-// continues, Wf for the inference kernel
+// continues, Wf and Wh (the same values in FP16) for the inference kernels
 static int sur_refresh_bf16(hipStream_t st, TpSurrogate *s, bool for_inference = true)
 {
     hipLaunchKernelGGL(k_weights_bf16, dim3(grid((size_t)3 * H * H, 256)), dim3(256), 0, st, s->P, s->Wb);
-    if (for_inference)
-        hipLaunchKernelGGL(k_weights_frag, dim3(grid((size_t)NFRAG, 256)), dim3(256), 0, st, s->P, (u32x4 *)s->Wf);
+    if (for_inference) {
+        hipLaunchKernelGGL(k_weights_frag<false>, dim3(grid((size_t)NFRAG, 256)), dim3(256), 0, st, s->P,
+                           (u32x4 *)s->Wf);
+        hipLaunchKernelGGL(k_weights_frag<true>, dim3(grid((size_t)NFRAG, 256)), dim3(256), 0, st, s->P,
+                           (u32x4 *)s->Wh);
+    }
     TP_HIP(hipGetLastError());
     return 0;
 }
@@ -694,6 +702,9 @@ constexpr int NSTAGE = 12;  // stages per chunk: 3 layers x 16 K-blocks / KPS
 constexpr int SU4 = KPS * 512;  // 16-byte fragments per stage
 constexpr int SP_B1 = H * NIN, SP_W5 = SP_B1 + 4 * H, SP_N = SP_W5 + H;  // LDS copy: W1, b1..b4, W5
 
+// FP16: the BF16-rounded weight (the value the network was trained with), exact in FP16 down to
+// |w| = 2^-14 (subnormal below; 2^-24 of a weight is far below the activations' rounding)
+template <bool FP16>
 __global__ void k_weights_frag(const float *__restrict__ P, u32x4 *__restrict__ Wf)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -707,7 +718,8 @@ __global__ void k_weights_frag(const float *__restrict__ P, u32x4 *__restrict__ 
         for (int m = 0; m < 2; ++m) {
             const int k = 16 * kb + 8 * h + 2 * q2 + m;
             const int u = (k & ~12) | ((k & 4) << 1) | ((k & 8) >> 1);
-            v[m] = f2bf(W[(size_t)(32 * t + r) * H + u]);
+            const unsigned short b = f2bf(W[(size_t)(32 * t + r) * H + u]);
+            v[m] = FP16 ? __builtin_bit_cast(unsigned short, (_Float16)bf2f(b)) : b;
         }
         w[q2] = v[0] | (v[1] << 16);
     }
@@ -889,6 +901,153 @@ __global__ void __launch_bounds__(256, 1) k_mlp_reg(const float *__restrict__ X,
         if (h == 0 && valid) logb[row] = (double)(o + P[OFF_B5]) * (double)sd + (double)mean;
     }
 }
+
+// ---------------------------------------------------------------- k_mlp_h: FP16 activations, one pass
+// The same transposed layers, weight order and LDS weight stream as k_mlp_reg, with the hidden
+// activations rounded once to FP16 (11 significant bits; round to nearest even) and the weights in
+// FP16 (the trained BF16 values, exact): v_mfma_f32_32x32x16_f16, one MFMA per weight fragment
+// instead of two.  The rounding leaves a relative staircase of ~1e-5 in the output, which the 1e-4
+// cubature converges through with the same point counts (one BF16 rounding, 2^-8, did not).
+// One accumulator set: a layer's bias/SiLU/FP16 epilogue runs between its MFMA phases, and two
+// workgroups per CU (256 VGPRs per lane, 77 KB of LDS each) keep the matrix cores busy through it.
+typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
+typedef __attribute__((__vector_size__(2 * sizeof(float)))) float f32x2;
+typedef __attribute__((__vector_size__(2 * sizeof(_Float16)))) _Float16 f16x2;
+
+// bias + SiLU + FP16 of tile T of a layer's FP32 result: the B operands of K slices 2T and 2T+1
+template <int T>
+__device__ __forceinline__ void mlph_epi(const f32x16 (&acc)[8], const float *__restrict__ bias, int h,
+                                         f16x8 (&B)[16])
+{
+#pragma unroll
+    for (int JJ = 0; JJ < 2; ++JJ) {
+        const float4 b0 = *(const float4 *)(bias + 32 * T + 16 * JJ + 4 * h);
+        const float4 b1 = *(const float4 *)(bias + 32 * T + 16 * JJ + 8 + 4 * h);
+        const float b[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+        for (int q = 0; q < 8; q += 2) {
+            const f32x2 v = {silu_fast(acc[T][8 * JJ + q] + b[q]), silu_fast(acc[T][8 * JJ + q + 1] + b[q + 1])};
+            const f16x2 hv = __builtin_convertvector(v, f16x2);
+            B[2 * T + JJ][q] = hv[0];
+            B[2 * T + JJ][q + 1] = hv[1];
+        }
+    }
+}
+
+// one 16-wide K slice: the 8 weight fragments in two halves of 4 (16 registers in flight, not 32)
+template <int KB>
+__device__ __forceinline__ void mlph_kslice(const u32x4 *src, f32x16 (&acc)[8], const f16x8 (&B)[16], int lane)
+{
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+        f16x8 a[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) a[t] = __builtin_bit_cast(f16x8, src[(4 * half + t) * 64 + lane]);
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            acc[4 * half + t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[t], B[KB], acc[4 * half + t], 0, 0, 0);
+    }
+}
+
+template <int ST>
+__device__ __forceinline__ void mlph_stages(f32x16 (&acc)[8], const f16x8 (&B)[16], u32x4 (*ring)[SU4],
+                                            const u32x4 *__restrict__ Wf, int gs, int wave, int lane)
+{
+    if constexpr (ST < 16 / KPS) {
+        mlp_stage_load(ring[(ST + 1) & 1], Wf, (gs + ST + 1) % NSTAGE, wave, lane);
+        const u32x4 *src = ring[ST & 1];
+        mlph_kslice<KPS * ST + 0>(src, acc, B, lane);
+        mlph_kslice<KPS * ST + 1>(src + 512, acc, B, lane);
+        mlph_kslice<KPS * ST + 2>(src + 1024, acc, B, lane);
+        mlph_kslice<KPS * ST + 3>(src + 1536, acc, B, lane);
+        __syncthreads();
+        mlph_stages<ST + 1>(acc, B, ring, Wf, gs, wave, lane);
+    }
+}
+
+// one hidden layer in place: acc = W_L (SiLU(acc + bias_in) in FP16)
+template <int L>
+__device__ __forceinline__ void mlph_layer(f32x16 (&acc)[8], const float *__restrict__ bias_in, u32x4 (*ring)[SU4],
+                                           const u32x4 *__restrict__ Wf, int wave, int lane, int h)
+{
+    f16x8 B[16];
+    // the epilogue's LDS reads stay here (hoisted into the previous layer's MFMA phase they only spill)
+    __builtin_amdgcn_sched_barrier(0);
+    mlph_epi<0>(acc, bias_in, h, B);
+    mlph_epi<1>(acc, bias_in, h, B);
+    mlph_epi<2>(acc, bias_in, h, B);
+    mlph_epi<3>(acc, bias_in, h, B);
+    mlph_epi<4>(acc, bias_in, h, B);
+    mlph_epi<5>(acc, bias_in, h, B);
+    mlph_epi<6>(acc, bias_in, h, B);
+    mlph_epi<7>(acc, bias_in, h, B);
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[t][i] = 0.f;
+    __builtin_amdgcn_sched_barrier(0);
+    mlph_stages<0>(acc, B, ring, Wf, L * (16 / KPS), wave, lane);
+}
+
+__global__ void __launch_bounds__(256, 2) k_mlp_h(const float *__restrict__ X, int M, const float *__restrict__ P,
+                                                  const u32x4 *__restrict__ Wf, float mean, float sd,
+                                                  double *__restrict__ logb)
+{
+    __shared__ u32x4 ring[2][SU4];
+    __shared__ __attribute__((aligned(16))) float sP[SP_N];
+    // the wave index in a scalar register: the LDS slot addresses of the weight stream (M0) stay scalar
+    const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int r = lane & 31, h = lane >> 5;
+    const int nchunk = (M + RB - 1) / RB;
+    for (int e = tid; e < SP_N; e += 256) {
+        size_t src;
+        if (e < SP_B1) {
+            src = OFF_W1 + e;
+        } else if (e < SP_W5) {
+            const int which = (e - SP_B1) / H, k = (e - SP_B1) % H;
+            src = (which == 0 ? OFF_B1 : OFF_B[which - 1]) + k;
+        } else {
+            src = OFF_W5 + (e - SP_W5);
+        }
+        sP[e] = P[src];
+    }
+    mlp_stage_load(ring[0], Wf, 0, wave, lane);
+    __syncthreads();
+#pragma unroll 1
+    for (int c = blockIdx.x; c < nchunk; c += gridDim.x) {
+        const int row = c * RB + wave * 32 + r;
+        const bool valid = row < M;
+        f32x16 acc[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) acc[t][i] = 0.f;
+#pragma unroll
+        for (int cc = 0; cc < NIN / 2; ++cc) {
+            const float xb = valid ? X[(size_t)row * NIN + 2 * cc + h] : 0.f;
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(sP[(32 * t + r) * NIN + 2 * cc + h], xb, acc[t], 0, 0, 0);
+        }
+        const u32x4 *wf = Wf;
+        asm volatile("" : "+s"(wf));
+        mlph_layer<0>(acc, sP + SP_B1, ring, wf, wave, lane, h);
+        mlph_layer<1>(acc, sP + SP_B1 + H, ring, wf, wave, lane, h);
+        mlph_layer<2>(acc, sP + SP_B1 + 2 * H, ring, wf, wave, lane, h);
+        float o = 0.f;
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const float4 b = *(const float4 *)(sP + SP_B1 + 3 * H + 32 * t + 8 * j + 4 * h);
+                const float4 w = *(const float4 *)(sP + SP_W5 + 32 * t + 8 * j + 4 * h);
+                o += w.x * silu_fast(acc[t][4 * j] + b.x) + w.y * silu_fast(acc[t][4 * j + 1] + b.y) +
+                     w.z * silu_fast(acc[t][4 * j + 2] + b.z) + w.w * silu_fast(acc[t][4 * j + 3] + b.w);
+            }
+        o += __shfl_xor(o, 32, 64);
+        if (h == 0 && valid) logb[row] = (double)(o + P[OFF_B5]) * (double)sd + (double)mean;
+    }
+}
 }  // namespace
 
 // log B for rows [0, M) of X (M % 64 == 0) into logb, using two ping-pong activation buffers
@@ -906,8 +1065,12 @@ static int sur_infer(hipStream_t st, TpSurrogate *s, const float *X, int M, unsi
         if (ncu < 1) ncu = 1;
     }
     const int nchunk = (M + RB - 1) / RB;
-    hipLaunchKernelGGL(k_mlp_reg, dim3(std::min(nchunk, ncu)), dim3(256), 0, st, X, M, s->P, (const u32x4 *)s->Wf,
-                       s->t_mean, s->t_std, logb);
+    if (s->infer == TP_SUR_INFER_FP16)  // two workgroups per CU
+        hipLaunchKernelGGL(k_mlp_h, dim3(std::min(nchunk, 2 * ncu)), dim3(256), 0, st, X, M, s->P,
+                           (const u32x4 *)s->Wh, s->t_mean, s->t_std, logb);
+    else
+        hipLaunchKernelGGL(k_mlp_reg, dim3(std::min(nchunk, ncu)), dim3(256), 0, st, X, M, s->P,
+                           (const u32x4 *)s->Wf, s->t_mean, s->t_std, logb);
     TP_HIP(hipGetLastError());
     return 0;
 }
@@ -1152,6 +1315,16 @@ extern "C" int tp_surrogate_load(tp_ctx *ctx, const char *path)
     TP_HIP(hipMemcpy(ctx->sur->P, p.data(), NPARAM * sizeof(float), hipMemcpyHostToDevice));
     ctx->sur->trained = true;
     return sur_refresh_bf16(ctx->stream, ctx->sur);
+}
+
+extern "C" int tp_surrogate_inference(tp_ctx *ctx, int kind)
+{
+    if (!ctx || (kind != TP_SUR_INFER_HILO_BF16 && kind != TP_SUR_INFER_FP16))
+        return tp_set_error(TP_EINVAL, "tp_surrogate_inference: kind must be TP_SUR_INFER_HILO_BF16 or _FP16");
+    int rc;
+    if ((rc = ensure_sur(ctx))) return rc;
+    ctx->sur->infer = kind;
+    return 0;
 }
 
 extern "C" int tp_surrogate_config(tp_ctx *ctx, double check_frac, double tol, uint64_t seed)
