@@ -93,14 +93,19 @@ def fp16(x):
     return np.asarray(x, dtype=np.float32).astype(np.float16).astype(np.float64)
 
 
+NEG_LOG2E = -1.4426950408889634
+
+
 def forward_fp16(P, X):
-    """the FP16 inference kernel's arithmetic: FP16 activations (one rounding), the BF16 weights of layers 2-4
-    (exact in FP16 within its range)"""
-    a = fp16(silu(X @ P["W1"].T + P["b1"]))
+    """the FP16 inference kernel's arithmetic: the BF16 weights of layers 2-4 (exact in FP16 within its range),
+    pre-activations scaled by -log2(e) (u = -log2(e) z, from W1 and the biases scaled in FP32) and the FP16
+    activations a' = u / (1 + 2^u) = -log2(e) SiLU(z), rounded once; the output weights divided by -log2(e)"""
+    f32 = lambda v: np.asarray(v, dtype=np.float32).astype(np.float64)  # noqa: E731
+    sil = lambda u: u / (1.0 + np.exp2(u))  # noqa: E731
+    u = X @ f32(P["W1"] * NEG_LOG2E).T + f32(P["b1"] * NEG_LOG2E)
     for k in (2, 3, 4):
-        z = a @ fp16(bf16(P["W%d" % k])).T + P["b%d" % k]
-        a = fp16(silu(z)) if k < 4 else silu(z)
-    return a @ P["W5"] + P["b5"][0]
+        u = fp16(sil(u)) @ fp16(bf16(P["W%d" % k])).T + f32(P["b%d" % k] * NEG_LOG2E)
+    return sil(u) @ f32(P["W5"] / NEG_LOG2E) + P["b5"][0]
 
 
 MODEL_MAGIC = b"TPMLP002"

@@ -189,3 +189,22 @@ def test_cv_style_caller_compiles(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-I" + os.path.join(ROOT, "include"),
                         str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_bench_process_loads_no_torch():
+    """The bench's process holds no PyTorch: importing bench.py (numpy, ctypes and this package) leaves torch out
+    of sys.modules, and the HIP library links no torch library.  (Profiled runs use benchMap3.x, a C++ program
+    linked only against the library.)"""
+    import subprocess
+    import sys
+    code = ("import importlib.util, sys; sys.path.insert(0, %r); "
+            "spec = importlib.util.spec_from_file_location('bench', %r); m = importlib.util.module_from_spec(spec); "
+            "spec.loader.exec_module(m); import threepoint_amd; "
+            "print(sorted(k for k in sys.modules if k.split('.')[0] == 'torch'))") % (ROOT, os.path.join(ROOT, "bench.py"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip() == "[]", r.stdout
+    lib = os.path.join(ROOT, "threepoint_amd", "_lib", "libthreepoint_amd.so")
+    if os.path.exists(lib):
+        ldd = subprocess.run(["ldd", lib], capture_output=True, text=True).stdout
+        assert "torch" not in ldd and "c10" not in ldd

@@ -907,31 +907,50 @@ __global__ void __launch_bounds__(256, 1) k_mlp_reg(const float *__restrict__ X,
 // activations rounded once to FP16 (11 significant bits; round to nearest even) and the weights in
 // FP16 (the trained BF16 values, exact): v_mfma_f32_32x32x16_f16, one MFMA per weight fragment
 // instead of two.  The rounding leaves a relative staircase of ~1e-5 in the output, which the 1e-4
-// cubature converges through with the same point counts (one BF16 rounding, 2^-8, did not).
-// One accumulator set: a layer's bias/SiLU/FP16 epilogue runs between its MFMA phases, and two
-// workgroups per CU (256 VGPRs per lane, 77 KB of LDS each) keep the matrix cores busy through it.
+// cubature converges through with about the same point counts (one BF16 rounding, 2^-8, did not).
+// One accumulator set: a layer's SiLU/FP16 epilogue runs between its MFMA phases, and two workgroups
+// per CU (256 VGPRs per lane, 77 KB of LDS each) keep the matrix cores busy through it.
+//
+// Scaled pre-activations: the accumulators hold u = -log2(e) z instead of z (layer 1's weights and
+// every bias are scaled by -log2(e) in LDS, and each layer's accumulators start at its scaled bias),
+// and the activations carry the same factor, a' = -log2(e) SiLU(z) = u / (1 + 2^u): the hidden
+// weights are unchanged (the factor passes through them) and the output weights are divided by
+// -log2(e).  SiLU then costs v_exp_f32, one add, v_rcp_f32 and one multiply.
 typedef __attribute__((__vector_size__(8 * sizeof(_Float16)))) _Float16 f16x8;
 typedef __attribute__((__vector_size__(2 * sizeof(float)))) float f32x2;
 typedef __attribute__((__vector_size__(2 * sizeof(_Float16)))) _Float16 f16x2;
+constexpr float kNegLog2e = -1.44269504088896340736f;
 
-// bias + SiLU + FP16 of tile T of a layer's FP32 result: the B operands of K slices 2T and 2T+1
+__device__ __forceinline__ float silu_scaled(float u) { return u * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(u)); }
+
+// SiLU + FP16 of tile T of a layer's scaled result: the B operands of K slices 2T and 2T+1
 template <int T>
-__device__ __forceinline__ void mlph_epi(const f32x16 (&acc)[8], const float *__restrict__ bias, int h,
-                                         f16x8 (&B)[16])
+__device__ __forceinline__ void mlph_epi(const f32x16 (&acc)[8], f16x8 (&B)[16])
 {
 #pragma unroll
-    for (int JJ = 0; JJ < 2; ++JJ) {
-        const float4 b0 = *(const float4 *)(bias + 32 * T + 16 * JJ + 4 * h);
-        const float4 b1 = *(const float4 *)(bias + 32 * T + 16 * JJ + 8 + 4 * h);
-        const float b[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+    for (int JJ = 0; JJ < 2; ++JJ)
 #pragma unroll
         for (int q = 0; q < 8; q += 2) {
-            const f32x2 v = {silu_fast(acc[T][8 * JJ + q] + b[q]), silu_fast(acc[T][8 * JJ + q + 1] + b[q + 1])};
+            const f32x2 v = {silu_scaled(acc[T][8 * JJ + q]), silu_scaled(acc[T][8 * JJ + q + 1])};
             const f16x2 hv = __builtin_convertvector(v, f16x2);
             B[2 * T + JJ][q] = hv[0];
             B[2 * T + JJ][q + 1] = hv[1];
         }
-    }
+}
+
+// accumulators := the layer's scaled bias (element i of tile t is unit 32 t + 8 (i / 4) + 4 h + i % 4)
+__device__ __forceinline__ void mlph_bias_init(f32x16 (&acc)[8], const float *__restrict__ bias, int h)
+{
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const float4 b = *(const float4 *)(bias + 32 * t + 8 * j + 4 * h);
+            acc[t][4 * j] = b.x;
+            acc[t][4 * j + 1] = b.y;
+            acc[t][4 * j + 2] = b.z;
+            acc[t][4 * j + 3] = b.w;
+        }
 }
 
 // one 16-wide K slice: the 8 weight fragments in two halves of 4 (16 registers in flight, not 32)
@@ -965,26 +984,23 @@ __device__ __forceinline__ void mlph_stages(f32x16 (&acc)[8], const f16x8 (&B)[1
     }
 }
 
-// one hidden layer in place: acc = W_L (SiLU(acc + bias_in) in FP16)
+// one hidden layer in place: acc = W_L a' + bias_out (scaled), from acc = the previous layer's scaled result
 template <int L>
-__device__ __forceinline__ void mlph_layer(f32x16 (&acc)[8], const float *__restrict__ bias_in, u32x4 (*ring)[SU4],
+__device__ __forceinline__ void mlph_layer(f32x16 (&acc)[8], const float *__restrict__ bias_out, u32x4 (*ring)[SU4],
                                            const u32x4 *__restrict__ Wf, int wave, int lane, int h)
 {
     f16x8 B[16];
-    // the epilogue's LDS reads stay here (hoisted into the previous layer's MFMA phase they only spill)
+    // the epilogue and the bias reads stay here (hoisted into the previous layer's MFMA phase they only spill)
     __builtin_amdgcn_sched_barrier(0);
-    mlph_epi<0>(acc, bias_in, h, B);
-    mlph_epi<1>(acc, bias_in, h, B);
-    mlph_epi<2>(acc, bias_in, h, B);
-    mlph_epi<3>(acc, bias_in, h, B);
-    mlph_epi<4>(acc, bias_in, h, B);
-    mlph_epi<5>(acc, bias_in, h, B);
-    mlph_epi<6>(acc, bias_in, h, B);
-    mlph_epi<7>(acc, bias_in, h, B);
-#pragma unroll
-    for (int t = 0; t < 8; ++t)
-#pragma unroll
-        for (int i = 0; i < 16; ++i) acc[t][i] = 0.f;
+    mlph_epi<0>(acc, B);
+    mlph_epi<1>(acc, B);
+    mlph_epi<2>(acc, B);
+    mlph_epi<3>(acc, B);
+    mlph_epi<4>(acc, B);
+    mlph_epi<5>(acc, B);
+    mlph_epi<6>(acc, B);
+    mlph_epi<7>(acc, B);
+    mlph_bias_init(acc, bias_out, h);
     __builtin_amdgcn_sched_barrier(0);
     mlph_stages<0>(acc, B, ring, Wf, L * (16 / KPS), wave, lane);
 }
@@ -994,6 +1010,7 @@ __global__ void __launch_bounds__(256, 2) k_mlp_h(const float *__restrict__ X, i
                                                   double *__restrict__ logb)
 {
     __shared__ u32x4 ring[2][SU4];
+    // W1 and b1..b4 times -log2(e), W5 divided by it
     __shared__ __attribute__((aligned(16))) float sP[SP_N];
     // the wave index in a scalar register: the LDS slot addresses of the weight stream (M0) stay scalar
     const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1001,15 +1018,17 @@ __global__ void __launch_bounds__(256, 2) k_mlp_h(const float *__restrict__ X, i
     const int nchunk = (M + RB - 1) / RB;
     for (int e = tid; e < SP_N; e += 256) {
         size_t src;
+        float scale = kNegLog2e;
         if (e < SP_B1) {
             src = OFF_W1 + e;
         } else if (e < SP_W5) {
-            const int which = (e - SP_B1) / H, k = (e - SP_B1) % H;
+            const int which = (e - SP_B1) / H, k = (e - SP_B1) % H;  // b1, b2, b3, b4
             src = (which == 0 ? OFF_B1 : OFF_B[which - 1]) + k;
         } else {
             src = OFF_W5 + (e - SP_W5);
+            scale = 1.0f / kNegLog2e;
         }
-        sP[e] = P[src];
+        sP[e] = P[src] * scale;
     }
     mlp_stage_load(ring[0], Wf, 0, wave, lane);
     __syncthreads();
@@ -1018,10 +1037,7 @@ __global__ void __launch_bounds__(256, 2) k_mlp_h(const float *__restrict__ X, i
         const int row = c * RB + wave * 32 + r;
         const bool valid = row < M;
         f32x16 acc[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t)
-#pragma unroll
-            for (int i = 0; i < 16; ++i) acc[t][i] = 0.f;
+        mlph_bias_init(acc, sP + SP_B1, h);
 #pragma unroll
         for (int cc = 0; cc < NIN / 2; ++cc) {
             const float xb = valid ? X[(size_t)row * NIN + 2 * cc + h] : 0.f;
@@ -1031,18 +1047,17 @@ __global__ void __launch_bounds__(256, 2) k_mlp_h(const float *__restrict__ X, i
         }
         const u32x4 *wf = Wf;
         asm volatile("" : "+s"(wf));
-        mlph_layer<0>(acc, sP + SP_B1, ring, wf, wave, lane, h);
-        mlph_layer<1>(acc, sP + SP_B1 + H, ring, wf, wave, lane, h);
-        mlph_layer<2>(acc, sP + SP_B1 + 2 * H, ring, wf, wave, lane, h);
+        mlph_layer<0>(acc, sP + SP_B1 + H, ring, wf, wave, lane, h);
+        mlph_layer<1>(acc, sP + SP_B1 + 2 * H, ring, wf, wave, lane, h);
+        mlph_layer<2>(acc, sP + SP_B1 + 3 * H, ring, wf, wave, lane, h);
         float o = 0.f;
 #pragma unroll
         for (int t = 0; t < 8; ++t)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const float4 b = *(const float4 *)(sP + SP_B1 + 3 * H + 32 * t + 8 * j + 4 * h);
                 const float4 w = *(const float4 *)(sP + SP_W5 + 32 * t + 8 * j + 4 * h);
-                o += w.x * silu_fast(acc[t][4 * j] + b.x) + w.y * silu_fast(acc[t][4 * j + 1] + b.y) +
-                     w.z * silu_fast(acc[t][4 * j + 2] + b.z) + w.w * silu_fast(acc[t][4 * j + 3] + b.w);
+                o += w.x * silu_scaled(acc[t][4 * j]) + w.y * silu_scaled(acc[t][4 * j + 1]) +
+                     w.z * silu_scaled(acc[t][4 * j + 2]) + w.w * silu_scaled(acc[t][4 * j + 3]);
             }
         o += __shfl_xor(o, 32, 64);
         if (h == 0 && valid) logb[row] = (double)(o + P[OFF_B5]) * (double)sd + (double)mean;
@@ -1427,7 +1442,10 @@ __global__ void __launch_bounds__(256) k_sur_points(const TpRegion *__restrict__
     gm_point3(R, p, &l1, &l2, &phi);
     double l3 = tp_ell3(l1, l2, phi);
     if (!isfinite(l1) || !isfinite(l2) || !isfinite(l3)) atomicOr(err, 1);
-    F[t] = l1 * l2 * tp_uhat<double>(l1 * D.th[0]) * tp_uhat<double>(l2 * D.th[1]) * tp_uhat<double>(l3 * D.th[2]);
+    // l1 l2 u(l1 t1) u(l2 t2) u(l3 t3), u(x) = x^2/2 exp(-x^2/2), with one exponential for the three filters
+    const double a = l1 * D.th[0], b = l2 * D.th[1], e = l3 * D.th[2];
+    const double a2 = a * a, b2 = b * b, e2 = e * e;
+    F[t] = l1 * l2 * (0.125 * a2 * b2 * e2) * exp(-0.5 * (a2 + b2 + e2));
     const TpCosmoConst &C = cc[D.icosmo];
     double cp[5] = {C.om, C.sigma8, C.h, C.w, C.ns};
     make_input(box, l1, l2, l3 > 0 ? l3 : 1.0, cp, X + t * NIN);
@@ -1444,14 +1462,16 @@ __global__ void __launch_bounds__(256) k_sur_points(const TpRegion *__restrict__
     if (f == 1 || f == 2) list[atomicAdd(nlist, 1)] = (int)t;
 }
 
-// exact B_kappa of the listed points: 16 lanes per point, 6 of the 96 z-nodes each, so a round's
-// few thousand checked points still fill the device.  The list's length is known on the device only: the
-// grid is sized for the expected checks (kSurExactBlocks blocks of 16 points) and strides over the list.
+// exact B_kappa of the listed points in the FP32 form (FP32 bispectrum, FP64 z-sum: the arithmetic of the
+// exact regions of a surrogate round, so the checks correct the network toward the same values): 16 lanes
+// per point, 6 of the 96 z-nodes each, so a round's few thousand checked points still fill the device.  The
+// list's length is known on the device only: the grid is sized for the expected checks (kSurExactBlocks
+// blocks of 16 points) and strides over the list.
 constexpr int XL = 16;
 constexpr int kSurExactBlocks = 1024;
 __global__ void __launch_bounds__(256) k_sur_exact(const int *list, const int *nlist, const TpRegion *__restrict__ reg,
                                                    size_t r0, const TpIntegralDesc *__restrict__ desc,
-                                                   const TpNode64 *__restrict__ nodes, const TpEH64 *__restrict__ eh,
+                                                   const TpNode32 *__restrict__ nodes, const TpEH32 *__restrict__ eh,
                                                    double *Bex, int *err)
 {
     const int n = *nlist;
@@ -1469,14 +1489,18 @@ __global__ void __launch_bounds__(256) k_sur_exact(const int *list, const int *n
         const double l3 = tp_ell3(l1, l2, phi);
         TpPointPre P;
         tp_point_pre<double>(l1, l2, l3, &P);
-        const TpNode64 *nd = nodes + (size_t)D.icosmo * TP_NNODES;
-        const TpEH64 E = eh[D.icosmo];
+        const float L[3] = {(float)l1, (float)l2, (float)l3};
+        const float LL[3] = {(float)P.ll[0], (float)P.ll[1], (float)P.ll[2]};
+        const float F2T[3] = {(float)P.f2t[0], (float)P.f2t[1], (float)P.f2t[2]};
+        const float LR1 = (float)P.lr1, R2 = (float)P.r2;
+        const TpNode32 *nd = nodes + (size_t)D.icosmo * TP_NNODES;
+        const TpEH32 E = eh[D.icosmo];
         double part = 0.0;
         for (int k = sub; k < TP_NNODES; k += XL) {
-            const TpNode64 &N = nd[k];
-            double q[3] = {l1 * N.ks, l2 * N.ks, l3 * N.ks};
-            double lq[3] = {P.ll[0] + N.lnks, P.ll[1] + N.lnks, P.ll[2] + N.lnks};
-            part += N.wz * tp_bispec_node<double>(N, E, q, lq, P.f2t, P.lr1, P.r2);
+            const TpNode32 &N = nd[k];
+            float q[3] = {L[0] * N.ks, L[1] * N.ks, L[2] * N.ks};
+            float lq[3] = {LL[0] + N.lnks, LL[1] + N.lnks, LL[2] + N.lnks};
+            part += (double)(N.wz * tp_bispec_node<float>(N, E, q, lq, F2T, LR1, R2));
         }
         for (int o = XL / 2; o >= 1; o >>= 1) part += __shfl_xor(part, o, XL);
         if (sub == 0) {
@@ -1492,7 +1516,8 @@ __global__ void k_sur_combine(size_t n, const int *flag, const double *F, const 
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     int f = flag[t];
-    double bs = Bb[t] * exp(logb[t]);  // baseline x learned correction
+    // baseline x learned correction; the network's output carries FP32 precision, so its exp is FP32
+    double bs = Bb[t] * (double)__expf((float)logb[t]);
     double v = 0, r = 0;
     if (f == 0) {
         v = F[t] * bs;
@@ -1598,7 +1623,7 @@ int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, co
         B.ev_points.push_back((long long)npp);
         B.nev += 1;
         hipLaunchKernelGGL(k_sur_exact, dim3(kSurExactBlocks), dim3(256), 0, st, (const int *)B.list.p, dn, d_reg, r0,
-                           d_desc, c->d_node64, c->d_eh64, (double *)B.Bex.p, d_flag);
+                           d_desc, c->d_node32, c->d_eh32, (double *)B.Bex.p, d_flag);
         hipLaunchKernelGGL(k_sur_combine, dim3(grid(np, 256)), dim3(256), 0, st, np, (const int *)B.flag.p,
                            (const double *)B.F.p, (const double *)B.logb.p, (const double *)B.Bb.p,
                            (const double *)B.Bex.p, d_fvals + r0 * TP_NPTS_GM3, (double *)B.resid.p, dmaxrel);
