@@ -63,6 +63,56 @@ def test_calculate_gamma_cli(tmp_path):
     assert np.allclose(rows[:, 9], 0.5) and np.allclose(rows[:, 10], [-0.5, 0.5])
 
 
+def test_map3_cli_chunks_and_resume(tmp_path):
+    """--chunk N writes the triples in batches with the same lines as one batch; --resume after an interrupted
+    run (9 complete lines and a cut-off 10th) keeps the 9 and completes the file to the same 20 lines."""
+    cf, nf = _inputs(tmp_path)
+    tf = tmp_path / "thetas.dat"
+    tf.write_text("2\n4\n8\n16\n")
+    exe = os.path.join(LIBDIR, "calculateApertureStatistics.x")
+
+    def run(out, *extra):
+        r = subprocess.run([exe, str(cf), str(tf), str(out), str(nf)] + list(extra), capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr
+        return out.read_text()
+
+    full = run(tmp_path / "a.dat")
+    assert run(tmp_path / "b.dat", "--chunk", "7") == full
+    lines = full.splitlines(keepends=True)
+    part = tmp_path / "c.dat"
+    part.write_text("".join(lines[:9]) + lines[9][:8])
+    assert run(part, "--resume", "--chunk", "7") == full
+    # a file of another list is refused
+    bad = tmp_path / "d.dat"
+    bad.write_text("3 3 3 1e-6\n")
+    r = subprocess.run([exe, str(cf), str(tf), str(bad), str(nf), "--resume"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 1 and "does not hold" in r.stderr
+
+
+def test_gamma_cli_resume(tmp_path):
+    """--resume keeps the complete r bins of an interrupted grid run and computes the rest: same file."""
+    cf, nf = _inputs(tmp_path)
+    gc = tmp_path / "gamma.param"
+    gc.write_text("rsteps 2\nusteps 1\nvsteps 2\numin 0.4\numax 0.6\nvmin -1\nvmax 1\nrmin 4\nrmax 6\n")
+    exe = os.path.join(LIBDIR, "calculateGamma.x")
+    args = ["--prec-h", "0.1", "--prec-k", "2", "--reltol", "1e-2"]
+
+    def run(out, *extra):
+        r = subprocess.run([exe, str(cf), str(gc), str(out), str(nf)] + args + list(extra), capture_output=True,
+                           text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        return out.read_text()
+
+    full = run(tmp_path / "a.dat")
+    lines = full.splitlines(keepends=True)
+    assert len(lines) == 4
+    part = tmp_path / "b.dat"
+    part.write_text("".join(lines[:3]))  # the first r bin and half of the second
+    assert run(part, "--resume") == full
+
+
 def test_calculate_powerspectrum_cli(tmp_path, fid_oracle):
     """ell = 1.05^i < 5e4 and P_kappa(ell) with the T17 factor (the cuda_version's default) or without."""
     cf, nf = _inputs(tmp_path)

@@ -2,12 +2,13 @@
 //
 //   calculateGamma.x <cosmology file> <gamma config file> <output file> <n(z) file> [device]
 //                    [--nz-bins N] [--nz-averaging] [--prec-h H] [--prec-k K] [--reltol X]
-//                    [--triangles FILE] [--fp32]
+//                    [--triangles FILE] [--fp32] [--resume]
 //
 // Gamma_0..Gamma_3 (centroid convention) on the (r, u, v) grid of the config file, r2 = r pi/180/60,
 // r3 = r2 u, r1 = v r3 + r2 (the TreeCorr binning of cuda_version/calculateGamma.cu:107-121), one
 // output line per triangle: Re/Im Gamma0 ... Re/Im Gamma3, r, u, v.  All triangles of one r bin are
-// evaluated in one batched GPU pass and written before the next r bin starts.
+// evaluated in one batched GPU pass and written (and flushed) before the next r bin starts; --resume keeps
+// the complete r bins an interrupted run left in the output file and computes the rest.
 //
 // --triangles FILE: the classed_version's mode (calculateGamma.cpp:64-160): the (r, u, v) bins of a
 // TreeCorr triangle list ("rind uind vind r u v" rows, read_triangle_configurations), sides converted
@@ -24,26 +25,28 @@
 #include <iostream>
 #include <vector>
 
+#include "app_resume.h"
 #include "threepoint_amd.hpp"
 
 int main(int argc, char **argv)
 {
     const char *usage = "calculateGamma.x <cosmology> <gamma config> <output> <n(z)> [device] [--nz-bins N]\n"
                         "                 [--nz-averaging] [--prec-h H] [--prec-k K] [--reltol X] [--triangles FILE]\n"
-                        "                 [--fp32]\n";
+                        "                 [--fp32] [--resume]\n";
     if (argc < 5) {
         std::cerr << usage;
         return 1;
     }
     int nbins = 100, device = 0;
     double prec_h = 1e-3, prec_k = 5.0, reltol = 1e-3;
-    bool nz_avg = false, fp32 = false;
+    bool nz_avg = false, fp32 = false, resume = false;
     std::string trifile;
     for (int i = 5; i < argc; ++i) {
         std::string a = argv[i];
         if (a == "--nz-bins" && i + 1 < argc) nbins = std::atoi(argv[++i]);
         else if (a == "--nz-averaging") nz_avg = true;
         else if (a == "--fp32") fp32 = true;
+        else if (a == "--resume") resume = true;
         else if (a == "--triangles" && i + 1 < argc) trifile = argv[++i];
         else if (a == "--prec-h" && i + 1 < argc) prec_h = std::atof(argv[++i]);
         else if (a == "--prec-k" && i + 1 < argc) prec_k = std::atof(argv[++i]);
@@ -60,12 +63,13 @@ int main(int argc, char **argv)
         std::vector<double> nz;
         if (nz_avg) read_n_of_z_averaging(argv[4], nbins, cosmo.zmax, nz);
         else read_n_of_z(argv[4], nbins, cosmo.zmax, nz);
-        std::ofstream out(argv[3]);
-        if (!out.is_open()) {
-            std::cerr << "Couldn't open " << argv[3] << std::endl;
-            return 1;
-        }
         if (!trifile.empty()) {
+            if (resume) throw std::runtime_error("--resume applies to the (r, u, v) grid, not to --triangles");
+            std::ofstream out(argv[3]);
+            if (!out.is_open()) {
+                std::cerr << "Couldn't open " << argv[3] << std::endl;
+                return 1;
+            }
             std::vector<treecorr_bin> bins;
             std::vector<std::array<int, 3>> idx;
             {  // the row indices travel with the bins
@@ -117,9 +121,33 @@ int main(int argc, char **argv)
         gc.reltol = reltol;
         if (fp32) gc.mode = TP_MODE_FP32;
         double lrmin = std::log(config.rmin), lrmax = std::log(config.rmax);
+        auto rbin = [&](int i) { return std::exp(lrmin + (lrmax - lrmin) / config.rsteps * (i + 0.5)); };
+        // --resume: the r bins the output file holds in full (each line ends with r u v)
+        const size_t per_bin = (size_t)config.usteps * config.vsteps;
+        int first = 0;
+        std::vector<std::string> old;
+        if (resume) {
+            old = tp_app::complete_lines(argv[3]);
+            size_t ok = 0;
+            for (; ok < old.size() && ok / per_bin < (size_t)config.rsteps; ++ok) {
+                const auto v = tp_app::leading_numbers(old[ok], 9);
+                const double r = rbin((int)(ok / per_bin));
+                if (v.size() < 9) break;
+                if (std::fabs(v[8] - r) > 1e-5 * r)
+                    throw std::runtime_error("--resume: line " + std::to_string(ok + 1) + " of " + argv[3] +
+                                             " does not belong to this grid");
+            }
+            first = (int)(ok / per_bin);
+            std::cerr << "resuming after " << first << " of " << config.rsteps << " r bins" << std::endl;
+        }
+        std::ofstream out = tp_app::keep_and_append(argv[3], old, (size_t)first * per_bin);
+        if (!out.is_open()) {
+            std::cerr << "Couldn't open " << argv[3] << std::endl;
+            return 1;
+        }
         auto t0 = std::chrono::steady_clock::now();
-        for (int i = 0; i < config.rsteps; i++) {
-            double r = std::exp(lrmin + (lrmax - lrmin) / config.rsteps * (i + 0.5));
+        for (int i = first; i < config.rsteps; i++) {
+            double r = rbin(i);
             std::vector<std::array<double, 3>> x;
             std::vector<std::array<double, 2>> uv;
             for (int j = 0; j < config.usteps; j++) {

@@ -1459,7 +1459,17 @@ __global__ void __launch_bounds__(256) k_sur_points(const TpRegion *__restrict__
     flag[t] = f;
     Bb[t] = (f <= 1) ? tp_bkappa_sum_n<float, TP_NBASE>(base + (size_t)D.icosmo * TP_NBASE, eh32[D.icosmo], l1, l2, l3)
                      : 0.0;
-    if (f == 1 || f == 2) list[atomicAdd(nlist, 1)] = (int)t;
+    // the listed points take one atomic per wave (one counter for the whole device: per-point atomics on it
+    // serialise); lanes that left above take no part in the ballot
+    const bool want = f == 1 || f == 2;
+    const unsigned long long mask = __ballot(want);
+    if (mask) {
+        const int lane = __lane_id(), leader = __ffsll((long long)mask) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(nlist, __popcll(mask));
+        base = __shfl(base, leader, 64);
+        if (want) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int)t;
+    }
 }
 
 // exact B_kappa of the listed points in the FP32 form (FP32 bispectrum, FP64 z-sum: the arithmetic of the
@@ -1513,24 +1523,29 @@ __global__ void __launch_bounds__(256) k_sur_exact(const int *list, const int *n
 __global__ void k_sur_combine(size_t n, const int *flag, const double *F, const double *logb, const double *Bb,
                               const double *Bex, double *fvals, double *resid, float *maxrel)
 {
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    int f = flag[t];
-    // baseline x learned correction; the network's output carries FP32 precision, so its exp is FP32
-    double bs = Bb[t] * (double)__expf((float)logb[t]);
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = t < n;  // every lane stays for the wave's reduction below
+    const int f = in ? flag[t] : 3;
     double v = 0, r = 0;
-    if (f == 0) {
+    float e = 0.f;
+    if (f <= 1) {
+        // baseline x learned correction; the network's output carries FP32 precision, so its exp is FP32
+        const double bs = Bb[t] * (double)__expf((float)logb[t]);
         v = F[t] * bs;
-    } else if (f == 1) {
-        v = F[t] * bs;
-        r = F[t] * (Bex[t] - bs);
-        float e = (float)(fabs(Bex[t] - bs) / fmax(fabs(Bex[t]), 1e-300));
-        atomicMax((int *)maxrel, __float_as_int(e));
+        if (f == 1) {
+            r = F[t] * (Bex[t] - bs);
+            e = (float)(fabs(Bex[t] - bs) / fmax(fabs(Bex[t]), 1e-300));
+        }
     } else if (f == 2) {
         v = F[t] * Bex[t];
     }
-    fvals[t] = v;
-    resid[t] = r;
+    if (in) {
+        fvals[t] = v;
+        resid[t] = r;
+    }
+    // the worst checked error: one atomic per wave
+    for (int o = 32; o >= 1; o >>= 1) e = fmaxf(e, __shfl_xor(e, o, 64));
+    if (__lane_id() == 0 && e > 0.f) atomicMax((int *)maxrel, __float_as_int(e));
 }
 
 __global__ void __launch_bounds__(256) k_sur_rule(const TpRegion *__restrict__ reg, size_t r0, size_t nr,
@@ -1538,32 +1553,41 @@ __global__ void __launch_bounds__(256) k_sur_rule(const TpRegion *__restrict__ r
                                                   const double *F, const double *Bex, double inv_frac,
                                                   TpRegionOut *out, double *stats)
 {
-    size_t rr = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (rr >= nr) return;
-    const double vol = reg[r0 + rr].vol;
-    TpRegionOut o;
-    gm_rule3(vol, fvals + rr * TP_NPTS_GM3, &o);
+    const size_t rr = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double num = 0, den = 0, sq = 0;
     int nchk = 0;
-    for (int p = 0; p < TP_NPTS_GM3; ++p) {
-        size_t t = rr * TP_NPTS_GM3 + p;
-        if (flag[t] == 1) {
-            double w = gm_weight3(p) * vol;
-            num += w * resid[t];
-            sq += (w * resid[t]) * (w * resid[t]);
-            den += fabs(w * F[t] * Bex[t]);
-            ++nchk;
+    if (rr < nr) {  // every lane stays for the wave's reduction below
+        const double vol = reg[r0 + rr].vol;
+        TpRegionOut o;
+        gm_rule3(vol, fvals + rr * TP_NPTS_GM3, &o);
+        for (int p = 0; p < TP_NPTS_GM3; ++p) {
+            size_t t = rr * TP_NPTS_GM3 + p;
+            if (flag[t] == 1) {
+                double w = gm_weight3(p) * vol;
+                num += w * resid[t];
+                sq += (w * resid[t]) * (w * resid[t]);
+                den += fabs(w * F[t] * Bex[t]);
+                ++nchk;
+            }
         }
+        // Horvitz-Thompson: corr = sum w r / pi, Var ~ (1 - pi) sum (w r / pi)^2
+        o.corr = num * inv_frac;
+        o.cvar = (1.0 - 1.0 / inv_frac) * sq * inv_frac * inv_frac;
+        out[r0 + rr] = o;
     }
-    // Horvitz-Thompson: corr = sum w r / pi, Var ~ (1 - pi) sum (w r / pi)^2
-    o.corr = num * inv_frac;
-    o.cvar = (1.0 - 1.0 / inv_frac) * sq * inv_frac * inv_frac;
-    out[r0 + rr] = o;
-    if (nchk) {
+    // the round's check statistics: one set of atomics per wave
+    double cnt = (double)nchk;
+    for (int w = 32; w >= 1; w >>= 1) {
+        num += __shfl_xor(num, w, 64);
+        den += __shfl_xor(den, w, 64);
+        sq += __shfl_xor(sq, w, 64);
+        cnt += __shfl_xor(cnt, w, 64);
+    }
+    if (__lane_id() == 0 && cnt > 0) {
         atomicAdd(&stats[0], num);
         atomicAdd(&stats[1], den);
         atomicAdd(&stats[2], sq);
-        atomicAdd(&stats[3], (double)nchk);
+        atomicAdd(&stats[3], cnt);
     }
 }
 
