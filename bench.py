@@ -13,8 +13,9 @@ Paths timed in the same run:
   plain_fp64    one triple after another through tp_map3_single: the reference's hcubature_v driving an
                 integrand callback that crosses to the GPU every cubature round
                 (cuda_version/apertureStatistics.cu:155-229); on the C3 workload one cosmology's 35 triples.
-  surrogate     batched; an integral's regions run the exact FP32 form until it holds 256 regions, then
-                the learned B_kappa (BF16 MFMA network) with a seeded 1% of points checked exactly.
+  surrogate     batched; triples whose largest radius is >= 16x their smallest run the exact FP32 form, the
+                others run it until an integral holds 1024 regions and then the learned B_kappa (FP16 MFMA
+                network over an 8-node FP32 baseline) with a seeded 1% of points checked exactly.
   batched_fp32  all integrals advance together; FP32 bispectrum, FP64 z-sum and cubature accumulation.
   batched_fp64  the same in FP64: the reference's arithmetic, the parity anchor of the other paths.
 The headline (value) is the surrogate form when its max relative error against batched FP64 (itself
@@ -33,23 +34,31 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "hbm": 8.0e12}
-# HBM bytes per integrand point of the region kernels: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE over the
-# C1 batched paths (the launch-shape tuner's probe launches excluded; tuned shapes: 8 waves per point for
-# small rounds, the region-cached kernel for large ones), each counter scaled by its reading for k_copy16
-# of 256 MiB (FETCH_SIZE x2.000, WRITE_SIZE x1.000).  Algorithmic: 64/33 B of region in + 8 B of
-# integrand value out.
-HBM_BYTES_PER_POINT = {"double": 13.29, "float": 12.12}
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16": 2.5e15, "fp16": 2.5e15, "hbm": 8.0e12}
+# HBM bytes per integrand point of the region-cached kernels on the C3 step: rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE over benchMap3.x (profiles/map3_c3_counters.md), each counter scaled by its reading for
+# k_copy16 copying 1 GiB (FETCH_SIZE x2.000, WRITE_SIZE x1.000).  Algorithmic: 64/33 B of region in +
+# 8 B of integrand value out.
+HBM_BYTES_PER_POINT = {"double": 12.92, "float": 11.66}
 # The surrogate network 8 -> 4 x 256 -> 1: algorithmic flops per point 2 (8 x 256 + 3 x 256 x 256 + 256); the
-# inference kernel executes the three 256 x 256 layers on hi + lo BF16 activations (two MFMAs per weight
-# fragment) and layer 1 on the FP32 MFMA.
+# inference kernel k_mlp_h executes the three 256 x 256 layers as one FP16 MFMA per weight fragment and
+# layer 1 on the FP32 MFMA, i.e. the algorithmic MFMA work.
 MLP_FLOPS_PER_POINT = 2 * (8 * 256 + 3 * 256 * 256 + 256)
-MLP_MFMA_FLOPS_EXECUTED = 2 * (8 * 256 + 2 * 3 * 256 * 256)
-# HBM bytes per point of k_mlp_reg and its counters per point (rocprofv3 --pmc over the C3 surrogate step,
-# profiles/mlp_c3_counters.md); None until measured
-MLP_HBM_BYTES_PER_POINT = 40.0
-MLP_COUNTERS = None
-REGION_COUNTERS = {}
+MLP_MFMA_FLOPS_EXECUTED = 2 * (8 * 256 + 3 * 256 * 256)
+# Counter readings of the C3 step (profiles/map3_c3_counters.md): MFMA flops = SQ_INSTS_VALU_MFMA_MOPS_* x 512,
+# VALU flops = SQ_INSTS_VALU_FLOPS_* x 64, MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / 8 XCDs
+# x 256 CUs x 4 SIMDs), each rate over the kernel's summed dispatch time in the counter run.
+MLP_HBM_BYTES_PER_POINT = 43.6
+MLP_COUNTERS = {"kernel": "k_mlp_h", "mfma_tflops": 769.3, "mfma_frac_of_fp16_peak": 0.308, "mfma_busy": 0.41,
+                "valu_fp32_tflops": 9.0, "hbm_read_bytes_per_point": 35.6, "hbm_write_bytes_per_point": 8.0}
+REGION_COUNTERS = {
+    "float": {"kernel": "k_region_cached<float, 1>", "valu_fp32_tflops": 36.2, "frac_of_fp32_peak": 0.230,
+              "frac_of_scalar_fp32": 0.460, "valu_fp32_flops_per_point": 22240, "trans_fp32_per_point": 2720,
+              "hbm_read_bytes_per_point": 3.59, "hbm_write_bytes_per_point": 8.08},
+    "double": {"kernel": "k_region_cached<double, 1>", "valu_fp64_tflops": 39.5, "frac_of_fp64_peak": 0.502,
+               "valu_fp64_flops_per_point": 93400, "hbm_read_bytes_per_point": 4.85,
+               "hbm_write_bytes_per_point": 8.08},
+}
 FLOPS_PER_ELEM = 20.0
 CENSUS_FILE = os.path.join("tests", "golden", "census.json")
 
@@ -657,7 +666,7 @@ def main():
                                      "device_busy_frac": fe_point * exact_pts / (r["integrand_ms"] * 1e-3) / peak}
         if r.get("mlp_ms", 0) > 0:
             ach = MLP_FLOPS_PER_POINT * r["mlp_points"] / (r["mlp_ms"] * 1e-3)
-            r["mlp_kernel"] = {"achieved_flops": ach, "peak": PEAK["bf16"], "frac": ach / PEAK["bf16"],
+            r["mlp_kernel"] = {"achieved_flops": ach, "peak": PEAK["fp16"], "frac": ach / PEAK["fp16"],
                                "points": r["mlp_points"], "launches": r["mlp_launches"],
                                "avg_launch_ms": r["mlp_ms"] / max(1, r["mlp_launches"])}
     ok = [p for p in results if results[p].get("max_rel_err_vs_fp64", 1.0) <= 1e-3]
@@ -670,8 +679,8 @@ def main():
     use_mlp = "mlp_kernel" in hr and ("integrand_kernel" not in hr or hr["mlp_ms"] >= hr["integrand_sum_ms"])
     if use_mlp:
         mk_ = hr["mlp_kernel"]
-        roof = {"bound": "mfma", "kernel": "k_mlp_reg (BF16 MFMA, hi/lo activations, FP32 accumulation)",
-                "achieved": mk_["achieved_flops"] / 1e12, "peak": PEAK["bf16"] / 1e12, "unit": "TFLOP/s",
+        roof = {"bound": "mfma", "kernel": "k_mlp_h (FP16 MFMA, FP16 activations, FP32 accumulation)",
+                "achieved": mk_["achieved_flops"] / 1e12, "peak": PEAK["fp16"] / 1e12, "unit": "TFLOP/s",
                 "frac": mk_["frac"],
                 "traffic": MLP_HBM_BYTES_PER_POINT * mk_["points"] / max(1, mk_["launches"]),
                 "traffic_unit": "HBM bytes per launch (per-point bytes from the PMC passes x points per launch)",
@@ -703,12 +712,18 @@ def main():
                 "executed_work_per_point_flop_eq_region_cached": fe_exec_cached,
                 "executed_frac_if_all_region_cached": dom["frac"] * fe_exec_cached / fe_point,
                 "census": census, "census_file": CENSUS_FILE, "counters": REGION_COUNTERS.get(prec)}
+        if "mlp_kernel" in hr:  # the surrogate form's other kernel: the network
+            mk_ = hr["mlp_kernel"]
+            roof["secondary"] = {"kernel": "k_mlp_h (FP16 MFMA)", "bound": "mfma",
+                                 "achieved": mk_["achieved_flops"] / 1e12, "peak": PEAK["fp16"] / 1e12,
+                                 "frac": mk_["frac"], "unit": "TFLOP/s", "mlp_ms": hr["mlp_ms"],
+                                 "avg_launch_ms": mk_["avg_launch_ms"], "counters": MLP_COUNTERS}
     line = {
         "metric": "Map3 triples/s on 1 GPU: plain FP64 vs surrogate form at <=1e-3 rel err",
         "value": hr["triples_per_s"], "unit": "triples/s", "n_gpus": 1,
         "steps": hr["steps"], "warmup": hr["warmup"], "ms_per_step": hr["ms_per_step"],
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": {"batched_fp32": "f32", "surrogate": "bf16 network + f32 exact regions, f64 accumulation"}.get(
+        "dtype": {"batched_fp32": "f32", "surrogate": "fp16 network + f32 exact regions, f64 accumulation"}.get(
             head, "f64"),
         "data": "synthetic",
         "config": {"workload": wl, "path": head, "triples_per_step": int(thetas.shape[0]),

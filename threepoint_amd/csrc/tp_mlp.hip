@@ -394,7 +394,7 @@ int grid(size_t n, int b) { return (int)((n + b - 1) / b); }
 // Buffers of one group's surrogate rounds (each group of tp_map3_batch has its own, so the groups'
 // rounds can be in flight together on their streams).
 struct TpSurRoundBufs {
-    DevBuf X, F, Bb, flag, logb, Bex, resid, list, misc;
+    DevBuf X, F, Bb, flag, logb, Bex, resid, list, seg, rpart, misc;
     HostPinned stats;  // the round's check statistics, copied back at the end of the round
     // events around the round's network launches (one pair per chunk) and those launches' points
     std::vector<hipEvent_t> ev;
@@ -402,7 +402,7 @@ struct TpSurRoundBufs {
     int nev = 0;
     void release()
     {
-        for (DevBuf *b : {&X, &F, &Bb, &flag, &logb, &Bex, &resid, &list, &misc}) b->release();
+        for (DevBuf *b : {&X, &F, &Bb, &flag, &logb, &Bex, &resid, &list, &seg, &rpart, &misc}) b->release();
         stats.release();
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         ev.clear();
@@ -1413,15 +1413,29 @@ __device__ __forceinline__ double gm_weight3(int p)
     return nz == 1 ? 620.0 / 19683.0 : 200. / 19683.;  // lambda4 axis points / (lambda4, lambda4) pairs
 }
 
-__device__ __forceinline__ bool in_box(const SurBox &B, double l1, double l2, double l3, const double cp[5])
+// The network's input row of a cubature point and its box test (every side's log inside the training range,
+// every cosmology parameter inside its box) from one set of FP32 logarithms of the sorted sides; make_input
+// (training data, tp_surrogate_predict) takes FP64 logs: the rows agree to ~2e-7, far inside the network's
+// own precision.  l3 > 0.
+__device__ __forceinline__ bool make_input_in_box(const SurBox &B, double l1, double l2, double l3, const double cp[5],
+                                                  float *x)
 {
-    double lmin = log(fmin(l1, fmin(l2, l3))), lmax = log(fmax(l1, fmax(l2, l3)));
-    if (!(lmin >= B.lnl_min && lmax <= B.lnl_max)) return false;
+    double a = l1, b = l2, c = l3, t;
+    if (a < b) { t = a; a = b; b = t; }
+    if (b < c) { t = b; b = c; c = t; }
+    if (a < b) { t = a; a = b; b = t; }
+    const float ga = logf((float)a), gb = logf((float)b), gc = logf((float)c);
+    const float mid = (float)(0.5 * (B.lnl_min + B.lnl_max)), ihalf = (float)(2.0 / (B.lnl_max - B.lnl_min));
+    x[0] = (ga - mid) * ihalf;
+    x[1] = (gb - mid) * ihalf;
+    x[2] = (gc - mid) * ihalf;
+    bool in = gc >= (float)B.lnl_min && ga <= (float)B.lnl_max;
     for (int k = 0; k < 5; ++k) {
-        double tol = 1e-9 * (B.hi[k] - B.lo[k]);
-        if (cp[k] < B.lo[k] - tol || cp[k] > B.hi[k] + tol) return false;
+        x[3 + k] = (float)(2.0 * (cp[k] - B.lo[k]) / (B.hi[k] - B.lo[k]) - 1.0);
+        const double tol = 1e-9 * (B.hi[k] - B.lo[k]);
+        in = in && cp[k] >= B.lo[k] - tol && cp[k] <= B.hi[k] + tol;
     }
-    return true;
+    return in;
 }
 
 // flag: 0 surrogate, 1 surrogate + exact check, 2 exact only (outside the training box), 3 zero (an l = 0)
@@ -1430,66 +1444,66 @@ __global__ void __launch_bounds__(256) k_sur_points(const TpRegion *__restrict__
                                                     const TpCosmoConst *__restrict__ cc, const TpNode32 *__restrict__ base,
                                                     const TpEH32 *__restrict__ eh32, SurBox box, double frac,
                                                     unsigned long long key, float *X, double *F, double *Bb, int *flag,
-                                                    int *list, int *nlist, int *err)
+                                                    int *list, int *seg, int *err)
 {
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nr * TP_NPTS_GM3) return;
-    size_t r = r0 + t / TP_NPTS_GM3;
-    int p = (int)(t % TP_NPTS_GM3);
-    const TpRegion R = reg[r];
-    const TpIntegralDesc D = desc[R.integral];
-    double l1, l2, phi;
-    gm_point3(R, p, &l1, &l2, &phi);
-    double l3 = tp_ell3(l1, l2, phi);
-    if (!isfinite(l1) || !isfinite(l2) || !isfinite(l3)) atomicOr(err, 1);
-    // l1 l2 u(l1 t1) u(l2 t2) u(l3 t3), u(x) = x^2/2 exp(-x^2/2), with one exponential for the three filters
-    const double a = l1 * D.th[0], b = l2 * D.th[1], e = l3 * D.th[2];
-    const double a2 = a * a, b2 = b * b, e2 = e * e;
-    F[t] = l1 * l2 * (0.125 * a2 * b2 * e2) * exp(-0.5 * (a2 + b2 + e2));
-    const TpCosmoConst &C = cc[D.icosmo];
-    double cp[5] = {C.om, C.sigma8, C.h, C.w, C.ns};
-    make_input(box, l1, l2, l3 > 0 ? l3 : 1.0, cp, X + t * NIN);
-    int f;
-    if (l1 == 0 || l2 == 0 || l3 == 0)
-        f = 3;
-    else if (!in_box(box, l1, l2, l3, cp))
-        f = 2;
-    else
-        f = (u01(key, r * TP_NPTS_GM3 + p) < frac) ? 1 : 0;
-    flag[t] = f;
-    Bb[t] = (f <= 1) ? tp_bkappa_sum_n<float, TP_NBASE>(base + (size_t)D.icosmo * TP_NBASE, eh32[D.icosmo], l1, l2, l3)
-                     : 0.0;
-    // the listed points take one atomic per wave (one counter for the whole device: per-point atomics on it
-    // serialise); lanes that left above take no part in the ballot
-    const bool want = f == 1 || f == 2;
-    const unsigned long long mask = __ballot(want);
-    if (mask) {
-        const int lane = __lane_id(), leader = __ffsll((long long)mask) - 1;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(nlist, __popcll(mask));
-        base = __shfl(base, leader, 64);
-        if (want) list[base + __popcll(mask & ((1ull << lane) - 1ull))] = (int)t;
+    __shared__ int s_cnt;
+    if (threadIdx.x == 0) s_cnt = 0;
+    __syncthreads();
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int f = 3;
+    if (t < nr * TP_NPTS_GM3) {  // every thread reaches the barriers below
+        size_t r = r0 + t / TP_NPTS_GM3;
+        int p = (int)(t % TP_NPTS_GM3);
+        const TpRegion R = reg[r];
+        const TpIntegralDesc D = desc[R.integral];
+        double l1, l2, phi;
+        gm_point3(R, p, &l1, &l2, &phi);
+        double l3 = tp_ell3(l1, l2, phi);
+        if (!isfinite(l1) || !isfinite(l2) || !isfinite(l3)) atomicOr(err, 1);
+        // l1 l2 u(l1 t1) u(l2 t2) u(l3 t3), u(x) = x^2/2 exp(-x^2/2), with one exponential for the three filters
+        const double a = l1 * D.th[0], b = l2 * D.th[1], e = l3 * D.th[2];
+        const double a2 = a * a, b2 = b * b, e2 = e * e;
+        F[t] = l1 * l2 * (0.125 * a2 * b2 * e2) * exp(-0.5 * (a2 + b2 + e2));
+        const TpCosmoConst &C = cc[D.icosmo];
+        double cp[5] = {C.om, C.sigma8, C.h, C.w, C.ns};
+        const bool inb = make_input_in_box(box, l1, l2, l3 > 0 ? l3 : 1.0, cp, X + t * NIN);
+        if (l1 == 0 || l2 == 0 || l3 == 0)
+            f = 3;
+        else if (!inb)
+            f = 2;
+        else
+            f = (u01(key, r * TP_NPTS_GM3 + p) < frac) ? 1 : 0;
+        flag[t] = f;
+        Bb[t] = (f <= 1) ? tp_bkappa_sum_n<float, TP_NBASE>(base + (size_t)D.icosmo * TP_NBASE, eh32[D.icosmo], l1, l2,
+                                                            l3)
+                         : 0.0;
     }
+    // the points to evaluate exactly (checked or outside the box) go to this block's own segment of the
+    // list: an LDS counter, no device-wide atomic (atomics on one address serialise)
+    const bool want = f == 1 || f == 2;
+    int slot = 0;
+    if (want) slot = atomicAdd(&s_cnt, 1);
+    __syncthreads();
+    if (want) list[(size_t)blockIdx.x * blockDim.x + slot] = (int)t;
+    if (threadIdx.x == 0) seg[blockIdx.x] = s_cnt;
 }
 
 // exact B_kappa of the listed points in the FP32 form (FP32 bispectrum, FP64 z-sum: the arithmetic of the
-// exact regions of a surrogate round, so the checks correct the network toward the same values): 16 lanes
-// per point, 6 of the 96 z-nodes each, so a round's few thousand checked points still fill the device.  The
-// list's length is known on the device only: the grid is sized for the expected checks (kSurExactBlocks
-// blocks of 16 points) and strides over the list.
+// exact regions of a surrogate round, so the checks correct the network toward the same values): one
+// workgroup per list segment (one block of k_sur_points: typically 1% of its 256 points plus the out-of-box
+// ones, which cluster, so a whole workgroup shares a full segment), 16 lanes per point, 6 of the 96 z-nodes
+// each.
 constexpr int XL = 16;
-constexpr int kSurExactBlocks = 1024;
-__global__ void __launch_bounds__(256) k_sur_exact(const int *list, const int *nlist, const TpRegion *__restrict__ reg,
+__global__ void __launch_bounds__(256) k_sur_exact(const int *list, const int *seg, const TpRegion *__restrict__ reg,
                                                    size_t r0, const TpIntegralDesc *__restrict__ desc,
                                                    const TpNode32 *__restrict__ nodes, const TpEH32 *__restrict__ eh,
                                                    double *Bex, int *err)
 {
-    const int n = *nlist;
+    const int n = seg[blockIdx.x];
     const int sub = threadIdx.x % XL;
     // the XL lanes of a group share i, so a group runs the loop (and the shuffle below) with all its lanes
-    const int stride = gridDim.x * (blockDim.x / XL);
-    for (int i = (blockIdx.x * blockDim.x + threadIdx.x) / XL; i < n; i += stride) {
-        const int t = list[i];
+    for (int i = threadIdx.x / XL; i < n; i += blockDim.x / XL) {
+        const int t = list[(size_t)blockIdx.x * blockDim.x + i];
         const size_t r = r0 + t / TP_NPTS_GM3;
         const int p = t % TP_NPTS_GM3;
         const TpRegion R = reg[r];
@@ -1520,39 +1534,46 @@ __global__ void __launch_bounds__(256) k_sur_exact(const int *list, const int *n
     }
 }
 
-__global__ void k_sur_combine(size_t n, const int *flag, const double *F, const double *logb, const double *Bb,
+// f = F B_8 e^net (checked points also give the residual F (B_exact - B_surr)); every load is unconditional
+// (Bex is read for all points and used only where the point was evaluated exactly), so a wave waits for
+// memory once.  Bb is overwritten with |F B_exact| of the checked points (0 elsewhere) for k_sur_rule.
+__global__ void k_sur_combine(size_t n, const int *flag, const double *F, const double *logb, double *Bb,
                               const double *Bex, double *fvals, double *resid, float *maxrel)
 {
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool in = t < n;  // every lane stays for the wave's reduction below
-    const int f = in ? flag[t] : 3;
-    double v = 0, r = 0;
+    const size_t u = in ? t : 0;
+    const int f = in ? flag[u] : 3;
+    const double Fv = F[u], lb = logb[u], bb = Bb[u], bx = Bex[u];
+    // baseline x learned correction; the network's output carries FP32 precision, so its exp is FP32
+    const double bs = bb * (double)__expf((float)lb);
+    double v = 0, r = 0, a = 0;
     float e = 0.f;
-    if (f <= 1) {
-        // baseline x learned correction; the network's output carries FP32 precision, so its exp is FP32
-        const double bs = Bb[t] * (double)__expf((float)logb[t]);
-        v = F[t] * bs;
-        if (f == 1) {
-            r = F[t] * (Bex[t] - bs);
-            e = (float)(fabs(Bex[t] - bs) / fmax(fabs(Bex[t]), 1e-300));
-        }
-    } else if (f == 2) {
-        v = F[t] * Bex[t];
+    if (f <= 1) v = Fv * bs;
+    if (f == 1) {
+        r = Fv * (bx - bs);
+        a = fabs(Fv * bx);
+        e = (float)(fabs(bx - bs) / fmax(fabs(bx), 1e-300));
     }
+    if (f == 2) v = Fv * bx;
     if (in) {
         fvals[t] = v;
         resid[t] = r;
+        Bb[t] = a;
     }
-    // the worst checked error: one atomic per wave
+    // the worst checked error: a wave that cannot raise the running maximum takes no atomic (atomics on one
+    // address serialise; the maximum is reached early and rarely moves)
     for (int o = 32; o >= 1; o >>= 1) e = fmaxf(e, __shfl_xor(e, o, 64));
-    if (__lane_id() == 0 && e > 0.f) atomicMax((int *)maxrel, __float_as_int(e));
+    if (__lane_id() == 0 && e > __int_as_float(__atomic_load_n((const int *)maxrel, __ATOMIC_RELAXED)))
+        atomicMax((int *)maxrel, __float_as_int(e));
 }
 
 __global__ void __launch_bounds__(256) k_sur_rule(const TpRegion *__restrict__ reg, size_t r0, size_t nr,
                                                   const double *fvals, const double *resid, const int *flag,
-                                                  const double *F, const double *Bex, double inv_frac,
-                                                  TpRegionOut *out, double *stats)
+                                                  const double *absfb, double inv_frac,
+                                                  TpRegionOut *out, double *part)
 {
+    __shared__ double red[4][4];
     const size_t rr = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     double num = 0, den = 0, sq = 0;
     int nchk = 0;
@@ -1560,22 +1581,22 @@ __global__ void __launch_bounds__(256) k_sur_rule(const TpRegion *__restrict__ r
         const double vol = reg[r0 + rr].vol;
         TpRegionOut o;
         gm_rule3(vol, fvals + rr * TP_NPTS_GM3, &o);
+        // resid and absfb (|F B_exact|) are 0 on unchecked points: unconditional loads, no dependent chain
         for (int p = 0; p < TP_NPTS_GM3; ++p) {
-            size_t t = rr * TP_NPTS_GM3 + p;
-            if (flag[t] == 1) {
-                double w = gm_weight3(p) * vol;
-                num += w * resid[t];
-                sq += (w * resid[t]) * (w * resid[t]);
-                den += fabs(w * F[t] * Bex[t]);
-                ++nchk;
-            }
+            const size_t t = rr * TP_NPTS_GM3 + p;
+            const double w = gm_weight3(p) * vol, x = w * resid[t];
+            num += x;
+            sq += x * x;
+            den += fabs(w) * absfb[t];
+            nchk += flag[t] == 1;
         }
         // Horvitz-Thompson: corr = sum w r / pi, Var ~ (1 - pi) sum (w r / pi)^2
         o.corr = num * inv_frac;
         o.cvar = (1.0 - 1.0 / inv_frac) * sq * inv_frac * inv_frac;
         out[r0 + rr] = o;
     }
-    // the round's check statistics: one set of atomics per wave
+    // the round's check statistics: the block's sums into its own slot of part (k_sur_count adds the slots;
+    // atomics on one address serialise)
     double cnt = (double)nchk;
     for (int w = 32; w >= 1; w >>= 1) {
         num += __shfl_xor(num, w, 64);
@@ -1583,23 +1604,44 @@ __global__ void __launch_bounds__(256) k_sur_rule(const TpRegion *__restrict__ r
         sq += __shfl_xor(sq, w, 64);
         cnt += __shfl_xor(cnt, w, 64);
     }
-    if (__lane_id() == 0 && cnt > 0) {
-        atomicAdd(&stats[0], num);
-        atomicAdd(&stats[1], den);
-        atomicAdd(&stats[2], sq);
-        atomicAdd(&stats[3], cnt);
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wv][0] = num;
+        red[wv][1] = den;
+        red[wv][2] = sq;
+        red[wv][3] = cnt;
     }
+    __syncthreads();
+    if (threadIdx.x < 4)
+        part[(size_t)blockIdx.x * 4 + threadIdx.x] =
+            red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
 }
 
 }  // namespace
 
 
 // layout of misc / stats: [0] sum w r, [1] sum |w F B|, [2] sum (w r)^2, [3] checks, [4] exactly evaluated
-// points (as a double), then the worst checked relative error (float) and the chunk's list length (int)
+// points (as a double), then the worst checked relative error (float)
 constexpr int kSurStatsD = 5;
 
 namespace {
-__global__ void k_sur_count(const int *n, double *total) { *total += (double)*n; }
+// a chunk's statistics into the round's: the exactly evaluated points (the sum of the list segments) and the
+// rule blocks' check sums
+__global__ void __launch_bounds__(256) k_sur_count(const int *seg, int nseg, const double *part, int nrb, double *stats)
+{
+    __shared__ double red[4][5];
+    double c[5] = {0, 0, 0, 0, 0};
+    for (int i = threadIdx.x; i < nseg; i += 256) c[4] += (double)seg[i];
+    for (int i = threadIdx.x; i < nrb; i += 256)
+        for (int k = 0; k < 4; ++k) c[k] += part[(size_t)i * 4 + k];
+    for (int k = 0; k < 5; ++k)
+        for (int o = 32; o >= 1; o >>= 1) c[k] += __shfl_xor(c[k], o, 64);
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 5; ++k) red[threadIdx.x >> 6][k] = c[k];
+    __syncthreads();
+    if (threadIdx.x < 5) stats[threadIdx.x] += red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] +
+                                              red[3][threadIdx.x];
+}
 }  // namespace
 
 int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, const TpRegion *d_reg,
@@ -1612,17 +1654,18 @@ int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, co
     if (nreg == 0) return 0;
     const size_t RC = 32768;  // regions per chunk (1 081 344 points): bounds the per-point buffers
     const size_t PC = RC * TP_NPTS_GM3, PCP = (PC + 63) / 64 * 64;
+    const size_t NSEG = (PC + 255) / 256;  // k_sur_points blocks per chunk: one list segment of 256 each
     int rc;
     TpSurRoundBufs &B = s->rb[lane];
     if ((rc = B.X.ensure(PCP * NIN * sizeof(float))) || (rc = B.F.ensure(PC * sizeof(double))) ||
         (rc = B.Bb.ensure(PC * sizeof(double))) || (rc = B.flag.ensure(PC * sizeof(int))) ||
         (rc = B.logb.ensure(PCP * sizeof(double))) || (rc = B.Bex.ensure(PC * sizeof(double))) ||
-        (rc = B.resid.ensure(PC * sizeof(double))) || (rc = B.list.ensure(PC * sizeof(int))) ||
+        (rc = B.resid.ensure(PC * sizeof(double))) || (rc = B.list.ensure(NSEG * 256 * sizeof(int))) ||
+        (rc = B.seg.ensure(NSEG * sizeof(int))) || (rc = B.rpart.ensure((RC + 255) / 256 * 4 * sizeof(double))) ||
         (rc = B.misc.ensure(64)) || (rc = B.stats.ensure(64)))
         return rc;
     double *dstats = (double *)B.misc.p;
     float *dmaxrel = (float *)(dstats + kSurStatsD);
-    int *dn = (int *)(dmaxrel + 1);
     TP_HIP(hipMemsetAsync(B.misc.p, 0, 64, st));
     const double frac = ctx->sur_check_frac;
     const unsigned long long key = ctx->sur_seed * 0x9E3779B97F4A7C15ULL + round_key * 7919ULL;
@@ -1631,10 +1674,10 @@ int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, co
     B.ev_points.clear();
     for (size_t r0 = 0; r0 < nreg; r0 += RC) {
         size_t nr = std::min(RC, nreg - r0), np = nr * TP_NPTS_GM3, npp = (np + 63) / 64 * 64;
-        TP_HIP(hipMemsetAsync(dn, 0, sizeof(int), st));
-        hipLaunchKernelGGL(k_sur_points, dim3(grid(np, 256)), dim3(256), 0, st, d_reg, r0, nr, d_desc, c->d_const,
+        const int nseg = grid(np, 256);
+        hipLaunchKernelGGL(k_sur_points, dim3(nseg), dim3(256), 0, st, d_reg, r0, nr, d_desc, c->d_const,
                            c->d_base32, c->d_eh32, s->box, frac, key, (float *)B.X.p, (double *)B.F.p,
-                           (double *)B.Bb.p, (int *)B.flag.p, (int *)B.list.p, dn, d_flag);
+                           (double *)B.Bb.p, (int *)B.flag.p, (int *)B.list.p, (int *)B.seg.p, d_flag);
         if (npp > np) TP_HIP(hipMemsetAsync((float *)B.X.p + np * NIN, 0, (npp - np) * NIN * sizeof(float), st));
         while ((int)B.ev.size() < 2 * (B.nev + 1)) {
             hipEvent_t e;
@@ -1646,16 +1689,17 @@ int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, co
         TP_HIP(hipEventRecord(B.ev[2 * B.nev + 1], st));
         B.ev_points.push_back((long long)npp);
         B.nev += 1;
-        hipLaunchKernelGGL(k_sur_exact, dim3(kSurExactBlocks), dim3(256), 0, st, (const int *)B.list.p, dn, d_reg, r0,
-                           d_desc, c->d_node32, c->d_eh32, (double *)B.Bex.p, d_flag);
+        hipLaunchKernelGGL(k_sur_exact, dim3(nseg), dim3(256), 0, st, (const int *)B.list.p, (const int *)B.seg.p,
+                           d_reg, r0, d_desc, c->d_node32, c->d_eh32, (double *)B.Bex.p, d_flag);
         hipLaunchKernelGGL(k_sur_combine, dim3(grid(np, 256)), dim3(256), 0, st, np, (const int *)B.flag.p,
-                           (const double *)B.F.p, (const double *)B.logb.p, (const double *)B.Bb.p,
+                           (const double *)B.F.p, (const double *)B.logb.p, (double *)B.Bb.p,
                            (const double *)B.Bex.p, d_fvals + r0 * TP_NPTS_GM3, (double *)B.resid.p, dmaxrel);
-        hipLaunchKernelGGL(k_sur_rule, dim3(grid(nr, 256)), dim3(256), 0, st, d_reg, r0, nr,
+        const int nrb = grid(nr, 256);
+        hipLaunchKernelGGL(k_sur_rule, dim3(nrb), dim3(256), 0, st, d_reg, r0, nr,
                            (const double *)(d_fvals + r0 * TP_NPTS_GM3), (const double *)B.resid.p,
-                           (const int *)B.flag.p, (const double *)B.F.p, (const double *)B.Bex.p, 1.0 / frac, d_out,
-                           dstats);
-        hipLaunchKernelGGL(k_sur_count, dim3(1), dim3(1), 0, st, (const int *)dn, dstats + 4);
+                           (const int *)B.flag.p, (const double *)B.Bb.p, 1.0 / frac, d_out, (double *)B.rpart.p);
+        hipLaunchKernelGGL(k_sur_count, dim3(1), dim3(256), 0, st, (const int *)B.seg.p, nseg,
+                           (const double *)B.rpart.p, nrb, dstats);
         TP_HIP(hipGetLastError());
     }
     TP_HIP(hipMemcpyAsync(B.stats.p, B.misc.p, 64, hipMemcpyDeviceToHost, st));
