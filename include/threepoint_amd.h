@@ -183,6 +183,12 @@ int tp_round_log(const tp_ctx *ctx, double *out, size_t cap, size_t *n);
  * on it.  No counterpart in the reference. */
 int tp_set_zero_copy_regions(tp_ctx *ctx, long long max_regions);
 
+/* tp_map3_batch runs the rounds of its two groups of integrals one after the other on the device when
+ * both hold at least min_regions regions (each then fills the device; the host absorbs one round while
+ * the other's kernels run), side by side otherwise.  0 restores the default (65536).  Results do not
+ * depend on it.  No counterpart in the reference. */
+int tp_set_serial_regions(tp_ctx *ctx, long long min_regions);
+
 /* Guards of every batched cubature on this context (tp_map3_batch, tp_gamma_batch, tp_map2_batch,
  * tp_pkappa_batch and the covariance batches): a call whose integrals need more than max_rounds
  * rounds, or evaluate more than max_regions regions in total, ends with TP_ECAPACITY instead of

@@ -128,6 +128,7 @@ _SIGS = {
     "tp_set_graph_rounds": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "tp_set_batch_groups": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "tp_set_zero_copy_regions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
+    "tp_set_serial_regions": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_longlong]),
     "tp_set_round_log": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "tp_set_host_threads": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "tp_set_keep_pool_bytes": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_ulonglong]),

@@ -162,6 +162,9 @@ struct TpLane {
 constexpr int TP_NLANES = 4;  // groups of a batched call (at most; tp_ctx::nlanes are used)
 
 constexpr size_t kTpKeepPoolBytes = size_t(16) << 30;  // default cap on the host region pools kept
+// tp_map3_batch: two in-flight rounds of at least this many regions each run one after the other on the
+// device (tp_set_serial_regions)
+constexpr long long kTpSerialRegions = 65536;
 
 // Guards of the batched cubatures (tp_set_batch_limits): a call whose integrals need more rounds or
 // evaluate more regions than these ends with TP_ECAPACITY instead of growing without bound (the
@@ -201,6 +204,7 @@ struct tp_ctx {
     bool graph_rounds = false;  // batched rounds replayed as one graph per lane (tp_set_graph_rounds)
     int nlanes = 2;             // groups in flight in tp_map3_batch (tp_set_batch_groups)
     long long zero_copy_regions = 2048;  // rounds up to this many regions skip the copies (pinned buffers)
+    long long serial_regions = kTpSerialRegions;  // tp_set_serial_regions
     int host_threads = 0;             // OpenMP threads of the host-side cubature work (0: default)
     // tp_map3_batch's integrators of the last call, reused by the next (their region pools keep
     // their capacity, so a call of the same size allocates and page-faults nothing); dropped when

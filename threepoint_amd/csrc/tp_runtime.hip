@@ -434,6 +434,11 @@ extern "C" int tp_map3_single(tp_ctx *ctx, const tp_cosmo *c, int icosmo, const 
 // (tp_set_batch_limits) end a run that needs more with TP_ECAPACITY.
 // adaptive grouping of tp_map3_batch (two groups): split above, merge below these round sizes
 static const size_t kSplitRegions = 8192, kMergeRegions = 2048;
+// Rounds of at least tp_ctx::serial_regions regions fill the device on their own: such a round's kernels
+// wait for the other group's in-flight round's kernels (an event wait on the device), so the two groups'
+// rounds run one after the other instead of side by side.  Side by side they end together and the device
+// idles while the host absorbs the first of them; one after the other, each round's host work overlaps the
+// other's kernels.
 static const size_t kSurrogateMaxEval = 4000000;  // points per triple before an exact redo
 int tp_region_lanes(tp_ctx *ctx, const tp_cosmo *c, int mode, size_t np);  // tp_map3.hip
 // tp_mlp.hip: a surrogate round queued on a group's stream, and its check statistics once it ran
@@ -589,6 +594,12 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
         const bool zc = mode != TP_MODE_SURROGATE && nreg <= (size_t)ctx->zero_copy_regions;
         if (!zc)
             TP_HIP(hipMemcpyAsync(L.d_regions.p, hr, sizeof(TpRegion) * nreg, hipMemcpyHostToDevice, gst[g]));
+        // a large round's kernels queue behind the other group's in-flight large round (its copy in still
+        // overlaps that round); ev1 marks the end of that round's kernels
+        for (int o = 0; o < nl; ++o)
+            if (o != g && G[o].inflight && G[o].nreg >= (size_t)ctx->serial_regions &&
+                nreg >= (size_t)ctx->serial_regions)
+                TP_HIP(hipStreamWaitEvent(gst[g], ctx->lane[o].ev1, 0));
         TP_HIP(hipEventRecord(L.ev0, gst[g]));
         if (mode == TP_MODE_SURROGATE) {
             const TpRegion *dr = (const TpRegion *)L.d_regions.p;
@@ -916,6 +927,13 @@ extern "C" int tp_round_log(const tp_ctx *ctx, double *out, size_t cap, size_t *
     *n = ctx->round_log.size();
     if (cap && !ctx->round_log.empty())
         std::memcpy(out, ctx->round_log.data(), sizeof(double) * std::min(cap, ctx->round_log.size()));
+    return 0;
+}
+
+extern "C" int tp_set_serial_regions(tp_ctx *ctx, long long min_regions)
+{
+    if (!ctx || min_regions < 0) return tp_set_error(TP_EINVAL, "tp_set_serial_regions: bad arguments");
+    ctx->serial_regions = min_regions ? min_regions : kTpSerialRegions;
     return 0;
 }
 
