@@ -394,7 +394,7 @@ int grid(size_t n, int b) { return (int)((n + b - 1) / b); }
 // Buffers of one group's surrogate rounds (each group of tp_map3_batch has its own, so the groups'
 // rounds can be in flight together on their streams).
 struct TpSurRoundBufs {
-    DevBuf X, F, Bb, flag, logb, Bex, resid, list, seg, rpart, misc;
+    DevBuf X, F, Bb, flag, logb, Bex, list, seg, rpart, misc;
     HostPinned stats;  // the round's check statistics, copied back at the end of the round
     // events around the round's network launches (one pair per chunk) and those launches' points
     std::vector<hipEvent_t> ev;
@@ -402,7 +402,7 @@ struct TpSurRoundBufs {
     int nev = 0;
     void release()
     {
-        for (DevBuf *b : {&X, &F, &Bb, &flag, &logb, &Bex, &resid, &list, &seg, &rpart, &misc}) b->release();
+        for (DevBuf *b : {&X, &F, &Bb, &flag, &logb, &Bex, &list, &seg, &rpart, &misc}) b->release();
         stats.release();
         for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         ev.clear();
@@ -1534,87 +1534,78 @@ __global__ void __launch_bounds__(256) k_sur_exact(const int *list, const int *s
     }
 }
 
-// f = F B_8 e^net (checked points also give the residual F (B_exact - B_surr)); every load is unconditional
-// (Bex is read for all points and used only where the point was evaluated exactly), so a wave waits for
-// memory once.  Bb is overwritten with |F B_exact| of the checked points (0 elsewhere) for k_sur_rule.
-__global__ void k_sur_combine(size_t n, const int *flag, const double *F, const double *logb, double *Bb,
-                              const double *Bex, double *fvals, double *resid, float *maxrel)
+// The end of a surrogate chunk, per region: each point's value f = F B_8 e^net (F B_exact where the point was
+// evaluated exactly), the Genz-Malik rule over the region's 33 values, and the Horvitz-Thompson correction
+// from its checked points.  A workgroup holds kSurFinRegions whole regions (33 threads each), so the values
+// stay in LDS and every global read is coalesced; the round's check sums go per workgroup into part
+// (k_sur_count adds them up; atomics on one address serialise).
+constexpr int kSurFinRegions = 7;  // 7 x 33 = 231 of 256 threads
+__global__ void __launch_bounds__(256) k_sur_fin(const TpRegion *__restrict__ reg, size_t r0, size_t nr,
+                                                 const int *__restrict__ flag, const double *__restrict__ F,
+                                                 const double *__restrict__ logb, const double *__restrict__ Bb,
+                                                 const double *__restrict__ Bex, double inv_frac, TpRegionOut *out,
+                                                 float *maxrel, double *part)
 {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool in = t < n;  // every lane stays for the wave's reduction below
-    const size_t u = in ? t : 0;
-    const int f = in ? flag[u] : 3;
-    const double Fv = F[u], lb = logb[u], bb = Bb[u], bx = Bex[u];
-    // baseline x learned correction; the network's output carries FP32 precision, so its exp is FP32
-    const double bs = bb * (double)__expf((float)lb);
-    double v = 0, r = 0, a = 0;
+    __shared__ double sv[kSurFinRegions][TP_NPTS_GM3];
+    __shared__ double sx[4][kSurFinRegions][TP_NPTS_GM3];  // w r, (w r)^2, |w| |F B_exact|, checked
+    __shared__ double red[kSurFinRegions][4];
+    const int rl = threadIdx.x / TP_NPTS_GM3, p = threadIdx.x % TP_NPTS_GM3;
+    const size_t rr = (size_t)blockIdx.x * kSurFinRegions + rl;
+    const bool in = rl < kSurFinRegions && rr < nr;  // every thread reaches the barriers below
     float e = 0.f;
-    if (f <= 1) v = Fv * bs;
-    if (f == 1) {
-        r = Fv * (bx - bs);
-        a = fabs(Fv * bx);
-        e = (float)(fabs(bx - bs) / fmax(fabs(bx), 1e-300));
-    }
-    if (f == 2) v = Fv * bx;
     if (in) {
-        fvals[t] = v;
-        resid[t] = r;
-        Bb[t] = a;
-    }
-    // the worst checked error: a wave that cannot raise the running maximum takes no atomic (atomics on one
-    // address serialise; the maximum is reached early and rarely moves)
-    for (int o = 32; o >= 1; o >>= 1) e = fmaxf(e, __shfl_xor(e, o, 64));
-    if (__lane_id() == 0 && e > __int_as_float(__atomic_load_n((const int *)maxrel, __ATOMIC_RELAXED)))
-        atomicMax((int *)maxrel, __float_as_int(e));
-}
-
-__global__ void __launch_bounds__(256) k_sur_rule(const TpRegion *__restrict__ reg, size_t r0, size_t nr,
-                                                  const double *fvals, const double *resid, const int *flag,
-                                                  const double *absfb, double inv_frac,
-                                                  TpRegionOut *out, double *part)
-{
-    __shared__ double red[4][4];
-    const size_t rr = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    double num = 0, den = 0, sq = 0;
-    int nchk = 0;
-    if (rr < nr) {  // every lane stays for the wave's reduction below
+        const size_t t = rr * TP_NPTS_GM3 + p;
+        const int f = flag[t];
+        // unconditional loads (Bex is read for every point and used only where it was computed)
+        const double Fv = F[t], lb = logb[t], bb = Bb[t], bx = Bex[t];
         const double vol = reg[r0 + rr].vol;
-        TpRegionOut o;
-        gm_rule3(vol, fvals + rr * TP_NPTS_GM3, &o);
-        // resid and absfb (|F B_exact|) are 0 on unchecked points: unconditional loads, no dependent chain
-        for (int p = 0; p < TP_NPTS_GM3; ++p) {
-            const size_t t = rr * TP_NPTS_GM3 + p;
-            const double w = gm_weight3(p) * vol, x = w * resid[t];
-            num += x;
-            sq += x * x;
-            den += fabs(w) * absfb[t];
-            nchk += flag[t] == 1;
+        // baseline x learned correction; the network's output carries FP32 precision, so its exp is FP32
+        const double bs = bb * (double)__expf((float)lb);
+        double v = 0, r = 0, a = 0;
+        if (f <= 1) v = Fv * bs;
+        if (f == 1) {
+            r = Fv * (bx - bs);
+            a = fabs(Fv * bx);
+            e = (float)(fabs(bx - bs) / fmax(fabs(bx), 1e-300));
         }
-        // Horvitz-Thompson: corr = sum w r / pi, Var ~ (1 - pi) sum (w r / pi)^2
-        o.corr = num * inv_frac;
-        o.cvar = (1.0 - 1.0 / inv_frac) * sq * inv_frac * inv_frac;
-        out[r0 + rr] = o;
+        if (f == 2) v = Fv * bx;
+        const double w = gm_weight3(p) * vol, x = w * r;
+        sv[rl][p] = v;
+        sx[0][rl][p] = x;
+        sx[1][rl][p] = x * x;
+        sx[2][rl][p] = fabs(w) * a;
+        sx[3][rl][p] = f == 1 ? 1.0 : 0.0;
     }
-    // the round's check statistics: the block's sums into its own slot of part (k_sur_count adds the slots;
-    // atomics on one address serialise)
-    double cnt = (double)nchk;
-    for (int w = 32; w >= 1; w >>= 1) {
-        num += __shfl_xor(num, w, 64);
-        den += __shfl_xor(den, w, 64);
-        sq += __shfl_xor(sq, w, 64);
-        cnt += __shfl_xor(cnt, w, 64);
-    }
-    const int wv = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        red[wv][0] = num;
-        red[wv][1] = den;
-        red[wv][2] = sq;
-        red[wv][3] = cnt;
+    // the worst checked error: a wave that cannot raise the running maximum takes no atomic
+    for (int o = 32; o >= 1; o >>= 1) e = fmaxf(e, __shfl_xor(e, o, 64));
+    if ((threadIdx.x & 63) == 0 && e > __int_as_float(__atomic_load_n((const int *)maxrel, __ATOMIC_RELAXED)))
+        atomicMax((int *)maxrel, __float_as_int(e));
+    __syncthreads();
+    if (threadIdx.x < kSurFinRegions) {
+        const int q = threadIdx.x;
+        const size_t rq = (size_t)blockIdx.x * kSurFinRegions + q;
+        double sum[4] = {0, 0, 0, 0};
+        if (rq < nr) {
+            for (int k = 0; k < TP_NPTS_GM3; ++k)
+                for (int c = 0; c < 4; ++c) sum[c] += sx[c][q][k];
+            TpRegionOut o;
+            gm_rule3(reg[r0 + rq].vol, sv[q], &o);
+            // Horvitz-Thompson: corr = sum w r / pi, Var ~ (1 - pi) sum (w r / pi)^2
+            o.corr = sum[0] * inv_frac;
+            o.cvar = (1.0 - 1.0 / inv_frac) * sum[1] * inv_frac * inv_frac;
+            out[r0 + rq] = o;
+        }
+        red[q][0] = sum[0];
+        red[q][1] = sum[2];
+        red[q][2] = sum[1];
+        red[q][3] = sum[3];
     }
     __syncthreads();
-    if (threadIdx.x < 4)
-        part[(size_t)blockIdx.x * 4 + threadIdx.x] =
-            red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+    if (threadIdx.x < 4) {
+        double acc = 0;
+        for (int q = 0; q < kSurFinRegions; ++q) acc += red[q][threadIdx.x];
+        part[(size_t)blockIdx.x * 4 + threadIdx.x] = acc;
+    }
 }
 
 }  // namespace
@@ -1645,7 +1636,7 @@ __global__ void __launch_bounds__(256) k_sur_count(const int *seg, int nseg, con
 }  // namespace
 
 int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, const TpRegion *d_reg,
-                       const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int *d_flag, hipStream_t st,
+                       const TpIntegralDesc *d_desc, TpRegionOut *d_out, int *d_flag, hipStream_t st,
                        unsigned long long round_key)
 {
     TpSurrogate *s = ctx->sur;
@@ -1660,8 +1651,8 @@ int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, co
     if ((rc = B.X.ensure(PCP * NIN * sizeof(float))) || (rc = B.F.ensure(PC * sizeof(double))) ||
         (rc = B.Bb.ensure(PC * sizeof(double))) || (rc = B.flag.ensure(PC * sizeof(int))) ||
         (rc = B.logb.ensure(PCP * sizeof(double))) || (rc = B.Bex.ensure(PC * sizeof(double))) ||
-        (rc = B.resid.ensure(PC * sizeof(double))) || (rc = B.list.ensure(NSEG * 256 * sizeof(int))) ||
-        (rc = B.seg.ensure(NSEG * sizeof(int))) || (rc = B.rpart.ensure((RC + 255) / 256 * 4 * sizeof(double))) ||
+        (rc = B.list.ensure(NSEG * 256 * sizeof(int))) ||
+        (rc = B.seg.ensure(NSEG * sizeof(int))) || (rc = B.rpart.ensure((RC + kSurFinRegions - 1) / kSurFinRegions * 4 * sizeof(double))) ||
         (rc = B.misc.ensure(64)) || (rc = B.stats.ensure(64)))
         return rc;
     double *dstats = (double *)B.misc.p;
@@ -1691,13 +1682,10 @@ int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, co
         B.nev += 1;
         hipLaunchKernelGGL(k_sur_exact, dim3(nseg), dim3(256), 0, st, (const int *)B.list.p, (const int *)B.seg.p,
                            d_reg, r0, d_desc, c->d_node32, c->d_eh32, (double *)B.Bex.p, d_flag);
-        hipLaunchKernelGGL(k_sur_combine, dim3(grid(np, 256)), dim3(256), 0, st, np, (const int *)B.flag.p,
-                           (const double *)B.F.p, (const double *)B.logb.p, (double *)B.Bb.p,
-                           (const double *)B.Bex.p, d_fvals + r0 * TP_NPTS_GM3, (double *)B.resid.p, dmaxrel);
-        const int nrb = grid(nr, 256);
-        hipLaunchKernelGGL(k_sur_rule, dim3(nrb), dim3(256), 0, st, d_reg, r0, nr,
-                           (const double *)(d_fvals + r0 * TP_NPTS_GM3), (const double *)B.resid.p,
-                           (const int *)B.flag.p, (const double *)B.Bb.p, 1.0 / frac, d_out, (double *)B.rpart.p);
+        const int nrb = (int)((nr + kSurFinRegions - 1) / kSurFinRegions);
+        hipLaunchKernelGGL(k_sur_fin, dim3(nrb), dim3(256), 0, st, d_reg, r0, nr, (const int *)B.flag.p,
+                           (const double *)B.F.p, (const double *)B.logb.p, (const double *)B.Bb.p,
+                           (const double *)B.Bex.p, 1.0 / frac, d_out, dmaxrel, (double *)B.rpart.p);
         hipLaunchKernelGGL(k_sur_count, dim3(1), dim3(256), 0, st, (const int *)B.seg.p, nseg,
                            (const double *)B.rpart.p, nrb, dstats);
         TP_HIP(hipGetLastError());

@@ -443,7 +443,7 @@ static const size_t kSurrogateMaxEval = 4000000;  // points per triple before an
 int tp_region_lanes(tp_ctx *ctx, const tp_cosmo *c, int mode, size_t np);  // tp_map3.hip
 // tp_mlp.hip: a surrogate round queued on a group's stream, and its check statistics once it ran
 int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, const TpRegion *d_reg,
-                       const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int *d_flag, hipStream_t st,
+                       const TpIntegralDesc *d_desc, TpRegionOut *d_out, int *d_flag, hipStream_t st,
                        unsigned long long round_key);
 int tp_surrogate_result(tp_ctx *ctx, int lane, size_t nreg, bool *fallback);
 
@@ -610,8 +610,8 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
                 return e;
             if (nreg > Gr.nexact &&
                 (e = tp_surrogate_queue(ctx, c, g, nreg - Gr.nexact, dr + Gr.nexact,
-                                        (const TpIntegralDesc *)ctx->d_desc.p, dfv + Gr.nexact * TP_NPTS_GM3,
-                                        dout + Gr.nexact, (int *)L.d_flag.p, gst[g],
+                                        (const TpIntegralDesc *)ctx->d_desc.p, dout + Gr.nexact, (int *)L.d_flag.p,
+                                        gst[g],
                                         (unsigned long long)(Gr.rounds * TP_NLANES + g))))
                 return e;
         } else {
