@@ -156,3 +156,26 @@ def test_gamma_production_grid_matches_oracle_fixture(fid_tables, gpu_ctx):
               % (row["triangle"], row["reltol"], e64.max(), e32.max(), list(n64[0]), row["neval"]))
         assert e64.max() < 1e-4
         assert e32.max() < 1e-3
+
+
+
+def test_gamma_points_beyond_one_launch(fid_tables, gpu_ctx):
+    """A round of more points than one launch covers (one 256-thread block per point; a dispatch counts at most
+    2^32 work-items, so the host splits a round into launches of 2^22 points): every point's value, on either
+    side of a launch boundary, equals the value of the same point evaluated in a call of its own.  Batched
+    Gamma rounds of 100 wide triangles exceed 2^24 points; without the split, the points beyond the first
+    2^32 work-items of a round were never evaluated and kept the previous round's values."""
+    tp.gamma_setup(0.1, 2.0, gpu_ctx)  # 20 Ogata terms per point: the big call stays cheap
+    x = _tri()
+    rng = np.random.default_rng(11)
+    m = 997
+    few = np.stack([rng.uniform(0.05, 2.9, m), rng.uniform(0, 2 * np.pi, m), rng.uniform(0.05, 1.5, m)], 1)
+    n = (1 << 22) + 2 * m
+    pts = np.ascontiguousarray(np.tile(few, (-(-n // m), 1))[:n])
+    for which in (0, 1):
+        big = fid_tables.gamma_eval(which, tuple(x), pts)
+        ref = fid_tables.gamma_eval(which, tuple(x), few)
+        assert np.all(np.isfinite(big))
+        for lo in (0, (1 << 22) - m, 1 << 22, n - m):
+            idx = np.arange(lo, min(lo + m, n))
+            assert np.array_equal(big[idx], ref[idx % m]), (which, lo)

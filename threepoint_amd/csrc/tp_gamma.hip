@@ -24,6 +24,7 @@
 namespace {
 
 constexpr int kGammaBlock = 256;
+constexpr size_t kGammaLaunchPts = (size_t)1 << 22;  // points per launch: 2^22 blocks x 256 threads = 2^30 work-items
 // the FP32 form's miss mask holds 3 bits per k slot of a thread in 128 bits: 42 slots of 256 k
 constexpr int kGammaMaxK32 = 42 * kGammaBlock + 1;
 
@@ -214,7 +215,7 @@ __global__ void __launch_bounds__(kGammaBlock) __attribute__((amdgpu_waves_per_e
                                                               const TpEH32 *__restrict__ eh32,
                                                               int n_z, const double *__restrict__ ogata, int prec_k,
                                                               const TpGammaDesc *__restrict__ desc,
-                                                              const int *__restrict__ pdesc, size_t npts,
+                                                              const int *__restrict__ pdesc, size_t p0, size_t npts,
                                                               const double *__restrict__ x, double *__restrict__ f,
                                                               int *flag, double q32lo, double q32hi)
 {
@@ -224,7 +225,7 @@ __global__ void __launch_bounds__(kGammaBlock) __attribute__((amdgpu_waves_per_e
     __shared__ TpShapeNodeT<double> sS;
     __shared__ int sok;
     __shared__ double red[3][kGammaBlock / 64];
-    size_t p = blockIdx.x;
+    size_t p = p0 + blockIdx.x;  // one block per point; a launch covers points [p0, p0 + gridDim.x)
     if (p >= npts) return;
     const TpGammaDesc D = desc[pdesc[p]];
     const double z = x[3 * p], phi = x[3 * p + 1], psi = x[3 * p + 2];
@@ -646,23 +647,28 @@ static int gamma_launch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpGamm
     TP_HIP(hipMemcpyAsync(ctx->d_desc.p, desc.data(), bd, hipMemcpyHostToDevice, ctx->stream));
     TP_HIP(hipMemcpyAsync(ctx->d_misc.p, pdesc.data(), bp, hipMemcpyHostToDevice, ctx->stream));
     TP_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-    if (mode == TP_MODE_FP32)
-        hipLaunchKernelGGL(k_gamma_points<true>, dim3((unsigned)npts), dim3(kGammaBlock), 0, ctx->stream, c->d_const,
-                           c->d_tab, c->d_eh64, c->d_eh32, c->n_z, ctx->d_ogata, (int)ctx->gamma_prec_k,
-                           (const TpGammaDesc *)ctx->d_desc.p, (const int *)ctx->d_misc.p, npts,
-                           (const double *)ctx->d_in.p, (double *)ctx->d_out.p, (int *)ctx->d_flag.p,
-                           ctx->gamma_q32[0], ctx->gamma_q32[1]);
-    else if (mode == TP_MODE_ANALYTIC)
-        hipLaunchKernelGGL((k_gamma_points<false, true>), dim3((unsigned)npts), dim3(kGammaBlock), 0, ctx->stream,
-                           c->d_const, c->d_tab, c->d_eh64, c->d_eh32, c->n_z, ctx->d_ogata, (int)ctx->gamma_prec_k,
-                           (const TpGammaDesc *)ctx->d_desc.p, (const int *)ctx->d_misc.p, npts,
-                           (const double *)ctx->d_in.p, (double *)ctx->d_out.p, (int *)ctx->d_flag.p, 0.0, 0.0);
-    else
-        hipLaunchKernelGGL(k_gamma_points<false>, dim3((unsigned)npts), dim3(kGammaBlock), 0, ctx->stream, c->d_const,
-                           c->d_tab, c->d_eh64, c->d_eh32, c->n_z, ctx->d_ogata, (int)ctx->gamma_prec_k,
-                           (const TpGammaDesc *)ctx->d_desc.p, (const int *)ctx->d_misc.p, npts,
-                           (const double *)ctx->d_in.p, (double *)ctx->d_out.p, (int *)ctx->d_flag.p, 0.0, 0.0);
-    TP_HIP(hipGetLastError());
+    // One block of kGammaBlock threads per point.  A dispatch's grid size is a 32-bit count of work-items,
+    // so one launch covers at most kGammaLaunchPts points; larger rounds run as several launches.
+    for (size_t p0 = 0; p0 < npts; p0 += kGammaLaunchPts) {
+        const dim3 gr((unsigned)std::min(npts - p0, kGammaLaunchPts));
+        if (mode == TP_MODE_FP32)
+            hipLaunchKernelGGL(k_gamma_points<true>, gr, dim3(kGammaBlock), 0, ctx->stream, c->d_const, c->d_tab,
+                               c->d_eh64, c->d_eh32, c->n_z, ctx->d_ogata, (int)ctx->gamma_prec_k,
+                               (const TpGammaDesc *)ctx->d_desc.p, (const int *)ctx->d_misc.p, p0, npts,
+                               (const double *)ctx->d_in.p, (double *)ctx->d_out.p, (int *)ctx->d_flag.p,
+                               ctx->gamma_q32[0], ctx->gamma_q32[1]);
+        else if (mode == TP_MODE_ANALYTIC)
+            hipLaunchKernelGGL((k_gamma_points<false, true>), gr, dim3(kGammaBlock), 0, ctx->stream, c->d_const,
+                               c->d_tab, c->d_eh64, c->d_eh32, c->n_z, ctx->d_ogata, (int)ctx->gamma_prec_k,
+                               (const TpGammaDesc *)ctx->d_desc.p, (const int *)ctx->d_misc.p, p0, npts,
+                               (const double *)ctx->d_in.p, (double *)ctx->d_out.p, (int *)ctx->d_flag.p, 0.0, 0.0);
+        else
+            hipLaunchKernelGGL(k_gamma_points<false>, gr, dim3(kGammaBlock), 0, ctx->stream, c->d_const, c->d_tab,
+                               c->d_eh64, c->d_eh32, c->n_z, ctx->d_ogata, (int)ctx->gamma_prec_k,
+                               (const TpGammaDesc *)ctx->d_desc.p, (const int *)ctx->d_misc.p, p0, npts,
+                               (const double *)ctx->d_in.p, (double *)ctx->d_out.p, (int *)ctx->d_flag.p, 0.0, 0.0);
+        TP_HIP(hipGetLastError());
+    }
     TP_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     TP_HIP(hipMemcpyAsync(ctx->h_out.p, ctx->d_out.p, bf, hipMemcpyDeviceToHost, ctx->stream));
     TP_HIP(hipStreamSynchronize(ctx->stream));
