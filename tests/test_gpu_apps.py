@@ -92,7 +92,8 @@ def test_map3_cli_chunks_and_resume(tmp_path):
 
 
 def test_gamma_cli_resume(tmp_path):
-    """--resume keeps the complete r bins of an interrupted grid run and computes the rest: same file."""
+    """--resume keeps the complete r bins of an interrupted grid run and computes the rest, and --chunk N
+    evaluates an r bin in batches of N triangles: same file."""
     cf, nf = _inputs(tmp_path)
     gc = tmp_path / "gamma.param"
     gc.write_text("rsteps 2\nusteps 1\nvsteps 2\numin 0.4\numax 0.6\nvmin -1\nvmax 1\nrmin 4\nrmax 6\n")
@@ -111,6 +112,8 @@ def test_gamma_cli_resume(tmp_path):
     part = tmp_path / "b.dat"
     part.write_text("".join(lines[:3]))  # the first r bin and half of the second
     assert run(part, "--resume") == full
+    # --chunk 1: each triangle its own batch, the same lines (each integral's decisions are its own)
+    assert run(tmp_path / "c.dat", "--chunk", "1") == full
 
 
 def test_calculate_powerspectrum_cli(tmp_path, fid_oracle):

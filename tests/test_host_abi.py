@@ -135,6 +135,30 @@ def test_triples_and_gamma_grid():
     assert np.allclose(g[:, 1], r2) and np.allclose(g[:, 2], r2 * u) and np.allclose(g[:, 0], v * r2 * u + r2)
 
 
+def test_executable_inputs_round_trip(tmp_path):
+    """The input files written for the executables read back, with the package's readers and the oracle's
+    n(z) reader, as the workloads they stand for."""
+    p = W.write_executable_inputs(str(tmp_path))
+    c = tp.Cosmology.from_file(p["cosmo.param"])
+    for k, v in W.FIDUCIAL.items():
+        assert getattr(c, k) == v, k
+    bad = tmp_path / "bad.param"
+    bad.write_text("h 0.7\nOmega_k 0.1\n")
+    with pytest.raises(ValueError):
+        tp.Cosmology.from_file(str(bad))
+    out = np.zeros(100)
+    assert O.lib().orc_read_n_of_z(p["nz.dat"].encode(), 100, 3.0, O.d(out)) == 0
+    zz = np.loadtxt(p["nz.dat"])
+    assert np.allclose(W.resample_n_of_z(zz[:, 0], zz[:, 1]), out, rtol=1e-15, atol=0)
+    cfg = dict(ln.split() for ln in open(p["gamma_c2.param"]))
+    g = W.gamma_c2_triangles(int(cfg["rsteps"]), int(cfg["usteps"]), int(cfg["vsteps"]), float(cfg["rmin"]),
+                             float(cfg["rmax"]), float(cfg["umin"]), float(cfg["umax"]), float(cfg["vmin"]),
+                             float(cfg["vmax"]))
+    assert np.array_equal(g, W.gamma_c2_triangles())
+    th = np.loadtxt(p["thetas_c1.dat"], comments="#")
+    assert np.array_equal(th, np.asarray(W.map3_c1_radii(), dtype=np.float64))
+
+
 def test_lhs_is_latin():
     cs = W.lhs_c3(64)
     om = np.array([c.om for c in cs])

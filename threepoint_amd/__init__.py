@@ -292,8 +292,19 @@ class Cosmology:
 
     @classmethod
     def from_file(cls, fn):
-        from .io import read_cosmology
-        return read_cosmology(fn)
+        """classed_version/cosmology.cpp:10-73: "name value" lines, '#' comments and empty lines skipped,
+        an unknown name is an error; parameters the file does not name keep their defaults."""
+        kw = {}
+        with open(fn) as f:
+            for line in f:
+                line = line.rstrip("\n")
+                if not line or line[0] == "#":
+                    continue
+                parts = line.split()
+                if not parts or parts[0] not in cls.KEYS:
+                    raise ValueError("cosmology: Parameter file in wrong format: %r" % line)
+                kw[cls.KEYS[parts[0]]] = float(parts[1]) if len(parts) > 1 else 0.0
+        return cls(**kw)
 
     def as_struct(self):
         return tp_cosmology(self.h, self.sigma8, self.omb, self.ns, self.w, self.om, self.zmax)

@@ -2,13 +2,14 @@
 //
 //   calculateGamma.x <cosmology file> <gamma config file> <output file> <n(z) file> [device]
 //                    [--nz-bins N] [--nz-averaging] [--prec-h H] [--prec-k K] [--reltol X]
-//                    [--triangles FILE] [--fp32] [--resume]
+//                    [--triangles FILE] [--fp32] [--chunk N] [--resume]
 //
 // Gamma_0..Gamma_3 (centroid convention) on the (r, u, v) grid of the config file, r2 = r pi/180/60,
 // r3 = r2 u, r1 = v r3 + r2 (the TreeCorr binning of cuda_version/calculateGamma.cu:107-121), one
 // output line per triangle: Re/Im Gamma0 ... Re/Im Gamma3, r, u, v.  All triangles of one r bin are
-// evaluated in one batched GPU pass and written (and flushed) before the next r bin starts; --resume keeps
-// the complete r bins an interrupted run left in the output file and computes the rest.
+// evaluated in one batched GPU pass and written (and flushed) before the next r bin starts (--chunk N: in
+// batches of N triangles, each written before the next starts); --resume keeps the complete r bins an
+// interrupted run left in the output file and computes the rest.
 //
 // --triangles FILE: the classed_version's mode (calculateGamma.cpp:64-160): the (r, u, v) bins of a
 // TreeCorr triangle list ("rind uind vind r u v" rows, read_triangle_configurations), sides converted
@@ -16,6 +17,7 @@
 // r u v" in %e format.  The config argument is not read then.  --nz-averaging reads n(z) with the
 // cuda_version's reader (overlap-weighted bin averages for coarse files).  --fp32 selects the FP32 form
 // of the integrand (FP32 Ogata terms inside their wavenumber window, FP64 accumulation).
+#include <algorithm>
 #include <array>
 #include <chrono>
 #include <cmath>
@@ -32,7 +34,7 @@ int main(int argc, char **argv)
 {
     const char *usage = "calculateGamma.x <cosmology> <gamma config> <output> <n(z)> [device] [--nz-bins N]\n"
                         "                 [--nz-averaging] [--prec-h H] [--prec-k K] [--reltol X] [--triangles FILE]\n"
-                        "                 [--fp32] [--resume]\n";
+                        "                 [--fp32] [--chunk N] [--resume]\n";
     if (argc < 5) {
         std::cerr << usage;
         return 1;
@@ -40,6 +42,7 @@ int main(int argc, char **argv)
     int nbins = 100, device = 0;
     double prec_h = 1e-3, prec_k = 5.0, reltol = 1e-3;
     bool nz_avg = false, fp32 = false, resume = false;
+    size_t chunk = 0;
     std::string trifile;
     for (int i = 5; i < argc; ++i) {
         std::string a = argv[i];
@@ -47,6 +50,7 @@ int main(int argc, char **argv)
         else if (a == "--nz-averaging") nz_avg = true;
         else if (a == "--fp32") fp32 = true;
         else if (a == "--resume") resume = true;
+        else if (a == "--chunk" && i + 1 < argc) chunk = (size_t)std::atoll(argv[++i]);
         else if (a == "--triangles" && i + 1 < argc) trifile = argv[++i];
         else if (a == "--prec-h" && i + 1 < argc) prec_h = std::atof(argv[++i]);
         else if (a == "--prec-k" && i + 1 < argc) prec_k = std::atof(argv[++i]);
@@ -161,11 +165,18 @@ int main(int argc, char **argv)
                     uv.push_back({u, v});
                 }
             }
-            auto g = gc.gammas(x);
-            for (size_t t = 0; t < x.size(); ++t) {
-                out << g[t][0].real() << " " << g[t][0].imag() << " " << g[t][1].real() << " " << g[t][1].imag() << " "
-                    << g[t][2].real() << " " << g[t][2].imag() << " " << g[t][3].real() << " " << g[t][3].imag() << " "
-                    << r << " " << uv[t][0] << " " << uv[t][1] << std::endl;
+            const size_t step = chunk ? chunk : x.size();
+            for (size_t s0 = 0; s0 < x.size(); s0 += step) {
+                const size_t s1 = std::min(x.size(), s0 + step);
+                auto g = gc.gammas(std::vector<std::array<double, 3>>(x.begin() + s0, x.begin() + s1));
+                for (size_t t = s0; t < s1; ++t) {
+                    const auto &gt = g[t - s0];
+                    out << gt[0].real() << " " << gt[0].imag() << " " << gt[1].real() << " " << gt[1].imag() << " "
+                        << gt[2].real() << " " << gt[2].imag() << " " << gt[3].real() << " " << gt[3].imag() << " " << r
+                        << " " << uv[t][0] << " " << uv[t][1] << std::endl;
+                }
+                if (chunk && s1 < x.size())
+                    std::cerr << "r bin " << i + 1 << ": " << s1 << "/" << x.size() << " triangles" << std::endl;
             }
             double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
             std::cerr << "r bin " << i + 1 << "/" << config.rsteps << " done, " << el << " s" << std::endl;

@@ -154,3 +154,34 @@ def treecorr_sides(r, u, v):
     r = np.asarray(r, dtype=np.float64)
     r3 = r * np.asarray(u, dtype=np.float64)
     return np.stack([np.asarray(v, dtype=np.float64) * r3 + r, r, r3], 1)
+
+
+def write_executable_inputs(out_dir):
+    """The executables' input files of these workloads, in the reference's formats: cosmo.param (the
+    fiducial cosmology, classed_version/cosmology.cpp reader), nz.dat (the (z, n) rows above), gamma_c2.param
+    (the gamma_c2 grid, cuda_version/calculateGamma.cu config reader) and thetas_c1.dat (the map3_c1 radii in
+    arcmin).  Returns the paths."""
+    import os
+    os.makedirs(out_dir, exist_ok=True)
+    c = FIDUCIAL
+    files = {
+        "cosmo.param": "# fiducial\nh %g\nsigma_8 %g\nOmega_b %g\nn_s %g\nw %g\nOmega_m %g\nz_max %g\n"
+                       % (c["h"], c["sigma8"], c["omb"], c["ns"], c["w"], c["om"], c["zmax"]),
+        "nz.dat": "".join("%.10g %.10g\n" % (a, b) for a, b in zip(*nz_file_rows())),
+        "gamma_c2.param": "rsteps 10\nusteps 10\nvsteps 10\numin 0\numax 1\nvmin -1\nvmax 1\nrmin 0.1\nrmax 100\n",
+        "thetas_c1.dat": "# arcmin\n" + "".join("%.17g\n" % t for t in map3_c1_radii()),
+    }
+    paths = {}
+    for name, text in files.items():
+        paths[name] = os.path.join(out_dir, name)
+        with open(paths[name], "w") as f:
+            f.write(text)
+    return paths
+
+
+if __name__ == "__main__":
+    import sys
+    if len(sys.argv) != 2:
+        sys.exit("usage: python -m threepoint_amd.workloads OUT_DIR   (writes the executables' input files)")
+    for p in write_executable_inputs(sys.argv[1]).values():
+        print(p)
