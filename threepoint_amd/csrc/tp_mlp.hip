@@ -953,19 +953,37 @@ __device__ __forceinline__ void mlph_bias_init(f32x16 (&acc)[8], const float *__
         }
 }
 
-// one 16-wide K slice: the 8 weight fragments in two halves of 4 (16 registers in flight, not 32)
-template <int KB>
-__device__ __forceinline__ void mlph_kslice(const u32x4 *src, f32x16 (&acc)[8], const f16x8 (&B)[16], int lane)
+// one stage (4 K slices x 8 weight fragments) as 8 groups of 4 fragments (K slice 4ST + g/2, tiles
+// 4(g%2)..4(g%2)+3): group g+1 is read from LDS while group g's MFMAs issue, so that an MFMA waits for a
+// read issued four MFMAs earlier.  Left to itself the compiler reads each fragment just before its MFMA
+// and waits for it there; the sched_group_barrier sequence (0x100 DS read, 0x008 MFMA) fixes the order.
+template <int ST>
+__device__ __forceinline__ void mlph_stage(const u32x4 *src, f32x16 (&acc)[8], const f16x8 (&B)[16], int lane)
 {
+    static_assert(KPS == 4, "8 groups of 4 fragments = 4 K slices of 8 fragments");
+    f16x8 a[2][4];
 #pragma unroll
-    for (int half = 0; half < 2; ++half) {
-        f16x8 a[4];
+    for (int t = 0; t < 4; ++t) a[0][t] = __builtin_bit_cast(f16x8, src[t * 64 + lane]);
 #pragma unroll
-        for (int t = 0; t < 4; ++t) a[t] = __builtin_bit_cast(f16x8, src[(4 * half + t) * 64 + lane]);
+    for (int g = 0; g < 8; ++g) {
+        if (g < 7) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                a[(g + 1) & 1][t] =
+                    __builtin_bit_cast(f16x8, src[((g + 1) >> 1) * 512 + (4 * ((g + 1) & 1) + t) * 64 + lane]);
+        }
 #pragma unroll
         for (int t = 0; t < 4; ++t)
-            acc[4 * half + t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[t], B[KB], acc[4 * half + t], 0, 0, 0);
+            acc[4 * (g & 1) + t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a[g & 1][t], B[KPS * ST + (g >> 1)],
+                                                                          acc[4 * (g & 1) + t], 0, 0, 0);
     }
+    __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+#pragma unroll
+    for (int g = 0; g < 7; ++g) {
+        __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
 }
 
 template <int ST>
@@ -974,11 +992,7 @@ __device__ __forceinline__ void mlph_stages(f32x16 (&acc)[8], const f16x8 (&B)[1
 {
     if constexpr (ST < 16 / KPS) {
         mlp_stage_load(ring[(ST + 1) & 1], Wf, (gs + ST + 1) % NSTAGE, wave, lane);
-        const u32x4 *src = ring[ST & 1];
-        mlph_kslice<KPS * ST + 0>(src, acc, B, lane);
-        mlph_kslice<KPS * ST + 1>(src + 512, acc, B, lane);
-        mlph_kslice<KPS * ST + 2>(src + 1024, acc, B, lane);
-        mlph_kslice<KPS * ST + 3>(src + 1536, acc, B, lane);
+        mlph_stage<ST>(ring[ST & 1], acc, B, lane);
         __syncthreads();
         mlph_stages<ST + 1>(acc, B, ring, Wf, gs, wave, lane);
     }
