@@ -14,9 +14,18 @@
     window (up to 45% low on these cases), so the limit is checked at 1e-3.
   * The classed_version's 4-D form is symmetric under theta_123 <-> theta_456 up to its ell truncation
     (the permutation sum makes the integrand symmetric on the full domain): swapped pairs agree to the
-    1e-3 tolerance.  Its single-z triple spectrum is pinned through P_NL (test_power_oracle.py) and the
-    restatement; the 4-D cubature driver is bit-exact against the reference's (test_host_abi.py).
+    1e-3 tolerance.  The 4-D cubature driver is bit-exact against the reference's (test_host_abi.py).
+  * Its parts, each against a form written independently of the oracle's: P_kappa's Limber integrand
+    per unit z is (dchi/dz) W(chi)^2 / chi^2 P_NL(ell / chi, z) with the lensing kernel
+    W = 3/2 Omega_m (H0/c)^2 chi (1 + z) g and dchi/dz = (c/H0) / E (fixes the 9/4, the powers of H0/c,
+    (1 + z) and g, and the 1/E); the single-z triple spectrum is the product of three such integrands
+    at the same z; the covariance integrand is the uHat product of theta_123 times the sum over all six
+    orderings of (l1, l2, l3) against theta_456 times the triple spectrum times l1 l2; and the driver's
+    result is 2 / area / (2 pi)^3 times the integral over [1, lMax]^2 x [0, pi] x [0, z_max],
+    lMax = 10 / max(min theta_123, min theta_456), checked against a quasi-Monte-Carlo estimate of that
+    integral (a factor 2 or a domain mistake is far outside its ~0.5% error).
 """
+import itertools
 import math
 from fractions import Fraction
 
@@ -138,3 +147,68 @@ def test_t1_finite_tends_to_infinite_survey(orc, geometry, ratio2, bar):
     vf, nf = orc.t1_total_finite(th, th, geometry, tx, ty, 0.0, 0.3, ngal, 4, 1e-3)
     vi, _ = orc.t1_total(th, th, area, 0.0, 0.3, ngal, 4, 1e-5)
     assert nf > 0 and abs(vf / vi - 1) < bar, (geometry, vf / vi)
+
+
+def _uhat(x):
+    t = 0.5 * x * x
+    return t * np.exp(-t)
+
+
+def test_limber_integrand_is_the_lensing_kernel_form(orc):
+    tb = orc.tables()
+    fk, g = tb["f_K"], tb["g"]
+    n = fk.size
+    h0c, om = 100.0 / 299792.0, 0.3
+    for i in (5, 30, 61, 98):
+        z = i * 3.0 / (n - 1)  # a node of the lookup grid: no interpolation
+        chi = fk[i]
+        w = 1.5 * om * h0c ** 2 * chi * (1 + z) * g[i]
+        e = math.sqrt(om * (1 + z) ** 3 + 1 - om)
+        for ell in (10.0, 300.0, 3e3, 3e4):
+            want = (1.0 / h0c) / e * w * w / (chi * chi) * orc.scalar("pk_nonlinear", ell / chi, z)
+            assert abs(orc.scalar("limber_pkappa", ell, z, 0) / want - 1) < 1e-12, (i, ell)
+
+
+def test_triple_spectrum_is_three_single_spectra(orc):
+    rng = np.random.default_rng(41)
+    for _ in range(20):
+        l1, l2, l3 = np.exp(rng.uniform(1, 9, 3))
+        z = rng.uniform(0.01, 2.99)
+        single = [orc.scalar("limber_pkappa", l, z, 0) for l in (l1, l2, l3)]
+        assert abs(orc.scalar("limber_triple_pk", l1, l2, l3, z) / np.prod(single) - 1) < 1e-12
+    assert orc.scalar("limber_triple_pk", 100.0, 200.0, 250.0, 0.5e-5) == 0.0
+
+
+def test_cov_gauss_integrand_composition(orc):
+    from _oracle import d
+    rng = np.random.default_rng(42)
+    th6 = np.array([4.0, 8.0, 8.0, 6.0, 6.0, 12.0]) * A
+    for _ in range(20):
+        l1, l2 = np.exp(rng.uniform(2, 8, 2))
+        phi, z = rng.uniform(0, math.pi), rng.uniform(0.01, 2.99)
+        l3 = math.sqrt(l1 * l1 + l2 * l2 + 2 * l1 * l2 * math.cos(phi))
+        u123 = _uhat(l1 * th6[0]) * _uhat(l2 * th6[1]) * _uhat(l3 * th6[2])
+        u456 = sum(_uhat(a * th6[3]) * _uhat(b * th6[4]) * _uhat(c * th6[5])
+                   for a, b, c in itertools.permutations((l1, l2, l3)))
+        want = u123 * u456 * orc.scalar("limber_triple_pk", l1, l2, l3, z) * l1 * l2
+        got = orc.scalar("cov_gauss_integrand", l1, l2, phi, z, d(th6))
+        assert abs(got / want - 1) < 1e-12
+
+
+def test_cov_gauss_driver_normalisation(orc):
+    from scipy.stats import qmc
+    from _oracle import d
+    th6 = np.array([4.0, 8.0, 8.0, 6.0, 6.0, 12.0]) * A
+    area = (5 * 60 * A) ** 2
+    lmax = 10.0 / max(th6[:3].min(), th6[3:].min())
+    # scrambled Sobol points, uniform in (ln l1, ln l2) over [0, ln lMax]^2, phi over [0, pi], z over [0, z_max]
+    u = qmc.Sobol(4, scramble=True, seed=43).random(1 << 18)
+    l12 = np.exp(np.log(lmax) * u[:, :2])
+    jac = np.log(lmax) ** 2 * l12[:, 0] * l12[:, 1] * math.pi * 3.0
+    pth = d(th6)
+    f = np.array([orc.scalar("cov_gauss_integrand", a, b, math.pi * c, 3.0 * e, pth)
+                  for (a, b), (c, e) in zip(l12, u[:, 2:])])
+    est = (f * jac).mean() * 2 / area / (2 * math.pi) ** 3
+    v = orc.map3_cov_gauss(th6, area, 1e-4)[0]
+    print("4-D covariance: cubature %.6e, quasi-Monte-Carlo %.6e" % (v, est))
+    assert abs(est / v - 1) < 0.02  # the estimate's own error at 2^18 points: ~0.5%
