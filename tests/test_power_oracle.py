@@ -13,8 +13,9 @@ What each check fixes, independently of the oracle's own formulas:
     the filter and the 1 / 2 pi;
   * the shape-noise term of <Map^2> has the closed form sigma^2 / (2 n) / (8 pi theta^2)
     (int_0^inf ell uHat(ell theta)^2 d ell = 1 / (4 theta^2)).
-The Halofit and T17 fit constants themselves are pinned by restatement review only (parity unpinned
-for those constants): the reference ships no power-spectrum fixture.
+The Halofit and T17 fit constants themselves are pinned to the reference's source text, and the
+formulas around them to independent restatements built from those constants, in
+tests/test_fit_constants.py (the reference ships no power-spectrum fixture).
 """
 import math
 
