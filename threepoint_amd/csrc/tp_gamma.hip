@@ -678,7 +678,7 @@ static int gamma_launch(tp_ctx *ctx, const tp_cosmo *c, const std::vector<TpGamm
     ctx->stats.kernel_ms += ms;
     ctx->stats.integrand_ms += ms;
     ctx->stats.integrand_sum_ms += ms;
-    ctx->stats.launches += 1;
+    ctx->stats.launches += (long long)((npts + kGammaLaunchPts - 1) / kGammaLaunchPts);
     int flag = 0;
     TP_HIP(hipMemcpy(&flag, ctx->d_flag.p, sizeof(int), hipMemcpyDeviceToHost));
     if (flag) return tp_set_error(TP_ENONFINITE, "Gamma integrand: non-finite value");
