@@ -14,7 +14,9 @@
 //   k_norm          1 wave / cosmology     : norm = sigma8 / sigma_TH(8)
 //   k_fK            1 thread / (cosmo,bin) : f_K(z_i)
 //   k_g             1 thread / (cosmo,bin) : g(z_i)
-//   k_growth        1 wave / (cosmo,bin)   : D1, r_sigma, n_eff, ncur
+//   k_lgr0          1 thread / cosmology   : lgr(0)
+//   k_d1            1 thread / (cosmo,bin) : D1 = lgr(z_i) / lgr(0)
+//   k_growth        1 wave / (cosmo,bin)   : r_sigma, n_eff, ncur
 //   k_nodes         1 thread / (cosmo,node): 96-node Limber tables (FP64 + FP32 copy)
 #include <cfloat>
 #include <cmath>
@@ -363,6 +365,24 @@ __global__ void k_g(int ncosmo, int n_z, double dz, double z_max, const double *
     tab[(size_t)ic * 6 * n_z + n_z + i] = g;
 }
 
+// lgr(0) of each cosmology, one thread per cosmology (every z-bin's D1 divides by it)
+__global__ void k_lgr0(int ncosmo, const TpCosmoConst *cc, double *lgr0)
+{
+    const int ic = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ic < ncosmo) lgr0[ic] = lgr(cc[ic], 0.);
+}
+
+// D1 = lgr(z) / lgr(0) (classed_version/bispectrum.cpp:157), one thread per (cosmology, z-bin): the growth
+// ODE is sequential, so each lane integrates its own bin instead of a whole wave repeating one
+__global__ void k_d1(int ncosmo, int n_z, double dz, const TpCosmoConst *cc, const double *lgr0, double *tab)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncosmo * n_z) return;
+    const int ic = t / n_z, i = t % n_z;
+    tab[(size_t)ic * 6 * n_z + 2 * n_z + i] = lgr(cc[ic], i * dz) / lgr0[ic];
+}
+
+// r_sigma, n_eff and ncur of one (cosmology, z-bin) per wave, from the bin's D1
 __global__ void __launch_bounds__(64) k_growth(int ncosmo, int n_z, double dz, const TpCosmoConst *cc, double *tab)
 {
     int ic = blockIdx.x / n_z, i = blockIdx.x % n_z;
@@ -370,8 +390,7 @@ __global__ void __launch_bounds__(64) k_growth(int ncosmo, int n_z, double dz, c
     int lane = threadIdx.x;
     TpCosmoConst C = cc[ic];
     const double eps = 1.e-4;
-    double z = i * dz;
-    double D1 = lgr(C, z) / lgr(C, 0.);                  // classed_version/bispectrum.cpp:157
+    const double D1 = tab[(size_t)ic * 6 * n_z + 2 * n_z + i];
     // calc_r_sigma, classed_version/bispectrum.cpp:676-707
     double k, k1 = 1., k2 = 1.;
     for (int it = 0; it < 200; ++it) {
@@ -397,7 +416,6 @@ __global__ void __launch_bounds__(64) k_growth(int ncosmo, int n_z, double dz, c
     double d1 = -2. * (D1 * s2) * (D1 * s2);
     if (lane == 0) {
         double *T = tab + (size_t)ic * 6 * n_z;
-        T[2 * n_z + i] = D1;
         T[3 * n_z + i] = rs;
         T[4 * n_z + i] = -3. + 2. * (D1 * s2) * (D1 * s2);   // classed_version/bispectrum.cpp:160
         T[5 * n_z + i] = d1 * d1 + 4. * s3 * D1 * D1;          // :161
@@ -658,7 +676,8 @@ static int cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *params, cons
         TPC(hipMemcpy(d_idx, hidx.data(), ngt * sizeof(int), hipMemcpyHostToDevice));
     }
     tp_cosmology *d_params = nullptr;
-    TPC(hipMalloc(&d_params, sizeof(tp_cosmology) * ncosmo));
+    TPC(hipMalloc(&d_params, sizeof(tp_cosmology) * ncosmo + sizeof(double) * ncosmo));
+    double *d_lgr0 = (double *)(d_params + ncosmo);  // lgr(0) per cosmology, freed with d_params
     TPC(hipMemcpyAsync(d_params, params, sizeof(tp_cosmology) * ncosmo, hipMemcpyHostToDevice, s));
     if (nz_stride == 0) {
         for (int i = 0; i < ncosmo; ++i)
@@ -679,6 +698,8 @@ static int cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *params, cons
     hipLaunchKernelGGL(k_fK, dim3((nt + 127) / 128), dim3(128), 0, s, ncosmo, n_z, dz, c->d_const, c->d_tab,
                        c->d_err);
     hipLaunchKernelGGL(k_g, dim3((nt + 127) / 128), dim3(128), 0, s, ncosmo, n_z, dz, z_max, c->d_nz, c->d_tab);
+    hipLaunchKernelGGL(k_lgr0, dim3((ncosmo + 63) / 64), dim3(64), 0, s, ncosmo, c->d_const, d_lgr0);
+    hipLaunchKernelGGL(k_d1, dim3((nt + 63) / 64), dim3(64), 0, s, ncosmo, n_z, dz, c->d_const, d_lgr0, c->d_tab);
     hipLaunchKernelGGL(k_growth, dim3(nt), dim3(64), 0, s, ncosmo, n_z, dz, c->d_const, c->d_tab);
     int nn = ncosmo * (TP_NNODES + TP_NBASE);
     hipLaunchKernelGGL(k_nodes, dim3((nn + 127) / 128), dim3(128), 0, s, ncosmo, n_z, c->d_const, c->d_tab,
