@@ -1453,18 +1453,26 @@ __device__ __forceinline__ bool make_input_in_box(const SurBox &B, double l1, do
 }
 
 // flag: 0 surrogate, 1 surrogate + exact check, 2 exact only (outside the training box), 3 zero (an l = 0)
+// a point to evaluate exactly, as k_sur_points formed it: its sides, its index in the chunk and its cosmology
+// (k_sur_exact reads this one record instead of the region, the integral's descriptor and the point rule)
+struct SurPt {
+    double l[3];
+    int t, ic;
+};
+
 __global__ void __launch_bounds__(256) k_sur_points(const TpRegion *__restrict__ reg, size_t r0, size_t nr,
                                                     const TpIntegralDesc *__restrict__ desc,
                                                     const TpCosmoConst *__restrict__ cc, const TpNode32 *__restrict__ base,
                                                     const TpEH32 *__restrict__ eh32, SurBox box, double frac,
                                                     unsigned long long key, float *X, double *F, double *Bb, int *flag,
-                                                    int *list, int *seg, int *err)
+                                                    SurPt *list, int *seg, int *err)
 {
     __shared__ int s_cnt;
     if (threadIdx.x == 0) s_cnt = 0;
     __syncthreads();
     const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     int f = 3;
+    SurPt pt{{0.0, 0.0, 0.0}, (int)t, 0};
     if (t < nr * TP_NPTS_GM3) {  // every thread reaches the barriers below
         size_t r = r0 + t / TP_NPTS_GM3;
         int p = (int)(t % TP_NPTS_GM3);
@@ -1474,6 +1482,10 @@ __global__ void __launch_bounds__(256) k_sur_points(const TpRegion *__restrict__
         gm_point3(R, p, &l1, &l2, &phi);
         double l3 = tp_ell3(l1, l2, phi);
         if (!isfinite(l1) || !isfinite(l2) || !isfinite(l3)) atomicOr(err, 1);
+        pt.l[0] = l1;
+        pt.l[1] = l2;
+        pt.l[2] = l3;
+        pt.ic = D.icosmo;
         // l1 l2 u(l1 t1) u(l2 t2) u(l3 t3), u(x) = x^2/2 exp(-x^2/2), with one exponential for the three filters
         const double a = l1 * D.th[0], b = l2 * D.th[1], e = l3 * D.th[2];
         const double a2 = a * a, b2 = b * b, e2 = e * e;
@@ -1498,7 +1510,7 @@ __global__ void __launch_bounds__(256) k_sur_points(const TpRegion *__restrict__
     int slot = 0;
     if (want) slot = atomicAdd(&s_cnt, 1);
     __syncthreads();
-    if (want) list[(size_t)blockIdx.x * blockDim.x + slot] = (int)t;
+    if (want) list[(size_t)blockIdx.x * blockDim.x + slot] = pt;
     if (threadIdx.x == 0) seg[blockIdx.x] = s_cnt;
 }
 
@@ -1508,8 +1520,7 @@ __global__ void __launch_bounds__(256) k_sur_points(const TpRegion *__restrict__
 // ones, which cluster, so a whole workgroup shares a full segment), 16 lanes per point, 6 of the 96 z-nodes
 // each.
 constexpr int XL = 16;
-__global__ void __launch_bounds__(256) k_sur_exact(const int *list, const int *seg, const TpRegion *__restrict__ reg,
-                                                   size_t r0, const TpIntegralDesc *__restrict__ desc,
+__global__ void __launch_bounds__(256) k_sur_exact(const SurPt *__restrict__ list, const int *seg,
                                                    const TpNode32 *__restrict__ nodes, const TpEH32 *__restrict__ eh,
                                                    double *Bex, int *err)
 {
@@ -1517,22 +1528,17 @@ __global__ void __launch_bounds__(256) k_sur_exact(const int *list, const int *s
     const int sub = threadIdx.x % XL;
     // the XL lanes of a group share i, so a group runs the loop (and the shuffle below) with all its lanes
     for (int i = threadIdx.x / XL; i < n; i += blockDim.x / XL) {
-        const int t = list[(size_t)blockIdx.x * blockDim.x + i];
-        const size_t r = r0 + t / TP_NPTS_GM3;
-        const int p = t % TP_NPTS_GM3;
-        const TpRegion R = reg[r];
-        const TpIntegralDesc D = desc[R.integral];
-        double l1, l2, phi;
-        gm_point3(R, p, &l1, &l2, &phi);
-        const double l3 = tp_ell3(l1, l2, phi);
+        const SurPt q0 = list[(size_t)blockIdx.x * blockDim.x + i];
+        const int t = q0.t;
+        const double l1 = q0.l[0], l2 = q0.l[1], l3 = q0.l[2];
         TpPointPre P;
         tp_point_pre<double>(l1, l2, l3, &P);
         const float L[3] = {(float)l1, (float)l2, (float)l3};
         const float LL[3] = {(float)P.ll[0], (float)P.ll[1], (float)P.ll[2]};
         const float F2T[3] = {(float)P.f2t[0], (float)P.f2t[1], (float)P.f2t[2]};
         const float LR1 = (float)P.lr1, R2 = (float)P.r2;
-        const TpNode32 *nd = nodes + (size_t)D.icosmo * TP_NNODES;
-        const TpEH32 E = eh[D.icosmo];
+        const TpNode32 *nd = nodes + (size_t)q0.ic * TP_NNODES;
+        const TpEH32 E = eh[q0.ic];
         double part = 0.0;
         for (int k = sub; k < TP_NNODES; k += XL) {
             const TpNode32 &N = nd[k];
@@ -1665,7 +1671,7 @@ int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, co
     if ((rc = B.X.ensure(PCP * NIN * sizeof(float))) || (rc = B.F.ensure(PC * sizeof(double))) ||
         (rc = B.Bb.ensure(PC * sizeof(double))) || (rc = B.flag.ensure(PC * sizeof(int))) ||
         (rc = B.logb.ensure(PCP * sizeof(double))) || (rc = B.Bex.ensure(PC * sizeof(double))) ||
-        (rc = B.list.ensure(NSEG * 256 * sizeof(int))) ||
+        (rc = B.list.ensure(NSEG * 256 * sizeof(SurPt))) ||
         (rc = B.seg.ensure(NSEG * sizeof(int))) || (rc = B.rpart.ensure((RC + kSurFinRegions - 1) / kSurFinRegions * 4 * sizeof(double))) ||
         (rc = B.misc.ensure(64)) || (rc = B.stats.ensure(64)))
         return rc;
@@ -1682,7 +1688,7 @@ int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, co
         const int nseg = grid(np, 256);
         hipLaunchKernelGGL(k_sur_points, dim3(nseg), dim3(256), 0, st, d_reg, r0, nr, d_desc, c->d_const,
                            c->d_base32, c->d_eh32, s->box, frac, key, (float *)B.X.p, (double *)B.F.p,
-                           (double *)B.Bb.p, (int *)B.flag.p, (int *)B.list.p, (int *)B.seg.p, d_flag);
+                           (double *)B.Bb.p, (int *)B.flag.p, (SurPt *)B.list.p, (int *)B.seg.p, d_flag);
         if (npp > np) TP_HIP(hipMemsetAsync((float *)B.X.p + np * NIN, 0, (npp - np) * NIN * sizeof(float), st));
         while ((int)B.ev.size() < 2 * (B.nev + 1)) {
             hipEvent_t e;
@@ -1694,8 +1700,8 @@ int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, co
         TP_HIP(hipEventRecord(B.ev[2 * B.nev + 1], st));
         B.ev_points.push_back((long long)npp);
         B.nev += 1;
-        hipLaunchKernelGGL(k_sur_exact, dim3(nseg), dim3(256), 0, st, (const int *)B.list.p, (const int *)B.seg.p,
-                           d_reg, r0, d_desc, c->d_node32, c->d_eh32, (double *)B.Bex.p, d_flag);
+        hipLaunchKernelGGL(k_sur_exact, dim3(nseg), dim3(256), 0, st, (const SurPt *)B.list.p, (const int *)B.seg.p,
+                           c->d_node32, c->d_eh32, (double *)B.Bex.p, d_flag);
         const int nrb = (int)((nr + kSurFinRegions - 1) / kSurFinRegions);
         hipLaunchKernelGGL(k_sur_fin, dim3(nrb), dim3(256), 0, st, d_reg, r0, nr, (const int *)B.flag.p,
                            (const double *)B.F.p, (const double *)B.logb.p, (const double *)B.Bb.p,
