@@ -49,8 +49,8 @@ MLP_MFMA_FLOPS_EXECUTED = 2 * (8 * 256 + 3 * 256 * 256)
 # VALU flops = SQ_INSTS_VALU_FLOPS_* x 64, MFMA busy = SQ_VALU_MFMA_BUSY_CYCLES / (GRBM_GUI_ACTIVE / 8 XCDs
 # x 256 CUs x 4 SIMDs), each rate over the kernel's summed dispatch time in the counter run.
 MLP_HBM_BYTES_PER_POINT = 43.5
-MLP_COUNTERS = {"kernel": "k_mlp_h", "mfma_tflops": 910.1, "mfma_frac_of_fp16_peak": 0.364, "mfma_busy": 0.47,
-                "valu_fp32_tflops": 10.7, "hbm_read_bytes_per_point": 35.5, "hbm_write_bytes_per_point": 8.0}
+MLP_COUNTERS = {"kernel": "k_mlp_h", "mfma_tflops": 877.8, "mfma_frac_of_fp16_peak": 0.351, "mfma_busy": 0.46,
+                "valu_fp32_tflops": 10.3, "hbm_read_bytes_per_point": 35.5, "hbm_write_bytes_per_point": 8.0}
 REGION_COUNTERS = {
     "float": {"kernel": "k_region_cached<float, 1>", "valu_fp32_tflops": 36.2, "frac_of_fp32_peak": 0.230,
               "frac_of_scalar_fp32": 0.460, "valu_fp32_flops_per_point": 22240, "trans_fp32_per_point": 2720,
