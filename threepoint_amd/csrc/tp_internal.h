@@ -96,6 +96,7 @@ struct tp_cosmo {
     TpNode32 *d_base32 = nullptr;     // [ncosmo][8]: 8-node FP32 Limber tables (surrogate baseline)
     int *d_err = nullptr;             // build error flag
     void *d_pk = nullptr;             // tabulated P(k): k, p [ncosmo][nk] then idx [ncosmo][ng]
+    void *d_arena = nullptr;          // one allocation holding every array above except d_pnode
     void *d_pnode = nullptr;          // [ncosmo][96] TpPNode: the P Limber sum's z-nodes (tp_power.hip)
     void *d_pnode32 = nullptr;        // [ncosmo][96] TpPNode32: their FP32 form (same allocation)
     int pk_nk = 0;                    // 0: Eisenstein & Hu

@@ -18,6 +18,7 @@
 //   k_d1            1 thread / (cosmo,bin) : D1 = lgr(z_i) / lgr(0)
 //   k_growth        1 wave / (cosmo,bin)   : r_sigma, n_eff, ncur
 //   k_nodes         1 thread / (cosmo,node): 96-node Limber tables (FP64 + FP32 copy)
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <string>
@@ -648,21 +649,42 @@ static int cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *params, cons
         hipError_t e_ = (call);                                                                    \
         if (e_ != hipSuccess) return fail(tp_set_error(TP_EHIP, std::string(#call) + ": " + hipGetErrorString(e_))); \
     } while (0)
-    TPC(hipMalloc(&c->d_const, sizeof(TpCosmoConst) * ncosmo));
-    TPC(hipMalloc(&c->d_tab, sizeof(double) * 6 * n_z * (size_t)ncosmo));
-    TPC(hipMalloc(&c->d_nz, sizeof(double) * n_z * (size_t)ncosmo));
-    TPC(hipMalloc(&c->d_node64, sizeof(TpNode64) * TP_NNODES * (size_t)ncosmo));
-    TPC(hipMalloc(&c->d_node32, sizeof(TpNode32) * TP_NNODES * (size_t)ncosmo));
-    TPC(hipMalloc(&c->d_eh64, sizeof(TpEH64) * (size_t)ncosmo));
-    TPC(hipMalloc(&c->d_eh32, sizeof(TpEH32) * (size_t)ncosmo));
-    TPC(hipMalloc(&c->d_base32, sizeof(TpNode32) * TP_NBASE * (size_t)ncosmo));
-    TPC(hipMalloc(&c->d_err, sizeof(int)));
+    // one device allocation for the whole build (separate allocations and frees cost ~0.1 ms each, a
+    // noticeable share of a one-cosmology build); each array starts on a 256-byte boundary
+    const size_t nkt = (size_t)ncosmo * (nk ? nk : 0), ngt = (size_t)ncosmo * (nk ? ng : 0);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    const size_t o_const = take(sizeof(TpCosmoConst) * ncosmo);
+    const size_t o_tab = take(sizeof(double) * 6 * n_z * (size_t)ncosmo);
+    const size_t o_nz = take(sizeof(double) * n_z * (size_t)ncosmo);
+    const size_t o_node64 = take(sizeof(TpNode64) * TP_NNODES * (size_t)ncosmo);
+    const size_t o_node32 = take(sizeof(TpNode32) * TP_NNODES * (size_t)ncosmo);
+    const size_t o_eh64 = take(sizeof(TpEH64) * (size_t)ncosmo);
+    const size_t o_eh32 = take(sizeof(TpEH32) * (size_t)ncosmo);
+    const size_t o_base32 = take(sizeof(TpNode32) * TP_NBASE * (size_t)ncosmo);
+    const size_t o_err = take(sizeof(int));
+    const size_t o_params = take(sizeof(tp_cosmology) * ncosmo + sizeof(double) * ncosmo);
+    const size_t o_pk = nk ? take((2 * nkt + 2 * (size_t)ncosmo) * sizeof(double) + ngt * sizeof(int)) : 0;
+    TPC(hipMalloc(&c->d_arena, off));
+    char *arena = (char *)c->d_arena;
+    c->d_const = (TpCosmoConst *)(arena + o_const);
+    c->d_tab = (double *)(arena + o_tab);
+    c->d_nz = (double *)(arena + o_nz);
+    c->d_node64 = (TpNode64 *)(arena + o_node64);
+    c->d_node32 = (TpNode32 *)(arena + o_node32);
+    c->d_eh64 = (TpEH64 *)(arena + o_eh64);
+    c->d_eh32 = (TpEH32 *)(arena + o_eh32);
+    c->d_base32 = (TpNode32 *)(arena + o_base32);
+    c->d_err = (int *)(arena + o_err);
     TPC(hipMemsetAsync(c->d_err, 0, sizeof(int), s));
     double *d_pkk = nullptr, *d_pkp = nullptr, *d_l0 = nullptr, *d_inv = nullptr;
     int *d_idx = nullptr;
     if (nk) {
-        const size_t nkt = (size_t)ncosmo * nk, ngt = (size_t)ncosmo * ng;
-        TPC(hipMalloc(&c->d_pk, (2 * nkt + 2 * (size_t)ncosmo) * sizeof(double) + ngt * sizeof(int)));
+        c->d_pk = arena + o_pk;
         d_pkk = (double *)c->d_pk;
         d_pkp = d_pkk + nkt;
         d_l0 = d_pkp + nkt;
@@ -675,18 +697,14 @@ static int cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *params, cons
         TPC(hipMemcpy(d_inv, hinv.data(), ncosmo * sizeof(double), hipMemcpyHostToDevice));
         TPC(hipMemcpy(d_idx, hidx.data(), ngt * sizeof(int), hipMemcpyHostToDevice));
     }
-    tp_cosmology *d_params = nullptr;
-    TPC(hipMalloc(&d_params, sizeof(tp_cosmology) * ncosmo + sizeof(double) * ncosmo));
-    double *d_lgr0 = (double *)(d_params + ncosmo);  // lgr(0) per cosmology, freed with d_params
+    tp_cosmology *d_params = (tp_cosmology *)(arena + o_params);
+    double *d_lgr0 = (double *)(d_params + ncosmo);  // lgr(0) per cosmology
     TPC(hipMemcpyAsync(d_params, params, sizeof(tp_cosmology) * ncosmo, hipMemcpyHostToDevice, s));
-    if (nz_stride == 0) {
-        for (int i = 0; i < ncosmo; ++i)
-            TPC(hipMemcpyAsync(c->d_nz + (size_t)i * n_z, nz, sizeof(double) * n_z, hipMemcpyHostToDevice, s));
-    } else {
-        for (int i = 0; i < ncosmo; ++i)
-            TPC(hipMemcpyAsync(c->d_nz + (size_t)i * n_z, nz + nz_stride * i, sizeof(double) * n_z,
-                               hipMemcpyHostToDevice, s));
-    }
+    // n(z) of every cosmology in one copy (the stream is synchronised before h_nz goes out of scope)
+    std::vector<double> h_nz((size_t)ncosmo * n_z);
+    for (int i = 0; i < ncosmo; ++i)
+        std::copy(nz + nz_stride * i, nz + nz_stride * i + n_z, h_nz.begin() + (size_t)i * n_z);
+    TPC(hipMemcpyAsync(c->d_nz, h_nz.data(), sizeof(double) * h_nz.size(), hipMemcpyHostToDevice, s));
     double dz = z_max / ((double)n_z);   // classed_version/bispectrum.cpp:98
     int nt = ncosmo * n_z;
     hipLaunchKernelGGL(k_cosmo_const, dim3((ncosmo + 63) / 64), dim3(64), 0, s, ncosmo, d_params, z_max, n_z,
@@ -710,7 +728,6 @@ static int cosmo_build(tp_ctx *ctx, int ncosmo, const tp_cosmology *params, cons
     c->h_const.resize(ncosmo);
     TPC(hipMemcpyAsync(c->h_const.data(), c->d_const, sizeof(TpCosmoConst) * ncosmo, hipMemcpyDeviceToHost, s));
     TPC(hipStreamSynchronize(s));
-    (void)hipFree(d_params);
     if (herr) return fail(tp_set_error(TP_ECAPACITY, "tp_cosmo_build: f_K quadrature exceeded its interval capacity"));
     int prc = tp_build_pnodes(ctx, c);
     if (prc) return fail(prc);
@@ -737,17 +754,8 @@ extern "C" int tp_cosmo_destroy(tp_cosmo *c)
 {
     if (!c) return 0;
     (void)hipSetDevice(c->device);
-    (void)hipFree(c->d_const);
-    (void)hipFree(c->d_tab);
-    (void)hipFree(c->d_nz);
-    (void)hipFree(c->d_node64);
     (void)hipFree(c->d_pnode);
-    (void)hipFree(c->d_node32);
-    (void)hipFree(c->d_eh64);
-    (void)hipFree(c->d_eh32);
-    (void)hipFree(c->d_base32);
-    (void)hipFree(c->d_err);
-    if (c->d_pk) (void)hipFree(c->d_pk);
+    (void)hipFree(c->d_arena);  // every other array lives in it
     delete c;
     return 0;
 }
