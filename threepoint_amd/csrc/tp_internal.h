@@ -254,6 +254,10 @@ int tp_launch_bkappa_points(tp_ctx *ctx, const tp_cosmo *c, int icosmo, size_t n
 int tp_launch_regions(tp_ctx *ctx, const tp_cosmo *c, size_t nreg, const TpRegion *d_reg,
                       const TpIntegralDesc *d_desc, double *d_fvals, TpRegionOut *d_out, int mode,
                       int *d_flag, hipEvent_t evm, hipStream_t st);
+// Copies `bytes` (a multiple of 4) from device memory to pinned host memory through its device
+// pointer, as a kernel on st: the batched <Map^3> rounds read their results back this way, so that
+// no DMA copy waiting on a kernel can hold up the other group's copies behind it.
+int tp_copy_to_pinned(const void *d_src, void *h_dp, size_t bytes, hipStream_t st);
 // One exact-mode round of lane L (nreg regions already in L.h_regions) as a graph replay: the same
 // work and events as the copy + tp_launch_regions + copy sequence, queued on ctx->stream.
 int tp_graph_round(tp_ctx *ctx, const tp_cosmo *c, TpLane &L, size_t nreg, int mode, hipStream_t st);

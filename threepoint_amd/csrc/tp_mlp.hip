@@ -1710,8 +1710,8 @@ int tp_surrogate_queue(tp_ctx *ctx, const tp_cosmo *c, int lane, size_t nreg, co
                            (const double *)B.rpart.p, nrb, dstats);
         TP_HIP(hipGetLastError());
     }
-    TP_HIP(hipMemcpyAsync(B.stats.p, B.misc.p, 64, hipMemcpyDeviceToHost, st));
-    return 0;
+    // read back by a kernel, not a DMA copy waiting on the round (tp_copy_to_pinned)
+    return tp_copy_to_pinned(B.misc.p, B.stats.dp, 64, st);
 }
 
 int tp_surrogate_result(tp_ctx *ctx, int lane, size_t nreg, bool *fallback)
