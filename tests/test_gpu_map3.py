@@ -279,7 +279,24 @@ def test_zero_copy_rounds_bit_identical(fid_tables, gpu_ctx, mode):
         v1, e1, n1 = fid_tables.map3_batch(th, None, 1e-4, mode)
         assert np.array_equal(v0, v1) and np.array_equal(e0, e1) and np.array_equal(n0, n1)
     finally:
-        tp.lib().tp_set_zero_copy_regions(gpu_ctx.h, 0)
+        tp.lib().tp_set_zero_copy_regions(gpu_ctx.h, 2048)  # the default
+
+
+@pytest.mark.parametrize("mode", [tp.MODE_FP64, tp.MODE_FP32])
+def test_dma_and_in_place_results_bit_identical(fid_tables, gpu_ctx, mode):
+    """A round below the serial size writes its region results to pinned host memory in place; a larger
+    round copies them out by DMA.  With the serial size at one region every round above the zero-copy size
+    takes the DMA path (and the two groups' rounds run one after the other): values, errors and point
+    counts are those of the default (C0)."""
+    th = W.to_rad(W.triples(W.map3_c0_radii()))
+    ref = fid_tables.map3_batch(th, None, 1e-4, mode)
+    try:
+        tp._check(tp.lib().tp_set_serial_regions(gpu_ctx.h, 1), "serial")
+        dma = fid_tables.map3_batch(th, None, 1e-4, mode)
+    finally:
+        tp.lib().tp_set_serial_regions(gpu_ctx.h, 0)  # the default
+    for a, b in zip(ref, dma):
+        assert np.array_equal(a, b)
 
 
 def test_host_settings_bit_identical(gpu_ctx, fid_tables):

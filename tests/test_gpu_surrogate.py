@@ -187,6 +187,33 @@ def test_surrogate_mode_gain_in_map3(gpu_ctx, fid_tables):
     assert frac2 < 0.2 and st2["fallback_batches"] > 0 and err2 < 1e-3
 
 
+def test_surrogate_dma_and_in_place_results_agree(gpu_ctx, fid_tables):
+    """Surrogate rounds below the serial size write their region results to pinned host memory in place
+    (the exact part, the network's part and, after a rejected check, the exact redo); larger rounds copy
+    them out by DMA.  Every round on the DMA path (serial size one region) gives the default's values and
+    point counts, with the trained network and with an untrained one whose rounds fall back (C0)."""
+    th = W.to_rad(W.triples(W.map3_c0_radii()))
+    rng = np.random.default_rng(7)
+    bad = (rng.standard_normal(MO.NPARAM) * 0.05).astype(np.float32)
+    try:
+        for trained in (True, False):
+            _load_model(gpu_ctx)
+            if not trained:
+                tp._check(tp.lib().tp_mlp_set_params(gpu_ctx.h, bad.ctypes.data_as(_fp)), "set")
+            v0, _, n0 = fid_tables.map3_batch(th, None, 1e-4, tp.MODE_SURROGATE)
+            fb0 = gpu_ctx.stats()["fallback_batches"]
+            tp._check(tp.lib().tp_set_serial_regions(gpu_ctx.h, 1), "serial")
+            v1, _, n1 = fid_tables.map3_batch(th, None, 1e-4, tp.MODE_SURROGATE)
+            fb1 = gpu_ctx.stats()["fallback_batches"]
+            tp._check(tp.lib().tp_set_serial_regions(gpu_ctx.h, 0), "serial")
+            assert (fb0 > 0) == (not trained) and fb1 == fb0
+            np.testing.assert_allclose(v1, v0, rtol=1e-12, atol=0)
+            assert np.array_equal(n0, n1)
+    finally:
+        tp.lib().tp_set_serial_regions(gpu_ctx.h, 0)
+        _load_model(gpu_ctx)
+
+
 @pytest.mark.parametrize("workload", ["c1", "c3_subset"])
 def test_surrogate_form_within_1e3_of_fp64(gpu_ctx, fid_nz, workload):
     """The surrogate form (triples with max/min radius >= 16 exact throughout; the others exact FP32 regions
