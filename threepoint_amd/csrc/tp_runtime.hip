@@ -657,6 +657,10 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     };
 
     std::vector<std::pair<float, float>> iv_pts, iv_ker;  // device intervals of the rounds [ms]
+    // each integral's final re-sums, taken when it finishes (on the host threads, while the device runs
+    // the other integrals' rounds) instead of all after the last round
+    std::vector<double> fin_v(ntriples), fin_e(ntriples), fin_cv(ntriples);
+    std::vector<char> fin_done(ntriples, 0);
     auto union_ms = [](std::vector<std::pair<float, float>> &iv) {
         std::sort(iv.begin(), iv.end());
         double tot = 0, a = 0, b = -1;
@@ -745,10 +749,18 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
             finished[a] = A.advance() ? 1 : 0;
         }
         if (bad) return tp_set_error(TP_ENONFINITE, "tp_map3_batch: non-finite region estimate");
-        std::vector<size_t> next;
+        std::vector<size_t> next, done_now;
         next.reserve(na);
         for (size_t a = 0; a < na; ++a)
-            if (!finished[a]) next.push_back(act[a]);
+            (finished[a] ? done_now : next).push_back(act[a]);
+        // the final re-sums of the integrals that finished in this round (each walks its whole heap)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads) if (done_now.size() > 1)
+        for (size_t k = 0; k < done_now.size(); ++k) {
+            const size_t t = done_now[k];
+            I[t].result(&fin_v[t], &fin_e[t]);
+            if (mode == TP_MODE_SURROGATE) fin_cv[t] = I[t].correction_var();
+            fin_done[t] = 1;
+        }
         Gr.active.swap(next);
         ctx->stats.host_ms += ms_since(th0);
         if (ctx->round_log_on)
@@ -816,18 +828,21 @@ extern "C" int tp_map3_batch(tp_ctx *ctx, const tp_cosmo *c, size_t ntriples, co
     }
     ctx->stats.kernel_ms = union_ms(iv_ker);
     ctx->stats.integrand_ms = union_ms(iv_pts);
-    // the final re-sums walk every integral's heap: spread them over the host threads
+    // results (the final re-sums of any integral that did not finish in a round: none, normally)
     std::vector<char> redo_flag(ntriples, 0);
 #pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads) if (ntriples > 16)
     for (size_t t = 0; t < ntriples; ++t) {
-        double r, e;
-        I[t].result(&r, &e);
+        if (!fin_done[t]) {
+            I[t].result(&fin_v[t], &fin_e[t]);
+            if (mode == TP_MODE_SURROGATE) fin_cv[t] = I[t].correction_var();
+        }
+        const double r = fin_v[t], e = fin_e[t];
         val[t] = r / 8 / M_PI / M_PI / M_PI;
         if (err) err[t] = e / 8 / M_PI / M_PI / M_PI;
         if (neval) neval[t] = (long long)I[t].num_eval();
         // surrogate mode: a triple whose correction has a standard error above tol is redone exactly (FP32 form)
         if (mode == TP_MODE_SURROGATE &&
-            (!I[t].converged_flag() || !(std::sqrt(I[t].correction_var()) <= ctx->sur_tol * std::fabs(r))))
+            (!I[t].converged_flag() || !(std::sqrt(fin_cv[t]) <= ctx->sur_tol * std::fabs(r))))
             redo_flag[t] = 1;
     }
     std::vector<size_t> redo;
