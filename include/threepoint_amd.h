@@ -154,8 +154,10 @@ int tp_region_lanes_table(tp_ctx *ctx, int mode, int *lanes4);
  * counterpart in the reference. */
 int tp_set_graph_rounds(tp_ctx *ctx, int on);
 /* Number of integral groups tp_map3_batch keeps in flight (1-4, default 2): while the device evaluates
- * one group's regions the host runs the others' heap decisions.  Results do not depend on it.  No
- * counterpart in the reference. */
+ * one group's regions the host runs the others' heap decisions.  Results of the FP64 and FP32 modes do
+ * not depend on it; in surrogate mode the exactly checked points are drawn per group and round, so
+ * another group count draws another sample (results within the surrogate's tolerance, not bit-equal).
+ * No counterpart in the reference. */
 int tp_set_batch_groups(tp_ctx *ctx, int groups);
 /* Per-round trace of tp_map3_batch (diagnostics, off by default).  While on, every cubature round
  * of a call appends 9 doubles: group, regions, host launch begin and end, completion begin (after

@@ -299,6 +299,23 @@ def test_dma_and_in_place_results_bit_identical(fid_tables, gpu_ctx, mode):
         assert np.array_equal(a, b)
 
 
+def test_batch_groups_bit_identical(fid_tables, gpu_ctx):
+    """One to four integral groups in flight (each on its own stream) give the default's values, errors
+    and point counts in the FP64 and FP32 modes (C0)."""
+    th = W.to_rad(W.triples(W.map3_c0_radii()))
+    try:
+        for mode in (tp.MODE_FP64, tp.MODE_FP32):
+            ref = fid_tables.map3_batch(th, None, 1e-4, mode)
+            for g in (1, 3, 4):
+                tp._check(tp.lib().tp_set_batch_groups(gpu_ctx.h, g), "groups")
+                got = fid_tables.map3_batch(th, None, 1e-4, mode)
+                for a, b in zip(ref, got):
+                    assert np.array_equal(a, b), (mode, g)
+            tp._check(tp.lib().tp_set_batch_groups(gpu_ctx.h, 2), "groups")
+    finally:
+        tp.lib().tp_set_batch_groups(gpu_ctx.h, 2)  # the default
+
+
 def test_host_settings_bit_identical(gpu_ctx, fid_tables):
     """Host threads, kept or fresh region pools, and a reused integrator set change no bit of a result."""
     th = W.to_rad(W.triples(W.map3_c0_radii()))
