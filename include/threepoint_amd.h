@@ -180,15 +180,17 @@ int tp_set_keep_pool_bytes(tp_ctx *ctx, unsigned long long bytes);
 int tp_set_host_threads(tp_ctx *ctx, int n);
 int tp_round_log(const tp_ctx *ctx, double *out, size_t cap, size_t *n);
 
-/* Exact-mode rounds of at most max_regions regions read their regions from, and write their results
- * to, pinned host memory directly instead of copying them (0 = always copy).  Results do not depend
- * on it.  No counterpart in the reference. */
+/* Exact-mode rounds of at most max_regions regions read their regions from pinned host memory directly
+ * instead of copying them in (0 = always copy in; default 2048).  Their results, and those of every
+ * round below the serial size (tp_set_serial_regions), are written to pinned host memory directly;
+ * larger rounds copy them out.  Results do not depend on it.  No counterpart in the reference. */
 int tp_set_zero_copy_regions(tp_ctx *ctx, long long max_regions);
 
 /* tp_map3_batch runs the rounds of its two groups of integrals one after the other on the device when
  * both hold at least min_regions regions (each then fills the device; the host absorbs one round while
- * the other's kernels run), side by side otherwise.  0 restores the default (65536).  Results do not
- * depend on it.  No counterpart in the reference. */
+ * the other's kernels run), side by side otherwise; rounds below min_regions also write their results
+ * to pinned host memory directly, larger ones copy them out of device memory.  0 restores the default
+ * (65536).  Results do not depend on it.  No counterpart in the reference. */
 int tp_set_serial_regions(tp_ctx *ctx, long long min_regions);
 
 /* Guards of every batched cubature on this context (tp_map3_batch, tp_gamma_batch, tp_map2_batch,
