@@ -231,4 +231,6 @@ def test_bench_process_loads_no_torch():
     lib = os.path.join(ROOT, "threepoint_amd", "_lib", "libthreepoint_amd.so")
     if os.path.exists(lib):
         ldd = subprocess.run(["ldd", lib], capture_output=True, text=True).stdout
-        assert "torch" not in ldd and "c10" not in ldd
+        # library names only: the load addresses ldd prints are random and may contain "c10"
+        names = [ln.split()[0] for ln in ldd.splitlines() if ln.strip()]
+        assert not any("torch" in n or "libc10" in n for n in names), names
